@@ -1,0 +1,244 @@
+"""Benchmark: batched lambda-window MD stepping on one MI355X.
+
+Workload (one "step" = one MD step of every resident system): 48 evenly spaced
+lambda windows x 4 replicas of the ligand-in-water system (192 systems of
+6372 atoms: 2100 flexible waters + a 72-atom synthetic ligand, 4.0 nm box,
+rc 1.2 nm, beta 2, 4D decoupling offsets on the ligand), BAOAB at 2.5 fs with
+HMR, 300 K, friction 1/ps, MC barostat at 1 bar every 15 steps, du/dlambda
+stored every step, all in one engine whose per-step launches are captured into
+hipGraphs. value = total atom-steps/s of the batch. The same system stepped
+alone through a Context (one system at a time) is reported beside it.
+
+Prints ONE JSON line. Only the cpu_baseline leg uses the oracle (FP64 port of
+the reference's potentials and BAOAB, timed on the host cores).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+METRIC = "atom-steps/s (batched λ-windows vs single system); p50/p99 job latency"
+PEAK_FP32_TFLOPS = 157.3  # MI355X published dense FP32 vector peak
+# Algorithmic FP32 work per pair inside the cutoff, du_dx only (FMA = 2,
+# rsqrt / exp / rcp = 1): 20 (delta, min-image, d^2) + 27 (erfc-damped
+# Coulomb force) + 9 (projection, fixed-point scaling); + 25 when both
+# epsilons are nonzero (Lennard-Jones).
+FLOP_PAIR_BASE = 56
+FLOP_PAIR_LJ = 25
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def pair_counts(sys_, lam: float):
+    """Exact pairs within the 4D cutoff (and LJ pairs) at lambda, from 3D candidates."""
+    from scipy.spatial import cKDTree
+
+    L = np.diag(sys_.box)
+    x = np.mod(sys_.x, L)
+    x = np.where(x >= L, 0.0, x)
+    pairs = cKDTree(x, boxsize=L).query_pairs(sys_.cutoff, output_type="ndarray")
+    d = sys_.x[pairs[:, 0]] - sys_.x[pairs[:, 1]]
+    d -= L * np.floor(d / L + 0.5)
+    w = sys_.cutoff * (sys_.lambda_plane_idxs + sys_.lambda_offset_idxs * lam)
+    dw = w[pairs[:, 0]] - w[pairs[:, 1]]
+    inside = (d * d).sum(1) + dw * dw < sys_.cutoff**2
+    eps = sys_.nb_params[:, 2]
+    lj = inside & (eps[pairs[:, 0]] != 0) & (eps[pairs[:, 1]] != 0)
+    return int(inside.sum()), int(lj.sum())
+
+
+def make_contexts(sys_, lams, replicas, seed0=2022, barostat_interval=15):
+    from timemachine_amd import testsystems as ts
+    from timemachine_amd.lib import LangevinIntegrator, MonteCarloBarostat, custom_ops
+
+    ctxs, keep, lam_list = [], [], []
+    for r in range(replicas):
+        for lam in lams:
+            seed = seed0 + 1000 * r + len(ctxs)
+            bps = [p.bound_impl(np.float32) for p in sys_.bound_potentials()]
+            intg = LangevinIntegrator(300.0, 0.0025, 1.0, sys_.masses, seed).impl()
+            baro = MonteCarloBarostat(sys_.N, 1.0, 300.0, sys_.groups, barostat_interval, seed + 7).impl(bps)
+            v0 = ts.maxwell_boltzmann(sys_.masses, 300.0, seed)
+            ctxs.append(custom_ops.Context(sys_.x, v0, sys_.box, intg, bps, barostat=baro))
+            keep.append((bps, intg, baro))
+            lam_list.append(float(lam))
+    return ctxs, keep, lam_list
+
+
+def cpu_baseline(sys_, budget_s: float = 12.0):
+    """FP64 oracle (reference formulas) MD steps on the host cores, bounded sample."""
+    import torch
+
+    from oracle import bonded as ob
+    from oracle import integrator as oi
+    from oracle import nonbonded as onb
+
+    threads = int(os.environ.get("TM_CPU_THREADS", min(16, os.cpu_count() or 1)))
+    torch.set_num_threads(threads)
+    ca, cb, cc = oi.langevin_coefficients(300.0, 0.0025, 1.0, sys_.masses)
+    cbs = -cb
+    rng = np.random.default_rng(0)
+    x = sys_.x.copy()
+    v = np.zeros_like(x)
+    lam = 0.5
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        f = onb.nonbonded_pairs(x, sys_.nb_params, sys_.box, lam, sys_.exclusion_idxs, sys_.scales, sys_.beta,
+                                sys_.cutoff, sys_.lambda_plane_idxs, sys_.lambda_offset_idxs)[0]
+        f = f + ob.bonded("bond", x, sys_.bond_params, lam, sys_.bond_idxs)[0]
+        f = f + ob.bonded("angle", x, sys_.angle_params, lam, sys_.angle_idxs)[0]
+        f = f + ob.bonded("torsion", x, sys_.torsion_params, lam, sys_.torsion_idxs)[0]
+        x, v = oi.baoab_step(x, v, f, 0.0025, ca, cbs, cc, rng.normal(size=x.shape))
+        steps += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {
+        "value": sys_.N * steps / dt,
+        "unit": "atom-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{steps} FP64 BAOAB steps (pair-list nonbonded + bonded, Torch CPU) of the {sys_.N}-atom "
+        f"ligand-in-water system at lambda 0.5, {dt:.1f} s",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=150)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--windows", type=int, default=48)
+    ap.add_argument("--replicas", type=int, default=4)
+    ap.add_argument("--single-steps", type=int, default=300)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--tile-grid", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(
+            f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (lambda windows shard across GPUs "
+            "with no per-step exchange; run one process per GPU instead)",
+            file=sys.stderr,
+        )
+        return 2
+
+    from timemachine_amd import _lib
+    from timemachine_amd import testsystems as ts
+    from timemachine_amd.lib import custom_ops
+
+    _lib.load()
+    assert _lib.device_count() >= 1, "no HIP device visible"
+    sys_ = ts.ligand_in_water()
+    lams = np.linspace(0.0, 1.0, args.windows)
+    log(f"building {args.windows * args.replicas} contexts of {sys_.N} atoms")
+    ctxs, keep, lam_list = make_contexts(sys_, lams, args.replicas)
+    batch = custom_ops.BatchedContexts(ctxs, lam_list)
+    if args.tile_grid:
+        batch.set_tile_grid(args.tile_grid)
+    S = len(ctxs)
+    use_graph = not args.no_graph
+    log(f"warmup {args.warmup} batched steps")
+    batch.multiple_steps(args.warmup, store_du_dl_interval=1, use_graph=use_graph)
+    log(f"timing {args.steps} batched steps")
+    t0 = time.perf_counter()
+    du_dl, _, _ = batch.multiple_steps(args.steps, store_du_dl_interval=1, use_graph=use_graph)
+    t_batch = time.perf_counter() - t0  # multiple_steps synchronises the device before returning
+    assert np.all(np.isfinite(du_dl))
+    atom_steps = S * sys_.N * args.steps
+    value = atom_steps / t_batch
+
+    log("profiling kernels (HIP events)")
+    prof = batch.profile(reps=10)
+    tiles = batch.total_tiles()
+
+    # one system at a time, the same kernels through the Context API
+    single_ctx, single_keep, _ = make_contexts(sys_, [0.5], 1, seed0=99)
+    single = single_ctx[0]
+    single.multiple_steps(np.full(args.warmup, 0.5), store_du_dl_interval=1)
+    t0 = time.perf_counter()
+    single.multiple_steps(np.full(args.single_steps, 0.5), store_du_dl_interval=1)
+    t_single = time.perf_counter() - t0
+    single_rate = sys_.N * args.single_steps / t_single
+
+    # roofline of the pair kernel: algorithmic FLOP per launch / HIP-event duration
+    log("counting pairs for the roofline")
+    flop = 0.0
+    pairs_total = 0
+    for lam in lams:
+        p, plj = pair_counts(sys_, float(lam))
+        flop += args.replicas * (FLOP_PAIR_BASE * p + FLOP_PAIR_LJ * plj)
+        pairs_total += args.replicas * p
+    achieved = flop / (prof["nonbonded"] * 1e-3) / 1e12
+    roofline = {
+        "bound": "valu",
+        "kernel": "k_nonbonded_tiles (du_dx)",
+        "achieved": achieved,
+        "peak": PEAK_FP32_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": achieved / PEAK_FP32_TFLOPS,
+        "traffic": None,
+        "pairs_per_launch": pairs_total,
+        "flop_per_launch": flop,
+        "avg_launch_ms": prof["nonbonded"],
+        "tiles_per_launch": tiles,
+        "pair_evaluations_per_launch": tiles * 64 * 64,
+    }
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (FP64 oracle)")
+        cpu = cpu_baseline(sys_)
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_batch / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched lambda windows: {args.windows} windows x {args.replicas} replicas of the "
+            f"{sys_.N}-atom ligand-in-water system (4.0 nm box, 2100 waters + 72-atom ligand)",
+            "systems": S,
+            "atoms_per_system": sys_.N,
+            "dt_ps": 0.0025,
+            "barostat_interval": 15,
+            "du_dl_interval": 1,
+            "hipgraph": use_graph,
+        },
+        "single_system": {
+            "value": single_rate,
+            "unit": "atom-steps/s",
+            "ms_per_step": 1e3 * t_single / args.single_steps,
+            "steps": args.single_steps,
+        },
+        "batched_over_single": value / single_rate,
+        "kernel_ms": prof,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

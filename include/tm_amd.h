@@ -1,0 +1,198 @@
+/*
+ * tm_amd.h - C ABI of the MI355X (gfx950) MD hot path.
+ *
+ * Each entry point replaces one method of the reference's pybind11 module
+ * (timemachine/cpp/src/wrap_kernels.cpp) or of the C++ class behind it; the
+ * citation above each declaration names it. Conventions:
+ *   - every function returns TM_OK (0) or an error code; tm_last_error()
+ *     returns the message (thread-local), which for validation errors is the
+ *     reference's own std::runtime_error text;
+ *   - handles are opaque and reference counted internally: a Context keeps
+ *     its integrator, bound potentials and barostat alive;
+ *   - arrays are C-contiguous; x is [N, 3] f64, box [3, 3] f64, params as the
+ *     potential defines them; uint64 outputs are 64-bit fixed point
+ *     (2^36 for du_dx, u, du_dl; nonbonded du_dp uses 2^36 / 2^37 / 2^38 for
+ *     charge / sigma / epsilon columns) and accumulate into caller-zeroed buffers;
+ *   - a NULL output pointer means "not requested".
+ * The library owns no global state besides the HIP device context.
+ */
+#ifndef TM_AMD_H
+#define TM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TM_OK 0
+#define TM_ERR_INVALID 1 /* validation error (reference message in tm_last_error) */
+#define TM_ERR_HIP 2     /* HIP runtime error / device fault */
+#define TM_ERR_INTERNAL 3
+
+typedef struct tm_potential tm_potential;
+typedef struct tm_bound_potential tm_bound_potential;
+typedef struct tm_neighborlist tm_neighborlist;
+typedef struct tm_integrator tm_integrator;
+typedef struct tm_barostat tm_barostat;
+typedef struct tm_context tm_context;
+typedef struct tm_batch tm_batch;
+
+/* Last error message of this thread. */
+const char *tm_last_error(void);
+/* Tile width of the neighborlist / pair kernel (64: one wave64 per tile;
+ * the reference uses 32, kernels/k_neighborlist.cuh:7). */
+int tm_tile_size(void);
+/* Number of visible HIP devices (0 when no GPU). */
+int tm_device_count(int *n);
+
+/* ---- potentials ------------------------------------------------------- */
+
+/* NonbondedAllPairs_f32(kernel_dir, lambda_plane_idxs, lambda_offset_idxs, beta, cutoff, atom_idxs)
+ * wrap_kernels.cpp:835-895, nonbonded_all_pairs.cu:16-141. n_atom_idxs < 0: all atoms. */
+int tm_nonbonded_all_pairs_create(int N, const int32_t *lambda_plane_idxs, const int32_t *lambda_offset_idxs,
+                                  double beta, double cutoff, const int32_t *atom_idxs, int n_atom_idxs,
+                                  tm_potential **out);
+/* NonbondedAllPairs::set_nblist_padding, nonbonded_all_pairs.cu:180-183 */
+int tm_nonbonded_all_pairs_set_nblist_padding(tm_potential *p, double padding);
+/* NonbondedAllPairs::disable_hilbert_sort, nonbonded_all_pairs.cu:185-187 */
+int tm_nonbonded_all_pairs_disable_hilbert_sort(tm_potential *p);
+/* Tiles of the current neighborlist (no reference counterpart; diagnostics). */
+int tm_nonbonded_all_pairs_num_tiles(tm_potential *p, int *out);
+
+/* NonbondedPairList_f32[_negated](kernel_dir, pair_idxs, scales, plane, offset, beta, cutoff)
+ * wrap_kernels.cpp:957-1014, nonbonded_pair_list.cu:10-91. */
+int tm_nonbonded_pair_list_create(int N, int M, const int32_t *pair_idxs, const double *scales,
+                                  const int32_t *lambda_plane_idxs, const int32_t *lambda_offset_idxs, double beta,
+                                  double cutoff, int negated, tm_potential **out);
+
+/* HarmonicBond_f32(bond_idxs, lamb_mult, lamb_offset), harmonic_bond.cu:10-56. mult/offset may be NULL. */
+int tm_harmonic_bond_create(int B, const int32_t *bond_idxs, const int32_t *lamb_mult, const int32_t *lamb_offset,
+                            tm_potential **out);
+/* HarmonicAngle_f32, harmonic_angle.cu:10-60 */
+int tm_harmonic_angle_create(int A, const int32_t *angle_idxs, const int32_t *lamb_mult, const int32_t *lamb_offset,
+                             tm_potential **out);
+/* PeriodicTorsion_f32, periodic_torsion.cu:10-62 */
+int tm_periodic_torsion_create(int T, const int32_t *torsion_idxs, const int32_t *lamb_mult,
+                               const int32_t *lamb_offset, tm_potential **out);
+/* SummedPotential(potentials, params_sizes), summed_potential.cu:8-15 */
+int tm_summed_potential_create(int n, tm_potential *const *pots, const int32_t *params_sizes, tm_potential **out);
+/* FanoutSummedPotential(potentials), fanout_summed_potential.cu:6-9 */
+int tm_fanout_summed_potential_create(int n, tm_potential *const *pots, tm_potential **out);
+void tm_potential_release(tm_potential *p);
+
+/* Potential::execute_device, potential.hpp:51-62. Device pointers; stream is a hipStream_t (NULL = default). */
+int tm_potential_execute_device(tm_potential *p, int N, int P, const double *d_x, const double *d_p,
+                                const double *d_box, double lambda, uint64_t *d_du_dx, uint64_t *d_du_dp,
+                                uint64_t *d_du_dl, uint64_t *d_u, void *stream);
+/* Potential::execute_host, potential.cu:110-185 (du_dl / u are [N] partial sums). */
+int tm_potential_execute_host(tm_potential *p, int N, int P, const double *x, const double *params,
+                              const double *box, double lambda, uint64_t *du_dx, uint64_t *du_dp, uint64_t *du_dl,
+                              uint64_t *u);
+/* Potential::execute_batch_host, potential.cu:15-108. Outputs [Bc, Bp, Bl, ...]. */
+int tm_potential_execute_batch_host(tm_potential *p, int coord_batch, int N, int param_batch, int P,
+                                    int lambda_batch, const double *x, const double *params, const double *boxes,
+                                    const double *lambdas, uint64_t *du_dx, uint64_t *du_dp, uint64_t *du_dl,
+                                    uint64_t *u);
+/* Potential::du_dp_fixed_to_float, potential.cu:227-231 / nonbonded_all_pairs.cu:435-450 */
+int tm_potential_du_dp_fixed_to_float(tm_potential *p, int N, int P, const uint64_t *du_dp, double *out);
+
+/* BoundPotential(potential, params), bound_potential.cu:6-12 */
+int tm_bound_potential_create(tm_potential *p, int ndim, const int32_t *shape, const double *params,
+                              tm_bound_potential **out);
+void tm_bound_potential_release(tm_bound_potential *bp);
+/* BoundPotential.size(), bound_potential.cu:47-57 */
+int tm_bound_potential_size(tm_bound_potential *bp, int *out);
+/* BoundPotential.execute, wrap_kernels.cpp:667-693 (du_dl / u are [N] partial sums) */
+int tm_bound_potential_execute_host(tm_bound_potential *bp, int N, const double *x, const double *box,
+                                    double lambda, uint64_t *du_dx, uint64_t *du_dl, uint64_t *u);
+
+/* ---- neighborlist ----------------------------------------------------- */
+
+/* Neighborlist_f32(N), neighborlist.cu:12-38 */
+int tm_neighborlist_create(int N, tm_neighborlist **out);
+void tm_neighborlist_release(tm_neighborlist *nl);
+/* Neighborlist::resize, neighborlist.cu:263-280 */
+int tm_neighborlist_resize(tm_neighborlist *nl, int size);
+/* Neighborlist::get_nblist_host, neighborlist.cu:83-124: builds the tile list of x
+ * (given order, no spatial sort) and returns the tile count. */
+int tm_neighborlist_build_host(tm_neighborlist *nl, int N, const double *x, const double *box, double cutoff,
+                               int *n_tiles);
+/* Copies the tiles of the last build: row block per tile, 64 column atoms per tile
+ * (0xFFFFFFFF = padding). */
+int tm_neighborlist_get_tiles(tm_neighborlist *nl, int capacity, int32_t *tile_rows, uint32_t *tile_atoms);
+/* Neighborlist::compute_block_bounds_host, neighborlist.cu:56-81 (64-atom blocks). */
+int tm_neighborlist_compute_block_bounds_host(tm_neighborlist *nl, int N, const double *x, const double *box,
+                                              double *ctr, double *ext);
+
+/* ---- integrator, barostat, context ------------------------------------ */
+
+/* LangevinIntegrator(dt, ca, cbs, ccs, seed), wrap_kernels.cpp:314-332, integrator.cu:19-29 */
+int tm_langevin_integrator_create(int N, double dt, double ca, const double *cbs, const double *ccs, uint64_t seed,
+                                  tm_integrator **out);
+void tm_integrator_release(tm_integrator *it);
+
+/* MonteCarloBarostat(N, pressure, temperature, group_idxs, interval, bps, seed),
+ * wrap_kernels.cpp:1016-1033, barostat.cu:16-116. Groups are given as sizes + concatenated atoms. */
+int tm_barostat_create(int N, double pressure, double temperature, int n_groups, const int32_t *group_sizes,
+                       const int32_t *group_atoms, int interval, int n_bps, tm_bound_potential *const *bps,
+                       uint64_t seed, tm_barostat **out);
+int tm_barostat_set_interval(tm_barostat *b, int interval);  /* barostat.cu:375-382 */
+int tm_barostat_get_interval(tm_barostat *b, int *interval); /* barostat.cu:384 */
+int tm_barostat_set_pressure(tm_barostat *b, double pressure); /* barostat.cu:386-391 */
+void tm_barostat_release(tm_barostat *b);
+
+/* Context(x0, v0, box, integrator, bps, barostat), wrap_kernels.cpp:89-121, context.cu:10-37 */
+int tm_context_create(int N, const double *x0, const double *v0, const double *box, tm_integrator *intg, int n_bps,
+                      tm_bound_potential *const *bps, tm_barostat *barostat, tm_context **out);
+void tm_context_release(tm_context *c);
+int tm_context_num_atoms(tm_context *c, int *out);
+/* Context::step, context.cu:235-238 */
+int tm_context_step(tm_context *c, double lambda);
+/* Context::multiple_steps, context.cu:55-137. Intervals > 0 (the Python layer maps 0 to n_steps).
+ * du_dl_out [n_steps / du_dl_interval], x_out [F, N, 3], box_out [F, 3, 3], F = n_steps / x_interval. */
+int tm_context_multiple_steps(tm_context *c, int n_steps, const double *lambda_schedule, int du_dl_interval,
+                              int x_interval, double *du_dl_out, double *x_out, double *box_out);
+/* Context getters / setters, context.cu:284-302 */
+int tm_context_get_x_t(tm_context *c, double *out);
+int tm_context_get_v_t(tm_context *c, double *out);
+int tm_context_get_box(tm_context *c, double *out);
+int tm_context_set_x_t(tm_context *c, const double *in);
+int tm_context_get_du_dx_t_minus_1(tm_context *c, uint64_t *out);
+
+/* ---- batched engine (no reference counterpart: the reference runs one
+ * Context per lambda window per process, fe/estimator.py:84-169) ---------- */
+
+/* Packs the current state of n contexts (each keeps its own potentials,
+ * integrator, barostat and seed) into one engine; lambdas [n] is the lambda
+ * of each system. */
+int tm_batch_create(int n, tm_context *const *contexts, const double *lambdas, tm_batch **out);
+void tm_batch_release(tm_batch *b);
+int tm_batch_num_systems(tm_batch *b, int *out);
+int tm_batch_total_atoms(tm_batch *b, int *out);
+/* Steps every system n_steps. lambda_sched [n, n_steps] or NULL (keep lambdas).
+ * du_dl_out [n, n_steps / du_dl_interval]; x_out concatenated [F, N_s, 3] per system;
+ * box_out [n, F, 3, 3]. Intervals of 0 disable the store. */
+int tm_batch_multiple_steps(tm_batch *b, int n_steps, const double *lambda_sched, int du_dl_interval,
+                            int x_interval, double *du_dl_out, double *x_out, double *box_out, int use_graph);
+/* u and du/dlambda of every system at its current state. */
+int tm_batch_energies(tm_batch *b, double *u_out, double *du_dl_out);
+int tm_batch_get_x(tm_batch *b, int s, double *out);
+int tm_batch_get_v(tm_batch *b, int s, double *out);
+int tm_batch_get_box(tm_batch *b, int s, double *out);
+/* Copies every system's state back into the contexts it came from. */
+int tm_batch_write_back(tm_batch *b);
+/* Average durations in ms over reps launches, timed with HIP events on the
+ * batch's own stream at its current state: [0] neighborlist preparation with a
+ * forced rebuild, [1] pair (tile) kernel, [2] pair lists + bonded, [3] integrator. */
+int tm_batch_profile(tm_batch *b, int reps, double *ms_out);
+/* Tiles in the current neighborlists of all systems. */
+int tm_batch_total_tiles(tm_batch *b, int *out);
+/* Workgroups of the persistent pair kernel (0 = default, CUs x 8). */
+int tm_batch_set_tile_grid(tm_batch *b, int grid);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TM_AMD_H */

@@ -1,0 +1,234 @@
+"""CPU checks of the arithmetic headers shared with the HIP kernels (host build).
+
+The pair, bonded, fixed-point and Philox code in timemachine_amd/csrc is
+compiled here with g++ (-ffp-contract=off) and compared with the FP64
+oracle, closed forms, published known-answer vectors and llrint.
+"""
+
+import ctypes
+import math
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+from scipy.special import erfc
+
+ROOT = Path(__file__).resolve().parents[1]
+P = ctypes.POINTER
+
+
+@pytest.fixture(scope="module")
+def hc(tmp_path_factory):
+    out = tmp_path_factory.mktemp("hc") / "host_check.so"
+    subprocess.run(
+        [
+            "g++",
+            "-O2",
+            "-std=c++17",
+            "-ffp-contract=off",
+            "-shared",
+            "-fPIC",
+            f"-I{ROOT / 'timemachine_amd' / 'csrc'}",
+            str(ROOT / "tests" / "native" / "host_check.cpp"),
+            "-o",
+            str(out),
+        ],
+        check=True,
+    )
+    lib = ctypes.CDLL(str(out))
+    lib.hc_round_to_fixed.restype = ctypes.c_uint64
+    lib.hc_round_to_fixed.argtypes = [ctypes.c_double]
+    lib.hc_f32_to_fixed.restype = ctypes.c_uint64
+    lib.hc_f32_to_fixed.argtypes = [ctypes.c_float, ctypes.c_float]
+    lib.hc_pair_terms.argtypes = [ctypes.c_float] * 10 + [P(ctypes.c_float)]
+    lib.hc_min_image.restype = ctypes.c_float
+    lib.hc_min_image.argtypes = [ctypes.c_float, ctypes.c_float]
+    lib.hc_bond.argtypes = [P(ctypes.c_double)] * 2 + [ctypes.c_float] * 2 + [ctypes.c_int] * 2 + [ctypes.c_double, P(ctypes.c_float)]
+    lib.hc_angle.argtypes = [P(ctypes.c_double)] * 3 + [ctypes.c_float] * 2 + [ctypes.c_int] * 2 + [ctypes.c_double, P(ctypes.c_float)]
+    lib.hc_torsion.argtypes = [P(ctypes.c_double)] * 4 + [ctypes.c_float] * 3 + [ctypes.c_int] * 2 + [ctypes.c_double, P(ctypes.c_float)]
+    return lib
+
+
+def _pd(a):
+    return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(P(ctypes.c_double))
+
+
+# Random123 known-answer vectors for philox4x32-10 (counter, key) -> output.
+PHILOX_KAT = [
+    ([0, 0, 0, 0], (0, 0), [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+    ([0xFFFFFFFF] * 4, (0xFFFFFFFF, 0xFFFFFFFF), [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+    (
+        [0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344],
+        (0xA4093822, 0x299F31D0),
+        [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1],
+    ),
+]
+
+
+@pytest.mark.parametrize("ctr,key,expected", PHILOX_KAT)
+def test_philox_known_answers(hc, ctr, key, expected):
+    c = np.array(ctr, dtype=np.uint32)
+    o = np.zeros(4, dtype=np.uint32)
+    hc.hc_philox(c.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint32(key[0]), ctypes.c_uint32(key[1]), o.ctypes.data_as(ctypes.c_void_p))
+    assert o.tolist() == expected
+
+
+def test_box_muller_statistics(hc):
+    """Normals from consecutive Philox blocks: mean 0, variance 1, no zero-log."""
+    n = 20000
+    z = np.zeros((n, 2))
+    out = np.zeros(4, dtype=np.uint32)
+    for i in range(n):
+        c = np.array([i, 7, 0, 0x4C414E47], dtype=np.uint32)
+        hc.hc_philox(c.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint32(1234), ctypes.c_uint32(0), out.ctypes.data_as(ctypes.c_void_p))
+        hc.hc_normal2(out.ctypes.data_as(ctypes.c_void_p), z[i].ctypes.data_as(P(ctypes.c_double)))
+    assert np.all(np.isfinite(z))
+    assert abs(z.mean()) < 0.02
+    assert abs(z.var() - 1.0) < 0.03
+    assert abs(np.corrcoef(z[:, 0], z[:, 1])[0, 1]) < 0.03
+
+
+def test_round_to_fixed_matches_llrint(hc):
+    """Fast path (|v| < 2^51, magic-number add) and exact path agree with llrint."""
+    from oracle import nblist
+
+    rng = np.random.default_rng(0)
+    vals = list(rng.normal(size=2000) * 10.0 ** rng.uniform(-3, 17, 2000))
+    # ties to even, path boundaries, extremes
+    vals += [0.5, 1.5, 2.5, -0.5, -1.5, -2.5, 2.0**51 - 0.5, 2.0**51, -(2.0**51), 2.0**51 + 1, 2.0**62, -(2.0**62),
+             2.0**63, -(2.0**63), float("nan"), float("inf"), -float("inf"), 0.0, -0.0, 1e-300]
+    for v in vals:
+        assert hc.hc_round_to_fixed(v) == nblist.round_to_fixed(v), v
+
+
+def test_f32_to_fixed_is_odd(hc):
+    """Newton's third law in fixed point needs fixed(-v) == -fixed(v) exactly."""
+    rng = np.random.default_rng(1)
+    for v in rng.normal(size=3000).astype(np.float32) * np.float32(1e4):
+        a = hc.hc_f32_to_fixed(float(v), 2.0**36)
+        b = hc.hc_f32_to_fixed(float(-v), 2.0**36)
+        assert (a + b) % 2**64 == 0
+
+
+def _pair_ref(d, qi, qj, si, sj, ei, ej, beta, sq, slj):
+    """FP64 closed form: values and the magnitudes of their summed parts
+    (the scale a relative error is measured against when parts cancel)."""
+    u_es = sq * qi * qj * erfc(beta * d) / d
+    du_es = sq * qi * qj * (-2 * beta / math.sqrt(math.pi) * math.exp(-((beta * d) ** 2)) / d - erfc(beta * d) / d**2)
+    vals = [du_es / d, u_es, erfc(beta * d) / d, 0.0, 0.0]
+    mags = [abs(du_es / d), abs(u_es), erfc(beta * d) / d, 0.0, 0.0]
+    if ei != 0 and ej != 0:
+        s = si + sj
+        e = ei * ej
+        vals[0] += slj * 4 * e * (-12 * s**12 / d**13 + 6 * s**6 / d**7) / d
+        mags[0] += abs(slj) * 4 * e * (12 * s**12 / d**13 + 6 * s**6 / d**7) / d
+        vals[1] += slj * 4 * e * ((s / d) ** 12 - (s / d) ** 6)
+        mags[1] += abs(slj) * 4 * e * ((s / d) ** 12 + (s / d) ** 6)
+        vals[3] = slj * 4 * e * (12 * s**11 / d**12 - 6 * s**5 / d**6)
+        mags[3] = abs(slj) * 4 * e * (12 * s**11 / d**12 + 6 * s**5 / d**6)
+        vals[4] = slj * 4 * ((s / d) ** 12 - (s / d) ** 6)
+        mags[4] = abs(slj) * 4 * ((s / d) ** 12 + (s / d) ** 6)
+    return vals, mags
+
+
+def test_pair_terms_against_closed_form(hc):
+    """FP32 pair arithmetic (erfc fit included) vs the FP64 closed form, relative
+    to the magnitude of the parts that are summed (errors must stay FP32-sized)."""
+    rng = np.random.default_rng(3)
+    out = (ctypes.c_float * 6)()
+    worst = 0.0
+    f = lambda v: float(np.float32(v))  # noqa: E731
+    for d in np.linspace(0.12, 1.19, 400):
+        qi, qj = rng.normal(0, 5, 2)
+        si, sj = rng.uniform(0.1, 0.2, 2)
+        ei, ej = rng.uniform(0.3, 1.0, 2)
+        d = f(d)
+        hc.hc_pair_terms(d * d, qi, qj, si, sj, ei, ej, 2.0, 1.0, 1.0, out)
+        d2 = f(d * d)
+        vals, mags = _pair_ref(math.sqrt(d2), f(qi), f(qj), f(si), f(sj), f(ei), f(ej), 2.0, 1.0, 1.0)
+        for k in range(5):
+            worst = max(worst, abs(out[k] - vals[k]) / max(mags[k], 1e-30))
+        assert out[5] == 1.0
+    assert worst < 5e-6, worst
+
+
+def test_erfc_fit_relative_error(hc):
+    """erfc(beta d)/d of the pair code vs scipy over the cutoff range: < 1.5e-6 relative."""
+    out = (ctypes.c_float * 6)()
+    errs = []
+    for d in np.linspace(0.05, 1.3, 2000).astype(np.float32):
+        hc.hc_pair_terms(float(d) * float(d), 1.0, 1.0, 0.0, 0.0, 0.0, 0.0, 2.0, 1.0, 1.0, out)
+        dd = math.sqrt(float(np.float32(float(d) * float(d))))
+        ref = erfc(2.0 * dd) / dd
+        errs.append(abs(out[2] / ref - 1))
+    assert max(errs) < 1.5e-6
+    assert abs(np.mean(errs)) < 4e-7
+
+
+def test_lj_skipped_when_either_eps_zero(hc):
+    out = (ctypes.c_float * 6)()
+    hc.hc_pair_terms(0.09, 1.0, -1.0, 0.15, 0.0, 0.8, 0.0, 2.0, 1.0, 1.0, out)
+    assert out[5] == 0.0 and out[3] == 0.0 and out[4] == 0.0
+
+
+def test_min_image_symmetric(hc):
+    rng = np.random.default_rng(4)
+    for d in rng.uniform(-6, 6, 500).astype(np.float32):
+        a = hc.hc_min_image(float(d), 3.0)
+        b = hc.hc_min_image(float(-d), 3.0)
+        assert a == -b
+        assert abs(a) <= 1.5 + 1e-6
+
+
+def _bonded_host(hc, kind, x, idx, params, mult, offset, lam):
+    n = {"bond": 10, "angle": 13, "torsion": 17}[kind]
+    out = (ctypes.c_float * n)()
+    xs = [_pd(x[i]) for i in idx]
+    if kind == "bond":
+        hc.hc_bond(*xs, float(params[0]), float(params[1]), mult, offset, lam, out)
+    elif kind == "angle":
+        hc.hc_angle(*xs, float(params[0]), float(params[1]), mult, offset, lam, out)
+    else:
+        hc.hc_torsion(*xs, float(params[0]), float(params[1]), float(params[2]), mult, offset, lam, out)
+    return np.array(out[:])
+
+
+@pytest.mark.parametrize("kind,arity,params", [
+    ("bond", 2, [2e5, 0.13]),
+    ("bond", 2, [1e3, 0.0]),
+    ("angle", 3, [400.0, 1.95]),
+    ("torsion", 4, [6.0, math.pi, 3.0]),
+    ("torsion", 4, [2.5, 0.0, 1.0]),
+])
+def test_bonded_terms_against_oracle(hc, kind, arity, params):
+    from oracle import bonded
+
+    rng = np.random.default_rng(5)
+    for trial in range(20):
+        x = rng.normal(size=(arity, 3)) * 0.1
+        x = x.astype(np.float32).astype(np.float64)
+        mult, offset, lam = (1, 0, 0.3) if trial % 2 else (0, 1, 0.0)
+        idx = list(range(arity))
+        h = _bonded_host(hc, kind, x, idx, params, mult, offset, lam)
+        p = np.array(params, dtype=np.float64)[None, :]
+        gx, gp, gl, u = bonded.bonded(kind, x, p, lam, np.array([idx]), mult=[mult], offset=[offset])
+        g = h[: arity * 3].reshape(arity, 3)
+        np.testing.assert_allclose(g, gx, rtol=2e-4, atol=1e-3 * max(1.0, np.abs(gx).max()) * 1e-2)
+        np.testing.assert_allclose(h[arity * 3 + len(params)], u, rtol=2e-4, atol=1e-6)
+        np.testing.assert_allclose(h[arity * 3 : arity * 3 + len(params)], gp[0], rtol=2e-4, atol=1e-4)
+        np.testing.assert_allclose(h[-1], gl, rtol=2e-4, atol=1e-5)
+
+
+def test_bonded_index_reversal_is_bitwise(hc):
+    """Reversed index tuples give bitwise identical per-atom forces (tests/test_bonded.py:179-197)."""
+    rng = np.random.default_rng(6)
+    for kind, arity, params in [("bond", 2, [1e4, 0.1]), ("angle", 3, [300.0, 2.0]), ("torsion", 4, [3.0, 0.5, 2.0])]:
+        for _ in range(50):
+            x = rng.normal(size=(arity, 3)).astype(np.float32).astype(np.float64)
+            fwd = _bonded_host(hc, kind, x, list(range(arity)), params, 0, 1, 0.0)
+            rev = _bonded_host(hc, kind, x, list(reversed(range(arity))), params, 0, 1, 0.0)
+            gf = fwd[: arity * 3].reshape(arity, 3)
+            gr = rev[: arity * 3].reshape(arity, 3)[::-1]
+            assert np.array_equal(gf, gr)
+            assert fwd[arity * 3 + len(params)] == rev[arity * 3 + len(params)]

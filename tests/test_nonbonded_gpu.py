@@ -1,0 +1,205 @@
+"""Nonbonded parity on the GPU (HIP path through the C ABI) against the FP64 oracle.
+
+Tolerances (stated targets): forces |dF| / max(|F|, 1) <= 1e-5 per atom,
+energy |dU| / |U| <= 1e-6, du_dl |d| <= 1e-5 * max(|du_dl|, 1), du_dp per
+column |d| <= 1e-5 * max|column|. Fixed-point outputs must be bitwise
+identical run to run, across sort orders, paddings, output selections and
+batch composition.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import nblist as onblist
+from oracle import nonbonded as onb
+from timemachine_amd import testsystems as ts
+from timemachine_amd.lib import custom_ops, potentials
+
+pytestmark = pytest.mark.gpu
+
+F32 = np.float32
+
+
+@pytest.fixture(scope="module")
+def water(gpu):
+    return ts.water_box()
+
+
+@pytest.fixture(scope="module")
+def ligand(gpu):
+    return ts.ligand_in_water()
+
+
+def _oracle(sys, lam):
+    return onb.nonbonded_pairs(
+        sys.x, sys.nb_params, sys.box, lam, sys.exclusion_idxs, sys.scales, sys.beta, sys.cutoff,
+        sys.lambda_plane_idxs, sys.lambda_offset_idxs,
+    )
+
+
+def _check_parity(out, ref, label):
+    du_dx, du_dp, du_dl, u = out
+    rdx, rdp, rdl, ru = ref
+    ferr = onb.force_rel_err(du_dx, rdx)
+    print(f"[{label}] max force rel err {ferr.max():.3e}  u {u:.6f} ref {ru:.6f} rel {abs(u - ru) / abs(ru):.3e}  "
+          f"du_dl {du_dl:.6f} ref {rdl:.6f}")
+    assert ferr.max() <= 1e-5
+    assert abs(u - ru) <= 1e-6 * abs(ru)
+    assert abs(du_dl - rdl) <= 1e-5 * max(abs(rdl), 1.0)
+    for c in range(3):
+        scale = max(np.abs(rdp[:, c]).max(), 1e-12)
+        err = np.abs(du_dp[:, c] - rdp[:, c]).max() / scale
+        print(f"[{label}] du_dp column {c} max err / max|ref| = {err:.3e}")
+        assert err <= 1e-5
+
+
+def test_water_box_parity(water):
+    """Water box, 887 waters, 3.0 nm, rc 1.2, beta 2, lambda 0: all four outputs vs FP64."""
+    pot = water.nonbonded().unbound_impl(F32)
+    out = pot.execute(water.x, water.nb_params, water.box, 0.0)
+    _check_parity(out, _oracle(water, 0.0), "water")
+    assert out[2] == 0.0  # every tile is vanilla
+
+
+@pytest.mark.parametrize("lam", [0.0, 0.35, 0.5, 1.0])
+def test_ligand_parity(ligand, lam):
+    """Ligand in water with 4D offsets on the ligand: decoupling tiles and du_dl."""
+    pot = ligand.nonbonded().unbound_impl(F32)
+    out = pot.execute(ligand.x, ligand.nb_params, ligand.box, lam)
+    _check_parity(out, _oracle(ligand, lam), f"ligand lam={lam}")
+
+
+def test_run_to_run_bitwise(water):
+    pot = water.nonbonded().unbound_impl(F32)
+    a = pot.execute_fixed_raw(water.x, water.nb_params, water.box, 0.0)
+    b = pot.execute_fixed_raw(water.x, water.nb_params, water.box, 0.0)
+    for p, q in zip(a, b):
+        np.testing.assert_array_equal(p, q)
+
+
+def test_sort_and_padding_do_not_change_bits(ligand):
+    """Hilbert sort on/off and any neighborlist padding give identical fixed-point
+    results (tests/test_nonbonded.py:48-158 in the reference)."""
+    base = None
+    for sort in (True, False):
+        for pad in (0.0, 0.1, 0.25):
+            pot = ligand.nonbonded().unbound_impl(F32)
+            if not sort:
+                pot.disable_hilbert_sort()
+            pot.set_nblist_padding(pad)
+            res = pot.execute_fixed_raw(ligand.x, ligand.nb_params, ligand.box, 0.4)
+            sums = [np.sum(r, dtype=np.uint64) for r in res[2:]]
+            cur = [res[0], res[1], *sums]
+            if base is None:
+                base = cur
+            else:
+                for p, q in zip(base, cur):
+                    np.testing.assert_array_equal(p, q)
+
+
+def test_newton_third_law_exact(ligand):
+    """Every pair adds +g to one atom and -g to the other: the fixed-point sum is 0."""
+    pot = ligand.nonbonded().unbound_impl(F32)
+    du_dx = pot.execute_fixed_raw(ligand.x, ligand.nb_params, ligand.box, 0.3)[0].reshape(-1, 3)
+    assert np.all(np.sum(du_dx, axis=0, dtype=np.uint64) == 0)
+
+
+def test_exclusion_cancels_exactly(gpu):
+    """AllPairs + PairListNegated(scale 1) on an excluded pair is exactly zero."""
+    x = np.array([[0.1, 0.2, 0.3], [0.35, 0.3, 0.25], [1.6, 1.6, 1.6]])
+    p = np.array([[3.0, 0.15, 0.8], [-2.0, 0.12, 0.9], [0.0, 0.0, 0.0]])
+    box = np.eye(3) * 3.0
+    nb = potentials.Nonbonded(np.array([[0, 1]], np.int32), np.array([[1.0, 1.0]]), np.zeros(3, np.int32),
+                              np.array([0, 1, 0], np.int32), 2.0, 1.2).unbound_impl(F32)
+    for lam in (0.0, 0.3):
+        du_dx, du_dp, du_dl, u = nb.execute_fixed_raw(x, p, box, lam)
+        assert np.all(du_dx == 0) and np.all(du_dp == 0)
+        assert np.sum(u, dtype=np.uint64) == 0 and np.sum(du_dl, dtype=np.uint64) == 0
+
+
+def test_selective_outputs_are_bitwise_consistent(ligand):
+    """All 16 output selections give the same bits for what they compute."""
+    pot = ligand.nonbonded().unbound_impl(F32)
+    full = pot.execute_selective(ligand.x, ligand.nb_params, ligand.box, 0.25, True, True, True, True)
+    for mask in range(16):
+        flags = [bool(mask & (1 << k)) for k in range(4)]
+        res = pot.execute_selective(ligand.x, ligand.nb_params, ligand.box, 0.25, *flags)
+        for k in range(4):
+            if flags[k]:
+                np.testing.assert_array_equal(np.asarray(res[k]), np.asarray(full[k]))
+            else:
+                assert res[k] is None
+
+
+def test_batch_matches_single(water):
+    """execute_selective_batch over coords x params x lambdas == per-item execute, bitwise."""
+    pot = water.nonbonded().unbound_impl(F32)
+    rng = np.random.default_rng(0)
+    xs = np.stack([water.x, (water.x + rng.normal(0, 0.002, water.x.shape)).astype(F32).astype(np.float64)])
+    boxes = np.stack([water.box, water.box])
+    ps = np.stack([water.nb_params, water.nb_params * np.array([1.0, 1.01, 0.99])])
+    lams = np.array([0.0, 0.7])
+    dx, dp, dl, u = pot.execute_selective_batch(xs, ps, boxes, lams, True, True, True, True)
+    for i in range(2):
+        for j in range(2):
+            for k in range(2):
+                r = pot.execute(xs[i], ps[j], boxes[i], lams[k])
+                np.testing.assert_array_equal(dx[i, j, k], r[0])
+                np.testing.assert_array_equal(dp[i, j, k], r[1])
+                assert dl[i, j, k] == r[2] and u[i, j, k] == r[3]
+
+
+def test_box_validation_messages(water):
+    pot = water.nonbonded().unbound_impl(F32)
+    with pytest.raises(RuntimeError, match="Cutoff with padding is more than half of the box width"):
+        pot.execute(water.x, water.nb_params, np.eye(3) * 2.0, 0.0)
+    bad = water.box.copy()
+    bad[0, 1] = 0.1
+    with pytest.raises(RuntimeError, match="Provided non-ortholinear box"):
+        pot.execute(water.x, water.nb_params, bad, 0.0)
+    ap = custom_ops.NonbondedAllPairs_f32("", np.zeros(water.N, np.int32), np.zeros(water.N, np.int32), 2.0, 1.2)
+    with pytest.raises(RuntimeError, match=r"expected N == N_, got N=3, N_=2661"):
+        ap.execute(np.zeros((3, 3)), np.zeros((3, 3)), water.box, 0.0)
+    with pytest.raises(RuntimeError, match=r"expected P == M\*N_\*3"):
+        ap.execute(water.x, np.zeros((2, 3)), water.box, 0.0)
+
+
+@pytest.mark.parametrize("size", [35, 64, 129, 1025, 1259, 2029])
+def test_neighborlist_exact(gpu, size):
+    """GPU tile neighborlist == the FP32 predicate, bit for bit (64-atom blocks), and
+    == the f64 reference of tests/test_nblist.py:70-94 away from the cutoff boundary."""
+    from pathlib import Path
+
+    water = np.load(Path(__file__).parent / "golden" / "water.npy", allow_pickle=False)[:, :3]
+    water = water.astype(np.float32).astype(np.float64)
+    rng = np.random.default_rng(1234)
+    x = water[rng.choice(np.arange(size), size, replace=False)]
+    box = np.eye(3) * (x.max(0) - x.min(0) + 0.1)
+    cutoff = 1.0
+    nl = custom_ops.Neighborlist_f32(size)
+    for _ in range(2):
+        test = nl.get_nblist(x, box, cutoff)
+        ref32 = onblist.fp32_ixn_list(x, box, cutoff, 64)
+        assert [sorted(t) for t in test] == ref32
+    ref64 = onblist.build_reference_ixn_list(x, box, cutoff, 64)
+    nmis = sum(len(set(a) ^ set(b)) for a, b in zip(ref64, ref32))
+    assert nmis <= 2
+
+
+def test_neighborlist_resize_errors(gpu):
+    nl = custom_ops.Neighborlist_f32(3)
+    with pytest.raises(RuntimeError, match="size is must be at least 1"):
+        nl.resize(0)
+    with pytest.raises(RuntimeError, match="size is greater than max size"):
+        nl.resize(4)
+
+
+def test_neighborlist_covers_all_pairs_within_cutoff(ligand):
+    """Every pair i < j within the cutoff has j in the list of i's row block."""
+    nl = custom_ops.Neighborlist_f32(ligand.N)
+    lists = nl.get_nblist(ligand.x, ligand.box, 1.3)
+    M = np.zeros((len(lists), ligand.N), dtype=bool)
+    for r, lst in enumerate(lists):
+        M[r, lst] = True
+    pairs = onb.candidate_pairs(ligand.x, ligand.box, 1.2)
+    assert M[pairs[:, 0] // 64, pairs[:, 1]].all()

@@ -1,0 +1,277 @@
+"""Pins the oracle (CPU, no GPU): closed forms, brute force, finite differences,
+invariants and the reference's own fixture data. A plausible mistake in the
+oracle (dropped term, wrong sign, transposed index) fails one of these."""
+
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+from scipy.special import erfc
+
+from oracle import barostat, bonded, integrator, nblist, nonbonded
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def _tiny_system(rng, N=24, L=2.7, lam=0.3, with_offsets=True):
+    x = rng.uniform(0, L, size=(N, 3))
+    # keep atoms apart so energies are moderate
+    params = np.stack([rng.normal(0, 3, N), rng.uniform(0.08, 0.16, N), rng.uniform(0.0, 1.0, N)], 1)
+    params[::5, 2] = 0.0  # some eps = 0 atoms (LJ off)
+    box = np.eye(3) * L
+    excl = np.array([[0, 1], [2, 3], [4, 7], [10, 11]])
+    scales = np.array([[1.0, 1.0], [0.5, 0.5], [1.0, 0.0], [0.3, 0.8]])
+    plane = np.zeros(N, np.int32)
+    offset = np.zeros(N, np.int32)
+    if with_offsets:
+        offset[:6] = 1
+    return x, params, box, lam, excl, scales, plane, offset
+
+
+def test_two_particle_closed_form():
+    """u = q1 q2 erfc(beta r)/r + 4 eps ((s/r)^12 - (s/r)^6) with s = s1 + s2, eps = e1 e2."""
+    r = 0.37
+    x = np.array([[0.2, 0.3, 0.4], [0.2 + r, 0.3, 0.4]])
+    p = np.array([[1.3, 0.12, 0.8], [-0.7, 0.14, 0.6]])
+    box = np.eye(3) * 3.0
+    beta, cutoff = 2.0, 1.2
+    gx, gp, gl, u = nonbonded.nonbonded_v3(x, p, box, 0.0, np.zeros((0, 2), int), np.zeros((0, 2)), beta, cutoff, [0, 0], [0, 0])
+    s, e = 0.26, 0.48
+    u_ref = 1.3 * -0.7 * erfc(beta * r) / r + 4 * e * ((s / r) ** 12 - (s / r) ** 6)
+    du_dr = 1.3 * -0.7 * (-2 * beta / math.sqrt(math.pi) * math.exp(-((beta * r) ** 2)) / r - erfc(beta * r) / r**2)
+    du_dr += 4 * e * (-12 * s**12 / r**13 + 6 * s**6 / r**7)
+    assert u == pytest.approx(u_ref, rel=1e-12)
+    assert gx[1, 0] == pytest.approx(du_dr, rel=1e-10)
+    assert gx[0, 0] == pytest.approx(-du_dr, rel=1e-10)
+    # d/d q1 = q2 erfc/r ; d/d eps1 = e2 * 4((s/r)^12 - (s/r)^6) ; d/d sig1 = d/ds
+    assert gp[0, 0] == pytest.approx(-0.7 * erfc(beta * r) / r, rel=1e-12)
+    assert gp[0, 2] == pytest.approx(0.6 * 4 * ((s / r) ** 12 - (s / r) ** 6), rel=1e-10)
+    assert gp[0, 1] == pytest.approx(4 * e * (12 * s**11 / r**12 - 6 * s**5 / r**6), rel=1e-10)
+    assert gl == 0.0
+
+
+def test_cutoff_and_4d_decoupling():
+    """Beyond the cutoff nothing interacts; w = lambda * cutoff on one atom turns the pair off at lambda = 1."""
+    x = np.array([[0.0, 0.0, 0.0], [0.5, 0.0, 0.0]])
+    p = np.array([[1.0, 0.1, 0.5], [1.0, 0.1, 0.5]])
+    box = np.eye(3) * 3.0
+    E = np.zeros((0, 2), int), np.zeros((0, 2))
+    u_on = nonbonded.nonbonded_v3(x, p, box, 0.0, *E, 2.0, 1.2, [0, 0], [0, 1])[3]
+    u_off = nonbonded.nonbonded_v3(x, p, box, 1.0, *E, 2.0, 1.2, [0, 0], [0, 1])[3]
+    u_half = nonbonded.nonbonded_v3(x, p, box, 0.5, *E, 2.0, 1.2, [0, 0], [0, 1])[3]
+    r = math.sqrt(0.25 + 0.6**2)
+    assert u_off == 0.0
+    assert u_on != 0.0
+    s6 = (0.2 / r) ** 6
+    assert u_half == pytest.approx(erfc(2 * r) / r + 4 * 0.25 * (s6 * s6 - s6), rel=1e-12)
+
+
+def test_dense_equals_pair_evaluator(rng):
+    args = _tiny_system(rng)
+    a = nonbonded.nonbonded_v3(*args[:4], args[4], args[5], 2.0, 1.2, args[6], args[7])
+    b = nonbonded.nonbonded_pairs(*args[:4], args[4], args[5], 2.0, 1.2, args[6], args[7])
+    np.testing.assert_allclose(a[0], b[0], rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(a[1], b[1], rtol=1e-10, atol=1e-10)
+    assert a[2] == pytest.approx(b[2], rel=1e-10, abs=1e-10)
+    assert a[3] == pytest.approx(b[3], rel=1e-12)
+
+
+def test_finite_differences(rng):
+    x, p, box, lam, excl, sc, pl, of = _tiny_system(rng, N=12)
+    gx, gp, gl, u = nonbonded.nonbonded_v3(x, p, box, lam, excl, sc, 2.0, 1.2, pl, of)
+    qm, ljm = nonbonded.convert_exclusions_to_rescale_masks(excl, sc, len(x))
+
+    def U(xx, pp, ll):
+        T = lambda a: torch.tensor(a, dtype=torch.float64)  # noqa: E731
+        return float(nonbonded.nonbonded_v3_energy(T(xx), T(pp), box, T(ll), qm, ljm, 2.0, 1.2, pl, of))
+
+    h = 1e-6
+    for i, k in [(0, 0), (3, 2), (7, 1), (11, 0)]:
+        xp, xm = x.copy(), x.copy()
+        xp[i, k] += h
+        xm[i, k] -= h
+        assert gx[i, k] == pytest.approx((U(xp, p, lam) - U(xm, p, lam)) / (2 * h), rel=1e-5, abs=1e-5)
+    for i, k in [(1, 0), (2, 1), (4, 2)]:
+        pp, pm = p.copy(), p.copy()
+        pp[i, k] += h
+        pm[i, k] -= h
+        assert gp[i, k] == pytest.approx((U(x, pp, lam) - U(x, pm, lam)) / (2 * h), rel=1e-5, abs=1e-5)
+    assert gl == pytest.approx((U(x, p, lam + h) - U(x, p, lam - h)) / (2 * h), rel=1e-5, abs=1e-5)
+
+
+def test_invariants(rng):
+    x, p, box, lam, excl, sc, pl, of = _tiny_system(rng)
+    gx, _, _, u = nonbonded.nonbonded_pairs(x, p, box, lam, excl, sc, 2.0, 1.2, pl, of)
+    np.testing.assert_allclose(gx.sum(0), 0.0, atol=1e-9)  # Newton's third law
+    u2 = nonbonded.nonbonded_pairs(x + np.array([0.31, -1.7, 2.9]), p, box, lam, excl, sc, 2.0, 1.2, pl, of)[3]
+    assert u2 == pytest.approx(u, rel=1e-12)  # periodic translation invariance
+    gl0 = nonbonded.nonbonded_pairs(x, p, box, lam, excl, sc, 2.0, 1.2, pl, np.zeros_like(of))[2]
+    assert gl0 == 0.0  # no offsets -> no lambda dependence
+
+
+def test_full_exclusion_removes_pair(rng):
+    x, p, box, lam, _, _, pl, of = _tiny_system(rng, with_offsets=False)
+    full = nonbonded.nonbonded_pairs(x, p, box, 0.0, np.array([[2, 9]]), np.array([[1.0, 1.0]]), 2.0, 1.2, pl, of)[3]
+    none = nonbonded.nonbonded_pairs(x, p, box, 0.0, np.zeros((0, 2), int), np.zeros((0, 2)), 2.0, 1.2, pl, of)[3]
+    d = x[2] - x[9]
+    d -= np.diag(box) * np.floor(d / np.diag(box) + 0.5)
+    r = np.linalg.norm(d)
+    u_pair = 0.0
+    if r < 1.2:
+        u_pair = p[2, 0] * p[9, 0] * erfc(2 * r) / r
+        if p[2, 2] != 0 and p[9, 2] != 0:
+            s = p[2, 1] + p[9, 1]
+            u_pair += 4 * p[2, 2] * p[9, 2] * ((s / r) ** 12 - (s / r) ** 6)
+    assert full == pytest.approx(none - u_pair, rel=1e-12)
+
+
+def test_force_error_metric():
+    ref = np.array([[3.0, 4.0, 0.0], [0.1, 0.0, 0.0]])
+    test = np.array([[3.0, 4.0, 0.5], [0.1, 0.2, 0.0]])
+    np.testing.assert_allclose(nonbonded.force_rel_err(test, ref), [0.1, 0.2])
+
+
+# ------------------------------------------------------------------- bonded
+
+
+def test_bond_closed_form():
+    x = np.array([[0.0, 0.0, 0.0], [0.0, 0.3, 0.4]])
+    gx, gp, gl, u = bonded.bonded("bond", x, np.array([[100.0, 0.4]]), 0.0, np.array([[0, 1]]))
+    assert u == pytest.approx(100 / 2 * 0.1**2)
+    np.testing.assert_allclose(gx[1], 100 * 0.1 * np.array([0, 0.6, 0.8]), rtol=1e-12)
+    np.testing.assert_allclose(gp[0], [0.005, -10.0], rtol=1e-12)
+
+
+def test_angle_closed_form():
+    x = np.array([[1.0, 0.0, 0.0], [0.0, 0.0, 0.0], [0.0, 2.0, 0.0]])  # 90 degrees
+    gx, gp, gl, u = bonded.bonded("angle", x, np.array([[50.0, math.pi / 3]]), 0.0, np.array([[0, 1, 2]]))
+    assert u == pytest.approx(25.0 * (0.0 - 0.5) ** 2)
+
+
+def test_torsion_closed_form():
+    # trans (phi = pi) and cis (phi = 0) geometries
+    trans = np.array([[1.0, 1.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 0.0], [-1.0, 0.0, 0.0]])
+    cis = np.array([[1.0, 1.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 0.0], [1.0, 0.0, 0.0]])
+    ang_t = float(bonded.signed_torsion_angle(*[torch.tensor(v[None]) for v in trans]))
+    ang_c = float(bonded.signed_torsion_angle(*[torch.tensor(v[None]) for v in cis]))
+    assert abs(abs(ang_t) - math.pi) < 1e-12 and abs(ang_c) < 1e-12
+    u = bonded.bonded("torsion", trans, np.array([[3.0, 0.0, 1.0]]), 0.0, np.array([[0, 1, 2, 3]]))[3]
+    assert u == pytest.approx(3.0 * (1 + math.cos(math.pi)), abs=1e-12)
+    # a 90 degree dihedral has the sign of the reference convention
+    q = np.array([[1.0, 1.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 0.0], [0.0, 0.0, 1.0]])
+    a = float(bonded.signed_torsion_angle(*[torch.tensor(v[None]) for v in q]))
+    assert abs(abs(a) - math.pi / 2) < 1e-12
+
+
+@pytest.mark.parametrize("kind,arity,params", [("bond", 2, [300.0, 0.12]), ("angle", 3, [200.0, 1.9]), ("torsion", 4, [4.0, 0.7, 2.0])])
+def test_bonded_finite_differences(rng, kind, arity, params):
+    x = rng.normal(size=(arity, 3)) * 0.2
+    idx = np.arange(arity)[None]
+    p = np.array([params])
+    gx, gp, gl, u = bonded.bonded(kind, x, p, 0.4, idx, mult=[1], offset=[0])
+    h = 1e-6
+    for i in range(arity):
+        for k in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[i, k] += h
+            xm[i, k] -= h
+            fd = (bonded.bonded(kind, xp, p, 0.4, idx, [1], [0])[3] - bonded.bonded(kind, xm, p, 0.4, idx, [1], [0])[3]) / (2 * h)
+            assert gx[i, k] == pytest.approx(fd, rel=1e-5, abs=1e-6)
+    # du/dlambda = mult * u / prefactor
+    assert gl == pytest.approx(u / 0.4, rel=1e-10)
+
+
+# ------------------------------------------------------------------- neighborlist
+
+
+def _water(n):
+    x = np.load(GOLDEN / "water.npy", allow_pickle=False)[:, :3].astype(np.float32).astype(np.float64)
+    x = x[:n]
+    box = np.eye(3) * (x.max(0) - x.min(0) + 0.1)
+    return x, box
+
+
+@pytest.mark.parametrize("n", [35, 64, 129, 301])
+def test_nblist_reference_equals_brute_force(n):
+    x, box = _water(n)
+    assert nblist.build_reference_ixn_list(x, box, 1.0, 64) == nblist.brute_force_ixn_list(x, box, 1.0, 64)
+
+
+@pytest.mark.parametrize("n", [35, 129, 1025, 2029])
+def test_nblist_fp32_predicate_matches_f64_away_from_the_boundary(n):
+    """The FP32 set differs from the f64 set only on pairs within 1e-5 nm of the cutoff."""
+    x, box = _water(n)
+    cutoff = 1.0
+    ref = nblist.build_reference_ixn_list(x, box, cutoff, 64)
+    f32 = nblist.fp32_ixn_list(x, box, cutoff, 64)
+    L = np.diag(box)
+    for r, (a, b) in enumerate(zip(ref, f32)):
+        for j in set(a) ^ set(b):
+            rows = x[r * 64 : min(r * 64 + 64, n)]
+            d = rows - x[j]
+            d -= L * np.floor(d / L + 0.5)
+            dmin = np.linalg.norm(d, axis=1).min()
+            assert abs(dmin - cutoff) < 1e-5, (r, j, dmin)
+
+
+def test_nblist_reference_with_32_blocks_matches_reference_definition():
+    """Block size 32 reproduces the reference's own definition (tests/test_nblist.py:70-94)."""
+    x, box = _water(200)
+    lists = nblist.build_reference_ixn_list(x, box, 1.0, 32)
+    assert len(lists) == 7
+    assert all(all(j >= 32 * r for j in lst) for r, lst in enumerate(lists))
+    assert all(set(range(32 * r, min(32 * r + 32, 200))) <= set(lst) for r, lst in enumerate(lists))
+
+
+# ------------------------------------------------------------------- integrator, barostat
+
+
+def test_langevin_coefficients_closed_form():
+    ca, cb, cc = integrator.langevin_coefficients(300.0, 0.0025, 1.0, np.array([1.0, 16.0]))
+    assert ca == pytest.approx(math.exp(-0.0025))
+    np.testing.assert_allclose(cb, [0.0025, 0.0025 / 16])
+    kT = integrator.BOLTZ * 300
+    np.testing.assert_allclose(cc, math.sqrt(1 - math.exp(-0.005)) * np.sqrt(kT / np.array([1.0, 16.0])))
+
+
+def test_baoab_friction_zero_is_leapfrog():
+    """ca = 1, cc = 0: the rotated BAOAB step is leapfrog (v lives at half steps).
+    A harmonic oscillator started at rest follows cos(w t) to O(dt) phase and
+    its leapfrog energy stays bounded."""
+    k, m, dt = 4.0, 1.0, 0.01
+    x = np.array([[1.0, 0.0, 0.0]])
+    v = np.zeros((1, 3))
+    cbs = np.array([-dt / m])
+    w = math.sqrt(k / m)
+    energies = []
+    for n in range(1, 1001):
+        x, v = integrator.baoab_step(x, v, k * x, dt, 1.0, cbs, np.zeros(1), np.zeros((1, 3)))
+        energies.append(0.5 * k * x[0, 0] ** 2 + 0.5 * m * v[0, 0] ** 2)
+        assert x[0, 0] == pytest.approx(math.cos(w * n * dt), abs=2e-2)
+    assert max(energies) / min(energies) < 1.03
+
+
+def test_barostat_scaling_preserves_molecules(rng):
+    box = np.eye(3) * 3.0
+    groups = [np.arange(3 * i, 3 * i + 3) for i in range(10)]
+    x = rng.uniform(0, 3, size=(30, 3))
+    x2, box2 = barostat.scale_centroids(x, box, groups, 1.01)
+    for g in groups:
+        np.testing.assert_allclose(np.diff(x2[g], axis=0), np.diff(x[g], axis=0), atol=1e-12)
+        c, c2 = x[g].mean(0), x2[g].mean(0)
+        np.testing.assert_allclose(c2 - 1.5, (c - 1.5) * 1.01, atol=1e-12)
+    np.testing.assert_allclose(np.diag(box2), 3.03)
+
+
+def test_barostat_ideal_gas_acceptance():
+    """With no energy change the volume move only sees P dV - N kT ln(V'/V)."""
+    V, dV = 27.0, 0.2
+    w = barostat.acceptance_work(0.0, 0.0, V, dV, 100, 1.0, 300.0)
+    p = 1.0 * barostat.AVOGADRO * 1e-25
+    assert w == pytest.approx(p * dV - 100 * barostat.BOLTZ * 300 * math.log((V + dV) / V))
+    assert barostat.accept(-1.0, 0.999, 300.0)
+    assert not barostat.accept(100.0, 0.5, 300.0)
+    assert barostat.adapt_volume_scale(1.0, 1, 10, 27.0) == (1.0 / 1.1, 0, 0)
+    assert barostat.adapt_volume_scale(1.0, 9, 10, 27.0) == (1.1, 0, 0)

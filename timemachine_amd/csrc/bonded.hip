@@ -1,0 +1,162 @@
+// HarmonicBond / HarmonicAngle / PeriodicTorsion kernels, one thread per term,
+// over the terms of one or many systems (term_sys maps a term to its system
+// for lambda and the reduction index). Atom indices are global.
+#include "bonded.h"
+#include "bonded_terms.h"
+#include "fixed_point.h"
+#include "wave.h"
+
+namespace tmamd {
+
+namespace {
+
+__device__ __forceinline__ void add_fixed(uint64_t *p, float v) {
+    atomicAdd(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(f32_to_fixed(v, FIXED_SCALE_F)));
+}
+
+__device__ __forceinline__ void term_context(const BondedTerms &t, int term, const double *lambdas, double lambda0,
+                                             const int *red_idx, double &lam, int &red) {
+    const int s = t.term_sys ? t.term_sys[term] : 0;
+    lam = lambdas ? lambdas[s] : lambda0;
+    red = red_idx ? red_idx[s] : 0;
+}
+
+__global__ void k_harmonic_bond(const BondedTerms t, const double *__restrict__ x, const double *__restrict__ lambdas,
+                                double lambda0, const int *__restrict__ red_idx, uint64_t *__restrict__ du_dx,
+                                uint64_t *__restrict__ du_dp, uint64_t *__restrict__ du_dl, uint64_t *__restrict__ u) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= t.n) {
+        return;
+    }
+    double lam;
+    int red;
+    term_context(t, b, lambdas, lambda0, red_idx, lam, red);
+    const int i = t.idx[b * 2 + 0];
+    const int j = t.idx[b * 2 + 1];
+    const int mult = t.mult ? t.mult[b] : 0;
+    const int off = t.offset ? t.offset[b] : 1;
+    const float pf = lam_prefactor(mult, off, lam);
+    const BondOut o = bond_term(x + i * 3, x + j * 3, static_cast<float>(t.params[b * 2 + 0]),
+                                static_cast<float>(t.params[b * 2 + 1]), mult, pf);
+    if (du_dx) {
+        for (int d = 0; d < 3; d++) {
+            add_fixed(du_dx + i * 3 + d, o.g[0][d]);
+            add_fixed(du_dx + j * 3 + d, o.g[1][d]);
+        }
+    }
+    if (du_dp) {
+        add_fixed(du_dp + b * 2 + 0, o.dkb);
+        add_fixed(du_dp + b * 2 + 1, o.db0);
+    }
+    if (u) {
+        add_fixed(u + red, o.u);
+    }
+    if (du_dl) {
+        add_fixed(du_dl + red, o.dl);
+    }
+}
+
+__global__ void k_harmonic_angle(const BondedTerms t, const double *__restrict__ x, const double *__restrict__ lambdas,
+                                 double lambda0, const int *__restrict__ red_idx, uint64_t *__restrict__ du_dx,
+                                 uint64_t *__restrict__ du_dp, uint64_t *__restrict__ du_dl,
+                                 uint64_t *__restrict__ u) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= t.n) {
+        return;
+    }
+    double lam;
+    int red;
+    term_context(t, a, lambdas, lambda0, red_idx, lam, red);
+    const int i = t.idx[a * 3 + 0];
+    const int j = t.idx[a * 3 + 1];
+    const int k = t.idx[a * 3 + 2];
+    const int mult = t.mult ? t.mult[a] : 0;
+    const int off = t.offset ? t.offset[a] : 1;
+    const float pf = lam_prefactor(mult, off, lam);
+    const AngleOut o = angle_term(x + i * 3, x + j * 3, x + k * 3, static_cast<float>(t.params[a * 2 + 0]),
+                                  static_cast<float>(t.params[a * 2 + 1]), mult, pf);
+    if (du_dx) {
+        for (int d = 0; d < 3; d++) {
+            add_fixed(du_dx + i * 3 + d, o.g[0][d]);
+            add_fixed(du_dx + j * 3 + d, o.g[1][d]);
+            add_fixed(du_dx + k * 3 + d, o.g[2][d]);
+        }
+    }
+    if (du_dp) {
+        add_fixed(du_dp + a * 2 + 0, o.dka);
+        add_fixed(du_dp + a * 2 + 1, o.da0);
+    }
+    if (u) {
+        add_fixed(u + red, o.u);
+    }
+    if (du_dl) {
+        add_fixed(du_dl + red, o.dl);
+    }
+}
+
+__global__ void k_periodic_torsion(const BondedTerms t, const double *__restrict__ x,
+                                   const double *__restrict__ lambdas, double lambda0, const int *__restrict__ red_idx,
+                                   uint64_t *__restrict__ du_dx, uint64_t *__restrict__ du_dp,
+                                   uint64_t *__restrict__ du_dl, uint64_t *__restrict__ u) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= t.n) {
+        return;
+    }
+    double lam;
+    int red;
+    term_context(t, q, lambdas, lambda0, red_idx, lam, red);
+    const int i = t.idx[q * 4 + 0];
+    const int j = t.idx[q * 4 + 1];
+    const int k = t.idx[q * 4 + 2];
+    const int l = t.idx[q * 4 + 3];
+    const int mult = t.mult ? t.mult[q] : 0;
+    const int off = t.offset ? t.offset[q] : 1;
+    const float pf = lam_prefactor(mult, off, lam);
+    const TorsionOut o = torsion_term(x + i * 3, x + j * 3, x + k * 3, x + l * 3, static_cast<float>(t.params[q * 3 + 0]),
+                                      static_cast<float>(t.params[q * 3 + 1]), static_cast<float>(t.params[q * 3 + 2]),
+                                      mult, pf);
+    if (du_dx) {
+        const int at[4] = {i, j, k, l};
+        for (int a = 0; a < 4; a++) {
+            for (int d = 0; d < 3; d++) {
+                add_fixed(du_dx + at[a] * 3 + d, o.g[a][d]);
+            }
+        }
+    }
+    if (du_dp) {
+        add_fixed(du_dp + q * 3 + 0, o.dk);
+        add_fixed(du_dp + q * 3 + 1, o.dphase);
+        add_fixed(du_dp + q * 3 + 2, o.dperiod);
+    }
+    if (u) {
+        add_fixed(u + red, o.u);
+    }
+    if (du_dl) {
+        add_fixed(du_dl + red, o.dl);
+    }
+}
+
+} // namespace
+
+void launch_bonded(BondedKind kind, const BondedTerms &t, const double *x, const double *lambdas, double lambda0,
+                   const int *red_idx, uint64_t *du_dx, uint64_t *du_dp, uint64_t *du_dl, uint64_t *u,
+                   hipStream_t stream) {
+    if (t.n == 0 || (!du_dx && !du_dp && !du_dl && !u)) {
+        return;
+    }
+    const int grid = ceil_div(t.n, 128);
+    switch (kind) {
+    case BondedKind::Bond:
+        k_harmonic_bond<<<grid, 128, 0, stream>>>(t, x, lambdas, lambda0, red_idx, du_dx, du_dp, du_dl, u);
+        break;
+    case BondedKind::Angle:
+        k_harmonic_angle<<<grid, 128, 0, stream>>>(t, x, lambdas, lambda0, red_idx, du_dx, du_dp, du_dl, u);
+        break;
+    case BondedKind::Torsion:
+        k_periodic_torsion<<<grid, 128, 0, stream>>>(t, x, lambdas, lambda0, red_idx, du_dx, du_dp, du_dl, u);
+        break;
+    }
+    TM_LAUNCH_CHECK();
+}
+
+} // namespace tmamd

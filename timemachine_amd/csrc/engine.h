@@ -1,0 +1,114 @@
+// Batched MD engine: S independent systems (lambda windows, replicas, or
+// unrelated serving jobs of uneven size) packed into one HBM arena and
+// stepped together. Every per-step launch covers all systems, sizes are read
+// on the device, so a chunk of steps is captured once into a hipGraph and
+// replayed.
+//
+// A reference Context (timemachine/cpp/src/context.cu:10-304) is an Engine
+// with S = 1: the same kernels, the same per-system arithmetic and the same
+// counter-based noise, which is why one system run alone and the same system
+// run inside a batch produce bitwise identical trajectories.
+//
+// Per-step sequence (context.cu:240-280 restated for S systems):
+//   zero du_dx -> nonbonded (rebuild check, sort, gather, neighborlist, tiles)
+//   -> pair lists -> bonded -> du_dl store -> BAOAB (integrator.cu:37-86)
+//   -> every `interval` steps the MC barostat (barostat.cu:286-373).
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "bonded.h"
+#include "nb_state.h"
+#include "potentials.h"
+
+namespace tmamd {
+
+struct IntegratorSpec {
+    double dt = 0.0;
+    double ca = 0.0;
+    std::vector<double> cbs, ccs;
+    uint64_t seed = 0;
+};
+
+struct BarostatSpec {
+    int N = 0;
+    double pressure = 1.0;    // bar
+    double temperature = 300; // K
+    int interval = 0;
+    uint64_t seed = 0;
+    std::vector<std::vector<int>> groups;
+};
+
+struct SystemSpec {
+    int N = 0;
+    std::vector<double> x, v, box;
+    IntegratorSpec integ;
+    std::vector<BoundPotentialPtr> bps;
+    std::shared_ptr<BarostatSpec> baro; // may be null
+    // carried state (so a system can move between engines bit-exactly)
+    int64_t step = 0;
+    double baro_vscale = 0.0;
+    int baro_accepted = 0;
+    int baro_attempted = 0;
+    int baro_step = -1; // barostat calls since construction / set_interval; -1: derive from step
+};
+
+class Engine {
+  public:
+    explicit Engine(const std::vector<SystemSpec> &systems);
+    ~Engine();
+
+    int num_systems() const { return S_; }
+    int num_atoms(int s) const { return h_n_[s]; }
+    int total_atoms() const { return A_; }
+
+    // Current lambda of each system (host copy pushed to the device).
+    void set_lambdas(const std::vector<double> &lams);
+
+    // Runs n_steps. lambda_sched: [S, n_steps] per-step lambdas or nullptr
+    // (keep the current lambdas). du_dl is stored every du_dl_interval steps
+    // (0: never) into du_dl_out [S, n_steps / du_dl_interval]; frames every
+    // x_interval steps (0: never) into x_out (per system, concatenated
+    // [F, N_s, 3] blocks in system order) and box_out [S, F, 9].
+    void run(int n_steps, const double *lambda_sched, int du_dl_interval, int x_interval, double *du_dl_out,
+             double *x_out, double *box_out, bool use_graph);
+
+    // Energies of every system at its current state and lambda (u, du_dl).
+    void energies(double *u_out, double *du_dl_out);
+
+    void get_x(int s, double *out);
+    void get_v(int s, double *out);
+    void get_box(int s, double *out);
+    void get_du_dx(int s, uint64_t *out);
+    void set_x(int s, const double *in);
+    void set_v(int s, const double *in);
+    void set_box(int s, const double *in);
+    int64_t get_step(int s);
+    void get_barostat_state(int s, double *vscale, int *accepted, int *attempted, int *baro_step);
+    // Raises the reference's box messages if a system's box went bad.
+    void check_status();
+    // Average durations (ms) over `reps` launches, timed with HIP events on the
+    // engine's stream at the current state: [0] neighborlist/sort/gather
+    // preparation (forced rebuild), [1] pair (tile) kernel, [2] pair lists +
+    // bonded, [3] integrator.
+    void profile(int reps, double *ms_out);
+    // Tiles currently in the neighborlist of every system (sum).
+    int total_tiles();
+    // Device bytes owned by the engine.
+    size_t device_bytes() const;
+
+    hipStream_t stream() const { return stream_; }
+
+    int tile_grid = 0; // nonbonded grid override (autotuner), 0 = default
+
+  private:
+    struct Impl;
+    std::unique_ptr<Impl> impl_;
+    int S_ = 0;
+    int A_ = 0;
+    std::vector<int> h_n_, h_atom0_;
+    hipStream_t stream_ = nullptr;
+};
+
+} // namespace tmamd

@@ -1,0 +1,1026 @@
+// Batched MD engine (see engine.h).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+
+#include "engine.h"
+#include "fixed_point.h"
+#include "philox.h"
+
+namespace tmamd {
+
+namespace {
+
+// Constants of the reference barostat (timemachine/cpp/src/barostat.cu:11-12).
+constexpr double BARO_AVOGADRO = 6.0221367e23;
+constexpr double BARO_BOLTZ = 0.008314462618;
+
+__global__ void k_set_lambda(int S, const double *__restrict__ sched, int n_steps, const int *__restrict__ run_step,
+                             double *__restrict__ lam) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) {
+        return;
+    }
+    const int r = run_step[s];
+    lam[s] = sched[static_cast<size_t>(s) * n_steps + (r < n_steps ? r : n_steps - 1)];
+}
+
+// BAOAB rotated by half a step (integrator.cu:37-69, integrator.py:81-88):
+//   v_mid = v + cb * F, v' = ca v_mid + cc xi, x += dt/2 (v_mid + v'),  cb = -dt/m.
+// Noise: Philox keyed by the system seed, counter (2 * atom + block, step).
+__global__ void k_langevin(int A, const int *__restrict__ atom_sys, const int *__restrict__ atom0,
+                           const int64_t *__restrict__ step, const uint64_t *__restrict__ seed,
+                           const double *__restrict__ ca, const double *__restrict__ dt,
+                           const double *__restrict__ cbs, const double *__restrict__ ccs,
+                           const uint64_t *__restrict__ du_dx, double *__restrict__ x, double *__restrict__ v) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= A) {
+        return;
+    }
+    const int s = atom_sys[a];
+    const uint32_t local = static_cast<uint32_t>(a - atom0[s]);
+    const uint64_t st = static_cast<uint64_t>(step[s]);
+    const uint64_t sd = seed[s];
+    const uint32_t k0 = static_cast<uint32_t>(sd), k1 = static_cast<uint32_t>(sd >> 32);
+    Philox4 c0{{2 * local + 0, static_cast<uint32_t>(st), static_cast<uint32_t>(st >> 32), RNG_STREAM_LANGEVIN}};
+    Philox4 c1{{2 * local + 1, static_cast<uint32_t>(st), static_cast<uint32_t>(st >> 32), RNG_STREAM_LANGEVIN}};
+    double z[4];
+    normal2(philox4x32_10(c0, k0, k1), z[0], z[1]);
+    normal2(philox4x32_10(c1, k0, k1), z[2], z[3]);
+    const double cb = cbs[a], cc = ccs[a], cas = ca[s], h = 0.5 * dt[s];
+    for (int d = 0; d < 3; d++) {
+        const double f = fixed_to_double(du_dx[a * 3 + d], FIXED_SCALE);
+        const double v_mid = v[a * 3 + d] + cb * f;
+        const double v_new = cas * v_mid + cc * z[d];
+        v[a * 3 + d] = v_new;
+        x[a * 3 + d] += h * (v_mid + v_new);
+    }
+}
+
+// du_dl store, box frame store, per-system step counters.
+__global__ void k_step_end(int S, int64_t *__restrict__ step, int *__restrict__ run_step,
+                           uint64_t *__restrict__ du_dl_buf, int dl_interval, int K, uint64_t *__restrict__ du_dl_traj,
+                           int x_interval, int F, const double *__restrict__ box, double *__restrict__ box_frames) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) {
+        return;
+    }
+    const int r = run_step[s] + 1;
+    if (dl_interval > 0) {
+        if (r % dl_interval == 0) {
+            const int k = r / dl_interval - 1;
+            if (k < K) {
+                du_dl_traj[static_cast<size_t>(s) * K + k] = du_dl_buf[s];
+            }
+        }
+        du_dl_buf[s] = 0;
+    }
+    if (x_interval > 0 && r % x_interval == 0) {
+        const int f = r / x_interval - 1;
+        if (f < F) {
+            for (int d = 0; d < 9; d++) {
+                box_frames[(static_cast<size_t>(s) * F + f) * 9 + d] = box[s * 9 + d];
+            }
+        }
+    }
+    run_step[s] = r;
+    step[s] += 1;
+}
+
+// Frame store with an explicit per-system atom count table.
+__global__ void k_store_frames2(int A, const int *__restrict__ atom_sys, const int *__restrict__ atom0,
+                                const int *__restrict__ natoms, const int *__restrict__ run_step, int x_interval,
+                                int F, const double *__restrict__ x, double *__restrict__ frames) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= A) {
+        return;
+    }
+    const int s = atom_sys[a];
+    const int r = run_step[s] + 1;
+    if (r % x_interval != 0) {
+        return;
+    }
+    const int f = r / x_interval - 1;
+    if (f >= F) {
+        return;
+    }
+    const int a0 = atom0[s];
+    const int ns = natoms[s];
+    const size_t dst = static_cast<size_t>(F) * a0 * 3 + (static_cast<size_t>(f) * ns + (a - a0)) * 3;
+    frames[dst + 0] = x[a * 3 + 0];
+    frames[dst + 1] = x[a * 3 + 1];
+    frames[dst + 2] = x[a * 3 + 2];
+}
+
+// ---------------------------------------------------------------- barostat
+// Restates MonteCarloBarostat::inplace_move (barostat.cu:286-373). The move is
+// decided by one thread per system and applied by a separate kernel, so no
+// thread reads a box another thread is overwriting.
+
+struct BaroDev {
+    int *has;          // [S]
+    int *interval;     // [S]
+    int *bstep;        // [S] barostat call counter (reset by set_interval)
+    uint64_t *seed;    // [S]
+    double *pressure;  // [S] internal units (bar * AVOGADRO * 1e-25)
+    double *kT;        // [S]
+    int *nmol;         // [S]
+    double *vscale;    // [S]
+    int *accepted;     // [S]
+    int *attempted;    // [S]
+    int *trig;         // [S]
+    int *accept;       // [S]
+    double *dvol;      // [S]
+    double *lscale;    // [S]
+    double *r2;        // [S]
+    uint64_t *u_before; // [S]
+    uint64_t *u_after;  // [S]
+};
+
+__global__ void k_baro_setup(int S, BaroDev b, const double *__restrict__ box, double *__restrict__ box_after) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) {
+        return;
+    }
+    b.trig[s] = 0;
+    b.accept[s] = 0;
+    b.u_before[s] = 0;
+    b.u_after[s] = 0;
+    for (int d = 0; d < 9; d++) {
+        box_after[s * 9 + d] = box[s * 9 + d];
+    }
+    if (!b.has[s]) {
+        return;
+    }
+    const int call = b.bstep[s] + 1;
+    b.bstep[s] = call;
+    if (call % b.interval[s] != 0) {
+        return;
+    }
+    b.trig[s] = 1;
+    const uint64_t sd = b.seed[s];
+    Philox4 c{{static_cast<uint32_t>(call), 0u, 0u, RNG_STREAM_BAROSTAT}};
+    const Philox4 r = philox4x32_10(c, static_cast<uint32_t>(sd), static_cast<uint32_t>(sd >> 32));
+    const double r1 = u01_open0(r.v[0], r.v[1]);
+    b.r2[s] = u01_open0(r.v[2], r.v[3]);
+    const double volume = box[s * 9 + 0] * box[s * 9 + 4] * box[s * 9 + 8];
+    if (b.vscale[s] == 0.0) {
+        b.vscale[s] = 0.01 * volume;
+    }
+    const double dv = b.vscale[s] * 2.0 * (r1 - 0.5);
+    const double new_volume = volume + dv;
+    b.dvol[s] = dv;
+    const double ls = cbrt(new_volume / volume);
+    b.lscale[s] = ls;
+    box_after[s * 9 + 0] = box[s * 9 + 0] * ls;
+    box_after[s * 9 + 4] = box[s * 9 + 4] * ls;
+    box_after[s * 9 + 8] = box[s * 9 + 8] * ls;
+}
+
+__global__ void k_baro_zero_centroids(int nmol_total, const int *__restrict__ mol_sys, const int *__restrict__ trig,
+                                      uint64_t *__restrict__ centroids) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmol_total || !trig[mol_sys[m]]) {
+        return;
+    }
+    centroids[m * 3 + 0] = 0;
+    centroids[m * 3 + 1] = 0;
+    centroids[m * 3 + 2] = 0;
+}
+
+// find_group_centroids (barostat.cu:181-197): fixed-point sums of coordinates.
+__global__ void k_baro_centroids(int G, const int *__restrict__ g_atom, const int *__restrict__ g_mol,
+                                 const int *__restrict__ mol_sys, const int *__restrict__ trig,
+                                 const double *__restrict__ x, uint64_t *__restrict__ centroids) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G) {
+        return;
+    }
+    const int m = g_mol[i];
+    if (!trig[mol_sys[m]]) {
+        return;
+    }
+    const int a = g_atom[i];
+    for (int d = 0; d < 3; d++) {
+        atomicAdd(reinterpret_cast<unsigned long long *>(centroids + m * 3 + d),
+                  static_cast<unsigned long long>(f64_to_fixed(x[a * 3 + d], FIXED_SCALE)));
+    }
+}
+
+// rescale_positions (barostat.cu:139-179): move each molecule's centroid
+// towards/away from the box centre by the length scale.
+__global__ void k_baro_rescale(int A, const int *__restrict__ atom_sys, const int *__restrict__ atom_mol,
+                               const int *__restrict__ mol_count, const int *__restrict__ trig,
+                               const double *__restrict__ lscale, const double *__restrict__ box,
+                               const uint64_t *__restrict__ centroids, const double *__restrict__ x,
+                               double *__restrict__ x_after) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= A) {
+        return;
+    }
+    const int s = atom_sys[a];
+    const int m = atom_mol[a];
+    if (!trig[s] || m < 0) {
+        x_after[a * 3 + 0] = x[a * 3 + 0];
+        x_after[a * 3 + 1] = x[a * 3 + 1];
+        x_after[a * 3 + 2] = x[a * 3 + 2];
+        return;
+    }
+    const double n = static_cast<double>(mol_count[m]);
+    const double ls = lscale[s];
+    for (int d = 0; d < 3; d++) {
+        const double center = box[s * 9 + d * 4] * 0.5;
+        const double centroid = fixed_to_double(centroids[m * 3 + d], FIXED_SCALE) / n;
+        const double disp = ((centroid - center) * ls) + center - centroid;
+        x_after[a * 3 + d] = x[a * 3 + d] + disp;
+    }
+}
+
+// k_decide_move (barostat.cu:220-279) for one system per thread.
+__global__ void k_baro_decide(int S, BaroDev b, double *__restrict__ box, const double *__restrict__ box_after) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S || !b.trig[s]) {
+        return;
+    }
+    const double volume = box[s * 9 + 0] * box[s * 9 + 4] * box[s * 9 + 8];
+    const double new_volume = volume + b.dvol[s];
+    const double de = fixed_to_double(b.u_after[s] - b.u_before[s], FIXED_SCALE);
+    const double kT = b.kT[s];
+    const double w = de + b.pressure[s] * b.dvol[s] - b.nmol[s] * kT * log(new_volume / volume);
+    const bool rejected = w > 0 && b.r2[s] > exp(-w / kT);
+    if (!rejected) {
+        b.accepted[s] += 1;
+    }
+    b.attempted[s] += 1;
+    if (b.attempted[s] >= 10) {
+        if (b.accepted[s] < 0.25 * b.attempted[s]) {
+            b.vscale[s] /= 1.1;
+            b.attempted[s] = 0;
+            b.accepted[s] = 0;
+        } else if (b.accepted[s] > 0.75 * b.attempted[s]) {
+            b.vscale[s] = fmin(b.vscale[s] * 1.1, volume * 0.3);
+            b.attempted[s] = 0;
+            b.accepted[s] = 0;
+        }
+    }
+    b.accept[s] = rejected ? 0 : 1;
+    if (!rejected) {
+        for (int d = 0; d < 9; d++) {
+            box[s * 9 + d] = box_after[s * 9 + d];
+        }
+    }
+}
+
+__global__ void k_baro_apply(int A, const int *__restrict__ atom_sys, const int *__restrict__ accept,
+                             const double *__restrict__ x_after, double *__restrict__ x) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= A || !accept[atom_sys[a]]) {
+        return;
+    }
+    x[a * 3 + 0] = x_after[a * 3 + 0];
+    x[a * 3 + 1] = x_after[a * 3 + 1];
+    x[a * 3 + 2] = x_after[a * 3 + 2];
+}
+
+// Flattened potential terms of one system.
+struct Flat {
+    std::shared_ptr<NonbondedAllPairs> nb;
+    std::vector<double> nb_params;
+    std::vector<std::pair<std::shared_ptr<NonbondedPairList>, std::vector<double>>> pls;
+    std::vector<std::pair<std::shared_ptr<BondedPotential>, std::vector<double>>> bonded;
+};
+
+void flatten(const PotentialPtr &p, const double *params, int P, Flat &out) {
+    switch (p->kind()) {
+    case PotKind::NonbondedAllPairs: {
+        if (out.nb) {
+            throw Error("Engine: at most one NonbondedAllPairs per system is supported");
+        }
+        out.nb = std::static_pointer_cast<NonbondedAllPairs>(p);
+        out.nb_params.assign(params, params + P);
+        break;
+    }
+    case PotKind::NonbondedPairList:
+        out.pls.emplace_back(std::static_pointer_cast<NonbondedPairList>(p), std::vector<double>(params, params + P));
+        break;
+    case PotKind::HarmonicBond:
+    case PotKind::HarmonicAngle:
+    case PotKind::PeriodicTorsion:
+        out.bonded.emplace_back(std::static_pointer_cast<BondedPotential>(p), std::vector<double>(params, params + P));
+        break;
+    case PotKind::Fanout: {
+        auto f = std::static_pointer_cast<FanoutSummedPotential>(p);
+        for (auto &c : f->pots_) {
+            flatten(c, params, P, out);
+        }
+        break;
+    }
+    case PotKind::Summed: {
+        auto f = std::static_pointer_cast<SummedPotential>(p);
+        int off = 0;
+        for (size_t i = 0; i < f->pots_.size(); i++) {
+            flatten(f->pots_[i], params + off, f->sizes_[i], out);
+            off += f->sizes_[i];
+        }
+        break;
+    }
+    }
+}
+
+template <typename T> void upload_vec(DeviceArray<T> &d, const std::vector<T> &h) {
+    d.alloc(std::max<size_t>(h.size(), 1));
+    if (!h.empty()) {
+        d.upload(h.data(), h.size());
+    }
+}
+
+} // namespace
+
+struct Engine::Impl {
+    int S = 0, A = 0;
+    std::vector<int> atom0, natoms;
+
+    DeviceArray<double> x, v, x_after, box, box_after, cbs, ccs, ca, dt, lam;
+    DeviceArray<uint64_t> du_dx, seed, u_buf, du_dl_buf;
+    DeviceArray<int64_t> step;
+    DeviceArray<int> atom_sys, d_atom0, d_natoms, red_idx, run_step;
+
+    // nonbonded
+    bool has_nb = false;
+    NbState nb;
+    DeviceArray<double> nb_params;
+    DeviceArray<int> nb_plane, nb_offset;
+
+    // pair lists (index 0: plain, 1: negated)
+    int pl_n[2] = {0, 0};
+    DeviceArray<int> pl_pairs[2], pl_sys[2];
+    DeviceArray<double> pl_scales[2];
+    DeviceArray<double> pl_params;
+    DeviceArray<int> pl_plane, pl_offset;
+    DeviceArray<double> pl_cutoffs;
+    DeviceArray<float> pl_betas;
+    bool has_pl = false;
+
+    // bonded
+    int bt_n[3] = {0, 0, 0};
+    DeviceArray<int> bt_idx[3], bt_mult[3], bt_offset[3], bt_sys[3];
+    DeviceArray<double> bt_params[3];
+
+    // barostat
+    bool has_baro = false;
+    std::vector<int> h_has_baro, h_interval, h_bstep;
+    DeviceArray<int> b_has, b_interval, b_bstep, b_nmol, b_accepted, b_attempted, b_trig, b_accept;
+    DeviceArray<uint64_t> b_seed, b_u_before, b_u_after;
+    DeviceArray<double> b_pressure, b_kT, b_vscale, b_dvol, b_lscale, b_r2;
+    int G = 0, nmol_total = 0;
+    DeviceArray<int> g_atom, g_mol, mol_sys, mol_count, atom_mol;
+    DeviceArray<uint64_t> centroids;
+
+    BaroDev baro_dev() {
+        return BaroDev{b_has.ptr,      b_interval.ptr, b_bstep.ptr,    b_seed.ptr,     b_pressure.ptr,  b_kT.ptr,
+                       b_nmol.ptr,     b_vscale.ptr,   b_accepted.ptr, b_attempted.ptr, b_trig.ptr,     b_accept.ptr,
+                       b_dvol.ptr,     b_lscale.ptr,   b_r2.ptr,       b_u_before.ptr, b_u_after.ptr};
+    }
+
+    void eval(const double *xp, const double *boxp, uint64_t *ddx, uint64_t *ddl, uint64_t *uu, hipStream_t st,
+              int tile_grid) {
+        if (has_nb) {
+            nb.prepare(xp, nb_params.ptr, boxp, nb_plane.ptr, nb_offset.ptr, lam.ptr, 0.0, st);
+            nb.compute(boxp, ddx, nullptr, ddl, uu, st, tile_grid);
+        }
+        for (int neg = 0; neg < 2; neg++) {
+            if (pl_n[neg] == 0) {
+                continue;
+            }
+            PairListArgs a{};
+            a.M = pl_n[neg];
+            a.pairs = pl_pairs[neg].ptr;
+            a.scales = pl_scales[neg].ptr;
+            a.pair_sys = pl_sys[neg].ptr;
+            a.red_idx = red_idx.ptr;
+            a.x = xp;
+            a.params = pl_params.ptr;
+            a.boxes = boxp;
+            a.plane = pl_plane.ptr;
+            a.offset = pl_offset.ptr;
+            a.lambdas = lam.ptr;
+            a.cutoffs = pl_cutoffs.ptr;
+            a.betas = pl_betas.ptr;
+            a.negated = neg;
+            launch_pair_list(a, ddx, nullptr, ddl, uu, st);
+        }
+        for (int k = 0; k < 3; k++) {
+            if (bt_n[k] == 0) {
+                continue;
+            }
+            BondedTerms t{bt_n[k], bt_idx[k].ptr, bt_params[k].ptr, bt_mult[k].ptr, bt_offset[k].ptr, bt_sys[k].ptr};
+            launch_bonded(static_cast<BondedKind>(k), t, xp, lam.ptr, 0.0, red_idx.ptr, ddx, nullptr, ddl, uu, st);
+        }
+    }
+};
+
+Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
+    Impl &I = *impl_;
+    S_ = static_cast<int>(systems.size());
+    if (S_ == 0) {
+        throw Error("Engine: no systems");
+    }
+    TM_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    I.S = S_;
+    // layout
+    int a0 = 0;
+    for (auto &sp : systems) {
+        if (static_cast<int>(sp.x.size()) != sp.N * 3 || static_cast<int>(sp.v.size()) != sp.N * 3 ||
+            sp.box.size() != 9) {
+            throw Error("Engine: bad system state shapes");
+        }
+        if (static_cast<int>(sp.integ.cbs.size()) != sp.N || static_cast<int>(sp.integ.ccs.size()) != sp.N) {
+            throw Error("Engine: integrator coefficient size does not match N");
+        }
+        I.atom0.push_back(a0);
+        I.natoms.push_back(sp.N);
+        a0 += sp.N;
+    }
+    A_ = I.A = a0;
+    h_n_ = I.natoms;
+    h_atom0_ = I.atom0;
+
+    std::vector<double> hx(A_ * 3), hv(A_ * 3), hbox(S_ * 9), hcbs(A_), hccs(A_), hca(S_), hdt(S_), hlam(S_, 0.0);
+    std::vector<uint64_t> hseed(S_);
+    std::vector<int64_t> hstep(S_);
+    std::vector<int> hatom_sys(A_), hred(S_);
+    for (int s = 0; s < S_; s++) {
+        const SystemSpec &sp = systems[s];
+        std::copy(sp.x.begin(), sp.x.end(), hx.begin() + I.atom0[s] * 3);
+        std::copy(sp.v.begin(), sp.v.end(), hv.begin() + I.atom0[s] * 3);
+        std::copy(sp.box.begin(), sp.box.end(), hbox.begin() + s * 9);
+        std::copy(sp.integ.cbs.begin(), sp.integ.cbs.end(), hcbs.begin() + I.atom0[s]);
+        std::copy(sp.integ.ccs.begin(), sp.integ.ccs.end(), hccs.begin() + I.atom0[s]);
+        hca[s] = sp.integ.ca;
+        hdt[s] = sp.integ.dt;
+        hseed[s] = sp.integ.seed;
+        hstep[s] = sp.step;
+        hred[s] = s;
+        for (int i = 0; i < sp.N; i++) {
+            hatom_sys[I.atom0[s] + i] = s;
+        }
+    }
+    upload_vec(I.x, hx);
+    upload_vec(I.v, hv);
+    I.x_after.alloc(std::max(A_ * 3, 1));
+    upload_vec(I.box, hbox);
+    I.box_after.alloc(S_ * 9);
+    upload_vec(I.cbs, hcbs);
+    upload_vec(I.ccs, hccs);
+    upload_vec(I.ca, hca);
+    upload_vec(I.dt, hdt);
+    upload_vec(I.lam, hlam);
+    upload_vec(I.seed, hseed);
+    upload_vec(I.step, hstep);
+    upload_vec(I.atom_sys, hatom_sys);
+    upload_vec(I.d_atom0, I.atom0);
+    upload_vec(I.d_natoms, I.natoms);
+    upload_vec(I.red_idx, hred);
+    I.run_step.alloc(S_);
+    I.run_step.zero();
+    I.du_dx.alloc(std::max(A_ * 3, 1));
+    I.du_dx.zero();
+    I.u_buf.alloc(S_);
+    I.du_dl_buf.alloc(S_);
+    I.u_buf.zero();
+    I.du_dl_buf.zero();
+
+    // potentials
+    std::vector<Flat> flats(S_);
+    for (int s = 0; s < S_; s++) {
+        for (auto &bp : systems[s].bps) {
+            flatten(bp->potential, bp->h_params.data(), bp->size(), flats[s]);
+        }
+    }
+    int n_nb = 0;
+    for (auto &f : flats) {
+        n_nb += f.nb ? 1 : 0;
+    }
+    if (n_nb != 0 && n_nb != S_) {
+        throw Error("Engine: either every system or none has a NonbondedAllPairs");
+    }
+    I.has_nb = n_nb == S_;
+    if (I.has_nb) {
+        std::vector<NbSys> ys(S_);
+        std::vector<int> subset;
+        std::vector<double> params(A_ * 3, 0.0);
+        std::vector<int> plane(A_, 0), offset(A_, 0);
+        for (int s = 0; s < S_; s++) {
+            const auto &nb = flats[s].nb;
+            if (nb->N_ != I.natoms[s]) {
+                throw Error("Engine: NonbondedAllPairs N does not match the system");
+            }
+            NbSys y{};
+            y.atom0 = I.atom0[s];
+            y.n_atoms = I.natoms[s];
+            y.k = nb->K_;
+            y.red_idx = s;
+            y.subset0 = nb->subset_.empty() ? -1 : static_cast<int>(subset.size());
+            subset.insert(subset.end(), nb->subset_.begin(), nb->subset_.end());
+            y.sorted = nb->sorted_ ? 1 : 0;
+            y.beta = static_cast<float>(nb->beta_);
+            y.cutoff_d = nb->cutoff_;
+            y.padding_d = nb->padding_;
+            ys[s] = y;
+            std::copy(flats[s].nb_params.begin(), flats[s].nb_params.end(), params.begin() + I.atom0[s] * 3);
+            std::copy(nb->plane_.begin(), nb->plane_.end(), plane.begin() + I.atom0[s]);
+            std::copy(nb->offset_.begin(), nb->offset_.end(), offset.begin() + I.atom0[s]);
+        }
+        I.nb.init(ys, subset);
+        upload_vec(I.nb_params, params);
+        upload_vec(I.nb_plane, plane);
+        upload_vec(I.nb_offset, offset);
+    }
+    // pair lists
+    {
+        std::vector<int> pairs[2], psys[2];
+        std::vector<double> scales[2];
+        std::vector<double> params(A_ * 3, 0.0), cutoffs(S_, 1.0);
+        std::vector<float> betas(S_, 0.0f);
+        std::vector<int> plane(A_, 0), offset(A_, 0);
+        for (int s = 0; s < S_; s++) {
+            const std::vector<double> *first = nullptr;
+            for (auto &pr : flats[s].pls) {
+                const auto &pl = pr.first;
+                if (pl->N_ != I.natoms[s]) {
+                    throw Error("Engine: NonbondedPairList N does not match the system");
+                }
+                if (!first) {
+                    first = &pr.second;
+                    std::copy(pr.second.begin(), pr.second.end(), params.begin() + I.atom0[s] * 3);
+                    std::copy(pl->plane_.begin(), pl->plane_.end(), plane.begin() + I.atom0[s]);
+                    std::copy(pl->offset_.begin(), pl->offset_.end(), offset.begin() + I.atom0[s]);
+                    cutoffs[s] = pl->cutoff_;
+                    betas[s] = static_cast<float>(pl->beta_);
+                } else if (*first != pr.second || pl->cutoff_ != cutoffs[s] ||
+                           static_cast<float>(pl->beta_) != betas[s]) {
+                    throw Error("Engine: pair lists of one system must share parameters, beta and cutoff");
+                }
+                const int neg = pl->negated_ ? 1 : 0;
+                for (int m = 0; m < pl->M_; m++) {
+                    pairs[neg].push_back(pl->pairs_[m * 2 + 0] + I.atom0[s]);
+                    pairs[neg].push_back(pl->pairs_[m * 2 + 1] + I.atom0[s]);
+                    scales[neg].push_back(pl->scales_[m * 2 + 0]);
+                    scales[neg].push_back(pl->scales_[m * 2 + 1]);
+                    psys[neg].push_back(s);
+                }
+            }
+        }
+        for (int neg = 0; neg < 2; neg++) {
+            I.pl_n[neg] = static_cast<int>(psys[neg].size());
+            upload_vec(I.pl_pairs[neg], pairs[neg]);
+            upload_vec(I.pl_scales[neg], scales[neg]);
+            upload_vec(I.pl_sys[neg], psys[neg]);
+        }
+        I.has_pl = I.pl_n[0] + I.pl_n[1] > 0;
+        upload_vec(I.pl_params, params);
+        upload_vec(I.pl_plane, plane);
+        upload_vec(I.pl_offset, offset);
+        upload_vec(I.pl_cutoffs, cutoffs);
+        upload_vec(I.pl_betas, betas);
+    }
+    // bonded
+    {
+        std::vector<int> idx[3], mult[3], off[3], sys[3];
+        std::vector<double> prm[3];
+        for (int s = 0; s < S_; s++) {
+            for (auto &pr : flats[s].bonded) {
+                const auto &b = pr.first;
+                const int k = static_cast<int>(b->bk_);
+                const int ar = bonded_arity(b->bk_);
+                for (int t = 0; t < b->n_; t++) {
+                    for (int q = 0; q < ar; q++) {
+                        const int ai = b->idxs_[t * ar + q];
+                        if (ai < 0 || ai >= I.natoms[s]) {
+                            throw Error("Engine: bonded index out of range");
+                        }
+                        idx[k].push_back(ai + I.atom0[s]);
+                    }
+                    mult[k].push_back(b->mult_[t]);
+                    off[k].push_back(b->offset_[t]);
+                    sys[k].push_back(s);
+                }
+                if (static_cast<int>(pr.second.size()) != b->n_ * bonded_nparams(b->bk_)) {
+                    throw Error("Engine: bonded parameter size mismatch");
+                }
+                prm[k].insert(prm[k].end(), pr.second.begin(), pr.second.end());
+            }
+        }
+        for (int k = 0; k < 3; k++) {
+            I.bt_n[k] = static_cast<int>(sys[k].size());
+            upload_vec(I.bt_idx[k], idx[k]);
+            upload_vec(I.bt_mult[k], mult[k]);
+            upload_vec(I.bt_offset[k], off[k]);
+            upload_vec(I.bt_sys[k], sys[k]);
+            upload_vec(I.bt_params[k], prm[k]);
+        }
+    }
+    // barostat
+    {
+        std::vector<int> has(S_, 0), interval(S_, 1), bstep(S_, 0), nmol(S_, 0), acc(S_, 0), att(S_, 0);
+        std::vector<uint64_t> bseed(S_, 0);
+        std::vector<double> pressure(S_, 0.0), kT(S_, 0.0), vscale(S_, 0.0);
+        std::vector<int> gatom, gmol, msys, mcount, amol(A_, -1);
+        for (int s = 0; s < S_; s++) {
+            const auto &b = systems[s].baro;
+            if (!b) {
+                continue;
+            }
+            if (b->interval <= 0) {
+                throw Error("Barostat interval must be greater than 0");
+            }
+            if (b->N != I.natoms[s]) {
+                throw Error("Engine: barostat N does not match the system");
+            }
+            has[s] = 1;
+            interval[s] = b->interval;
+            bstep[s] = systems[s].baro_step >= 0 ? systems[s].baro_step % b->interval
+                                                 : static_cast<int>(systems[s].step % b->interval);
+            bseed[s] = b->seed;
+            pressure[s] = b->pressure * BARO_AVOGADRO * 1e-25;
+            kT[s] = BARO_BOLTZ * b->temperature;
+            nmol[s] = static_cast<int>(b->groups.size());
+            vscale[s] = systems[s].baro_vscale;
+            acc[s] = systems[s].baro_accepted;
+            att[s] = systems[s].baro_attempted;
+            for (auto &grp : b->groups) {
+                const int m = static_cast<int>(msys.size());
+                msys.push_back(s);
+                mcount.push_back(static_cast<int>(grp.size()));
+                for (int a : grp) {
+                    if (a < 0 || a >= I.natoms[s]) {
+                        throw Error("Grouped indices must be between 0 and N");
+                    }
+                    gatom.push_back(a + I.atom0[s]);
+                    gmol.push_back(m);
+                    if (amol[a + I.atom0[s]] != -1) {
+                        throw Error("All grouped indices must be unique");
+                    }
+                    amol[a + I.atom0[s]] = m;
+                }
+            }
+        }
+        I.has_baro = std::any_of(has.begin(), has.end(), [](int h) { return h != 0; });
+        I.h_has_baro = has;
+        I.h_interval = interval;
+        I.h_bstep = bstep;
+        I.G = static_cast<int>(gatom.size());
+        I.nmol_total = static_cast<int>(msys.size());
+        upload_vec(I.b_has, has);
+        upload_vec(I.b_interval, interval);
+        upload_vec(I.b_bstep, bstep);
+        upload_vec(I.b_nmol, nmol);
+        upload_vec(I.b_accepted, acc);
+        upload_vec(I.b_attempted, att);
+        upload_vec(I.b_seed, bseed);
+        upload_vec(I.b_pressure, pressure);
+        upload_vec(I.b_kT, kT);
+        upload_vec(I.b_vscale, vscale);
+        I.b_trig.alloc(S_);
+        I.b_accept.alloc(S_);
+        I.b_trig.zero();
+        I.b_accept.zero();
+        I.b_u_before.alloc(S_);
+        I.b_u_after.alloc(S_);
+        I.b_dvol.alloc(S_);
+        I.b_lscale.alloc(S_);
+        I.b_r2.alloc(S_);
+        upload_vec(I.g_atom, gatom);
+        upload_vec(I.g_mol, gmol);
+        upload_vec(I.mol_sys, msys);
+        upload_vec(I.mol_count, mcount);
+        upload_vec(I.atom_mol, amol);
+        I.centroids.alloc(std::max(I.nmol_total * 3, 1));
+    }
+    TM_HIP_CHECK(hipDeviceSynchronize());
+}
+
+Engine::~Engine() {
+    if (stream_) {
+        (void)hipStreamSynchronize(stream_);
+        (void)hipStreamDestroy(stream_);
+    }
+}
+
+void Engine::set_lambdas(const std::vector<double> &lams) {
+    if (static_cast<int>(lams.size()) != S_) {
+        throw Error("Engine::set_lambdas: expected one lambda per system");
+    }
+    impl_->lam.upload(lams.data(), S_, stream_);
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, int x_interval, double *du_dl_out,
+                 double *x_out, double *box_out, bool use_graph) {
+    Impl &I = *impl_;
+    if (n_steps <= 0) {
+        return;
+    }
+    const int K = du_dl_interval > 0 ? n_steps / du_dl_interval : 0;
+    const int F = x_interval > 0 ? n_steps / x_interval : 0;
+    DeviceArray<double> sched;
+    if (lambda_sched) {
+        sched.alloc(static_cast<size_t>(S_) * n_steps);
+        sched.upload(lambda_sched, static_cast<size_t>(S_) * n_steps, stream_);
+    }
+    DeviceArray<uint64_t> dl_traj(std::max<size_t>(static_cast<size_t>(S_) * K, 1));
+    DeviceArray<double> frames(std::max<size_t>(static_cast<size_t>(F) * A_ * 3, 1));
+    DeviceArray<double> box_frames(std::max<size_t>(static_cast<size_t>(S_) * F * 9, 1));
+    launch_fill_i32(I.run_step.ptr, S_, 0, stream_);
+    launch_fill_u64(I.du_dl_buf.ptr, S_, 0, stream_);
+
+    const int TPB = 256;
+    auto one_step = [&](bool baro_now) {
+        if (lambda_sched) {
+            k_set_lambda<<<ceil_div(S_, 64), 64, 0, stream_>>>(S_, sched.ptr, n_steps, I.run_step.ptr, I.lam.ptr);
+            TM_LAUNCH_CHECK();
+        }
+        launch_fill_u64(I.du_dx.ptr, static_cast<size_t>(A_) * 3, 0, stream_);
+        I.eval(I.x.ptr, I.box.ptr, I.du_dx.ptr, du_dl_interval > 0 ? I.du_dl_buf.ptr : nullptr, nullptr, stream_,
+               tile_grid);
+        k_langevin<<<ceil_div(A_, TPB), TPB, 0, stream_>>>(A_, I.atom_sys.ptr, I.d_atom0.ptr, I.step.ptr, I.seed.ptr,
+                                                         I.ca.ptr, I.dt.ptr, I.cbs.ptr, I.ccs.ptr, I.du_dx.ptr,
+                                                         I.x.ptr, I.v.ptr);
+        TM_LAUNCH_CHECK();
+        if (I.has_baro) {
+            const BaroDev b = I.baro_dev();
+            k_baro_setup<<<ceil_div(S_, 64), 64, 0, stream_>>>(S_, b, I.box.ptr, I.box_after.ptr);
+            TM_LAUNCH_CHECK();
+            if (baro_now) {
+                I.eval(I.x.ptr, I.box.ptr, nullptr, nullptr, I.b_u_before.ptr, stream_, tile_grid);
+                if (I.nmol_total > 0) {
+                    k_baro_zero_centroids<<<ceil_div(I.nmol_total, TPB), TPB, 0, stream_>>>(
+                        I.nmol_total, I.mol_sys.ptr, I.b_trig.ptr, I.centroids.ptr);
+                    TM_LAUNCH_CHECK();
+                    k_baro_centroids<<<ceil_div(I.G, TPB), TPB, 0, stream_>>>(I.G, I.g_atom.ptr, I.g_mol.ptr,
+                                                                           I.mol_sys.ptr, I.b_trig.ptr, I.x.ptr,
+                                                                           I.centroids.ptr);
+                    TM_LAUNCH_CHECK();
+                }
+                k_baro_rescale<<<ceil_div(A_, TPB), TPB, 0, stream_>>>(
+                    A_, I.atom_sys.ptr, I.atom_mol.ptr, I.mol_count.ptr, I.b_trig.ptr, I.b_lscale.ptr, I.box.ptr,
+                    I.centroids.ptr, I.x.ptr, I.x_after.ptr);
+                TM_LAUNCH_CHECK();
+                I.eval(I.x_after.ptr, I.box_after.ptr, nullptr, nullptr, I.b_u_after.ptr, stream_, tile_grid);
+                k_baro_decide<<<ceil_div(S_, 64), 64, 0, stream_>>>(S_, b, I.box.ptr, I.box_after.ptr);
+                TM_LAUNCH_CHECK();
+                k_baro_apply<<<ceil_div(A_, TPB), TPB, 0, stream_>>>(A_, I.atom_sys.ptr, I.b_accept.ptr,
+                                                                   I.x_after.ptr, I.x.ptr);
+                TM_LAUNCH_CHECK();
+            }
+        }
+        if (x_interval > 0) {
+            k_store_frames2<<<ceil_div(A_, TPB), TPB, 0, stream_>>>(A_, I.atom_sys.ptr, I.d_atom0.ptr,
+                                                                  I.d_natoms.ptr, I.run_step.ptr, x_interval, F,
+                                                                  I.x.ptr, frames.ptr);
+            TM_LAUNCH_CHECK();
+        }
+        k_step_end<<<ceil_div(S_, 64), 64, 0, stream_>>>(S_, I.step.ptr, I.run_step.ptr, I.du_dl_buf.ptr,
+                                                        du_dl_interval, K, dl_traj.ptr, x_interval, F, I.box.ptr,
+                                                        box_frames.ptr);
+        TM_LAUNCH_CHECK();
+    };
+
+    // which steps of the run need the barostat energies (host mirror of the
+    // per-system barostat call counters)
+    auto baro_at = [&](int j) {
+        if (!I.has_baro) {
+            return false;
+        }
+        for (int s = 0; s < S_; s++) {
+            if (I.h_has_baro[s] && (I.h_bstep[s] + j + 1) % I.h_interval[s] == 0) {
+                return true;
+            }
+        }
+        return false;
+    };
+
+    int j = 0;
+    if (use_graph) {
+        int chunk = 10;
+        if (I.has_baro) {
+            chunk = I.h_interval[0];
+            for (int s = 0; s < S_; s++) {
+                if (I.h_has_baro[s]) {
+                    chunk = std::max(chunk, I.h_interval[s]);
+                }
+            }
+        }
+        std::map<std::vector<bool>, hipGraphExec_t> cache;
+        while (j + chunk <= n_steps) {
+            std::vector<bool> pattern(chunk);
+            for (int q = 0; q < chunk; q++) {
+                pattern[q] = baro_at(j + q);
+            }
+            auto it = cache.find(pattern);
+            if (it == cache.end()) {
+                hipGraph_t g;
+                TM_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+                for (int q = 0; q < chunk; q++) {
+                    one_step(pattern[q]);
+                }
+                TM_HIP_CHECK(hipStreamEndCapture(stream_, &g));
+                hipGraphExec_t ge;
+                TM_HIP_CHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+                TM_HIP_CHECK(hipGraphDestroy(g));
+                it = cache.emplace(pattern, ge).first;
+            }
+            TM_HIP_CHECK(hipGraphLaunch(it->second, stream_));
+            j += chunk;
+        }
+        TM_HIP_CHECK(hipStreamSynchronize(stream_));
+        for (auto &kv : cache) {
+            TM_HIP_CHECK(hipGraphExecDestroy(kv.second));
+        }
+    }
+    for (; j < n_steps; j++) {
+        one_step(baro_at(j));
+    }
+    for (int s = 0; s < S_; s++) {
+        if (I.h_has_baro[s]) {
+            I.h_bstep[s] = (I.h_bstep[s] + n_steps) % I.h_interval[s];
+        }
+    }
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+    if (du_dl_out && K > 0) {
+        std::vector<uint64_t> h(static_cast<size_t>(S_) * K);
+        dl_traj.download(h.data(), h.size(), stream_);
+        TM_HIP_CHECK(hipStreamSynchronize(stream_));
+        for (size_t i = 0; i < h.size(); i++) {
+            du_dl_out[i] = fixed_to_double(h[i], FIXED_SCALE);
+        }
+    }
+    if (x_out && F > 0) {
+        frames.download(x_out, static_cast<size_t>(F) * A_ * 3, stream_);
+    }
+    if (box_out && F > 0) {
+        box_frames.download(box_out, static_cast<size_t>(S_) * F * 9, stream_);
+    }
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void Engine::energies(double *u_out, double *du_dl_out) {
+    Impl &I = *impl_;
+    launch_fill_u64(I.u_buf.ptr, S_, 0, stream_);
+    launch_fill_u64(I.du_dl_buf.ptr, S_, 0, stream_);
+    I.eval(I.x.ptr, I.box.ptr, nullptr, I.du_dl_buf.ptr, I.u_buf.ptr, stream_, tile_grid);
+    std::vector<uint64_t> hu(S_), hl(S_);
+    I.u_buf.download(hu.data(), S_, stream_);
+    I.du_dl_buf.download(hl.data(), S_, stream_);
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+    launch_fill_u64(I.du_dl_buf.ptr, S_, 0, stream_);
+    for (int s = 0; s < S_; s++) {
+        if (u_out) {
+            u_out[s] = fixed_to_double(hu[s], FIXED_SCALE);
+        }
+        if (du_dl_out) {
+            du_dl_out[s] = fixed_to_double(hl[s], FIXED_SCALE);
+        }
+    }
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void Engine::get_x(int s, double *out) {
+    TM_HIP_CHECK(hipMemcpyAsync(out, impl_->x.ptr + h_atom0_[s] * 3, sizeof(double) * h_n_[s] * 3,
+                                hipMemcpyDeviceToHost, stream_));
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+void Engine::get_v(int s, double *out) {
+    TM_HIP_CHECK(hipMemcpyAsync(out, impl_->v.ptr + h_atom0_[s] * 3, sizeof(double) * h_n_[s] * 3,
+                                hipMemcpyDeviceToHost, stream_));
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+void Engine::get_box(int s, double *out) {
+    TM_HIP_CHECK(hipMemcpyAsync(out, impl_->box.ptr + s * 9, sizeof(double) * 9, hipMemcpyDeviceToHost, stream_));
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+void Engine::get_du_dx(int s, uint64_t *out) {
+    TM_HIP_CHECK(hipMemcpyAsync(out, impl_->du_dx.ptr + h_atom0_[s] * 3, sizeof(uint64_t) * h_n_[s] * 3,
+                                hipMemcpyDeviceToHost, stream_));
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+void Engine::set_x(int s, const double *in) {
+    TM_HIP_CHECK(hipMemcpyAsync(impl_->x.ptr + h_atom0_[s] * 3, in, sizeof(double) * h_n_[s] * 3,
+                                hipMemcpyHostToDevice, stream_));
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+void Engine::set_v(int s, const double *in) {
+    TM_HIP_CHECK(hipMemcpyAsync(impl_->v.ptr + h_atom0_[s] * 3, in, sizeof(double) * h_n_[s] * 3,
+                                hipMemcpyHostToDevice, stream_));
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+void Engine::set_box(int s, const double *in) {
+    TM_HIP_CHECK(
+        hipMemcpyAsync(impl_->box.ptr + s * 9, in, sizeof(double) * 9, hipMemcpyHostToDevice, stream_));
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+int64_t Engine::get_step(int s) {
+    int64_t v = 0;
+    TM_HIP_CHECK(hipMemcpyAsync(&v, impl_->step.ptr + s, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+    return v;
+}
+void Engine::get_barostat_state(int s, double *vscale, int *accepted, int *attempted, int *baro_step) {
+    Impl &I = *impl_;
+    *baro_step = I.h_bstep[s];
+    TM_HIP_CHECK(hipMemcpyAsync(vscale, I.b_vscale.ptr + s, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    TM_HIP_CHECK(hipMemcpyAsync(accepted, I.b_accepted.ptr + s, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    TM_HIP_CHECK(hipMemcpyAsync(attempted, I.b_attempted.ptr + s, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void Engine::check_status() {
+    Impl &I = *impl_;
+    if (!I.has_nb) {
+        return;
+    }
+    std::vector<int> st(S_);
+    I.nb.status.download(st.data(), S_, stream_);
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+    for (int s = 0; s < S_; s++) {
+        if (st[s] & NB_STATUS_BOX_TOO_SMALL) {
+            throw Error("Cutoff with padding is more than half of the box width, neighborlist is no longer reliable");
+        }
+        if (st[s] & NB_STATUS_BOX_NOT_ORTHO) {
+            throw Error("Provided non-ortholinear box, unable to compute nonbonded energy");
+        }
+        if (st[s] & NB_STATUS_TILE_OVERFLOW) {
+            throw Error("Engine: neighborlist tile buffer overflow");
+        }
+    }
+}
+
+void Engine::profile(int reps, double *ms_out) {
+    Impl &I = *impl_;
+    hipEvent_t e0, e1;
+    TM_HIP_CHECK(hipEventCreate(&e0));
+    TM_HIP_CHECK(hipEventCreate(&e1));
+    auto timed = [&](auto &&fn) {
+        fn(); // warm
+        TM_HIP_CHECK(hipEventRecord(e0, stream_));
+        for (int r = 0; r < reps; r++) {
+            fn();
+        }
+        TM_HIP_CHECK(hipEventRecord(e1, stream_));
+        TM_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        TM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        return static_cast<double>(ms) / reps;
+    };
+    ms_out[0] = ms_out[1] = ms_out[2] = ms_out[3] = 0.0;
+    if (I.has_nb) {
+        ms_out[0] = timed([&] {
+            I.nb.invalidate(stream_);
+            I.nb.prepare(I.x.ptr, I.nb_params.ptr, I.box.ptr, I.nb_plane.ptr, I.nb_offset.ptr, I.lam.ptr, 0.0, stream_);
+        });
+        ms_out[1] = timed([&] { I.nb.compute(I.box.ptr, I.du_dx.ptr, nullptr, nullptr, nullptr, stream_, tile_grid); });
+    }
+    ms_out[2] = timed([&] {
+        const bool had = I.has_nb;
+        I.has_nb = false;
+        I.eval(I.x.ptr, I.box.ptr, I.du_dx.ptr, nullptr, nullptr, stream_, tile_grid);
+        I.has_nb = had;
+    });
+    DeviceArray<double> xs(static_cast<size_t>(A_) * 3), vs(static_cast<size_t>(A_) * 3);
+    TM_HIP_CHECK(hipMemcpyAsync(xs.ptr, I.x.ptr, xs.bytes(), hipMemcpyDeviceToDevice, stream_));
+    TM_HIP_CHECK(hipMemcpyAsync(vs.ptr, I.v.ptr, vs.bytes(), hipMemcpyDeviceToDevice, stream_));
+    ms_out[3] = timed([&] {
+        k_langevin<<<ceil_div(A_, 256), 256, 0, stream_>>>(A_, I.atom_sys.ptr, I.d_atom0.ptr, I.step.ptr, I.seed.ptr,
+                                                         I.ca.ptr, I.dt.ptr, I.cbs.ptr, I.ccs.ptr, I.du_dx.ptr,
+                                                         xs.ptr, vs.ptr);
+    });
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+    launch_fill_u64(I.du_dx.ptr, static_cast<size_t>(A_) * 3, 0, stream_);
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+    TM_HIP_CHECK(hipEventDestroy(e0));
+    TM_HIP_CHECK(hipEventDestroy(e1));
+}
+
+int Engine::total_tiles() {
+    Impl &I = *impl_;
+    if (!I.has_nb) {
+        return 0;
+    }
+    int n = 0;
+    TM_HIP_CHECK(hipMemcpyAsync(&n, I.nb.tile_prefix.ptr + S_, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+    return n;
+}
+
+size_t Engine::device_bytes() const {
+    const Impl &I = *impl_;
+    size_t b = I.x.bytes() * 3 + I.v.bytes() + I.du_dx.bytes() + I.cbs.bytes() * 2 + I.atom_sys.bytes();
+    if (I.has_nb) {
+        b += I.nb.tile_atoms.bytes() + I.nb.tile_row.bytes() + I.nb.pos.bytes() * 2 + I.nb.perm.bytes() * 2 +
+             I.nb.nblist_x.bytes() + I.nb_params.bytes();
+    }
+    return b;
+}
+
+} // namespace tmamd

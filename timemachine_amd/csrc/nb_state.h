@@ -1,0 +1,155 @@
+// Device-resident nonbonded state for S >= 1 systems.
+//
+// One code path serves both the single-system Potential API (S = 1, caller
+// buffers) and the batched engine (S systems packed into one arena). Every
+// kernel reads a per-system descriptor table (NbSys) and never needs a host
+// round trip: the rebuild decision, the spatial sort, the neighborlist and
+// the tile count all live on the device, so the whole sequence can be
+// replayed from a captured hipGraph.
+//
+// HBM layout per system s (all arrays are global, offsets are in NbSys):
+//   x, params, plane, offset      [atom0 .. atom0 + n_atoms)         caller / engine arrays, original order
+//   perm, pos (float4), par (float4) [slot0 .. slot0 + nblocks*64)   sorted order, padded to whole tiles
+//   bb_ctr, bb_ext (float4)       [block0 .. block0 + nblocks)
+//   tile_row, tile_atoms[64]      [tile0 .. tile0 + tile_cap)
+// pos = (x, y, z, w) and par = (q, sig, eps, dw/dlambda) are the FP32 values
+// the pair arithmetic consumes, so one 16-byte load fetches each.
+#pragma once
+
+#include <vector>
+
+#include "tm_common.h"
+
+namespace tmamd {
+
+constexpr uint32_t NB_INVALID = 0xFFFFFFFFu;
+
+// Status bits raised on the device (box checks of
+// timemachine/cpp/src/nonbonded_all_pairs.cu:315-329, tile buffer overflow).
+constexpr int NB_STATUS_BOX_TOO_SMALL = 1;
+constexpr int NB_STATUS_BOX_NOT_ORTHO = 2;
+constexpr int NB_STATUS_TILE_OVERFLOW = 4;
+
+struct NbSys {
+    int atom0;    // first global atom (rows of x / du_dx / params)
+    int n_atoms;  // atoms of the system
+    int k;        // atoms in the nonbonded set (== n_atoms unless a subset is given)
+    int slot0;    // first sorted slot
+    int nblocks;  // ceil(k / 64)
+    int block0;   // first global bounding-box index
+    int tile0;    // first tile
+    int tile_cap; // tile capacity
+    int red_idx;  // index of the system in u / du_dl reduction buffers
+    int subset0;  // offset into the subset atom list, -1 when all atoms take part
+    int sorted;   // 1: spatial (Hilbert) sort on rebuild, 0: identity order
+    int pad0;
+    float beta;
+    float cutoff;
+    float nb_cutoff; // cutoff + padding, FP32 (neighborlist predicate)
+    float pad1;
+    double cutoff_d;
+    double padding_d;
+};
+
+// Upper bound on tiles of one system: row block r holds at most
+// ceil((k - 64 r) / 64) full tiles plus one partial tile.
+inline int nb_tile_capacity(int k) {
+    const int nb = ceil_div(k, TILE);
+    return nb * (nb + 1) / 2 + nb;
+}
+
+struct NbState {
+    int S = 0;
+    int total_slots = 0;
+    int total_blocks = 0;
+    int total_tiles = 0;
+    int max_blocks = 0;
+    int max_slots = 0;
+    int max_atoms = 0;
+    std::vector<NbSys> h_sys;
+
+    DeviceArray<NbSys> sys;
+    DeviceArray<int> subset;      // subset atom lists (local indices)
+    DeviceArray<int> rebuild;     // [S]
+    DeviceArray<int> status;      // [S]
+    DeviceArray<int> tile_count;  // [S]
+    DeviceArray<int> tile_prefix; // [S + 1]
+    DeviceArray<int> tile_row;    // [total_tiles]
+    DeviceArray<uint32_t> tile_atoms; // [total_tiles * 64]
+    DeviceArray<uint32_t> perm;   // [total_slots] sorted slot -> local atom index
+    DeviceArray<uint32_t> keys;   // [total_slots] sort keys (scratch)
+    DeviceArray<float4> pos;      // [total_slots]
+    DeviceArray<float4> par;      // [total_slots]
+    DeviceArray<float4> bb_ctr;   // [total_blocks]
+    DeviceArray<float4> bb_ext;   // [total_blocks]
+    DeviceArray<double> nblist_x; // [sum n_atoms * 3] coordinates at the last build
+    DeviceArray<double> nblist_box; // [S * 9]
+
+    // sys_in: atom0/n_atoms/k/subset0/sorted/beta/cutoff/padding filled in,
+    // the rest is laid out here. subset_atoms concatenates subset lists.
+    void init(const std::vector<NbSys> &sys_in, const std::vector<int> &subset_atoms);
+
+    // Force a rebuild of every system on the next prepare().
+    void invalidate(hipStream_t stream);
+
+    // Rebuild check, spatial sort, gather to sorted FP32, bounding boxes,
+    // neighborlist and tile prefix. No host synchronisation.
+    //   x      [*, 3] f64, rows atom0 + i
+    //   params [*, 3] f64 (q, sig/2, sqrt(eps)), rows atom0 + i
+    //   plane, offset [*] int32 per atom
+    //   boxes  [S, 9] f64
+    //   lambdas [S] f64 or nullptr (then lambda0 for every system)
+    void prepare(
+        const double *x,
+        const double *params,
+        const double *boxes,
+        const int *plane,
+        const int *offset,
+        const double *lambdas,
+        double lambda0,
+        hipStream_t stream);
+
+    // Just the neighborlist part (used by the Neighborlist query API).
+    void build_only(const double *x, const double *boxes, hipStream_t stream);
+
+    // Pair kernel over all tiles of all systems. Outputs are accumulated:
+    //   du_dx [*, 3], du_dp [*, 3] (rows atom0 + i), du_dl / u [red_idx].
+    void compute(
+        const double *boxes,
+        uint64_t *du_dx,
+        uint64_t *du_dp,
+        uint64_t *du_dl,
+        uint64_t *u,
+        hipStream_t stream,
+        int grid_override = 0) const;
+};
+
+// Pair-list kernel (exclusions / explicit pairs). Pairs index global atoms.
+struct PairListArgs {
+    int M;
+    const int *pairs;      // [M, 2] global atom indices
+    const double *scales;  // [M, 2] (charge, lj)
+    const int *pair_sys;   // [M] system of each pair, nullptr -> 0
+    const int *red_idx;    // [S] reduction index per system, nullptr -> 0
+    const double *x;
+    const double *params;
+    const double *boxes;   // [S, 9]
+    const int *plane;
+    const int *offset;
+    const double *lambdas; // [S] or nullptr
+    double lambda0;
+    const double *cutoffs; // [S] or nullptr
+    double cutoff0;
+    const float *betas;    // [S] or nullptr
+    float beta0;
+    int negated;
+};
+
+void launch_pair_list(
+    const PairListArgs &a, uint64_t *du_dx, uint64_t *du_dp, uint64_t *du_dl, uint64_t *u, hipStream_t stream);
+
+// Helpers shared by several translation units.
+void launch_fill_u64(uint64_t *p, size_t n, uint64_t v, hipStream_t stream);
+void launch_fill_i32(int *p, size_t n, int v, hipStream_t stream);
+
+} // namespace tmamd

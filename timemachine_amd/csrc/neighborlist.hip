@@ -1,0 +1,679 @@
+// Rebuild check, spatial sort, sorted FP32 gather, block bounds and the
+// 64-atom tile neighborlist for S systems at once.
+//
+// Behaviour follows the reference (timemachine/cpp/src/nonbonded_all_pairs.cu:222-433,
+// timemachine/cpp/src/neighborlist.cu:126-185, kernels/k_neighborlist.cuh:9-427):
+//  * rebuild when the box changed or any atom moved more than padding/2 since
+//    the last build;
+//  * atoms are permuted along a Hilbert curve before building (any order gives
+//    the same results, only the tile occupancy changes);
+//  * row block r of the upper-triangular list holds every column atom j >= 64 r
+//    (sorted order) within nb_cutoff of at least one atom of the block.
+// The design is wave64-native: one wave per 64-atom row block, column atoms
+// tested lane-parallel, interacting atoms compacted with ballot/mbcnt into LDS
+// and emitted as full 64-atom tiles.
+#include <algorithm>
+
+#include "nb_pair.h"
+#include "nb_state.h"
+#include "wave.h"
+
+namespace tmamd {
+
+namespace {
+
+constexpr int SORT_THREADS = 1024;
+constexpr int HILBERT_BITS = 4;
+constexpr int HILBERT_GRID = 1 << HILBERT_BITS;
+constexpr int HILBERT_CELLS = HILBERT_GRID * HILBERT_GRID * HILBERT_GRID;
+
+// 3D Hilbert index of a cell, Skilling's transpose algorithm
+// (J. Skilling, "Programming the Hilbert curve", AIP Conf. Proc. 707, 2004).
+__device__ uint32_t hilbert_index(uint32_t x, uint32_t y, uint32_t z) {
+    uint32_t X[3] = {x, y, z};
+    const uint32_t M = 1u << (HILBERT_BITS - 1);
+    for (uint32_t Q = M; Q > 1; Q >>= 1) {
+        const uint32_t P = Q - 1;
+        for (int i = 0; i < 3; i++) {
+            if (X[i] & Q) {
+                X[0] ^= P;
+            } else {
+                const uint32_t t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    X[1] ^= X[0];
+    X[2] ^= X[1];
+    uint32_t t = 0;
+    for (uint32_t Q = M; Q > 1; Q >>= 1) {
+        if (X[2] & Q) {
+            t ^= Q - 1;
+        }
+    }
+    X[0] ^= t;
+    X[1] ^= t;
+    X[2] ^= t;
+    uint32_t key = 0;
+    for (int b = HILBERT_BITS - 1; b >= 0; b--) {
+        for (int i = 0; i < 3; i++) {
+            key = (key << 1) | ((X[i] >> b) & 1u);
+        }
+    }
+    return key;
+}
+
+__device__ __forceinline__ uint32_t cell_of(double v, double box) {
+    double f = v / box;
+    f -= floor(f);
+    int c = static_cast<int>(f * HILBERT_GRID);
+    c = c < 0 ? 0 : (c >= HILBERT_GRID ? HILBERT_GRID - 1 : c);
+    return static_cast<uint32_t>(c);
+}
+
+__global__ void k_fill_u64(uint64_t *p, size_t n, uint64_t v) {
+    size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+    if (i < n) {
+        p[i] = v;
+    }
+}
+
+__global__ void k_fill_i32(int *p, size_t n, int v) {
+    size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+    if (i < n) {
+        p[i] = v;
+    }
+}
+
+// grid (ceil(max_atoms / 256), S). Sets rebuild[s] when the box changed or an
+// atom of the nonbonded set moved more than padding / 2 since the last build;
+// validates the box as the reference does on rebuild.
+__global__ void k_check_rebuild(
+    const NbSys *__restrict__ sys,
+    const int *__restrict__ subset,
+    const double *__restrict__ x,
+    const double *__restrict__ nblist_x,
+    const double *__restrict__ boxes,
+    const double *__restrict__ nblist_box,
+    int *__restrict__ rebuild,
+    int *__restrict__ status) {
+    const int s = blockIdx.y;
+    const NbSys S = sys[s];
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const double *box = boxes + s * 9;
+    if (a < 9) {
+        if (box[a] != nblist_box[s * 9 + a]) {
+            rebuild[s] = 1;
+        }
+        if (a == 0) {
+            const double db_cutoff = 2.0 * (S.cutoff_d + S.padding_d);
+            int st = 0;
+            for (int d = 0; d < 9; d++) {
+                if (d == 0 || d == 4 || d == 8) {
+                    if (box[d] < db_cutoff) {
+                        st |= NB_STATUS_BOX_TOO_SMALL;
+                    }
+                } else if (box[d] != 0.0) {
+                    st |= NB_STATUS_BOX_NOT_ORTHO;
+                }
+            }
+            if (st) {
+                atomicOr(status + s, st);
+            }
+        }
+    }
+    if (a >= S.k) {
+        return;
+    }
+    const int atom = S.subset0 >= 0 ? subset[S.subset0 + a] : a;
+    const int g = S.atom0 + atom;
+    const double dx = x[g * 3 + 0] - nblist_x[g * 3 + 0];
+    const double dy = x[g * 3 + 1] - nblist_x[g * 3 + 1];
+    const double dz = x[g * 3 + 2] - nblist_x[g * 3 + 2];
+    const double half = 0.5 * S.padding_d;
+    if (dx * dx + dy * dy + dz * dz > half * half) {
+        rebuild[s] = 1; // every writer stores the same value
+    }
+}
+
+// Exclusive scan of HILBERT_CELLS counts held in LDS, SORT_THREADS threads.
+__device__ void block_exclusive_scan_cells(int *cells, int *wave_tot) {
+    constexpr int PER = HILBERT_CELLS / SORT_THREADS;
+    const int t = threadIdx.x;
+    int local[PER];
+    int sum = 0;
+    for (int i = 0; i < PER; i++) {
+        local[i] = cells[t * PER + i];
+        sum += local[i];
+    }
+    // inclusive scan of per-thread sums within the wave
+    int inc = sum;
+    const int lane = t & 63;
+    for (int off = 1; off < 64; off <<= 1) {
+        int v = __shfl_up(inc, off, 64);
+        if (lane >= off) {
+            inc += v;
+        }
+    }
+    const int w = t >> 6;
+    if (lane == 63) {
+        wave_tot[w] = inc;
+    }
+    __syncthreads();
+    if (t == 0) {
+        int acc = 0;
+        for (int i = 0; i < SORT_THREADS / 64; i++) {
+            int v = wave_tot[i];
+            wave_tot[i] = acc;
+            acc += v;
+        }
+    }
+    __syncthreads();
+    int run = wave_tot[w] + inc - sum;
+    for (int i = 0; i < PER; i++) {
+        cells[t * PER + i] = run;
+        run += local[i];
+    }
+    __syncthreads();
+}
+
+// One workgroup per system; only systems flagged for rebuild do work.
+// Counting sort of the nonbonded atoms by Hilbert cell, then insertion sort
+// by atom index inside each cell so the permutation is deterministic.
+// Also records the coordinates/box the neighborlist is built from and
+// resets the tile counter.
+__global__ __launch_bounds__(SORT_THREADS) void k_sort(
+    const NbSys *__restrict__ sys,
+    const int *__restrict__ subset,
+    const double *__restrict__ x,
+    const double *__restrict__ boxes,
+    const int *__restrict__ rebuild,
+    uint32_t *__restrict__ keys,
+    uint32_t *__restrict__ perm,
+    double *__restrict__ nblist_x,
+    double *__restrict__ nblist_box,
+    int *__restrict__ tile_count) {
+    const int s = blockIdx.x;
+    if (!rebuild[s]) {
+        return;
+    }
+    const NbSys S = sys[s];
+    const int t = threadIdx.x;
+    __shared__ int count[HILBERT_CELLS];
+    __shared__ int start[HILBERT_CELLS];
+    __shared__ int wave_tot[SORT_THREADS / 64];
+
+    const double bx = boxes[s * 9 + 0];
+    const double by = boxes[s * 9 + 4];
+    const double bz = boxes[s * 9 + 8];
+
+    if (!S.sorted) {
+        for (int a = t; a < S.k; a += SORT_THREADS) {
+            perm[S.slot0 + a] = static_cast<uint32_t>(S.subset0 >= 0 ? subset[S.subset0 + a] : a);
+        }
+    } else {
+        for (int c = t; c < HILBERT_CELLS; c += SORT_THREADS) {
+            count[c] = 0;
+        }
+        __syncthreads();
+        for (int a = t; a < S.k; a += SORT_THREADS) {
+            const int atom = S.subset0 >= 0 ? subset[S.subset0 + a] : a;
+            const int g = S.atom0 + atom;
+            const uint32_t key =
+                hilbert_index(cell_of(x[g * 3 + 0], bx), cell_of(x[g * 3 + 1], by), cell_of(x[g * 3 + 2], bz));
+            keys[S.slot0 + a] = key;
+            atomicAdd(&count[key], 1);
+        }
+        __syncthreads();
+        for (int c = t; c < HILBERT_CELLS; c += SORT_THREADS) {
+            start[c] = count[c];
+        }
+        __syncthreads();
+        block_exclusive_scan_cells(start, wave_tot);
+        // count[] becomes the running fill position
+        for (int c = t; c < HILBERT_CELLS; c += SORT_THREADS) {
+            count[c] = start[c];
+        }
+        __syncthreads();
+        for (int a = t; a < S.k; a += SORT_THREADS) {
+            const int atom = S.subset0 >= 0 ? subset[S.subset0 + a] : a;
+            const uint32_t key = keys[S.slot0 + a];
+            const int p = atomicAdd(&count[key], 1);
+            perm[S.slot0 + p] = static_cast<uint32_t>(atom);
+        }
+        __syncthreads();
+        for (int c = t; c < HILBERT_CELLS; c += SORT_THREADS) {
+            const int b = start[c];
+            const int e = count[c];
+            uint32_t *seg = perm + S.slot0;
+            for (int i = b + 1; i < e; i++) {
+                const uint32_t v = seg[i];
+                int j = i - 1;
+                while (j >= b && seg[j] > v) {
+                    seg[j + 1] = seg[j];
+                    j--;
+                }
+                seg[j + 1] = v;
+            }
+        }
+    }
+    for (int a = S.k + t; a < S.nblocks * TILE; a += SORT_THREADS) {
+        perm[S.slot0 + a] = NB_INVALID;
+    }
+    for (int i = t; i < S.n_atoms * 3; i += SORT_THREADS) {
+        nblist_x[S.atom0 * 3 + i] = x[S.atom0 * 3 + i];
+    }
+    if (t < 9) {
+        nblist_box[s * 9 + t] = boxes[s * 9 + t];
+    }
+    if (t == 0) {
+        tile_count[s] = 0;
+    }
+}
+
+// grid (ceil(max_slots / 256), S): sorted FP32 coordinates and parameters.
+// w = (plane + offset * lambda) * cutoff and dw/dlambda = offset * cutoff, in
+// f64 before rounding to FP32 (kernels/k_lambda_transformer_jit.cuh:25-55 with
+// the identity transform).
+__global__ void k_gather(
+    const NbSys *__restrict__ sys,
+    const double *__restrict__ x,
+    const double *__restrict__ params,
+    const int *__restrict__ plane,
+    const int *__restrict__ offset,
+    const double *__restrict__ lambdas,
+    const double lambda0,
+    const uint32_t *__restrict__ perm,
+    float4 *__restrict__ pos,
+    float4 *__restrict__ par) {
+    const int s = blockIdx.y;
+    const NbSys S = sys[s];
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= S.nblocks * TILE) {
+        return;
+    }
+    const uint32_t atom = perm[S.slot0 + slot];
+    if (atom == NB_INVALID) {
+        pos[S.slot0 + slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+        par[S.slot0 + slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    const int g = S.atom0 + static_cast<int>(atom);
+    const double lam = lambdas ? lambdas[s] : lambda0;
+    const int pl = plane ? plane[g] : 0;
+    const int of = offset ? offset[g] : 0;
+    const double w = (pl + of * lam) * S.cutoff_d;
+    const double dw_dl = of * S.cutoff_d;
+    pos[S.slot0 + slot] = make_float4(static_cast<float>(x[g * 3 + 0]), static_cast<float>(x[g * 3 + 1]),
+                                      static_cast<float>(x[g * 3 + 2]), static_cast<float>(w));
+    if (params) {
+        par[S.slot0 + slot] = make_float4(static_cast<float>(params[g * 3 + 0]), static_cast<float>(params[g * 3 + 1]),
+                                          static_cast<float>(params[g * 3 + 2]), static_cast<float>(dw_dl));
+    }
+}
+
+// grid (ceil(max_blocks / 4), S), 256 threads: one wave per 64-atom block.
+// Bounding box of the block with every atom imaged next to the block's first
+// atom (a valid periodic box for any block narrower than half the box).
+__global__ __launch_bounds__(256) void k_block_bounds(
+    const NbSys *__restrict__ sys,
+    const double *__restrict__ boxes,
+    const int *__restrict__ rebuild,
+    const float4 *__restrict__ pos,
+    float4 *__restrict__ bb_ctr,
+    float4 *__restrict__ bb_ext) {
+    const int s = blockIdx.y;
+    if (!rebuild[s]) {
+        return;
+    }
+    const NbSys S = sys[s];
+    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (b >= S.nblocks) {
+        return;
+    }
+    const int lane = threadIdx.x & 63;
+    const float bx = static_cast<float>(boxes[s * 9 + 0]);
+    const float by = static_cast<float>(boxes[s * 9 + 4]);
+    const float bz = static_cast<float>(boxes[s * 9 + 8]);
+    const float ibx = 1.0f / bx;
+    const float iby = 1.0f / by;
+    const float ibz = 1.0f / bz;
+    const float4 ref = pos[S.slot0 + b * TILE];
+    const int slot = b * TILE + lane;
+    float px = ref.x, py = ref.y, pz = ref.z;
+    if (slot < S.k) {
+        const float4 p = pos[S.slot0 + slot];
+        px = ref.x + min_image_f(p.x - ref.x, bx, ibx);
+        py = ref.y + min_image_f(p.y - ref.y, by, iby);
+        pz = ref.z + min_image_f(p.z - ref.z, bz, ibz);
+    }
+    const float lox = wave_min(px), hix = wave_max(px);
+    const float loy = wave_min(py), hiy = wave_max(py);
+    const float loz = wave_min(pz), hiz = wave_max(pz);
+    if (lane == 0) {
+        bb_ctr[S.block0 + b] = make_float4(0.5f * (lox + hix), 0.5f * (loy + hiy), 0.5f * (loz + hiz), 0.f);
+        bb_ext[S.block0 + b] = make_float4(0.5f * (hix - lox), 0.5f * (hiy - loy), 0.5f * (hiz - loz), 0.f);
+    }
+}
+
+__device__ __forceinline__ float box_gap(float d, float ext, float b, float ib) {
+    d = min_image_f(d, b, ib);
+    return fmaxf(0.0f, fabsf(d) - ext);
+}
+
+// grid (ceil(max_blocks / 4), S), 256 threads: one wave per row block.
+// Column blocks c >= r are screened by bounding boxes (with a small margin
+// so the screen never rejects what the exact test would accept); candidate
+// column atoms are then tested exactly against the row atoms with the FP32
+// min-image predicate d^2 < nb_cutoff^2 (oracle/nblist_ref.c restates it).
+__global__ __launch_bounds__(256) void k_find_tiles(
+    const NbSys *__restrict__ sys,
+    const double *__restrict__ boxes,
+    const int *__restrict__ rebuild,
+    const float4 *__restrict__ pos,
+    const float4 *__restrict__ bb_ctr,
+    const float4 *__restrict__ bb_ext,
+    int *__restrict__ tile_count,
+    int *__restrict__ tile_row,
+    uint32_t *__restrict__ tile_atoms,
+    int *__restrict__ status) {
+    const int s = blockIdx.y;
+    if (!rebuild[s]) {
+        return;
+    }
+    const NbSys S = sys[s];
+    const int wave = threadIdx.x >> 6;
+    const int r = blockIdx.x * 4 + wave;
+    if (r >= S.nblocks) {
+        return;
+    }
+    const int lane = threadIdx.x & 63;
+    __shared__ float4 row_lds[4][TILE];
+    __shared__ uint32_t buf_lds[4][2 * TILE];
+    float4 *rows = row_lds[wave];
+    uint32_t *buf = buf_lds[wave];
+
+    const float bx = static_cast<float>(boxes[s * 9 + 0]);
+    const float by = static_cast<float>(boxes[s * 9 + 4]);
+    const float bz = static_cast<float>(boxes[s * 9 + 8]);
+    const float ibx = 1.0f / bx;
+    const float iby = 1.0f / by;
+    const float ibz = 1.0f / bz;
+    const float cut = S.nb_cutoff;
+    const float cut2 = cut * cut;
+    const float scr = cut + 1e-3f; // screening radius, strictly larger
+    const float scr2 = scr * scr;
+
+    const int i = r * TILE + lane;
+    const bool row_valid = i < S.k;
+    const float4 pi = row_valid ? pos[S.slot0 + i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    rows[lane] = pi;
+    wave_lds_sync();
+    const float4 rc = bb_ctr[S.block0 + r];
+    const float4 re = bb_ext[S.block0 + r];
+
+    int nbuf = 0;
+    for (int cbase = r; cbase < S.nblocks; cbase += TILE) {
+        const int c = cbase + lane;
+        bool cand = false;
+        if (c < S.nblocks) {
+            const float4 cc = bb_ctr[S.block0 + c];
+            const float4 ce = bb_ext[S.block0 + c];
+            const float gx = box_gap(rc.x - cc.x, re.x + ce.x, bx, ibx);
+            const float gy = box_gap(rc.y - cc.y, re.y + ce.y, by, iby);
+            const float gz = box_gap(rc.z - cc.z, re.z + ce.z, bz, ibz);
+            cand = gx * gx + gy * gy + gz * gz < scr2;
+        }
+        uint64_t cmask = ballot(cand);
+        while (cmask) {
+            const int off = __builtin_ctzll(cmask);
+            cmask &= cmask - 1;
+            const int cb = cbase + off;
+            const float4 cc = bb_ctr[S.block0 + cb];
+            const float4 ce = bb_ext[S.block0 + cb];
+            // row atoms near the column block
+            bool row_near = false;
+            if (row_valid) {
+                const float gx = box_gap(pi.x - cc.x, ce.x, bx, ibx);
+                const float gy = box_gap(pi.y - cc.y, ce.y, by, iby);
+                const float gz = box_gap(pi.z - cc.z, ce.z, bz, ibz);
+                row_near = gx * gx + gy * gy + gz * gz < scr2;
+            }
+            const uint64_t rmask = ballot(row_near);
+            const int j = cb * TILE + lane;
+            const bool col_valid = j < S.k;
+            bool pending = false;
+            float4 pj = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (col_valid && rmask) {
+                pj = pos[S.slot0 + j];
+                const float gx = box_gap(pj.x - rc.x, re.x, bx, ibx);
+                const float gy = box_gap(pj.y - rc.y, re.y, by, iby);
+                const float gz = box_gap(pj.z - rc.z, re.z, bz, ibz);
+                pending = gx * gx + gy * gy + gz * gz < scr2;
+            }
+            bool found = false;
+            uint64_t m = rmask;
+            while (m && ballot(pending && !found)) {
+                const int il = __builtin_ctzll(m);
+                m &= m - 1;
+                const float4 q = rows[il];
+                const float dx = min_image_f(q.x - pj.x, bx, ibx);
+                const float dy = min_image_f(q.y - pj.y, by, iby);
+                const float dz = min_image_f(q.z - pj.z, bz, ibz);
+                const float d2 = dx * dx + dy * dy + dz * dz;
+                found = found || (pending && d2 < cut2);
+            }
+            const uint64_t fmask = ballot(found);
+            if (found) {
+                buf[nbuf + lanes_below(fmask)] = static_cast<uint32_t>(j);
+            }
+            nbuf += __popcll(fmask);
+            wave_lds_sync();
+            if (nbuf >= TILE) {
+                int tile = 0;
+                if (lane == 0) {
+                    tile = atomicAdd(tile_count + s, 1);
+                }
+                tile = __shfl(tile, 0, 64);
+                if (tile < S.tile_cap) {
+                    tile_atoms[static_cast<size_t>(S.tile0 + tile) * TILE + lane] = buf[lane];
+                    if (lane == 0) {
+                        tile_row[S.tile0 + tile] = r;
+                    }
+                } else if (lane == 0) {
+                    atomicOr(status + s, NB_STATUS_TILE_OVERFLOW);
+                }
+                const uint32_t rest = buf[TILE + lane];
+                wave_lds_sync();
+                buf[lane] = rest;
+                nbuf -= TILE;
+                wave_lds_sync();
+            }
+        }
+    }
+    if (nbuf > 0) {
+        const uint32_t v = lane < nbuf ? buf[lane] : NB_INVALID;
+        int tile = 0;
+        if (lane == 0) {
+            tile = atomicAdd(tile_count + s, 1);
+        }
+        tile = __shfl(tile, 0, 64);
+        if (tile < S.tile_cap) {
+            tile_atoms[static_cast<size_t>(S.tile0 + tile) * TILE + lane] = v;
+            if (lane == 0) {
+                tile_row[S.tile0 + tile] = r;
+            }
+        } else if (lane == 0) {
+            atomicOr(status + s, NB_STATUS_TILE_OVERFLOW);
+        }
+    }
+}
+
+// One workgroup: clamp counts to capacity, exclusive prefix over systems
+// (tile_prefix[S] = total), clear the rebuild flags.
+__global__ __launch_bounds__(256) void k_tile_prefix(
+    const NbSys *__restrict__ sys, int S, int *__restrict__ tile_count, int *__restrict__ tile_prefix,
+    int *__restrict__ rebuild) {
+    __shared__ int part[256];
+    const int t = threadIdx.x;
+    const int per = (S + 255) / 256;
+    int sum = 0;
+    for (int k = 0; k < per; k++) {
+        const int s = t * per + k;
+        if (s < S) {
+            int c = tile_count[s];
+            const int cap = sys[s].tile_cap;
+            sum += c < cap ? c : cap;
+        }
+    }
+    part[t] = sum;
+    __syncthreads();
+    if (t == 0) {
+        int acc = 0;
+        for (int i = 0; i < 256; i++) {
+            int v = part[i];
+            part[i] = acc;
+            acc += v;
+        }
+        tile_prefix[S] = acc;
+    }
+    __syncthreads();
+    int run = part[t];
+    for (int k = 0; k < per; k++) {
+        const int s = t * per + k;
+        if (s < S) {
+            tile_prefix[s] = run;
+            int c = tile_count[s];
+            const int cap = sys[s].tile_cap;
+            run += c < cap ? c : cap;
+            rebuild[s] = 0;
+        }
+    }
+}
+
+} // namespace
+
+void launch_fill_u64(uint64_t *p, size_t n, uint64_t v, hipStream_t stream) {
+    if (n == 0) {
+        return;
+    }
+    k_fill_u64<<<ceil_div(static_cast<long long>(n), 256), 256, 0, stream>>>(p, n, v);
+    TM_LAUNCH_CHECK();
+}
+
+void launch_fill_i32(int *p, size_t n, int v, hipStream_t stream) {
+    if (n == 0) {
+        return;
+    }
+    k_fill_i32<<<ceil_div(static_cast<long long>(n), 256), 256, 0, stream>>>(p, n, v);
+    TM_LAUNCH_CHECK();
+}
+
+void NbState::init(const std::vector<NbSys> &sys_in, const std::vector<int> &subset_atoms) {
+    S = static_cast<int>(sys_in.size());
+    h_sys = sys_in;
+    total_slots = total_blocks = total_tiles = 0;
+    max_blocks = max_slots = max_atoms = 0;
+    int total_atoms_end = 0;
+    for (int s = 0; s < S; s++) {
+        NbSys &y = h_sys[s];
+        y.nblocks = ceil_div(y.k, TILE);
+        y.slot0 = total_slots;
+        y.block0 = total_blocks;
+        y.tile0 = total_tiles;
+        y.tile_cap = nb_tile_capacity(y.k);
+        y.nb_cutoff = static_cast<float>(y.cutoff_d + y.padding_d);
+        y.cutoff = static_cast<float>(y.cutoff_d);
+        total_slots += y.nblocks * TILE;
+        total_blocks += y.nblocks;
+        total_tiles += y.tile_cap;
+        max_blocks = std::max(max_blocks, y.nblocks);
+        max_slots = std::max(max_slots, y.nblocks * TILE);
+        max_atoms = std::max(max_atoms, y.n_atoms);
+        total_atoms_end = std::max(total_atoms_end, y.atom0 + y.n_atoms);
+    }
+    sys.alloc(S);
+    sys.upload(h_sys.data(), S);
+    subset.alloc(std::max<size_t>(subset_atoms.size(), 1));
+    if (!subset_atoms.empty()) {
+        subset.upload(subset_atoms.data(), subset_atoms.size());
+    }
+    rebuild.alloc(S);
+    status.alloc(S);
+    tile_count.alloc(S);
+    tile_prefix.alloc(S + 1);
+    tile_row.alloc(total_tiles);
+    tile_atoms.alloc(static_cast<size_t>(total_tiles) * TILE);
+    perm.alloc(total_slots);
+    keys.alloc(total_slots);
+    pos.alloc(total_slots);
+    par.alloc(total_slots);
+    bb_ctr.alloc(total_blocks);
+    bb_ext.alloc(total_blocks);
+    nblist_x.alloc(static_cast<size_t>(total_atoms_end) * 3);
+    nblist_box.alloc(static_cast<size_t>(S) * 9);
+    status.zero();
+    tile_count.zero();
+    tile_prefix.zero();
+    nblist_x.zero();
+    nblist_box.zero();
+    launch_fill_i32(rebuild.ptr, S, 1, 0);
+    TM_HIP_CHECK(hipDeviceSynchronize());
+}
+
+void NbState::invalidate(hipStream_t stream) { launch_fill_i32(rebuild.ptr, S, 1, stream); }
+
+void NbState::build_only(const double *x, const double *boxes, hipStream_t stream) {
+    const int gx_atoms = ceil_div(std::max(max_atoms, 9), 256);
+    k_check_rebuild<<<dim3(gx_atoms, S), 256, 0, stream>>>(sys.ptr, subset.ptr, x, nblist_x.ptr, boxes,
+                                                          nblist_box.ptr, rebuild.ptr, status.ptr);
+    TM_LAUNCH_CHECK();
+    k_sort<<<S, SORT_THREADS, 0, stream>>>(sys.ptr, subset.ptr, x, boxes, rebuild.ptr, keys.ptr, perm.ptr, nblist_x.ptr,
+                                          nblist_box.ptr, tile_count.ptr);
+    TM_LAUNCH_CHECK();
+    k_gather<<<dim3(ceil_div(max_slots, 256), S), 256, 0, stream>>>(sys.ptr, x, nullptr, nullptr, nullptr, nullptr,
+                                                                    0.0, perm.ptr, pos.ptr, par.ptr);
+    TM_LAUNCH_CHECK();
+    k_block_bounds<<<dim3(ceil_div(max_blocks, 4), S), 256, 0, stream>>>(sys.ptr, boxes, rebuild.ptr, pos.ptr,
+                                                                         bb_ctr.ptr, bb_ext.ptr);
+    TM_LAUNCH_CHECK();
+    k_find_tiles<<<dim3(ceil_div(max_blocks, 4), S), 256, 0, stream>>>(
+        sys.ptr, boxes, rebuild.ptr, pos.ptr, bb_ctr.ptr, bb_ext.ptr, tile_count.ptr, tile_row.ptr, tile_atoms.ptr,
+        status.ptr);
+    TM_LAUNCH_CHECK();
+    k_tile_prefix<<<1, 256, 0, stream>>>(sys.ptr, S, tile_count.ptr, tile_prefix.ptr, rebuild.ptr);
+    TM_LAUNCH_CHECK();
+}
+
+void NbState::prepare(
+    const double *x,
+    const double *params,
+    const double *boxes,
+    const int *plane,
+    const int *offset,
+    const double *lambdas,
+    double lambda0,
+    hipStream_t stream) {
+    const int gx_atoms = ceil_div(std::max(max_atoms, 9), 256);
+    k_check_rebuild<<<dim3(gx_atoms, S), 256, 0, stream>>>(sys.ptr, subset.ptr, x, nblist_x.ptr, boxes,
+                                                          nblist_box.ptr, rebuild.ptr, status.ptr);
+    TM_LAUNCH_CHECK();
+    k_sort<<<S, SORT_THREADS, 0, stream>>>(sys.ptr, subset.ptr, x, boxes, rebuild.ptr, keys.ptr, perm.ptr, nblist_x.ptr,
+                                          nblist_box.ptr, tile_count.ptr);
+    TM_LAUNCH_CHECK();
+    k_gather<<<dim3(ceil_div(max_slots, 256), S), 256, 0, stream>>>(sys.ptr, x, params, plane, offset, lambdas,
+                                                                    lambda0, perm.ptr, pos.ptr, par.ptr);
+    TM_LAUNCH_CHECK();
+    k_block_bounds<<<dim3(ceil_div(max_blocks, 4), S), 256, 0, stream>>>(sys.ptr, boxes, rebuild.ptr, pos.ptr,
+                                                                         bb_ctr.ptr, bb_ext.ptr);
+    TM_LAUNCH_CHECK();
+    k_find_tiles<<<dim3(ceil_div(max_blocks, 4), S), 256, 0, stream>>>(
+        sys.ptr, boxes, rebuild.ptr, pos.ptr, bb_ctr.ptr, bb_ext.ptr, tile_count.ptr, tile_row.ptr, tile_atoms.ptr,
+        status.ptr);
+    TM_LAUNCH_CHECK();
+    k_tile_prefix<<<1, 256, 0, stream>>>(sys.ptr, S, tile_count.ptr, tile_prefix.ptr, rebuild.ptr);
+    TM_LAUNCH_CHECK();
+}
+
+} // namespace tmamd

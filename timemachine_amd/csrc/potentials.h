@@ -1,0 +1,193 @@
+// Host-side potential classes: the Potential / BoundPotential hierarchy of
+// the reference (timemachine/cpp/src/potential.hpp:8-66,
+// bound_potential.hpp:12-58, summed_potential.hpp, fanout_summed_potential.hpp)
+// rebuilt over the kernels of this library. The contract is the reference's:
+// execute_device accumulates into caller-zeroed buffers; a null output
+// pointer means "not requested"; du_dl and u are [N] partial buffers that the
+// caller sums with wrapping uint64 addition.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "bonded.h"
+#include "nb_state.h"
+#include "tm_common.h"
+
+namespace tmamd {
+
+enum class PotKind { NonbondedAllPairs, NonbondedPairList, HarmonicBond, HarmonicAngle, PeriodicTorsion, Summed, Fanout };
+
+class Potential {
+  public:
+    virtual ~Potential() = default;
+    virtual PotKind kind() const = 0;
+
+    virtual void execute_device(
+        int N,
+        int P,
+        const double *d_x,
+        const double *d_p,
+        const double *d_box,
+        double lambda,
+        uint64_t *d_du_dx,
+        uint64_t *d_du_dp,
+        uint64_t *d_du_dl,
+        uint64_t *d_u,
+        hipStream_t stream) = 0;
+
+    // Host-side validation possible before launching with host inputs.
+    virtual void validate_host(int N, int P, const double *h_box) const {}
+    // Reads device status flags (synchronises the stream) and raises the
+    // reference's error message for a bad box.
+    virtual void check_device_status(hipStream_t stream) {}
+
+    virtual void du_dp_fixed_to_float(int N, int P, const uint64_t *du_dp, double *out) const;
+
+    void execute_host(
+        int N,
+        int P,
+        const double *h_x,
+        const double *h_p,
+        const double *h_box,
+        double lambda,
+        uint64_t *h_du_dx,
+        uint64_t *h_du_dp,
+        uint64_t *h_du_dl,
+        uint64_t *h_u);
+
+    void execute_batch_host(
+        int coord_batch_size,
+        int N,
+        int param_batch_size,
+        int P,
+        int lambda_batch_size,
+        const double *h_x,
+        const double *h_p,
+        const double *h_box,
+        const double *h_lambdas,
+        uint64_t *h_du_dx,
+        uint64_t *h_du_dp,
+        uint64_t *h_du_dl,
+        uint64_t *h_u);
+};
+
+using PotentialPtr = std::shared_ptr<Potential>;
+
+class NonbondedAllPairs : public Potential {
+  public:
+    NonbondedAllPairs(const std::vector<int> &lambda_plane_idxs, const std::vector<int> &lambda_offset_idxs, double beta,
+                      double cutoff, const std::vector<int> *atom_idxs);
+    PotKind kind() const override { return PotKind::NonbondedAllPairs; }
+    void execute_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
+                        uint64_t *d_du_dx, uint64_t *d_du_dp, uint64_t *d_du_dl, uint64_t *d_u,
+                        hipStream_t stream) override;
+    void validate_host(int N, int P, const double *h_box) const override;
+    void check_device_status(hipStream_t stream) override;
+    void du_dp_fixed_to_float(int N, int P, const uint64_t *du_dp, double *out) const override;
+
+    void set_nblist_padding(double padding);
+    void disable_hilbert_sort();
+    // Number of tiles of the current neighborlist (synchronises).
+    int num_tiles(hipStream_t stream);
+
+    int N_, K_;
+    double beta_, cutoff_, padding_ = 0.1;
+    bool sorted_ = true;
+    std::vector<int> plane_, offset_, subset_;
+    NbState st_;
+
+  private:
+    void ensure_state();
+    bool dirty_ = true;
+    DeviceArray<int> d_plane_, d_offset_;
+};
+
+class NonbondedPairList : public Potential {
+  public:
+    NonbondedPairList(const std::vector<int> &pair_idxs, const std::vector<double> &scales,
+                      const std::vector<int> &lambda_plane_idxs, const std::vector<int> &lambda_offset_idxs,
+                      double beta, double cutoff, bool negated);
+    PotKind kind() const override { return PotKind::NonbondedPairList; }
+    void execute_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
+                        uint64_t *d_du_dx, uint64_t *d_du_dp, uint64_t *d_du_dl, uint64_t *d_u,
+                        hipStream_t stream) override;
+    void du_dp_fixed_to_float(int N, int P, const uint64_t *du_dp, double *out) const override;
+
+    int N_, M_;
+    double beta_, cutoff_;
+    bool negated_;
+    std::vector<int> pairs_, plane_, offset_;
+    std::vector<double> scales_;
+
+  private:
+    DeviceArray<int> d_pairs_, d_plane_, d_offset_;
+    DeviceArray<double> d_scales_;
+};
+
+class BondedPotential : public Potential {
+  public:
+    BondedPotential(BondedKind bk, const std::vector<int> &idxs, const std::vector<int> *mult,
+                    const std::vector<int> *offset);
+    PotKind kind() const override;
+    void execute_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
+                        uint64_t *d_du_dx, uint64_t *d_du_dp, uint64_t *d_du_dl, uint64_t *d_u,
+                        hipStream_t stream) override;
+
+    BondedKind bk_;
+    int n_;
+    std::vector<int> idxs_, mult_, offset_;
+
+  private:
+    DeviceArray<int> d_idxs_, d_mult_, d_offset_;
+};
+
+class SummedPotential : public Potential {
+  public:
+    SummedPotential(const std::vector<PotentialPtr> &pots, const std::vector<int> &sizes);
+    PotKind kind() const override { return PotKind::Summed; }
+    void execute_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
+                        uint64_t *d_du_dx, uint64_t *d_du_dp, uint64_t *d_du_dl, uint64_t *d_u,
+                        hipStream_t stream) override;
+    void validate_host(int N, int P, const double *h_box) const override;
+    void check_device_status(hipStream_t stream) override;
+    void du_dp_fixed_to_float(int N, int P, const uint64_t *du_dp, double *out) const override;
+    std::vector<PotentialPtr> pots_;
+    std::vector<int> sizes_;
+    int P_;
+};
+
+class FanoutSummedPotential : public Potential {
+  public:
+    explicit FanoutSummedPotential(const std::vector<PotentialPtr> &pots);
+    PotKind kind() const override { return PotKind::Fanout; }
+    void execute_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
+                        uint64_t *d_du_dx, uint64_t *d_du_dp, uint64_t *d_du_dl, uint64_t *d_u,
+                        hipStream_t stream) override;
+    void validate_host(int N, int P, const double *h_box) const override;
+    void check_device_status(hipStream_t stream) override;
+    void du_dp_fixed_to_float(int N, int P, const uint64_t *du_dp, double *out) const override;
+    std::vector<PotentialPtr> pots_;
+};
+
+class BoundPotential {
+  public:
+    BoundPotential(PotentialPtr p, const std::vector<int> &shape, const double *h_params);
+    int size() const;
+    void execute_device(int N, const double *d_x, const double *d_box, double lambda, uint64_t *d_du_dx,
+                        uint64_t *d_du_dp, uint64_t *d_du_dl, uint64_t *d_u, hipStream_t stream);
+    void execute_host(int N, const double *h_x, const double *h_box, double lambda, uint64_t *h_du_dx,
+                      uint64_t *h_du_dl, uint64_t *h_u);
+    PotentialPtr potential;
+    std::vector<int> shape;
+    std::vector<double> h_params;
+    DeviceArray<double> d_params;
+};
+
+using BoundPotentialPtr = std::shared_ptr<BoundPotential>;
+
+// Reference validation messages for a box (nonbonded_all_pairs.cu:315-329).
+void validate_nonbonded_box(const double *h_box, double cutoff, double padding);
+
+} // namespace tmamd

@@ -1,0 +1,529 @@
+// Host-side potential classes (see potentials.h).
+#include <cstring>
+#include <numeric>
+#include <set>
+
+#include "fixed_point.h"
+#include "potentials.h"
+
+namespace tmamd {
+
+void Potential::du_dp_fixed_to_float(int N, int P, const uint64_t *du_dp, double *out) const {
+    for (int i = 0; i < P; i++) {
+        out[i] = fixed_to_double(du_dp[i], FIXED_SCALE);
+    }
+}
+
+void Potential::execute_host(int N, int P, const double *h_x, const double *h_p, const double *h_box, double lambda,
+                             uint64_t *h_du_dx, uint64_t *h_du_dp, uint64_t *h_du_dl, uint64_t *h_u) {
+    validate_host(N, P, h_box);
+    DeviceArray<double> x(static_cast<size_t>(N) * 3), box(9), p(std::max(P, 1));
+    x.upload(h_x, static_cast<size_t>(N) * 3);
+    box.upload(h_box, 9);
+    if (P > 0) {
+        p.upload(h_p, P);
+    }
+    DeviceArray<uint64_t> du_dx, du_dp, du_dl, u;
+    if (h_du_dx) {
+        du_dx.alloc(static_cast<size_t>(N) * 3);
+        du_dx.zero();
+    }
+    if (h_du_dp) {
+        du_dp.alloc(std::max(P, 1));
+        du_dp.zero();
+    }
+    if (h_du_dl) {
+        du_dl.alloc(std::max(N, 1));
+        du_dl.zero();
+    }
+    if (h_u) {
+        u.alloc(std::max(N, 1));
+        u.zero();
+    }
+    execute_device(N, P, x.ptr, P > 0 ? p.ptr : nullptr, box.ptr, lambda, du_dx.ptr, du_dp.ptr, du_dl.ptr, u.ptr, 0);
+    check_device_status(0);
+    if (h_du_dx) {
+        du_dx.download(h_du_dx, static_cast<size_t>(N) * 3);
+    }
+    if (h_du_dp) {
+        du_dp.download(h_du_dp, P);
+    }
+    if (h_du_dl) {
+        du_dl.download(h_du_dl, N);
+    }
+    if (h_u) {
+        u.download(h_u, N);
+    }
+    TM_HIP_CHECK(hipStreamSynchronize(0));
+}
+
+void Potential::execute_batch_host(int coord_batch_size, int N, int param_batch_size, int P, int lambda_batch_size,
+                                   const double *h_x, const double *h_p, const double *h_box, const double *h_lambdas,
+                                   uint64_t *h_du_dx, uint64_t *h_du_dp, uint64_t *h_du_dl, uint64_t *h_u) {
+    for (int i = 0; i < coord_batch_size; i++) {
+        validate_host(N, P, h_box + i * 9);
+    }
+    const size_t total = static_cast<size_t>(coord_batch_size) * param_batch_size * lambda_batch_size;
+    DeviceArray<double> x(static_cast<size_t>(coord_batch_size) * N * 3), box(static_cast<size_t>(coord_batch_size) * 9);
+    DeviceArray<double> p(std::max<size_t>(static_cast<size_t>(param_batch_size) * P, 1));
+    x.upload(h_x, static_cast<size_t>(coord_batch_size) * N * 3);
+    box.upload(h_box, static_cast<size_t>(coord_batch_size) * 9);
+    if (P > 0) {
+        p.upload(h_p, static_cast<size_t>(param_batch_size) * P);
+    }
+    DeviceArray<uint64_t> du_dx, du_dp, du_dl, u;
+    if (h_du_dx) {
+        du_dx.alloc(total * N * 3);
+        du_dx.zero();
+    }
+    if (h_du_dp) {
+        du_dp.alloc(std::max<size_t>(total * P, 1));
+        du_dp.zero();
+    }
+    if (h_du_dl) {
+        du_dl.alloc(total * N);
+        du_dl.zero();
+    }
+    if (h_u) {
+        u.alloc(total * N);
+        u.zero();
+    }
+    for (int i = 0; i < coord_batch_size; i++) {
+        for (int j = 0; j < param_batch_size; j++) {
+            for (int k = 0; k < lambda_batch_size; k++) {
+                const size_t o = (static_cast<size_t>(i) * param_batch_size + j) * lambda_batch_size + k;
+                execute_device(N, P, x.ptr + static_cast<size_t>(i) * N * 3,
+                               P > 0 ? p.ptr + static_cast<size_t>(j) * P : nullptr, box.ptr + i * 9, h_lambdas[k],
+                               h_du_dx ? du_dx.ptr + o * N * 3 : nullptr, h_du_dp ? du_dp.ptr + o * P : nullptr,
+                               h_du_dl ? du_dl.ptr + o * N : nullptr, h_u ? u.ptr + o * N : nullptr, 0);
+            }
+        }
+    }
+    check_device_status(0);
+    if (h_du_dx) {
+        du_dx.download(h_du_dx, total * N * 3);
+    }
+    if (h_du_dp) {
+        du_dp.download(h_du_dp, total * P);
+    }
+    if (h_du_dl) {
+        du_dl.download(h_du_dl, total * N);
+    }
+    if (h_u) {
+        u.download(h_u, total * N);
+    }
+    TM_HIP_CHECK(hipStreamSynchronize(0));
+}
+
+void validate_nonbonded_box(const double *h_box, double cutoff, double padding) {
+    const double db_cutoff = (cutoff + padding) * 2;
+    for (int i = 0; i < 9; i++) {
+        if (i == 0 || i == 4 || i == 8) {
+            if (h_box[i] < db_cutoff) {
+                throw Error("Cutoff with padding is more than half of the box width, neighborlist is no longer reliable");
+            }
+        } else if (h_box[i] != 0.0) {
+            throw Error("Provided non-ortholinear box, unable to compute nonbonded energy");
+        }
+    }
+}
+
+// ---------------------------------------------------------------- all pairs
+
+NonbondedAllPairs::NonbondedAllPairs(const std::vector<int> &plane, const std::vector<int> &offset, double beta,
+                                     double cutoff, const std::vector<int> *atom_idxs)
+    : N_(static_cast<int>(offset.size())), K_(atom_idxs ? static_cast<int>(atom_idxs->size()) : N_), beta_(beta),
+      cutoff_(cutoff), plane_(plane), offset_(offset) {
+    if (plane.size() != offset.size()) {
+        throw Error("lambda offset idxs and plane idxs need to be equivalent");
+    }
+    if (atom_idxs) {
+        std::set<int> uniq(atom_idxs->begin(), atom_idxs->end());
+        if (uniq.size() != atom_idxs->size()) {
+            throw Error("atom indices must be unique");
+        }
+        for (int a : *atom_idxs) {
+            if (a < 0 || a >= N_) {
+                throw Error("atom indices must be in [0, N)");
+            }
+        }
+        subset_ = *atom_idxs;
+    }
+    d_plane_.alloc(std::max(N_, 1));
+    d_offset_.alloc(std::max(N_, 1));
+    d_plane_.upload(plane.data(), N_);
+    d_offset_.upload(offset.data(), N_);
+    TM_HIP_CHECK(hipDeviceSynchronize());
+}
+
+void NonbondedAllPairs::set_nblist_padding(double padding) {
+    padding_ = padding;
+    dirty_ = true;
+}
+
+void NonbondedAllPairs::disable_hilbert_sort() {
+    sorted_ = false;
+    dirty_ = true;
+}
+
+void NonbondedAllPairs::ensure_state() {
+    if (!dirty_) {
+        return;
+    }
+    NbSys y{};
+    y.atom0 = 0;
+    y.n_atoms = N_;
+    y.k = K_;
+    y.red_idx = 0;
+    y.subset0 = subset_.empty() ? -1 : 0;
+    y.sorted = sorted_ ? 1 : 0;
+    y.beta = static_cast<float>(beta_);
+    y.cutoff_d = cutoff_;
+    y.padding_d = padding_;
+    st_.init({y}, subset_);
+    dirty_ = false;
+}
+
+void NonbondedAllPairs::validate_host(int N, int P, const double *h_box) const {
+    if (N != N_) {
+        throw Error("NonbondedAllPairs::execute_device(): expected N == N_, got N=" + std::to_string(N) +
+                    ", N_=" + std::to_string(N_));
+    }
+    if (P != N_ * 3) {
+        throw Error("NonbondedAllPairs::execute_device(): expected P == M*N_*3, got P=" + std::to_string(P) +
+                    ", M*N_*3=" + std::to_string(N_ * 3));
+    }
+    validate_nonbonded_box(h_box, cutoff_, padding_);
+}
+
+void NonbondedAllPairs::execute_device(int N, int P, const double *d_x, const double *d_p, const double *d_box,
+                                       double lambda, uint64_t *d_du_dx, uint64_t *d_du_dp, uint64_t *d_du_dl,
+                                       uint64_t *d_u, hipStream_t stream) {
+    if (N != N_) {
+        throw Error("NonbondedAllPairs::execute_device(): expected N == N_, got N=" + std::to_string(N) +
+                    ", N_=" + std::to_string(N_));
+    }
+    if (P != N_ * 3) {
+        throw Error("NonbondedAllPairs::execute_device(): expected P == M*N_*3, got P=" + std::to_string(P) +
+                    ", M*N_*3=" + std::to_string(N_ * 3));
+    }
+    ensure_state();
+    if (K_ == 0) {
+        return;
+    }
+    st_.prepare(d_x, d_p, d_box, d_plane_.ptr, d_offset_.ptr, nullptr, lambda, stream);
+    st_.compute(d_box, d_du_dx, d_du_dp, d_du_dl, d_u, stream);
+}
+
+void NonbondedAllPairs::check_device_status(hipStream_t stream) {
+    if (dirty_ || K_ == 0) {
+        return;
+    }
+    int status = 0;
+    TM_HIP_CHECK(hipMemcpyAsync(&status, st_.status.ptr, sizeof(int), hipMemcpyDeviceToHost, stream));
+    TM_HIP_CHECK(hipStreamSynchronize(stream));
+    if (status) {
+        TM_HIP_CHECK(hipMemsetAsync(st_.status.ptr, 0, sizeof(int), stream));
+        TM_HIP_CHECK(hipStreamSynchronize(stream));
+        if (status & NB_STATUS_BOX_TOO_SMALL) {
+            throw Error("Cutoff with padding is more than half of the box width, neighborlist is no longer reliable");
+        }
+        if (status & NB_STATUS_BOX_NOT_ORTHO) {
+            throw Error("Provided non-ortholinear box, unable to compute nonbonded energy");
+        }
+        if (status & NB_STATUS_TILE_OVERFLOW) {
+            throw Error("NonbondedAllPairs: neighborlist tile buffer overflow");
+        }
+    }
+}
+
+int NonbondedAllPairs::num_tiles(hipStream_t stream) {
+    ensure_state();
+    int n = 0;
+    TM_HIP_CHECK(hipMemcpyAsync(&n, st_.tile_prefix.ptr + 1, sizeof(int), hipMemcpyDeviceToHost, stream));
+    TM_HIP_CHECK(hipStreamSynchronize(stream));
+    return n;
+}
+
+static void nonbonded_du_dp_to_float(int P, const uint64_t *du_dp, double *out) {
+    for (int i = 0; i + 2 < P; i += 3) {
+        out[i + 0] = fixed_to_double(du_dp[i + 0], FIXED_SCALE);
+        out[i + 1] = fixed_to_double(du_dp[i + 1], FIXED_SCALE_DSIG);
+        out[i + 2] = fixed_to_double(du_dp[i + 2], FIXED_SCALE_DEPS);
+    }
+}
+
+void NonbondedAllPairs::du_dp_fixed_to_float(int N, int P, const uint64_t *du_dp, double *out) const {
+    nonbonded_du_dp_to_float(P, du_dp, out);
+}
+
+// ---------------------------------------------------------------- pair list
+
+NonbondedPairList::NonbondedPairList(const std::vector<int> &pair_idxs, const std::vector<double> &scales,
+                                     const std::vector<int> &plane, const std::vector<int> &offset, double beta,
+                                     double cutoff, bool negated)
+    : N_(static_cast<int>(offset.size())), M_(static_cast<int>(pair_idxs.size() / 2)), beta_(beta), cutoff_(cutoff),
+      negated_(negated), pairs_(pair_idxs), plane_(plane), offset_(offset), scales_(scales) {
+    if (pair_idxs.size() % 2 != 0) {
+        throw Error("pair_idxs.size() must be even, but got " + std::to_string(pair_idxs.size()));
+    }
+    for (int i = 0; i < M_; i++) {
+        const int src = pair_idxs[i * 2 + 0];
+        const int dst = pair_idxs[i * 2 + 1];
+        if (src == dst) {
+            throw Error("illegal pair with src == dst: " + std::to_string(src) + ", " + std::to_string(dst));
+        }
+    }
+    if (static_cast<int>(scales.size() / 2) != M_) {
+        throw Error("expected same number of pairs and scale tuples, but got " + std::to_string(M_) +
+                    " != " + std::to_string(scales.size() / 2));
+    }
+    if (plane.size() != offset.size()) {
+        throw Error("lambda offset idxs and plane idxs need to be equivalent");
+    }
+    d_pairs_.alloc(std::max(M_ * 2, 1));
+    d_scales_.alloc(std::max(M_ * 2, 1));
+    d_plane_.alloc(std::max(N_, 1));
+    d_offset_.alloc(std::max(N_, 1));
+    d_pairs_.upload(pair_idxs.data(), M_ * 2);
+    d_scales_.upload(scales.data(), M_ * 2);
+    d_plane_.upload(plane.data(), N_);
+    d_offset_.upload(offset.data(), N_);
+    TM_HIP_CHECK(hipDeviceSynchronize());
+}
+
+void NonbondedPairList::execute_device(int N, int P, const double *d_x, const double *d_p, const double *d_box,
+                                       double lambda, uint64_t *d_du_dx, uint64_t *d_du_dp, uint64_t *d_du_dl,
+                                       uint64_t *d_u, hipStream_t stream) {
+    if (N != N_) {
+        throw Error("NonbondedPairList::execute_device(): expected N == N_, got N=" + std::to_string(N) +
+                    ", N_=" + std::to_string(N_));
+    }
+    if (P != N_ * 3) {
+        throw Error("NonbondedPairList::execute_device(): expected P == M*N_*3, got P=" + std::to_string(P) +
+                    ", M*N_*3=" + std::to_string(N_ * 3));
+    }
+    PairListArgs a{};
+    a.M = M_;
+    a.pairs = d_pairs_.ptr;
+    a.scales = d_scales_.ptr;
+    a.pair_sys = nullptr;
+    a.red_idx = nullptr;
+    a.x = d_x;
+    a.params = d_p;
+    a.boxes = d_box;
+    a.plane = d_plane_.ptr;
+    a.offset = d_offset_.ptr;
+    a.lambdas = nullptr;
+    a.lambda0 = lambda;
+    a.cutoffs = nullptr;
+    a.cutoff0 = cutoff_;
+    a.betas = nullptr;
+    a.beta0 = static_cast<float>(beta_);
+    a.negated = negated_ ? 1 : 0;
+    launch_pair_list(a, d_du_dx, d_du_dp, d_du_dl, d_u, stream);
+}
+
+void NonbondedPairList::du_dp_fixed_to_float(int N, int P, const uint64_t *du_dp, double *out) const {
+    nonbonded_du_dp_to_float(P, du_dp, out);
+}
+
+// ---------------------------------------------------------------- bonded
+
+static const char *bonded_name(BondedKind k) {
+    return k == BondedKind::Bond ? "HarmonicBond" : (k == BondedKind::Angle ? "HarmonicAngle" : "PeriodicTorsion");
+}
+
+BondedPotential::BondedPotential(BondedKind bk, const std::vector<int> &idxs, const std::vector<int> *mult,
+                                 const std::vector<int> *offset)
+    : bk_(bk), idxs_(idxs) {
+    const int ar = bonded_arity(bk);
+    if (idxs.size() % ar != 0) {
+        if (bk == BondedKind::Bond) {
+            throw Error("bond_idxs.size() must be exactly 2*k!");
+        } else if (bk == BondedKind::Angle) {
+            throw Error("angle_idxs.size() must be exactly 3*A");
+        } else {
+            throw Error("torsion_idxs.size() must be exactly 4*k");
+        }
+    }
+    n_ = static_cast<int>(idxs.size()) / ar;
+    if (mult && !mult->empty() && static_cast<int>(mult->size()) != n_) {
+        throw Error("bad lambda_mult size()");
+    }
+    if (offset && !offset->empty() && static_cast<int>(offset->size()) != n_) {
+        throw Error("bad lambda_offset size()");
+    }
+    for (int t = 0; t < n_; t++) {
+        const int *q = idxs.data() + t * ar;
+        bool dup = false;
+        for (int a = 0; a < ar; a++) {
+            for (int b = a + 1; b < ar; b++) {
+                dup = dup || q[a] == q[b];
+            }
+        }
+        if (dup) {
+            if (bk == BondedKind::Bond) {
+                throw Error("src == dst");
+            } else if (bk == BondedKind::Angle) {
+                throw Error("angle triplets must be unique");
+            } else {
+                throw Error("torsion quads must be unique");
+            }
+        }
+    }
+    mult_ = (mult && !mult->empty()) ? *mult : std::vector<int>(n_, 0);
+    offset_ = (offset && !offset->empty()) ? *offset : std::vector<int>(n_, 1);
+    d_idxs_.alloc(std::max<size_t>(idxs.size(), 1));
+    d_mult_.alloc(std::max(n_, 1));
+    d_offset_.alloc(std::max(n_, 1));
+    d_idxs_.upload(idxs.data(), idxs.size());
+    d_mult_.upload(mult_.data(), n_);
+    d_offset_.upload(offset_.data(), n_);
+    TM_HIP_CHECK(hipDeviceSynchronize());
+}
+
+PotKind BondedPotential::kind() const {
+    return bk_ == BondedKind::Bond ? PotKind::HarmonicBond
+                                   : (bk_ == BondedKind::Angle ? PotKind::HarmonicAngle : PotKind::PeriodicTorsion);
+}
+
+void BondedPotential::execute_device(int N, int P, const double *d_x, const double *d_p, const double *d_box,
+                                     double lambda, uint64_t *d_du_dx, uint64_t *d_du_dp, uint64_t *d_du_dl,
+                                     uint64_t *d_u, hipStream_t stream) {
+    if (n_ == 0) {
+        return;
+    }
+    const int np = bonded_nparams(bk_);
+    if (P != np * n_) {
+        const char *nm = bonded_name(bk_);
+        const char *sym = bk_ == BondedKind::Bond ? "B_" : (bk_ == BondedKind::Angle ? "A_" : "T_");
+        throw Error(std::string(nm) + "::execute_device(): expected P == " + std::to_string(np) + "*" + sym +
+                    ", got P=" + std::to_string(P) + ", " + std::to_string(np) + "*" + sym + "=" +
+                    std::to_string(np * n_));
+    }
+    BondedTerms t{n_, d_idxs_.ptr, d_p, d_mult_.ptr, d_offset_.ptr, nullptr};
+    launch_bonded(bk_, t, d_x, nullptr, lambda, nullptr, d_du_dx, d_du_dp, d_du_dl, d_u, stream);
+}
+
+// ---------------------------------------------------------------- sums
+
+SummedPotential::SummedPotential(const std::vector<PotentialPtr> &pots, const std::vector<int> &sizes)
+    : pots_(pots), sizes_(sizes), P_(std::accumulate(sizes.begin(), sizes.end(), 0)) {
+    if (pots.size() != sizes.size()) {
+        throw Error("number of potentials != number of parameter sizes");
+    }
+}
+
+void SummedPotential::execute_device(int N, int P, const double *d_x, const double *d_p, const double *d_box,
+                                     double lambda, uint64_t *d_du_dx, uint64_t *d_du_dp, uint64_t *d_du_dl,
+                                     uint64_t *d_u, hipStream_t stream) {
+    if (P != P_) {
+        throw Error("SummedPotential::execute_device(): expected " + std::to_string(P_) + " parameters, got " +
+                    std::to_string(P));
+    }
+    int off = 0;
+    for (size_t i = 0; i < pots_.size(); i++) {
+        pots_[i]->execute_device(N, sizes_[i], d_x, d_p + off, d_box, lambda, d_du_dx,
+                                 d_du_dp ? d_du_dp + off : nullptr, d_du_dl, d_u, stream);
+        off += sizes_[i];
+    }
+}
+
+void SummedPotential::validate_host(int N, int P, const double *h_box) const {
+    for (size_t i = 0; i < pots_.size(); i++) {
+        pots_[i]->validate_host(N, sizes_[i], h_box);
+    }
+}
+
+void SummedPotential::check_device_status(hipStream_t stream) {
+    for (auto &p : pots_) {
+        p->check_device_status(stream);
+    }
+}
+
+void SummedPotential::du_dp_fixed_to_float(int N, int P, const uint64_t *du_dp, double *out) const {
+    int off = 0;
+    for (size_t i = 0; i < pots_.size(); i++) {
+        pots_[i]->du_dp_fixed_to_float(N, sizes_[i], du_dp + off, out + off);
+        off += sizes_[i];
+    }
+}
+
+FanoutSummedPotential::FanoutSummedPotential(const std::vector<PotentialPtr> &pots) : pots_(pots) {}
+
+void FanoutSummedPotential::execute_device(int N, int P, const double *d_x, const double *d_p, const double *d_box,
+                                           double lambda, uint64_t *d_du_dx, uint64_t *d_du_dp, uint64_t *d_du_dl,
+                                           uint64_t *d_u, hipStream_t stream) {
+    for (auto &p : pots_) {
+        p->execute_device(N, P, d_x, d_p, d_box, lambda, d_du_dx, d_du_dp, d_du_dl, d_u, stream);
+    }
+}
+
+void FanoutSummedPotential::validate_host(int N, int P, const double *h_box) const {
+    for (auto &p : pots_) {
+        p->validate_host(N, P, h_box);
+    }
+}
+
+void FanoutSummedPotential::check_device_status(hipStream_t stream) {
+    for (auto &p : pots_) {
+        p->check_device_status(stream);
+    }
+}
+
+void FanoutSummedPotential::du_dp_fixed_to_float(int N, int P, const uint64_t *du_dp, double *out) const {
+    if (pots_.empty()) {
+        Potential::du_dp_fixed_to_float(N, P, du_dp, out);
+    } else {
+        pots_[0]->du_dp_fixed_to_float(N, P, du_dp, out);
+    }
+}
+
+// ---------------------------------------------------------------- bound
+
+BoundPotential::BoundPotential(PotentialPtr p, const std::vector<int> &shp, const double *h_p)
+    : potential(std::move(p)), shape(shp) {
+    const int n = size();
+    h_params.assign(h_p, h_p + n);
+    d_params.alloc(std::max(n, 1));
+    d_params.upload(h_params.data(), n);
+    TM_HIP_CHECK(hipDeviceSynchronize());
+}
+
+int BoundPotential::size() const {
+    if (shape.empty()) {
+        return 0;
+    }
+    int total = 1;
+    for (int s : shape) {
+        total *= s;
+    }
+    return total;
+}
+
+void BoundPotential::execute_device(int N, const double *d_x, const double *d_box, double lambda, uint64_t *d_du_dx,
+                                    uint64_t *d_du_dp, uint64_t *d_du_dl, uint64_t *d_u, hipStream_t stream) {
+    potential->execute_device(N, size(), d_x, size() > 0 ? d_params.ptr : nullptr, d_box, lambda, d_du_dx, d_du_dp,
+                              d_du_dl, d_u, stream);
+}
+
+void BoundPotential::execute_host(int N, const double *h_x, const double *h_box, double lambda, uint64_t *h_du_dx,
+                                  uint64_t *h_du_dl, uint64_t *h_u) {
+    potential->validate_host(N, size(), h_box);
+    DeviceArray<double> x(static_cast<size_t>(N) * 3), box(9);
+    x.upload(h_x, static_cast<size_t>(N) * 3);
+    box.upload(h_box, 9);
+    DeviceArray<uint64_t> du_dx(static_cast<size_t>(N) * 3), du_dl(std::max(N, 1)), u(std::max(N, 1));
+    du_dx.zero();
+    du_dl.zero();
+    u.zero();
+    execute_device(N, x.ptr, box.ptr, lambda, du_dx.ptr, nullptr, du_dl.ptr, u.ptr, 0);
+    potential->check_device_status(0);
+    du_dx.download(h_du_dx, static_cast<size_t>(N) * 3);
+    du_dl.download(h_du_dl, N);
+    u.download(h_u, N);
+    TM_HIP_CHECK(hipStreamSynchronize(0));
+}
+
+} // namespace tmamd

@@ -1,0 +1,775 @@
+"""Drop-in replacement for the reference's ``timemachine.lib.custom_ops``.
+
+Class names, constructor signatures, methods, return shapes and error
+messages follow the reference's pybind11 module
+(timemachine/cpp/src/wrap_kernels.cpp, stubs in timemachine/lib/custom_ops.pyi)
+so that ``lib.potentials``-style wrappers resolving ``type(name) + "_f32"``
+keep working. Every call goes through the C ABI of ``libtm_amd.so``; there
+is no CPU path.
+
+Differences a caller can observe:
+  * arithmetic is FP32 with 64-bit fixed-point accumulation; ``*_f64`` names
+    exist but raise (the MI355X path is the FP32 one);
+  * the neighborlist works on 64-atom blocks (one wave64 per tile), so
+    ``Neighborlist_f32.get_nblist`` returns one list per 64-atom row block;
+  * Langevin and barostat noise come from a counter-based generator
+    (Philox4x32-10), so trajectories differ from curand's but are bitwise
+    reproducible, checkpointable, and identical alone or in a batch.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from timemachine_amd import _lib
+from timemachine_amd._lib import call, f64, fixed_to_float, i32, pd, pi, pu64, wrapping_sum
+
+FIXED_EXPONENT_DU_DSIG = 2.0**37
+FIXED_EXPONENT_DU_DEPS = 2.0**38
+
+
+class _Handle:
+    _release = None
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+        self._keep = []
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        rel = type(self)._release
+        if h is not None and h.value and rel is not None and _lib._lib is not None:
+            getattr(_lib._lib, rel)(h)
+            self._h = ctypes.c_void_p()
+
+
+# ---------------------------------------------------------------- potentials
+
+
+class Potential(_Handle):
+    """Base class; mirrors the reference's Potential binding (wrap_kernels.cpp:334-652)."""
+
+    _release = "tm_potential_release"
+
+    def _du_dp_to_float(self, N: int, P: int, du_dp_u64: np.ndarray, shape) -> np.ndarray:
+        out = np.empty(P, dtype=np.float64)
+        if P:
+            call("tm_potential_du_dp_fixed_to_float", self._h, N, P, pu64(du_dp_u64), pd(out))
+        return out.reshape(shape)
+
+    def execute_selective(self, coords, params, box, lam, compute_du_dx, compute_du_dp, compute_du_dl, compute_u):
+        coords = f64(coords)
+        params = f64(params)
+        box = f64(box)
+        N = coords.shape[0]
+        P = params.size
+        du_dx = np.zeros(N * 3, dtype=np.uint64) if compute_du_dx else None
+        du_dp = np.zeros(max(P, 1), dtype=np.uint64) if compute_du_dp else None
+        du_dl = np.zeros(max(N, 1), dtype=np.uint64) if compute_du_dl else None
+        u = np.zeros(max(N, 1), dtype=np.uint64) if compute_u else None
+        call(
+            "tm_potential_execute_host",
+            self._h,
+            N,
+            P,
+            pd(coords),
+            pd(params.reshape(-1)) if P else None,
+            pd(box),
+            float(lam),
+            pu64(du_dx),
+            pu64(du_dp),
+            pu64(du_dl),
+            pu64(u),
+        )
+        r_dx = fixed_to_float(du_dx).reshape(N, 3) if compute_du_dx else None
+        r_dp = self._du_dp_to_float(N, P, du_dp, params.shape) if compute_du_dp else None
+        r_dl = fixed_to_float(wrapping_sum(du_dl)) if compute_du_dl else None
+        r_u = fixed_to_float(wrapping_sum(u)) if compute_u else None
+        return r_dx, r_dp, r_dl, r_u
+
+    def execute(self, coords, params, box, lam):
+        return self.execute_selective(coords, params, box, lam, True, True, True, True)
+
+    def execute_fixed_raw(self, coords, params, box, lam):
+        """Raw 64-bit fixed-point outputs (du_dx [N*3], du_dp [P], du_dl [N], u [N]) before decoding."""
+        coords = f64(coords)
+        params = f64(params)
+        box = f64(box)
+        N = coords.shape[0]
+        P = params.size
+        out = [np.zeros(N * 3, np.uint64), np.zeros(max(P, 1), np.uint64), np.zeros(max(N, 1), np.uint64),
+               np.zeros(max(N, 1), np.uint64)]
+        call("tm_potential_execute_host", self._h, N, P, pd(coords), pd(params.reshape(-1)) if P else None, pd(box),
+             float(lam), *[pu64(a) for a in out])
+        return out
+
+    def execute_du_dx(self, coords, params, box, lam):
+        return self.execute_selective(coords, params, box, lam, True, False, False, False)[0]
+
+    def execute_selective_batch(
+        self, coords, params, boxes, lambs, compute_du_dx, compute_du_dp, compute_du_dl, compute_u
+    ):
+        coords = f64(coords)
+        params = f64(params)
+        boxes = f64(boxes)
+        lambs = f64(lambs).reshape(-1)
+        if coords.ndim != 3 and boxes.ndim != 3:
+            raise RuntimeError("coords and boxes must have 3 dimensions")
+        if coords.shape[0] != boxes.shape[0]:
+            raise RuntimeError("number of batches of coords and boxes don't match")
+        if params.ndim < 2:
+            raise RuntimeError("parameters must have at least 2 dimensions")
+        if lambs.size < 1:
+            raise RuntimeError("lambdas must have at least 1 value")
+        Bc, N = coords.shape[0], coords.shape[1]
+        Bp = params.shape[0]
+        P = params.size // Bp
+        Bl = lambs.size
+        T = Bc * Bp * Bl
+        du_dx = np.zeros(T * N * 3, dtype=np.uint64) if compute_du_dx else None
+        du_dp = np.zeros(max(T * P, 1), dtype=np.uint64) if compute_du_dp else None
+        du_dl = np.zeros(T * N, dtype=np.uint64) if compute_du_dl else None
+        u = np.zeros(T * N, dtype=np.uint64) if compute_u else None
+        call(
+            "tm_potential_execute_batch_host",
+            self._h,
+            Bc,
+            N,
+            Bp,
+            P,
+            Bl,
+            pd(coords),
+            pd(params.reshape(-1)) if P else None,
+            pd(boxes),
+            pd(lambs),
+            pu64(du_dx),
+            pu64(du_dp),
+            pu64(du_dl),
+            pu64(u),
+        )
+        out = [None, None, None, None]
+        if compute_du_dx:
+            out[0] = fixed_to_float(du_dx).reshape(Bc, Bp, Bl, N, 3)
+        if compute_du_dp:
+            flat = np.stack([self._du_dp_to_float(N, P, du_dp[i * P : (i + 1) * P], (P,)) for i in range(T)])
+            out[1] = flat.reshape((Bc, Bp, Bl) + params.shape[1:])
+        if compute_du_dl:
+            out[2] = np.array([fixed_to_float(wrapping_sum(du_dl[i * N : (i + 1) * N])) for i in range(T)]).reshape(
+                Bc, Bp, Bl
+            )
+        if compute_u:
+            out[3] = np.array([fixed_to_float(wrapping_sum(u[i * N : (i + 1) * N])) for i in range(T)]).reshape(
+                Bc, Bp, Bl
+            )
+        return tuple(out)
+
+
+_BONDED_SIZE_MSG = {
+    2: "bond_idxs.size() must be exactly 2*k!",
+    3: "angle_idxs.size() must be exactly 3*A",
+    4: "torsion_idxs.size() must be exactly 4*k",
+}
+
+
+def _bonded(create, idxs, arity, lamb_mult, lamb_offset):
+    """Validates in the reference's order (harmonic_bond.cu:15-33) and creates the potential."""
+    idxs = i32(idxs).reshape(-1)
+    if idxs.size % arity != 0:
+        raise RuntimeError(_BONDED_SIZE_MSG[arity])
+    n = idxs.size // arity
+    mult = None if lamb_mult is None else i32(lamb_mult).reshape(-1)
+    off = None if lamb_offset is None else i32(lamb_offset).reshape(-1)
+    if mult is not None and mult.size > 0 and mult.size != n:
+        raise RuntimeError("bad lambda_mult size()")
+    if off is not None and off.size > 0 and off.size != n:
+        raise RuntimeError("bad lambda_offset size()")
+    h = ctypes.c_void_p()
+    call(
+        create,
+        n,
+        pi(idxs),
+        pi(mult) if mult is not None and mult.size else None,
+        pi(off) if off is not None and off.size else None,
+        ctypes.byref(h),
+    )
+    return h, [idxs, mult, off]
+
+
+class HarmonicBond_f32(Potential):
+    def __init__(self, bond_idxs, lamb_mult=None, lamb_offset=None):
+        super().__init__()
+        self._h, self._keep = _bonded("tm_harmonic_bond_create", bond_idxs, 2, lamb_mult, lamb_offset)
+
+
+class HarmonicAngle_f32(Potential):
+    def __init__(self, angle_idxs, lamb_mult=None, lamb_offset=None):
+        super().__init__()
+        self._h, self._keep = _bonded("tm_harmonic_angle_create", angle_idxs, 3, lamb_mult, lamb_offset)
+
+
+class PeriodicTorsion_f32(Potential):
+    def __init__(self, angle_idxs, lamb_mult=None, lamb_offset=None):
+        super().__init__()
+        self._h, self._keep = _bonded("tm_periodic_torsion_create", angle_idxs, 4, lamb_mult, lamb_offset)
+
+
+_IDENTITY_TRANSFORMS = ("lambda",)
+
+
+def _check_transforms(*exprs):
+    for e in exprs:
+        if e not in _IDENTITY_TRANSFORMS:
+            raise RuntimeError(
+                f"lambda transform {e!r} is not supported: only the identity transform 'lambda' is compiled in"
+            )
+
+
+class NonbondedAllPairs_f32(Potential):
+    """NonbondedAllPairs (wrap_kernels.cpp:835-895); kernel_dir is accepted and ignored."""
+
+    def __init__(
+        self,
+        kernel_dir,
+        lambda_plane_idxs_i,
+        lambda_offset_idxs_i,
+        beta,
+        cutoff,
+        atom_idxs_i=None,
+        transform_lambda_charge="lambda",
+        transform_lambda_sigma="lambda",
+        transform_lambda_epsilon="lambda",
+        transform_lambda_w="lambda",
+    ):
+        super().__init__()
+        _check_transforms(transform_lambda_charge, transform_lambda_sigma, transform_lambda_epsilon, transform_lambda_w)
+        plane = i32(lambda_plane_idxs_i)
+        offset = i32(lambda_offset_idxs_i)
+        N = offset.size
+        if plane.size != offset.size:
+            raise RuntimeError("lambda offset idxs and plane idxs need to be equivalent")
+        atoms = None
+        n_atoms = -1
+        if atom_idxs_i is not None:
+            atoms = i32(atom_idxs_i)
+            if len(set(atoms.tolist())) != atoms.size:
+                raise RuntimeError("atom indices must be unique")
+            n_atoms = atoms.size
+        call(
+            "tm_nonbonded_all_pairs_create",
+            N,
+            pi(plane),
+            pi(offset),
+            float(beta),
+            float(cutoff),
+            pi(atoms),
+            n_atoms,
+            ctypes.byref(self._h),
+        )
+        self._keep = [plane, offset, atoms]
+
+    def set_nblist_padding(self, padding):
+        call("tm_nonbonded_all_pairs_set_nblist_padding", self._h, float(padding))
+
+    def disable_hilbert_sort(self):
+        call("tm_nonbonded_all_pairs_disable_hilbert_sort", self._h)
+
+    def num_tiles(self) -> int:
+        n = ctypes.c_int(0)
+        call("tm_nonbonded_all_pairs_num_tiles", self._h, ctypes.byref(n))
+        return n.value
+
+
+class NonbondedPairList_f32(Potential):
+    """NonbondedPairList (wrap_kernels.cpp:957-1014)."""
+
+    _negated = False
+
+    def __init__(
+        self,
+        kernel_dir,
+        pair_idxs_i,
+        scales_i,
+        lambda_plane_idxs_i,
+        lambda_offset_idxs_i,
+        beta,
+        cutoff,
+        transform_lambda_charge="lambda",
+        transform_lambda_sigma="lambda",
+        transform_lambda_epsilon="lambda",
+        transform_lambda_w="lambda",
+    ):
+        super().__init__()
+        _check_transforms(transform_lambda_charge, transform_lambda_sigma, transform_lambda_epsilon, transform_lambda_w)
+        pairs = i32(pair_idxs_i).reshape(-1)
+        scales = f64(scales_i).reshape(-1)
+        plane = i32(lambda_plane_idxs_i)
+        offset = i32(lambda_offset_idxs_i)
+        if pairs.size % 2 != 0:
+            raise RuntimeError(f"pair_idxs.size() must be even, but got {pairs.size}")
+        M = pairs.size // 2
+        if scales.size // 2 != M:
+            raise RuntimeError(f"expected same number of pairs and scale tuples, but got {M} != {scales.size // 2}")
+        call(
+            "tm_nonbonded_pair_list_create",
+            offset.size,
+            M,
+            pi(pairs),
+            pd(scales),
+            pi(plane),
+            pi(offset),
+            float(beta),
+            float(cutoff),
+            1 if self._negated else 0,
+            ctypes.byref(self._h),
+        )
+        self._keep = [pairs, scales, plane, offset]
+
+
+class NonbondedPairList_f32_negated(NonbondedPairList_f32):
+    _negated = True
+
+
+class _CompositePotential(Potential):
+    def get_potentials(self) -> List[Potential]:
+        return list(self._pots)
+
+
+class FanoutSummedPotential(_CompositePotential):
+    """FanoutSummedPotential (wrap_kernels.cpp:1052-1063)."""
+
+    def __init__(self, potentials: Sequence[Potential]):
+        super().__init__()
+        self._pots = list(potentials)
+        arr = _lib.handle_array([p.handle for p in self._pots])
+        call("tm_fanout_summed_potential_create", len(self._pots), arr, ctypes.byref(self._h))
+
+
+class SummedPotential(_CompositePotential):
+    """SummedPotential (wrap_kernels.cpp:1035-1050)."""
+
+    def __init__(self, potentials: Sequence[Potential], params_sizes: Sequence[int]):
+        super().__init__()
+        self._pots = list(potentials)
+        if len(self._pots) != len(params_sizes):
+            raise RuntimeError("number of potentials != number of parameter sizes")
+        sizes = i32(params_sizes)
+        arr = _lib.handle_array([p.handle for p in self._pots])
+        call("tm_summed_potential_create", len(self._pots), arr, pi(sizes), ctypes.byref(self._h))
+        self._keep = [sizes]
+
+
+def _f64_unavailable(name):
+    class _Unavailable:
+        def __init__(self, *args, **kwargs):
+            raise NotImplementedError(
+                f"{name}: the MI355X path computes in FP32 with 64-bit fixed-point accumulation; use the _f32 class"
+            )
+
+    _Unavailable.__name__ = name
+    return _Unavailable
+
+
+HarmonicBond_f64 = _f64_unavailable("HarmonicBond_f64")
+HarmonicAngle_f64 = _f64_unavailable("HarmonicAngle_f64")
+PeriodicTorsion_f64 = _f64_unavailable("PeriodicTorsion_f64")
+NonbondedAllPairs_f64 = _f64_unavailable("NonbondedAllPairs_f64")
+NonbondedPairList_f64 = _f64_unavailable("NonbondedPairList_f64")
+NonbondedPairList_f64_negated = _f64_unavailable("NonbondedPairList_f64_negated")
+Neighborlist_f64 = _f64_unavailable("Neighborlist_f64")
+
+
+class BoundPotential(_Handle):
+    """BoundPotential (wrap_kernels.cpp:654-723)."""
+
+    _release = "tm_bound_potential_release"
+
+    def __init__(self, potential: Potential, params):
+        super().__init__()
+        params = f64(params)
+        shape = i32(params.shape)
+        self._potential = potential
+        self._params = params
+        call(
+            "tm_bound_potential_create",
+            potential.handle,
+            shape.size,
+            pi(shape),
+            pd(params.reshape(-1)) if params.size else None,
+            ctypes.byref(self._h),
+        )
+
+    def get_potential(self) -> Potential:
+        return self._potential
+
+    def size(self) -> int:
+        n = ctypes.c_int(0)
+        call("tm_bound_potential_size", self._h, ctypes.byref(n))
+        return n.value
+
+    @property
+    def params(self) -> np.ndarray:
+        return self._params
+
+    def _execute_raw(self, coords, box, lam):
+        coords = f64(coords)
+        box = f64(box)
+        N = coords.shape[0]
+        du_dx = np.zeros(N * 3, dtype=np.uint64)
+        du_dl = np.zeros(max(N, 1), dtype=np.uint64)
+        u = np.zeros(max(N, 1), dtype=np.uint64)
+        call(
+            "tm_bound_potential_execute_host",
+            self._h,
+            N,
+            pd(coords),
+            pd(box),
+            float(lam),
+            pu64(du_dx),
+            pu64(du_dl),
+            pu64(u),
+        )
+        return du_dx, du_dl, u
+
+    def execute(self, coords, box, lam):
+        du_dx, du_dl, u = self._execute_raw(coords, box, lam)
+        N = np.asarray(coords).shape[0]
+        return fixed_to_float(du_dx).reshape(N, 3), fixed_to_float(wrapping_sum(du_dl)), fixed_to_float(wrapping_sum(u))
+
+    def execute_fixed(self, coords, box, lam) -> np.ndarray:
+        _, _, u = self._execute_raw(coords, box, lam)
+        return np.array([wrapping_sum(u)], dtype=np.uint64)
+
+
+# ---------------------------------------------------------------- neighborlist
+
+
+class Neighborlist_f32(_Handle):
+    """Neighborlist (wrap_kernels.cpp:29-87) on 64-atom blocks."""
+
+    _release = "tm_neighborlist_release"
+    BLOCK = 64
+
+    def __init__(self, N: int):
+        super().__init__()
+        self._N = int(N)
+        call("tm_neighborlist_create", self._N, ctypes.byref(self._h))
+
+    def resize(self, size: int):
+        call("tm_neighborlist_resize", self._h, int(size))
+        self._N = int(size)
+
+    def reset_row_idxs(self):
+        pass
+
+    def set_row_idxs(self, idxs):
+        idxs = np.asarray(idxs, dtype=np.uint32)
+        if idxs.size == 0:
+            raise RuntimeError("idxs can't be empty")
+        if len(set(idxs.tolist())) != idxs.size:
+            raise RuntimeError("atom indices must be unique")
+        if idxs.size >= self._N:
+            raise RuntimeError("number of idxs must be less than N")
+        if idxs.max() >= self._N:
+            raise RuntimeError("indices values must be less than N")
+        raise NotImplementedError("row subsets (interaction groups) are not implemented on the MI355X path yet")
+
+    def _build(self, coords, box, cutoff):
+        coords = f64(coords)
+        box = f64(box)
+        n = ctypes.c_int(0)
+        call("tm_neighborlist_build_host", self._h, coords.shape[0], pd(coords), pd(box), float(cutoff), ctypes.byref(n))
+        rows = np.zeros(max(n.value, 1), dtype=np.int32)
+        atoms = np.zeros(max(n.value, 1) * self.BLOCK, dtype=np.uint32)
+        call(
+            "tm_neighborlist_get_tiles",
+            self._h,
+            max(n.value, 1),
+            pi(rows),
+            atoms.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+        )
+        return rows[: n.value], atoms[: n.value * self.BLOCK].reshape(n.value, self.BLOCK)
+
+    def get_tiles(self, coords, box, cutoff):
+        """Raw tiles: (row block per tile [T], column atoms [T, 64], 0xFFFFFFFF = padding)."""
+        return self._build(coords, box, cutoff)
+
+    def get_nblist(self, coords, box, cutoff) -> List[List[int]]:
+        N = np.asarray(coords).shape[0]
+        rows, atoms = self._build(coords, box, cutoff)
+        n_blocks = (N + self.BLOCK - 1) // self.BLOCK
+        out: List[List[int]] = [[] for _ in range(n_blocks)]
+        for r, a in zip(rows, atoms):
+            out[int(r)].extend(int(j) for j in a if j < N)
+        return out
+
+    def compute_block_bounds(self, coords, box, block_size):
+        if block_size != self.BLOCK:
+            raise RuntimeError(f"Block size must be {self.BLOCK}.")
+        coords = f64(coords)
+        box = f64(box)
+        if coords.shape[1] != 3:
+            raise RuntimeError("Dimensions must be 3")
+        N = coords.shape[0]
+        nb = (N + self.BLOCK - 1) // self.BLOCK
+        ctr = np.zeros((nb, 3))
+        ext = np.zeros((nb, 3))
+        call("tm_neighborlist_compute_block_bounds_host", self._h, N, pd(coords), pd(box), pd(ctr), pd(ext))
+        return ctr, ext
+
+
+# ---------------------------------------------------------------- integrator, barostat, context
+
+
+class Integrator(_Handle):
+    _release = "tm_integrator_release"
+
+
+class LangevinIntegrator(Integrator):
+    """LangevinIntegrator(dt, ca, cbs, ccs, seed) (wrap_kernels.cpp:314-332)."""
+
+    def __init__(self, dt, ca, cbs, ccs, seed):
+        super().__init__()
+        cbs = f64(cbs)
+        ccs = f64(ccs)
+        self.N = cbs.size
+        call(
+            "tm_langevin_integrator_create",
+            cbs.size,
+            float(dt),
+            float(ca),
+            pd(cbs),
+            pd(ccs),
+            int(seed) & 0xFFFFFFFFFFFFFFFF,
+            ctypes.byref(self._h),
+        )
+
+
+class MonteCarloBarostat(_Handle):
+    """MonteCarloBarostat(N, pressure, temperature, group_idxs, interval, bps, seed) (wrap_kernels.cpp:1016-1033)."""
+
+    _release = "tm_barostat_release"
+
+    def __init__(self, N, pressure, temperature, group_idxs, interval, bps, seed):
+        super().__init__()
+        sizes = i32([len(g) for g in group_idxs])
+        atoms = i32(np.concatenate([np.asarray(g, dtype=np.int32) for g in group_idxs]) if len(group_idxs) else [])
+        self._bps = list(bps)
+        arr = _lib.handle_array([bp.handle for bp in self._bps])
+        call(
+            "tm_barostat_create",
+            int(N),
+            float(pressure),
+            float(temperature),
+            sizes.size,
+            pi(sizes),
+            pi(atoms),
+            int(interval),
+            len(self._bps),
+            arr,
+            int(seed) & 0xFFFFFFFFFFFFFFFF,
+            ctypes.byref(self._h),
+        )
+
+    def set_interval(self, interval):
+        call("tm_barostat_set_interval", self._h, int(interval))
+
+    def get_interval(self) -> int:
+        n = ctypes.c_int(0)
+        call("tm_barostat_get_interval", self._h, ctypes.byref(n))
+        return n.value
+
+    def set_pressure(self, pressure):
+        call("tm_barostat_set_pressure", self._h, float(pressure))
+
+
+class Context(_Handle):
+    """Context(x0, v0, box, integrator, bps, barostat=None) (wrap_kernels.cpp:89-305)."""
+
+    _release = "tm_context_release"
+
+    def __init__(self, x0, v0, box, integrator, bps, barostat: Optional[MonteCarloBarostat] = None):
+        super().__init__()
+        x0 = f64(x0)
+        v0 = f64(v0)
+        box = f64(box)
+        N = x0.shape[0]
+        if v0.shape[0] != N:
+            raise RuntimeError("v0 N != x0 N")
+        if v0.shape[1] != x0.shape[1]:
+            raise RuntimeError("v0 D != x0 D")
+        self.N = N
+        self._integrator = integrator
+        self._bps = list(bps)
+        self._barostat = barostat
+        arr = _lib.handle_array([bp.handle for bp in self._bps])
+        call(
+            "tm_context_create",
+            N,
+            pd(x0),
+            pd(v0),
+            pd(box),
+            integrator.handle,
+            len(self._bps),
+            arr,
+            barostat.handle if barostat is not None else None,
+            ctypes.byref(self._h),
+        )
+
+    def step(self, lam):
+        call("tm_context_step", self._h, float(lam))
+
+    def multiple_steps(self, lambda_schedule, store_du_dl_interval=0, store_x_interval=0):
+        sched = f64(lambda_schedule).reshape(-1)
+        n = sched.size
+        dl_int = n if store_du_dl_interval <= 0 else int(store_du_dl_interval)
+        x_int = n if store_x_interval <= 0 else int(store_x_interval)
+        if n % x_int != 0:
+            print("warning:: length of lambda_schedule modulo store_x_interval does not equal zero")
+        if n % dl_int != 0:
+            print("warning:: length of lambda_schedule modulo store_du_dl_interval does not equal zero")
+        K = n // dl_int
+        F = n // x_int
+        du_dl = np.zeros(K)
+        xs = np.zeros((F, self.N, 3))
+        boxes = np.zeros((F, 3, 3))
+        call(
+            "tm_context_multiple_steps",
+            self._h,
+            n,
+            pd(sched),
+            dl_int,
+            x_int,
+            pd(du_dl),
+            pd(xs),
+            pd(boxes),
+        )
+        return du_dl, xs, boxes
+
+    def multiple_steps_U(self, lamb, n_steps, lambda_windows, store_u_interval, store_x_interval):
+        raise NotImplementedError("multiple_steps_U is not implemented on the MI355X path yet")
+
+    def set_x_t(self, x):
+        x = f64(x)
+        call("tm_context_set_x_t", self._h, pd(x))
+
+    def get_x_t(self) -> np.ndarray:
+        out = np.zeros((self.N, 3))
+        call("tm_context_get_x_t", self._h, pd(out))
+        return out
+
+    def get_v_t(self) -> np.ndarray:
+        out = np.zeros((self.N, 3))
+        call("tm_context_get_v_t", self._h, pd(out))
+        return out
+
+    def get_box(self) -> np.ndarray:
+        out = np.zeros((3, 3))
+        call("tm_context_get_box", self._h, pd(out))
+        return out
+
+    def _get_du_dx_t_minus_1(self) -> np.ndarray:
+        out = np.zeros(self.N * 3, dtype=np.uint64)
+        call("tm_context_get_du_dx_t_minus_1", self._h, pu64(out))
+        return fixed_to_float(out).reshape(self.N, 3)
+
+
+class BatchedContexts(_Handle):
+    """Steps many Contexts together in one engine (no reference counterpart).
+
+    Each context keeps its own potentials, integrator, barostat and seed; the
+    batch packs their current states into one HBM arena and advances all of
+    them with the same kernel launches (captured into hipGraphs). A system's
+    trajectory is bitwise identical to running its Context alone.
+    """
+
+    _release = "tm_batch_release"
+
+    def __init__(self, contexts: Sequence[Context], lambdas: Sequence[float]):
+        super().__init__()
+        self._ctxs = list(contexts)
+        lam = f64(lambdas).reshape(-1)
+        if lam.size != len(self._ctxs):
+            raise RuntimeError("one lambda per context is required")
+        self._lam = lam
+        arr = _lib.handle_array([c.handle for c in self._ctxs])
+        call("tm_batch_create", len(self._ctxs), arr, pd(lam), ctypes.byref(self._h))
+        self.sizes = [c.N for c in self._ctxs]
+
+    def __len__(self):
+        return len(self._ctxs)
+
+    def set_tile_grid(self, grid: int):
+        call("tm_batch_set_tile_grid", self._h, int(grid))
+
+    def multiple_steps(self, n_steps, lambda_schedules=None, store_du_dl_interval=0, store_x_interval=0, use_graph=True):
+        S = len(self._ctxs)
+        sched = None
+        if lambda_schedules is not None:
+            sched = f64(lambda_schedules).reshape(S, n_steps)
+        K = n_steps // store_du_dl_interval if store_du_dl_interval > 0 else 0
+        F = n_steps // store_x_interval if store_x_interval > 0 else 0
+        du_dl = np.zeros((S, max(K, 0)))
+        total = sum(self.sizes)
+        xs = np.zeros(max(F * total * 3, 1))
+        boxes = np.zeros((S, max(F, 0), 3, 3))
+        call(
+            "tm_batch_multiple_steps",
+            self._h,
+            int(n_steps),
+            pd(sched),
+            int(store_du_dl_interval),
+            int(store_x_interval),
+            pd(du_dl) if K else None,
+            pd(xs) if F else None,
+            pd(boxes) if F else None,
+            1 if use_graph else 0,
+        )
+        frames = []
+        off = 0
+        for n in self.sizes:
+            frames.append(xs[off : off + F * n * 3].reshape(F, n, 3))
+            off += F * n * 3
+        return du_dl, frames, boxes
+
+    def energies(self):
+        S = len(self._ctxs)
+        u = np.zeros(S)
+        dl = np.zeros(S)
+        call("tm_batch_energies", self._h, pd(u), pd(dl))
+        return u, dl
+
+    def get_x_t(self, s):
+        out = np.zeros((self.sizes[s], 3))
+        call("tm_batch_get_x", self._h, int(s), pd(out))
+        return out
+
+    def get_v_t(self, s):
+        out = np.zeros((self.sizes[s], 3))
+        call("tm_batch_get_v", self._h, int(s), pd(out))
+        return out
+
+    def get_box(self, s):
+        out = np.zeros((3, 3))
+        call("tm_batch_get_box", self._h, int(s), pd(out))
+        return out
+
+    def write_back(self):
+        call("tm_batch_write_back", self._h)
+
+    def profile(self, reps: int = 20) -> dict:
+        """Per-kernel-group durations (ms) timed with HIP events on the batch's stream."""
+        ms = np.zeros(4)
+        call("tm_batch_profile", self._h, int(reps), pd(ms))
+        return dict(zip(["prepare", "nonbonded", "pairlist_bonded", "integrator"], ms.tolist()))
+
+    def total_tiles(self) -> int:
+        n = ctypes.c_int(0)
+        call("tm_batch_total_tiles", self._h, ctypes.byref(n))
+        return n.value

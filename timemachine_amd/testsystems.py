@@ -1,0 +1,431 @@
+"""Seeded synthetic workloads (no OpenMM / RDKit / network needed).
+
+* :func:`water_box` - rigid-geometry 3-site waters on a jittered cubic lattice
+  with random orientations (the water-box execute workload: 3.0 nm, 887 waters).
+* :func:`ligand_in_water` - a random self-avoiding tree "ligand" with
+  hydrogens, bonded terms and 4D decoupling offsets, solvated in flexible
+  TIP3P-geometry water (the single-system MD workload: 4.0 nm, 2100 waters +
+  72 ligand atoms; also the unit system of the batched and serving workloads).
+
+Parameter conventions are the reference's: nonbonded params per atom are
+``[q * sqrt(ONE_4PI_EPS0), sigma / 2, sqrt(eps)]``
+(ff/handlers/openmm_deserializer.py:109,189-190); exclusions carry
+(charge, LJ) scales with 1-2 and 1-3 at 1.0 and 1-4 at 0.5 (fe/topology.py:11-13).
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import List, Optional
+
+import numpy as np
+
+ONE_4PI_EPS0 = 138.935456
+SQRT_K = math.sqrt(ONE_4PI_EPS0)
+
+# TIP3P geometry and charges; flexible-water force constants from OpenMM's
+# tip3p (bond 462750.4 kJ/mol/nm^2, angle 836.8 kJ/mol/rad^2). The angle term
+# here is the cosine form k/2 (cos t - cos t0)^2, so the harmonic constant is
+# divided by sin^2(t0) to keep the same curvature at the minimum.
+WATER_OH = 0.09572
+WATER_HOH = math.radians(104.52)
+WATER_Q_O = -0.834
+WATER_Q_H = 0.417
+WATER_SIG_O = 0.315
+WATER_EPS_O = 0.635
+WATER_K_BOND = 462750.4
+WATER_K_ANGLE_COS = 836.8 / math.sin(WATER_HOH) ** 2
+
+MASS_H = 1.00794
+MASS_O = 15.9994
+MASS_C = 12.0107
+
+
+@dataclasses.dataclass
+class System:
+    x: np.ndarray  # [N, 3] f64 (values exactly representable in f32)
+    box: np.ndarray  # [3, 3]
+    masses: np.ndarray  # [N]
+    nb_params: np.ndarray  # [N, 3]
+    exclusion_idxs: np.ndarray  # [E, 2] int32
+    scales: np.ndarray  # [E, 2]
+    lambda_plane_idxs: np.ndarray  # [N] int32
+    lambda_offset_idxs: np.ndarray  # [N] int32
+    bond_idxs: np.ndarray  # [B, 2]
+    bond_params: np.ndarray  # [B, 2] (k, b0)
+    angle_idxs: np.ndarray  # [A, 3]
+    angle_params: np.ndarray  # [A, 2] (k, theta0)
+    torsion_idxs: np.ndarray  # [T, 4]
+    torsion_params: np.ndarray  # [T, 3] (k, phase, period)
+    groups: List[np.ndarray]
+    ligand_idxs: np.ndarray
+    beta: float = 2.0
+    cutoff: float = 1.2
+
+    @property
+    def N(self) -> int:
+        return self.x.shape[0]
+
+    def nonbonded(self):
+        from timemachine_amd.lib import potentials
+
+        return potentials.Nonbonded(
+            self.exclusion_idxs,
+            self.scales,
+            self.lambda_plane_idxs,
+            self.lambda_offset_idxs,
+            self.beta,
+            self.cutoff,
+        ).bind(self.nb_params)
+
+    def bound_potentials(self, include_nonbonded: bool = True):
+        """lib.potentials wrappers bound to their parameters."""
+        from timemachine_amd.lib import potentials
+
+        out = []
+        if len(self.bond_idxs):
+            out.append(potentials.HarmonicBond(self.bond_idxs).bind(self.bond_params))
+        if len(self.angle_idxs):
+            out.append(potentials.HarmonicAngle(self.angle_idxs).bind(self.angle_params))
+        if len(self.torsion_idxs):
+            out.append(potentials.PeriodicTorsion(self.torsion_idxs).bind(self.torsion_params))
+        if include_nonbonded:
+            out.append(self.nonbonded())
+        return out
+
+
+def _random_rotations(rng: np.random.Generator, n: int) -> np.ndarray:
+    """Uniform random rotation matrices from normalised Gaussian quaternions."""
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    return np.stack(
+        [
+            np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+            np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+            np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1),
+        ],
+        1,
+    )
+
+
+def _water_frame() -> np.ndarray:
+    return np.array(
+        [
+            [0.0, 0.0, 0.0],
+            [WATER_OH, 0.0, 0.0],
+            [WATER_OH * math.cos(WATER_HOH), WATER_OH * math.sin(WATER_HOH), 0.0],
+        ]
+    )
+
+
+def _min_image_dist(a: np.ndarray, b: np.ndarray, box_diag: np.ndarray) -> np.ndarray:
+    d = a[:, None, :] - b[None, :, :]
+    d -= box_diag * np.floor(d / box_diag + 0.5)
+    return np.sqrt((d * d).sum(-1))
+
+
+def _place_waters(
+    rng: np.random.Generator,
+    side: float,
+    n_waters: int,
+    jitter: float,
+    avoid: Optional[np.ndarray] = None,
+    avoid_radius: float = 0.25,
+    clash: float = 0.0,
+    tries: int = 30,
+) -> np.ndarray:
+    """Waters on a jittered cubic lattice with random orientations.
+
+    With clash > 0, each water is re-oriented (up to `tries` times) until its
+    atoms are at least `clash` nm from every atom placed before it.
+    """
+    box_diag = np.full(3, side)
+    n_side = 1
+    while n_side**3 < n_waters * (1.05 if avoid is not None else 1.0):
+        n_side += 1
+    spacing = side / n_side
+    g = (np.arange(n_side) + 0.5) * spacing
+    sites = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    if avoid is not None and len(avoid):
+        dmin = _min_image_dist(sites, avoid, box_diag).min(1)
+        sites = sites[dmin > avoid_radius + 0.5 * spacing]
+    if len(sites) < n_waters:
+        raise ValueError(f"only {len(sites)} lattice sites for {n_waters} waters")
+    sites = sites[np.sort(rng.choice(len(sites), n_waters, replace=False))]
+    sites = sites + rng.uniform(-jitter, jitter, size=sites.shape)
+    frame = _water_frame()
+    if clash <= 0:
+        rot = _random_rotations(rng, n_waters)
+        return (sites[:, None, :] + np.einsum("wij,aj->wai", rot, frame)).reshape(-1, 3)
+    # clash-avoiding placement with a cell grid
+    placed = np.zeros((n_waters, 3, 3))
+    ncell = max(1, int(side / 0.45))
+    csize = side / ncell
+    cells: dict = {}
+    extra = [] if avoid is None else list(avoid)
+    for p in extra:
+        key = tuple((np.floor((p % side) / csize).astype(int) % ncell).tolist())
+        cells.setdefault(key, []).append(p)
+    for w in range(n_waters):
+        best = None
+        best_d = -1.0
+        key0 = np.floor((sites[w] % side) / csize).astype(int) % ncell
+        neigh = []
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    k = tuple(((key0 + np.array([dx, dy, dz])) % ncell).tolist())
+                    neigh.extend(cells.get(k, []))
+        neigh = np.array(neigh) if neigh else np.zeros((0, 3))
+        for _ in range(tries):
+            rot = _random_rotations(rng, 1)[0]
+            cand = sites[w] + frame @ rot.T
+            if len(neigh) == 0:
+                best = cand
+                break
+            d = _min_image_dist(cand, neigh, box_diag).min()
+            if d > best_d:
+                best_d, best = d, cand
+            if d >= clash:
+                break
+        placed[w] = best
+        for p in best:
+            k = tuple((np.floor((p % side) / csize).astype(int) % ncell).tolist())
+            cells.setdefault(k, []).append(p)
+    return placed.reshape(-1, 3)
+
+
+def _water_topology(n_waters: int, offset: int, rng: np.random.Generator, flexible: bool):
+    o = offset + 3 * np.arange(n_waters)
+    excl = np.concatenate([np.stack([o, o + 1], 1), np.stack([o, o + 2], 1), np.stack([o + 1, o + 2], 1)])
+    scales = np.ones((len(excl), 2))
+    bonds = np.concatenate([np.stack([o, o + 1], 1), np.stack([o, o + 2], 1)]) if flexible else np.zeros((0, 2), int)
+    bond_params = np.tile([WATER_K_BOND, WATER_OH], (len(bonds), 1))
+    angles = np.stack([o + 1, o, o + 2], 1) if flexible else np.zeros((0, 3), int)
+    angle_params = np.tile([WATER_K_ANGLE_COS, WATER_HOH], (len(angles), 1))
+    groups = [np.array([a, a + 1, a + 2]) for a in o]
+    return excl, scales, bonds, bond_params, angles, angle_params, groups
+
+
+def _water_nb_params(n_waters: int) -> np.ndarray:
+    p = np.zeros((n_waters, 3, 3))
+    p[:, 0] = [WATER_Q_O * SQRT_K, WATER_SIG_O / 2, math.sqrt(WATER_EPS_O)]
+    p[:, 1] = [WATER_Q_H * SQRT_K, 0.0, 0.0]
+    p[:, 2] = [WATER_Q_H * SQRT_K, 0.0, 0.0]
+    return p.reshape(-1, 3)
+
+
+def _f32_round(x: np.ndarray) -> np.ndarray:
+    return x.astype(np.float32).astype(np.float64)
+
+
+def water_box(side: float = 3.0, n_waters: int = 887, seed: int = 2022, jitter: float = 0.03) -> System:
+    """Water-box execute workload: rigid TIP3P geometry, exclusions O-H, O-H, H-H at (1, 1)."""
+    rng = np.random.default_rng(seed)
+    x = _f32_round(_place_waters(rng, side, n_waters, jitter))
+    N = x.shape[0]
+    excl, scales, _, _, _, _, groups = _water_topology(n_waters, 0, rng, flexible=False)
+    masses = np.tile([MASS_O, MASS_H, MASS_H], n_waters)
+    z = np.zeros
+    return System(
+        x=x,
+        box=np.eye(3) * side,
+        masses=masses,
+        nb_params=_water_nb_params(n_waters),
+        exclusion_idxs=excl.astype(np.int32),
+        scales=scales,
+        lambda_plane_idxs=z(N, np.int32),
+        lambda_offset_idxs=z(N, np.int32),
+        bond_idxs=z((0, 2), np.int32),
+        bond_params=z((0, 2)),
+        angle_idxs=z((0, 3), np.int32),
+        angle_params=z((0, 2)),
+        torsion_idxs=z((0, 4), np.int32),
+        torsion_params=z((0, 3)),
+        groups=groups,
+        ligand_idxs=z(0, np.int32),
+    )
+
+
+def _ligand(rng: np.random.Generator, n_heavy: int, n_total: int):
+    """Random self-avoiding tree of heavy atoms (valence <= 4) plus hydrogens."""
+    pos = [np.zeros(3)]
+    parent = [-1]
+    degree = [0]
+    bonds = []
+    bond_len = []
+    while len(pos) < n_heavy:
+        cands = [i for i in range(len(pos)) if degree[i] < 3]
+        i = int(rng.choice(cands))
+        for _ in range(200):
+            L = rng.uniform(0.13, 0.15)
+            # keep the new bond roughly tetrahedral to the parent bond
+            v = rng.normal(size=3)
+            v /= np.linalg.norm(v)
+            if parent[i] >= 0:
+                u = pos[i] - pos[parent[i]]
+                u /= np.linalg.norm(u)
+                if np.dot(u, v) < 0.2:
+                    continue
+            p = pos[i] + L * v
+            if min((np.linalg.norm(p - q) for k, q in enumerate(pos) if k != i), default=np.inf) > 0.22:
+                break
+        else:
+            continue
+        pos.append(p)
+        parent.append(i)
+        degree.append(1)
+        degree[i] += 1
+        bonds.append((i, len(pos) - 1))
+        bond_len.append(L)
+    n_h = n_total - n_heavy
+    heavy_free = [i for i in range(n_heavy) for _ in range(max(0, 4 - degree[i]))]
+    rng.shuffle(heavy_free)
+    h_hosts = heavy_free[:n_h]
+    if len(h_hosts) < n_h:
+        raise ValueError("not enough free valence for hydrogens")
+    for i in h_hosts:
+        best, best_d = None, -1.0
+        for _ in range(100):
+            v = rng.normal(size=3)
+            v /= np.linalg.norm(v)
+            L = rng.uniform(0.10, 0.11)
+            p = pos[i] + L * v
+            d = min((np.linalg.norm(p - q) for k, q in enumerate(pos) if k != i), default=np.inf)
+            if d > best_d:
+                best, best_d = p, d
+            if d > 0.16:
+                break
+        pos.append(best)
+        parent.append(i)
+        degree.append(1)
+        degree[i] += 1
+        bonds.append((i, len(pos) - 1))
+        bond_len.append(np.linalg.norm(best - pos[i]))
+    return np.array(pos), np.array(bonds), np.array(bond_len)
+
+
+def _angles_torsions(n: int, bonds: np.ndarray):
+    nbr = [[] for _ in range(n)]
+    for i, j in bonds:
+        nbr[i].append(j)
+        nbr[j].append(i)
+    angles = []
+    for j in range(n):
+        for a in range(len(nbr[j])):
+            for b in range(a + 1, len(nbr[j])):
+                angles.append((nbr[j][a], j, nbr[j][b]))
+    torsions = []
+    for j, k in bonds:
+        for i in nbr[j]:
+            if i == k:
+                continue
+            for l in nbr[k]:
+                if l == j or l == i:
+                    continue
+                torsions.append((i, j, k, l))
+    return np.array(angles, dtype=np.int32).reshape(-1, 3), np.array(torsions, dtype=np.int32).reshape(-1, 4)
+
+
+def ligand_in_water(
+    side: float = 4.0,
+    n_waters: int = 2100,
+    n_heavy: int = 40,
+    n_ligand: int = 72,
+    seed: int = 2022,
+    hmr: bool = True,
+) -> System:
+    """Single-system MD workload: synthetic ligand + flexible water, 4D offsets on the ligand."""
+    rng = np.random.default_rng(seed)
+    lpos, lbonds, lbond_len = _ligand(rng, n_heavy, n_ligand)
+    lpos = lpos - lpos.mean(0) + side / 2
+    wx = _place_waters(rng, side, n_waters, 0.02, avoid=lpos, avoid_radius=0.22, clash=0.17)
+    x = _f32_round(np.concatenate([lpos, wx]))
+    NL = n_ligand
+    N = x.shape[0]
+    langles, ltors = _angles_torsions(NL, lbonds)
+    # ligand parameters
+    q = rng.normal(0.0, 0.3, NL)
+    q -= q.mean()
+    sig = rng.uniform(0.25, 0.35, NL)
+    eps = rng.uniform(0.1, 0.9, NL)
+    lig_nb = np.stack([q * SQRT_K, sig / 2, np.sqrt(eps)], 1)
+    bond_params = np.stack([np.full(len(lbonds), 2e5), lbond_len], 1)
+    angle_params = np.stack([np.full(len(langles), 400.0), rng.uniform(1.9, 2.1, len(langles))], 1)
+    period = rng.integers(1, 4, len(ltors)).astype(np.float64)
+    phase = rng.choice([0.0, math.pi], len(ltors))
+    tk = rng.uniform(0.0, 10.0, len(ltors))
+    torsion_params = np.stack([tk, phase, period], 1)
+    # ligand exclusions: 1-2, 1-3 at 1.0; 1-4 at 0.5
+    pairs = {}
+    for i, j in lbonds:
+        pairs[tuple(sorted((int(i), int(j))))] = 1.0
+    for i, _, k in langles:
+        pairs.setdefault(tuple(sorted((int(i), int(k)))), 1.0)
+    for i, _, _, l in ltors:
+        pairs.setdefault(tuple(sorted((int(i), int(l)))), 0.5)
+    lexcl = np.array(sorted(pairs), dtype=np.int32).reshape(-1, 2)
+    lscales = np.array([[pairs[tuple(p)], pairs[tuple(p)]] for p in lexcl.tolist()]).reshape(-1, 2)
+    # water
+    n_w = n_waters
+    wexcl, wscales, wbonds, wbp, wangles, wap, wgroups = _water_topology(n_w, NL, rng, flexible=True)
+    # masses (ligand heavy atoms as carbon) and hydrogen mass repartitioning
+    masses = np.concatenate([np.where(np.arange(NL) < n_heavy, MASS_C, MASS_H), np.tile([MASS_O, MASS_H, MASS_H], n_w)])
+    all_bonds = np.concatenate([lbonds, wbonds]).astype(np.int32)
+    if hmr:
+        masses = apply_hmr(masses, all_bonds)
+    offsets = np.zeros(N, np.int32)
+    offsets[:NL] = 1
+    return System(
+        x=x,
+        box=np.eye(3) * side,
+        masses=masses,
+        nb_params=np.concatenate([lig_nb, _water_nb_params(n_w)]),
+        exclusion_idxs=np.concatenate([lexcl, wexcl]).astype(np.int32),
+        scales=np.concatenate([lscales, wscales]),
+        lambda_plane_idxs=np.zeros(N, np.int32),
+        lambda_offset_idxs=offsets,
+        bond_idxs=all_bonds,
+        bond_params=np.concatenate([bond_params, wbp]),
+        angle_idxs=np.concatenate([langles, wangles]).astype(np.int32),
+        angle_params=np.concatenate([angle_params, wap]),
+        torsion_idxs=ltors,
+        torsion_params=torsion_params,
+        groups=[np.arange(NL)] + wgroups,
+        ligand_idxs=np.arange(NL, dtype=np.int32),
+    )
+
+
+def apply_hmr(masses: np.ndarray, bond_list: np.ndarray, multiplier: float = 2) -> np.ndarray:
+    """Hydrogen mass repartitioning as fe/model_utils.py:22-66: each H bonded to a
+    heavy atom gains multiplier * m_H, taken from the heavy atom."""
+    masses = np.array(masses, dtype=np.float64)
+
+    def is_h(i):
+        return abs(masses[i] - MASS_H) < 1e-3
+
+    for i, j in bond_list:
+        i, j = (i, j) if masses[i] <= masses[j] else (j, i)
+        if is_h(i) and not is_h(j):
+            masses[j] -= multiplier * masses[i]
+            masses[i] += multiplier * masses[i]
+    return masses
+
+
+def maxwell_boltzmann(masses: np.ndarray, temperature: float, seed: int) -> np.ndarray:
+    from timemachine_amd.lib import BOLTZ
+
+    rng = np.random.default_rng(seed)
+    sd = np.sqrt(BOLTZ * temperature / np.asarray(masses))
+    return rng.normal(size=(len(masses), 3)) * sd[:, None]
+
+
+def job_water_box(n_atoms_target: int, n_heavy: int, seed: int) -> System:
+    """Serving job: a synthetic ligand solvated in a water box of ~n_atoms_target atoms."""
+    n_lig = n_heavy + int(round(0.8 * n_heavy))
+    n_waters = max(100, (n_atoms_target - n_lig) // 3)
+    side = (n_waters / 33.4) ** (1.0 / 3.0)
+    side = max(side, 2 * 1.3 + 0.05)
+    return ligand_in_water(side=side, n_waters=n_waters, n_heavy=n_heavy, n_ligand=n_lig, seed=seed)
