@@ -29,13 +29,15 @@ if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
 
 METRIC = "atom-steps/s (batched λ-windows vs single system); p50/p99 job latency"
-PEAK_FP32_TFLOPS = 157.3  # MI355X published dense FP32 vector peak
-# Algorithmic FP32 work per pair inside the cutoff, du_dx only (FMA = 2,
-# rsqrt / exp / rcp = 1): 20 (delta, min-image, d^2) + 27 (erfc-damped
-# Coulomb force) + 9 (projection, fixed-point scaling); + 25 when both
+PEAK_FP64_TFLOPS = 78.6  # MI355X published FP64 peak (vector and matrix are equal)
+# Algorithmic FP64 work per pair inside the cutoff, du_dx only (FMA = 2,
+# FP32 rsq / rcp / exp seeds = 1): 15 (delta, min-image) + 5 (d^2) + 10
+# (1/d Newton, d, 1/d^2, beta d, x^2) + 4 (exp with remainder) + 6 (1/(1+px)
+# Newton) + 14 (degree-7 erfc polynomial) + 8 (charge product, Coulomb
+# prefactor) + 9 (projection, fixed-point scaling) = 71; + 13 when both
 # epsilons are nonzero (Lennard-Jones).
-FLOP_PAIR_BASE = 56
-FLOP_PAIR_LJ = 25
+FLOP_PAIR_BASE = 71
+FLOP_PAIR_LJ = 13
 
 
 def log(msg: str) -> None:
@@ -188,9 +190,9 @@ def main() -> int:
         "bound": "valu",
         "kernel": "k_nonbonded_tiles (du_dx)",
         "achieved": achieved,
-        "peak": PEAK_FP32_TFLOPS,
+        "peak": PEAK_FP64_TFLOPS,
         "unit": "TFLOP/s",
-        "frac": achieved / PEAK_FP32_TFLOPS,
+        "frac": achieved / PEAK_FP64_TFLOPS,
         "traffic": None,
         "pairs_per_launch": pairs_total,
         "flop_per_launch": flop,
@@ -213,7 +215,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f64",
         "data": "synthetic",
         "config": {
             "workload": f"batched lambda windows: {args.windows} windows x {args.replicas} replicas of the "

@@ -30,22 +30,23 @@ uint64_t hc_round_to_fixed(double v) { return round_to_fixed(v); }
 uint64_t hc_f32_to_fixed(float v, float scale) { return f32_to_fixed(v, scale); }
 
 // out: prefactor, u, erfc_inv_d, sig_grad, eps_grad, lj
-void hc_pair_terms(float d2, float qi, float qj, float si, float sj, float ei, float ej, float beta, float sq,
-                   float slj, float *out) {
+void hc_pair_terms(double d2, double qi, double qj, double si, double sj, double ei, double ej, double beta, double sq,
+                   double slj, double *out) {
     PairTerms t = nb_pair_terms<true, true>(d2, qi, qj, si, sj, ei, ej, beta, sq, slj);
     out[0] = t.prefactor;
     out[1] = t.u;
     out[2] = t.erfc_inv_d;
     out[3] = t.sig_grad;
     out[4] = t.eps_grad;
-    out[5] = t.lj ? 1.0f : 0.0f;
+    out[5] = t.lj ? 1.0 : 0.0;
 }
 
 float hc_min_image(float d, float b) { return min_image_f(d, b, 1.0f / b); }
+double hc_min_image_d(double d, double b) { return min_image_d(d, b, 1.0 / b); }
 
-// out: g[2][3], dkb, db0, u, dl
-void hc_bond(const double *xi, const double *xj, float kb, float b0, int mult, int offset, double lam, float *out) {
-    BondOut o = bond_term(xi, xj, kb, b0, mult, lam_prefactor(mult, offset, lam));
+// out: g[2][3], dkb, db0, u, dl (FP64, as the kernels compute them)
+void hc_bond(const double *xi, const double *xj, double kb, double b0, int mult, int offset, double lam, double *out) {
+    BondOut<double> o = bond_term<double>(xi, xj, kb, b0, mult, lam_prefactor<double>(mult, offset, lam));
     for (int a = 0; a < 2; a++)
         for (int d = 0; d < 3; d++) out[a * 3 + d] = o.g[a][d];
     out[6] = o.dkb;
@@ -55,9 +56,9 @@ void hc_bond(const double *xi, const double *xj, float kb, float b0, int mult, i
 }
 
 // out: g[3][3], dka, da0, u, dl
-void hc_angle(const double *xi, const double *xj, const double *xk, float ka, float a0, int mult, int offset, double lam,
-              float *out) {
-    AngleOut o = angle_term(xi, xj, xk, ka, a0, mult, lam_prefactor(mult, offset, lam));
+void hc_angle(const double *xi, const double *xj, const double *xk, double ka, double a0, int mult, int offset,
+              double lam, double *out) {
+    AngleOut<double> o = angle_term<double>(xi, xj, xk, ka, a0, mult, lam_prefactor<double>(mult, offset, lam));
     for (int a = 0; a < 3; a++)
         for (int d = 0; d < 3; d++) out[a * 3 + d] = o.g[a][d];
     out[9] = o.dka;
@@ -67,9 +68,10 @@ void hc_angle(const double *xi, const double *xj, const double *xk, float ka, fl
 }
 
 // out: g[4][3], dk, dphase, dperiod, u, dl
-void hc_torsion(const double *xi, const double *xj, const double *xk, const double *xl, float k, float phase,
-                float period, int mult, int offset, double lam, float *out) {
-    TorsionOut o = torsion_term(xi, xj, xk, xl, k, phase, period, mult, lam_prefactor(mult, offset, lam));
+void hc_torsion(const double *xi, const double *xj, const double *xk, const double *xl, double k, double phase,
+                double period, int mult, int offset, double lam, double *out) {
+    TorsionOut<double> o =
+        torsion_term<double>(xi, xj, xk, xl, k, phase, period, mult, lam_prefactor<double>(mult, offset, lam));
     for (int a = 0; a < 4; a++)
         for (int d = 0; d < 3; d++) out[a * 3 + d] = o.g[a][d];
     out[12] = o.dk;
@@ -78,4 +80,32 @@ void hc_torsion(const double *xi, const double *xj, const double *xk, const doub
     out[15] = o.u;
     out[16] = o.dl;
 }
+}
+
+// CPU model of the tile kernel's accumulation: every pair of `pairs` inside the
+// cutoff adds fixed(prefactor * d) to i and subtracts it from j, exactly as the
+// kernel does (min-image and d2 in FP32 on FP32-rounded coordinates).
+extern "C" void hc_nonbonded_du_dx(int N, const double *x, const double *params, const double *box, double beta,
+                                   double cutoff, int M, const int64_t *pairs, uint64_t *du_dx, uint64_t *u) {
+    const double bx = box[0], by = box[4], bz = box[8];
+    const double ibx = 1.0 / bx, iby = 1.0 / by, ibz = 1.0 / bz;
+    for (int m = 0; m < M; m++) {
+        const int i = (int)pairs[2 * m], j = (int)pairs[2 * m + 1];
+        const double dx = min_image_d(x[3 * i] - x[3 * j], bx, ibx);
+        const double dy = min_image_d(x[3 * i + 1] - x[3 * j + 1], by, iby);
+        const double dz = min_image_d(x[3 * i + 2] - x[3 * j + 2], bz, ibz);
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (!(d2 < cutoff * cutoff)) continue;
+        auto f32 = [](double v) { return (double)(float)v; };
+        PairTerms t = nb_pair_terms<true, false>(d2, f32(params[3 * i]), f32(params[3 * j]), f32(params[3 * i + 1]),
+                                                 f32(params[3 * j + 1]), f32(params[3 * i + 2]), f32(params[3 * j + 2]),
+                                                 (double)(float)beta, 1.0, 1.0);
+        const uint64_t f[3] = {f64_to_fixed(t.prefactor * dx, FIXED_SCALE), f64_to_fixed(t.prefactor * dy, FIXED_SCALE),
+                               f64_to_fixed(t.prefactor * dz, FIXED_SCALE)};
+        for (int d = 0; d < 3; d++) {
+            du_dx[3 * i + d] += f[d];
+            du_dx[3 * j + d] -= f[d];
+        }
+        u[0] += f64_to_fixed(t.u, FIXED_SCALE);
+    }
 }

@@ -41,12 +41,16 @@ def hc(tmp_path_factory):
     lib.hc_round_to_fixed.argtypes = [ctypes.c_double]
     lib.hc_f32_to_fixed.restype = ctypes.c_uint64
     lib.hc_f32_to_fixed.argtypes = [ctypes.c_float, ctypes.c_float]
-    lib.hc_pair_terms.argtypes = [ctypes.c_float] * 10 + [P(ctypes.c_float)]
+    lib.hc_pair_terms.argtypes = [ctypes.c_double] * 10 + [P(ctypes.c_double)]
+    lib.hc_nonbonded_du_dx.argtypes = [ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_double),
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_int, P(ctypes.c_int64),
+                                       P(ctypes.c_uint64), P(ctypes.c_uint64)]
     lib.hc_min_image.restype = ctypes.c_float
     lib.hc_min_image.argtypes = [ctypes.c_float, ctypes.c_float]
-    lib.hc_bond.argtypes = [P(ctypes.c_double)] * 2 + [ctypes.c_float] * 2 + [ctypes.c_int] * 2 + [ctypes.c_double, P(ctypes.c_float)]
-    lib.hc_angle.argtypes = [P(ctypes.c_double)] * 3 + [ctypes.c_float] * 2 + [ctypes.c_int] * 2 + [ctypes.c_double, P(ctypes.c_float)]
-    lib.hc_torsion.argtypes = [P(ctypes.c_double)] * 4 + [ctypes.c_float] * 3 + [ctypes.c_int] * 2 + [ctypes.c_double, P(ctypes.c_float)]
+    D = ctypes.c_double
+    lib.hc_bond.argtypes = [P(D)] * 2 + [D] * 2 + [ctypes.c_int] * 2 + [D, P(D)]
+    lib.hc_angle.argtypes = [P(D)] * 3 + [D] * 2 + [ctypes.c_int] * 2 + [D, P(D)]
+    lib.hc_torsion.argtypes = [P(D)] * 4 + [D] * 3 + [ctypes.c_int] * 2 + [D, P(D)]
     return lib
 
 
@@ -133,41 +137,62 @@ def _pair_ref(d, qi, qj, si, sj, ei, ej, beta, sq, slj):
 
 
 def test_pair_terms_against_closed_form(hc):
-    """FP32 pair arithmetic (erfc fit included) vs the FP64 closed form, relative
-    to the magnitude of the parts that are summed (errors must stay FP32-sized)."""
+    """Pair arithmetic (FP64 with FP32 exp and rsq/rcp seeds) vs the FP64 closed form,
+    relative to the magnitude of the parts that are summed."""
     rng = np.random.default_rng(3)
-    out = (ctypes.c_float * 6)()
+    out = (ctypes.c_double * 6)()
     worst = 0.0
-    f = lambda v: float(np.float32(v))  # noqa: E731
-    for d in np.linspace(0.12, 1.19, 400):
+    for d in np.linspace(0.08, 1.19, 600):
         qi, qj = rng.normal(0, 5, 2)
         si, sj = rng.uniform(0.1, 0.2, 2)
         ei, ej = rng.uniform(0.3, 1.0, 2)
-        d = f(d)
         hc.hc_pair_terms(d * d, qi, qj, si, sj, ei, ej, 2.0, 1.0, 1.0, out)
-        d2 = f(d * d)
-        vals, mags = _pair_ref(math.sqrt(d2), f(qi), f(qj), f(si), f(sj), f(ei), f(ej), 2.0, 1.0, 1.0)
+        vals, mags = _pair_ref(d, qi, qj, si, sj, ei, ej, 2.0, 1.0, 1.0)
         for k in range(5):
             worst = max(worst, abs(out[k] - vals[k]) / max(mags[k], 1e-30))
         assert out[5] == 1.0
-    assert worst < 5e-6, worst
+    assert worst < 4e-7, worst
 
 
 def test_erfc_fit_relative_error(hc):
-    """erfc(beta d)/d of the pair code vs scipy over the cutoff range: < 1.5e-6 relative."""
-    out = (ctypes.c_float * 6)()
+    """erfc(beta d)/d of the pair code vs scipy over the cutoff range (incl. 4D distances)."""
+    out = (ctypes.c_double * 6)()
     errs = []
-    for d in np.linspace(0.05, 1.3, 2000).astype(np.float32):
-        hc.hc_pair_terms(float(d) * float(d), 1.0, 1.0, 0.0, 0.0, 0.0, 0.0, 2.0, 1.0, 1.0, out)
-        dd = math.sqrt(float(np.float32(float(d) * float(d))))
-        ref = erfc(2.0 * dd) / dd
-        errs.append(abs(out[2] / ref - 1))
-    assert max(errs) < 1.5e-6
-    assert abs(np.mean(errs)) < 4e-7
+    for d in np.linspace(0.05, 1.3, 2000):
+        hc.hc_pair_terms(d * d, 1.0, 1.0, 0.0, 0.0, 0.0, 0.0, 2.0, 1.0, 1.0, out)
+        errs.append(abs(out[2] / (erfc(2.0 * d) / d) - 1))
+    assert max(errs) < 3e-7
+    assert np.mean(errs) < 1e-7
+
+
+def test_nonbonded_sum_model_meets_force_target(hc):
+    """The kernel's per-pair arithmetic summed in fixed point over a water box
+    (exclusions removed) stays under 1e-5 per-atom force error vs FP64 with a
+    margin for the GPU's ~1-ulp exp / rsq / rcp seeds."""
+    from oracle import nonbonded as onb
+    from timemachine_amd import testsystems as ts
+
+    w = ts.water_box()
+    pairs = onb.candidate_pairs(w.x, w.box, 1.2).astype(np.int64)
+    excl = set(map(tuple, np.sort(w.exclusion_idxs, 1).tolist()))
+    keep = np.array([tuple(q) not in excl for q in pairs.tolist()])
+    pp = np.ascontiguousarray(pairs[keep])
+    du = np.zeros(w.N * 3, np.uint64)
+    u = np.zeros(1, np.uint64)
+    hc.hc_nonbonded_du_dx(w.N, _pd(w.x), _pd(w.nb_params), _pd(w.box), 2.0, 1.2, len(pp),
+                          pp.ctypes.data_as(P(ctypes.c_int64)), du.ctypes.data_as(P(ctypes.c_uint64)),
+                          u.ctypes.data_as(P(ctypes.c_uint64)))
+    f = du.view(np.int64).astype(np.float64).reshape(-1, 3) / 2**36
+    ref = onb.nonbonded_pairs(w.x, w.nb_params, w.box, 0.0, w.exclusion_idxs, w.scales, 2.0, 1.2,
+                              w.lambda_plane_idxs, w.lambda_offset_idxs)
+    err = onb.force_rel_err(f, ref[0])
+    assert err.max() < 3e-6, err.max()
+    uu = u.view(np.int64)[0] / 2**36
+    assert abs(uu - ref[3]) / abs(ref[3]) < 5e-7
 
 
 def test_lj_skipped_when_either_eps_zero(hc):
-    out = (ctypes.c_float * 6)()
+    out = (ctypes.c_double * 6)()
     hc.hc_pair_terms(0.09, 1.0, -1.0, 0.15, 0.0, 0.8, 0.0, 2.0, 1.0, 1.0, out)
     assert out[5] == 0.0 and out[3] == 0.0 and out[4] == 0.0
 
@@ -183,7 +208,7 @@ def test_min_image_symmetric(hc):
 
 def _bonded_host(hc, kind, x, idx, params, mult, offset, lam):
     n = {"bond": 10, "angle": 13, "torsion": 17}[kind]
-    out = (ctypes.c_float * n)()
+    out = (ctypes.c_double * n)()
     xs = [_pd(x[i]) for i in idx]
     if kind == "bond":
         hc.hc_bond(*xs, float(params[0]), float(params[1]), mult, offset, lam, out)
@@ -214,10 +239,10 @@ def test_bonded_terms_against_oracle(hc, kind, arity, params):
         p = np.array(params, dtype=np.float64)[None, :]
         gx, gp, gl, u = bonded.bonded(kind, x, p, lam, np.array([idx]), mult=[mult], offset=[offset])
         g = h[: arity * 3].reshape(arity, 3)
-        np.testing.assert_allclose(g, gx, rtol=2e-4, atol=1e-3 * max(1.0, np.abs(gx).max()) * 1e-2)
-        np.testing.assert_allclose(h[arity * 3 + len(params)], u, rtol=2e-4, atol=1e-6)
-        np.testing.assert_allclose(h[arity * 3 : arity * 3 + len(params)], gp[0], rtol=2e-4, atol=1e-4)
-        np.testing.assert_allclose(h[-1], gl, rtol=2e-4, atol=1e-5)
+        np.testing.assert_allclose(g, gx, rtol=1e-10, atol=1e-10 * max(1.0, np.abs(gx).max()))
+        np.testing.assert_allclose(h[arity * 3 + len(params)], u, rtol=1e-10, atol=1e-12)
+        np.testing.assert_allclose(h[arity * 3 : arity * 3 + len(params)], gp[0], rtol=1e-10, atol=1e-10)
+        np.testing.assert_allclose(h[-1], gl, rtol=1e-10, atol=1e-12)
 
 
 def test_bonded_index_reversal_is_bitwise(hc):

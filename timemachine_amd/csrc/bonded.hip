@@ -1,6 +1,7 @@
 // HarmonicBond / HarmonicAngle / PeriodicTorsion kernels, one thread per term,
 // over the terms of one or many systems (term_sys maps a term to its system
-// for lambda and the reduction index). Atom indices are global.
+// for lambda and the reduction index). Atom indices are global. Arithmetic is
+// FP64 (bonded_terms.h), accumulation 64-bit fixed point at 2^36.
 #include "bonded.h"
 #include "bonded_terms.h"
 #include "fixed_point.h"
@@ -10,8 +11,8 @@ namespace tmamd {
 
 namespace {
 
-__device__ __forceinline__ void add_fixed(uint64_t *p, float v) {
-    atomicAdd(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(f32_to_fixed(v, FIXED_SCALE_F)));
+__device__ __forceinline__ void add_fixed(uint64_t *p, double v) {
+    atomicAdd(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(f64_to_fixed(v, FIXED_SCALE)));
 }
 
 __device__ __forceinline__ void term_context(const BondedTerms &t, int term, const double *lambdas, double lambda0,
@@ -35,9 +36,8 @@ __global__ void k_harmonic_bond(const BondedTerms t, const double *__restrict__ 
     const int j = t.idx[b * 2 + 1];
     const int mult = t.mult ? t.mult[b] : 0;
     const int off = t.offset ? t.offset[b] : 1;
-    const float pf = lam_prefactor(mult, off, lam);
-    const BondOut o = bond_term(x + i * 3, x + j * 3, static_cast<float>(t.params[b * 2 + 0]),
-                                static_cast<float>(t.params[b * 2 + 1]), mult, pf);
+    const double pf = lam_prefactor<double>(mult, off, lam);
+    const BondOut<double> o = bond_term<double>(x + i * 3, x + j * 3, t.params[b * 2 + 0], t.params[b * 2 + 1], mult, pf);
     if (du_dx) {
         for (int d = 0; d < 3; d++) {
             add_fixed(du_dx + i * 3 + d, o.g[0][d]);
@@ -72,9 +72,9 @@ __global__ void k_harmonic_angle(const BondedTerms t, const double *__restrict__
     const int k = t.idx[a * 3 + 2];
     const int mult = t.mult ? t.mult[a] : 0;
     const int off = t.offset ? t.offset[a] : 1;
-    const float pf = lam_prefactor(mult, off, lam);
-    const AngleOut o = angle_term(x + i * 3, x + j * 3, x + k * 3, static_cast<float>(t.params[a * 2 + 0]),
-                                  static_cast<float>(t.params[a * 2 + 1]), mult, pf);
+    const double pf = lam_prefactor<double>(mult, off, lam);
+    const AngleOut<double> o =
+        angle_term<double>(x + i * 3, x + j * 3, x + k * 3, t.params[a * 2 + 0], t.params[a * 2 + 1], mult, pf);
     if (du_dx) {
         for (int d = 0; d < 3; d++) {
             add_fixed(du_dx + i * 3 + d, o.g[0][d]);
@@ -111,10 +111,9 @@ __global__ void k_periodic_torsion(const BondedTerms t, const double *__restrict
     const int l = t.idx[q * 4 + 3];
     const int mult = t.mult ? t.mult[q] : 0;
     const int off = t.offset ? t.offset[q] : 1;
-    const float pf = lam_prefactor(mult, off, lam);
-    const TorsionOut o = torsion_term(x + i * 3, x + j * 3, x + k * 3, x + l * 3, static_cast<float>(t.params[q * 3 + 0]),
-                                      static_cast<float>(t.params[q * 3 + 1]), static_cast<float>(t.params[q * 3 + 2]),
-                                      mult, pf);
+    const double pf = lam_prefactor<double>(mult, off, lam);
+    const TorsionOut<double> o = torsion_term<double>(x + i * 3, x + j * 3, x + k * 3, x + l * 3, t.params[q * 3 + 0],
+                                                      t.params[q * 3 + 1], t.params[q * 3 + 2], mult, pf);
     if (du_dx) {
         const int at[4] = {i, j, k, l};
         for (int a = 0; a < 4; a++) {

@@ -532,6 +532,7 @@ Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
             std::copy(nb->plane_.begin(), nb->plane_.end(), plane.begin() + I.atom0[s]);
             std::copy(nb->offset_.begin(), nb->offset_.end(), offset.begin() + I.atom0[s]);
         }
+        I.nb.nb_margin = NB_LIST_MARGIN;
         I.nb.init(ys, subset);
         upload_vec(I.nb_params, params);
         upload_vec(I.nb_plane, plane);

@@ -11,8 +11,9 @@
 // (nonbonded.py:214-225 masks eps_ij == 0 pairs, gradients included).
 //
 // Exclusions cancel exactly only if both kernels produce identical bits for
-// the same pair, so every kernel that evaluates a pair calls these functions,
-// and all translation units are compiled with -ffp-contract=off.
+// the same pair, so every kernel that evaluates a pair calls these functions
+// with the same inputs (FP64 coordinates and w, FP32 parameters), and all
+// translation units are compiled with -ffp-contract=off.
 #pragma once
 
 #include <cmath>
@@ -23,92 +24,108 @@
 namespace tmamd {
 
 // erfc(x) = exp(-x^2) * P(1 / (1 + ERFC_P x)); coefficients from erfc_fit.py
-// (approximation error 5.4e-9 relative on [0, 3], below FP32 rounding).
-constexpr float ERFC_P = 0.3400000035762787f;
-constexpr float ERFC_C0 = 0.0032670979853719473f;
-constexpr float ERFC_C1 = 0.15022893249988556f;
-constexpr float ERFC_C2 = 0.4216855764389038f;
-constexpr float ERFC_C3 = -0.5369299054145813f;
-constexpr float ERFC_C4 = 1.532942295074463f;
-constexpr float ERFC_C5 = -1.497012972831726f;
-constexpr float ERFC_C6 = 1.1914864778518677f;
-constexpr float ERFC_C7 = -0.26566749811172485f;
-constexpr float TWO_OVER_SQRT_PI_F = 1.1283791670955126f;
+// (approximation error 5.4e-9 relative on [0, 3]).
+constexpr double ERFC_P = 0.3400000035762787;
+constexpr double ERFC_C0 = 0.0032670979853719473;
+constexpr double ERFC_C1 = 0.15022893249988556;
+constexpr double ERFC_C2 = 0.4216855764389038;
+constexpr double ERFC_C3 = -0.5369299054145813;
+constexpr double ERFC_C4 = 1.532942295074463;
+constexpr double ERFC_C5 = -1.497012972831726;
+constexpr double ERFC_C6 = 1.1914864778518677;
+constexpr double ERFC_C7 = -0.26566749811172485;
+constexpr double TWO_OVER_SQRT_PI = 1.1283791670955126;
 
 TM_HD inline float min_image_f(float d, float box, float inv_box) { return d - box * std::rint(d * inv_box); }
+TM_HD inline double min_image_d(double d, double box, double inv_box) { return d - box * std::rint(d * inv_box); }
+
+// 1/sqrt(a): FP32 hardware seed, one Newton step in FP64 (relative error ~1e-14).
+TM_HD inline double rsqrt_nr(double a) {
+    const double y = static_cast<double>(TM_RSQF(static_cast<float>(a)));
+    return y * (1.5 - (0.5 * a) * (y * y));
+}
+
+// 1/a: FP32 hardware seed, one Newton step in FP64.
+TM_HD inline double rcp_nr(double a) {
+    const double y = static_cast<double>(TM_RCPF(static_cast<float>(a)));
+    return y * (2.0 - a * y);
+}
 
 struct PairTerms {
-    float prefactor; // (dU/dd) / d : du/dx_i = prefactor * (x_i - x_j)
-    float u;         // pair energy (valid when requested)
-    float erfc_inv_d; // erfc(beta d) / d, the charge-derivative factor
-    float sig_grad;  // dU/dsig_ij (LJ pairs only)
-    float eps_grad;  // dU/deps_ij (LJ pairs only)
+    double prefactor;  // (dU/dd) / d : du/dx_i = prefactor * (x_i - x_j)
+    double u;          // pair energy (valid when requested)
+    double erfc_inv_d; // erfc(beta d) / d, the charge-derivative factor
+    double sig_grad;   // dU/dsig_ij (LJ pairs only)
+    double eps_grad;   // dU/deps_ij (LJ pairs only)
     bool lj;
 };
 
+// Pair terms from the squared (4D) distance. Mixed precision chosen for the
+// 1e-5 per-atom force target: FP64 everywhere except exp(-x^2), which is the
+// FP32 hardware exp of x^2 rounded to FP32 with the rounding remainder folded
+// back in FP64 (relative error ~1 FP32 ulp), and the FP32 rsq/rcp seeds that
+// one FP64 Newton step refines. Parameters arrive as FP32 values (the
+// reference's per-atom parameter precision).
 template <bool COMPUTE_U, bool COMPUTE_DP>
 TM_HD inline PairTerms nb_pair_terms(
-    float d2,
-    float qi,
-    float qj,
-    float sig_i,
-    float sig_j,
-    float eps_i,
-    float eps_j,
-    float beta,
-    float scale_q,
-    float scale_lj) {
+    double d2,
+    double qi,
+    double qj,
+    double sig_i,
+    double sig_j,
+    double eps_i,
+    double eps_j,
+    double beta,
+    double scale_q,
+    double scale_lj) {
     PairTerms o;
-    const float inv_d = TM_RSQF(d2);
-    const float d = d2 * inv_d;
-    const float inv_d2 = inv_d * inv_d;
+    const double inv_d = rsqrt_nr(d2);
+    const double d = d2 * inv_d;
+    const double inv_d2 = inv_d * inv_d;
 
-    // erfc(beta d) and exp(-(beta d)^2), the latter with the rounding error of
-    // x^2 folded back in (x2lo is the exact low part of x*x).
-    const float x = beta * d;
-    const float x2 = x * x;
-    const float x2lo = __builtin_fmaf(x, x, -x2);
-    float e = std::exp(-x2);
-    e = e - e * x2lo;
-    const float t = TM_RCPF(1.0f + ERFC_P * x);
-    float poly = ERFC_C7;
-    poly = poly * t + ERFC_C6;
-    poly = poly * t + ERFC_C5;
-    poly = poly * t + ERFC_C4;
-    poly = poly * t + ERFC_C3;
-    poly = poly * t + ERFC_C2;
-    poly = poly * t + ERFC_C1;
-    poly = poly * t + ERFC_C0;
-    const float erfc_x = e * poly;
+    const double x = beta * d;
+    const double x2 = x * x;
+    const float x2f = static_cast<float>(x2);
+    double e = static_cast<double>(std::exp(-x2f));
+    e = e - e * (x2 - static_cast<double>(x2f));
+    const double t = rcp_nr(1.0 + ERFC_P * x);
+    double poly = ERFC_C7;
+    poly = __builtin_fma(poly, t, ERFC_C6);
+    poly = __builtin_fma(poly, t, ERFC_C5);
+    poly = __builtin_fma(poly, t, ERFC_C4);
+    poly = __builtin_fma(poly, t, ERFC_C3);
+    poly = __builtin_fma(poly, t, ERFC_C2);
+    poly = __builtin_fma(poly, t, ERFC_C1);
+    poly = __builtin_fma(poly, t, ERFC_C0);
+    const double erfc_x = e * poly;
 
-    const float sq = scale_q * (qi * qj);
-    const float es_factor = -inv_d2 * (TWO_OVER_SQRT_PI_F * x * e + erfc_x);
+    const double sq = scale_q * (qi * qj);
     o.erfc_inv_d = inv_d * erfc_x;
-    o.prefactor = sq * inv_d * es_factor;
-    o.u = 0.0f;
+    o.prefactor = -(sq * inv_d * inv_d2) * ((TWO_OVER_SQRT_PI * x + poly) * e);
+    o.u = 0.0;
     if (COMPUTE_U) {
         o.u = sq * o.erfc_inv_d;
     }
-    o.sig_grad = 0.0f;
-    o.eps_grad = 0.0f;
-    o.lj = (eps_i != 0.0f) && (eps_j != 0.0f);
+    o.sig_grad = 0.0;
+    o.eps_grad = 0.0;
+    o.lj = (eps_i != 0.0) && (eps_j != 0.0);
     if (o.lj) {
-        const float eps_ij = eps_i * eps_j;
-        const float sig_ij = sig_i + sig_j;
-        const float s1 = sig_ij * inv_d;
-        const float s2 = s1 * s1;
-        const float s4 = s2 * s2;
-        const float s6 = s4 * s2;
-        const float se = scale_lj * eps_ij;
-        o.prefactor = o.prefactor - se * (s6 * inv_d2) * (s6 * 48.0f - 24.0f);
+        const double eps_ij = eps_i * eps_j;
+        const double sig_ij = sig_i + sig_j;
+        const double s1 = sig_ij * inv_d;
+        const double s2 = s1 * s1;
+        const double s4 = s2 * s2;
+        const double s6 = s4 * s2;
+        const double se = scale_lj * eps_ij;
+        o.prefactor = o.prefactor - se * (s6 * inv_d2) * (s6 * 48.0 - 24.0);
         if (COMPUTE_U || COMPUTE_DP) {
-            const float lj_shape = (s6 - 1.0f) * s6;
+            const double lj_shape = (s6 - 1.0) * s6;
             if (COMPUTE_U) {
-                o.u = o.u + 4.0f * se * lj_shape;
+                o.u = o.u + 4.0 * se * lj_shape;
             }
             if (COMPUTE_DP) {
-                o.sig_grad = 24.0f * se * (sig_ij * s4 * inv_d2) * (2.0f * s6 - 1.0f);
-                o.eps_grad = 4.0f * scale_lj * lj_shape;
+                o.sig_grad = 24.0 * se * (sig_ij * s4 * inv_d2) * (2.0 * s6 - 1.0);
+                o.eps_grad = 4.0 * scale_lj * lj_shape;
             }
         }
     }

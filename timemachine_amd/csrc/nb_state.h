@@ -9,11 +9,12 @@
 //
 // HBM layout per system s (all arrays are global, offsets are in NbSys):
 //   x, params, plane, offset      [atom0 .. atom0 + n_atoms)         caller / engine arrays, original order
-//   perm, pos (float4), par (float4) [slot0 .. slot0 + nblocks*64)   sorted order, padded to whole tiles
+//   perm, pos (double4), par (float4) [slot0 .. slot0 + nblocks*64)  sorted order, padded to whole tiles
 //   bb_ctr, bb_ext (float4)       [block0 .. block0 + nblocks)
 //   tile_row, tile_atoms[64]      [tile0 .. tile0 + tile_cap)
-// pos = (x, y, z, w) and par = (q, sig, eps, dw/dlambda) are the FP32 values
-// the pair arithmetic consumes, so one 16-byte load fetches each.
+// pos = (x, y, z, w) in FP64 and par = (q, sig, eps, dw/dlambda) in FP32 are the
+// values the pair arithmetic consumes (FP64 coordinate differences keep the
+// per-atom force error under 1e-5; parameters keep the reference's FP32).
 #pragma once
 
 #include <vector>
@@ -23,6 +24,7 @@
 namespace tmamd {
 
 constexpr uint32_t NB_INVALID = 0xFFFFFFFFu;
+constexpr double NB_LIST_MARGIN = 1e-5; // nm, see NbState::nb_margin
 
 // Status bits raised on the device (box checks of
 // timemachine/cpp/src/nonbonded_all_pairs.cu:315-329, tile buffer overflow).
@@ -45,7 +47,7 @@ struct NbSys {
     int pad0;
     float beta;
     float cutoff;
-    float nb_cutoff; // cutoff + padding, FP32 (neighborlist predicate)
+    float nb_cutoff; // neighborlist radius, FP32 predicate (cutoff + padding, plus a margin for potentials)
     float pad1;
     double cutoff_d;
     double padding_d;
@@ -66,6 +68,10 @@ struct NbState {
     int max_blocks = 0;
     int max_slots = 0;
     int max_atoms = 0;
+    // Added to cutoff + padding for the neighborlist radius. Potentials use a
+    // small positive margin so the FP32 list is a strict superset of the pairs
+    // the FP64 pair test accepts; the Neighborlist query API uses 0 (exact set).
+    double nb_margin = 0.0;
     std::vector<NbSys> h_sys;
 
     DeviceArray<NbSys> sys;
@@ -78,7 +84,7 @@ struct NbState {
     DeviceArray<uint32_t> tile_atoms; // [total_tiles * 64]
     DeviceArray<uint32_t> perm;   // [total_slots] sorted slot -> local atom index
     DeviceArray<uint32_t> keys;   // [total_slots] sort keys (scratch)
-    DeviceArray<float4> pos;      // [total_slots]
+    DeviceArray<double4> pos;     // [total_slots] x, y, z, w (FP64)
     DeviceArray<float4> par;      // [total_slots]
     DeviceArray<float4> bb_ctr;   // [total_blocks]
     DeviceArray<float4> bb_ext;   // [total_blocks]

@@ -64,6 +64,12 @@ __device__ uint32_t hilbert_index(uint32_t x, uint32_t y, uint32_t z) {
     return key;
 }
 
+// FP32 coordinates of the neighborlist predicate (the oracle rounds the same way).
+__device__ __forceinline__ float4 to_f32(const double4 &p) {
+    return make_float4(static_cast<float>(p.x), static_cast<float>(p.y), static_cast<float>(p.z),
+                       static_cast<float>(p.w));
+}
+
 __device__ __forceinline__ uint32_t cell_of(double v, double box) {
     double f = v / box;
     f -= floor(f);
@@ -285,7 +291,7 @@ __global__ void k_gather(
     const double *__restrict__ lambdas,
     const double lambda0,
     const uint32_t *__restrict__ perm,
-    float4 *__restrict__ pos,
+    double4 *__restrict__ pos,
     float4 *__restrict__ par) {
     const int s = blockIdx.y;
     const NbSys S = sys[s];
@@ -295,7 +301,7 @@ __global__ void k_gather(
     }
     const uint32_t atom = perm[S.slot0 + slot];
     if (atom == NB_INVALID) {
-        pos[S.slot0 + slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+        pos[S.slot0 + slot] = make_double4(0.0, 0.0, 0.0, 0.0);
         par[S.slot0 + slot] = make_float4(0.f, 0.f, 0.f, 0.f);
         return;
     }
@@ -305,8 +311,7 @@ __global__ void k_gather(
     const int of = offset ? offset[g] : 0;
     const double w = (pl + of * lam) * S.cutoff_d;
     const double dw_dl = of * S.cutoff_d;
-    pos[S.slot0 + slot] = make_float4(static_cast<float>(x[g * 3 + 0]), static_cast<float>(x[g * 3 + 1]),
-                                      static_cast<float>(x[g * 3 + 2]), static_cast<float>(w));
+    pos[S.slot0 + slot] = make_double4(x[g * 3 + 0], x[g * 3 + 1], x[g * 3 + 2], w);
     if (params) {
         par[S.slot0 + slot] = make_float4(static_cast<float>(params[g * 3 + 0]), static_cast<float>(params[g * 3 + 1]),
                                           static_cast<float>(params[g * 3 + 2]), static_cast<float>(dw_dl));
@@ -320,7 +325,7 @@ __global__ __launch_bounds__(256) void k_block_bounds(
     const NbSys *__restrict__ sys,
     const double *__restrict__ boxes,
     const int *__restrict__ rebuild,
-    const float4 *__restrict__ pos,
+    const double4 *__restrict__ pos,
     float4 *__restrict__ bb_ctr,
     float4 *__restrict__ bb_ext) {
     const int s = blockIdx.y;
@@ -339,11 +344,11 @@ __global__ __launch_bounds__(256) void k_block_bounds(
     const float ibx = 1.0f / bx;
     const float iby = 1.0f / by;
     const float ibz = 1.0f / bz;
-    const float4 ref = pos[S.slot0 + b * TILE];
+    const float4 ref = to_f32(pos[S.slot0 + b * TILE]);
     const int slot = b * TILE + lane;
     float px = ref.x, py = ref.y, pz = ref.z;
     if (slot < S.k) {
-        const float4 p = pos[S.slot0 + slot];
+        const float4 p = to_f32(pos[S.slot0 + slot]);
         px = ref.x + min_image_f(p.x - ref.x, bx, ibx);
         py = ref.y + min_image_f(p.y - ref.y, by, iby);
         pz = ref.z + min_image_f(p.z - ref.z, bz, ibz);
@@ -371,7 +376,7 @@ __global__ __launch_bounds__(256) void k_find_tiles(
     const NbSys *__restrict__ sys,
     const double *__restrict__ boxes,
     const int *__restrict__ rebuild,
-    const float4 *__restrict__ pos,
+    const double4 *__restrict__ pos,
     const float4 *__restrict__ bb_ctr,
     const float4 *__restrict__ bb_ext,
     int *__restrict__ tile_count,
@@ -407,7 +412,7 @@ __global__ __launch_bounds__(256) void k_find_tiles(
 
     const int i = r * TILE + lane;
     const bool row_valid = i < S.k;
-    const float4 pi = row_valid ? pos[S.slot0 + i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 pi = row_valid ? to_f32(pos[S.slot0 + i]) : make_float4(0.f, 0.f, 0.f, 0.f);
     rows[lane] = pi;
     wave_lds_sync();
     const float4 rc = bb_ctr[S.block0 + r];
@@ -446,7 +451,7 @@ __global__ __launch_bounds__(256) void k_find_tiles(
             bool pending = false;
             float4 pj = make_float4(0.f, 0.f, 0.f, 0.f);
             if (col_valid && rmask) {
-                pj = pos[S.slot0 + j];
+                pj = to_f32(pos[S.slot0 + j]);
                 const float gx = box_gap(pj.x - rc.x, re.x, bx, ibx);
                 const float gy = box_gap(pj.y - rc.y, re.y, by, iby);
                 const float gz = box_gap(pj.z - rc.z, re.z, bz, ibz);
@@ -583,7 +588,7 @@ void NbState::init(const std::vector<NbSys> &sys_in, const std::vector<int> &sub
         y.block0 = total_blocks;
         y.tile0 = total_tiles;
         y.tile_cap = nb_tile_capacity(y.k);
-        y.nb_cutoff = static_cast<float>(y.cutoff_d + y.padding_d);
+        y.nb_cutoff = static_cast<float>(y.cutoff_d + y.padding_d + nb_margin);
         y.cutoff = static_cast<float>(y.cutoff_d);
         total_slots += y.nblocks * TILE;
         total_blocks += y.nblocks;

@@ -34,9 +34,9 @@ __device__ __forceinline__ void tile_body(
     const int s,
     const int r,
     const uint32_t jl0,
-    const float4 pi,
+    const double4 pi,
     const float4 ai,
-    float4 pj,
+    double4 pj,
     float4 aj,
     const uint32_t *__restrict__ perm,
     const double *__restrict__ boxes,
@@ -49,14 +49,14 @@ __device__ __forceinline__ void tile_body(
     const uint32_t k = static_cast<uint32_t>(S.k);
     const bool vi = i < k;
 
-    const float bx = static_cast<float>(boxes[s * 9 + 0]);
-    const float by = static_cast<float>(boxes[s * 9 + 4]);
-    const float bz = static_cast<float>(boxes[s * 9 + 8]);
-    const float ibx = 1.0f / bx;
-    const float iby = 1.0f / by;
-    const float ibz = 1.0f / bz;
-    const float cut2 = S.cutoff * S.cutoff;
-    const float beta = S.beta;
+    const double bx = boxes[s * 9 + 0];
+    const double by = boxes[s * 9 + 4];
+    const double bz = boxes[s * 9 + 8];
+    const double ibx = 1.0 / bx;
+    const double iby = 1.0 / by;
+    const double ibz = 1.0 / bz;
+    const double cut2 = S.cutoff_d * S.cutoff_d;
+    const double beta = static_cast<double>(S.beta);
 
     uint64_t gix = 0, giy = 0, giz = 0;
     uint64_t gjx = 0, gjy = 0, gjz = 0;
@@ -67,22 +67,22 @@ __device__ __forceinline__ void tile_body(
 
 #pragma unroll 2
     for (int round = 0; round < TILE; round++) {
-        const float dx = min_image_f(pi.x - pj.x, bx, ibx);
-        const float dy = min_image_f(pi.y - pj.y, by, iby);
-        const float dz = min_image_f(pi.z - pj.z, bz, ibz);
-        float d2 = dx * dx + dy * dy + dz * dz;
-        float dw = 0.0f;
+        const double dx = min_image_d(pi.x - pj.x, bx, ibx);
+        const double dy = min_image_d(pi.y - pj.y, by, iby);
+        const double dz = min_image_d(pi.z - pj.z, bz, ibz);
+        double d2 = dx * dx + dy * dy + dz * dz;
+        double dw = 0.0;
         if (ALCH) {
             dw = pi.w - pj.w;
             d2 = d2 + dw * dw;
         }
         const bool valid = vi && (jc < k) && (i < jc);
         if (valid && d2 < cut2) {
-            const PairTerms T = nb_pair_terms<U, DP>(d2, ai.x, aj.x, ai.y, aj.y, ai.z, aj.z, beta, 1.0f, 1.0f);
+            const PairTerms T = nb_pair_terms<U, DP>(d2, ai.x, aj.x, ai.y, aj.y, ai.z, aj.z, beta, 1.0, 1.0);
             if (DX) {
-                const uint64_t fx = f32_to_fixed(T.prefactor * dx, FIXED_SCALE_F);
-                const uint64_t fy = f32_to_fixed(T.prefactor * dy, FIXED_SCALE_F);
-                const uint64_t fz = f32_to_fixed(T.prefactor * dz, FIXED_SCALE_F);
+                const uint64_t fx = f64_to_fixed(T.prefactor * dx, FIXED_SCALE);
+                const uint64_t fy = f64_to_fixed(T.prefactor * dy, FIXED_SCALE);
+                const uint64_t fz = f64_to_fixed(T.prefactor * dz, FIXED_SCALE);
                 gix += fx;
                 giy += fy;
                 giz += fz;
@@ -91,21 +91,22 @@ __device__ __forceinline__ void tile_body(
                 gjz -= fz;
             }
             if (DP) {
-                gqi += f32_to_fixed((1.0f * aj.x) * T.erfc_inv_d, FIXED_SCALE_F);
-                gqj += f32_to_fixed((1.0f * ai.x) * T.erfc_inv_d, FIXED_SCALE_F);
+                gqi += f64_to_fixed((1.0 * aj.x) * T.erfc_inv_d, FIXED_SCALE);
+                gqj += f64_to_fixed((1.0 * ai.x) * T.erfc_inv_d, FIXED_SCALE);
                 if (T.lj) {
-                    const uint64_t gs = f32_to_fixed(T.sig_grad, FIXED_SCALE_DSIG_F);
+                    const uint64_t gs = f64_to_fixed(T.sig_grad, FIXED_SCALE_DSIG);
                     gsi += gs;
                     gsj += gs;
-                    gei += f32_to_fixed(T.eps_grad * aj.z, FIXED_SCALE_DEPS_F);
-                    gej += f32_to_fixed(T.eps_grad * ai.z, FIXED_SCALE_DEPS_F);
+                    gei += f64_to_fixed(T.eps_grad * aj.z, FIXED_SCALE_DEPS);
+                    gej += f64_to_fixed(T.eps_grad * ai.z, FIXED_SCALE_DEPS);
                 }
             }
             if (DL && ALCH) {
-                dlacc += f32_to_fixed(dw * T.prefactor * (ai.w - aj.w), FIXED_SCALE_F);
+                dlacc += f64_to_fixed(dw * T.prefactor * (static_cast<double>(ai.w) - static_cast<double>(aj.w)),
+                                      FIXED_SCALE);
             }
             if (U) {
-                uacc += f32_to_fixed(T.u, FIXED_SCALE_F);
+                uacc += f64_to_fixed(T.u, FIXED_SCALE);
             }
         }
         pj.x = rot1(pj.x);
@@ -186,7 +187,7 @@ __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded_tiles(
     const int *__restrict__ tile_row,
     const uint32_t *__restrict__ tile_atoms,
     const uint32_t *__restrict__ perm,
-    const float4 *__restrict__ pos,
+    const double4 *__restrict__ pos,
     const float4 *__restrict__ par,
     const double *__restrict__ boxes,
     const int *__restrict__ status,
@@ -218,12 +219,12 @@ __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded_tiles(
         const int r = tile_row[kt];
         const uint32_t jl = tile_atoms[static_cast<size_t>(kt) * TILE + lane];
         const int i = r * TILE + lane;
-        const float4 pi = pos[Sd.slot0 + i];
+        const double4 pi = pos[Sd.slot0 + i];
         const float4 ai = par[Sd.slot0 + i];
         const bool vj = jl < static_cast<uint32_t>(Sd.k);
-        const float4 pj = vj ? pos[Sd.slot0 + jl] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const double4 pj = vj ? pos[Sd.slot0 + jl] : make_double4(0.0, 0.0, 0.0, 0.0);
         const float4 aj = vj ? par[Sd.slot0 + jl] : make_float4(0.f, 0.f, 0.f, 0.f);
-        const bool alch = ballot(pi.w != 0.0f || pj.w != 0.0f) != 0;
+        const bool alch = ballot(pi.w != 0.0 || pj.w != 0.0) != 0;
         if (alch) {
             tile_body<true, U, DX, DL, DP>(Sd, s, r, jl, pi, ai, pj, aj, perm, boxes, du_dx, du_dp, du_dl, u);
         } else {
@@ -233,7 +234,7 @@ __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded_tiles(
 }
 
 using TileKernel = void (*)(const NbSys *, int, const int *, const int *, const uint32_t *, const uint32_t *,
-                            const float4 *, const float4 *, const double *, const int *, uint64_t *, uint64_t *,
+                            const double4 *, const float4 *, const double *, const int *, uint64_t *, uint64_t *,
                             uint64_t *, uint64_t *);
 
 // index bits: 8 = u, 4 = du_dx, 2 = du_dl, 1 = du_dp
@@ -263,42 +264,41 @@ __global__ __launch_bounds__(256) void k_pair_list(
     const float beta = a.betas ? a.betas[s] : a.beta0;
     const int red = a.red_idx ? a.red_idx[s] : 0;
 
-    const float bx = static_cast<float>(box[0]);
-    const float by = static_cast<float>(box[4]);
-    const float bz = static_cast<float>(box[8]);
-    const float ibx = 1.0f / bx;
-    const float iby = 1.0f / by;
-    const float ibz = 1.0f / bz;
+    const double bx = box[0];
+    const double by = box[4];
+    const double bz = box[8];
+    const double ibx = 1.0 / bx;
+    const double iby = 1.0 / by;
+    const double ibz = 1.0 / bz;
 
     const int pli = a.plane ? a.plane[gi] : 0, plj = a.plane ? a.plane[gj] : 0;
     const int ofi = a.offset ? a.offset[gi] : 0, ofj = a.offset ? a.offset[gj] : 0;
-    const float wi = static_cast<float>((pli + ofi * lam) * cutoff_d);
-    const float wj = static_cast<float>((plj + ofj * lam) * cutoff_d);
-    const float dwdl_i = static_cast<float>(ofi * cutoff_d);
-    const float dwdl_j = static_cast<float>(ofj * cutoff_d);
+    const double wi = (pli + ofi * lam) * cutoff_d;
+    const double wj = (plj + ofj * lam) * cutoff_d;
+    const double dwdl_i = static_cast<double>(static_cast<float>(ofi * cutoff_d));
+    const double dwdl_j = static_cast<double>(static_cast<float>(ofj * cutoff_d));
 
-    const float dx = min_image_f(static_cast<float>(a.x[gi * 3 + 0]) - static_cast<float>(a.x[gj * 3 + 0]), bx, ibx);
-    const float dy = min_image_f(static_cast<float>(a.x[gi * 3 + 1]) - static_cast<float>(a.x[gj * 3 + 1]), by, iby);
-    const float dz = min_image_f(static_cast<float>(a.x[gi * 3 + 2]) - static_cast<float>(a.x[gj * 3 + 2]), bz, ibz);
-    const float dw = wi - wj;
-    float d2 = dx * dx + dy * dy + dz * dz;
+    const double dx = min_image_d(a.x[gi * 3 + 0] - a.x[gj * 3 + 0], bx, ibx);
+    const double dy = min_image_d(a.x[gi * 3 + 1] - a.x[gj * 3 + 1], by, iby);
+    const double dz = min_image_d(a.x[gi * 3 + 2] - a.x[gj * 3 + 2], bz, ibz);
+    const double dw = wi - wj;
+    double d2 = dx * dx + dy * dy + dz * dz;
     d2 = d2 + dw * dw;
-    const float cutoff = static_cast<float>(cutoff_d);
-    if (!(d2 < cutoff * cutoff)) {
+    if (!(d2 < cutoff_d * cutoff_d)) {
         return;
     }
-    const float qi = static_cast<float>(a.params[gi * 3 + 0]), qj = static_cast<float>(a.params[gj * 3 + 0]);
-    const float si = static_cast<float>(a.params[gi * 3 + 1]), sj = static_cast<float>(a.params[gj * 3 + 1]);
-    const float ei = static_cast<float>(a.params[gi * 3 + 2]), ej = static_cast<float>(a.params[gj * 3 + 2]);
-    const float scale_q = static_cast<float>(a.scales[p * 2 + 0]);
-    const float scale_lj = static_cast<float>(a.scales[p * 2 + 1]);
-    const PairTerms T = nb_pair_terms<U, DP>(d2, qi, qj, si, sj, ei, ej, beta, scale_q, scale_lj);
+    const double qi = static_cast<float>(a.params[gi * 3 + 0]), qj = static_cast<float>(a.params[gj * 3 + 0]);
+    const double si = static_cast<float>(a.params[gi * 3 + 1]), sj = static_cast<float>(a.params[gj * 3 + 1]);
+    const double ei = static_cast<float>(a.params[gi * 3 + 2]), ej = static_cast<float>(a.params[gj * 3 + 2]);
+    const double scale_q = a.scales[p * 2 + 0];
+    const double scale_lj = a.scales[p * 2 + 1];
+    const PairTerms T = nb_pair_terms<U, DP>(d2, qi, qj, si, sj, ei, ej, static_cast<double>(beta), scale_q, scale_lj);
     const bool neg = a.negated != 0;
     auto acc = [neg](uint64_t *dst, uint64_t v) { atomic_add_u64(dst, neg ? (0ull - v) : v); };
     if (DX) {
-        const uint64_t fx = f32_to_fixed(T.prefactor * dx, FIXED_SCALE_F);
-        const uint64_t fy = f32_to_fixed(T.prefactor * dy, FIXED_SCALE_F);
-        const uint64_t fz = f32_to_fixed(T.prefactor * dz, FIXED_SCALE_F);
+        const uint64_t fx = f64_to_fixed(T.prefactor * dx, FIXED_SCALE);
+        const uint64_t fy = f64_to_fixed(T.prefactor * dy, FIXED_SCALE);
+        const uint64_t fz = f64_to_fixed(T.prefactor * dz, FIXED_SCALE);
         acc(du_dx + gi * 3 + 0, fx);
         acc(du_dx + gi * 3 + 1, fy);
         acc(du_dx + gi * 3 + 2, fz);
@@ -307,24 +307,24 @@ __global__ __launch_bounds__(256) void k_pair_list(
         acc(du_dx + gj * 3 + 2, 0ull - fz);
     }
     if (DP) {
-        acc(du_dp + gi * 3 + 0, f32_to_fixed((scale_q * qj) * T.erfc_inv_d, FIXED_SCALE_F));
-        acc(du_dp + gj * 3 + 0, f32_to_fixed((scale_q * qi) * T.erfc_inv_d, FIXED_SCALE_F));
+        acc(du_dp + gi * 3 + 0, f64_to_fixed((scale_q * qj) * T.erfc_inv_d, FIXED_SCALE));
+        acc(du_dp + gj * 3 + 0, f64_to_fixed((scale_q * qi) * T.erfc_inv_d, FIXED_SCALE));
         if (T.lj) {
-            const uint64_t gs = f32_to_fixed(T.sig_grad, FIXED_SCALE_DSIG_F);
+            const uint64_t gs = f64_to_fixed(T.sig_grad, FIXED_SCALE_DSIG);
             acc(du_dp + gi * 3 + 1, gs);
             acc(du_dp + gj * 3 + 1, gs);
-            acc(du_dp + gi * 3 + 2, f32_to_fixed(T.eps_grad * ej, FIXED_SCALE_DEPS_F));
-            acc(du_dp + gj * 3 + 2, f32_to_fixed(T.eps_grad * ei, FIXED_SCALE_DEPS_F));
+            acc(du_dp + gi * 3 + 2, f64_to_fixed(T.eps_grad * ej, FIXED_SCALE_DEPS));
+            acc(du_dp + gj * 3 + 2, f64_to_fixed(T.eps_grad * ei, FIXED_SCALE_DEPS));
         }
     }
     if (DL) {
-        const bool vanilla = (wi == 0.0f) && (wj == 0.0f);
+        const bool vanilla = (wi == 0.0) && (wj == 0.0);
         if (!vanilla) {
-            acc(du_dl + red, f32_to_fixed(dw * T.prefactor * (dwdl_i - dwdl_j), FIXED_SCALE_F));
+            acc(du_dl + red, f64_to_fixed(dw * T.prefactor * (dwdl_i - dwdl_j), FIXED_SCALE));
         }
     }
     if (U) {
-        acc(u + red, f32_to_fixed(T.u, FIXED_SCALE_F));
+        acc(u + red, f64_to_fixed(T.u, FIXED_SCALE));
     }
 }
 

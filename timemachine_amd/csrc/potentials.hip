@@ -180,6 +180,7 @@ void NonbondedAllPairs::ensure_state() {
     y.beta = static_cast<float>(beta_);
     y.cutoff_d = cutoff_;
     y.padding_d = padding_;
+    st_.nb_margin = NB_LIST_MARGIN;
     st_.init({y}, subset_);
     dirty_ = false;
 }

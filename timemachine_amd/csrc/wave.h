@@ -41,6 +41,10 @@ __device__ __forceinline__ uint64_t rot1(uint64_t v) {
     return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
+__device__ __forceinline__ double rot1(double v) {
+    return __longlong_as_double(static_cast<long long>(rot1(static_cast<uint64_t>(__double_as_longlong(v)))));
+}
+
 __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
     uint32_t lo = static_cast<uint32_t>(__shfl_xor(static_cast<int>(static_cast<uint32_t>(v)), m, 64));
     uint32_t hi = static_cast<uint32_t>(__shfl_xor(static_cast<int>(static_cast<uint32_t>(v >> 32)), m, 64));

@@ -43,10 +43,15 @@ class _Handle:
         return self._h
 
     def __del__(self):
-        h = getattr(self, "_h", None)
-        rel = type(self)._release
-        if h is not None and h.value and rel is not None and _lib._lib is not None:
-            getattr(_lib._lib, rel)(h)
+        # At interpreter shutdown module globals may already be None.
+        try:
+            h = getattr(self, "_h", None)
+            rel = type(self)._release
+            lib = _lib._lib
+        except (AttributeError, TypeError):
+            return
+        if h is not None and h.value and rel is not None and lib is not None:
+            getattr(lib, rel)(h)
             self._h = ctypes.c_void_p()
 
 

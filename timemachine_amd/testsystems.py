@@ -169,8 +169,6 @@ def _place_waters(
         key = tuple((np.floor((p % side) / csize).astype(int) % ncell).tolist())
         cells.setdefault(key, []).append(p)
     for w in range(n_waters):
-        best = None
-        best_d = -1.0
         key0 = np.floor((sites[w] % side) / csize).astype(int) % ncell
         neigh = []
         for dx in (-1, 0, 1):
@@ -178,18 +176,17 @@ def _place_waters(
                 for dz in (-1, 0, 1):
                     k = tuple(((key0 + np.array([dx, dy, dz])) % ncell).tolist())
                     neigh.extend(cells.get(k, []))
-        neigh = np.array(neigh) if neigh else np.zeros((0, 3))
-        for _ in range(tries):
-            rot = _random_rotations(rng, 1)[0]
-            cand = sites[w] + frame @ rot.T
-            if len(neigh) == 0:
-                best = cand
-                break
-            d = _min_image_dist(cand, neigh, box_diag).min()
-            if d > best_d:
-                best_d, best = d, cand
-            if d >= clash:
-                break
+        rots = _random_rotations(rng, tries)
+        cands = sites[w][None, None, :] + np.einsum("tij,aj->tai", rots, frame)  # [tries, 3, 3]
+        if neigh:
+            nb_ = np.array(neigh)
+            d = cands[:, :, None, :] - nb_[None, None, :, :]
+            d -= box_diag * np.floor(d / box_diag + 0.5)
+            dmin = np.sqrt((d * d).sum(-1)).min(axis=(1, 2))
+            ok = dmin >= clash
+            best = cands[int(np.argmax(ok))] if ok.any() else cands[int(np.argmax(dmin))]
+        else:
+            best = cands[0]
         placed[w] = best
         for p in best:
             k = tuple((np.floor((p % side) / csize).astype(int) % ncell).tolist())
@@ -250,60 +247,69 @@ def water_box(side: float = 3.0, n_waters: int = 887, seed: int = 2022, jitter: 
 
 
 def _ligand(rng: np.random.Generator, n_heavy: int, n_total: int):
-    """Random self-avoiding tree of heavy atoms (valence <= 4) plus hydrogens."""
+    """Random self-avoiding tree of heavy atoms (valence <= 4) plus hydrogens.
+
+    Each new atom is bonded to a host and placed in a random direction that is
+    at least D13 from its 1-3 partners, D14 from its 1-4 partners and D_FAR
+    from every other atom, so no pair beyond 1-4 starts inside the LJ wall
+    (ligand sigma is 0.25-0.35 nm).
+    """
+    D13, D14, D_FAR = 0.17, 0.22, 0.26
     pos = [np.zeros(3)]
-    parent = [-1]
-    degree = [0]
-    bonds = []
-    bond_len = []
+    nbr = [[]]
+    bonds, bond_len = [], []
+
+    def place(host, lo, hi, max_deg, batch=128, rounds=8):
+        P = np.array(pos)
+        n13 = set(nbr[host])
+        n14 = {m for j in n13 for m in nbr[j]}
+        lim = np.full(len(pos), D_FAR)
+        for k in n14:
+            lim[k] = D14
+        for k in n13:
+            lim[k] = D13
+        lim[host] = 0.0
+        for _ in range(rounds):
+            v = rng.normal(size=(batch, 3))
+            v /= np.linalg.norm(v, axis=1, keepdims=True)
+            L = rng.uniform(lo, hi, size=batch)
+            cand = P[host] + L[:, None] * v
+            d = np.linalg.norm(cand[:, None, :] - P[None, :, :], axis=-1)
+            ok = np.all(d >= lim[None, :], axis=1)
+            if ok.any():
+                k = int(np.argmax(ok))
+                return cand[k], float(L[k])
+        return None, None
+
     while len(pos) < n_heavy:
-        cands = [i for i in range(len(pos)) if degree[i] < 3]
-        i = int(rng.choice(cands))
-        for _ in range(200):
-            L = rng.uniform(0.13, 0.15)
-            # keep the new bond roughly tetrahedral to the parent bond
-            v = rng.normal(size=3)
-            v /= np.linalg.norm(v)
-            if parent[i] >= 0:
-                u = pos[i] - pos[parent[i]]
-                u /= np.linalg.norm(u)
-                if np.dot(u, v) < 0.2:
-                    continue
-            p = pos[i] + L * v
-            if min((np.linalg.norm(p - q) for k, q in enumerate(pos) if k != i), default=np.inf) > 0.22:
-                break
-        else:
+        cands = [i for i in range(len(pos)) if len(nbr[i]) < 3]
+        host = int(rng.choice(cands))
+        p, L = place(host, 0.13, 0.15, 3)
+        if p is None:
             continue
         pos.append(p)
-        parent.append(i)
-        degree.append(1)
-        degree[i] += 1
-        bonds.append((i, len(pos) - 1))
+        nbr.append([host])
+        nbr[host].append(len(pos) - 1)
+        bonds.append((host, len(pos) - 1))
         bond_len.append(L)
     n_h = n_total - n_heavy
-    heavy_free = [i for i in range(n_heavy) for _ in range(max(0, 4 - degree[i]))]
-    rng.shuffle(heavy_free)
-    h_hosts = heavy_free[:n_h]
-    if len(h_hosts) < n_h:
-        raise ValueError("not enough free valence for hydrogens")
-    for i in h_hosts:
-        best, best_d = None, -1.0
-        for _ in range(100):
-            v = rng.normal(size=3)
-            v /= np.linalg.norm(v)
-            L = rng.uniform(0.10, 0.11)
-            p = pos[i] + L * v
-            d = min((np.linalg.norm(p - q) for k, q in enumerate(pos) if k != i), default=np.inf)
-            if d > best_d:
-                best, best_d = p, d
-            if d > 0.16:
-                break
-        pos.append(best)
-        parent.append(i)
-        degree.append(1)
-        degree[i] += 1
-        bonds.append((i, len(pos) - 1))
-        bond_len.append(np.linalg.norm(best - pos[i]))
+    added = 0
+    tries = 0
+    while added < n_h:
+        tries += 1
+        if tries > 20000:
+            raise ValueError("could not place hydrogens")
+        free = [i for i in range(n_heavy) if len(nbr[i]) < 4]
+        host = int(rng.choice(free))
+        p, L = place(host, 0.10, 0.11, 4)
+        if p is None:
+            continue
+        pos.append(p)
+        nbr.append([host])
+        nbr[host].append(len(pos) - 1)
+        bonds.append((host, len(pos) - 1))
+        bond_len.append(L)
+        added += 1
     return np.array(pos), np.array(bonds), np.array(bond_len)
 
 
@@ -341,7 +347,7 @@ def ligand_in_water(
     rng = np.random.default_rng(seed)
     lpos, lbonds, lbond_len = _ligand(rng, n_heavy, n_ligand)
     lpos = lpos - lpos.mean(0) + side / 2
-    wx = _place_waters(rng, side, n_waters, 0.02, avoid=lpos, avoid_radius=0.22, clash=0.17)
+    wx = _place_waters(rng, side, n_waters, 0.02, avoid=lpos, avoid_radius=0.28, clash=0.20)
     x = _f32_round(np.concatenate([lpos, wx]))
     NL = n_ligand
     N = x.shape[0]
