@@ -96,6 +96,15 @@ int tm_potential_execute_batch_host(tm_potential *p, int coord_batch, int N, int
                                     uint64_t *u);
 /* Potential::du_dp_fixed_to_float, potential.cu:227-231 / nonbonded_all_pairs.cu:435-450 */
 int tm_potential_du_dp_fixed_to_float(tm_potential *p, int N, int P, const uint64_t *du_dp, double *out);
+/* d(du/dlambda)/dp, accumulated into a caller-zeroed [P] buffer with the du_dp fixed-point
+ * exponents (decode with tm_potential_du_dp_fixed_to_float). No counterpart in the reference's
+ * custom ops: the reference obtains this mixed second derivative from its JAX potentials
+ * (timemachine/potentials/nonbonded.py:133-264 under jax.grad). It is the kernel behind
+ * gradients of lambda-window averages of du/dlambda. */
+int tm_potential_execute_dl_dp_device(tm_potential *p, int N, int P, const double *d_x, const double *d_p,
+                                      const double *d_box, double lambda, uint64_t *d_dl_dp, void *stream);
+int tm_potential_execute_dl_dp_host(tm_potential *p, int N, int P, const double *x, const double *params,
+                                    const double *box, double lambda, uint64_t *dl_dp);
 
 /* BoundPotential(potential, params), bound_potential.cu:6-12 */
 int tm_bound_potential_create(tm_potential *p, int ndim, const int32_t *shape, const double *params,
@@ -190,6 +199,22 @@ int tm_batch_profile(tm_batch *b, int reps, double *ms_out);
 int tm_batch_total_tiles(tm_batch *b, int *out);
 /* Workgroups of the persistent pair kernel (0 = default, CUs x 8). */
 int tm_batch_set_tile_grid(tm_batch *b, int grid);
+
+/* Gradient observer (no reference counterpart; the statistics behind gradients of
+ * lambda-window averages of A = du/dlambda with respect to forcefield parameters).
+ * Every `interval` steps of tm_batch_multiple_steps (0 = off), at the states where du_dl
+ * is stored with the same interval, each system's A, g = du/dp and h = d(du/dlambda)/dp
+ * are summed on the device.
+ * Setting the interval resets the sums. */
+int tm_batch_set_grad_observer(tm_batch *b, int interval);
+/* One observation of every system at its current state (no step). */
+int tm_batch_observe(tm_batch *b);
+/* Flat parameters of system s: nonbonded [N, 3], then bond [B, 2], angle [A, 2],
+ * torsion [T, 3] terms, in the order the system's potentials were bound. */
+int tm_batch_grad_size(tm_batch *b, int s, int *out);
+/* n frames, sum A, sum A^2 (scalars) and sum g, sum A g, sum h ([grad_size]). */
+int tm_batch_grad_stats(tm_batch *b, int s, int *n_frames, double *sum_a, double *sum_aa, double *sum_g,
+                        double *sum_ag, double *sum_h);
 
 #ifdef __cplusplus
 }

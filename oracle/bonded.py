@@ -75,3 +75,17 @@ def bonded(kind, x, params, lamb, idxs, mult=None, offset=None):
     gx, gp, gl = torch.autograd.grad(u, (xt, pt, lt), allow_unused=True)
     gl = 0.0 if gl is None else float(gl)
     return gx.numpy(), gp.numpy(), gl, float(u.detach())
+
+
+def bonded_dl_dp(kind, x, params, lamb, idxs, mult=None, offset=None):
+    """d(du/dlambda)/dparams in FP64 (Torch double backward); shaped like params."""
+    params = np.asarray(params, dtype=np.float64)
+    if len(np.asarray(idxs)) == 0:
+        return np.zeros_like(params)
+    xt, pt, lt = _t(x), _t(params, True), _t(lamb, True)
+    u = _ENERGY[kind](xt, pt, lt, idxs, mult, offset)
+    (gl,) = torch.autograd.grad(u, (lt,), create_graph=True)
+    if not gl.requires_grad:
+        return np.zeros_like(params)
+    (gp,) = torch.autograd.grad(gl, (pt,), allow_unused=True)
+    return np.zeros_like(params) if gp is None else gp.numpy()

@@ -161,6 +161,42 @@ def nonbonded_pairs(
     return _grads(fn, x, params, lamb)
 
 
+def _dl_dp(energy_fn, x, params, lamb):
+    """d(dU/dlambda)/dparams by Torch FP64 double backward (the mixed second
+    derivative jax.grad(jax.grad(U, 3), 1) of the reference's potentials)."""
+    xt = _t(x)
+    pt = _t(params, True)
+    lt = _t(lamb, True)
+    u = energy_fn(xt, pt, lt)
+    (gl,) = torch.autograd.grad(u, (lt,), create_graph=True)
+    if not gl.requires_grad:
+        return np.zeros_like(np.asarray(params, dtype=np.float64))
+    (gp,) = torch.autograd.grad(gl, (pt,), allow_unused=True)
+    return np.zeros_like(pt.detach().numpy()) if gp is None else gp.numpy()
+
+
+def nonbonded_pairs_dl_dp(
+    x, params, box, lamb, exclusion_idxs, scales, beta, cutoff, plane, offset, pairs: Optional[np.ndarray] = None
+) -> np.ndarray:
+    """d(du/dlambda)/dp [N, 3] of the energy of nonbonded_pairs (exclusions included)."""
+    x = np.asarray(x, dtype=np.float64)
+    if pairs is None:
+        pairs = candidate_pairs(x, box, cutoff)
+    box_diag = torch.tensor(np.diag(np.asarray(box)).copy(), dtype=torch.float64)
+    plane_t = torch.as_tensor(np.asarray(plane), dtype=torch.float64)
+    offset_t = torch.as_tensor(np.asarray(offset), dtype=torch.float64)
+    excl = np.asarray(exclusion_idxs, dtype=np.int64).reshape(-1, 2)
+    sc = torch.as_tensor(np.asarray(scales, dtype=np.float64).reshape(-1, 2))
+
+    def fn(xt, pt, lt):
+        u = _pair_energy(xt, pt, lt, pairs, box_diag, beta, cutoff, plane_t, offset_t)
+        if len(excl):
+            u = u - _pair_energy(xt, pt, lt, excl, box_diag, beta, cutoff, plane_t, offset_t, sc[:, 0], sc[:, 1])
+        return u
+
+    return _dl_dp(fn, x, params, lamb)
+
+
 def force_rel_err(test: np.ndarray, ref: np.ndarray) -> np.ndarray:
     """Per-atom |dF| / max(|F_ref|, 1) (tests/common.py:386-409, the OpenMM convention)."""
     num = np.linalg.norm(np.asarray(test) - np.asarray(ref), axis=-1)

@@ -117,6 +117,29 @@ def test_exclusion_cancels_exactly(gpu):
         assert np.sum(u, dtype=np.uint64) == 0 and np.sum(du_dl, dtype=np.uint64) == 0
 
 
+@pytest.mark.parametrize("lam", [0.0, 0.5])
+def test_tile_kernel_equals_pair_list_bitwise(ligand, lam):
+    """The all-pairs tile kernel and the one-thread-per-pair list kernel use the
+    same per-pair arithmetic, and fixed-point sums are exact: over the same set
+    of pairs (every pair within the 3D cutoff, a superset of the 4D set) all
+    four outputs must agree bit for bit."""
+    pairs = onb.candidate_pairs(ligand.x, ligand.box, ligand.cutoff).astype(np.int32)
+    ones = np.ones((len(pairs), 2))
+    allp = custom_ops.NonbondedAllPairs_f32(None, ligand.lambda_plane_idxs, ligand.lambda_offset_idxs, ligand.beta,
+                                            ligand.cutoff)
+    plist = custom_ops.NonbondedPairList_f32(None, pairs, ones, ligand.lambda_plane_idxs, ligand.lambda_offset_idxs,
+                                             ligand.beta, ligand.cutoff)
+    a = allp.execute_fixed_raw(ligand.x, ligand.nb_params, ligand.box, lam)
+    b = plist.execute_fixed_raw(ligand.x, ligand.nb_params, ligand.box, lam)
+    assert np.array_equal(a[0], b[0])  # du_dx
+    assert np.array_equal(a[1], b[1])  # du_dp
+    assert np.sum(a[2], dtype=np.uint64) == np.sum(b[2], dtype=np.uint64)  # du_dl
+    assert np.sum(a[3], dtype=np.uint64) == np.sum(b[3], dtype=np.uint64)  # u
+    ga = allp.execute_dl_dp(ligand.x, ligand.nb_params, ligand.box, lam)
+    gb = plist.execute_dl_dp(ligand.x, ligand.nb_params, ligand.box, lam)
+    assert np.array_equal(ga, gb)
+
+
 def test_selective_outputs_are_bitwise_consistent(ligand):
     """All 16 output selections give the same bits for what they compute."""
     pot = ligand.nonbonded().unbound_impl(F32)

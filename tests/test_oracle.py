@@ -275,3 +275,43 @@ def test_barostat_ideal_gas_acceptance():
     assert not barostat.accept(100.0, 0.5, 300.0)
     assert barostat.adapt_volume_scale(1.0, 1, 10, 27.0) == (1.0 / 1.1, 0, 0)
     assert barostat.adapt_volume_scale(1.0, 9, 10, 27.0) == (1.1, 0, 0)
+
+
+def test_dl_dp_matches_finite_differences_of_du_dl(rng):
+    """d(du/dlambda)/dp (double backward) vs central differences of the
+    oracle's du/dlambda, which is itself pinned above."""
+    x, p, box, lam, excl, sc, pl, of = _tiny_system(rng, N=16)
+    g = nonbonded.nonbonded_pairs_dl_dp(x, p, box, lam, excl, sc, 2.0, 1.2, pl, of)
+    assert np.abs(g).max() > 0
+
+    def dl(pp):
+        return nonbonded.nonbonded_pairs(x, pp, box, lam, excl, sc, 2.0, 1.2, pl, of)[2]
+
+    h = 1e-6
+    for i, k in [(0, 0), (1, 1), (2, 2), (4, 0), (7, 1), (9, 2), (12, 0)]:
+        pp, pm = p.copy(), p.copy()
+        pp[i, k] += h
+        pm[i, k] -= h
+        assert g[i, k] == pytest.approx((dl(pp) - dl(pm)) / (2 * h), rel=1e-5, abs=1e-6)
+    # atoms without an offset and without an offset partner inside the cutoff do not enter
+    g0 = nonbonded.nonbonded_pairs_dl_dp(x, p, box, lam, excl, sc, 2.0, 1.2, pl, np.zeros_like(of))
+    assert np.all(g0 == 0.0)
+
+
+def test_bonded_dl_dp_closed_form(rng):
+    """With prefactor offset + mult * lambda, d(du/dlambda)/dp = mult * d(shape)/dp."""
+    x = rng.normal(size=(6, 3))
+    idxs = np.array([[0, 1], [2, 3], [4, 5]])
+    params = np.array([[100.0, 0.5], [200.0, 0.3], [50.0, 1.2]])
+    mult = np.array([0, 1, -2])
+    off = np.array([1, 0, 3])
+    g = bonded.bonded_dl_dp("bond", x, params, 0.4, idxs, mult, off)
+    r = np.linalg.norm(x[idxs[:, 0]] - x[idxs[:, 1]], axis=1)
+    np.testing.assert_allclose(g[:, 0], mult * 0.5 * (r - params[:, 1]) ** 2, rtol=1e-12)
+    np.testing.assert_allclose(g[:, 1], -mult * params[:, 0] * (r - params[:, 1]), rtol=1e-12)
+    tidx = np.array([[0, 1, 2, 3], [1, 2, 3, 4]])
+    tpar = np.array([[3.0, 0.0, 1.0], [5.0, np.pi, 2.0]])
+    gt = bonded.bonded_dl_dp("torsion", x, tpar, 0.7, tidx, np.array([2, 0]), np.array([1, 1]))
+    phi = bonded.signed_torsion_angle(*[torch.tensor(x[tidx[:, c]]) for c in range(4)]).numpy()
+    np.testing.assert_allclose(gt[0, 0], 2 * (1 + np.cos(tpar[0, 2] * phi[0] - tpar[0, 1])), rtol=1e-12)
+    assert np.all(gt[1] == 0.0)

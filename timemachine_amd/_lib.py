@@ -52,6 +52,11 @@ _SIGS = {
         [_P, c_int, c_int, c_int, c_int, c_int, _PD, _PD, _PD, _PD, _PU64, _PU64, _PU64, _PU64],
     ),
     "tm_potential_du_dp_fixed_to_float": (c_int, [_P, c_int, c_int, _PU64, _PD]),
+    "tm_potential_execute_dl_dp_device": (
+        c_int,
+        [_P, c_int, c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p],
+    ),
+    "tm_potential_execute_dl_dp_host": (c_int, [_P, c_int, c_int, _PD, _PD, _PD, c_double, _PU64]),
     "tm_bound_potential_create": (c_int, [_P, c_int, _PI, _PD, _PPV]),
     "tm_bound_potential_release": (None, [_P]),
     "tm_bound_potential_size": (c_int, [_P, POINTER(c_int)]),
@@ -93,6 +98,10 @@ _SIGS = {
     "tm_batch_get_box": (c_int, [_P, c_int, _PD]),
     "tm_batch_write_back": (c_int, [_P]),
     "tm_batch_set_tile_grid": (c_int, [_P, c_int]),
+    "tm_batch_set_grad_observer": (c_int, [_P, c_int]),
+    "tm_batch_observe": (c_int, [_P]),
+    "tm_batch_grad_size": (c_int, [_P, c_int, POINTER(c_int)]),
+    "tm_batch_grad_stats": (c_int, [_P, c_int, POINTER(c_int), _PD, _PD, _PD, _PD, _PD]),
     "tm_batch_profile": (c_int, [_P, c_int, _PD]),
     "tm_batch_total_tiles": (c_int, [_P, POINTER(c_int)]),
 }

@@ -19,8 +19,9 @@ struct BondedTerms {
     const int *term_sys; // [n] system of each term (nullptr -> 0)
 };
 
+// dl_dp: write d(du/dlambda)/dp into du_dp (every other output must be null).
 void launch_bonded(BondedKind kind, const BondedTerms &t, const double *x, const double *lambdas, double lambda0,
                    const int *red_idx, uint64_t *du_dx, uint64_t *du_dp, uint64_t *du_dl, uint64_t *u,
-                   hipStream_t stream);
+                   hipStream_t stream, bool dl_dp = false);
 
 } // namespace tmamd

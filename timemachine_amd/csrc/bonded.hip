@@ -24,7 +24,7 @@ __device__ __forceinline__ void term_context(const BondedTerms &t, int term, con
 
 __global__ void k_harmonic_bond(const BondedTerms t, const double *__restrict__ x, const double *__restrict__ lambdas,
                                 double lambda0, const int *__restrict__ red_idx, uint64_t *__restrict__ du_dx,
-                                uint64_t *__restrict__ du_dp, uint64_t *__restrict__ du_dl, uint64_t *__restrict__ u) {
+                                uint64_t *__restrict__ du_dp, uint64_t *__restrict__ du_dl, uint64_t *__restrict__ u, int dl_dp) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= t.n) {
         return;
@@ -36,7 +36,8 @@ __global__ void k_harmonic_bond(const BondedTerms t, const double *__restrict__ 
     const int j = t.idx[b * 2 + 1];
     const int mult = t.mult ? t.mult[b] : 0;
     const int off = t.offset ? t.offset[b] : 1;
-    const double pf = lam_prefactor<double>(mult, off, lam);
+    // dl_dp: d(du/dlambda)/dp = mult * d(shape)/dp, i.e. du/dp at prefactor mult
+    const double pf = dl_dp ? static_cast<double>(mult) : lam_prefactor<double>(mult, off, lam);
     const BondOut<double> o = bond_term<double>(x + i * 3, x + j * 3, t.params[b * 2 + 0], t.params[b * 2 + 1], mult, pf);
     if (du_dx) {
         for (int d = 0; d < 3; d++) {
@@ -59,7 +60,7 @@ __global__ void k_harmonic_bond(const BondedTerms t, const double *__restrict__ 
 __global__ void k_harmonic_angle(const BondedTerms t, const double *__restrict__ x, const double *__restrict__ lambdas,
                                  double lambda0, const int *__restrict__ red_idx, uint64_t *__restrict__ du_dx,
                                  uint64_t *__restrict__ du_dp, uint64_t *__restrict__ du_dl,
-                                 uint64_t *__restrict__ u) {
+                                 uint64_t *__restrict__ u, int dl_dp) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= t.n) {
         return;
@@ -72,7 +73,8 @@ __global__ void k_harmonic_angle(const BondedTerms t, const double *__restrict__
     const int k = t.idx[a * 3 + 2];
     const int mult = t.mult ? t.mult[a] : 0;
     const int off = t.offset ? t.offset[a] : 1;
-    const double pf = lam_prefactor<double>(mult, off, lam);
+    // dl_dp: d(du/dlambda)/dp = mult * d(shape)/dp, i.e. du/dp at prefactor mult
+    const double pf = dl_dp ? static_cast<double>(mult) : lam_prefactor<double>(mult, off, lam);
     const AngleOut<double> o =
         angle_term<double>(x + i * 3, x + j * 3, x + k * 3, t.params[a * 2 + 0], t.params[a * 2 + 1], mult, pf);
     if (du_dx) {
@@ -97,7 +99,7 @@ __global__ void k_harmonic_angle(const BondedTerms t, const double *__restrict__
 __global__ void k_periodic_torsion(const BondedTerms t, const double *__restrict__ x,
                                    const double *__restrict__ lambdas, double lambda0, const int *__restrict__ red_idx,
                                    uint64_t *__restrict__ du_dx, uint64_t *__restrict__ du_dp,
-                                   uint64_t *__restrict__ du_dl, uint64_t *__restrict__ u) {
+                                   uint64_t *__restrict__ du_dl, uint64_t *__restrict__ u, int dl_dp) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= t.n) {
         return;
@@ -111,7 +113,8 @@ __global__ void k_periodic_torsion(const BondedTerms t, const double *__restrict
     const int l = t.idx[q * 4 + 3];
     const int mult = t.mult ? t.mult[q] : 0;
     const int off = t.offset ? t.offset[q] : 1;
-    const double pf = lam_prefactor<double>(mult, off, lam);
+    // dl_dp: d(du/dlambda)/dp = mult * d(shape)/dp, i.e. du/dp at prefactor mult
+    const double pf = dl_dp ? static_cast<double>(mult) : lam_prefactor<double>(mult, off, lam);
     const TorsionOut<double> o = torsion_term<double>(x + i * 3, x + j * 3, x + k * 3, x + l * 3, t.params[q * 3 + 0],
                                                       t.params[q * 3 + 1], t.params[q * 3 + 2], mult, pf);
     if (du_dx) {
@@ -139,20 +142,23 @@ __global__ void k_periodic_torsion(const BondedTerms t, const double *__restrict
 
 void launch_bonded(BondedKind kind, const BondedTerms &t, const double *x, const double *lambdas, double lambda0,
                    const int *red_idx, uint64_t *du_dx, uint64_t *du_dp, uint64_t *du_dl, uint64_t *u,
-                   hipStream_t stream) {
+                   hipStream_t stream, bool dl_dp) {
     if (t.n == 0 || (!du_dx && !du_dp && !du_dl && !u)) {
         return;
+    }
+    if (dl_dp && (du_dx || du_dl || u || !du_dp)) {
+        throw Error("launch_bonded: the dl_dp mode writes du_dp only");
     }
     const int grid = ceil_div(t.n, 128);
     switch (kind) {
     case BondedKind::Bond:
-        k_harmonic_bond<<<grid, 128, 0, stream>>>(t, x, lambdas, lambda0, red_idx, du_dx, du_dp, du_dl, u);
+        k_harmonic_bond<<<grid, 128, 0, stream>>>(t, x, lambdas, lambda0, red_idx, du_dx, du_dp, du_dl, u, dl_dp ? 1 : 0);
         break;
     case BondedKind::Angle:
-        k_harmonic_angle<<<grid, 128, 0, stream>>>(t, x, lambdas, lambda0, red_idx, du_dx, du_dp, du_dl, u);
+        k_harmonic_angle<<<grid, 128, 0, stream>>>(t, x, lambdas, lambda0, red_idx, du_dx, du_dp, du_dl, u, dl_dp ? 1 : 0);
         break;
     case BondedKind::Torsion:
-        k_periodic_torsion<<<grid, 128, 0, stream>>>(t, x, lambdas, lambda0, red_idx, du_dx, du_dp, du_dl, u);
+        k_periodic_torsion<<<grid, 128, 0, stream>>>(t, x, lambdas, lambda0, red_idx, du_dx, du_dp, du_dl, u, dl_dp ? 1 : 0);
         break;
     }
     TM_LAUNCH_CHECK();

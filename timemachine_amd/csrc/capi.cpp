@@ -268,6 +268,26 @@ int tm_potential_execute_batch_host(tm_potential *p, int coord_batch, int N, int
     });
 }
 
+int tm_potential_execute_dl_dp_device(tm_potential *p, int N, int P, const double *d_x, const double *d_p,
+                                      const double *d_box, double lambda, uint64_t *d_dl_dp, void *stream) {
+    return guarded([&] {
+        if (!d_dl_dp) {
+            throw Error("execute_dl_dp: output buffer is null");
+        }
+        p->p->execute_dl_dp_device(N, P, d_x, d_p, d_box, lambda, d_dl_dp, static_cast<hipStream_t>(stream));
+    });
+}
+
+int tm_potential_execute_dl_dp_host(tm_potential *p, int N, int P, const double *x, const double *params,
+                                    const double *box, double lambda, uint64_t *dl_dp) {
+    return guarded([&] {
+        if (!dl_dp) {
+            throw Error("execute_dl_dp: output buffer is null");
+        }
+        p->p->execute_dl_dp_host(N, P, x, params, box, lambda, dl_dp);
+    });
+}
+
 int tm_potential_du_dp_fixed_to_float(tm_potential *p, int N, int P, const uint64_t *du_dp, double *out) {
     return guarded([&] { p->p->du_dp_fixed_to_float(N, P, du_dp, out); });
 }
@@ -658,6 +678,33 @@ int tm_batch_total_tiles(tm_batch *b, int *out) {
 
 int tm_batch_set_tile_grid(tm_batch *b, int grid) {
     return guarded([&] { b->eng->tile_grid = grid; });
+}
+
+int tm_batch_set_grad_observer(tm_batch *b, int interval) {
+    return guarded([&] { b->eng->set_grad_observer(interval); });
+}
+
+int tm_batch_observe(tm_batch *b) {
+    return guarded([&] { b->eng->observe(); });
+}
+
+int tm_batch_grad_size(tm_batch *b, int s, int *out) {
+    return guarded([&] {
+        if (s < 0 || s >= b->eng->num_systems()) {
+            throw Error("tm_batch_grad_size: system index out of range");
+        }
+        *out = b->eng->grad_size(s);
+    });
+}
+
+int tm_batch_grad_stats(tm_batch *b, int s, int *n_frames, double *sum_a, double *sum_aa, double *sum_g,
+                        double *sum_ag, double *sum_h) {
+    return guarded([&] {
+        if (s < 0 || s >= b->eng->num_systems()) {
+            throw Error("tm_batch_grad_stats: system index out of range");
+        }
+        b->eng->grad_stats(s, n_frames, sum_a, sum_aa, sum_g, sum_ag, sum_h);
+    });
 }
 
 } // extern "C"

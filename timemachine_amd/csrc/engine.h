@@ -77,6 +77,21 @@ class Engine {
     // Energies of every system at its current state and lambda (u, du_dl).
     void energies(double *u_out, double *du_dl_out);
 
+    // Gradient observer for lambda-window averages of A = du/dlambda. Every
+    // `interval` steps of run() (0: off), at the states where run() stores
+    // du_dl with the same interval (start of run step j, (j + 1) % interval
+    // == 0), each system's A, g = du/dp and h = d(du/dlambda)/dp are summed
+    // on the device: n, sum A, sum A^2, sum g, sum A g, sum h. This resets the
+    // sums. Parameters are those of the system's potentials: nonbonded
+    // [N, 3], then bond [B, 2], angle [A, 2], torsion [T, 3] terms.
+    void set_grad_observer(int interval);
+    // One observation at the current state (no step).
+    void observe();
+    // Number of flat parameters of system s (N*3 + 2B + 2A + 3T).
+    int grad_size(int s) const;
+    void grad_stats(int s, int *n_frames, double *sum_a, double *sum_aa, double *sum_g, double *sum_ag,
+                    double *sum_h);
+
     void get_x(int s, double *out);
     void get_v(int s, double *out);
     void get_box(int s, double *out);

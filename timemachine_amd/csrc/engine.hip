@@ -113,6 +113,40 @@ __global__ void k_store_frames2(int A, const int *__restrict__ atom_sys, const i
     frames[dst + 2] = x[a * 3 + 2];
 }
 
+// Gradient observer: per flat parameter element e (nonbonded [A, 3] rows, then
+// bond / angle / torsion parameters) of system s = elem_sys[e], with
+// A_s = du/dlambda, g = du/dp, h = d(du/dlambda)/dp at the observed frame:
+//   sum_g += g, sum_ag += A_s g, sum_h += h      (FP64, one thread per element,
+// frames in order, so the sums are deterministic).
+__global__ void k_obs_accum(int Pg, const int *__restrict__ elem_sys, const double *__restrict__ elem_inv_scale,
+                            const uint64_t *__restrict__ dl, const uint64_t *__restrict__ du_dp,
+                            const uint64_t *__restrict__ dl_dp, double *__restrict__ sum_g,
+                            double *__restrict__ sum_ag, double *__restrict__ sum_h) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= Pg) {
+        return;
+    }
+    const double a = fixed_to_double(dl[elem_sys[e]], FIXED_SCALE);
+    const double inv = elem_inv_scale[e];
+    const double g = static_cast<double>(static_cast<int64_t>(du_dp[e])) * inv;
+    const double h = static_cast<double>(static_cast<int64_t>(dl_dp[e])) * inv;
+    sum_g[e] += g;
+    sum_ag[e] += a * g;
+    sum_h[e] += h;
+}
+
+__global__ void k_obs_accum_sys(int S, const uint64_t *__restrict__ dl, double *__restrict__ sum_a,
+                                double *__restrict__ sum_aa, int *__restrict__ n_frames) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) {
+        return;
+    }
+    const double a = fixed_to_double(dl[s], FIXED_SCALE);
+    sum_a[s] += a;
+    sum_aa[s] += a * a;
+    n_frames[s] += 1;
+}
+
 // ---------------------------------------------------------------- barostat
 // Restates MonteCarloBarostat::inplace_move (barostat.cu:286-373). The move is
 // decided by one thread per system and applied by a separate kernel, so no
@@ -377,6 +411,16 @@ struct Engine::Impl {
     DeviceArray<int> g_atom, g_mol, mol_sys, mol_count, atom_mol;
     DeviceArray<uint64_t> centroids;
 
+    // gradient observer
+    bool pl_shares_nb = true;            // pair lists use the all-pairs parameters
+    int obs_interval = 0;
+    int Pg = 0;                          // flat parameter elements
+    int bt_off[3] = {0, 0, 0};           // offset of each bonded kind in the flat space
+    std::vector<int> bt_first[3], bt_count[3]; // per-system term ranges
+    DeviceArray<uint64_t> obs_dudp, obs_dldp, obs_dl;
+    DeviceArray<int> obs_sys, obs_n;
+    DeviceArray<double> obs_inv_scale, obs_sum_g, obs_sum_ag, obs_sum_h, obs_sum_a, obs_sum_aa;
+
     BaroDev baro_dev() {
         return BaroDev{b_has.ptr,      b_interval.ptr, b_bstep.ptr,    b_seed.ptr,     b_pressure.ptr,  b_kT.ptr,
                        b_nmol.ptr,     b_vscale.ptr,   b_accepted.ptr, b_attempted.ptr, b_trig.ptr,     b_accept.ptr,
@@ -393,22 +437,7 @@ struct Engine::Impl {
             if (pl_n[neg] == 0) {
                 continue;
             }
-            PairListArgs a{};
-            a.M = pl_n[neg];
-            a.pairs = pl_pairs[neg].ptr;
-            a.scales = pl_scales[neg].ptr;
-            a.pair_sys = pl_sys[neg].ptr;
-            a.red_idx = red_idx.ptr;
-            a.x = xp;
-            a.params = pl_params.ptr;
-            a.boxes = boxp;
-            a.plane = pl_plane.ptr;
-            a.offset = pl_offset.ptr;
-            a.lambdas = lam.ptr;
-            a.cutoffs = pl_cutoffs.ptr;
-            a.betas = pl_betas.ptr;
-            a.negated = neg;
-            launch_pair_list(a, ddx, nullptr, ddl, uu, st);
+            launch_pair_list(pl_args(neg, xp, boxp), ddx, nullptr, ddl, uu, st);
         }
         for (int k = 0; k < 3; k++) {
             if (bt_n[k] == 0) {
@@ -417,6 +446,62 @@ struct Engine::Impl {
             BondedTerms t{bt_n[k], bt_idx[k].ptr, bt_params[k].ptr, bt_mult[k].ptr, bt_offset[k].ptr, bt_sys[k].ptr};
             launch_bonded(static_cast<BondedKind>(k), t, xp, lam.ptr, 0.0, red_idx.ptr, ddx, nullptr, ddl, uu, st);
         }
+    }
+
+    PairListArgs pl_args(int neg, const double *xp, const double *boxp) const {
+        PairListArgs a{};
+        a.M = pl_n[neg];
+        a.pairs = pl_pairs[neg].ptr;
+        a.scales = pl_scales[neg].ptr;
+        a.pair_sys = pl_sys[neg].ptr;
+        a.red_idx = red_idx.ptr;
+        a.x = xp;
+        a.params = pl_params.ptr;
+        a.boxes = boxp;
+        a.plane = pl_plane.ptr;
+        a.offset = pl_offset.ptr;
+        a.lambdas = lam.ptr;
+        a.cutoffs = pl_cutoffs.ptr;
+        a.betas = pl_betas.ptr;
+        a.negated = neg;
+        return a;
+    }
+
+    // One observed frame at the current state of every system: du/dlambda,
+    // du/dp and d(du/dlambda)/dp into the flat buffers, then the running sums.
+    void observe(hipStream_t st, int tile_grid) {
+        const int TPB = 256;
+        launch_fill_u64(obs_dudp.ptr, Pg, 0, st);
+        launch_fill_u64(obs_dldp.ptr, Pg, 0, st);
+        launch_fill_u64(obs_dl.ptr, S, 0, st);
+        if (has_nb) {
+            nb.prepare(x.ptr, nb_params.ptr, box.ptr, nb_plane.ptr, nb_offset.ptr, lam.ptr, 0.0, st);
+            nb.compute(box.ptr, nullptr, obs_dudp.ptr, obs_dl.ptr, nullptr, st, tile_grid);
+            nb.compute_dl_dp(box.ptr, obs_dldp.ptr, st, tile_grid);
+        }
+        for (int neg = 0; neg < 2; neg++) {
+            if (pl_n[neg] == 0) {
+                continue;
+            }
+            const PairListArgs a = pl_args(neg, x.ptr, box.ptr);
+            launch_pair_list(a, nullptr, obs_dudp.ptr, obs_dl.ptr, nullptr, st);
+            launch_pair_list_dl_dp(a, obs_dldp.ptr, st);
+        }
+        for (int k = 0; k < 3; k++) {
+            if (bt_n[k] == 0) {
+                continue;
+            }
+            BondedTerms t{bt_n[k], bt_idx[k].ptr, bt_params[k].ptr, bt_mult[k].ptr, bt_offset[k].ptr, bt_sys[k].ptr};
+            launch_bonded(static_cast<BondedKind>(k), t, x.ptr, lam.ptr, 0.0, red_idx.ptr, nullptr,
+                          obs_dudp.ptr + bt_off[k], obs_dl.ptr, nullptr, st);
+            launch_bonded(static_cast<BondedKind>(k), t, x.ptr, lam.ptr, 0.0, red_idx.ptr, nullptr,
+                          obs_dldp.ptr + bt_off[k], nullptr, nullptr, st, true);
+        }
+        k_obs_accum<<<ceil_div(Pg, TPB), TPB, 0, st>>>(Pg, obs_sys.ptr, obs_inv_scale.ptr, obs_dl.ptr, obs_dudp.ptr,
+                                                      obs_dldp.ptr, obs_sum_g.ptr, obs_sum_ag.ptr, obs_sum_h.ptr);
+        TM_LAUNCH_CHECK();
+        k_obs_accum_sys<<<ceil_div(S, 64), 64, 0, st>>>(S, obs_dl.ptr, obs_sum_a.ptr, obs_sum_aa.ptr, obs_n.ptr);
+        TM_LAUNCH_CHECK();
     }
 };
 
@@ -492,6 +577,7 @@ Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
     I.du_dl_buf.zero();
 
     // potentials
+    std::vector<double> h_nb_params;
     std::vector<Flat> flats(S_);
     for (int s = 0; s < S_; s++) {
         for (auto &bp : systems[s].bps) {
@@ -534,6 +620,7 @@ Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
         }
         I.nb.nb_margin = NB_LIST_MARGIN;
         I.nb.init(ys, subset);
+        h_nb_params = params;
         upload_vec(I.nb_params, params);
         upload_vec(I.nb_plane, plane);
         upload_vec(I.nb_offset, offset);
@@ -580,6 +667,20 @@ Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
             upload_vec(I.pl_sys[neg], psys[neg]);
         }
         I.has_pl = I.pl_n[0] + I.pl_n[1] > 0;
+        if (I.has_pl) {
+            // every system with pair lists must share the all-pairs parameters for the
+            // observer's du/dp of the pair lists to land on the nonbonded rows
+            for (int s = 0; s < S_ && I.pl_shares_nb; s++) {
+                if (flats[s].pls.empty()) {
+                    continue;
+                }
+                if (!I.has_nb || !std::equal(params.begin() + I.atom0[s] * 3,
+                                             params.begin() + (I.atom0[s] + I.natoms[s]) * 3,
+                                             h_nb_params.begin() + I.atom0[s] * 3)) {
+                    I.pl_shares_nb = false;
+                }
+            }
+        }
         upload_vec(I.pl_params, params);
         upload_vec(I.pl_plane, plane);
         upload_vec(I.pl_offset, offset);
@@ -614,6 +715,14 @@ Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
             }
         }
         for (int k = 0; k < 3; k++) {
+            I.bt_first[k].assign(S_, 0);
+            I.bt_count[k].assign(S_, 0);
+            for (size_t t = 0; t < sys[k].size(); t++) {
+                I.bt_count[k][sys[k][t]] += 1;
+            }
+            for (int s = 1; s < S_; s++) {
+                I.bt_first[k][s] = I.bt_first[k][s - 1] + I.bt_count[k][s - 1];
+            }
             I.bt_n[k] = static_cast<int>(sys[k].size());
             upload_vec(I.bt_idx[k], idx[k]);
             upload_vec(I.bt_mult[k], mult[k]);
@@ -737,7 +846,8 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
     launch_fill_u64(I.du_dl_buf.ptr, S_, 0, stream_);
 
     const int TPB = 256;
-    auto one_step = [&](bool baro_now) {
+    const int obs_k = I.obs_interval;
+    auto one_step = [&](bool baro_now, bool obs_now) {
         if (lambda_sched) {
             k_set_lambda<<<ceil_div(S_, 64), 64, 0, stream_>>>(S_, sched.ptr, n_steps, I.run_step.ptr, I.lam.ptr);
             TM_LAUNCH_CHECK();
@@ -745,6 +855,9 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
         launch_fill_u64(I.du_dx.ptr, static_cast<size_t>(A_) * 3, 0, stream_);
         I.eval(I.x.ptr, I.box.ptr, I.du_dx.ptr, du_dl_interval > 0 ? I.du_dl_buf.ptr : nullptr, nullptr, stream_,
                tile_grid);
+        if (obs_now) {
+            I.observe(stream_, tile_grid);
+        }
         k_langevin<<<ceil_div(A_, TPB), TPB, 0, stream_>>>(A_, I.atom_sys.ptr, I.d_atom0.ptr, I.step.ptr, I.seed.ptr,
                                                          I.ca.ptr, I.dt.ptr, I.cbs.ptr, I.ccs.ptr, I.du_dx.ptr,
                                                          I.x.ptr, I.v.ptr);
@@ -802,6 +915,10 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
         return false;
     };
 
+    // observed frames: the states at which du_dl is stored with interval obs_k
+    // (the state at the start of run step j, (j + 1) % obs_k == 0)
+    auto obs_at = [&](int j) { return obs_k > 0 && (j + 1) % obs_k == 0; };
+
     int j = 0;
     if (use_graph) {
         int chunk = 10;
@@ -813,18 +930,18 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
                 }
             }
         }
-        std::map<std::vector<bool>, hipGraphExec_t> cache;
+        std::map<std::vector<uint8_t>, hipGraphExec_t> cache;
         while (j + chunk <= n_steps) {
-            std::vector<bool> pattern(chunk);
+            std::vector<uint8_t> pattern(chunk);
             for (int q = 0; q < chunk; q++) {
-                pattern[q] = baro_at(j + q);
+                pattern[q] = static_cast<uint8_t>((baro_at(j + q) ? 1 : 0) | (obs_at(j + q) ? 2 : 0));
             }
             auto it = cache.find(pattern);
             if (it == cache.end()) {
                 hipGraph_t g;
                 TM_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
                 for (int q = 0; q < chunk; q++) {
-                    one_step(pattern[q]);
+                    one_step((pattern[q] & 1) != 0, (pattern[q] & 2) != 0);
                 }
                 TM_HIP_CHECK(hipStreamEndCapture(stream_, &g));
                 hipGraphExec_t ge;
@@ -841,7 +958,7 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
         }
     }
     for (; j < n_steps; j++) {
-        one_step(baro_at(j));
+        one_step(baro_at(j), obs_at(j));
     }
     for (int s = 0; s < S_; s++) {
         if (I.h_has_baro[s]) {
@@ -863,6 +980,112 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
     if (box_out && F > 0) {
         box_frames.download(box_out, static_cast<size_t>(S_) * F * 9, stream_);
     }
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void Engine::set_grad_observer(int interval) {
+    Impl &I = *impl_;
+    if (interval < 0) {
+        throw Error("Engine::set_grad_observer: interval must be >= 0");
+    }
+    if (interval > 0 && !I.pl_shares_nb) {
+        throw Error("Engine::set_grad_observer: pair lists must share the nonbonded parameters");
+    }
+    if (I.Pg == 0) {
+        // flat parameter space: nonbonded [A, 3], then bond [B, 2], angle [A, 2], torsion [T, 3]
+        static const int np[3] = {2, 2, 3};
+        I.bt_off[0] = A_ * 3;
+        I.bt_off[1] = I.bt_off[0] + I.bt_n[0] * np[0];
+        I.bt_off[2] = I.bt_off[1] + I.bt_n[1] * np[1];
+        I.Pg = I.bt_off[2] + I.bt_n[2] * np[2];
+        std::vector<int> esys(I.Pg);
+        std::vector<double> inv(I.Pg, 1.0 / FIXED_SCALE);
+        for (int a = 0; a < A_; a++) {
+            const int s = static_cast<int>(std::upper_bound(I.atom0.begin(), I.atom0.end(), a) - I.atom0.begin()) - 1;
+            for (int c = 0; c < 3; c++) {
+                esys[a * 3 + c] = s;
+            }
+            inv[a * 3 + 1] = 1.0 / FIXED_SCALE_DSIG;
+            inv[a * 3 + 2] = 1.0 / FIXED_SCALE_DEPS;
+        }
+        for (int k = 0; k < 3; k++) {
+            for (int s = 0; s < S_; s++) {
+                for (int t = I.bt_first[k][s]; t < I.bt_first[k][s] + I.bt_count[k][s]; t++) {
+                    for (int c = 0; c < np[k]; c++) {
+                        esys[I.bt_off[k] + t * np[k] + c] = s;
+                    }
+                }
+            }
+        }
+        upload_vec(I.obs_sys, esys);
+        upload_vec(I.obs_inv_scale, inv);
+        I.obs_dudp.alloc(I.Pg);
+        I.obs_dldp.alloc(I.Pg);
+        I.obs_dl.alloc(S_);
+        I.obs_sum_g.alloc(I.Pg);
+        I.obs_sum_ag.alloc(I.Pg);
+        I.obs_sum_h.alloc(I.Pg);
+        I.obs_sum_a.alloc(S_);
+        I.obs_sum_aa.alloc(S_);
+        I.obs_n.alloc(S_);
+    }
+    I.obs_interval = interval;
+    I.obs_sum_g.zero();
+    I.obs_sum_ag.zero();
+    I.obs_sum_h.zero();
+    I.obs_sum_a.zero();
+    I.obs_sum_aa.zero();
+    I.obs_n.zero();
+    TM_HIP_CHECK(hipDeviceSynchronize());
+}
+
+void Engine::observe() {
+    Impl &I = *impl_;
+    if (I.Pg == 0) {
+        throw Error("Engine::observe: call set_grad_observer first");
+    }
+    I.observe(stream_, tile_grid);
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+int Engine::grad_size(int s) const {
+    const Impl &I = *impl_;
+    static const int np[3] = {2, 2, 3};
+    int P = h_n_[s] * 3;
+    for (int k = 0; k < 3; k++) {
+        P += I.bt_count[k][s] * np[k];
+    }
+    return P;
+}
+
+void Engine::grad_stats(int s, int *n_frames, double *sum_a, double *sum_aa, double *sum_g, double *sum_ag,
+                        double *sum_h) {
+    Impl &I = *impl_;
+    if (I.Pg == 0) {
+        throw Error("Engine::grad_stats: call set_grad_observer first");
+    }
+    static const int np[3] = {2, 2, 3};
+    std::vector<std::pair<size_t, size_t>> ranges; // (device offset, count) in output order
+    ranges.emplace_back(static_cast<size_t>(h_atom0_[s]) * 3, static_cast<size_t>(h_n_[s]) * 3);
+    for (int k = 0; k < 3; k++) {
+        ranges.emplace_back(static_cast<size_t>(I.bt_off[k]) + static_cast<size_t>(I.bt_first[k][s]) * np[k],
+                            static_cast<size_t>(I.bt_count[k][s]) * np[k]);
+    }
+    double *outs[3] = {sum_g, sum_ag, sum_h};
+    const DeviceArray<double> *srcs[3] = {&I.obs_sum_g, &I.obs_sum_ag, &I.obs_sum_h};
+    for (int q = 0; q < 3; q++) {
+        size_t o = 0;
+        for (auto &r : ranges) {
+            if (r.second > 0) {
+                TM_HIP_CHECK(hipMemcpyAsync(outs[q] + o, srcs[q]->ptr + r.first, r.second * sizeof(double),
+                                            hipMemcpyDeviceToHost, stream_));
+            }
+            o += r.second;
+        }
+    }
+    TM_HIP_CHECK(hipMemcpyAsync(n_frames, I.obs_n.ptr + s, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    TM_HIP_CHECK(hipMemcpyAsync(sum_a, I.obs_sum_a.ptr + s, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    TM_HIP_CHECK(hipMemcpyAsync(sum_aa, I.obs_sum_aa.ptr + s, sizeof(double), hipMemcpyDeviceToHost, stream_));
     TM_HIP_CHECK(hipStreamSynchronize(stream_));
 }
 

@@ -132,4 +132,57 @@ TM_HD inline PairTerms nb_pair_terms(
     return o;
 }
 
+// Parameter derivatives of the pair's force prefactor, for d(du/dlambda)/dp.
+// A pair contributes prefactor * dw * (dw/dl_i - dw/dl_j) to du/dlambda, and
+// the prefactor is linear in q_i q_j (Coulomb) and in eps_i eps_j (LJ):
+//   es_unit = prefactor_es / (scale_q q_i q_j)
+//   lj_unit = prefactor_lj / (eps_i eps_j)       (LJ pairs only; includes scale_lj)
+//   lj_dsig = d prefactor_lj / d sig_ij          (LJ pairs only)
+// Same arithmetic as nb_pair_terms for the shared parts.
+struct PairDlDp {
+    double es_unit;
+    double lj_unit;
+    double lj_dsig;
+    bool lj;
+};
+
+TM_HD inline PairDlDp nb_pair_dl_dp(double d2, double sig_i, double sig_j, double eps_i, double eps_j, double beta,
+                                    double scale_lj) {
+    PairDlDp o;
+    const double inv_d = rsqrt_nr(d2);
+    const double d = d2 * inv_d;
+    const double inv_d2 = inv_d * inv_d;
+
+    const double x = beta * d;
+    const double x2 = x * x;
+    const float x2f = static_cast<float>(x2);
+    double e = static_cast<double>(std::exp(-x2f));
+    e = e - e * (x2 - static_cast<double>(x2f));
+    const double t = rcp_nr(1.0 + ERFC_P * x);
+    double poly = ERFC_C7;
+    poly = __builtin_fma(poly, t, ERFC_C6);
+    poly = __builtin_fma(poly, t, ERFC_C5);
+    poly = __builtin_fma(poly, t, ERFC_C4);
+    poly = __builtin_fma(poly, t, ERFC_C3);
+    poly = __builtin_fma(poly, t, ERFC_C2);
+    poly = __builtin_fma(poly, t, ERFC_C1);
+    poly = __builtin_fma(poly, t, ERFC_C0);
+    o.es_unit = -(inv_d * inv_d2) * ((TWO_OVER_SQRT_PI * x + poly) * e);
+    o.lj_unit = 0.0;
+    o.lj_dsig = 0.0;
+    o.lj = (eps_i != 0.0) && (eps_j != 0.0);
+    if (o.lj) {
+        const double sig_ij = sig_i + sig_j;
+        const double s1 = sig_ij * inv_d;
+        const double s2 = s1 * s1;
+        const double s4 = s2 * s2;
+        const double s6 = s4 * s2;
+        // prefactor_lj = -se s^6 / d^2 (48 s^6 - 24), s = sig_ij / d, se = scale_lj eps_i eps_j
+        o.lj_unit = -scale_lj * (s6 * inv_d2) * (s6 * 48.0 - 24.0);
+        // d/dsig_ij: s^6 -> 6 s^5 / d, s^12 -> 12 s^11 / d
+        o.lj_dsig = -(scale_lj * (eps_i * eps_j)) * (inv_d2 * inv_d) * (s4 * s1) * (s6 * 576.0 - 144.0);
+    }
+    return o;
+}
+
 } // namespace tmamd

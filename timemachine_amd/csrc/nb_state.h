@@ -128,6 +128,11 @@ struct NbState {
         uint64_t *u,
         hipStream_t stream,
         int grid_override = 0) const;
+
+    // d(du/dlambda)/dp [*, 3] (rows atom0 + i), accumulated with the du_dp
+    // fixed-point exponents. Uses the tiles and sorted data of the last
+    // prepare().
+    void compute_dl_dp(const double *boxes, uint64_t *dl_dp, hipStream_t stream, int grid_override = 0) const;
 };
 
 // Pair-list kernel (exclusions / explicit pairs). Pairs index global atoms.
@@ -153,6 +158,8 @@ struct PairListArgs {
 
 void launch_pair_list(
     const PairListArgs &a, uint64_t *du_dx, uint64_t *du_dp, uint64_t *du_dl, uint64_t *u, hipStream_t stream);
+// d(du/dlambda)/dp of the pair list into dl_dp [*, 3] (negated when a.negated).
+void launch_pair_list_dl_dp(const PairListArgs &a, uint64_t *dl_dp, hipStream_t stream);
 
 // Helpers shared by several translation units.
 void launch_fill_u64(uint64_t *p, size_t n, uint64_t v, hipStream_t stream);

@@ -2,13 +2,17 @@
 // pair-list kernel used for exclusions.
 //
 // Tile kernel (restates the semantics of kernels/k_nonbonded.cuh:311-671 for
-// wave64): one wave per tile, the row atom of the tile in each lane and the
-// 64 column atoms rotated one lane per round with DPP wave_rol:1, so after
-// 64 rounds every (row, column) pair has met exactly once and every column
-// accumulator is back in its own lane. Force, du/dp, du/dl and energy terms
-// are rounded to 64-bit fixed point per pair and summed in registers;
-// results leave through 64-bit global atomics (row and column atoms) and one
-// wave-reduced atomic per tile for u and du/dl.
+// wave64): one wave per 64 x 64 tile. The reference evaluates every
+// (row, column) slot of a tile; here a first pass over the 64 rounds
+// (lane l meets column slot (l + r) mod 64 in round r) computes only the
+// exact FP64 4D distance and compacts the pairs inside the cutoff into an
+// LDS queue, and a second pass evaluates the queued pairs 64 at a time with
+// every lane busy. Force, du/dp, du/dl and energy terms are rounded to
+// 64-bit fixed point per pair and summed exactly (LDS u64 atomics per atom,
+// registers for u and du/dl); results leave through 64-bit global atomics
+// (row and column atoms) and one wave-reduced atomic per tile for u and
+// du/dl. Integer sums do not depend on the order, so the outputs are
+// bitwise those of evaluating the pairs one by one.
 //
 // Pair-list kernel (kernels/k_nonbonded_pair_list.cuh:17-216): one thread
 // per pair; with negated=1 it subtracts exactly what the tile kernel added,
@@ -21,29 +25,129 @@ namespace tmamd {
 
 namespace {
 
-constexpr int NB_BLOCK = 256;
+constexpr int NB_BLOCK = 128; // two waves: the per-wave LDS (13-16 KB) bounds occupancy
 constexpr int NB_BAD_BOX = NB_STATUS_BOX_TOO_SMALL | NB_STATUS_BOX_NOT_ORTHO;
 
 __device__ __forceinline__ void atomic_add_u64(uint64_t *p, uint64_t v) {
     atomicAdd(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(v));
 }
 
-template <bool ALCH, bool U, bool DX, bool DL, bool DP>
+constexpr int PAIR_RING = 128; // queued pairs per wave (a round adds at most 64)
+
+// Per-wave LDS of the tile kernel. NACC accumulators per atom: x, y, z
+// forces, then q, sig, eps parameter derivatives when requested.
+template <int NACC> struct TileLds {
+    double4 colpos[TILE];        // column atom x, y, z, w
+    float4 colpar[TILE];         // column atom q, sig, eps, dw/dlambda
+    float4 rowpar[TILE];         // row atom q, sig, eps, dw/dlambda
+    uint32_t colidx[TILE];       // column sorted index (NB_INVALID for padding)
+    double4 pd[PAIR_RING];       // queued pair: dx, dy, dz, d^2 (4D)
+    double pw[PAIR_RING];        // queued pair: dw
+    uint32_t pij[PAIR_RING];     // queued pair: row lane | column slot << 8
+    uint64_t acc[2][NACC][TILE]; // [row, column][component][lane / slot]
+};
+
+__device__ __forceinline__ void lds_add_u64(uint64_t *p, uint64_t v) {
+    atomicAdd(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(v));
+}
+
+// Pass 2: evaluates n <= 64 queued pairs starting at ring entry `base`, one
+// pair per lane, and adds the fixed-point terms to the LDS accumulators of
+// the pair's row and column atom (u and du/dl to per-lane registers).
+// DLDP: instead of du/dp, accumulate d(du/dlambda)/dp into the du_dp slots
+// (same fixed-point exponents); only alchemical tiles contribute.
+template <bool ALCH, bool U, bool DX, bool DL, bool DP, bool DLDP, int NACC>
+__device__ __forceinline__ void eval_pairs(TileLds<NACC> &L, const int base, const int n, const double beta,
+                                           uint64_t &uacc, uint64_t &dlacc) {
+    constexpr int PQ = DX ? 3 : 0; // first parameter-derivative component
+    const int lane = threadIdx.x & 63;
+    if (lane < n) {
+        const int e = (base + lane) & (PAIR_RING - 1);
+        const double4 q = L.pd[e];
+        const uint32_t ij = L.pij[e];
+        const int il = static_cast<int>(ij & 0xFFu);
+        const int sl = static_cast<int>(ij >> 8);
+        const float4 ai = L.rowpar[il];
+        const float4 aj = L.colpar[sl];
+        const double d2 = q.w;
+        const PairTerms T = nb_pair_terms<U, DP>(d2, ai.x, aj.x, ai.y, aj.y, ai.z, aj.z, beta, 1.0, 1.0);
+        if (DX) {
+            const uint64_t fx = f64_to_fixed(T.prefactor * q.x, FIXED_SCALE);
+            const uint64_t fy = f64_to_fixed(T.prefactor * q.y, FIXED_SCALE);
+            const uint64_t fz = f64_to_fixed(T.prefactor * q.z, FIXED_SCALE);
+            lds_add_u64(&L.acc[0][0][il], fx);
+            lds_add_u64(&L.acc[0][1][il], fy);
+            lds_add_u64(&L.acc[0][2][il], fz);
+            lds_add_u64(&L.acc[1][0][sl], 0ull - fx);
+            lds_add_u64(&L.acc[1][1][sl], 0ull - fy);
+            lds_add_u64(&L.acc[1][2][sl], 0ull - fz);
+        }
+        if (DP) {
+            lds_add_u64(&L.acc[0][PQ + 0][il], f64_to_fixed((1.0 * aj.x) * T.erfc_inv_d, FIXED_SCALE));
+            lds_add_u64(&L.acc[1][PQ + 0][sl], f64_to_fixed((1.0 * ai.x) * T.erfc_inv_d, FIXED_SCALE));
+            if (T.lj) {
+                const uint64_t gs = f64_to_fixed(T.sig_grad, FIXED_SCALE_DSIG);
+                lds_add_u64(&L.acc[0][PQ + 1][il], gs);
+                lds_add_u64(&L.acc[1][PQ + 1][sl], gs);
+                lds_add_u64(&L.acc[0][PQ + 2][il], f64_to_fixed(T.eps_grad * aj.z, FIXED_SCALE_DEPS));
+                lds_add_u64(&L.acc[1][PQ + 2][sl], f64_to_fixed(T.eps_grad * ai.z, FIXED_SCALE_DEPS));
+            }
+        }
+        if (ALCH && (DL || DLDP)) {
+            const double dw = L.pw[e];
+            if (DL) {
+                dlacc += f64_to_fixed(dw * T.prefactor * (static_cast<double>(ai.w) - static_cast<double>(aj.w)),
+                                      FIXED_SCALE);
+            }
+            if (DLDP) {
+                const PairDlDp D = nb_pair_dl_dp(d2, ai.y, aj.y, ai.z, aj.z, beta, 1.0);
+                const double dd = dw * (static_cast<double>(ai.w) - static_cast<double>(aj.w));
+                lds_add_u64(&L.acc[0][PQ + 0][il],
+                            f64_to_fixed((static_cast<double>(aj.x) * D.es_unit) * dd, FIXED_SCALE));
+                lds_add_u64(&L.acc[1][PQ + 0][sl],
+                            f64_to_fixed((static_cast<double>(ai.x) * D.es_unit) * dd, FIXED_SCALE));
+                if (D.lj) {
+                    const uint64_t gs = f64_to_fixed(D.lj_dsig * dd, FIXED_SCALE_DSIG);
+                    lds_add_u64(&L.acc[0][PQ + 1][il], gs);
+                    lds_add_u64(&L.acc[1][PQ + 1][sl], gs);
+                    lds_add_u64(&L.acc[0][PQ + 2][il],
+                                f64_to_fixed((D.lj_unit * static_cast<double>(aj.z)) * dd, FIXED_SCALE_DEPS));
+                    lds_add_u64(&L.acc[1][PQ + 2][sl],
+                                f64_to_fixed((D.lj_unit * static_cast<double>(ai.z)) * dd, FIXED_SCALE_DEPS));
+                }
+            }
+        }
+        if (U) {
+            uacc += f64_to_fixed(T.u, FIXED_SCALE);
+        }
+    }
+}
+
+// One 64 x 64 tile on one wave.
+// Pass 1: 64 rounds; in round r lane l pairs its row atom with column slot
+// (l + r) mod 64 (read from LDS), computes the exact FP64 4D distance and
+// queues the pairs inside the cutoff (ballot + mbcnt compaction into an LDS
+// ring). Whenever 64 pairs are queued, pass 2 evaluates them with every
+// lane busy. Fixed-point sums are exact, so the order of evaluation does
+// not change any output bit.
+template <bool ALCH, bool U, bool DX, bool DL, bool DP, bool DLDP, int NACC>
 __device__ __forceinline__ void tile_body(
+    TileLds<NACC> &L,
     const NbSys &S,
     const int s,
     const int r,
     const uint32_t jl0,
     const double4 pi,
     const float4 ai,
-    double4 pj,
-    float4 aj,
+    const double4 pj0,
+    const float4 aj0,
     const uint32_t *__restrict__ perm,
     const double *__restrict__ boxes,
     uint64_t *__restrict__ du_dx,
     uint64_t *__restrict__ du_dp,
     uint64_t *__restrict__ du_dl,
     uint64_t *__restrict__ u) {
+    constexpr int PQ = DX ? 3 : 0;
     const int lane = threadIdx.x & 63;
     const uint32_t i = static_cast<uint32_t>(r * TILE + lane);
     const uint32_t k = static_cast<uint32_t>(S.k);
@@ -58,15 +162,24 @@ __device__ __forceinline__ void tile_body(
     const double cut2 = S.cutoff_d * S.cutoff_d;
     const double beta = static_cast<double>(S.beta);
 
-    uint64_t gix = 0, giy = 0, giz = 0;
-    uint64_t gjx = 0, gjy = 0, gjz = 0;
-    uint64_t gqi = 0, gsi = 0, gei = 0;
-    uint64_t gqj = 0, gsj = 0, gej = 0;
-    uint64_t uacc = 0, dlacc = 0;
-    uint32_t jc = jl0;
+    wave_lds_sync(); // the previous tile's LDS reads are complete
+    L.colpos[lane] = pj0;
+    L.colpar[lane] = aj0;
+    L.rowpar[lane] = ai;
+    L.colidx[lane] = jl0;
+#pragma unroll
+    for (int c = 0; c < NACC; c++) {
+        L.acc[0][c][lane] = 0;
+        L.acc[1][c][lane] = 0;
+    }
+    wave_lds_sync();
 
-#pragma unroll 2
+    uint64_t uacc = 0, dlacc = 0;
+    int queued = 0, done = 0; // wave-uniform
     for (int round = 0; round < TILE; round++) {
+        const int slot = (lane + round) & (TILE - 1);
+        const double4 pj = L.colpos[slot];
+        const uint32_t jc = L.colidx[slot];
         const double dx = min_image_d(pi.x - pj.x, bx, ibx);
         const double dy = min_image_d(pi.y - pj.y, by, iby);
         const double dz = min_image_d(pi.z - pj.z, bz, ibz);
@@ -76,90 +189,74 @@ __device__ __forceinline__ void tile_body(
             dw = pi.w - pj.w;
             d2 = d2 + dw * dw;
         }
-        const bool valid = vi && (jc < k) && (i < jc);
-        if (valid && d2 < cut2) {
-            const PairTerms T = nb_pair_terms<U, DP>(d2, ai.x, aj.x, ai.y, aj.y, ai.z, aj.z, beta, 1.0, 1.0);
-            if (DX) {
-                const uint64_t fx = f64_to_fixed(T.prefactor * dx, FIXED_SCALE);
-                const uint64_t fy = f64_to_fixed(T.prefactor * dy, FIXED_SCALE);
-                const uint64_t fz = f64_to_fixed(T.prefactor * dz, FIXED_SCALE);
-                gix += fx;
-                giy += fy;
-                giz += fz;
-                gjx -= fx;
-                gjy -= fy;
-                gjz -= fz;
-            }
-            if (DP) {
-                gqi += f64_to_fixed((1.0 * aj.x) * T.erfc_inv_d, FIXED_SCALE);
-                gqj += f64_to_fixed((1.0 * ai.x) * T.erfc_inv_d, FIXED_SCALE);
-                if (T.lj) {
-                    const uint64_t gs = f64_to_fixed(T.sig_grad, FIXED_SCALE_DSIG);
-                    gsi += gs;
-                    gsj += gs;
-                    gei += f64_to_fixed(T.eps_grad * aj.z, FIXED_SCALE_DEPS);
-                    gej += f64_to_fixed(T.eps_grad * ai.z, FIXED_SCALE_DEPS);
+        const bool inside = vi && (jc < k) && (i < jc) && (d2 < cut2);
+        const uint64_t m = ballot(inside);
+        if (m != 0) {
+            if (inside) {
+                const int e = (queued + lanes_below(m)) & (PAIR_RING - 1);
+                L.pd[e] = make_double4(dx, dy, dz, d2);
+                if (ALCH) {
+                    L.pw[e] = dw;
                 }
+                L.pij[e] = static_cast<uint32_t>(lane) | (static_cast<uint32_t>(slot) << 8);
             }
-            if (DL && ALCH) {
-                dlacc += f64_to_fixed(dw * T.prefactor * (static_cast<double>(ai.w) - static_cast<double>(aj.w)),
-                                      FIXED_SCALE);
+            queued += __popcll(m);
+            if (queued - done >= TILE) {
+                wave_lds_sync();
+                eval_pairs<ALCH, U, DX, DL, DP, DLDP, NACC>(L, done, TILE, beta, uacc, dlacc);
+                done += TILE;
+                wave_lds_sync(); // later rounds reuse these ring entries
             }
-            if (U) {
-                uacc += f64_to_fixed(T.u, FIXED_SCALE);
-            }
-        }
-        pj.x = rot1(pj.x);
-        pj.y = rot1(pj.y);
-        pj.z = rot1(pj.z);
-        aj.x = rot1(aj.x);
-        aj.y = rot1(aj.y);
-        aj.z = rot1(aj.z);
-        jc = rot1(jc);
-        if (ALCH) {
-            pj.w = rot1(pj.w);
-            if (DL) {
-                aj.w = rot1(aj.w);
-            }
-        }
-        if (DX) {
-            gjx = rot1(gjx);
-            gjy = rot1(gjy);
-            gjz = rot1(gjz);
-        }
-        if (DP) {
-            gqj = rot1(gqj);
-            gsj = rot1(gsj);
-            gej = rot1(gej);
         }
     }
+    if (queued > done) {
+        wave_lds_sync();
+        eval_pairs<ALCH, U, DX, DL, DP, DLDP, NACC>(L, done, queued - done, beta, uacc, dlacc);
+    }
+    wave_lds_sync();
 
     const bool vj = jl0 < k;
-    if (DX || DP) {
+    if (DX || DP || DLDP) {
         if (vi) {
             const int gi = S.atom0 + static_cast<int>(perm[S.slot0 + i]);
             if (DX) {
-                atomic_add_u64(du_dx + gi * 3 + 0, gix);
-                atomic_add_u64(du_dx + gi * 3 + 1, giy);
-                atomic_add_u64(du_dx + gi * 3 + 2, giz);
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const uint64_t v = L.acc[0][c][lane];
+                    if (v != 0) {
+                        atomic_add_u64(du_dx + gi * 3 + c, v);
+                    }
+                }
             }
-            if (DP) {
-                atomic_add_u64(du_dp + gi * 3 + 0, gqi);
-                atomic_add_u64(du_dp + gi * 3 + 1, gsi);
-                atomic_add_u64(du_dp + gi * 3 + 2, gei);
+            if (DP || DLDP) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const uint64_t v = L.acc[0][PQ + c][lane];
+                    if (v != 0) {
+                        atomic_add_u64(du_dp + gi * 3 + c, v);
+                    }
+                }
             }
         }
         if (vj) {
             const int gj = S.atom0 + static_cast<int>(perm[S.slot0 + jl0]);
             if (DX) {
-                atomic_add_u64(du_dx + gj * 3 + 0, gjx);
-                atomic_add_u64(du_dx + gj * 3 + 1, gjy);
-                atomic_add_u64(du_dx + gj * 3 + 2, gjz);
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const uint64_t v = L.acc[1][c][lane];
+                    if (v != 0) {
+                        atomic_add_u64(du_dx + gj * 3 + c, v);
+                    }
+                }
             }
-            if (DP) {
-                atomic_add_u64(du_dp + gj * 3 + 0, gqj);
-                atomic_add_u64(du_dp + gj * 3 + 1, gsj);
-                atomic_add_u64(du_dp + gj * 3 + 2, gej);
+            if (DP || DLDP) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const uint64_t v = L.acc[1][PQ + c][lane];
+                    if (v != 0) {
+                        atomic_add_u64(du_dp + gj * 3 + c, v);
+                    }
+                }
             }
         }
     }
@@ -179,7 +276,7 @@ __device__ __forceinline__ void tile_body(
 
 // Grid-stride over all tiles of all systems (the grid is sized to the
 // machine, the tile count is read on the device).
-template <bool U, bool DX, bool DL, bool DP>
+template <bool U, bool DX, bool DL, bool DP, bool DLDP = false>
 __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded_tiles(
     const NbSys *__restrict__ sys,
     const int S,
@@ -195,10 +292,14 @@ __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded_tiles(
     uint64_t *__restrict__ du_dp,
     uint64_t *__restrict__ du_dl,
     uint64_t *__restrict__ u) {
+    constexpr int NACC = (DX ? 3 : 0) + ((DP || DLDP) ? 3 : 0) > 0 ? (DX ? 3 : 0) + ((DP || DLDP) ? 3 : 0) : 1;
+    constexpr int WAVES = NB_BLOCK / 64;
+    __shared__ TileLds<NACC> lds[WAVES];
     const int lane = threadIdx.x & 63;
-    const int waves_per_block = NB_BLOCK / 64;
-    const int first = uniform(blockIdx.x * waves_per_block + (threadIdx.x >> 6));
-    const int stride = gridDim.x * waves_per_block;
+    const int wave = threadIdx.x >> 6;
+    TileLds<NACC> &L = lds[wave];
+    const int first = uniform(blockIdx.x * WAVES + wave);
+    const int stride = gridDim.x * WAVES;
     const int total = tile_prefix[S];
     for (int t = first; t < total; t += stride) {
         int lo = 0, hi = S - 1;
@@ -226,9 +327,11 @@ __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded_tiles(
         const float4 aj = vj ? par[Sd.slot0 + jl] : make_float4(0.f, 0.f, 0.f, 0.f);
         const bool alch = ballot(pi.w != 0.0 || pj.w != 0.0) != 0;
         if (alch) {
-            tile_body<true, U, DX, DL, DP>(Sd, s, r, jl, pi, ai, pj, aj, perm, boxes, du_dx, du_dp, du_dl, u);
-        } else {
-            tile_body<false, U, DX, DL, DP>(Sd, s, r, jl, pi, ai, pj, aj, perm, boxes, du_dx, du_dp, du_dl, u);
+            tile_body<true, U, DX, DL, DP, DLDP, NACC>(L, Sd, s, r, jl, pi, ai, pj, aj, perm, boxes, du_dx, du_dp,
+                                                       du_dl, u);
+        } else if (!DLDP) {
+            tile_body<false, U, DX, DL, DP, false, NACC>(L, Sd, s, r, jl, pi, ai, pj, aj, perm, boxes, du_dx, du_dp,
+                                                         du_dl, u);
         }
     }
 }
@@ -247,7 +350,7 @@ const TileKernel tile_kernels[16] = {
     k_nonbonded_tiles<1, 1, 1, 1>,
 };
 
-template <bool U, bool DX, bool DL, bool DP>
+template <bool U, bool DX, bool DL, bool DP, bool DLDP = false>
 __global__ __launch_bounds__(256) void k_pair_list(
     const PairListArgs a, uint64_t *__restrict__ du_dx, uint64_t *__restrict__ du_dp, uint64_t *__restrict__ du_dl,
     uint64_t *__restrict__ u) {
@@ -323,6 +426,22 @@ __global__ __launch_bounds__(256) void k_pair_list(
             acc(du_dl + red, f64_to_fixed(dw * T.prefactor * (dwdl_i - dwdl_j), FIXED_SCALE));
         }
     }
+    if (DLDP) {
+        const bool vanilla = (wi == 0.0) && (wj == 0.0);
+        if (!vanilla) {
+            const PairDlDp D = nb_pair_dl_dp(d2, si, sj, ei, ej, static_cast<double>(beta), scale_lj);
+            const double dd = dw * (dwdl_i - dwdl_j);
+            acc(du_dp + gi * 3 + 0, f64_to_fixed(((scale_q * qj) * D.es_unit) * dd, FIXED_SCALE));
+            acc(du_dp + gj * 3 + 0, f64_to_fixed(((scale_q * qi) * D.es_unit) * dd, FIXED_SCALE));
+            if (D.lj) {
+                const uint64_t gs = f64_to_fixed(D.lj_dsig * dd, FIXED_SCALE_DSIG);
+                acc(du_dp + gi * 3 + 1, gs);
+                acc(du_dp + gj * 3 + 1, gs);
+                acc(du_dp + gi * 3 + 2, f64_to_fixed((D.lj_unit * ej) * dd, FIXED_SCALE_DEPS));
+                acc(du_dp + gj * 3 + 2, f64_to_fixed((D.lj_unit * ei) * dd, FIXED_SCALE_DEPS));
+            }
+        }
+    }
     if (U) {
         acc(u + red, f64_to_fixed(T.u, FIXED_SCALE));
     }
@@ -347,7 +466,7 @@ int default_tile_grid() {
         TM_HIP_CHECK(hipGetDevice(&dev));
         int cus = 0;
         TM_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        grid = cus * 8; // 8 workgroups (32 waves) per CU: enough to hide latency, no tail blow-up
+        grid = cus * 16; // 16 two-wave workgroups (32 waves) per CU requested; LDS and VGPRs admit fewer
     }
     return grid;
 }
@@ -380,6 +499,23 @@ void launch_pair_list(
         return;
     }
     pair_kernels[idx]<<<ceil_div(a.M, 256), 256, 0, stream>>>(a, du_dx, du_dp, du_dl, u);
+    TM_LAUNCH_CHECK();
+}
+
+void NbState::compute_dl_dp(const double *boxes, uint64_t *dl_dp, hipStream_t stream, int grid_override) const {
+    const int grid = grid_override > 0 ? grid_override : default_tile_grid();
+    k_nonbonded_tiles<false, false, false, false, true><<<grid, NB_BLOCK, 0, stream>>>(
+        sys.ptr, S, tile_prefix.ptr, tile_row.ptr, tile_atoms.ptr, perm.ptr, pos.ptr, par.ptr, boxes, status.ptr,
+        nullptr, dl_dp, nullptr, nullptr);
+    TM_LAUNCH_CHECK();
+}
+
+void launch_pair_list_dl_dp(const PairListArgs &a, uint64_t *dl_dp, hipStream_t stream) {
+    if (a.M == 0) {
+        return;
+    }
+    k_pair_list<false, false, false, false, true><<<ceil_div(a.M, 256), 256, 0, stream>>>(a, nullptr, dl_dp, nullptr,
+                                                                                          nullptr);
     TM_LAUNCH_CHECK();
 }
 

@@ -37,6 +37,15 @@ class Potential {
         uint64_t *d_u,
         hipStream_t stream) = 0;
 
+    // d(du/dlambda)/dp accumulated into d_dl_dp [P] with the same fixed-point
+    // exponents as du_dp. No reference counterpart (the reference takes
+    // second derivatives from its JAX potentials); used for gradients of
+    // lambda-window averages of du/dlambda.
+    virtual void execute_dl_dp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box,
+                                      double lambda, uint64_t *d_dl_dp, hipStream_t stream) = 0;
+    void execute_dl_dp_host(int N, int P, const double *h_x, const double *h_p, const double *h_box, double lambda,
+                            uint64_t *h_dl_dp);
+
     // Host-side validation possible before launching with host inputs.
     virtual void validate_host(int N, int P, const double *h_box) const {}
     // Reads device status flags (synchronises the stream) and raises the
@@ -83,6 +92,8 @@ class NonbondedAllPairs : public Potential {
     void execute_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
                         uint64_t *d_du_dx, uint64_t *d_du_dp, uint64_t *d_du_dl, uint64_t *d_u,
                         hipStream_t stream) override;
+    void execute_dl_dp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
+                              uint64_t *d_dl_dp, hipStream_t stream) override;
     void validate_host(int N, int P, const double *h_box) const override;
     void check_device_status(hipStream_t stream) override;
     void du_dp_fixed_to_float(int N, int P, const uint64_t *du_dp, double *out) const override;
@@ -100,6 +111,7 @@ class NonbondedAllPairs : public Potential {
 
   private:
     void ensure_state();
+    void validate_sizes(int N, int P) const;
     bool dirty_ = true;
     DeviceArray<int> d_plane_, d_offset_;
 };
@@ -113,6 +125,8 @@ class NonbondedPairList : public Potential {
     void execute_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
                         uint64_t *d_du_dx, uint64_t *d_du_dp, uint64_t *d_du_dl, uint64_t *d_u,
                         hipStream_t stream) override;
+    void execute_dl_dp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
+                              uint64_t *d_dl_dp, hipStream_t stream) override;
     void du_dp_fixed_to_float(int N, int P, const uint64_t *du_dp, double *out) const override;
 
     int N_, M_;
@@ -122,6 +136,7 @@ class NonbondedPairList : public Potential {
     std::vector<double> scales_;
 
   private:
+    PairListArgs args(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda) const;
     DeviceArray<int> d_pairs_, d_plane_, d_offset_;
     DeviceArray<double> d_scales_;
 };
@@ -134,6 +149,8 @@ class BondedPotential : public Potential {
     void execute_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
                         uint64_t *d_du_dx, uint64_t *d_du_dp, uint64_t *d_du_dl, uint64_t *d_u,
                         hipStream_t stream) override;
+    void execute_dl_dp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
+                              uint64_t *d_dl_dp, hipStream_t stream) override;
 
     BondedKind bk_;
     int n_;
@@ -150,6 +167,8 @@ class SummedPotential : public Potential {
     void execute_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
                         uint64_t *d_du_dx, uint64_t *d_du_dp, uint64_t *d_du_dl, uint64_t *d_u,
                         hipStream_t stream) override;
+    void execute_dl_dp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
+                              uint64_t *d_dl_dp, hipStream_t stream) override;
     void validate_host(int N, int P, const double *h_box) const override;
     void check_device_status(hipStream_t stream) override;
     void du_dp_fixed_to_float(int N, int P, const uint64_t *du_dp, double *out) const override;
@@ -165,6 +184,8 @@ class FanoutSummedPotential : public Potential {
     void execute_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
                         uint64_t *d_du_dx, uint64_t *d_du_dp, uint64_t *d_du_dl, uint64_t *d_u,
                         hipStream_t stream) override;
+    void execute_dl_dp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
+                              uint64_t *d_dl_dp, hipStream_t stream) override;
     void validate_host(int N, int P, const double *h_box) const override;
     void check_device_status(hipStream_t stream) override;
     void du_dp_fixed_to_float(int N, int P, const uint64_t *du_dp, double *out) const override;

@@ -57,6 +57,23 @@ void Potential::execute_host(int N, int P, const double *h_x, const double *h_p,
     TM_HIP_CHECK(hipStreamSynchronize(0));
 }
 
+void Potential::execute_dl_dp_host(int N, int P, const double *h_x, const double *h_p, const double *h_box,
+                                   double lambda, uint64_t *h_dl_dp) {
+    validate_host(N, P, h_box);
+    DeviceArray<double> x(static_cast<size_t>(N) * 3), box(9), p(std::max(P, 1));
+    x.upload(h_x, static_cast<size_t>(N) * 3);
+    box.upload(h_box, 9);
+    if (P > 0) {
+        p.upload(h_p, P);
+    }
+    DeviceArray<uint64_t> dl_dp(std::max(P, 1));
+    dl_dp.zero();
+    execute_dl_dp_device(N, P, x.ptr, P > 0 ? p.ptr : nullptr, box.ptr, lambda, dl_dp.ptr, 0);
+    check_device_status(0);
+    dl_dp.download(h_dl_dp, P);
+    TM_HIP_CHECK(hipStreamSynchronize(0));
+}
+
 void Potential::execute_batch_host(int coord_batch_size, int N, int param_batch_size, int P, int lambda_batch_size,
                                    const double *h_x, const double *h_p, const double *h_box, const double *h_lambdas,
                                    uint64_t *h_du_dx, uint64_t *h_du_dp, uint64_t *h_du_dl, uint64_t *h_u) {
@@ -186,6 +203,11 @@ void NonbondedAllPairs::ensure_state() {
 }
 
 void NonbondedAllPairs::validate_host(int N, int P, const double *h_box) const {
+    validate_sizes(N, P);
+    validate_nonbonded_box(h_box, cutoff_, padding_);
+}
+
+void NonbondedAllPairs::validate_sizes(int N, int P) const {
     if (N != N_) {
         throw Error("NonbondedAllPairs::execute_device(): expected N == N_, got N=" + std::to_string(N) +
                     ", N_=" + std::to_string(N_));
@@ -194,26 +216,29 @@ void NonbondedAllPairs::validate_host(int N, int P, const double *h_box) const {
         throw Error("NonbondedAllPairs::execute_device(): expected P == M*N_*3, got P=" + std::to_string(P) +
                     ", M*N_*3=" + std::to_string(N_ * 3));
     }
-    validate_nonbonded_box(h_box, cutoff_, padding_);
 }
 
 void NonbondedAllPairs::execute_device(int N, int P, const double *d_x, const double *d_p, const double *d_box,
                                        double lambda, uint64_t *d_du_dx, uint64_t *d_du_dp, uint64_t *d_du_dl,
                                        uint64_t *d_u, hipStream_t stream) {
-    if (N != N_) {
-        throw Error("NonbondedAllPairs::execute_device(): expected N == N_, got N=" + std::to_string(N) +
-                    ", N_=" + std::to_string(N_));
-    }
-    if (P != N_ * 3) {
-        throw Error("NonbondedAllPairs::execute_device(): expected P == M*N_*3, got P=" + std::to_string(P) +
-                    ", M*N_*3=" + std::to_string(N_ * 3));
-    }
+    validate_sizes(N, P);
     ensure_state();
     if (K_ == 0) {
         return;
     }
     st_.prepare(d_x, d_p, d_box, d_plane_.ptr, d_offset_.ptr, nullptr, lambda, stream);
     st_.compute(d_box, d_du_dx, d_du_dp, d_du_dl, d_u, stream);
+}
+
+void NonbondedAllPairs::execute_dl_dp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box,
+                                             double lambda, uint64_t *d_dl_dp, hipStream_t stream) {
+    validate_sizes(N, P);
+    ensure_state();
+    if (K_ == 0) {
+        return;
+    }
+    st_.prepare(d_x, d_p, d_box, d_plane_.ptr, d_offset_.ptr, nullptr, lambda, stream);
+    st_.compute_dl_dp(d_box, d_dl_dp, stream);
 }
 
 void NonbondedAllPairs::check_device_status(hipStream_t stream) {
@@ -293,9 +318,19 @@ NonbondedPairList::NonbondedPairList(const std::vector<int> &pair_idxs, const st
     TM_HIP_CHECK(hipDeviceSynchronize());
 }
 
+void NonbondedPairList::execute_dl_dp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box,
+                                             double lambda, uint64_t *d_dl_dp, hipStream_t stream) {
+    launch_pair_list_dl_dp(args(N, P, d_x, d_p, d_box, lambda), d_dl_dp, stream);
+}
+
 void NonbondedPairList::execute_device(int N, int P, const double *d_x, const double *d_p, const double *d_box,
                                        double lambda, uint64_t *d_du_dx, uint64_t *d_du_dp, uint64_t *d_du_dl,
                                        uint64_t *d_u, hipStream_t stream) {
+    launch_pair_list(args(N, P, d_x, d_p, d_box, lambda), d_du_dx, d_du_dp, d_du_dl, d_u, stream);
+}
+
+PairListArgs NonbondedPairList::args(int N, int P, const double *d_x, const double *d_p, const double *d_box,
+                                     double lambda) const {
     if (N != N_) {
         throw Error("NonbondedPairList::execute_device(): expected N == N_, got N=" + std::to_string(N) +
                     ", N_=" + std::to_string(N_));
@@ -322,7 +357,7 @@ void NonbondedPairList::execute_device(int N, int P, const double *d_x, const do
     a.betas = nullptr;
     a.beta0 = static_cast<float>(beta_);
     a.negated = negated_ ? 1 : 0;
-    launch_pair_list(a, d_du_dx, d_du_dp, d_du_dl, d_u, stream);
+    return a;
 }
 
 void NonbondedPairList::du_dp_fixed_to_float(int N, int P, const uint64_t *du_dp, double *out) const {
@@ -407,6 +442,17 @@ void BondedPotential::execute_device(int N, int P, const double *d_x, const doub
     launch_bonded(bk_, t, d_x, nullptr, lambda, nullptr, d_du_dx, d_du_dp, d_du_dl, d_u, stream);
 }
 
+void BondedPotential::execute_dl_dp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box,
+                                           double lambda, uint64_t *d_dl_dp, hipStream_t stream) {
+    if (n_ == 0) {
+        return;
+    }
+    // same size validation as execute_device
+    execute_device(N, P, d_x, d_p, d_box, lambda, nullptr, nullptr, nullptr, nullptr, stream);
+    BondedTerms t{n_, d_idxs_.ptr, d_p, d_mult_.ptr, d_offset_.ptr, nullptr};
+    launch_bonded(bk_, t, d_x, nullptr, lambda, nullptr, nullptr, d_dl_dp, nullptr, nullptr, stream, true);
+}
+
 // ---------------------------------------------------------------- sums
 
 SummedPotential::SummedPotential(const std::vector<PotentialPtr> &pots, const std::vector<int> &sizes)
@@ -427,6 +473,19 @@ void SummedPotential::execute_device(int N, int P, const double *d_x, const doub
     for (size_t i = 0; i < pots_.size(); i++) {
         pots_[i]->execute_device(N, sizes_[i], d_x, d_p + off, d_box, lambda, d_du_dx,
                                  d_du_dp ? d_du_dp + off : nullptr, d_du_dl, d_u, stream);
+        off += sizes_[i];
+    }
+}
+
+void SummedPotential::execute_dl_dp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box,
+                                           double lambda, uint64_t *d_dl_dp, hipStream_t stream) {
+    if (P != P_) {
+        throw Error("SummedPotential::execute_device(): expected " + std::to_string(P_) + " parameters, got " +
+                    std::to_string(P));
+    }
+    int off = 0;
+    for (size_t i = 0; i < pots_.size(); i++) {
+        pots_[i]->execute_dl_dp_device(N, sizes_[i], d_x, d_p + off, d_box, lambda, d_dl_dp + off, stream);
         off += sizes_[i];
     }
 }
@@ -458,6 +517,14 @@ void FanoutSummedPotential::execute_device(int N, int P, const double *d_x, cons
                                            uint64_t *d_u, hipStream_t stream) {
     for (auto &p : pots_) {
         p->execute_device(N, P, d_x, d_p, d_box, lambda, d_du_dx, d_du_dp, d_du_dl, d_u, stream);
+    }
+}
+
+void FanoutSummedPotential::execute_dl_dp_device(int N, int P, const double *d_x, const double *d_p,
+                                                 const double *d_box, double lambda, uint64_t *d_dl_dp,
+                                                 hipStream_t stream) {
+    for (auto &p : pots_) {
+        p->execute_dl_dp_device(N, P, d_x, d_p, d_box, lambda, d_dl_dp, stream);
     }
 }
 

@@ -118,6 +118,20 @@ class Potential(_Handle):
     def execute_du_dx(self, coords, params, box, lam):
         return self.execute_selective(coords, params, box, lam, True, False, False, False)[0]
 
+    def execute_dl_dp(self, coords, params, box, lam):
+        """d(du/dlambda)/dp, shaped like params. Extension of this library (the
+        reference obtains it from its JAX potentials); used for gradients of
+        lambda-window averages of du/dlambda."""
+        coords = f64(coords)
+        params = f64(params)
+        box = f64(box)
+        N = coords.shape[0]
+        P = params.size
+        out = np.zeros(max(P, 1), dtype=np.uint64)
+        call("tm_potential_execute_dl_dp_host", self._h, N, P, pd(coords), pd(params.reshape(-1)) if P else None,
+             pd(box), float(lam), pu64(out))
+        return self._du_dp_to_float(N, P, out, params.shape)
+
     def execute_selective_batch(
         self, coords, params, boxes, lambs, compute_du_dx, compute_du_dp, compute_du_dl, compute_u
     ):
@@ -778,3 +792,32 @@ class BatchedContexts(_Handle):
         n = ctypes.c_int(0)
         call("tm_batch_total_tiles", self._h, ctypes.byref(n))
         return n.value
+
+    # -- gradient observer (extension): statistics for d<du/dlambda>/dp
+
+    def set_grad_observer(self, interval: int):
+        """Observe every `interval` steps of multiple_steps (at the du_dl store
+        states); resets the sums. 0 turns it off."""
+        call("tm_batch_set_grad_observer", self._h, int(interval))
+
+    def observe(self):
+        """One observation of every system at its current state."""
+        call("tm_batch_observe", self._h)
+
+    def grad_size(self, s: int) -> int:
+        n = ctypes.c_int(0)
+        call("tm_batch_grad_size", self._h, int(s), ctypes.byref(n))
+        return n.value
+
+    def grad_stats(self, s: int) -> dict:
+        """Sums over observed frames of system s: n, A, A^2 (A = du/dlambda) and,
+        per flat parameter (nonbonded [N,3], bond [B,2], angle [A,2], torsion
+        [T,3]), g = du/dp, A g and h = d(du/dlambda)/dp."""
+        P = self.grad_size(s)
+        n = ctypes.c_int(0)
+        a = ctypes.c_double(0.0)
+        aa = ctypes.c_double(0.0)
+        g, ag, h = np.zeros(P), np.zeros(P), np.zeros(P)
+        call("tm_batch_grad_stats", self._h, int(s), ctypes.byref(n), ctypes.byref(a), ctypes.byref(aa), pd(g),
+             pd(ag), pd(h))
+        return {"n": n.value, "sum_a": a.value, "sum_aa": aa.value, "sum_g": g, "sum_ag": ag, "sum_h": h}
