@@ -51,6 +51,20 @@ TM_HD inline double rcp_nr(double a) {
     return y * (2.0 - a * y);
 }
 
+constexpr double LOG2_E = 1.4426950408889634;
+constexpr double LN_2 = 0.6931471805599453;
+
+// exp(-a), a >= 0: 2^(-t) with t = a log2(e) in FP64; the FP32 hardware 2^x of
+// t rounded to FP32, times 2^(-r) ~ 1 - r ln2 for the rounding remainder r
+// (|r ln2| < 4e-7 for a < 9, so the dropped second-order term is < 1e-13).
+// Relative error ~1 FP32 ulp, from the hardware 2^x.
+TM_HD inline double exp_neg(double a) {
+    const double t = a * LOG2_E;
+    const float tf = static_cast<float>(t);
+    const double e = static_cast<double>(TM_EXP2F(-tf));
+    return e - e * ((t - static_cast<double>(tf)) * LN_2);
+}
+
 struct PairTerms {
     double prefactor;  // (dU/dd) / d : du/dx_i = prefactor * (x_i - x_j)
     double u;          // pair energy (valid when requested)
@@ -61,10 +75,9 @@ struct PairTerms {
 };
 
 // Pair terms from the squared (4D) distance. Mixed precision chosen for the
-// 1e-5 per-atom force target: FP64 everywhere except exp(-x^2), which is the
-// FP32 hardware exp of x^2 rounded to FP32 with the rounding remainder folded
-// back in FP64 (relative error ~1 FP32 ulp), and the FP32 rsq/rcp seeds that
-// one FP64 Newton step refines. Parameters arrive as FP32 values (the
+// 1e-5 per-atom force target: FP64 everywhere except exp(-x^2) (exp_neg,
+// relative error ~1 FP32 ulp) and the FP32 rsq/rcp seeds that one FP64
+// Newton step refines. Parameters arrive as FP32 values (the
 // reference's per-atom parameter precision).
 template <bool COMPUTE_U, bool COMPUTE_DP>
 TM_HD inline PairTerms nb_pair_terms(
@@ -84,10 +97,7 @@ TM_HD inline PairTerms nb_pair_terms(
     const double inv_d2 = inv_d * inv_d;
 
     const double x = beta * d;
-    const double x2 = x * x;
-    const float x2f = static_cast<float>(x2);
-    double e = static_cast<double>(std::exp(-x2f));
-    e = e - e * (x2 - static_cast<double>(x2f));
+    const double e = exp_neg(x * x);
     const double t = rcp_nr(1.0 + ERFC_P * x);
     double poly = ERFC_C7;
     poly = __builtin_fma(poly, t, ERFC_C6);
@@ -154,10 +164,7 @@ TM_HD inline PairDlDp nb_pair_dl_dp(double d2, double sig_i, double sig_j, doubl
     const double inv_d2 = inv_d * inv_d;
 
     const double x = beta * d;
-    const double x2 = x * x;
-    const float x2f = static_cast<float>(x2);
-    double e = static_cast<double>(std::exp(-x2f));
-    e = e - e * (x2 - static_cast<double>(x2f));
+    const double e = exp_neg(x * x);
     const double t = rcp_nr(1.0 + ERFC_P * x);
     double poly = ERFC_C7;
     poly = __builtin_fma(poly, t, ERFC_C6);
