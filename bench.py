@@ -130,6 +130,11 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--tile-grid", type=int, default=0)
+    ap.add_argument("--profile-only", type=int, default=0, metavar="REPS",
+                    help="build the batch, warm up, launch each kernel group REPS times eagerly and exit "
+                    "(for rocprofv3 counter runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="one 256 MiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -138,6 +143,17 @@ def main() -> int:
             file=sys.stderr,
         )
         return 2
+
+    if args.copy_calibration:
+        import torch
+
+        n = 256 * 2**20
+        src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copied {n} bytes ({n // 1024} KiB read, {n // 1024} KiB written)")
+        return 0
 
     from timemachine_amd import _lib
     from timemachine_amd import testsystems as ts
@@ -156,6 +172,10 @@ def main() -> int:
     use_graph = not args.no_graph
     log(f"warmup {args.warmup} batched steps")
     batch.multiple_steps(args.warmup, store_du_dl_interval=1, use_graph=use_graph)
+    if args.profile_only:
+        prof = batch.profile(reps=args.profile_only)
+        log(f"kernel ms: {prof}, tiles {batch.total_tiles()}")
+        return 0
     log(f"timing {args.steps} batched steps")
     t0 = time.perf_counter()
     du_dl, _, _ = batch.multiple_steps(args.steps, store_du_dl_interval=1, use_graph=use_graph)

@@ -177,6 +177,14 @@ int tm_context_get_du_dx_t_minus_1(tm_context *c, uint64_t *out);
  * of each system. */
 int tm_batch_create(int n, tm_context *const *contexts, const double *lambdas, tm_batch **out);
 void tm_batch_release(tm_batch *b);
+/* Continuous batching (no reference counterpart): a new batch holding systems keep[0..n_keep)
+ * of `old` (their x, v, box, step counters and barostat state copied device to device, never
+ * through the host; neighborlists rebuilt on the next step) followed by n_new further contexts.
+ * lambdas: [n_keep + n_new]. `old` is left unchanged and may then be released. */
+int tm_batch_repack(tm_batch *old, int n_keep, const int32_t *keep, int n_new, tm_context *const *new_contexts,
+                    const double *lambdas, tm_batch **out);
+/* Device bytes held by the batch's engine (admission control). */
+int tm_batch_device_bytes(tm_batch *b, uint64_t *out);
 int tm_batch_num_systems(tm_batch *b, int *out);
 int tm_batch_total_atoms(tm_batch *b, int *out);
 /* Steps every system n_steps. lambda_sched [n, n_steps] or NULL (keep lambdas).

@@ -94,7 +94,7 @@ extern "C" void hc_nonbonded_du_dx(int N, const double *x, const double *params,
         const double dx = min_image_d(x[3 * i] - x[3 * j], bx, ibx);
         const double dy = min_image_d(x[3 * i + 1] - x[3 * j + 1], by, iby);
         const double dz = min_image_d(x[3 * i + 2] - x[3 * j + 2], bz, ibz);
-        const double d2 = dx * dx + dy * dy + dz * dz;
+        const double d2 = pair_d2(dx, dy, dz);
         if (!(d2 < cutoff * cutoff)) continue;
         auto f32 = [](double v) { return (double)(float)v; };
         PairTerms t = nb_pair_terms<true, false>(d2, f32(params[3 * i]), f32(params[3 * j]), f32(params[3 * i + 1]),

@@ -1,0 +1,86 @@
+"""Continuous batching (GPU): device-to-device state handover, results
+independent of the mix (bitwise), HBM budget respected."""
+
+import numpy as np
+import pytest
+
+from timemachine_amd import serving
+from timemachine_amd import testsystems as ts
+from timemachine_amd.lib import custom_ops
+
+pytestmark = pytest.mark.gpu
+
+
+def _small_job(i, n_waters, W, steps, arrival, heavy=8, barostat=5):
+    side = max((n_waters / 33.4) ** (1 / 3), 2.62)
+    sys_ = ts.ligand_in_water(side=side, n_waters=n_waters, n_heavy=heavy, n_ligand=int(heavy * 1.5), seed=100 + i)
+    return serving.Job(i, sys_, np.linspace(0, 1, W), steps, seed=7 + i, arrival=arrival, barostat_interval=barostat)
+
+
+@pytest.fixture(scope="module")
+def jobs(gpu):
+    return [
+        _small_job(0, 620, 3, 60, 0.0),
+        _small_job(1, 700, 2, 40, 0.0),
+        _small_job(2, 640, 4, 80, 0.01),
+        _small_job(3, 900, 2, 20, 0.02),
+    ]
+
+
+def test_repack_hands_state_over_bitwise(jobs):
+    """Run 30 steps in one batch, or 15 + repack (dropping one system, adding
+    another) + 15: the systems that stay end bitwise identical, du/dl series too."""
+    job = jobs[0]
+    a = serving.make_window_contexts(job)
+    ref = custom_ops.BatchedContexts([c for c, _ in a], job.lambdas)
+    dl_ref, _, _ = ref.multiple_steps(30, store_du_dl_interval=1)
+    x_ref = [ref.get_x_t(s) for s in range(3)]
+    v_ref = [ref.get_v_t(s) for s in range(3)]
+    box_ref = [ref.get_box(s) for s in range(3)]
+
+    b = serving.make_window_contexts(job)
+    extra = serving.make_window_contexts(jobs[1])
+    batch = custom_ops.BatchedContexts([c for c, _ in b], job.lambdas)
+    dl1, _, _ = batch.multiple_steps(15, store_du_dl_interval=1)
+    batch2 = batch.repack([2, 0], [extra[0][0]], [job.lambdas[2], job.lambdas[0], jobs[1].lambdas[0]])
+    del batch
+    dl2, _, _ = batch2.multiple_steps(15, store_du_dl_interval=1)
+    for new_s, old_s in ((0, 2), (1, 0)):
+        assert np.array_equal(batch2.get_x_t(new_s), x_ref[old_s])
+        assert np.array_equal(batch2.get_v_t(new_s), v_ref[old_s])
+        assert np.array_equal(batch2.get_box(new_s), box_ref[old_s])
+        assert np.array_equal(np.concatenate([dl1[old_s], dl2[new_s]]), dl_ref[old_s])
+
+
+def test_repack_validation(jobs):
+    a = serving.make_window_contexts(jobs[1])
+    batch = custom_ops.BatchedContexts([c for c, _ in a], jobs[1].lambdas)
+    with pytest.raises(RuntimeError, match="unique systems"):
+        batch.repack([0, 0], [], [0.0, 0.0])
+    with pytest.raises(RuntimeError, match="at least one system"):
+        batch.repack([], [], [])
+    with pytest.raises(RuntimeError, match="one lambda per system"):
+        batch.repack([0], [], [0.0, 1.0])
+
+
+def test_served_mix_equals_each_job_alone(jobs):
+    """The budget admits about two jobs at a time, so jobs wait, share the
+    engine with different neighbours and leave at different chunk boundaries;
+    every job's du/dl series is bitwise the one it gets alone."""
+    budget = sum(serving.estimate_system_bytes(j.system.N) * j.windows for j in jobs[:2]) + 1
+    srv = serving.Server(budget, max_chunk=30, du_dl_interval=10)
+    served = srv.run(jobs)
+    alone = serving.run_fifo(jobs, du_dl_interval=10)
+    assert [r.job_id for r in served] == [0, 1, 2, 3]
+    for r, q in zip(served, alone):
+        assert r.du_dl.shape == (jobs[r.job_id].windows, jobs[r.job_id].n_steps // 10)
+        assert np.array_equal(r.du_dl, q.du_dl), r.job_id
+        assert r.dG == q.dG
+        assert r.finished >= r.admitted >= r.arrival
+    # admission control: the estimate bounds what the engine holds, within the budget
+    for rec in srv.log:
+        assert rec["budget_used"] <= budget
+        assert rec["device_bytes"] <= rec["budget_used"]
+    assert max(rec["systems"] for rec in srv.log) < sum(j.windows for j in jobs)  # never all at once
+    stats = serving.latency_stats(served)
+    assert stats["p99_s"] >= stats["p50_s"] > 0

@@ -97,6 +97,8 @@ _SIGS = {
     "tm_batch_get_v": (c_int, [_P, c_int, _PD]),
     "tm_batch_get_box": (c_int, [_P, c_int, _PD]),
     "tm_batch_write_back": (c_int, [_P]),
+    "tm_batch_repack": (c_int, [_P, c_int, _PI, c_int, _PPV, _PD, _PPV]),
+    "tm_batch_device_bytes": (c_int, [_P, _PU64]),
     "tm_batch_set_tile_grid": (c_int, [_P, c_int]),
     "tm_batch_set_grad_observer": (c_int, [_P, c_int]),
     "tm_batch_observe": (c_int, [_P]),

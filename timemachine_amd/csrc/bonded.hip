@@ -1,7 +1,9 @@
 // HarmonicBond / HarmonicAngle / PeriodicTorsion kernels, one thread per term,
 // over the terms of one or many systems (term_sys maps a term to its system
 // for lambda and the reduction index). Atom indices are global. Arithmetic is
-// FP64 (bonded_terms.h), accumulation 64-bit fixed point at 2^36.
+// FP64 (bonded_terms.h), accumulation 64-bit fixed point at 2^36. u and
+// du/dlambda are summed over the wave per system before one atomic each
+// (a per-term atomic on one address per system serialises the whole launch).
 #include "bonded.h"
 #include "bonded_terms.h"
 #include "fixed_point.h"
@@ -12,7 +14,10 @@ namespace tmamd {
 namespace {
 
 __device__ __forceinline__ void add_fixed(uint64_t *p, double v) {
-    atomicAdd(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(f64_to_fixed(v, FIXED_SCALE)));
+    const uint64_t f = f64_to_fixed(v, FIXED_SCALE);
+    if (f != 0) {
+        atomicAdd(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(f));
+    }
 }
 
 __device__ __forceinline__ void term_context(const BondedTerms &t, int term, const double *lambdas, double lambda0,
@@ -26,34 +31,38 @@ __global__ void k_harmonic_bond(const BondedTerms t, const double *__restrict__ 
                                 double lambda0, const int *__restrict__ red_idx, uint64_t *__restrict__ du_dx,
                                 uint64_t *__restrict__ du_dp, uint64_t *__restrict__ du_dl, uint64_t *__restrict__ u, int dl_dp) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= t.n) {
-        return;
-    }
-    double lam;
-    int red;
-    term_context(t, b, lambdas, lambda0, red_idx, lam, red);
-    const int i = t.idx[b * 2 + 0];
-    const int j = t.idx[b * 2 + 1];
-    const int mult = t.mult ? t.mult[b] : 0;
-    const int off = t.offset ? t.offset[b] : 1;
-    // dl_dp: d(du/dlambda)/dp = mult * d(shape)/dp, i.e. du/dp at prefactor mult
-    const double pf = dl_dp ? static_cast<double>(mult) : lam_prefactor<double>(mult, off, lam);
-    const BondOut<double> o = bond_term<double>(x + i * 3, x + j * 3, t.params[b * 2 + 0], t.params[b * 2 + 1], mult, pf);
-    if (du_dx) {
-        for (int d = 0; d < 3; d++) {
-            add_fixed(du_dx + i * 3 + d, o.g[0][d]);
-            add_fixed(du_dx + j * 3 + d, o.g[1][d]);
+    const bool active = b < t.n;
+    double lam = 0.0;
+    int red = -1;
+    uint64_t fu = 0, fdl = 0;
+    if (active) {
+        term_context(t, b, lambdas, lambda0, red_idx, lam, red);
+        const int i = t.idx[b * 2 + 0];
+        const int j = t.idx[b * 2 + 1];
+        const int mult = t.mult ? t.mult[b] : 0;
+        const int off = t.offset ? t.offset[b] : 1;
+        // dl_dp: d(du/dlambda)/dp = mult * d(shape)/dp, i.e. du/dp at prefactor mult
+        const double pf = dl_dp ? static_cast<double>(mult) : lam_prefactor<double>(mult, off, lam);
+        const BondOut<double> o =
+            bond_term<double>(x + i * 3, x + j * 3, t.params[b * 2 + 0], t.params[b * 2 + 1], mult, pf);
+        if (du_dx) {
+            for (int d = 0; d < 3; d++) {
+                add_fixed(du_dx + i * 3 + d, o.g[0][d]);
+                add_fixed(du_dx + j * 3 + d, o.g[1][d]);
+            }
         }
-    }
-    if (du_dp) {
-        add_fixed(du_dp + b * 2 + 0, o.dkb);
-        add_fixed(du_dp + b * 2 + 1, o.db0);
+        if (du_dp) {
+            add_fixed(du_dp + b * 2 + 0, o.dkb);
+            add_fixed(du_dp + b * 2 + 1, o.db0);
+        }
+        fu = f64_to_fixed(o.u, FIXED_SCALE);
+        fdl = f64_to_fixed(o.dl, FIXED_SCALE);
     }
     if (u) {
-        add_fixed(u + red, o.u);
+        wave_add_by_key(u, red, fu);
     }
     if (du_dl) {
-        add_fixed(du_dl + red, o.dl);
+        wave_add_by_key(du_dl, red, fdl);
     }
 }
 
@@ -62,37 +71,40 @@ __global__ void k_harmonic_angle(const BondedTerms t, const double *__restrict__
                                  uint64_t *__restrict__ du_dp, uint64_t *__restrict__ du_dl,
                                  uint64_t *__restrict__ u, int dl_dp) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= t.n) {
-        return;
-    }
-    double lam;
-    int red;
-    term_context(t, a, lambdas, lambda0, red_idx, lam, red);
-    const int i = t.idx[a * 3 + 0];
-    const int j = t.idx[a * 3 + 1];
-    const int k = t.idx[a * 3 + 2];
-    const int mult = t.mult ? t.mult[a] : 0;
-    const int off = t.offset ? t.offset[a] : 1;
-    // dl_dp: d(du/dlambda)/dp = mult * d(shape)/dp, i.e. du/dp at prefactor mult
-    const double pf = dl_dp ? static_cast<double>(mult) : lam_prefactor<double>(mult, off, lam);
-    const AngleOut<double> o =
-        angle_term<double>(x + i * 3, x + j * 3, x + k * 3, t.params[a * 2 + 0], t.params[a * 2 + 1], mult, pf);
-    if (du_dx) {
-        for (int d = 0; d < 3; d++) {
-            add_fixed(du_dx + i * 3 + d, o.g[0][d]);
-            add_fixed(du_dx + j * 3 + d, o.g[1][d]);
-            add_fixed(du_dx + k * 3 + d, o.g[2][d]);
+    const bool active = a < t.n;
+    double lam = 0.0;
+    int red = -1;
+    uint64_t fu = 0, fdl = 0;
+    if (active) {
+        term_context(t, a, lambdas, lambda0, red_idx, lam, red);
+        const int i = t.idx[a * 3 + 0];
+        const int j = t.idx[a * 3 + 1];
+        const int k = t.idx[a * 3 + 2];
+        const int mult = t.mult ? t.mult[a] : 0;
+        const int off = t.offset ? t.offset[a] : 1;
+        // dl_dp: d(du/dlambda)/dp = mult * d(shape)/dp, i.e. du/dp at prefactor mult
+        const double pf = dl_dp ? static_cast<double>(mult) : lam_prefactor<double>(mult, off, lam);
+        const AngleOut<double> o =
+            angle_term<double>(x + i * 3, x + j * 3, x + k * 3, t.params[a * 2 + 0], t.params[a * 2 + 1], mult, pf);
+        if (du_dx) {
+            for (int d = 0; d < 3; d++) {
+                add_fixed(du_dx + i * 3 + d, o.g[0][d]);
+                add_fixed(du_dx + j * 3 + d, o.g[1][d]);
+                add_fixed(du_dx + k * 3 + d, o.g[2][d]);
+            }
         }
-    }
-    if (du_dp) {
-        add_fixed(du_dp + a * 2 + 0, o.dka);
-        add_fixed(du_dp + a * 2 + 1, o.da0);
+        if (du_dp) {
+            add_fixed(du_dp + a * 2 + 0, o.dka);
+            add_fixed(du_dp + a * 2 + 1, o.da0);
+        }
+        fu = f64_to_fixed(o.u, FIXED_SCALE);
+        fdl = f64_to_fixed(o.dl, FIXED_SCALE);
     }
     if (u) {
-        add_fixed(u + red, o.u);
+        wave_add_by_key(u, red, fu);
     }
     if (du_dl) {
-        add_fixed(du_dl + red, o.dl);
+        wave_add_by_key(du_dl, red, fdl);
     }
 }
 
@@ -101,40 +113,44 @@ __global__ void k_periodic_torsion(const BondedTerms t, const double *__restrict
                                    uint64_t *__restrict__ du_dx, uint64_t *__restrict__ du_dp,
                                    uint64_t *__restrict__ du_dl, uint64_t *__restrict__ u, int dl_dp) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= t.n) {
-        return;
-    }
-    double lam;
-    int red;
-    term_context(t, q, lambdas, lambda0, red_idx, lam, red);
-    const int i = t.idx[q * 4 + 0];
-    const int j = t.idx[q * 4 + 1];
-    const int k = t.idx[q * 4 + 2];
-    const int l = t.idx[q * 4 + 3];
-    const int mult = t.mult ? t.mult[q] : 0;
-    const int off = t.offset ? t.offset[q] : 1;
-    // dl_dp: d(du/dlambda)/dp = mult * d(shape)/dp, i.e. du/dp at prefactor mult
-    const double pf = dl_dp ? static_cast<double>(mult) : lam_prefactor<double>(mult, off, lam);
-    const TorsionOut<double> o = torsion_term<double>(x + i * 3, x + j * 3, x + k * 3, x + l * 3, t.params[q * 3 + 0],
-                                                      t.params[q * 3 + 1], t.params[q * 3 + 2], mult, pf);
-    if (du_dx) {
-        const int at[4] = {i, j, k, l};
-        for (int a = 0; a < 4; a++) {
-            for (int d = 0; d < 3; d++) {
-                add_fixed(du_dx + at[a] * 3 + d, o.g[a][d]);
+    const bool active = q < t.n;
+    double lam = 0.0;
+    int red = -1;
+    uint64_t fu = 0, fdl = 0;
+    if (active) {
+        term_context(t, q, lambdas, lambda0, red_idx, lam, red);
+        const int i = t.idx[q * 4 + 0];
+        const int j = t.idx[q * 4 + 1];
+        const int k = t.idx[q * 4 + 2];
+        const int l = t.idx[q * 4 + 3];
+        const int mult = t.mult ? t.mult[q] : 0;
+        const int off = t.offset ? t.offset[q] : 1;
+        // dl_dp: d(du/dlambda)/dp = mult * d(shape)/dp, i.e. du/dp at prefactor mult
+        const double pf = dl_dp ? static_cast<double>(mult) : lam_prefactor<double>(mult, off, lam);
+        const TorsionOut<double> o = torsion_term<double>(x + i * 3, x + j * 3, x + k * 3, x + l * 3,
+                                                          t.params[q * 3 + 0], t.params[q * 3 + 1],
+                                                          t.params[q * 3 + 2], mult, pf);
+        if (du_dx) {
+            const int at[4] = {i, j, k, l};
+            for (int a = 0; a < 4; a++) {
+                for (int d = 0; d < 3; d++) {
+                    add_fixed(du_dx + at[a] * 3 + d, o.g[a][d]);
+                }
             }
         }
-    }
-    if (du_dp) {
-        add_fixed(du_dp + q * 3 + 0, o.dk);
-        add_fixed(du_dp + q * 3 + 1, o.dphase);
-        add_fixed(du_dp + q * 3 + 2, o.dperiod);
+        if (du_dp) {
+            add_fixed(du_dp + q * 3 + 0, o.dk);
+            add_fixed(du_dp + q * 3 + 1, o.dphase);
+            add_fixed(du_dp + q * 3 + 2, o.dperiod);
+        }
+        fu = f64_to_fixed(o.u, FIXED_SCALE);
+        fdl = f64_to_fixed(o.dl, FIXED_SCALE);
     }
     if (u) {
-        add_fixed(u + red, o.u);
+        wave_add_by_key(u, red, fu);
     }
     if (du_dl) {
-        add_fixed(du_dl + red, o.dl);
+        wave_add_by_key(du_dl, red, fdl);
     }
 }
 

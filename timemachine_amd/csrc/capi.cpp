@@ -620,6 +620,59 @@ int tm_batch_create(int n, tm_context *const *contexts, const double *lambdas, t
 
 void tm_batch_release(tm_batch *b) { delete b; }
 
+int tm_batch_repack(tm_batch *old, int n_keep, const int32_t *keep, int n_new, tm_context *const *new_contexts,
+                    const double *lambdas, tm_batch **out) {
+    return guarded([&] {
+        require(old, "old batch");
+        const int S_old = old->eng->num_systems();
+        if (n_keep < 0 || n_new < 0 || n_keep + n_new == 0) {
+            throw Error("tm_batch_repack: the new batch must hold at least one system");
+        }
+        std::vector<char> seen(S_old, 0);
+        for (int i = 0; i < n_keep; i++) {
+            if (keep[i] < 0 || keep[i] >= S_old || seen[keep[i]]) {
+                throw Error("tm_batch_repack: keep indices must be unique systems of the old batch");
+            }
+            seen[keep[i]] = 1;
+        }
+        std::vector<SystemSpec> specs;
+        auto *b = new tm_batch();
+        try {
+            for (int i = 0; i < n_keep; i++) {
+                tm_context *c = old->ctxs[keep[i]];
+                SystemSpec sp = c->spec;
+                sp.state_on_device = true; // handed over below, device to device
+                sp.x.clear();
+                sp.v.clear();
+                sp.box.clear();
+                specs.push_back(std::move(sp));
+                b->ctxs.push_back(c);
+            }
+            for (int i = 0; i < n_new; i++) {
+                tm_context *c = new_contexts[i];
+                ensure_engine(c);
+                sync_spec_from_engine(c);
+                specs.push_back(c->spec);
+                b->ctxs.push_back(c);
+            }
+            b->eng.reset(new Engine(specs));
+            for (int i = 0; i < n_keep; i++) {
+                b->eng->copy_state_from(*old->eng, keep[i], i);
+            }
+            std::vector<double> lam(lambdas, lambdas + n_keep + n_new);
+            b->eng->set_lambdas(lam);
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+    });
+}
+
+int tm_batch_device_bytes(tm_batch *b, uint64_t *out) {
+    return guarded([&] { *out = static_cast<uint64_t>(b->eng->device_bytes()); });
+}
+
 int tm_batch_num_systems(tm_batch *b, int *out) {
     return guarded([&] { *out = b->eng->num_systems(); });
 }

@@ -52,6 +52,9 @@ struct SystemSpec {
     int baro_accepted = 0;
     int baro_attempted = 0;
     int baro_step = -1; // barostat calls since construction / set_interval; -1: derive from step
+    // x, v, box, step and barostat state come from another engine through
+    // Engine::copy_state_from (x, v, box may then be empty here)
+    bool state_on_device = false;
 };
 
 class Engine {
@@ -73,6 +76,11 @@ class Engine {
     // [F, N_s, 3] blocks in system order) and box_out [S, F, 9].
     void run(int n_steps, const double *lambda_sched, int du_dl_interval, int x_interval, double *du_dl_out,
              double *x_out, double *box_out, bool use_graph);
+
+    // Device-to-device handover of system src_s of `src` (x, v, box, step
+    // counter, barostat state) into system dst_s of this engine; the
+    // neighborlists are rebuilt on the next step. Same atom count required.
+    void copy_state_from(Engine &src, int src_s, int dst_s);
 
     // Energies of every system at its current state and lambda (u, du_dl).
     void energies(double *u_out, double *du_dl_out);

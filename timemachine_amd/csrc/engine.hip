@@ -516,8 +516,8 @@ Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
     // layout
     int a0 = 0;
     for (auto &sp : systems) {
-        if (static_cast<int>(sp.x.size()) != sp.N * 3 || static_cast<int>(sp.v.size()) != sp.N * 3 ||
-            sp.box.size() != 9) {
+        if (!sp.state_on_device && (static_cast<int>(sp.x.size()) != sp.N * 3 ||
+                                    static_cast<int>(sp.v.size()) != sp.N * 3 || sp.box.size() != 9)) {
             throw Error("Engine: bad system state shapes");
         }
         if (static_cast<int>(sp.integ.cbs.size()) != sp.N || static_cast<int>(sp.integ.ccs.size()) != sp.N) {
@@ -537,9 +537,11 @@ Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
     std::vector<int> hatom_sys(A_), hred(S_);
     for (int s = 0; s < S_; s++) {
         const SystemSpec &sp = systems[s];
-        std::copy(sp.x.begin(), sp.x.end(), hx.begin() + I.atom0[s] * 3);
-        std::copy(sp.v.begin(), sp.v.end(), hv.begin() + I.atom0[s] * 3);
-        std::copy(sp.box.begin(), sp.box.end(), hbox.begin() + s * 9);
+        if (!sp.state_on_device) {
+            std::copy(sp.x.begin(), sp.x.end(), hx.begin() + I.atom0[s] * 3);
+            std::copy(sp.v.begin(), sp.v.end(), hv.begin() + I.atom0[s] * 3);
+            std::copy(sp.box.begin(), sp.box.end(), hbox.begin() + s * 9);
+        }
         std::copy(sp.integ.cbs.begin(), sp.integ.cbs.end(), hcbs.begin() + I.atom0[s]);
         std::copy(sp.integ.ccs.begin(), sp.integ.ccs.end(), hccs.begin() + I.atom0[s]);
         hca[s] = sp.integ.ca;
@@ -551,10 +553,21 @@ Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
             hatom_sys[I.atom0[s] + i] = s;
         }
     }
-    upload_vec(I.x, hx);
-    upload_vec(I.v, hv);
+    // state of systems handed over from another engine stays on the device
+    // (copy_state_from); only the others are uploaded
+    I.x.alloc(std::max(A_ * 3, 1));
+    I.v.alloc(std::max(A_ * 3, 1));
+    I.box.alloc(S_ * 9);
+    for (int s = 0; s < S_; s++) {
+        if (systems[s].state_on_device) {
+            continue;
+        }
+        const size_t a3 = static_cast<size_t>(I.atom0[s]) * 3, n3 = static_cast<size_t>(I.natoms[s]) * 3;
+        TM_HIP_CHECK(hipMemcpy(I.x.ptr + a3, hx.data() + a3, n3 * sizeof(double), hipMemcpyHostToDevice));
+        TM_HIP_CHECK(hipMemcpy(I.v.ptr + a3, hv.data() + a3, n3 * sizeof(double), hipMemcpyHostToDevice));
+        TM_HIP_CHECK(hipMemcpy(I.box.ptr + s * 9, hbox.data() + s * 9, 9 * sizeof(double), hipMemcpyHostToDevice));
+    }
     I.x_after.alloc(std::max(A_ * 3, 1));
-    upload_vec(I.box, hbox);
     I.box_after.alloc(S_ * 9);
     upload_vec(I.cbs, hcbs);
     upload_vec(I.ccs, hccs);
@@ -979,6 +992,42 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
     }
     if (box_out && F > 0) {
         box_frames.download(box_out, static_cast<size_t>(S_) * F * 9, stream_);
+    }
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void Engine::copy_state_from(Engine &src, int src_s, int dst_s) {
+    Impl &I = *impl_;
+    Impl &J = *src.impl_;
+    if (src_s < 0 || src_s >= src.S_ || dst_s < 0 || dst_s >= S_) {
+        throw Error("Engine::copy_state_from: system index out of range");
+    }
+    if (src.h_n_[src_s] != h_n_[dst_s]) {
+        throw Error("Engine::copy_state_from: atom counts differ");
+    }
+    if ((J.h_has_baro[src_s] != 0) != (I.h_has_baro[dst_s] != 0)) {
+        throw Error("Engine::copy_state_from: barostat presence differs");
+    }
+    TM_HIP_CHECK(hipStreamSynchronize(src.stream_));
+    const size_t n3 = static_cast<size_t>(h_n_[dst_s]) * 3;
+    auto d2d = [&](void *dst, const void *from, size_t bytes) {
+        TM_HIP_CHECK(hipMemcpyAsync(dst, from, bytes, hipMemcpyDeviceToDevice, stream_));
+    };
+    d2d(I.x.ptr + static_cast<size_t>(h_atom0_[dst_s]) * 3, J.x.ptr + static_cast<size_t>(src.h_atom0_[src_s]) * 3,
+        n3 * sizeof(double));
+    d2d(I.v.ptr + static_cast<size_t>(h_atom0_[dst_s]) * 3, J.v.ptr + static_cast<size_t>(src.h_atom0_[src_s]) * 3,
+        n3 * sizeof(double));
+    d2d(I.box.ptr + dst_s * 9, J.box.ptr + src_s * 9, 9 * sizeof(double));
+    d2d(I.step.ptr + dst_s, J.step.ptr + src_s, sizeof(int64_t));
+    if (I.h_has_baro[dst_s]) {
+        d2d(I.b_vscale.ptr + dst_s, J.b_vscale.ptr + src_s, sizeof(double));
+        d2d(I.b_accepted.ptr + dst_s, J.b_accepted.ptr + src_s, sizeof(int));
+        d2d(I.b_attempted.ptr + dst_s, J.b_attempted.ptr + src_s, sizeof(int));
+        d2d(I.b_bstep.ptr + dst_s, J.b_bstep.ptr + src_s, sizeof(int));
+        I.h_bstep[dst_s] = J.h_bstep[src_s];
+    }
+    if (I.has_nb) {
+        I.nb.invalidate(stream_); // the new arena layout needs new neighborlists
     }
     TM_HIP_CHECK(hipStreamSynchronize(stream_));
 }

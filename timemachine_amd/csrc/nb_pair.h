@@ -37,7 +37,15 @@ constexpr double ERFC_C7 = -0.26566749811172485;
 constexpr double TWO_OVER_SQRT_PI = 1.1283791670955126;
 
 TM_HD inline float min_image_f(float d, float box, float inv_box) { return d - box * std::rint(d * inv_box); }
-TM_HD inline double min_image_d(double d, double box, double inv_box) { return d - box * std::rint(d * inv_box); }
+// d - box * rint(d / box) with one rounding for the product and difference.
+TM_HD inline double min_image_d(double d, double box, double inv_box) {
+    return __builtin_fma(-box, std::rint(d * inv_box), d);
+}
+
+// Squared 3D distance, dz^2 + dy^2 + dx^2 with fused multiply-adds.
+TM_HD inline double pair_d2(double dx, double dy, double dz) {
+    return __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
+}
 
 // 1/sqrt(a): FP32 hardware seed, one Newton step in FP64 (relative error ~1e-14).
 TM_HD inline double rsqrt_nr(double a) {

@@ -32,17 +32,21 @@ __device__ __forceinline__ void atomic_add_u64(uint64_t *p, uint64_t v) {
     atomicAdd(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(v));
 }
 
-constexpr int PAIR_RING = 128; // queued pairs per wave (a round adds at most 64)
+constexpr int PASS1_UNROLL = 4; // rounds per pass-1 group
+// Queued pairs per wave: fewer than 64 remain after a drain and a group adds
+// at most 64 per round, so 64 * (PASS1_UNROLL + 1) entries always suffice.
+constexpr int PAIR_RING = 512;
+static_assert(PAIR_RING >= TILE * (PASS1_UNROLL + 1), "pair ring too small");
+static_assert((PAIR_RING & (PAIR_RING - 1)) == 0 && TILE % PASS1_UNROLL == 0, "ring / unroll shape");
 
 // Per-wave LDS of the tile kernel. NACC accumulators per atom: x, y, z
 // forces, then q, sig, eps parameter derivatives when requested.
 template <int NACC> struct TileLds {
     double4 colpos[TILE];        // column atom x, y, z, w
+    double4 rowpos[TILE];        // row atom x, y, z, w
     float4 colpar[TILE];         // column atom q, sig, eps, dw/dlambda
     float4 rowpar[TILE];         // row atom q, sig, eps, dw/dlambda
     uint32_t colidx[TILE];       // column sorted index (NB_INVALID for padding)
-    double4 pd[PAIR_RING];       // queued pair: dx, dy, dz, d^2 (4D)
-    double pw[PAIR_RING];        // queued pair: dw
     uint32_t pij[PAIR_RING];     // queued pair: row lane | column slot << 8
     uint64_t acc[2][NACC][TILE]; // [row, column][component][lane / slot]
 };
@@ -51,30 +55,54 @@ __device__ __forceinline__ void lds_add_u64(uint64_t *p, uint64_t v) {
     atomicAdd(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(v));
 }
 
+// Minimum-image displacement and squared 4D distance of a pair. Both passes
+// and the pair-list kernel evaluate exactly these expressions, so the pass-1
+// cutoff test and the pass-2 terms see identical bits.
+template <bool ALCH>
+__device__ __forceinline__ double pair_delta(const double4 &pi, const double4 &pj, const double bx, const double by,
+                                             const double bz, const double ibx, const double iby, const double ibz,
+                                             double &dx, double &dy, double &dz, double &dw) {
+    dx = min_image_d(pi.x - pj.x, bx, ibx);
+    dy = min_image_d(pi.y - pj.y, by, iby);
+    dz = min_image_d(pi.z - pj.z, bz, ibz);
+    double d2 = pair_d2(dx, dy, dz);
+    dw = 0.0;
+    if (ALCH) {
+        dw = pi.w - pj.w;
+        d2 = __builtin_fma(dw, dw, d2);
+    }
+    return d2;
+}
+
+struct BoxDev {
+    double bx, by, bz, ibx, iby, ibz;
+};
+
 // Pass 2: evaluates n <= 64 queued pairs starting at ring entry `base`, one
 // pair per lane, and adds the fixed-point terms to the LDS accumulators of
 // the pair's row and column atom (u and du/dl to per-lane registers).
 // DLDP: instead of du/dp, accumulate d(du/dlambda)/dp into the du_dp slots
 // (same fixed-point exponents); only alchemical tiles contribute.
 template <bool ALCH, bool U, bool DX, bool DL, bool DP, bool DLDP, int NACC>
-__device__ __forceinline__ void eval_pairs(TileLds<NACC> &L, const int base, const int n, const double beta,
-                                           uint64_t &uacc, uint64_t &dlacc) {
+__device__ __forceinline__ void eval_pairs(TileLds<NACC> &L, const int base, const int n, const BoxDev &B,
+                                           const double beta, uint64_t &uacc, uint64_t &dlacc) {
     constexpr int PQ = DX ? 3 : 0; // first parameter-derivative component
     const int lane = threadIdx.x & 63;
     if (lane < n) {
-        const int e = (base + lane) & (PAIR_RING - 1);
-        const double4 q = L.pd[e];
-        const uint32_t ij = L.pij[e];
+        const uint32_t ij = L.pij[(base + lane) & (PAIR_RING - 1)];
         const int il = static_cast<int>(ij & 0xFFu);
         const int sl = static_cast<int>(ij >> 8);
+        const double4 pi = L.rowpos[il];
+        const double4 pj = L.colpos[sl];
         const float4 ai = L.rowpar[il];
         const float4 aj = L.colpar[sl];
-        const double d2 = q.w;
+        double dx, dy, dz, dw;
+        const double d2 = pair_delta<ALCH>(pi, pj, B.bx, B.by, B.bz, B.ibx, B.iby, B.ibz, dx, dy, dz, dw);
         const PairTerms T = nb_pair_terms<U, DP>(d2, ai.x, aj.x, ai.y, aj.y, ai.z, aj.z, beta, 1.0, 1.0);
         if (DX) {
-            const uint64_t fx = f64_to_fixed(T.prefactor * q.x, FIXED_SCALE);
-            const uint64_t fy = f64_to_fixed(T.prefactor * q.y, FIXED_SCALE);
-            const uint64_t fz = f64_to_fixed(T.prefactor * q.z, FIXED_SCALE);
+            const uint64_t fx = f64_to_fixed(T.prefactor * dx, FIXED_SCALE);
+            const uint64_t fy = f64_to_fixed(T.prefactor * dy, FIXED_SCALE);
+            const uint64_t fz = f64_to_fixed(T.prefactor * dz, FIXED_SCALE);
             lds_add_u64(&L.acc[0][0][il], fx);
             lds_add_u64(&L.acc[0][1][il], fy);
             lds_add_u64(&L.acc[0][2][il], fz);
@@ -94,7 +122,6 @@ __device__ __forceinline__ void eval_pairs(TileLds<NACC> &L, const int base, con
             }
         }
         if (ALCH && (DL || DLDP)) {
-            const double dw = L.pw[e];
             if (DL) {
                 dlacc += f64_to_fixed(dw * T.prefactor * (static_cast<double>(ai.w) - static_cast<double>(aj.w)),
                                       FIXED_SCALE);
@@ -125,11 +152,12 @@ __device__ __forceinline__ void eval_pairs(TileLds<NACC> &L, const int base, con
 
 // One 64 x 64 tile on one wave.
 // Pass 1: 64 rounds; in round r lane l pairs its row atom with column slot
-// (l + r) mod 64 (read from LDS), computes the exact FP64 4D distance and
-// queues the pairs inside the cutoff (ballot + mbcnt compaction into an LDS
-// ring). Whenever 64 pairs are queued, pass 2 evaluates them with every
-// lane busy. Fixed-point sums are exact, so the order of evaluation does
-// not change any output bit.
+// (l + r) mod 64 (read from LDS one round ahead), computes the exact FP64 4D
+// distance and queues the indices of the pairs inside the cutoff (ballot +
+// mbcnt compaction into an LDS ring). Whenever 64 pairs are queued, pass 2
+// recomputes their displacements and evaluates them with every lane busy.
+// Fixed-point sums are exact, so the order of evaluation does not change
+// any output bit.
 template <bool ALCH, bool U, bool DX, bool DL, bool DP, bool DLDP, int NACC>
 __device__ __forceinline__ void tile_body(
     TileLds<NACC> &L,
@@ -153,17 +181,19 @@ __device__ __forceinline__ void tile_body(
     const uint32_t k = static_cast<uint32_t>(S.k);
     const bool vi = i < k;
 
-    const double bx = boxes[s * 9 + 0];
-    const double by = boxes[s * 9 + 4];
-    const double bz = boxes[s * 9 + 8];
-    const double ibx = 1.0 / bx;
-    const double iby = 1.0 / by;
-    const double ibz = 1.0 / bz;
+    BoxDev B;
+    B.bx = boxes[s * 9 + 0];
+    B.by = boxes[s * 9 + 4];
+    B.bz = boxes[s * 9 + 8];
+    B.ibx = 1.0 / B.bx;
+    B.iby = 1.0 / B.by;
+    B.ibz = 1.0 / B.bz;
     const double cut2 = S.cutoff_d * S.cutoff_d;
     const double beta = static_cast<double>(S.beta);
 
     wave_lds_sync(); // the previous tile's LDS reads are complete
     L.colpos[lane] = pj0;
+    L.rowpos[lane] = pi;
     L.colpar[lane] = aj0;
     L.rowpar[lane] = ai;
     L.colidx[lane] = jl0;
@@ -176,42 +206,44 @@ __device__ __forceinline__ void tile_body(
 
     uint64_t uacc = 0, dlacc = 0;
     int queued = 0, done = 0; // wave-uniform
-    for (int round = 0; round < TILE; round++) {
-        const int slot = (lane + round) & (TILE - 1);
-        const double4 pj = L.colpos[slot];
-        const uint32_t jc = L.colidx[slot];
-        const double dx = min_image_d(pi.x - pj.x, bx, ibx);
-        const double dy = min_image_d(pi.y - pj.y, by, iby);
-        const double dz = min_image_d(pi.z - pj.z, bz, ibz);
-        double d2 = dx * dx + dy * dy + dz * dz;
-        double dw = 0.0;
-        if (ALCH) {
-            dw = pi.w - pj.w;
-            d2 = d2 + dw * dw;
+    // round r pairs lane l with column slot (l + r) mod 64. PASS1_UNROLL
+    // rounds are evaluated as independent chains; full batches of 64 queued
+    // pairs are drained once per group of rounds.
+    for (int round = 0; round < TILE; round += PASS1_UNROLL) {
+        double4 pj[PASS1_UNROLL];
+        uint32_t jc[PASS1_UNROLL];
+#pragma unroll
+        for (int q = 0; q < PASS1_UNROLL; q++) {
+            const int slot = (lane + round + q) & (TILE - 1);
+            pj[q] = L.colpos[slot];
+            jc[q] = L.colidx[slot];
         }
-        const bool inside = vi && (jc < k) && (i < jc) && (d2 < cut2);
-        const uint64_t m = ballot(inside);
-        if (m != 0) {
-            if (inside) {
-                const int e = (queued + lanes_below(m)) & (PAIR_RING - 1);
-                L.pd[e] = make_double4(dx, dy, dz, d2);
-                if (ALCH) {
-                    L.pw[e] = dw;
-                }
-                L.pij[e] = static_cast<uint32_t>(lane) | (static_cast<uint32_t>(slot) << 8);
+        uint64_t m[PASS1_UNROLL];
+#pragma unroll
+        for (int q = 0; q < PASS1_UNROLL; q++) {
+            double dx, dy, dz, dw;
+            const double d2 = pair_delta<ALCH>(pi, pj[q], B.bx, B.by, B.bz, B.ibx, B.iby, B.ibz, dx, dy, dz, dw);
+            m[q] = ballot(vi && (jc[q] < k) && (i < jc[q]) && (d2 < cut2));
+        }
+#pragma unroll
+        for (int q = 0; q < PASS1_UNROLL; q++) {
+            if ((m[q] >> lane) & 1ull) {
+                const int slot = (lane + round + q) & (TILE - 1);
+                L.pij[(queued + lanes_below(m[q])) & (PAIR_RING - 1)] =
+                    static_cast<uint32_t>(lane) | (static_cast<uint32_t>(slot) << 8);
             }
-            queued += __popcll(m);
-            if (queued - done >= TILE) {
-                wave_lds_sync();
-                eval_pairs<ALCH, U, DX, DL, DP, DLDP, NACC>(L, done, TILE, beta, uacc, dlacc);
-                done += TILE;
-                wave_lds_sync(); // later rounds reuse these ring entries
-            }
+            queued += __popcll(m[q]);
+        }
+        while (queued - done >= TILE) {
+            wave_lds_sync();
+            eval_pairs<ALCH, U, DX, DL, DP, DLDP, NACC>(L, done, TILE, B, beta, uacc, dlacc);
+            done += TILE;
+            wave_lds_sync(); // later rounds reuse these ring entries
         }
     }
     if (queued > done) {
         wave_lds_sync();
-        eval_pairs<ALCH, U, DX, DL, DP, DLDP, NACC>(L, done, queued - done, beta, uacc, dlacc);
+        eval_pairs<ALCH, U, DX, DL, DP, DLDP, NACC>(L, done, queued - done, B, beta, uacc, dlacc);
     }
     wave_lds_sync();
 
@@ -355,95 +387,104 @@ __global__ __launch_bounds__(256) void k_pair_list(
     const PairListArgs a, uint64_t *__restrict__ du_dx, uint64_t *__restrict__ du_dp, uint64_t *__restrict__ du_dl,
     uint64_t *__restrict__ u) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.M) {
-        return;
-    }
-    const int s = a.pair_sys ? a.pair_sys[p] : 0;
-    const int gi = a.pairs[p * 2 + 0];
-    const int gj = a.pairs[p * 2 + 1];
-    const double *box = a.boxes + s * 9;
-    const double lam = a.lambdas ? a.lambdas[s] : a.lambda0;
-    const double cutoff_d = a.cutoffs ? a.cutoffs[s] : a.cutoff0;
-    const float beta = a.betas ? a.betas[s] : a.beta0;
-    const int red = a.red_idx ? a.red_idx[s] : 0;
-
-    const double bx = box[0];
-    const double by = box[4];
-    const double bz = box[8];
-    const double ibx = 1.0 / bx;
-    const double iby = 1.0 / by;
-    const double ibz = 1.0 / bz;
-
-    const int pli = a.plane ? a.plane[gi] : 0, plj = a.plane ? a.plane[gj] : 0;
-    const int ofi = a.offset ? a.offset[gi] : 0, ofj = a.offset ? a.offset[gj] : 0;
-    const double wi = (pli + ofi * lam) * cutoff_d;
-    const double wj = (plj + ofj * lam) * cutoff_d;
-    const double dwdl_i = static_cast<double>(static_cast<float>(ofi * cutoff_d));
-    const double dwdl_j = static_cast<double>(static_cast<float>(ofj * cutoff_d));
-
-    const double dx = min_image_d(a.x[gi * 3 + 0] - a.x[gj * 3 + 0], bx, ibx);
-    const double dy = min_image_d(a.x[gi * 3 + 1] - a.x[gj * 3 + 1], by, iby);
-    const double dz = min_image_d(a.x[gi * 3 + 2] - a.x[gj * 3 + 2], bz, ibz);
-    const double dw = wi - wj;
-    double d2 = dx * dx + dy * dy + dz * dz;
-    d2 = d2 + dw * dw;
-    if (!(d2 < cutoff_d * cutoff_d)) {
-        return;
-    }
-    const double qi = static_cast<float>(a.params[gi * 3 + 0]), qj = static_cast<float>(a.params[gj * 3 + 0]);
-    const double si = static_cast<float>(a.params[gi * 3 + 1]), sj = static_cast<float>(a.params[gj * 3 + 1]);
-    const double ei = static_cast<float>(a.params[gi * 3 + 2]), ej = static_cast<float>(a.params[gj * 3 + 2]);
-    const double scale_q = a.scales[p * 2 + 0];
-    const double scale_lj = a.scales[p * 2 + 1];
-    const PairTerms T = nb_pair_terms<U, DP>(d2, qi, qj, si, sj, ei, ej, static_cast<double>(beta), scale_q, scale_lj);
     const bool neg = a.negated != 0;
-    auto acc = [neg](uint64_t *dst, uint64_t v) { atomic_add_u64(dst, neg ? (0ull - v) : v); };
-    if (DX) {
-        const uint64_t fx = f64_to_fixed(T.prefactor * dx, FIXED_SCALE);
-        const uint64_t fy = f64_to_fixed(T.prefactor * dy, FIXED_SCALE);
-        const uint64_t fz = f64_to_fixed(T.prefactor * dz, FIXED_SCALE);
-        acc(du_dx + gi * 3 + 0, fx);
-        acc(du_dx + gi * 3 + 1, fy);
-        acc(du_dx + gi * 3 + 2, fz);
-        acc(du_dx + gj * 3 + 0, 0ull - fx);
-        acc(du_dx + gj * 3 + 1, 0ull - fy);
-        acc(du_dx + gj * 3 + 2, 0ull - fz);
-    }
-    if (DP) {
-        acc(du_dp + gi * 3 + 0, f64_to_fixed((scale_q * qj) * T.erfc_inv_d, FIXED_SCALE));
-        acc(du_dp + gj * 3 + 0, f64_to_fixed((scale_q * qi) * T.erfc_inv_d, FIXED_SCALE));
-        if (T.lj) {
-            const uint64_t gs = f64_to_fixed(T.sig_grad, FIXED_SCALE_DSIG);
-            acc(du_dp + gi * 3 + 1, gs);
-            acc(du_dp + gj * 3 + 1, gs);
-            acc(du_dp + gi * 3 + 2, f64_to_fixed(T.eps_grad * ej, FIXED_SCALE_DEPS));
-            acc(du_dp + gj * 3 + 2, f64_to_fixed(T.eps_grad * ei, FIXED_SCALE_DEPS));
-        }
-    }
-    if (DL) {
-        const bool vanilla = (wi == 0.0) && (wj == 0.0);
-        if (!vanilla) {
-            acc(du_dl + red, f64_to_fixed(dw * T.prefactor * (dwdl_i - dwdl_j), FIXED_SCALE));
-        }
-    }
-    if (DLDP) {
-        const bool vanilla = (wi == 0.0) && (wj == 0.0);
-        if (!vanilla) {
-            const PairDlDp D = nb_pair_dl_dp(d2, si, sj, ei, ej, static_cast<double>(beta), scale_lj);
-            const double dd = dw * (dwdl_i - dwdl_j);
-            acc(du_dp + gi * 3 + 0, f64_to_fixed(((scale_q * qj) * D.es_unit) * dd, FIXED_SCALE));
-            acc(du_dp + gj * 3 + 0, f64_to_fixed(((scale_q * qi) * D.es_unit) * dd, FIXED_SCALE));
-            if (D.lj) {
-                const uint64_t gs = f64_to_fixed(D.lj_dsig * dd, FIXED_SCALE_DSIG);
-                acc(du_dp + gi * 3 + 1, gs);
-                acc(du_dp + gj * 3 + 1, gs);
-                acc(du_dp + gi * 3 + 2, f64_to_fixed((D.lj_unit * ej) * dd, FIXED_SCALE_DEPS));
-                acc(du_dp + gj * 3 + 2, f64_to_fixed((D.lj_unit * ei) * dd, FIXED_SCALE_DEPS));
+    int red = -1;
+    uint64_t fu = 0, fdl = 0; // this lane's u and du/dl terms (reduced over the wave below)
+    if (p < a.M) {
+        const int s = a.pair_sys ? a.pair_sys[p] : 0;
+        const int gi = a.pairs[p * 2 + 0];
+        const int gj = a.pairs[p * 2 + 1];
+        const double *box = a.boxes + s * 9;
+        const double lam = a.lambdas ? a.lambdas[s] : a.lambda0;
+        const double cutoff_d = a.cutoffs ? a.cutoffs[s] : a.cutoff0;
+        const float beta = a.betas ? a.betas[s] : a.beta0;
+        red = a.red_idx ? a.red_idx[s] : 0;
+
+        const double bx = box[0];
+        const double by = box[4];
+        const double bz = box[8];
+        const double ibx = 1.0 / bx;
+        const double iby = 1.0 / by;
+        const double ibz = 1.0 / bz;
+
+        const int pli = a.plane ? a.plane[gi] : 0, plj = a.plane ? a.plane[gj] : 0;
+        const int ofi = a.offset ? a.offset[gi] : 0, ofj = a.offset ? a.offset[gj] : 0;
+        const double wi = (pli + ofi * lam) * cutoff_d;
+        const double wj = (plj + ofj * lam) * cutoff_d;
+        const double dwdl_i = static_cast<double>(static_cast<float>(ofi * cutoff_d));
+        const double dwdl_j = static_cast<double>(static_cast<float>(ofj * cutoff_d));
+
+        const double dx = min_image_d(a.x[gi * 3 + 0] - a.x[gj * 3 + 0], bx, ibx);
+        const double dy = min_image_d(a.x[gi * 3 + 1] - a.x[gj * 3 + 1], by, iby);
+        const double dz = min_image_d(a.x[gi * 3 + 2] - a.x[gj * 3 + 2], bz, ibz);
+        const double dw = wi - wj;
+        const double d2 = __builtin_fma(dw, dw, pair_d2(dx, dy, dz));
+        if (d2 < cutoff_d * cutoff_d) {
+            const double qi = static_cast<float>(a.params[gi * 3 + 0]), qj = static_cast<float>(a.params[gj * 3 + 0]);
+            const double si = static_cast<float>(a.params[gi * 3 + 1]), sj = static_cast<float>(a.params[gj * 3 + 1]);
+            const double ei = static_cast<float>(a.params[gi * 3 + 2]), ej = static_cast<float>(a.params[gj * 3 + 2]);
+            const double scale_q = a.scales[p * 2 + 0];
+            const double scale_lj = a.scales[p * 2 + 1];
+            const PairTerms T =
+                nb_pair_terms<U, DP>(d2, qi, qj, si, sj, ei, ej, static_cast<double>(beta), scale_q, scale_lj);
+            auto acc = [neg](uint64_t *dst, uint64_t v) {
+                if (v != 0) {
+                    atomic_add_u64(dst, neg ? (0ull - v) : v);
+                }
+            };
+            if (DX) {
+                const uint64_t fx = f64_to_fixed(T.prefactor * dx, FIXED_SCALE);
+                const uint64_t fy = f64_to_fixed(T.prefactor * dy, FIXED_SCALE);
+                const uint64_t fz = f64_to_fixed(T.prefactor * dz, FIXED_SCALE);
+                acc(du_dx + gi * 3 + 0, fx);
+                acc(du_dx + gi * 3 + 1, fy);
+                acc(du_dx + gi * 3 + 2, fz);
+                acc(du_dx + gj * 3 + 0, 0ull - fx);
+                acc(du_dx + gj * 3 + 1, 0ull - fy);
+                acc(du_dx + gj * 3 + 2, 0ull - fz);
+            }
+            if (DP) {
+                acc(du_dp + gi * 3 + 0, f64_to_fixed((scale_q * qj) * T.erfc_inv_d, FIXED_SCALE));
+                acc(du_dp + gj * 3 + 0, f64_to_fixed((scale_q * qi) * T.erfc_inv_d, FIXED_SCALE));
+                if (T.lj) {
+                    const uint64_t gs = f64_to_fixed(T.sig_grad, FIXED_SCALE_DSIG);
+                    acc(du_dp + gi * 3 + 1, gs);
+                    acc(du_dp + gj * 3 + 1, gs);
+                    acc(du_dp + gi * 3 + 2, f64_to_fixed(T.eps_grad * ej, FIXED_SCALE_DEPS));
+                    acc(du_dp + gj * 3 + 2, f64_to_fixed(T.eps_grad * ei, FIXED_SCALE_DEPS));
+                }
+            }
+            const bool vanilla = (wi == 0.0) && (wj == 0.0);
+            if (DL && !vanilla) {
+                fdl = f64_to_fixed(dw * T.prefactor * (dwdl_i - dwdl_j), FIXED_SCALE);
+            }
+            if (DLDP && !vanilla) {
+                const PairDlDp D = nb_pair_dl_dp(d2, si, sj, ei, ej, static_cast<double>(beta), scale_lj);
+                const double dd = dw * (dwdl_i - dwdl_j);
+                acc(du_dp + gi * 3 + 0, f64_to_fixed(((scale_q * qj) * D.es_unit) * dd, FIXED_SCALE));
+                acc(du_dp + gj * 3 + 0, f64_to_fixed(((scale_q * qi) * D.es_unit) * dd, FIXED_SCALE));
+                if (D.lj) {
+                    const uint64_t gs = f64_to_fixed(D.lj_dsig * dd, FIXED_SCALE_DSIG);
+                    acc(du_dp + gi * 3 + 1, gs);
+                    acc(du_dp + gj * 3 + 1, gs);
+                    acc(du_dp + gi * 3 + 2, f64_to_fixed((D.lj_unit * ej) * dd, FIXED_SCALE_DEPS));
+                    acc(du_dp + gj * 3 + 2, f64_to_fixed((D.lj_unit * ei) * dd, FIXED_SCALE_DEPS));
+                }
+            }
+            if (U) {
+                fu = f64_to_fixed(T.u, FIXED_SCALE);
+            }
+            if (neg) {
+                fu = 0ull - fu;
+                fdl = 0ull - fdl;
             }
         }
     }
     if (U) {
-        acc(u + red, f64_to_fixed(T.u, FIXED_SCALE));
+        wave_add_by_key(u, red, fu);
+    }
+    if (DL) {
+        wave_add_by_key(du_dl, red, fdl);
     }
 }
 

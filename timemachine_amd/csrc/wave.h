@@ -59,6 +59,25 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
     return v;
 }
 
+// base[key] += v summed over the lanes of each distinct key, one global atomic
+// per key present in the wave (terms of one system are contiguous, so a wave
+// holds one or two keys). Lanes with key < 0 or v == 0 contribute nothing.
+// Must be reached by every lane of the wave.
+__device__ __forceinline__ void wave_add_by_key(uint64_t *base, int key, uint64_t v) {
+    const bool has = key >= 0 && v != 0;
+    uint64_t pending = ballot(has);
+    while (pending != 0) {
+        const int leader = __ffsll(static_cast<long long>(pending)) - 1;
+        const int k = __builtin_amdgcn_readlane(key, leader);
+        const bool mine = has && key == k;
+        const uint64_t tot = wave_sum_u64(mine ? v : 0);
+        if (lane_id() == leader) {
+            atomicAdd(reinterpret_cast<unsigned long long *>(base + k), static_cast<unsigned long long>(tot));
+        }
+        pending &= ~ballot(mine);
+    }
+}
+
 __device__ __forceinline__ float wave_min(float v) {
     for (int m = 32; m >= 1; m >>= 1) {
         v = fminf(v, __shfl_xor(v, m, 64));

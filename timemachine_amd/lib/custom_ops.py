@@ -724,6 +724,36 @@ class BatchedContexts(_Handle):
     def __len__(self):
         return len(self._ctxs)
 
+    @classmethod
+    def _wrap(cls, handle, contexts, lambdas):
+        obj = cls.__new__(cls)
+        _Handle.__init__(obj)
+        obj._h = handle
+        obj._ctxs = list(contexts)
+        obj._lam = f64(lambdas).reshape(-1)
+        obj.sizes = [c.N for c in obj._ctxs]
+        return obj
+
+    def repack(self, keep: Sequence[int], new_contexts: Sequence[Context], lambdas: Sequence[float]):
+        """A new batch with systems ``keep`` of this one (state handed over on the
+        device) followed by ``new_contexts``; lambdas for all of them in that order.
+        This batch is left unchanged (release it when done)."""
+        keep = i32(list(keep)).reshape(-1)
+        new_contexts = list(new_contexts)
+        lam = f64(lambdas).reshape(-1)
+        if lam.size != keep.size + len(new_contexts):
+            raise RuntimeError("one lambda per system of the new batch is required")
+        arr = _lib.handle_array([c.handle for c in new_contexts]) if new_contexts else None
+        h = ctypes.c_void_p()
+        call("tm_batch_repack", self._h, int(keep.size), pi(keep) if keep.size else None, len(new_contexts), arr,
+             pd(lam), ctypes.byref(h))
+        return BatchedContexts._wrap(h, [self._ctxs[int(k)] for k in keep] + new_contexts, lam)
+
+    def device_bytes(self) -> int:
+        n = ctypes.c_uint64(0)
+        call("tm_batch_device_bytes", self._h, ctypes.byref(n))
+        return int(n.value)
+
     def set_tile_grid(self, grid: int):
         call("tm_batch_set_tile_grid", self._h, int(grid))
 

@@ -1,0 +1,268 @@
+"""Continuous-batching request loop for free-energy jobs on one GPU.
+
+A job is one system (one topology and parameter set) to be sampled at W
+lambda windows for n_steps each, with du/dlambda stored every
+``du_dl_interval`` steps and a TI estimate at the end. Jobs arrive over time
+and have uneven atom counts, window counts and lengths.
+
+All windows of all admitted jobs are systems of ONE batched engine
+(BatchedContexts), so every step of every resident window goes through the
+same kernel launches. The loop advances the engine in chunks; at each chunk
+boundary it
+
+* retires finished jobs (their windows leave the engine),
+* admits waiting jobs that have arrived, first come first served, while their
+  estimated device memory fits the HBM budget,
+* repacks the engine (BatchedContexts.repack): windows that stay keep their
+  coordinates, velocities, box, step counters and barostat state on the
+  device (device-to-device handover; only the neighborlists are rebuilt),
+  new windows are uploaded once.
+
+Chunks end exactly where the next running window finishes (capped by
+``max_chunk``), so no window runs past its length. Fixed-point forces and
+counter-based noise make every window's trajectory independent of what else
+is resident: a job's du/dlambda series is bitwise the same served alone, in
+FIFO order, or in any mix.
+
+The reference runs one process per lambda window per GPU through its worker
+pool (timemachine/parallel/client.py:122-153); there is no continuous batching
+there, so this module has no counterpart beyond the Context API it drives.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import math
+import time
+from typing import Callable, Dict, List, Optional, Sequence
+
+import numpy as np
+
+from timemachine_amd import testsystems as ts
+from timemachine_amd.lib import LangevinIntegrator, MonteCarloBarostat, custom_ops
+
+TILE = 64
+
+
+@dataclasses.dataclass
+class Job:
+    job_id: int
+    system: ts.System
+    lambdas: np.ndarray
+    n_steps: int
+    seed: int
+    arrival: float = 0.0  # seconds after the start of the stream
+    temperature: float = 300.0
+    dt: float = 0.0025
+    friction: float = 1.0
+    barostat_interval: int = 15
+
+    @property
+    def windows(self) -> int:
+        return len(self.lambdas)
+
+    @property
+    def atom_steps(self) -> int:
+        return self.system.N * self.windows * self.n_steps
+
+
+@dataclasses.dataclass
+class JobResult:
+    job_id: int
+    du_dl: np.ndarray  # [W, n_steps // du_dl_interval]
+    dG: float  # TI estimate, trapezoid over the windows of the per-window mean du/dlambda
+    arrival: float
+    admitted: float
+    finished: float
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+def estimate_system_bytes(n_atoms: int) -> int:
+    """Upper bound of the device memory one system takes in the engine: per-atom
+    state and sorted copies (~400 B/atom) plus the neighborlist tile buffer
+    (capacity nb (nb + 1) / 2 + nb tiles of 64 column indices and a row)."""
+    nb = -(-n_atoms // TILE)
+    tiles = nb * (nb + 1) // 2 + nb
+    return 400 * (nb * TILE) + tiles * (TILE * 4 + 4) + 4096
+
+
+def ti_estimate(lambdas: Sequence[float], du_dl: np.ndarray) -> float:
+    means = np.asarray(du_dl, dtype=np.float64).mean(axis=1)
+    lam = np.asarray(lambdas, dtype=np.float64)
+    return float(np.sum(0.5 * (means[1:] + means[:-1]) * np.diff(lam)))
+
+
+class _Window:
+    __slots__ = ("job", "w", "remaining", "chunks", "ctx", "keep")
+
+    def __init__(self, job: Job, w: int, ctx, keep):
+        self.job = job
+        self.w = w
+        self.remaining = job.n_steps
+        self.chunks: List[np.ndarray] = []
+        self.ctx = ctx
+        self.keep = keep
+
+
+def make_window_contexts(job: Job):
+    """One Context per lambda window of the job (seeded per job and window)."""
+    sys_ = job.system
+    out = []
+    for w in range(job.windows):
+        seed = job.seed * 1009 + w
+        bps = [p.bound_impl(np.float32) for p in sys_.bound_potentials()]
+        intg = LangevinIntegrator(job.temperature, job.dt, job.friction, sys_.masses, seed).impl()
+        baro = None
+        if job.barostat_interval > 0:
+            baro = MonteCarloBarostat(sys_.N, 1.0, job.temperature, sys_.groups, job.barostat_interval,
+                                      seed + 17).impl(bps)
+        v0 = ts.maxwell_boltzmann(sys_.masses, job.temperature, seed)
+        ctx = custom_ops.Context(sys_.x, v0, sys_.box, intg, bps, barostat=baro)
+        out.append((ctx, (bps, intg, baro)))
+    return out
+
+
+class Server:
+    """Continuous batching of jobs into one engine under an HBM budget."""
+
+    def __init__(self, hbm_budget_bytes: int, max_chunk: int = 600, du_dl_interval: int = 10,
+                 use_graph: bool = True, clock: Callable[[], float] = time.perf_counter,
+                 sleep: Callable[[float], None] = time.sleep):
+        if du_dl_interval <= 0 or max_chunk < du_dl_interval:
+            raise ValueError("need 0 < du_dl_interval <= max_chunk")
+        self.budget = int(hbm_budget_bytes)
+        self.max_chunk = int(max_chunk) - int(max_chunk) % int(du_dl_interval)
+        self.du_dl_interval = int(du_dl_interval)
+        self.use_graph = use_graph
+        self._clock = clock
+        self._sleep = sleep
+        self.log: List[dict] = []  # one record per chunk (resident systems, steps, duration)
+
+    def _job_bytes(self, job: Job) -> int:
+        return job.windows * estimate_system_bytes(job.system.N)
+
+    def run(self, jobs: Sequence[Job]) -> List[JobResult]:
+        for j in jobs:
+            if j.n_steps % self.du_dl_interval != 0:
+                raise ValueError(f"job {j.job_id}: n_steps must be a multiple of du_dl_interval")
+            if self._job_bytes(j) > self.budget:
+                raise ValueError(f"job {j.job_id} alone exceeds the HBM budget")
+        pending = sorted(jobs, key=lambda j: (j.arrival, j.job_id))
+        t0 = self._clock()
+        now = lambda: self._clock() - t0  # noqa: E731
+        running: List[_Window] = []
+        batch: Optional[custom_ops.BatchedContexts] = None
+        used = 0
+        admitted_at: Dict[int, float] = {}
+        results: List[JobResult] = []
+        while pending or running:
+            # admit, first come first served, while the budget allows
+            new: List[_Window] = []
+            while pending and pending[0].arrival <= now() and used + self._job_bytes(pending[0]) <= self.budget:
+                job = pending.pop(0)
+                used += self._job_bytes(job)
+                admitted_at[job.job_id] = now()
+                for w, (ctx, keep) in enumerate(make_window_contexts(job)):
+                    new.append(_Window(job, w, ctx, keep))
+            if not running and not new:
+                # idle until the next arrival
+                self._sleep(max(0.0, pending[0].arrival - now()))
+                continue
+            if new:
+                lam = [r.job.lambdas[r.w] for r in running] + [n.job.lambdas[n.w] for n in new]
+                if batch is None:
+                    batch = custom_ops.BatchedContexts([n.ctx for n in new], lam)
+                else:
+                    batch = batch.repack(range(len(running)), [n.ctx for n in new], lam)
+                running += new
+            chunk = min(self.max_chunk, min(r.remaining for r in running))
+            tc = now()
+            du_dl, _, _ = batch.multiple_steps(chunk, store_du_dl_interval=self.du_dl_interval,
+                                               use_graph=self.use_graph)
+            self.log.append({"t": tc, "systems": len(running), "atoms": int(sum(batch.sizes)), "steps": chunk,
+                             "seconds": now() - tc, "budget_used": used, "device_bytes": batch.device_bytes()})
+            for s, r in enumerate(running):
+                r.chunks.append(du_dl[s])
+                r.remaining -= chunk
+            done_jobs = {r.job.job_id for r in running if r.remaining == 0}
+            if done_jobs:
+                t_fin = now()
+                by_job: Dict[int, List[_Window]] = {}
+                for r in running:
+                    if r.job.job_id in done_jobs:
+                        by_job.setdefault(r.job.job_id, []).append(r)
+                for jid, ws in by_job.items():
+                    ws.sort(key=lambda r: r.w)
+                    job = ws[0].job
+                    series = np.stack([np.concatenate(r.chunks) for r in ws])
+                    results.append(JobResult(jid, series, ti_estimate(job.lambdas, series), job.arrival,
+                                             admitted_at[jid], t_fin))
+                    used -= self._job_bytes(job)
+                keep = [i for i, r in enumerate(running) if r.job.job_id not in done_jobs]
+                if keep:
+                    lam = [running[i].job.lambdas[running[i].w] for i in keep]
+                    batch = batch.repack(keep, [], lam)
+                else:
+                    batch = None
+                running = [running[i] for i in keep]
+        return sorted(results, key=lambda r: r.job_id)
+
+
+def run_fifo(jobs: Sequence[Job], du_dl_interval: int = 10, use_graph: bool = True,
+             clock: Callable[[], float] = time.perf_counter,
+             sleep: Callable[[float], None] = time.sleep) -> List[JobResult]:
+    """One job at a time in arrival order: each job's windows run as one batch."""
+    t0 = clock()
+    now = lambda: clock() - t0  # noqa: E731
+    results = []
+    for job in sorted(jobs, key=lambda j: (j.arrival, j.job_id)):
+        if job.arrival > now():
+            sleep(job.arrival - now())
+        ta = now()
+        wins = make_window_contexts(job)
+        batch = custom_ops.BatchedContexts([c for c, _ in wins], job.lambdas)
+        du_dl, _, _ = batch.multiple_steps(job.n_steps, store_du_dl_interval=du_dl_interval, use_graph=use_graph)
+        del batch
+        results.append(JobResult(job.job_id, du_dl, ti_estimate(job.lambdas, du_dl), job.arrival, ta, now()))
+    return results
+
+
+def synthetic_stream(n_jobs: int, rate_per_s: float, seed: int = 0, atoms=(1800, 36000), windows=(8, 48),
+                     steps=(2000, 20000), heavy=(20, 60), steps_multiple: int = 10) -> List[Job]:
+    """Poisson arrivals of ligand-in-water jobs: atoms log-uniform (the box is at
+    least 2.62 nm, twice cutoff plus padding), windows and steps uniform."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    jobs = []
+    for i in range(n_jobs):
+        t += rng.exponential(1.0 / rate_per_s) if rate_per_s > 0 else 0.0
+        n_target = math.exp(rng.uniform(math.log(atoms[0]), math.log(atoms[1])))
+        n_heavy = int(rng.integers(heavy[0], heavy[1] + 1))
+        n_waters = max(int((n_target - 1.6 * n_heavy) / 3), 1)
+        side = max((n_waters / 33.4) ** (1.0 / 3.0), 2.62)
+        sys_ = None
+        for attempt in range(8):  # the random tree sometimes has no room for every hydrogen
+            n_lig = int(round(n_heavy * (1.6 - 0.1 * attempt)))
+            try:
+                sys_ = ts.ligand_in_water(side=side, n_waters=n_waters, n_heavy=n_heavy, n_ligand=n_lig,
+                                          seed=int(rng.integers(1 << 30)))
+                break
+            except ValueError:
+                continue
+        if sys_ is None:
+            raise RuntimeError("could not build a synthetic ligand")
+        W = int(rng.integers(windows[0], windows[1] + 1))
+        n = int(rng.integers(steps[0], steps[1] + 1))
+        n -= n % steps_multiple
+        jobs.append(Job(i, sys_, np.linspace(0.0, 1.0, W), max(n, steps_multiple), int(rng.integers(1 << 30)),
+                        arrival=t))
+    return jobs
+
+
+def latency_stats(results: Sequence[JobResult]) -> dict:
+    lat = np.array([r.latency for r in results])
+    return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean()), "jobs": len(lat)}
