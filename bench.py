@@ -7,7 +7,9 @@ rc 1.2 nm, beta 2, 4D decoupling offsets on the ligand), BAOAB at 2.5 fs with
 HMR, 300 K, friction 1/ps, MC barostat at 1 bar every 15 steps, du/dlambda
 stored every step, all in one engine whose per-step launches are captured into
 hipGraphs. value = total atom-steps/s of the batch. The same system stepped
-alone through a Context (one system at a time) is reported beside it.
+alone through a Context (one system at a time) is reported beside it, and a
+serving leg reports p50/p99 job latency of a continuously batched stream of
+uneven jobs against one job at a time (FIFO).
 
 Prints ONE JSON line. Only the cpu_baseline leg uses the oracle (FP64 port of
 the reference's potentials and BAOAB, timed on the host cores).
@@ -38,6 +40,10 @@ PEAK_FP64_TFLOPS = 78.6  # MI355X published FP64 peak (vector and matrix are equ
 # epsilons are nonzero (Lennard-Jones).
 FLOP_PAIR_BASE = 71
 FLOP_PAIR_LJ = 13
+# HBM bytes per launch of the du_dx tile kernel on this workload: FETCH_SIZE x 2.0
+# + WRITE_SIZE x 1.0 (factors from a 256 MiB device copy), measured with
+# rocprofv3 --pmc on MI355X; profiles/counters_lambda_windows_192x6372.txt.
+TILE_TRAFFIC_BYTES = 1.0612e9 + 3.3288e9
 
 
 def log(msg: str) -> None:
@@ -119,6 +125,57 @@ def cpu_baseline(sys_, budget_s: float = 12.0):
     }
 
 
+def serving_leg(n_jobs: int, seed: int = 5) -> dict:
+    """Continuous batching of an uneven job stream vs one job at a time (FIFO).
+
+    The stream is a reduced form of the serving workload (ligand-in-water jobs,
+    atoms log-uniform, windows and steps uniform). Capacity is measured with all
+    jobs queued at t = 0; the timed stream then arrives as a Poisson process at
+    80% of it, and the same arrivals are replayed through FIFO."""
+    from timemachine_amd import serving
+
+    cfg = dict(atoms=(1800, 9000), windows=(4, 16), steps=(300, 1500), heavy=(20, 40))
+    jobs = serving.synthetic_stream(n_jobs, rate_per_s=0.0, seed=seed, **cfg)
+    work = sum(j.atom_steps for j in jobs)
+    budget = 32 * 2**30
+    log(f"serving: {n_jobs} jobs, {work:.3g} atom-steps; saturated run for capacity")
+    srv = serving.Server(budget, max_chunk=600, du_dl_interval=10)
+    t0 = time.perf_counter()
+    srv.run(jobs)
+    capacity = work / (time.perf_counter() - t0)
+    rate = 0.8 * capacity / (work / n_jobs)
+    rng = np.random.default_rng(seed + 1)
+    t = 0.0
+    for j in jobs:
+        t += rng.exponential(1.0 / rate)
+        j.arrival = t
+    log(f"serving: Poisson arrivals at {rate:.3g} jobs/s (80% of capacity {capacity:.3g} atom-steps/s)")
+    srv = serving.Server(budget, max_chunk=600, du_dl_interval=10)
+    t0 = time.perf_counter()
+    served = srv.run(jobs)
+    span = time.perf_counter() - t0
+    log("serving: FIFO one job at a time on the same arrivals")
+    t0 = time.perf_counter()
+    fifo = serving.run_fifo(jobs, du_dl_interval=10)
+    span_fifo = time.perf_counter() - t0
+    same = all(np.array_equal(a.du_dl, b.du_dl) for a, b in zip(served, fifo))
+    st, sf = serving.latency_stats(served), serving.latency_stats(fifo)
+    return {
+        "stream": f"{n_jobs} ligand-in-water jobs, atoms log-uniform {cfg['atoms']}, windows U{cfg['windows']}, "
+        f"steps U{cfg['steps']}, Poisson arrivals at 80% of measured capacity",
+        "jobs": n_jobs,
+        "atom_steps": work,
+        "capacity_atom_steps_per_s": capacity,
+        "arrival_rate_per_s": rate,
+        "p50_latency_s": st["p50_s"],
+        "p99_latency_s": st["p99_s"],
+        "atom_steps_per_s": work / span,
+        "max_resident_systems": max(r["systems"] for r in srv.log),
+        "fifo": {"p50_latency_s": sf["p50_s"], "p99_latency_s": sf["p99_s"], "atom_steps_per_s": work / span_fifo},
+        "results_bitwise_equal_to_fifo": bool(same),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +185,7 @@ def main() -> int:
     ap.add_argument("--replicas", type=int, default=4)
     ap.add_argument("--single-steps", type=int, default=300)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serving-jobs", type=int, default=12, help="jobs in the serving leg (0 skips it)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--tile-grid", type=int, default=0)
     ap.add_argument("--profile-only", type=int, default=0, metavar="REPS",
@@ -213,13 +271,16 @@ def main() -> int:
         "peak": PEAK_FP64_TFLOPS,
         "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP64_TFLOPS,
-        "traffic": None,
+        "traffic": TILE_TRAFFIC_BYTES,
         "pairs_per_launch": pairs_total,
         "flop_per_launch": flop,
         "avg_launch_ms": prof["nonbonded"],
         "tiles_per_launch": tiles,
         "pair_evaluations_per_launch": tiles * 64 * 64,
     }
+    serve = None
+    if args.serving_jobs > 0:
+        serve = serving_leg(args.serving_jobs)
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (FP64 oracle)")
@@ -256,6 +317,7 @@ def main() -> int:
         "batched_over_single": value / single_rate,
         "kernel_ms": prof,
         "roofline": roofline,
+        "serving": serve,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -105,7 +105,9 @@ void sync_spec_from_engine(tm_context *c) {
     }
 }
 
-void ensure_engine(tm_context *c) {
+// Brings the context's host spec up to date (its own engine's state, barostat
+// changes made through the barostat handle) without building an engine.
+void refresh_spec(tm_context *c) {
     if (c->baro && c->baro_version != c->baro->version) {
         sync_spec_from_engine(c);
         c->eng.reset();
@@ -119,9 +121,21 @@ void ensure_engine(tm_context *c) {
         c->spec.baro = std::make_shared<BarostatSpec>(*c->baro->spec);
         c->baro_version = c->baro->version;
     }
+}
+
+void ensure_engine(tm_context *c) {
+    refresh_spec(c);
     if (!c->eng) {
         c->eng.reset(new Engine({c->spec}));
     }
+}
+
+// The context's state moves into a batch: take it into the spec and drop the
+// context's own engine (the batch holds the state until tm_batch_write_back).
+void hand_to_batch(tm_context *c) {
+    refresh_spec(c);
+    sync_spec_from_engine(c);
+    c->eng.reset();
 }
 
 } // namespace
@@ -599,8 +613,7 @@ int tm_batch_create(int n, tm_context *const *contexts, const double *lambdas, t
         try {
             for (int i = 0; i < n; i++) {
                 tm_context *c = contexts[i];
-                ensure_engine(c);
-                sync_spec_from_engine(c);
+                hand_to_batch(c);
                 specs.push_back(c->spec);
                 b->ctxs.push_back(c);
             }
@@ -650,8 +663,7 @@ int tm_batch_repack(tm_batch *old, int n_keep, const int32_t *keep, int n_new, t
             }
             for (int i = 0; i < n_new; i++) {
                 tm_context *c = new_contexts[i];
-                ensure_engine(c);
-                sync_spec_from_engine(c);
+                hand_to_batch(c);
                 specs.push_back(c->spec);
                 b->ctxs.push_back(c);
             }
