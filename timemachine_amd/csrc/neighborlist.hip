@@ -394,8 +394,12 @@ __global__ __launch_bounds__(256) void k_find_tiles(
         return;
     }
     const int lane = threadIdx.x & 63;
+    constexpr int SUB = 8;
+    constexpr int SUBS = TILE / SUB;
     __shared__ float4 row_lds[4][TILE];
     __shared__ uint32_t buf_lds[4][2 * TILE];
+    __shared__ float4 sub_ctr[4][SUBS];
+    __shared__ float4 sub_ext[4][SUBS];
     float4 *rows = row_lds[wave];
     uint32_t *buf = buf_lds[wave];
 
@@ -418,6 +422,36 @@ __global__ __launch_bounds__(256) void k_find_tiles(
     const float4 rc = bb_ctr[S.block0 + r];
     const float4 re = bb_ext[S.block0 + r];
 
+    // bounding boxes of the 8 row sub-blocks of 8 atoms (lane q < 8 builds
+    // sub-block q; extents are relative to its first atom, min-image)
+    float4 *sctr = sub_ctr[wave];
+    float4 *sext = sub_ext[wave];
+    if (lane < SUBS) {
+        const int a0 = r * TILE + lane * SUB;
+        float4 c = make_float4(0.f, 0.f, 0.f, 0.f), e = make_float4(-1.f, -1.f, -1.f, 0.f); // empty
+        if (a0 < S.k) {
+            const float4 ref = rows[lane * SUB];
+            float lox = 0.f, loy = 0.f, loz = 0.f, hix = 0.f, hiy = 0.f, hiz = 0.f;
+            for (int a = 1; a < SUB && a0 + a < S.k; a++) {
+                const float4 q = rows[lane * SUB + a];
+                const float dx = min_image_f(q.x - ref.x, bx, ibx);
+                const float dy = min_image_f(q.y - ref.y, by, iby);
+                const float dz = min_image_f(q.z - ref.z, bz, ibz);
+                lox = fminf(lox, dx);
+                hix = fmaxf(hix, dx);
+                loy = fminf(loy, dy);
+                hiy = fmaxf(hiy, dy);
+                loz = fminf(loz, dz);
+                hiz = fmaxf(hiz, dz);
+            }
+            c = make_float4(ref.x + 0.5f * (lox + hix), ref.y + 0.5f * (loy + hiy), ref.z + 0.5f * (loz + hiz), 0.f);
+            e = make_float4(0.5f * (hix - lox), 0.5f * (hiy - loy), 0.5f * (hiz - loz), 0.f);
+        }
+        sctr[lane] = c;
+        sext[lane] = e;
+    }
+    wave_lds_sync();
+
     int nbuf = 0;
     for (int cbase = r; cbase < S.nblocks; cbase += TILE) {
         const int c = cbase + lane;
@@ -431,12 +465,33 @@ __global__ __launch_bounds__(256) void k_find_tiles(
             cand = gx * gx + gy * gy + gz * gz < scr2;
         }
         uint64_t cmask = ballot(cand);
-        while (cmask) {
-            const int off = __builtin_ctzll(cmask);
+        // the next candidate's bounding box and column atoms are loaded while the
+        // current one is tested (a row block visits tens of candidates in order)
+        auto load = [&](int cbx, float4 &cc, float4 &ce, float4 &pj) {
+            cc = bb_ctr[S.block0 + cbx];
+            ce = bb_ext[S.block0 + cbx];
+            const int jx = cbx * TILE + lane;
+            pj = jx < S.k ? to_f32(pos[S.slot0 + jx]) : make_float4(0.f, 0.f, 0.f, 0.f);
+        };
+        int cb_next = -1;
+        float4 cc_next, ce_next, pj_next;
+        if (cmask) {
+            cb_next = cbase + __builtin_ctzll(cmask);
             cmask &= cmask - 1;
-            const int cb = cbase + off;
-            const float4 cc = bb_ctr[S.block0 + cb];
-            const float4 ce = bb_ext[S.block0 + cb];
+            load(cb_next, cc_next, ce_next, pj_next);
+        }
+        while (cb_next >= 0) {
+            const int cb = cb_next;
+            const float4 cc = cc_next;
+            const float4 ce = ce_next;
+            const float4 pj_cur = pj_next;
+            if (cmask) {
+                cb_next = cbase + __builtin_ctzll(cmask);
+                cmask &= cmask - 1;
+                load(cb_next, cc_next, ce_next, pj_next);
+            } else {
+                cb_next = -1;
+            }
             // row atoms near the column block
             bool row_near = false;
             if (row_valid) {
@@ -449,25 +504,56 @@ __global__ __launch_bounds__(256) void k_find_tiles(
             const int j = cb * TILE + lane;
             const bool col_valid = j < S.k;
             bool pending = false;
-            float4 pj = make_float4(0.f, 0.f, 0.f, 0.f);
+            const float4 pj = pj_cur;
             if (col_valid && rmask) {
-                pj = to_f32(pos[S.slot0 + j]);
                 const float gx = box_gap(pj.x - rc.x, re.x, bx, ibx);
                 const float gy = box_gap(pj.y - rc.y, re.y, by, iby);
                 const float gz = box_gap(pj.z - rc.z, re.z, bz, ibz);
                 pending = gx * gx + gy * gy + gz * gz < scr2;
             }
+            // Exact test, per lane: a column atom visits only the row sub-blocks
+            // within the screening radius, nearest first, and stops at its first
+            // row atom inside the cutoff. Every row atom that can be within the
+            // cutoff lies in such a sub-block, so the accepted set is the one of
+            // testing all row atoms.
             bool found = false;
-            uint64_t m = rmask;
-            while (m && ballot(pending && !found)) {
-                const int il = __builtin_ctzll(m);
-                m &= m - 1;
-                const float4 q = rows[il];
-                const float dx = min_image_f(q.x - pj.x, bx, ibx);
-                const float dy = min_image_f(q.y - pj.y, by, iby);
-                const float dz = min_image_f(q.z - pj.z, bz, ibz);
-                const float d2 = dx * dx + dy * dy + dz * dz;
-                found = found || (pending && d2 < cut2);
+            if (pending) {
+                uint32_t near = 0;
+                int first = 0;
+                float best = 3.0e38f;
+                for (int q = 0; q < SUBS; q++) {
+                    const float4 se = sext[q];
+                    if (se.x < 0.f) {
+                        continue; // sub-block without atoms
+                    }
+                    const float4 sc = sctr[q];
+                    const float gx = box_gap(pj.x - sc.x, se.x, bx, ibx);
+                    const float gy = box_gap(pj.y - sc.y, se.y, by, iby);
+                    const float gz = box_gap(pj.z - sc.z, se.z, bz, ibz);
+                    const float g2 = gx * gx + gy * gy + gz * gz;
+                    if (g2 < scr2) {
+                        near |= 1u << q;
+                        if (g2 < best) {
+                            best = g2;
+                            first = q;
+                        }
+                    }
+                }
+                int q = first;
+                while (near != 0 && !found) {
+                    near &= ~(1u << q);
+                    const int a_end = min(SUB, S.k - (r * TILE + q * SUB));
+                    for (int a = 0; a < a_end && !found; a++) {
+                        const float4 rq = rows[q * SUB + a];
+                        const float dx = min_image_f(rq.x - pj.x, bx, ibx);
+                        const float dy = min_image_f(rq.y - pj.y, by, iby);
+                        const float dz = min_image_f(rq.z - pj.z, bz, ibz);
+                        found = dx * dx + dy * dy + dz * dz < cut2;
+                    }
+                    if (near != 0) {
+                        q = __builtin_ctz(near);
+                    }
+                }
             }
             const uint64_t fmask = ballot(found);
             if (found) {

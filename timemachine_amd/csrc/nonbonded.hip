@@ -41,14 +41,31 @@ static_assert((PAIR_RING & (PAIR_RING - 1)) == 0 && TILE % PASS1_UNROLL == 0, "r
 
 // Per-wave LDS of the tile kernel. NACC accumulators per atom: x, y, z
 // forces, then q, sig, eps parameter derivatives when requested.
+// Structure of arrays: consecutive lanes read consecutive 8- or 4-byte words,
+// which keeps LDS bank conflicts away (a 32-byte record per lane did not).
 template <int NACC> struct TileLds {
-    double4 colpos[TILE];        // column atom x, y, z, w
-    double4 rowpos[TILE];        // row atom x, y, z, w
-    float4 colpar[TILE];         // column atom q, sig, eps, dw/dlambda
-    float4 rowpar[TILE];         // row atom q, sig, eps, dw/dlambda
+    double pos[2][4][TILE];      // [row, column][x, y, z, w][lane / slot]
+    float par[2][4][TILE];       // [row, column][q, sig, eps, dw/dlambda][lane / slot]
     uint32_t colidx[TILE];       // column sorted index (NB_INVALID for padding)
     uint32_t pij[PAIR_RING];     // queued pair: row lane | column slot << 8
     uint64_t acc[2][NACC][TILE]; // [row, column][component][lane / slot]
+
+    __device__ __forceinline__ void put(int side, int l, const double4 &p, const float4 &a) {
+        pos[side][0][l] = p.x;
+        pos[side][1][l] = p.y;
+        pos[side][2][l] = p.z;
+        pos[side][3][l] = p.w;
+        par[side][0][l] = a.x;
+        par[side][1][l] = a.y;
+        par[side][2][l] = a.z;
+        par[side][3][l] = a.w;
+    }
+    template <bool W> __device__ __forceinline__ double4 get_pos(int side, int l) const {
+        return make_double4(pos[side][0][l], pos[side][1][l], pos[side][2][l], W ? pos[side][3][l] : 0.0);
+    }
+    __device__ __forceinline__ float4 get_par(int side, int l) const {
+        return make_float4(par[side][0][l], par[side][1][l], par[side][2][l], par[side][3][l]);
+    }
 };
 
 __device__ __forceinline__ void lds_add_u64(uint64_t *p, uint64_t v) {
@@ -92,10 +109,10 @@ __device__ __forceinline__ void eval_pairs(TileLds<NACC> &L, const int base, con
         const uint32_t ij = L.pij[(base + lane) & (PAIR_RING - 1)];
         const int il = static_cast<int>(ij & 0xFFu);
         const int sl = static_cast<int>(ij >> 8);
-        const double4 pi = L.rowpos[il];
-        const double4 pj = L.colpos[sl];
-        const float4 ai = L.rowpar[il];
-        const float4 aj = L.colpar[sl];
+        const double4 pi = L.template get_pos<ALCH>(0, il);
+        const double4 pj = L.template get_pos<ALCH>(1, sl);
+        const float4 ai = L.get_par(0, il);
+        const float4 aj = L.get_par(1, sl);
         double dx, dy, dz, dw;
         const double d2 = pair_delta<ALCH>(pi, pj, B.bx, B.by, B.bz, B.ibx, B.iby, B.ibz, dx, dy, dz, dw);
         const PairTerms T = nb_pair_terms<U, DP>(d2, ai.x, aj.x, ai.y, aj.y, ai.z, aj.z, beta, 1.0, 1.0);
@@ -192,10 +209,8 @@ __device__ __forceinline__ void tile_body(
     const double beta = static_cast<double>(S.beta);
 
     wave_lds_sync(); // the previous tile's LDS reads are complete
-    L.colpos[lane] = pj0;
-    L.rowpos[lane] = pi;
-    L.colpar[lane] = aj0;
-    L.rowpar[lane] = ai;
+    L.put(0, lane, pi, ai);
+    L.put(1, lane, pj0, aj0);
     L.colidx[lane] = jl0;
 #pragma unroll
     for (int c = 0; c < NACC; c++) {
@@ -215,7 +230,7 @@ __device__ __forceinline__ void tile_body(
 #pragma unroll
         for (int q = 0; q < PASS1_UNROLL; q++) {
             const int slot = (lane + round + q) & (TILE - 1);
-            pj[q] = L.colpos[slot];
+            pj[q] = L.template get_pos<ALCH>(1, slot);
             jc[q] = L.colidx[slot];
         }
         uint64_t m[PASS1_UNROLL];
