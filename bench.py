@@ -134,12 +134,12 @@ def serving_leg(n_jobs: int, seed: int = 5) -> dict:
     80% of it, and the same arrivals are replayed through FIFO."""
     from timemachine_amd import serving
 
-    cfg = dict(atoms=(1800, 9000), windows=(4, 16), steps=(300, 1500), heavy=(20, 40))
+    cfg = dict(atoms=(1800, 9000), windows=(4, 16), steps=(200, 1000), heavy=(20, 40))
     jobs = serving.synthetic_stream(n_jobs, rate_per_s=0.0, seed=seed, **cfg)
     work = sum(j.atom_steps for j in jobs)
     budget = 32 * 2**30
     log(f"serving: {n_jobs} jobs, {work:.3g} atom-steps; saturated run for capacity")
-    srv = serving.Server(budget, max_chunk=600, du_dl_interval=10)
+    srv = serving.Server(budget, max_chunk=200, du_dl_interval=10)
     t0 = time.perf_counter()
     srv.run(jobs)
     capacity = work / (time.perf_counter() - t0)
@@ -150,7 +150,7 @@ def serving_leg(n_jobs: int, seed: int = 5) -> dict:
         t += rng.exponential(1.0 / rate)
         j.arrival = t
     log(f"serving: Poisson arrivals at {rate:.3g} jobs/s (80% of capacity {capacity:.3g} atom-steps/s)")
-    srv = serving.Server(budget, max_chunk=600, du_dl_interval=10)
+    srv = serving.Server(budget, max_chunk=200, du_dl_interval=10)
     t0 = time.perf_counter()
     served = srv.run(jobs)
     span = time.perf_counter() - t0
@@ -171,6 +171,9 @@ def serving_leg(n_jobs: int, seed: int = 5) -> dict:
         "p99_latency_s": st["p99_s"],
         "atom_steps_per_s": work / span,
         "max_resident_systems": max(r["systems"] for r in srv.log),
+        "chunks": len(srv.log),
+        "chunk_seconds": sum(r["seconds"] for r in srv.log),
+        "repack_seconds": srv.repack_seconds,
         "fifo": {"p50_latency_s": sf["p50_s"], "p99_latency_s": sf["p99_s"], "atom_steps_per_s": work / span_fifo},
         "results_bitwise_equal_to_fifo": bool(same),
     }
@@ -185,9 +188,10 @@ def main() -> int:
     ap.add_argument("--replicas", type=int, default=4)
     ap.add_argument("--single-steps", type=int, default=300)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--serving-jobs", type=int, default=12, help="jobs in the serving leg (0 skips it)")
+    ap.add_argument("--serving-jobs", type=int, default=24, help="jobs in the serving leg (0 skips it)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--tile-grid", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true", help="skip the grid / padding autotuner")
     ap.add_argument("--profile-only", type=int, default=0, metavar="REPS",
                     help="build the batch, warm up, launch each kernel group REPS times eagerly and exit "
                     "(for rocprofv3 counter runs)")
@@ -224,8 +228,13 @@ def main() -> int:
     log(f"building {args.windows * args.replicas} contexts of {sys_.N} atoms")
     ctxs, keep, lam_list = make_contexts(sys_, lams, args.replicas)
     batch = custom_ops.BatchedContexts(ctxs, lam_list)
+    tuned = None
     if args.tile_grid:
         batch.set_tile_grid(args.tile_grid)
+    elif not args.no_autotune and not args.profile_only:
+        log("autotuning grid and neighborlist padding on a device copy of the batch")
+        tuned = batch.autotune(steps=30)
+        log(f"autotuned: {tuned['grid']} workgroups, padding {tuned['padding']} nm")
     S = len(ctxs)
     use_graph = not args.no_graph
     log(f"warmup {args.warmup} batched steps")
@@ -307,6 +316,7 @@ def main() -> int:
             "barostat_interval": 15,
             "du_dl_interval": 1,
             "hipgraph": use_graph,
+            "autotuned": None if tuned is None else {"grid": tuned["grid"], "padding_nm": tuned["padding"]},
         },
         "single_system": {
             "value": single_rate,

@@ -205,8 +205,13 @@ int tm_batch_write_back(tm_batch *b);
 int tm_batch_profile(tm_batch *b, int reps, double *ms_out);
 /* Tiles in the current neighborlists of all systems. */
 int tm_batch_total_tiles(tm_batch *b, int *out);
-/* Workgroups of the persistent pair kernel (0 = default, CUs x 8). */
+/* Workgroups of the pair (tile) kernel (0 = default, CUs x 64). No reference counterpart
+ * (the reference launches one block per tile); used by the autotuner. */
 int tm_batch_set_tile_grid(tm_batch *b, int grid);
+/* Neighborlist padding of every system of the batch (nm); the counterpart of
+ * NonbondedAllPairs::set_nblist_padding (nonbonded_all_pairs.cu:180-187) for a batch.
+ * Results do not depend on it. */
+int tm_batch_set_nblist_padding(tm_batch *b, double padding);
 
 /* Gradient observer (no reference counterpart; the statistics behind gradients of
  * lambda-window averages of A = du/dlambda with respect to forcefield parameters).

@@ -100,6 +100,7 @@ _SIGS = {
     "tm_batch_repack": (c_int, [_P, c_int, _PI, c_int, _PPV, _PD, _PPV]),
     "tm_batch_device_bytes": (c_int, [_P, _PU64]),
     "tm_batch_set_tile_grid": (c_int, [_P, c_int]),
+    "tm_batch_set_nblist_padding": (c_int, [_P, c_double]),
     "tm_batch_set_grad_observer": (c_int, [_P, c_int]),
     "tm_batch_observe": (c_int, [_P]),
     "tm_batch_grad_size": (c_int, [_P, c_int, POINTER(c_int)]),

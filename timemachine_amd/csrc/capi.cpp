@@ -742,7 +742,16 @@ int tm_batch_total_tiles(tm_batch *b, int *out) {
 }
 
 int tm_batch_set_tile_grid(tm_batch *b, int grid) {
-    return guarded([&] { b->eng->tile_grid = grid; });
+    return guarded([&] {
+        if (grid < 0) {
+            throw Error("tile grid must be >= 0");
+        }
+        b->eng->tile_grid = grid;
+    });
+}
+
+int tm_batch_set_nblist_padding(tm_batch *b, double padding) {
+    return guarded([&] { b->eng->set_nblist_padding(padding); });
 }
 
 int tm_batch_set_grad_observer(tm_batch *b, int interval) {

@@ -123,6 +123,9 @@ class Engine {
 
     hipStream_t stream() const { return stream_; }
 
+    // Neighborlist padding of every system (autotuner); forces a rebuild.
+    void set_nblist_padding(double padding);
+
     int tile_grid = 0; // nonbonded grid override (autotuner), 0 = default
 
   private:

@@ -996,6 +996,13 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
     TM_HIP_CHECK(hipStreamSynchronize(stream_));
 }
 
+void Engine::set_nblist_padding(double padding) {
+    Impl &I = *impl_;
+    if (I.has_nb) {
+        I.nb.set_padding(padding, stream_);
+    }
+}
+
 void Engine::copy_state_from(Engine &src, int src_s, int dst_s) {
     Impl &I = *impl_;
     Impl &J = *src.impl_;

@@ -97,6 +97,9 @@ struct NbState {
 
     // Force a rebuild of every system on the next prepare().
     void invalidate(hipStream_t stream);
+    // Neighborlist padding of every system (results do not depend on it; it
+    // trades rebuild frequency against tiles). Forces a rebuild.
+    void set_padding(double padding, hipStream_t stream);
 
     // Rebuild check, spatial sort, gather to sorted FP32, bounding boxes,
     // neighborlist and tile prefix. No host synchronisation.

@@ -715,6 +715,20 @@ void NbState::init(const std::vector<NbSys> &sys_in, const std::vector<int> &sub
 
 void NbState::invalidate(hipStream_t stream) { launch_fill_i32(rebuild.ptr, S, 1, stream); }
 
+void NbState::set_padding(double padding, hipStream_t stream) {
+    if (!(padding >= 0.0)) {
+        throw Error("neighborlist padding must be >= 0");
+    }
+    for (auto &y : h_sys) {
+        y.padding_d = padding;
+        y.nb_cutoff = static_cast<float>(y.cutoff_d + y.padding_d + nb_margin);
+    }
+    TM_HIP_CHECK(hipStreamSynchronize(stream));
+    sys.upload(h_sys.data(), S, stream);
+    invalidate(stream);
+    TM_HIP_CHECK(hipStreamSynchronize(stream));
+}
+
 void NbState::build_only(const double *x, const double *boxes, hipStream_t stream) {
     const int gx_atoms = ceil_div(std::max(max_atoms, 9), 256);
     k_check_rebuild<<<dim3(gx_atoms, S), 256, 0, stream>>>(sys.ptr, subset.ptr, x, nblist_x.ptr, boxes,

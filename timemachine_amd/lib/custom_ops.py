@@ -757,6 +757,40 @@ class BatchedContexts(_Handle):
     def set_tile_grid(self, grid: int):
         call("tm_batch_set_tile_grid", self._h, int(grid))
 
+    def set_nblist_padding(self, padding: float):
+        call("tm_batch_set_nblist_padding", self._h, float(padding))
+
+    def autotune(self, steps: int = 30, grids=(8192, 16384, 32768), paddings=(0.1, 0.125, 0.15),
+                 store_du_dl_interval: int = 1, use_graph: bool = True) -> dict:
+        """Chooses the pair-kernel grid and the neighborlist padding (extension).
+
+        Every candidate is timed on a scratch copy of this batch made on the
+        device (repack), so this batch's trajectories are untouched; results do
+        not depend on either setting (fixed-point sums are order-independent and
+        the tile kernel tests the exact cutoff), only the speed does. Padding
+        trades neighborlist rebuilds against tiles, the grid trades occupancy
+        against load balance. Applies and returns the fastest setting."""
+        import time
+
+        scratch = self.repack(range(len(self)), [], self._lam)
+        table = []
+        for pad in paddings:
+            scratch.set_nblist_padding(pad)
+            for g in grids:
+                scratch.set_tile_grid(g)
+                scratch.multiple_steps(max(steps // 3, 1), store_du_dl_interval=store_du_dl_interval,
+                                       use_graph=use_graph)
+                t0 = time.perf_counter()
+                scratch.multiple_steps(steps, store_du_dl_interval=store_du_dl_interval, use_graph=use_graph)
+                table.append({"grid": int(g), "padding": float(pad),
+                              "ms_per_step": 1e3 * (time.perf_counter() - t0) / steps})
+        del scratch
+        best = min(table, key=lambda r: r["ms_per_step"])
+        self.set_tile_grid(best["grid"])
+        self.set_nblist_padding(best["padding"])
+        return {"grid": best["grid"], "padding": best["padding"], "ms_per_step": best["ms_per_step"],
+                "candidates": table}
+
     def multiple_steps(self, n_steps, lambda_schedules=None, store_du_dl_interval=0, store_x_interval=0, use_graph=True):
         S = len(self._ctxs)
         sched = None

@@ -140,6 +140,7 @@ class Server:
         self._clock = clock
         self._sleep = sleep
         self.log: List[dict] = []  # one record per chunk (resident systems, steps, duration)
+        self.repack_seconds = 0.0  # time spent building engines at admissions and retirements
 
     def _job_bytes(self, job: Job) -> int:
         return job.windows * estimate_system_bytes(job.system.N)
@@ -172,12 +173,14 @@ class Server:
                 self._sleep(max(0.0, pending[0].arrival - now()))
                 continue
             if new:
+                tr = now()
                 lam = [r.job.lambdas[r.w] for r in running] + [n.job.lambdas[n.w] for n in new]
                 if batch is None:
                     batch = custom_ops.BatchedContexts([n.ctx for n in new], lam)
                 else:
                     batch = batch.repack(range(len(running)), [n.ctx for n in new], lam)
                 running += new
+                self.repack_seconds += now() - tr
             chunk = min(self.max_chunk, min(r.remaining for r in running))
             tc = now()
             du_dl, _, _ = batch.multiple_steps(chunk, store_du_dl_interval=self.du_dl_interval,
@@ -202,11 +205,13 @@ class Server:
                                              admitted_at[jid], t_fin))
                     used -= self._job_bytes(job)
                 keep = [i for i, r in enumerate(running) if r.job.job_id not in done_jobs]
+                tr = now()
                 if keep:
                     lam = [running[i].job.lambdas[running[i].w] for i in keep]
                     batch = batch.repack(keep, [], lam)
                 else:
                     batch = None
+                self.repack_seconds += now() - tr
                 running = [running[i] for i in keep]
         return sorted(results, key=lambda r: r.job_id)
 
