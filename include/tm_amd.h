@@ -162,6 +162,12 @@ int tm_context_step(tm_context *c, double lambda);
  * du_dl_out [n_steps / du_dl_interval], x_out [F, N, 3], box_out [F, 3, 3], F = n_steps / x_interval. */
 int tm_context_multiple_steps(tm_context *c, int n_steps, const double *lambda_schedule, int du_dl_interval,
                               int x_interval, double *du_dl_out, double *x_out, double *box_out);
+/* Context::multiple_steps_U, context.cu:139-233: steps at `lambda`; every u_interval steps the
+ * state after the step is evaluated at each of the n_windows lambdas, u_out [n_steps / u_interval,
+ * n_windows]; frames every x_interval steps, x_out [F, N, 3], box_out [F, 3, 3]. Intervals > 0
+ * (the Python layer maps 0 to n_steps, wrap_kernels.cpp:197-198). */
+int tm_context_multiple_steps_U(tm_context *c, double lambda, int n_steps, int n_windows, const double *windows,
+                                int u_interval, int x_interval, double *u_out, double *x_out, double *box_out);
 /* Context getters / setters, context.cu:284-302 */
 int tm_context_get_x_t(tm_context *c, double *out);
 int tm_context_get_v_t(tm_context *c, double *out);
@@ -192,6 +198,10 @@ int tm_batch_total_atoms(tm_batch *b, int *out);
  * box_out [n, F, 3, 3]. Intervals of 0 disable the store. */
 int tm_batch_multiple_steps(tm_batch *b, int n_steps, const double *lambda_sched, int du_dl_interval,
                             int x_interval, double *du_dl_out, double *x_out, double *box_out, int use_graph);
+/* multiple_steps_U for every system of the batch at its own lambda (no reference counterpart for
+ * a batch): u_out [n, n_steps / u_interval, n_windows]; x_interval 0 stores no frames. */
+int tm_batch_multiple_steps_U(tm_batch *b, int n_steps, int n_windows, const double *windows, int u_interval,
+                              int x_interval, double *u_out, double *x_out, double *box_out, int use_graph);
 /* u and du/dlambda of every system at its current state. */
 int tm_batch_energies(tm_batch *b, double *u_out, double *du_dl_out);
 int tm_batch_get_x(tm_batch *b, int s, double *out);

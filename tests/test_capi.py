@@ -17,7 +17,7 @@ ROOT = Path(__file__).resolve().parents[1]
 def _header_symbols():
     text = (ROOT / "include" / "tm_amd.h").read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(tm_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(tm_[A-Za-z0-9_]+)\s*\(", text)))
 
 
 def test_library_loads_and_exports_header_symbols():

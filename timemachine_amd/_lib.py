@@ -82,6 +82,8 @@ _SIGS = {
     "tm_context_num_atoms": (c_int, [_P, POINTER(c_int)]),
     "tm_context_step": (c_int, [_P, c_double]),
     "tm_context_multiple_steps": (c_int, [_P, c_int, _PD, c_int, c_int, _PD, _PD, _PD]),
+    "tm_context_multiple_steps_U": (c_int, [_P, c_double, c_int, c_int, _PD, c_int, c_int, _PD, _PD, _PD]),
+    "tm_batch_multiple_steps_U": (c_int, [_P, c_int, c_int, _PD, c_int, c_int, _PD, _PD, _PD, c_int]),
     "tm_context_get_x_t": (c_int, [_P, _PD]),
     "tm_context_get_v_t": (c_int, [_P, _PD]),
     "tm_context_get_box": (c_int, [_P, _PD]),

@@ -573,6 +573,26 @@ int tm_context_multiple_steps(tm_context *c, int n_steps, const double *lambda_s
     });
 }
 
+int tm_context_multiple_steps_U(tm_context *c, double lambda, int n_steps, int n_windows, const double *windows,
+                                int u_interval, int x_interval, double *u_out, double *x_out, double *box_out) {
+    return guarded([&] {
+        if (u_interval <= 0) {
+            throw Error("store_u_interval <= 0");
+        }
+        if (x_interval <= 0) {
+            throw Error("store_x_interval <= 0");
+        }
+        if (n_windows < 0 || (n_windows > 0 && !windows)) {
+            throw Error("lambda_windows is NULL");
+        }
+        ensure_engine(c);
+        c->eng->set_lambdas({lambda});
+        const std::vector<double> w(windows, windows + n_windows);
+        c->eng->run(n_steps, nullptr, 0, x_interval, nullptr, x_out, box_out, true, &w, u_interval, u_out);
+        c->eng->check_status();
+    });
+}
+
 int tm_context_get_x_t(tm_context *c, double *out) {
     return guarded([&] {
         ensure_engine(c);
@@ -697,6 +717,21 @@ int tm_batch_multiple_steps(tm_batch *b, int n_steps, const double *lambda_sched
                             double *du_dl_out, double *x_out, double *box_out, int use_graph) {
     return guarded([&] {
         b->eng->run(n_steps, lambda_sched, du_dl_interval, x_interval, du_dl_out, x_out, box_out, use_graph != 0);
+        b->eng->check_status();
+    });
+}
+
+int tm_batch_multiple_steps_U(tm_batch *b, int n_steps, int n_windows, const double *windows, int u_interval,
+                              int x_interval, double *u_out, double *x_out, double *box_out, int use_graph) {
+    return guarded([&] {
+        if (u_interval <= 0) {
+            throw Error("store_u_interval <= 0");
+        }
+        if (n_windows < 0 || (n_windows > 0 && !windows)) {
+            throw Error("lambda_windows is NULL");
+        }
+        const std::vector<double> w(windows, windows + n_windows);
+        b->eng->run(n_steps, nullptr, 0, x_interval, nullptr, x_out, box_out, use_graph != 0, &w, u_interval, u_out);
         b->eng->check_status();
     });
 }

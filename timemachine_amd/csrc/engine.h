@@ -74,8 +74,12 @@ class Engine {
     // (0: never) into du_dl_out [S, n_steps / du_dl_interval]; frames every
     // x_interval steps (0: never) into x_out (per system, concatenated
     // [F, N_s, 3] blocks in system order) and box_out [S, F, 9].
+    // With u_windows (W lambdas) and u_interval > 0, the state after every
+    // u_interval-th step is also evaluated at each window:
+    // u_out [S, n_steps / u_interval, W] (Context::multiple_steps_U).
     void run(int n_steps, const double *lambda_sched, int du_dl_interval, int x_interval, double *du_dl_out,
-             double *x_out, double *box_out, bool use_graph);
+             double *x_out, double *box_out, bool use_graph, const std::vector<double> *u_windows = nullptr,
+             int u_interval = 0, double *u_out = nullptr);
 
     // Device-to-device handover of system src_s of `src` (x, v, box, step
     // counter, barostat state) into system dst_s of this engine; the

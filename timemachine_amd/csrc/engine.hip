@@ -113,6 +113,29 @@ __global__ void k_store_frames2(int A, const int *__restrict__ atom_sys, const i
     frames[dst + 2] = x[a * 3 + 2];
 }
 
+// Energies at other lambda windows: every system evaluates window w.
+__global__ void k_set_eval_lambda(int S, const double *__restrict__ windows, int w, double *__restrict__ lam_eval) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < S) {
+        lam_eval[s] = windows[w];
+    }
+}
+
+// u_traj[s, f, w] = u of the state after run step run_step[s] (f-th stored
+// frame, (run_step + 1) % u_interval == 0), then clears the reduction slot.
+__global__ void k_store_u(int S, const int *__restrict__ run_step, int u_interval, int Fu, int W, int w,
+                          uint64_t *__restrict__ u_buf, uint64_t *__restrict__ u_traj) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) {
+        return;
+    }
+    const int f = (run_step[s] + 1) / u_interval - 1;
+    if (f >= 0 && f < Fu) {
+        u_traj[(static_cast<size_t>(s) * Fu + f) * W + w] = u_buf[s];
+    }
+    u_buf[s] = 0;
+}
+
 // Gradient observer: per flat parameter element e (nonbonded [A, 3] rows, then
 // bond / angle / torsion parameters) of system s = elem_sys[e], with
 // A_s = du/dlambda, g = du/dp, h = d(du/dlambda)/dp at the observed frame:
@@ -428,23 +451,28 @@ struct Engine::Impl {
     }
 
     void eval(const double *xp, const double *boxp, uint64_t *ddx, uint64_t *ddl, uint64_t *uu, hipStream_t st,
-              int tile_grid) {
+              int tile_grid, const double *lamp = nullptr) {
+        if (!lamp) {
+            lamp = lam.ptr;
+        }
         if (has_nb) {
-            nb.prepare(xp, nb_params.ptr, boxp, nb_plane.ptr, nb_offset.ptr, lam.ptr, 0.0, st);
+            nb.prepare(xp, nb_params.ptr, boxp, nb_plane.ptr, nb_offset.ptr, lamp, 0.0, st);
             nb.compute(boxp, ddx, nullptr, ddl, uu, st, tile_grid);
         }
         for (int neg = 0; neg < 2; neg++) {
             if (pl_n[neg] == 0) {
                 continue;
             }
-            launch_pair_list(pl_args(neg, xp, boxp), ddx, nullptr, ddl, uu, st);
+            PairListArgs a = pl_args(neg, xp, boxp);
+            a.lambdas = lamp;
+            launch_pair_list(a, ddx, nullptr, ddl, uu, st);
         }
         for (int k = 0; k < 3; k++) {
             if (bt_n[k] == 0) {
                 continue;
             }
             BondedTerms t{bt_n[k], bt_idx[k].ptr, bt_params[k].ptr, bt_mult[k].ptr, bt_offset[k].ptr, bt_sys[k].ptr};
-            launch_bonded(static_cast<BondedKind>(k), t, xp, lam.ptr, 0.0, red_idx.ptr, ddx, nullptr, ddl, uu, st);
+            launch_bonded(static_cast<BondedKind>(k), t, xp, lamp, 0.0, red_idx.ptr, ddx, nullptr, ddl, uu, st);
         }
     }
 
@@ -840,10 +868,19 @@ void Engine::set_lambdas(const std::vector<double> &lams) {
 }
 
 void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, int x_interval, double *du_dl_out,
-                 double *x_out, double *box_out, bool use_graph) {
+                 double *x_out, double *box_out, bool use_graph, const std::vector<double> *u_windows,
+                 int u_interval, double *u_out) {
     Impl &I = *impl_;
     if (n_steps <= 0) {
         return;
+    }
+    const int W = u_windows ? static_cast<int>(u_windows->size()) : 0;
+    const int Fu = (W > 0 && u_interval > 0) ? n_steps / u_interval : 0;
+    DeviceArray<double> d_windows(std::max(W, 1)), lam_eval(S_);
+    DeviceArray<uint64_t> u_traj(std::max<size_t>(static_cast<size_t>(S_) * Fu * W, 1));
+    if (Fu > 0) {
+        d_windows.upload(u_windows->data(), W, stream_);
+        launch_fill_u64(I.u_buf.ptr, S_, 0, stream_);
     }
     const int K = du_dl_interval > 0 ? n_steps / du_dl_interval : 0;
     const int F = x_interval > 0 ? n_steps / x_interval : 0;
@@ -860,7 +897,7 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
 
     const int TPB = 256;
     const int obs_k = I.obs_interval;
-    auto one_step = [&](bool baro_now, bool obs_now) {
+    auto one_step = [&](bool baro_now, bool obs_now, bool u_now) {
         if (lambda_sched) {
             k_set_lambda<<<ceil_div(S_, 64), 64, 0, stream_>>>(S_, sched.ptr, n_steps, I.run_step.ptr, I.lam.ptr);
             TM_LAUNCH_CHECK();
@@ -902,6 +939,17 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
                 TM_LAUNCH_CHECK();
             }
         }
+        if (u_now) {
+            // energies of the state after this step at every window (context.cu:139-233)
+            for (int w = 0; w < W; w++) {
+                k_set_eval_lambda<<<ceil_div(S_, 64), 64, 0, stream_>>>(S_, d_windows.ptr, w, lam_eval.ptr);
+                TM_LAUNCH_CHECK();
+                I.eval(I.x.ptr, I.box.ptr, nullptr, nullptr, I.u_buf.ptr, stream_, tile_grid, lam_eval.ptr);
+                k_store_u<<<ceil_div(S_, 64), 64, 0, stream_>>>(S_, I.run_step.ptr, u_interval, Fu, W, w, I.u_buf.ptr,
+                                                              u_traj.ptr);
+                TM_LAUNCH_CHECK();
+            }
+        }
         if (x_interval > 0) {
             k_store_frames2<<<ceil_div(A_, TPB), TPB, 0, stream_>>>(A_, I.atom_sys.ptr, I.d_atom0.ptr,
                                                                   I.d_natoms.ptr, I.run_step.ptr, x_interval, F,
@@ -931,6 +979,8 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
     // observed frames: the states at which du_dl is stored with interval obs_k
     // (the state at the start of run step j, (j + 1) % obs_k == 0)
     auto obs_at = [&](int j) { return obs_k > 0 && (j + 1) % obs_k == 0; };
+    // energies at the windows: the state after run step j, (j + 1) % u_interval == 0
+    auto u_at = [&](int j) { return Fu > 0 && (j + 1) % u_interval == 0; };
 
     int j = 0;
     if (use_graph) {
@@ -947,14 +997,15 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
         while (j + chunk <= n_steps) {
             std::vector<uint8_t> pattern(chunk);
             for (int q = 0; q < chunk; q++) {
-                pattern[q] = static_cast<uint8_t>((baro_at(j + q) ? 1 : 0) | (obs_at(j + q) ? 2 : 0));
+                pattern[q] = static_cast<uint8_t>((baro_at(j + q) ? 1 : 0) | (obs_at(j + q) ? 2 : 0) |
+                                                  (u_at(j + q) ? 4 : 0));
             }
             auto it = cache.find(pattern);
             if (it == cache.end()) {
                 hipGraph_t g;
                 TM_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
                 for (int q = 0; q < chunk; q++) {
-                    one_step((pattern[q] & 1) != 0, (pattern[q] & 2) != 0);
+                    one_step((pattern[q] & 1) != 0, (pattern[q] & 2) != 0, (pattern[q] & 4) != 0);
                 }
                 TM_HIP_CHECK(hipStreamEndCapture(stream_, &g));
                 hipGraphExec_t ge;
@@ -971,7 +1022,7 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
         }
     }
     for (; j < n_steps; j++) {
-        one_step(baro_at(j), obs_at(j));
+        one_step(baro_at(j), obs_at(j), u_at(j));
     }
     for (int s = 0; s < S_; s++) {
         if (I.h_has_baro[s]) {
@@ -989,6 +1040,14 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
     }
     if (x_out && F > 0) {
         frames.download(x_out, static_cast<size_t>(F) * A_ * 3, stream_);
+    }
+    if (u_out && Fu > 0) {
+        std::vector<uint64_t> h(static_cast<size_t>(S_) * Fu * W);
+        u_traj.download(h.data(), h.size(), stream_);
+        TM_HIP_CHECK(hipStreamSynchronize(stream_));
+        for (size_t i = 0; i < h.size(); i++) {
+            u_out[i] = fixed_to_double(h[i], FIXED_SCALE);
+        }
     }
     if (box_out && F > 0) {
         box_frames.download(box_out, static_cast<size_t>(S_) * F * 9, stream_);

@@ -672,7 +672,24 @@ class Context(_Handle):
         return du_dl, xs, boxes
 
     def multiple_steps_U(self, lamb, n_steps, lambda_windows, store_u_interval, store_x_interval):
-        raise NotImplementedError("multiple_steps_U is not implemented on the MI355X path yet")
+        """Steps at lamb; energies of the state after every store_u_interval-th step at
+        each lambda of lambda_windows (context.cu:139-233). Returns (u [F, K], xs, boxes)."""
+        windows = f64(lambda_windows).reshape(-1)
+        n_steps = int(n_steps)
+        u_int = n_steps if store_u_interval <= 0 else int(store_u_interval)
+        x_int = n_steps if store_x_interval <= 0 else int(store_x_interval)
+        if n_steps % x_int != 0:
+            print("warning:: n_steps modulo store_x_interval does not equal zero")
+        if n_steps % u_int != 0:
+            print("warning:: n_steps modulo store_u_interval does not equal zero")
+        UF = n_steps // u_int
+        F = n_steps // x_int
+        u = np.zeros((UF, windows.size))
+        xs = np.zeros((F, self.N, 3))
+        boxes = np.zeros((F, 3, 3))
+        call("tm_context_multiple_steps_U", self._h, float(lamb), n_steps, int(windows.size), pd(windows), u_int,
+             x_int, pd(u), pd(xs), pd(boxes))
+        return u, xs, boxes
 
     def set_x_t(self, x):
         x = f64(x)
@@ -820,6 +837,29 @@ class BatchedContexts(_Handle):
             frames.append(xs[off : off + F * n * 3].reshape(F, n, 3))
             off += F * n * 3
         return du_dl, frames, boxes
+
+    def multiple_steps_U(self, n_steps, lambda_windows, store_u_interval, store_x_interval=0, use_graph=True):
+        """Every system steps at its own lambda; the state after every store_u_interval-th
+        step is evaluated at each of lambda_windows. Returns (u [S, F, K], frames, boxes)."""
+        S = len(self._ctxs)
+        windows = f64(lambda_windows).reshape(-1)
+        if store_u_interval <= 0:
+            raise RuntimeError("store_u_interval <= 0")
+        UF = n_steps // store_u_interval
+        F = n_steps // store_x_interval if store_x_interval > 0 else 0
+        u = np.zeros((S, UF, windows.size))
+        total = sum(self.sizes)
+        xs = np.zeros(max(F * total * 3, 1))
+        boxes = np.zeros((S, max(F, 0), 3, 3))
+        call("tm_batch_multiple_steps_U", self._h, int(n_steps), int(windows.size), pd(windows),
+             int(store_u_interval), int(store_x_interval), pd(u), pd(xs) if F else None, pd(boxes) if F else None,
+             1 if use_graph else 0)
+        frames = []
+        off = 0
+        for n in self.sizes:
+            frames.append(xs[off : off + F * n * 3].reshape(F, n, 3))
+            off += F * n * 3
+        return u, frames, boxes
 
     def energies(self):
         S = len(self._ctxs)
