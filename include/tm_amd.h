@@ -53,6 +53,13 @@ int tm_device_count(int *n);
 int tm_nonbonded_all_pairs_create(int N, const int32_t *lambda_plane_idxs, const int32_t *lambda_offset_idxs,
                                   double beta, double cutoff, const int32_t *atom_idxs, int n_atom_idxs,
                                   tm_potential **out);
+/* NonbondedAllPairs_f32_interpolated: parameters [2N, 3] (initial, final) used as
+ * p = (1 - lambda) p0 + lambda p1 with dp/dlambda entering du/dlambda; du/dp returned for both
+ * halves (nonbonded_all_pairs.cu:340-351, 420-450; k_lambda_transformer_jit.cuh:57-137, identity
+ * transform). */
+int tm_nonbonded_all_pairs_interpolated_create(int N, const int32_t *lambda_plane_idxs,
+                                               const int32_t *lambda_offset_idxs, double beta, double cutoff,
+                                               const int32_t *atom_idxs, int n_atom_idxs, tm_potential **out);
 /* NonbondedAllPairs::set_nblist_padding, nonbonded_all_pairs.cu:180-183 */
 int tm_nonbonded_all_pairs_set_nblist_padding(tm_potential *p, double padding);
 /* NonbondedAllPairs::disable_hilbert_sort, nonbonded_all_pairs.cu:185-187 */
@@ -65,6 +72,11 @@ int tm_nonbonded_all_pairs_num_tiles(tm_potential *p, int *out);
 int tm_nonbonded_pair_list_create(int N, int M, const int32_t *pair_idxs, const double *scales,
                                   const int32_t *lambda_plane_idxs, const int32_t *lambda_offset_idxs, double beta,
                                   double cutoff, int negated, tm_potential **out);
+/* NonbondedPairList_f32[_negated]_interpolated, parameters [2N, 3] as above
+ * (nonbonded_pair_list.cu, Interpolated = true). */
+int tm_nonbonded_pair_list_interpolated_create(int N, int M, const int32_t *pair_idxs, const double *scales,
+                                               const int32_t *lambda_plane_idxs, const int32_t *lambda_offset_idxs,
+                                               double beta, double cutoff, int negated, tm_potential **out);
 
 /* HarmonicBond_f32(bond_idxs, lamb_mult, lamb_offset), harmonic_bond.cu:10-56. mult/offset may be NULL. */
 int tm_harmonic_bond_create(int B, const int32_t *bond_idxs, const int32_t *lamb_mult, const int32_t *lamb_offset,

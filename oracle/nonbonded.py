@@ -9,6 +9,8 @@ Two evaluators:
   * nonbonded_v3         dense O(N^2), a line-by-line restatement (small N);
   * nonbonded_pairs      the same energy summed over the pair list within the
                          cutoff (large N); pinned to the dense form in tests.
+Both take ``interpolated=True`` for parameters [2N, 3] = (initial, final) used
+at lambda as (1 - lambda) initial + lambda final (nonbonded.py:341-353).
 """
 
 from __future__ import annotations
@@ -77,6 +79,20 @@ def nonbonded_v3_energy(conf, params, box, lamb, q_mask, lj_mask, beta, cutoff, 
     return torch.sum(e_lj * ljm + e_q * qm) / 2
 
 
+def interpolated(energy_fn):
+    """nonbonded.py:341-353: energy over [2N, 3] parameters, the first N rows
+    weighted (1 - lambda) and the last N rows lambda."""
+
+    def fn(xt, pt, lt):
+        n = pt.shape[0] // 2
+        return energy_fn(xt, (1 - lt) * pt[:n] + lt * pt[n:], lt)
+
+    return fn
+
+
+_interpolated = interpolated  # the evaluators' keyword argument of the same name shadows it
+
+
 def _grads(energy_fn, x, params, lamb):
     xt = _t(x, True)
     pt = _t(params, True)
@@ -88,12 +104,12 @@ def _grads(energy_fn, x, params, lamb):
     return gx.numpy(), gp.numpy(), float(gl), float(u.detach())
 
 
-def nonbonded_v3(x, params, box, lamb, exclusion_idxs, scales, beta, cutoff, plane, offset):
-    """(du_dx [N,3], du_dp [N,3], du_dl, u) of the dense FP64 restatement."""
+def nonbonded_v3(x, params, box, lamb, exclusion_idxs, scales, beta, cutoff, plane, offset, interpolated=False):
+    """(du_dx [N,3], du_dp [N,3] ([2N,3] interpolated), du_dl, u) of the dense FP64 restatement."""
     N = x.shape[0]
     qm, ljm = convert_exclusions_to_rescale_masks(exclusion_idxs, scales, N)
     fn = lambda xt, pt, lt: nonbonded_v3_energy(xt, pt, box, lt, qm, ljm, beta, cutoff, plane, offset)  # noqa: E731
-    return _grads(fn, x, params, lamb)
+    return _grads(_interpolated(fn) if interpolated else fn, x, params, lamb)
 
 
 def candidate_pairs(x: np.ndarray, box: np.ndarray, r: float) -> np.ndarray:
@@ -139,7 +155,8 @@ def _pair_energy(xt, pt, lt, pairs, box_diag, beta, cutoff, plane, offset, scale
 
 
 def nonbonded_pairs(
-    x, params, box, lamb, exclusion_idxs, scales, beta, cutoff, plane, offset, pairs: Optional[np.ndarray] = None
+    x, params, box, lamb, exclusion_idxs, scales, beta, cutoff, plane, offset, pairs: Optional[np.ndarray] = None,
+    interpolated: bool = False,
 ) -> Tuple[np.ndarray, np.ndarray, float, float]:
     """Same energy as nonbonded_v3 summed over pairs (all pairs within the cutoff,
     minus scale * pair term for each excluded pair)."""
@@ -158,7 +175,7 @@ def nonbonded_pairs(
             u = u - _pair_energy(xt, pt, lt, excl, box_diag, beta, cutoff, plane_t, offset_t, sc[:, 0], sc[:, 1])
         return u
 
-    return _grads(fn, x, params, lamb)
+    return _grads(_interpolated(fn) if interpolated else fn, x, params, lamb)
 
 
 def _dl_dp(energy_fn, x, params, lamb):

@@ -315,3 +315,75 @@ def test_bonded_dl_dp_closed_form(rng):
     phi = bonded.signed_torsion_angle(*[torch.tensor(x[tidx[:, c]]) for c in range(4)]).numpy()
     np.testing.assert_allclose(gt[0, 0], 2 * (1 + np.cos(tpar[0, 2] * phi[0] - tpar[0, 1])), rtol=1e-12)
     assert np.all(gt[1] == 0.0)
+
+
+def _interp_params(rng, p0, appear=True):
+    """Final parameters for p0; with ``appear`` some LJ-off atoms (eps 0) switch on
+    and others off, otherwise the LJ-off atoms stay off."""
+    p1 = p0.copy()
+    p1[:, 0] = rng.normal(0, 3, p0.shape[0])
+    p1[:, 1] = np.clip(p0[:, 1] + rng.normal(0, 0.02, p0.shape[0]), 0.05, 0.2)
+    p1[:, 2] = np.abs(p0[:, 2] + rng.normal(0, 0.2, p0.shape[0]))
+    p1[p0[:, 2] == 0.0, 2] = 0.0
+    if appear:
+        p1[::5, 2] = rng.uniform(0.2, 1.0, p1[::5].shape[0])
+        p1[::7, 2] = 0.0
+    return np.concatenate([p0, p1])
+
+
+def test_interpolated_endpoints_and_split(rng):
+    """At lambda 0 (1) the interpolated energy is the plain energy of the initial
+    (final) parameters; du_dp splits as ((1 - lambda) g, lambda g) with g the plain
+    du_dp at p(lambda); du_dl gains sum g . (p1 - p0)."""
+    x, p0, box, _, excl, sc, pl, of = _tiny_system(rng, N=20)
+    P = _interp_params(rng, p0)
+    n = len(x)
+    for lam, pend in ((0.0, P[:n]), (1.0, P[n:])):
+        a = nonbonded.nonbonded_pairs(x, P, box, lam, excl, sc, 2.0, 1.2, pl, of, interpolated=True)
+        b = nonbonded.nonbonded_pairs(x, pend, box, lam, excl, sc, 2.0, 1.2, pl, of)
+        assert a[3] == pytest.approx(b[3], rel=1e-13)
+        np.testing.assert_allclose(a[0], b[0], rtol=1e-12, atol=1e-12)
+    lam = 0.37
+    pl_ = (1 - lam) * P[:n] + lam * P[n:]
+    gx, gp, gl, u = nonbonded.nonbonded_pairs(x, P, box, lam, excl, sc, 2.0, 1.2, pl, of, interpolated=True)
+    hx, hp, hl, v = nonbonded.nonbonded_pairs(x, pl_, box, lam, excl, sc, 2.0, 1.2, pl, of)
+    assert gp.shape == (2 * n, 3)
+    assert u == pytest.approx(v, rel=1e-13)
+    np.testing.assert_allclose(gx, hx, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(gp[:n], (1 - lam) * hp, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(gp[n:], lam * hp, rtol=1e-12, atol=1e-12)
+    assert gl == pytest.approx(hl + np.sum(hp * (P[n:] - P[:n])), rel=1e-11, abs=1e-11)
+
+
+def test_interpolated_finite_differences(rng):
+    """du_dl of the interpolated energy (parameters and the 4D coordinate both move
+    with lambda) vs central differences of u in lambda; dense == pair evaluator.
+    LJ-off atoms stay off: where eps_i eps_j crosses 0 the LJ term switches (the
+    reference masks it, nonbonded.py:227) and differences do not apply."""
+    x, p0, box, _, excl, sc, pl, of = _tiny_system(rng, N=16)
+    P = _interp_params(rng, p0, appear=False)
+    h = 1e-6
+    for lam in (0.0, 0.2, 0.7):
+        gx, gp, gl, u = nonbonded.nonbonded_v3(x, P, box, lam, excl, sc, 2.0, 1.2, pl, of, interpolated=True)
+        b = nonbonded.nonbonded_pairs(x, P, box, lam, excl, sc, 2.0, 1.2, pl, of, interpolated=True)
+        assert b[3] == pytest.approx(u, rel=1e-12)
+        np.testing.assert_allclose(b[1], gp, rtol=1e-10, atol=1e-10)
+        up = nonbonded.nonbonded_v3(x, P, box, lam + h, excl, sc, 2.0, 1.2, pl, of, interpolated=True)[3]
+        um = nonbonded.nonbonded_v3(x, P, box, lam - h, excl, sc, 2.0, 1.2, pl, of, interpolated=True)[3]
+        assert gl == pytest.approx((up - um) / (2 * h), rel=1e-5, abs=1e-5)
+        for i, k in [(0, 0), (3, 1), (17, 2), (30, 0)]:
+            pp, pm = P.copy(), P.copy()
+            pp[i, k] += h
+            pm[i, k] -= h
+            fp = nonbonded.nonbonded_v3(x, pp, box, lam, excl, sc, 2.0, 1.2, pl, of, interpolated=True)[3]
+            fm = nonbonded.nonbonded_v3(x, pm, box, lam, excl, sc, 2.0, 1.2, pl, of, interpolated=True)[3]
+            assert gp[i, k] == pytest.approx((fp - fm) / (2 * h), rel=1e-5, abs=1e-5)
+
+
+def test_interpolated_equal_endpoints_have_zero_du_dl(rng):
+    """Known answer of tests/test_nonbonded.py:303-320: with equal initial and final
+    parameters and no lambda offsets, du/dlambda is exactly zero."""
+    x, p0, box, lam, excl, sc, pl, of = _tiny_system(rng, N=20, with_offsets=False)
+    P = np.concatenate([p0, p0])
+    gl = nonbonded.nonbonded_pairs(x, P, box, 0.41, excl, sc, 2.0, 1.2, pl, of, interpolated=True)[2]
+    assert gl == 0.0

@@ -26,10 +26,15 @@ _SIGS = {
     "tm_tile_size": (c_int, []),
     "tm_device_count": (c_int, [POINTER(c_int)]),
     "tm_nonbonded_all_pairs_create": (c_int, [c_int, _PI, _PI, c_double, c_double, _PI, c_int, _PPV]),
+    "tm_nonbonded_all_pairs_interpolated_create": (c_int, [c_int, _PI, _PI, c_double, c_double, _PI, c_int, _PPV]),
     "tm_nonbonded_all_pairs_set_nblist_padding": (c_int, [_P, c_double]),
     "tm_nonbonded_all_pairs_disable_hilbert_sort": (c_int, [_P]),
     "tm_nonbonded_all_pairs_num_tiles": (c_int, [_P, POINTER(c_int)]),
     "tm_nonbonded_pair_list_create": (
+        c_int,
+        [c_int, c_int, _PI, _PD, _PI, _PI, c_double, c_double, c_int, _PPV],
+    ),
+    "tm_nonbonded_pair_list_interpolated_create": (
         c_int,
         [c_int, c_int, _PI, _PD, _PI, _PI, c_double, c_double, c_int, _PPV],
     ),

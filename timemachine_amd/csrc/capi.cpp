@@ -171,6 +171,20 @@ int tm_nonbonded_all_pairs_create(int N, const int32_t *plane, const int32_t *of
     });
 }
 
+int tm_nonbonded_all_pairs_interpolated_create(int N, const int32_t *plane, const int32_t *offset, double beta,
+                                               double cutoff, const int32_t *atom_idxs, int n_atom_idxs,
+                                               tm_potential **out) {
+    return guarded([&] {
+        std::vector<int> pl = to_vec(plane, N), of = to_vec(offset, N);
+        std::vector<int> sub;
+        if (n_atom_idxs >= 0) {
+            sub = to_vec(atom_idxs, n_atom_idxs);
+        }
+        auto p = std::make_shared<NonbondedAllPairs>(pl, of, beta, cutoff, n_atom_idxs >= 0 ? &sub : nullptr, true);
+        *out = new tm_potential{p};
+    });
+}
+
 int tm_nonbonded_all_pairs_set_nblist_padding(tm_potential *p, double padding) {
     return guarded([&] {
         auto nb = std::dynamic_pointer_cast<NonbondedAllPairs>(p->p);
@@ -208,6 +222,18 @@ int tm_nonbonded_pair_list_create(int N, int M, const int32_t *pair_idxs, const 
         std::vector<double> sc = scales && M > 0 ? std::vector<double>(scales, scales + M * 2) : std::vector<double>();
         auto p = std::make_shared<NonbondedPairList>(pairs, sc, to_vec(plane, N), to_vec(offset, N), beta, cutoff,
                                                      negated != 0);
+        *out = new tm_potential{p};
+    });
+}
+
+int tm_nonbonded_pair_list_interpolated_create(int N, int M, const int32_t *pair_idxs, const double *scales,
+                                               const int32_t *plane, const int32_t *offset, double beta, double cutoff,
+                                               int negated, tm_potential **out) {
+    return guarded([&] {
+        std::vector<int> pairs = to_vec(pair_idxs, M * 2);
+        std::vector<double> sc = scales && M > 0 ? std::vector<double>(scales, scales + M * 2) : std::vector<double>();
+        auto p = std::make_shared<NonbondedPairList>(pairs, sc, to_vec(plane, N), to_vec(offset, N), beta, cutoff,
+                                                     negated != 0, true);
         *out = new tm_potential{p};
     });
 }

@@ -406,14 +406,15 @@ struct Engine::Impl {
     // nonbonded
     bool has_nb = false;
     NbState nb;
-    DeviceArray<double> nb_params;
+    DeviceArray<double> nb_params, nb_params1; // nb_params1: final parameters of interpolated systems
     DeviceArray<int> nb_plane, nb_offset;
 
     // pair lists (index 0: plain, 1: negated)
     int pl_n[2] = {0, 0};
     DeviceArray<int> pl_pairs[2], pl_sys[2];
     DeviceArray<double> pl_scales[2];
-    DeviceArray<double> pl_params;
+    DeviceArray<double> pl_params, pl_params1;
+    bool interp = false; // interpolated nonbonded parameters (gradient observer unavailable)
     DeviceArray<int> pl_plane, pl_offset;
     DeviceArray<double> pl_cutoffs;
     DeviceArray<float> pl_betas;
@@ -456,7 +457,8 @@ struct Engine::Impl {
             lamp = lam.ptr;
         }
         if (has_nb) {
-            nb.prepare(xp, nb_params.ptr, boxp, nb_plane.ptr, nb_offset.ptr, lamp, 0.0, st);
+            nb.prepare(xp, nb_params.ptr, nb_params1.n ? nb_params1.ptr : nullptr, boxp, nb_plane.ptr, nb_offset.ptr,
+                       lamp, 0.0, st);
             nb.compute(boxp, ddx, nullptr, ddl, uu, st, tile_grid);
         }
         for (int neg = 0; neg < 2; neg++) {
@@ -489,6 +491,7 @@ struct Engine::Impl {
         a.plane = pl_plane.ptr;
         a.offset = pl_offset.ptr;
         a.lambdas = lam.ptr;
+        a.params1 = pl_params1.n ? pl_params1.ptr : nullptr;
         a.cutoffs = pl_cutoffs.ptr;
         a.betas = pl_betas.ptr;
         a.negated = neg;
@@ -503,7 +506,7 @@ struct Engine::Impl {
         launch_fill_u64(obs_dldp.ptr, Pg, 0, st);
         launch_fill_u64(obs_dl.ptr, S, 0, st);
         if (has_nb) {
-            nb.prepare(x.ptr, nb_params.ptr, box.ptr, nb_plane.ptr, nb_offset.ptr, lam.ptr, 0.0, st);
+            nb.prepare(x.ptr, nb_params.ptr, nb_params1.ptr, box.ptr, nb_plane.ptr, nb_offset.ptr, lam.ptr, 0.0, st);
             nb.compute(box.ptr, nullptr, obs_dudp.ptr, obs_dl.ptr, nullptr, st, tile_grid);
             nb.compute_dl_dp(box.ptr, obs_dldp.ptr, st, tile_grid);
         }
@@ -636,12 +639,22 @@ Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
     if (I.has_nb) {
         std::vector<NbSys> ys(S_);
         std::vector<int> subset;
-        std::vector<double> params(A_ * 3, 0.0);
+        std::vector<double> params(A_ * 3, 0.0), params1;
         std::vector<int> plane(A_, 0), offset(A_, 0);
+        const bool interp0 = flats[0].nb->interp_;
+        if (interp0) {
+            params1.assign(A_ * 3, 0.0);
+        }
         for (int s = 0; s < S_; s++) {
             const auto &nb = flats[s].nb;
             if (nb->N_ != I.natoms[s]) {
                 throw Error("Engine: NonbondedAllPairs N does not match the system");
+            }
+            if (nb->interp_ != interp0) {
+                throw Error("Engine: either every system or none has interpolated nonbonded parameters");
+            }
+            if (flats[s].nb_params.size() != static_cast<size_t>((interp0 ? 6 : 3) * I.natoms[s])) {
+                throw Error("Engine: nonbonded parameter size mismatch");
             }
             NbSys y{};
             y.atom0 = I.atom0[s];
@@ -651,11 +664,17 @@ Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
             y.subset0 = nb->subset_.empty() ? -1 : static_cast<int>(subset.size());
             subset.insert(subset.end(), nb->subset_.begin(), nb->subset_.end());
             y.sorted = nb->sorted_ ? 1 : 0;
+            y.interp = interp0 ? 1 : 0;
             y.beta = static_cast<float>(nb->beta_);
             y.cutoff_d = nb->cutoff_;
             y.padding_d = nb->padding_;
             ys[s] = y;
-            std::copy(flats[s].nb_params.begin(), flats[s].nb_params.end(), params.begin() + I.atom0[s] * 3);
+            const auto &fp = flats[s].nb_params;
+            const size_t n3 = static_cast<size_t>(I.natoms[s]) * 3;
+            std::copy(fp.begin(), fp.begin() + n3, params.begin() + I.atom0[s] * 3);
+            if (interp0) {
+                std::copy(fp.begin() + n3, fp.begin() + 2 * n3, params1.begin() + I.atom0[s] * 3);
+            }
             std::copy(nb->plane_.begin(), nb->plane_.end(), plane.begin() + I.atom0[s]);
             std::copy(nb->offset_.begin(), nb->offset_.end(), offset.begin() + I.atom0[s]);
         }
@@ -663,6 +682,10 @@ Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
         I.nb.init(ys, subset);
         h_nb_params = params;
         upload_vec(I.nb_params, params);
+        if (interp0) {
+            upload_vec(I.nb_params1, params1);
+            I.interp = true;
+        }
         upload_vec(I.nb_plane, plane);
         upload_vec(I.nb_offset, offset);
     }
@@ -670,9 +693,10 @@ Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
     {
         std::vector<int> pairs[2], psys[2];
         std::vector<double> scales[2];
-        std::vector<double> params(A_ * 3, 0.0), cutoffs(S_, 1.0);
+        std::vector<double> params(A_ * 3, 0.0), params1, cutoffs(S_, 1.0);
         std::vector<float> betas(S_, 0.0f);
         std::vector<int> plane(A_, 0), offset(A_, 0);
+        int pl_interp = -1; // unknown, 0 or 1: every pair list of the batch must agree
         for (int s = 0; s < S_; s++) {
             const std::vector<double> *first = nullptr;
             for (auto &pr : flats[s].pls) {
@@ -680,9 +704,24 @@ Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
                 if (pl->N_ != I.natoms[s]) {
                     throw Error("Engine: NonbondedPairList N does not match the system");
                 }
+                if (pl_interp < 0) {
+                    pl_interp = pl->interp_ ? 1 : 0;
+                    if (pl_interp) {
+                        params1.assign(A_ * 3, 0.0);
+                    }
+                } else if (pl_interp != (pl->interp_ ? 1 : 0)) {
+                    throw Error("Engine: either every pair list or none has interpolated parameters");
+                }
+                const size_t n3 = static_cast<size_t>(I.natoms[s]) * 3;
+                if (pr.second.size() != (pl_interp ? 2 : 1) * n3) {
+                    throw Error("Engine: pair-list parameter size mismatch");
+                }
                 if (!first) {
                     first = &pr.second;
-                    std::copy(pr.second.begin(), pr.second.end(), params.begin() + I.atom0[s] * 3);
+                    std::copy(pr.second.begin(), pr.second.begin() + n3, params.begin() + I.atom0[s] * 3);
+                    if (pl_interp) {
+                        std::copy(pr.second.begin() + n3, pr.second.begin() + 2 * n3, params1.begin() + I.atom0[s] * 3);
+                    }
                     std::copy(pl->plane_.begin(), pl->plane_.end(), plane.begin() + I.atom0[s]);
                     std::copy(pl->offset_.begin(), pl->offset_.end(), offset.begin() + I.atom0[s]);
                     cutoffs[s] = pl->cutoff_;
@@ -723,6 +762,10 @@ Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
             }
         }
         upload_vec(I.pl_params, params);
+        if (pl_interp == 1) {
+            upload_vec(I.pl_params1, params1);
+            I.interp = true;
+        }
         upload_vec(I.pl_plane, plane);
         upload_vec(I.pl_offset, offset);
         upload_vec(I.pl_cutoffs, cutoffs);
@@ -1106,6 +1149,9 @@ void Engine::set_grad_observer(int interval) {
     if (interval > 0 && !I.pl_shares_nb) {
         throw Error("Engine::set_grad_observer: pair lists must share the nonbonded parameters");
     }
+    if (interval > 0 && I.interp) {
+        throw Error("Engine::set_grad_observer: not available for interpolated nonbonded parameters");
+    }
     if (I.Pg == 0) {
         // flat parameter space: nonbonded [A, 3], then bond [B, 2], angle [A, 2], torsion [T, 3]
         static const int np[3] = {2, 2, 3};
@@ -1316,7 +1362,8 @@ void Engine::profile(int reps, double *ms_out) {
     if (I.has_nb) {
         ms_out[0] = timed([&] {
             I.nb.invalidate(stream_);
-            I.nb.prepare(I.x.ptr, I.nb_params.ptr, I.box.ptr, I.nb_plane.ptr, I.nb_offset.ptr, I.lam.ptr, 0.0, stream_);
+            I.nb.prepare(I.x.ptr, I.nb_params.ptr, I.nb_params1.ptr, I.box.ptr, I.nb_plane.ptr, I.nb_offset.ptr,
+                         I.lam.ptr, 0.0, stream_);
         });
         ms_out[1] = timed([&] { I.nb.compute(I.box.ptr, I.du_dx.ptr, nullptr, nullptr, nullptr, stream_, tile_grid); });
     }

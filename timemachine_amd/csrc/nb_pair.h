@@ -150,6 +150,23 @@ TM_HD inline PairTerms nb_pair_terms(
     return o;
 }
 
+// du/dlambda of one pair: the 4D term dw * prefactor * (dw/dl_i - dw/dl_j)
+// and, with interpolated parameters (dp/dl = p1 - p0), the parameter terms
+// (k_nonbonded.cuh:464-507): du/dq_i dq_i/dl + ... for q, sig and eps.
+// T must carry sig_grad / eps_grad (COMPUTE_DP) when interp is true.
+TM_HD inline double pair_du_dl(const PairTerms &T, double dw, double dwdl_i, double dwdl_j, bool interp,
+                               double scale_q, double qi, double qj, double ei, double ej, double dqi, double dqj,
+                               double dsi, double dsj, double dei, double dej) {
+    double dl = dw * T.prefactor * (dwdl_i - dwdl_j);
+    if (interp) {
+        dl = dl + T.erfc_inv_d * ((scale_q * qj) * dqi + (scale_q * qi) * dqj);
+        if (T.lj) {
+            dl = dl + T.sig_grad * (dsi + dsj) + T.eps_grad * (ej * dei + ei * dej);
+        }
+    }
+    return dl;
+}
+
 // Parameter derivatives of the pair's force prefactor, for d(du/dlambda)/dp.
 // A pair contributes prefactor * dw * (dw/dl_i - dw/dl_j) to du/dlambda, and
 // the prefactor is linear in q_i q_j (Coulomb) and in eps_i eps_j (LJ):

@@ -44,7 +44,7 @@ struct NbSys {
     int red_idx;  // index of the system in u / du_dl reduction buffers
     int subset0;  // offset into the subset atom list, -1 when all atoms take part
     int sorted;   // 1: spatial (Hilbert) sort on rebuild, 0: identity order
-    int pad0;
+    int interp;   // 1: parameters interpolated between two sets, p = (1 - lambda) p0 + lambda p1
     float beta;
     float cutoff;
     float nb_cutoff; // neighborlist radius, FP32 predicate (cutoff + padding, plus a margin for potentials)
@@ -86,6 +86,8 @@ struct NbState {
     DeviceArray<uint32_t> keys;   // [total_slots] sort keys (scratch)
     DeviceArray<double4> pos;     // [total_slots] x, y, z, w (FP64)
     DeviceArray<float4> par;      // [total_slots]
+    DeviceArray<float4> dpar;     // [total_slots] dq/dl, dsig/dl, deps/dl (interpolated systems only)
+    bool any_interp = false;
     DeviceArray<float4> bb_ctr;   // [total_blocks]
     DeviceArray<float4> bb_ext;   // [total_blocks]
     DeviceArray<double> nblist_x; // [sum n_atoms * 3] coordinates at the last build
@@ -105,12 +107,14 @@ struct NbState {
     // neighborlist and tile prefix. No host synchronisation.
     //   x      [*, 3] f64, rows atom0 + i
     //   params [*, 3] f64 (q, sig/2, sqrt(eps)), rows atom0 + i
+    //   params1 [*, 3] final parameters of interpolated systems (else unused, may be nullptr)
     //   plane, offset [*] int32 per atom
     //   boxes  [S, 9] f64
     //   lambdas [S] f64 or nullptr (then lambda0 for every system)
     void prepare(
         const double *x,
         const double *params,
+        const double *params1,
         const double *boxes,
         const int *plane,
         const int *offset,
@@ -152,6 +156,7 @@ struct PairListArgs {
     const int *offset;
     const double *lambdas; // [S] or nullptr
     double lambda0;
+    const double *params1; // final parameters when interpolated (p = (1 - lambda) p0 + lambda p1), else nullptr
     const double *cutoffs; // [S] or nullptr
     double cutoff0;
     const float *betas;    // [S] or nullptr
@@ -165,6 +170,11 @@ void launch_pair_list(
 void launch_pair_list_dl_dp(const PairListArgs &a, uint64_t *dl_dp, hipStream_t stream);
 
 // Helpers shared by several translation units.
+// du/dp of interpolated parameters: src [N, 3] is du/dp at p(lambda); adds
+// (1 - lambda) src to dst rows [0, N) and lambda src to rows [N, 2N), per
+// column in its fixed-point exponent (q 2^36, sig 2^37, eps 2^38).
+void launch_split_interpolated_du_dp(int N, double lambda, const uint64_t *src, uint64_t *dst, hipStream_t stream);
+
 void launch_fill_u64(uint64_t *p, size_t n, uint64_t v, hipStream_t stream);
 void launch_fill_i32(int *p, size_t n, int v, hipStream_t stream);
 

@@ -286,13 +286,15 @@ __global__ void k_gather(
     const NbSys *__restrict__ sys,
     const double *__restrict__ x,
     const double *__restrict__ params,
+    const double *__restrict__ params1,
     const int *__restrict__ plane,
     const int *__restrict__ offset,
     const double *__restrict__ lambdas,
     const double lambda0,
     const uint32_t *__restrict__ perm,
     double4 *__restrict__ pos,
-    float4 *__restrict__ par) {
+    float4 *__restrict__ par,
+    float4 *__restrict__ dpar) {
     const int s = blockIdx.y;
     const NbSys S = sys[s];
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
@@ -303,6 +305,9 @@ __global__ void k_gather(
     if (atom == NB_INVALID) {
         pos[S.slot0 + slot] = make_double4(0.0, 0.0, 0.0, 0.0);
         par[S.slot0 + slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (dpar) {
+            dpar[S.slot0 + slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
         return;
     }
     const int g = S.atom0 + static_cast<int>(atom);
@@ -313,9 +318,42 @@ __global__ void k_gather(
     const double dw_dl = of * S.cutoff_d;
     pos[S.slot0 + slot] = make_double4(x[g * 3 + 0], x[g * 3 + 1], x[g * 3 + 2], w);
     if (params) {
-        par[S.slot0 + slot] = make_float4(static_cast<float>(params[g * 3 + 0]), static_cast<float>(params[g * 3 + 1]),
-                                          static_cast<float>(params[g * 3 + 2]), static_cast<float>(dw_dl));
+        if (S.interp && params1) {
+            // parameter interpolation with the identity transform
+            // (k_lambda_transformer_jit.cuh:57-98): p = (1 - lambda) p0 + lambda p1, dp/dl = p1 - p0
+            double p[3], d[3];
+            for (int c = 0; c < 3; c++) {
+                const double a = params[g * 3 + c], b = params1[g * 3 + c];
+                p[c] = (1.0 - lam) * a + lam * b;
+                d[c] = b - a;
+            }
+            par[S.slot0 + slot] = make_float4(static_cast<float>(p[0]), static_cast<float>(p[1]),
+                                              static_cast<float>(p[2]), static_cast<float>(dw_dl));
+            dpar[S.slot0 + slot] =
+                make_float4(static_cast<float>(d[0]), static_cast<float>(d[1]), static_cast<float>(d[2]), 0.f);
+        } else {
+            par[S.slot0 + slot] =
+                make_float4(static_cast<float>(params[g * 3 + 0]), static_cast<float>(params[g * 3 + 1]),
+                            static_cast<float>(params[g * 3 + 2]), static_cast<float>(dw_dl));
+            if (dpar) {
+                dpar[S.slot0 + slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
     }
+}
+
+// du/dp split of interpolated parameters (see nb_state.h), grid (ceil(N / 256), 3).
+__global__ void k_split_interpolated_du_dp(int N, double lambda, const uint64_t *__restrict__ src,
+                                           uint64_t *__restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= N) {
+        return;
+    }
+    const double scale = c == 0 ? FIXED_SCALE : (c == 1 ? FIXED_SCALE_DSIG : FIXED_SCALE_DEPS);
+    const double g = fixed_to_double(src[i * 3 + c], scale);
+    dst[i * 3 + c] += f64_to_fixed((1.0 - lambda) * g, scale);
+    dst[(N + i) * 3 + c] += f64_to_fixed(lambda * g, scale);
 }
 
 // grid (ceil(max_blocks / 4), S), 256 threads: one wave per 64-atom block.
@@ -700,6 +738,15 @@ void NbState::init(const std::vector<NbSys> &sys_in, const std::vector<int> &sub
     keys.alloc(total_slots);
     pos.alloc(total_slots);
     par.alloc(total_slots);
+    any_interp = false;
+    for (const auto &y : h_sys) {
+        any_interp = any_interp || y.interp != 0;
+    }
+    if (any_interp) {
+        dpar.alloc(total_slots);
+    } else {
+        dpar.release();
+    }
     bb_ctr.alloc(total_blocks);
     bb_ext.alloc(total_blocks);
     nblist_x.alloc(static_cast<size_t>(total_atoms_end) * 3);
@@ -738,7 +785,7 @@ void NbState::build_only(const double *x, const double *boxes, hipStream_t strea
                                           nblist_box.ptr, tile_count.ptr);
     TM_LAUNCH_CHECK();
     k_gather<<<dim3(ceil_div(max_slots, 256), S), 256, 0, stream>>>(sys.ptr, x, nullptr, nullptr, nullptr, nullptr,
-                                                                    0.0, perm.ptr, pos.ptr, par.ptr);
+                                                                    nullptr, 0.0, perm.ptr, pos.ptr, par.ptr, nullptr);
     TM_LAUNCH_CHECK();
     k_block_bounds<<<dim3(ceil_div(max_blocks, 4), S), 256, 0, stream>>>(sys.ptr, boxes, rebuild.ptr, pos.ptr,
                                                                          bb_ctr.ptr, bb_ext.ptr);
@@ -751,9 +798,18 @@ void NbState::build_only(const double *x, const double *boxes, hipStream_t strea
     TM_LAUNCH_CHECK();
 }
 
+void launch_split_interpolated_du_dp(int N, double lambda, const uint64_t *src, uint64_t *dst, hipStream_t stream) {
+    if (N <= 0) {
+        return;
+    }
+    k_split_interpolated_du_dp<<<dim3(ceil_div(N, 256), 3), 256, 0, stream>>>(N, lambda, src, dst);
+    TM_LAUNCH_CHECK();
+}
+
 void NbState::prepare(
     const double *x,
     const double *params,
+    const double *params1,
     const double *boxes,
     const int *plane,
     const int *offset,
@@ -767,8 +823,9 @@ void NbState::prepare(
     k_sort<<<S, SORT_THREADS, 0, stream>>>(sys.ptr, subset.ptr, x, boxes, rebuild.ptr, keys.ptr, perm.ptr, nblist_x.ptr,
                                           nblist_box.ptr, tile_count.ptr);
     TM_LAUNCH_CHECK();
-    k_gather<<<dim3(ceil_div(max_slots, 256), S), 256, 0, stream>>>(sys.ptr, x, params, plane, offset, lambdas,
-                                                                    lambda0, perm.ptr, pos.ptr, par.ptr);
+    k_gather<<<dim3(ceil_div(max_slots, 256), S), 256, 0, stream>>>(
+        sys.ptr, x, params, params1, plane, offset, lambdas, lambda0, perm.ptr, pos.ptr, par.ptr,
+        any_interp ? dpar.ptr : nullptr);
     TM_LAUNCH_CHECK();
     k_block_bounds<<<dim3(ceil_div(max_blocks, 4), S), 256, 0, stream>>>(sys.ptr, boxes, rebuild.ptr, pos.ptr,
                                                                          bb_ctr.ptr, bb_ext.ptr);

@@ -41,14 +41,21 @@ static_assert((PAIR_RING & (PAIR_RING - 1)) == 0 && TILE % PASS1_UNROLL == 0, "r
 
 // Per-wave LDS of the tile kernel. NACC accumulators per atom: x, y, z
 // forces, then q, sig, eps parameter derivatives when requested.
+// dq/dl, dsig/dl, deps/dl of row and column atoms (interpolated parameters only)
+template <bool INTERP> struct TileDp {
+    float v[2][3][TILE];
+};
+template <> struct TileDp<false> {};
+
 // Structure of arrays: consecutive lanes read consecutive 8- or 4-byte words,
 // which keeps LDS bank conflicts away (a 32-byte record per lane did not).
-template <int NACC> struct TileLds {
+template <int NACC, bool INTERP> struct TileLds {
     double pos[2][4][TILE];      // [row, column][x, y, z, w][lane / slot]
     float par[2][4][TILE];       // [row, column][q, sig, eps, dw/dlambda][lane / slot]
     uint32_t colidx[TILE];       // column sorted index (NB_INVALID for padding)
     uint32_t pij[PAIR_RING];     // queued pair: row lane | column slot << 8
     uint64_t acc[2][NACC][TILE]; // [row, column][component][lane / slot]
+    TileDp<INTERP> dp;
 
     __device__ __forceinline__ void put(int side, int l, const double4 &p, const float4 &a) {
         pos[side][0][l] = p.x;
@@ -100,10 +107,12 @@ struct BoxDev {
 // the pair's row and column atom (u and du/dl to per-lane registers).
 // DLDP: instead of du/dp, accumulate d(du/dlambda)/dp into the du_dp slots
 // (same fixed-point exponents); only alchemical tiles contribute.
-template <bool ALCH, bool U, bool DX, bool DL, bool DP, bool DLDP, int NACC>
-__device__ __forceinline__ void eval_pairs(TileLds<NACC> &L, const int base, const int n, const BoxDev &B,
+template <bool ALCH, bool U, bool DX, bool DL, bool DP, bool DLDP, int NACC, bool INTERP>
+__device__ __forceinline__ void eval_pairs(TileLds<NACC, INTERP> &L, const int base, const int n, const BoxDev &B,
                                            const double beta, uint64_t &uacc, uint64_t &dlacc) {
     constexpr int PQ = DX ? 3 : 0; // first parameter-derivative component
+    // interpolated parameters add du/dp . dp/dl to du/dl, so the LJ parameter gradients are needed
+    constexpr bool NEED_DP = DP || (INTERP && ALCH && DL);
     const int lane = threadIdx.x & 63;
     if (lane < n) {
         const uint32_t ij = L.pij[(base + lane) & (PAIR_RING - 1)];
@@ -115,7 +124,7 @@ __device__ __forceinline__ void eval_pairs(TileLds<NACC> &L, const int base, con
         const float4 aj = L.get_par(1, sl);
         double dx, dy, dz, dw;
         const double d2 = pair_delta<ALCH>(pi, pj, B.bx, B.by, B.bz, B.ibx, B.iby, B.ibz, dx, dy, dz, dw);
-        const PairTerms T = nb_pair_terms<U, DP>(d2, ai.x, aj.x, ai.y, aj.y, ai.z, aj.z, beta, 1.0, 1.0);
+        const PairTerms T = nb_pair_terms<U, NEED_DP>(d2, ai.x, aj.x, ai.y, aj.y, ai.z, aj.z, beta, 1.0, 1.0);
         if (DX) {
             const uint64_t fx = f64_to_fixed(T.prefactor * dx, FIXED_SCALE);
             const uint64_t fy = f64_to_fixed(T.prefactor * dy, FIXED_SCALE);
@@ -139,9 +148,17 @@ __device__ __forceinline__ void eval_pairs(TileLds<NACC> &L, const int base, con
             }
         }
         if (ALCH && (DL || DLDP)) {
-            if (DL) {
-                dlacc += f64_to_fixed(dw * T.prefactor * (static_cast<double>(ai.w) - static_cast<double>(aj.w)),
-                                      FIXED_SCALE);
+            if constexpr (DL) {
+                if constexpr (INTERP) {
+                    dlacc += f64_to_fixed(
+                        pair_du_dl(T, dw, ai.w, aj.w, true, 1.0, ai.x, aj.x, ai.z, aj.z, L.dp.v[0][0][il],
+                                   L.dp.v[1][0][sl], L.dp.v[0][1][il], L.dp.v[1][1][sl], L.dp.v[0][2][il],
+                                   L.dp.v[1][2][sl]),
+                        FIXED_SCALE);
+                } else {
+                    dlacc += f64_to_fixed(
+                        dw * T.prefactor * (static_cast<double>(ai.w) - static_cast<double>(aj.w)), FIXED_SCALE);
+                }
             }
             if (DLDP) {
                 const PairDlDp D = nb_pair_dl_dp(d2, ai.y, aj.y, ai.z, aj.z, beta, 1.0);
@@ -175,9 +192,9 @@ __device__ __forceinline__ void eval_pairs(TileLds<NACC> &L, const int base, con
 // recomputes their displacements and evaluates them with every lane busy.
 // Fixed-point sums are exact, so the order of evaluation does not change
 // any output bit.
-template <bool ALCH, bool U, bool DX, bool DL, bool DP, bool DLDP, int NACC>
+template <bool ALCH, bool U, bool DX, bool DL, bool DP, bool DLDP, int NACC, bool INTERP>
 __device__ __forceinline__ void tile_body(
-    TileLds<NACC> &L,
+    TileLds<NACC, INTERP> &L,
     const NbSys &S,
     const int s,
     const int r,
@@ -186,6 +203,8 @@ __device__ __forceinline__ void tile_body(
     const float4 ai,
     const double4 pj0,
     const float4 aj0,
+    const float4 dpi,
+    const float4 dpj,
     const uint32_t *__restrict__ perm,
     const double *__restrict__ boxes,
     uint64_t *__restrict__ du_dx,
@@ -212,6 +231,14 @@ __device__ __forceinline__ void tile_body(
     L.put(0, lane, pi, ai);
     L.put(1, lane, pj0, aj0);
     L.colidx[lane] = jl0;
+    if constexpr (INTERP) {
+        L.dp.v[0][0][lane] = dpi.x;
+        L.dp.v[0][1][lane] = dpi.y;
+        L.dp.v[0][2][lane] = dpi.z;
+        L.dp.v[1][0][lane] = dpj.x;
+        L.dp.v[1][1][lane] = dpj.y;
+        L.dp.v[1][2][lane] = dpj.z;
+    }
 #pragma unroll
     for (int c = 0; c < NACC; c++) {
         L.acc[0][c][lane] = 0;
@@ -251,14 +278,14 @@ __device__ __forceinline__ void tile_body(
         }
         while (queued - done >= TILE) {
             wave_lds_sync();
-            eval_pairs<ALCH, U, DX, DL, DP, DLDP, NACC>(L, done, TILE, B, beta, uacc, dlacc);
+            eval_pairs<ALCH, U, DX, DL, DP, DLDP, NACC, INTERP>(L, done, TILE, B, beta, uacc, dlacc);
             done += TILE;
             wave_lds_sync(); // later rounds reuse these ring entries
         }
     }
     if (queued > done) {
         wave_lds_sync();
-        eval_pairs<ALCH, U, DX, DL, DP, DLDP, NACC>(L, done, queued - done, B, beta, uacc, dlacc);
+        eval_pairs<ALCH, U, DX, DL, DP, DLDP, NACC, INTERP>(L, done, queued - done, B, beta, uacc, dlacc);
     }
     wave_lds_sync();
 
@@ -323,7 +350,7 @@ __device__ __forceinline__ void tile_body(
 
 // Grid-stride over all tiles of all systems (the grid is sized to the
 // machine, the tile count is read on the device).
-template <bool U, bool DX, bool DL, bool DP, bool DLDP = false>
+template <bool U, bool DX, bool DL, bool DP, bool DLDP = false, bool INTERP = false>
 __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded_tiles(
     const NbSys *__restrict__ sys,
     const int S,
@@ -333,6 +360,7 @@ __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded_tiles(
     const uint32_t *__restrict__ perm,
     const double4 *__restrict__ pos,
     const float4 *__restrict__ par,
+    const float4 *__restrict__ dpar,
     const double *__restrict__ boxes,
     const int *__restrict__ status,
     uint64_t *__restrict__ du_dx,
@@ -341,10 +369,10 @@ __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded_tiles(
     uint64_t *__restrict__ u) {
     constexpr int NACC = (DX ? 3 : 0) + ((DP || DLDP) ? 3 : 0) > 0 ? (DX ? 3 : 0) + ((DP || DLDP) ? 3 : 0) : 1;
     constexpr int WAVES = NB_BLOCK / 64;
-    __shared__ TileLds<NACC> lds[WAVES];
+    __shared__ TileLds<NACC, INTERP> lds[WAVES];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    TileLds<NACC> &L = lds[wave];
+    TileLds<NACC, INTERP> &L = lds[wave];
     const int first = uniform(blockIdx.x * WAVES + wave);
     const int stride = gridDim.x * WAVES;
     const int total = tile_prefix[S];
@@ -372,20 +400,31 @@ __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded_tiles(
         const bool vj = jl < static_cast<uint32_t>(Sd.k);
         const double4 pj = vj ? pos[Sd.slot0 + jl] : make_double4(0.0, 0.0, 0.0, 0.0);
         const float4 aj = vj ? par[Sd.slot0 + jl] : make_float4(0.f, 0.f, 0.f, 0.f);
-        const bool alch = ballot(pi.w != 0.0 || pj.w != 0.0) != 0;
+        float4 dpi = make_float4(0.f, 0.f, 0.f, 0.f), dpj = dpi;
+        bool moving = pi.w != 0.0 || pj.w != 0.0;
+        if (INTERP && Sd.interp) {
+            dpi = dpar[Sd.slot0 + i];
+            if (vj) {
+                dpj = dpar[Sd.slot0 + jl];
+            }
+            moving = moving || dpi.x != 0.f || dpi.y != 0.f || dpi.z != 0.f || dpj.x != 0.f || dpj.y != 0.f ||
+                     dpj.z != 0.f;
+        }
+        // "vanilla" tiles (no lambda dependence, k_nonbonded.cuh:626-630) skip the 4D terms
+        const bool alch = ballot(moving) != 0;
         if (alch) {
-            tile_body<true, U, DX, DL, DP, DLDP, NACC>(L, Sd, s, r, jl, pi, ai, pj, aj, perm, boxes, du_dx, du_dp,
-                                                       du_dl, u);
+            tile_body<true, U, DX, DL, DP, DLDP, NACC, INTERP>(L, Sd, s, r, jl, pi, ai, pj, aj, dpi, dpj, perm, boxes,
+                                                               du_dx, du_dp, du_dl, u);
         } else if (!DLDP) {
-            tile_body<false, U, DX, DL, DP, false, NACC>(L, Sd, s, r, jl, pi, ai, pj, aj, perm, boxes, du_dx, du_dp,
-                                                         du_dl, u);
+            tile_body<false, U, DX, DL, DP, false, NACC, INTERP>(L, Sd, s, r, jl, pi, ai, pj, aj, dpi, dpj, perm,
+                                                                 boxes, du_dx, du_dp, du_dl, u);
         }
     }
 }
 
 using TileKernel = void (*)(const NbSys *, int, const int *, const int *, const uint32_t *, const uint32_t *,
-                            const double4 *, const float4 *, const double *, const int *, uint64_t *, uint64_t *,
-                            uint64_t *, uint64_t *);
+                            const double4 *, const float4 *, const float4 *, const double *, const int *, uint64_t *,
+                            uint64_t *, uint64_t *, uint64_t *);
 
 // index bits: 8 = u, 4 = du_dx, 2 = du_dl, 1 = du_dp
 const TileKernel tile_kernels[16] = {
@@ -395,6 +434,17 @@ const TileKernel tile_kernels[16] = {
     k_nonbonded_tiles<1, 0, 0, 1>, k_nonbonded_tiles<1, 0, 1, 0>, k_nonbonded_tiles<1, 0, 1, 1>,
     k_nonbonded_tiles<1, 1, 0, 0>, k_nonbonded_tiles<1, 1, 0, 1>, k_nonbonded_tiles<1, 1, 1, 0>,
     k_nonbonded_tiles<1, 1, 1, 1>,
+};
+
+// interpolated parameters: only the outputs that see dp/dl (du_dl) differ, but
+// the whole table is instantiated for uniform dispatch
+const TileKernel tile_kernels_interp[16] = {
+    k_nonbonded_tiles<0, 0, 0, 0, 0, 1>, k_nonbonded_tiles<0, 0, 0, 1, 0, 1>, k_nonbonded_tiles<0, 0, 1, 0, 0, 1>,
+    k_nonbonded_tiles<0, 0, 1, 1, 0, 1>, k_nonbonded_tiles<0, 1, 0, 0, 0, 1>, k_nonbonded_tiles<0, 1, 0, 1, 0, 1>,
+    k_nonbonded_tiles<0, 1, 1, 0, 0, 1>, k_nonbonded_tiles<0, 1, 1, 1, 0, 1>, k_nonbonded_tiles<1, 0, 0, 0, 0, 1>,
+    k_nonbonded_tiles<1, 0, 0, 1, 0, 1>, k_nonbonded_tiles<1, 0, 1, 0, 0, 1>, k_nonbonded_tiles<1, 0, 1, 1, 0, 1>,
+    k_nonbonded_tiles<1, 1, 0, 0, 0, 1>, k_nonbonded_tiles<1, 1, 0, 1, 0, 1>, k_nonbonded_tiles<1, 1, 1, 0, 0, 1>,
+    k_nonbonded_tiles<1, 1, 1, 1, 0, 1>,
 };
 
 template <bool U, bool DX, bool DL, bool DP, bool DLDP = false>
@@ -435,13 +485,32 @@ __global__ __launch_bounds__(256) void k_pair_list(
         const double dw = wi - wj;
         const double d2 = __builtin_fma(dw, dw, pair_d2(dx, dy, dz));
         if (d2 < cutoff_d * cutoff_d) {
-            const double qi = static_cast<float>(a.params[gi * 3 + 0]), qj = static_cast<float>(a.params[gj * 3 + 0]);
-            const double si = static_cast<float>(a.params[gi * 3 + 1]), sj = static_cast<float>(a.params[gj * 3 + 1]);
-            const double ei = static_cast<float>(a.params[gi * 3 + 2]), ej = static_cast<float>(a.params[gj * 3 + 2]);
+            // parameters as the tile kernel sees them: FP32 values of p, or of
+            // p = (1 - lambda) p0 + lambda p1 when interpolated (k_gather)
+            double pv[2][3], dp[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+            const int ga[2] = {gi, gj};
+            for (int t = 0; t < 2; t++) {
+                for (int c = 0; c < 3; c++) {
+                    const double p0 = a.params[ga[t] * 3 + c];
+                    if (a.params1) {
+                        const double p1 = a.params1[ga[t] * 3 + c];
+                        pv[t][c] = static_cast<float>((1.0 - lam) * p0 + lam * p1);
+                        dp[t][c] = static_cast<float>(p1 - p0);
+                    } else {
+                        pv[t][c] = static_cast<float>(p0);
+                    }
+                }
+            }
+            const double qi = pv[0][0], qj = pv[1][0];
+            const double si = pv[0][1], sj = pv[1][1];
+            const double ei = pv[0][2], ej = pv[1][2];
+            const bool interp = a.params1 != nullptr;
             const double scale_q = a.scales[p * 2 + 0];
             const double scale_lj = a.scales[p * 2 + 1];
-            const PairTerms T =
-                nb_pair_terms<U, DP>(d2, qi, qj, si, sj, ei, ej, static_cast<double>(beta), scale_q, scale_lj);
+            const PairTerms T = (DL && interp) ? nb_pair_terms<U, true>(d2, qi, qj, si, sj, ei, ej,
+                                                                      static_cast<double>(beta), scale_q, scale_lj)
+                                               : nb_pair_terms<U, DP>(d2, qi, qj, si, sj, ei, ej,
+                                                                      static_cast<double>(beta), scale_q, scale_lj);
             auto acc = [neg](uint64_t *dst, uint64_t v) {
                 if (v != 0) {
                     atomic_add_u64(dst, neg ? (0ull - v) : v);
@@ -469,11 +538,17 @@ __global__ __launch_bounds__(256) void k_pair_list(
                     acc(du_dp + gj * 3 + 2, f64_to_fixed(T.eps_grad * ei, FIXED_SCALE_DEPS));
                 }
             }
-            const bool vanilla = (wi == 0.0) && (wj == 0.0);
+            const bool vanilla = (wi == 0.0) && (wj == 0.0) && !interp;
             if (DL && !vanilla) {
-                fdl = f64_to_fixed(dw * T.prefactor * (dwdl_i - dwdl_j), FIXED_SCALE);
+                if (interp) {
+                    fdl = f64_to_fixed(pair_du_dl(T, dw, dwdl_i, dwdl_j, true, scale_q, qi, qj, ei, ej, dp[0][0],
+                                                  dp[1][0], dp[0][1], dp[1][1], dp[0][2], dp[1][2]),
+                                       FIXED_SCALE);
+                } else {
+                    fdl = f64_to_fixed(dw * T.prefactor * (dwdl_i - dwdl_j), FIXED_SCALE);
+                }
             }
-            if (DLDP && !vanilla) {
+            if (DLDP && !vanilla && !interp) {
                 const PairDlDp D = nb_pair_dl_dp(d2, si, sj, ei, ej, static_cast<double>(beta), scale_lj);
                 const double dd = dw * (dwdl_i - dwdl_j);
                 acc(du_dp + gi * 3 + 0, f64_to_fixed(((scale_q * qj) * D.es_unit) * dd, FIXED_SCALE));
@@ -545,9 +620,10 @@ void NbState::compute(
         return;
     }
     const int grid = grid_override > 0 ? grid_override : default_tile_grid();
-    tile_kernels[idx]<<<grid, NB_BLOCK, 0, stream>>>(sys.ptr, S, tile_prefix.ptr, tile_row.ptr, tile_atoms.ptr,
-                                                      perm.ptr, pos.ptr, par.ptr, boxes, status.ptr, du_dx, du_dp,
-                                                      du_dl, u);
+    const TileKernel kern = any_interp ? tile_kernels_interp[idx] : tile_kernels[idx];
+    kern<<<grid, NB_BLOCK, 0, stream>>>(sys.ptr, S, tile_prefix.ptr, tile_row.ptr, tile_atoms.ptr, perm.ptr, pos.ptr,
+                                        par.ptr, any_interp ? dpar.ptr : nullptr, boxes, status.ptr, du_dx, du_dp,
+                                        du_dl, u);
     TM_LAUNCH_CHECK();
 }
 
@@ -563,15 +639,21 @@ void launch_pair_list(
 
 void NbState::compute_dl_dp(const double *boxes, uint64_t *dl_dp, hipStream_t stream, int grid_override) const {
     const int grid = grid_override > 0 ? grid_override : default_tile_grid();
+    if (any_interp) {
+        throw Error("d(du/dlambda)/dp is not available for interpolated parameters");
+    }
     k_nonbonded_tiles<false, false, false, false, true><<<grid, NB_BLOCK, 0, stream>>>(
-        sys.ptr, S, tile_prefix.ptr, tile_row.ptr, tile_atoms.ptr, perm.ptr, pos.ptr, par.ptr, boxes, status.ptr,
-        nullptr, dl_dp, nullptr, nullptr);
+        sys.ptr, S, tile_prefix.ptr, tile_row.ptr, tile_atoms.ptr, perm.ptr, pos.ptr, par.ptr, nullptr, boxes,
+        status.ptr, nullptr, dl_dp, nullptr, nullptr);
     TM_LAUNCH_CHECK();
 }
 
 void launch_pair_list_dl_dp(const PairListArgs &a, uint64_t *dl_dp, hipStream_t stream) {
     if (a.M == 0) {
         return;
+    }
+    if (a.params1) {
+        throw Error("d(du/dlambda)/dp is not available for interpolated parameters");
     }
     k_pair_list<false, false, false, false, true><<<ceil_div(a.M, 256), 256, 0, stream>>>(a, nullptr, dl_dp, nullptr,
                                                                                           nullptr);

@@ -86,8 +86,10 @@ using PotentialPtr = std::shared_ptr<Potential>;
 
 class NonbondedAllPairs : public Potential {
   public:
+    // interpolated: params are [2N, 3], initial then final, used as
+    // p = (1 - lambda) p0 + lambda p1 (nonbonded_all_pairs.cu:340-351)
     NonbondedAllPairs(const std::vector<int> &lambda_plane_idxs, const std::vector<int> &lambda_offset_idxs, double beta,
-                      double cutoff, const std::vector<int> *atom_idxs);
+                      double cutoff, const std::vector<int> *atom_idxs, bool interpolated = false);
     PotKind kind() const override { return PotKind::NonbondedAllPairs; }
     void execute_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
                         uint64_t *d_du_dx, uint64_t *d_du_dp, uint64_t *d_du_dl, uint64_t *d_u,
@@ -106,6 +108,7 @@ class NonbondedAllPairs : public Potential {
     int N_, K_;
     double beta_, cutoff_, padding_ = 0.1;
     bool sorted_ = true;
+    bool interp_ = false;
     std::vector<int> plane_, offset_, subset_;
     NbState st_;
 
@@ -114,13 +117,14 @@ class NonbondedAllPairs : public Potential {
     void validate_sizes(int N, int P) const;
     bool dirty_ = true;
     DeviceArray<int> d_plane_, d_offset_;
+    DeviceArray<uint64_t> dp_tmp_; // du/dp at the interpolated parameters [N, 3]
 };
 
 class NonbondedPairList : public Potential {
   public:
     NonbondedPairList(const std::vector<int> &pair_idxs, const std::vector<double> &scales,
                       const std::vector<int> &lambda_plane_idxs, const std::vector<int> &lambda_offset_idxs,
-                      double beta, double cutoff, bool negated);
+                      double beta, double cutoff, bool negated, bool interpolated = false);
     PotKind kind() const override { return PotKind::NonbondedPairList; }
     void execute_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
                         uint64_t *d_du_dx, uint64_t *d_du_dp, uint64_t *d_du_dl, uint64_t *d_u,
@@ -132,12 +136,14 @@ class NonbondedPairList : public Potential {
     int N_, M_;
     double beta_, cutoff_;
     bool negated_;
+    bool interp_ = false;
     std::vector<int> pairs_, plane_, offset_;
     std::vector<double> scales_;
 
   private:
     PairListArgs args(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda) const;
     DeviceArray<int> d_pairs_, d_plane_, d_offset_;
+    DeviceArray<uint64_t> dp_tmp_;
     DeviceArray<double> d_scales_;
 };
 

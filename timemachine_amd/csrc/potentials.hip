@@ -148,9 +148,9 @@ void validate_nonbonded_box(const double *h_box, double cutoff, double padding) 
 // ---------------------------------------------------------------- all pairs
 
 NonbondedAllPairs::NonbondedAllPairs(const std::vector<int> &plane, const std::vector<int> &offset, double beta,
-                                     double cutoff, const std::vector<int> *atom_idxs)
+                                     double cutoff, const std::vector<int> *atom_idxs, bool interpolated)
     : N_(static_cast<int>(offset.size())), K_(atom_idxs ? static_cast<int>(atom_idxs->size()) : N_), beta_(beta),
-      cutoff_(cutoff), plane_(plane), offset_(offset) {
+      cutoff_(cutoff), interp_(interpolated), plane_(plane), offset_(offset) {
     if (plane.size() != offset.size()) {
         throw Error("lambda offset idxs and plane idxs need to be equivalent");
     }
@@ -194,6 +194,7 @@ void NonbondedAllPairs::ensure_state() {
     y.red_idx = 0;
     y.subset0 = subset_.empty() ? -1 : 0;
     y.sorted = sorted_ ? 1 : 0;
+    y.interp = interp_ ? 1 : 0;
     y.beta = static_cast<float>(beta_);
     y.cutoff_d = cutoff_;
     y.padding_d = padding_;
@@ -212,9 +213,10 @@ void NonbondedAllPairs::validate_sizes(int N, int P) const {
         throw Error("NonbondedAllPairs::execute_device(): expected N == N_, got N=" + std::to_string(N) +
                     ", N_=" + std::to_string(N_));
     }
-    if (P != N_ * 3) {
+    const int M = interp_ ? 2 : 1;
+    if (P != M * N_ * 3) {
         throw Error("NonbondedAllPairs::execute_device(): expected P == M*N_*3, got P=" + std::to_string(P) +
-                    ", M*N_*3=" + std::to_string(N_ * 3));
+                    ", M*N_*3=" + std::to_string(M * N_ * 3));
     }
 }
 
@@ -226,8 +228,19 @@ void NonbondedAllPairs::execute_device(int N, int P, const double *d_x, const do
     if (K_ == 0) {
         return;
     }
-    st_.prepare(d_x, d_p, d_box, d_plane_.ptr, d_offset_.ptr, nullptr, lambda, stream);
-    st_.compute(d_box, d_du_dx, d_du_dp, d_du_dl, d_u, stream);
+    st_.prepare(d_x, d_p, interp_ ? d_p + N_ * 3 : nullptr, d_box, d_plane_.ptr, d_offset_.ptr, nullptr, lambda,
+                stream);
+    if (interp_ && d_du_dp) {
+        // du/dp at p(lambda), then split onto the initial and final parameters
+        if (dp_tmp_.n != static_cast<size_t>(N_) * 3) {
+            dp_tmp_.alloc(static_cast<size_t>(N_) * 3);
+        }
+        launch_fill_u64(dp_tmp_.ptr, static_cast<size_t>(N_) * 3, 0, stream);
+        st_.compute(d_box, d_du_dx, dp_tmp_.ptr, d_du_dl, d_u, stream);
+        launch_split_interpolated_du_dp(N_, lambda, dp_tmp_.ptr, d_du_dp, stream);
+    } else {
+        st_.compute(d_box, d_du_dx, d_du_dp, d_du_dl, d_u, stream);
+    }
 }
 
 void NonbondedAllPairs::execute_dl_dp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box,
@@ -237,7 +250,8 @@ void NonbondedAllPairs::execute_dl_dp_device(int N, int P, const double *d_x, co
     if (K_ == 0) {
         return;
     }
-    st_.prepare(d_x, d_p, d_box, d_plane_.ptr, d_offset_.ptr, nullptr, lambda, stream);
+    st_.prepare(d_x, d_p, interp_ ? d_p + N_ * 3 : nullptr, d_box, d_plane_.ptr, d_offset_.ptr, nullptr, lambda,
+                stream);
     st_.compute_dl_dp(d_box, d_dl_dp, stream);
 }
 
@@ -287,9 +301,9 @@ void NonbondedAllPairs::du_dp_fixed_to_float(int N, int P, const uint64_t *du_dp
 
 NonbondedPairList::NonbondedPairList(const std::vector<int> &pair_idxs, const std::vector<double> &scales,
                                      const std::vector<int> &plane, const std::vector<int> &offset, double beta,
-                                     double cutoff, bool negated)
+                                     double cutoff, bool negated, bool interpolated)
     : N_(static_cast<int>(offset.size())), M_(static_cast<int>(pair_idxs.size() / 2)), beta_(beta), cutoff_(cutoff),
-      negated_(negated), pairs_(pair_idxs), plane_(plane), offset_(offset), scales_(scales) {
+      negated_(negated), interp_(interpolated), pairs_(pair_idxs), plane_(plane), offset_(offset), scales_(scales) {
     if (pair_idxs.size() % 2 != 0) {
         throw Error("pair_idxs.size() must be even, but got " + std::to_string(pair_idxs.size()));
     }
@@ -326,7 +340,17 @@ void NonbondedPairList::execute_dl_dp_device(int N, int P, const double *d_x, co
 void NonbondedPairList::execute_device(int N, int P, const double *d_x, const double *d_p, const double *d_box,
                                        double lambda, uint64_t *d_du_dx, uint64_t *d_du_dp, uint64_t *d_du_dl,
                                        uint64_t *d_u, hipStream_t stream) {
-    launch_pair_list(args(N, P, d_x, d_p, d_box, lambda), d_du_dx, d_du_dp, d_du_dl, d_u, stream);
+    const PairListArgs a = args(N, P, d_x, d_p, d_box, lambda);
+    if (interp_ && d_du_dp) {
+        if (dp_tmp_.n != static_cast<size_t>(N_) * 3) {
+            dp_tmp_.alloc(static_cast<size_t>(N_) * 3);
+        }
+        launch_fill_u64(dp_tmp_.ptr, static_cast<size_t>(N_) * 3, 0, stream);
+        launch_pair_list(a, d_du_dx, dp_tmp_.ptr, d_du_dl, d_u, stream);
+        launch_split_interpolated_du_dp(N_, lambda, dp_tmp_.ptr, d_du_dp, stream);
+    } else {
+        launch_pair_list(a, d_du_dx, d_du_dp, d_du_dl, d_u, stream);
+    }
 }
 
 PairListArgs NonbondedPairList::args(int N, int P, const double *d_x, const double *d_p, const double *d_box,
@@ -335,9 +359,10 @@ PairListArgs NonbondedPairList::args(int N, int P, const double *d_x, const doub
         throw Error("NonbondedPairList::execute_device(): expected N == N_, got N=" + std::to_string(N) +
                     ", N_=" + std::to_string(N_));
     }
-    if (P != N_ * 3) {
+    const int Mp = interp_ ? 2 : 1;
+    if (P != Mp * N_ * 3) {
         throw Error("NonbondedPairList::execute_device(): expected P == M*N_*3, got P=" + std::to_string(P) +
-                    ", M*N_*3=" + std::to_string(N_ * 3));
+                    ", M*N_*3=" + std::to_string(Mp * N_ * 3));
     }
     PairListArgs a{};
     a.M = M_;
@@ -352,6 +377,7 @@ PairListArgs NonbondedPairList::args(int N, int P, const double *d_x, const doub
     a.offset = d_offset_.ptr;
     a.lambdas = nullptr;
     a.lambda0 = lambda;
+    a.params1 = interp_ ? d_p + N_ * 3 : nullptr;
     a.cutoffs = nullptr;
     a.cutoff0 = cutoff_;
     a.betas = nullptr;

@@ -253,6 +253,8 @@ def _check_transforms(*exprs):
 class NonbondedAllPairs_f32(Potential):
     """NonbondedAllPairs (wrap_kernels.cpp:835-895); kernel_dir is accepted and ignored."""
 
+    _create = "tm_nonbonded_all_pairs_create"
+
     def __init__(
         self,
         kernel_dir,
@@ -281,7 +283,7 @@ class NonbondedAllPairs_f32(Potential):
                 raise RuntimeError("atom indices must be unique")
             n_atoms = atoms.size
         call(
-            "tm_nonbonded_all_pairs_create",
+            self._create,
             N,
             pi(plane),
             pi(offset),
@@ -309,6 +311,7 @@ class NonbondedPairList_f32(Potential):
     """NonbondedPairList (wrap_kernels.cpp:957-1014)."""
 
     _negated = False
+    _create = "tm_nonbonded_pair_list_create"
 
     def __init__(
         self,
@@ -336,7 +339,7 @@ class NonbondedPairList_f32(Potential):
         if scales.size // 2 != M:
             raise RuntimeError(f"expected same number of pairs and scale tuples, but got {M} != {scales.size // 2}")
         call(
-            "tm_nonbonded_pair_list_create",
+            self._create,
             offset.size,
             M,
             pi(pairs),
@@ -353,6 +356,22 @@ class NonbondedPairList_f32(Potential):
 
 class NonbondedPairList_f32_negated(NonbondedPairList_f32):
     _negated = True
+
+
+class NonbondedAllPairs_f32_interpolated(NonbondedAllPairs_f32):
+    """NonbondedAllPairs with parameters [2N, 3] (initial, final) interpolated as
+    (1 - lambda) p0 + lambda p1 (identity transforms only)."""
+
+    _create = "tm_nonbonded_all_pairs_interpolated_create"
+
+
+class NonbondedPairList_f32_interpolated(NonbondedPairList_f32):
+    _create = "tm_nonbonded_pair_list_interpolated_create"
+
+
+class NonbondedPairList_f32_negated_interpolated(NonbondedPairList_f32):
+    _negated = True
+    _create = "tm_nonbonded_pair_list_interpolated_create"
 
 
 class _CompositePotential(Potential):
@@ -388,7 +407,8 @@ def _f64_unavailable(name):
     class _Unavailable:
         def __init__(self, *args, **kwargs):
             raise NotImplementedError(
-                f"{name}: the MI355X path computes in FP32 with 64-bit fixed-point accumulation; use the _f32 class"
+                f"{name}: the MI355X path has one precision (FP64 pair arithmetic with FP32 parameters and "
+                "64-bit fixed-point accumulation, more accurate than the reference's f32 kernels); use the _f32 class"
             )
 
     _Unavailable.__name__ = name
@@ -401,6 +421,9 @@ PeriodicTorsion_f64 = _f64_unavailable("PeriodicTorsion_f64")
 NonbondedAllPairs_f64 = _f64_unavailable("NonbondedAllPairs_f64")
 NonbondedPairList_f64 = _f64_unavailable("NonbondedPairList_f64")
 NonbondedPairList_f64_negated = _f64_unavailable("NonbondedPairList_f64_negated")
+NonbondedAllPairs_f64_interpolated = _f64_unavailable("NonbondedAllPairs_f64_interpolated")
+NonbondedPairList_f64_interpolated = _f64_unavailable("NonbondedPairList_f64_interpolated")
+NonbondedPairList_f64_negated_interpolated = _f64_unavailable("NonbondedPairList_f64_negated_interpolated")
 Neighborlist_f64 = _f64_unavailable("Neighborlist_f64")
 
 

@@ -166,9 +166,40 @@ class Nonbonded(NonbondedCustomOpWrapper):
     def get_cutoff(self):
         return self.args[5]
 
+    def interpolate(self):
+        """The interpolated variant: parameters [2N, 3] = (initial, final), used at
+        lambda as (1 - lambda) initial + lambda final (lib/potentials.py:333-345)."""
+        return NonbondedInterpolated(*self.args)
+
+
+class NonbondedInterpolated(Nonbonded):
+    """Nonbonded over [2N, 3] interpolated parameters (lib/potentials.py:347-368)."""
+
+    def unbound_impl(self, precision):
+        all_pairs = NonbondedAllPairsInterpolated(
+            self.get_lambda_plane_idxs(), self.get_lambda_offset_idxs(), self.get_beta(), self.get_cutoff(), None,
+            *self.args[6:]
+        ).unbound_impl(precision)
+        exclusions = NonbondedPairListNegatedInterpolated(
+            self.get_exclusion_idxs(),
+            self.get_scale_factors(),
+            self.get_lambda_plane_idxs(),
+            self.get_lambda_offset_idxs(),
+            self.get_beta(),
+            self.get_cutoff(),
+            *self.args[6:],
+        ).unbound_impl(precision)
+        return NonbondedImplWrapper([all_pairs, exclusions])
+
 
 class NonbondedAllPairs(NonbondedCustomOpWrapper):
     pass
+
+
+class NonbondedAllPairsInterpolated(NonbondedAllPairs):
+    def unbound_impl(self, precision):
+        ctor = getattr(custom_ops, "NonbondedAllPairs" + _suffix(precision) + "_interpolated")
+        return ctor(_KERNEL_DIR, *self.args)
 
 
 class NonbondedPairList(NonbondedCustomOpWrapper):
@@ -188,4 +219,16 @@ class NonbondedPairList(NonbondedCustomOpWrapper):
 class NonbondedPairListNegated(NonbondedCustomOpWrapper):
     def unbound_impl(self, precision):
         ctor = getattr(custom_ops, "NonbondedPairList" + _suffix(precision) + "_negated")
+        return ctor(_KERNEL_DIR, *self.args)
+
+
+class NonbondedPairListInterpolated(NonbondedPairList):
+    def unbound_impl(self, precision):
+        ctor = getattr(custom_ops, "NonbondedPairList" + _suffix(precision) + "_interpolated")
+        return ctor(_KERNEL_DIR, *self.args)
+
+
+class NonbondedPairListNegatedInterpolated(NonbondedCustomOpWrapper):
+    def unbound_impl(self, precision):
+        ctor = getattr(custom_ops, "NonbondedPairList" + _suffix(precision) + "_negated_interpolated")
         return ctor(_KERNEL_DIR, *self.args)
