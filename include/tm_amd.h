@@ -60,6 +60,13 @@ int tm_nonbonded_all_pairs_create(int N, const int32_t *lambda_plane_idxs, const
 int tm_nonbonded_all_pairs_interpolated_create(int N, const int32_t *lambda_plane_idxs,
                                                const int32_t *lambda_offset_idxs, double beta, double cutoff,
                                                const int32_t *atom_idxs, int n_atom_idxs, tm_potential **out);
+/* NonbondedInteractionGroup_f32[_interpolated](kernel_dir, row_atom_idxs, lambda_plane_idxs,
+ * lambda_offset_idxs, beta, cutoff): pairs (i, j) with i among the NR row atoms and j among the
+ * others (wrap_kernels.cpp:896-954, nonbonded_interaction_group.cu:19-157). interpolated != 0:
+ * parameters [2N, 3] as above. The padding / Hilbert-sort / tile-count calls below accept it. */
+int tm_nonbonded_interaction_group_create(int N, int NR, const int32_t *row_atom_idxs,
+                                          const int32_t *lambda_plane_idxs, const int32_t *lambda_offset_idxs,
+                                          double beta, double cutoff, int interpolated, tm_potential **out);
 /* NonbondedAllPairs::set_nblist_padding, nonbonded_all_pairs.cu:180-183 */
 int tm_nonbonded_all_pairs_set_nblist_padding(tm_potential *p, double padding);
 /* NonbondedAllPairs::disable_hilbert_sort, nonbonded_all_pairs.cu:185-187 */
@@ -140,9 +147,16 @@ int tm_neighborlist_resize(tm_neighborlist *nl, int size);
 int tm_neighborlist_build_host(tm_neighborlist *nl, int N, const double *x, const double *box, double cutoff,
                                int *n_tiles);
 /* Copies the tiles of the last build: row block per tile, 64 column atoms per tile
- * (0xFFFFFFFF = padding). */
+ * (0xFFFFFFFF = padding). With row indices set, row block b holds row_idxs[64 b .. 64 b + 63]
+ * (in the order given) and the column atoms are the other atoms. */
 int tm_neighborlist_get_tiles(tm_neighborlist *nl, int capacity, int32_t *tile_rows, uint32_t *tile_atoms);
-/* Neighborlist::compute_block_bounds_host, neighborlist.cu:56-81 (64-atom blocks). */
+/* Neighborlist::set_row_idxs, neighborlist.cu:215-243: later builds pair only the NR row atoms
+ * with the other atoms. Same validation and messages. */
+int tm_neighborlist_set_row_idxs(tm_neighborlist *nl, int NR, const uint32_t *row_idxs);
+/* Neighborlist::reset_row_idxs, neighborlist.cu:245-260 */
+int tm_neighborlist_reset_row_idxs(tm_neighborlist *nl);
+/* Neighborlist::compute_block_bounds_host, neighborlist.cu:56-81 (64-atom blocks; with row
+ * indices set, the blocks of the column atoms). */
 int tm_neighborlist_compute_block_bounds_host(tm_neighborlist *nl, int N, const double *x, const double *box,
                                               double *ctr, double *ext);
 

@@ -11,6 +11,7 @@ Two evaluators:
                          cutoff (large N); pinned to the dense form in tests.
 Both take ``interpolated=True`` for parameters [2N, 3] = (initial, final) used
 at lambda as (1 - lambda) initial + lambda final (nonbonded.py:341-353).
+``interaction_group`` restricts the pair sum to (row atom, other atom) pairs.
 """
 
 from __future__ import annotations
@@ -174,6 +175,30 @@ def nonbonded_pairs(
         if len(excl):
             u = u - _pair_energy(xt, pt, lt, excl, box_diag, beta, cutoff, plane_t, offset_t, sc[:, 0], sc[:, 1])
         return u
+
+    return _grads(_interpolated(fn) if interpolated else fn, x, params, lamb)
+
+
+def interaction_group(
+    x, params, box, lamb, row_idxs, beta, cutoff, plane, offset, interpolated: bool = False
+) -> Tuple[np.ndarray, np.ndarray, float, float]:
+    """nonbonded_v3_interaction_groups (nonbonded.py:318-338, on the pairs of
+    nonbonded_v3_on_specific_pairs, :266-315) over the 4D coordinates, as
+    tests/nonbonded/test_nonbonded_interaction_group.py:93-104 builds them:
+    every pair (i, j) with i a row atom and j any other atom, within the cutoff.
+    Returns (du_dx, du_dp, du_dl, u) like nonbonded_pairs."""
+    x = np.asarray(x, dtype=np.float64)
+    N = x.shape[0]
+    is_row = np.zeros(N, dtype=bool)
+    is_row[np.asarray(row_idxs, dtype=np.int64).reshape(-1)] = True
+    pairs = candidate_pairs(x, box, cutoff)  # 3D within the cutoff: a superset of the 4D set
+    pairs = pairs[is_row[pairs[:, 0]] != is_row[pairs[:, 1]]]
+    box_diag = torch.tensor(np.diag(np.asarray(box)).copy(), dtype=torch.float64)
+    plane_t = torch.as_tensor(np.asarray(plane), dtype=torch.float64)
+    offset_t = torch.as_tensor(np.asarray(offset), dtype=torch.float64)
+
+    def fn(xt, pt, lt):
+        return _pair_energy(xt, pt, lt, pairs, box_diag, beta, cutoff, plane_t, offset_t)
 
     return _grads(_interpolated(fn) if interpolated else fn, x, params, lamb)
 

@@ -387,3 +387,73 @@ def test_interpolated_equal_endpoints_have_zero_du_dl(rng):
     P = np.concatenate([p0, p0])
     gl = nonbonded.nonbonded_pairs(x, P, box, 0.41, excl, sc, 2.0, 1.2, pl, of, interpolated=True)[2]
     assert gl == 0.0
+
+
+@pytest.mark.parametrize("interp", [False, True])
+def test_interaction_group_is_all_pairs_minus_split_planes(rng, interp):
+    """U_AB = U(all) - (U_A + U_B) (tests/nonbonded/test_nonbonded_interaction_group.py:200-260):
+    U_A + U_B is the dense all-pairs energy with the row atoms moved to a lambda plane
+    farther than the cutoff. Checks u, du_dx, du_dp and du_dl of the pair-restricted
+    evaluator against two dense ones."""
+    N, L, cutoff = 30, 2.6, 1.1
+    x = rng.uniform(0, L, size=(N, 3))
+    p0 = np.stack([rng.normal(0, 3, N), rng.uniform(0.08, 0.16, N), rng.uniform(0.1, 1.0, N)], 1)
+    P = _interp_params(rng, p0, appear=False) if interp else p0
+    box = np.eye(3) * L
+    offset = rng.integers(-2, 3, N).astype(np.int32)
+    rows = rng.choice(N, 9, replace=False)
+    plane0 = np.zeros(N, np.int32)
+    plane_split = plane0.copy()
+    plane_split[rows] = 5  # |dw| >= (5 - 4) cutoff for every row-column pair
+    none = np.zeros((0, 2), int), np.zeros((0, 2))
+    for lam in (0.0, 0.1, 0.8):
+        ig = nonbonded.interaction_group(x, P, box, lam, rows, 2.0, cutoff, plane0, offset, interpolated=interp)
+        a = nonbonded.nonbonded_v3(x, P, box, lam, *none, 2.0, cutoff, plane0, offset, interpolated=interp)
+        b = nonbonded.nonbonded_v3(x, P, box, lam, *none, 2.0, cutoff, plane_split, offset, interpolated=interp)
+        assert ig[3] != 0.0
+        assert ig[3] == pytest.approx(a[3] - b[3], rel=1e-10, abs=1e-10)
+        np.testing.assert_allclose(ig[0], a[0] - b[0], rtol=1e-9, atol=1e-9)
+        np.testing.assert_allclose(ig[1], a[1] - b[1], rtol=1e-9, atol=1e-9)
+        assert ig[2] == pytest.approx(a[2] - b[2], rel=1e-9, abs=1e-9)
+
+
+def test_interaction_group_two_particles():
+    """One row and one column atom: the pair term; two row atoms: nothing."""
+    x = np.array([[0.2, 0.3, 0.4], [0.6, 0.3, 0.4], [0.2, 0.7, 0.4]])
+    p = np.array([[1.3, 0.12, 0.8], [-0.7, 0.14, 0.6], [0.5, 0.1, 0.3]])
+    box = np.eye(3) * 3.0
+    u = nonbonded.interaction_group(x[:2], p[:2], box, 0.0, [0], 2.0, 1.2, [0, 0], [0, 0])[3]
+    r, s, e = 0.4, 0.26, 0.48
+    assert u == pytest.approx(1.3 * -0.7 * erfc(2 * r) / r + 4 * e * ((s / r) ** 12 - (s / r) ** 6), rel=1e-12)
+    # rows {0, 2} and column {1}: the 0-2 pair is not counted
+    u3 = nonbonded.interaction_group(x, p, box, 0.0, [0, 2], 2.0, 1.2, [0, 0, 0], [0, 0, 0])[3]
+    r12 = math.hypot(0.4, 0.4)
+    s12, e12 = 0.24, 0.6 * 0.3
+    u12 = -0.7 * 0.5 * erfc(2 * r12) / r12 + 4 * e12 * ((s12 / r12) ** 12 - (s12 / r12) ** 6)
+    assert u3 == pytest.approx(u + u12, rel=1e-12)
+
+
+def test_nblist_subset_reference_equals_brute_force_and_is_order_independent(rng):
+    """tests/test_nblist.py:95-193: the row-subset list against a double loop; shuffling
+    the row indices changes the blocks but not the union."""
+    x = rng.uniform(0, 3.0, size=(300, 3))
+    box = np.eye(3) * 3.0
+    rows = rng.choice(300, 140, replace=False)
+    ref = nblist.build_reference_ixn_list_with_subset(x, box, 0.9, rows, 64)
+    assert len(ref) == 3
+    L = 3.0
+    cols = np.setdiff1d(np.arange(300), rows)
+    for rb, lst in enumerate(ref):
+        brute = []
+        for j in cols:
+            d = x[rows[rb * 64 : (rb + 1) * 64]] - x[j]
+            d -= L * np.floor(d / L + 0.5)
+            if np.any(np.sqrt((d * d).sum(1)) < 0.9):
+                brute.append(int(j))
+        assert sorted(lst) == brute
+    shuffled = rows.copy()
+    rng.shuffle(shuffled)
+    ref2 = nblist.build_reference_ixn_list_with_subset(x, box, 0.9, shuffled, 64)
+    assert set(np.concatenate(ref).tolist()) == set(np.concatenate(ref2).tolist())
+    f32 = nblist.fp32_ixn_list_rows(x, box, 0.9, rows, 64)
+    assert sum(len(set(a) ^ set(b)) for a, b in zip(ref, f32)) <= 2

@@ -26,6 +26,10 @@ _SIGS = {
     "tm_tile_size": (c_int, []),
     "tm_device_count": (c_int, [POINTER(c_int)]),
     "tm_nonbonded_all_pairs_create": (c_int, [c_int, _PI, _PI, c_double, c_double, _PI, c_int, _PPV]),
+    "tm_nonbonded_interaction_group_create": (
+        c_int,
+        [c_int, c_int, _PI, _PI, _PI, c_double, c_double, c_int, _PPV],
+    ),
     "tm_nonbonded_all_pairs_interpolated_create": (c_int, [c_int, _PI, _PI, c_double, c_double, _PI, c_int, _PPV]),
     "tm_nonbonded_all_pairs_set_nblist_padding": (c_int, [_P, c_double]),
     "tm_nonbonded_all_pairs_disable_hilbert_sort": (c_int, [_P]),
@@ -72,6 +76,8 @@ _SIGS = {
     "tm_neighborlist_build_host": (c_int, [_P, c_int, _PD, _PD, c_double, POINTER(c_int)]),
     "tm_neighborlist_get_tiles": (c_int, [_P, c_int, _PI, POINTER(c_uint32)]),
     "tm_neighborlist_compute_block_bounds_host": (c_int, [_P, c_int, _PD, _PD, _PD, _PD]),
+    "tm_neighborlist_set_row_idxs": (c_int, [_P, c_int, POINTER(c_uint32)]),
+    "tm_neighborlist_reset_row_idxs": (c_int, [_P]),
     "tm_langevin_integrator_create": (c_int, [c_int, c_double, c_double, _PD, _PD, c_uint64, _PPV]),
     "tm_integrator_release": (None, [_P]),
     "tm_barostat_create": (

@@ -1,6 +1,7 @@
 // extern "C" boundary (include/tm_amd.h) over the C++ classes.
 #include <cstring>
 #include <memory>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -56,6 +57,7 @@ struct tm_neighborlist {
     bool ready = false;
     int built_n = 0;
     double built_cutoff = -1.0;
+    std::vector<int> rows; // row subset (empty: all pairs)
 };
 
 namespace {
@@ -181,6 +183,17 @@ int tm_nonbonded_all_pairs_interpolated_create(int N, const int32_t *plane, cons
             sub = to_vec(atom_idxs, n_atom_idxs);
         }
         auto p = std::make_shared<NonbondedAllPairs>(pl, of, beta, cutoff, n_atom_idxs >= 0 ? &sub : nullptr, true);
+        *out = new tm_potential{p};
+    });
+}
+
+int tm_nonbonded_interaction_group_create(int N, int NR, const int32_t *row_atom_idxs, const int32_t *plane,
+                                          const int32_t *offset, double beta, double cutoff, int interpolated,
+                                          tm_potential **out) {
+    return guarded([&] {
+        std::vector<int> rows = NR > 0 ? to_vec(row_atom_idxs, NR) : std::vector<int>();
+        auto p = std::make_shared<NonbondedInteractionGroup>(rows, to_vec(plane, N), to_vec(offset, N), beta, cutoff,
+                                                             interpolated != 0);
         *out = new tm_potential{p};
     });
 }
@@ -374,6 +387,35 @@ int tm_neighborlist_resize(tm_neighborlist *nl, int size) {
         }
         nl->n = size;
         nl->ready = false;
+        nl->rows.clear(); // resizing restores all-pairs operation (neighborlist.cu:268-280)
+    });
+}
+
+int tm_neighborlist_set_row_idxs(tm_neighborlist *nl, int NR, const uint32_t *row_idxs) {
+    return guarded([&] {
+        if (NR <= 0) {
+            throw Error("idxs can't be empty");
+        }
+        std::vector<int> rows(row_idxs, row_idxs + NR);
+        std::set<int> uniq(rows.begin(), rows.end());
+        if (static_cast<int>(uniq.size()) != NR) {
+            throw Error("atom indices must be unique");
+        }
+        if (NR >= nl->n) {
+            throw Error("number of idxs must be less than N");
+        }
+        if (*uniq.rbegin() >= nl->n) {
+            throw Error("indices values must be less than N");
+        }
+        nl->rows = rows;
+        nl->ready = false;
+    });
+}
+
+int tm_neighborlist_reset_row_idxs(tm_neighborlist *nl) {
+    return guarded([&] {
+        nl->rows.clear();
+        nl->ready = false;
     });
 }
 
@@ -392,7 +434,23 @@ int tm_neighborlist_build_host(tm_neighborlist *nl, int N, const double *x, cons
             y.sorted = 0;
             y.cutoff_d = cutoff;
             y.padding_d = 0.0;
-            nl->st.init({y}, {});
+            std::vector<int> subset;
+            if (!nl->rows.empty()) {
+                // rows in the given order, then the other atoms ascending
+                std::vector<char> is_row(N, 0);
+                for (int a : nl->rows) {
+                    is_row[a] = 1;
+                }
+                subset = nl->rows;
+                for (int a = 0; a < N; a++) {
+                    if (!is_row[a]) {
+                        subset.push_back(a);
+                    }
+                }
+                y.subset0 = 0;
+                y.nrow = static_cast<int>(nl->rows.size());
+            }
+            nl->st.init({y}, subset);
             nl->ready = true;
             nl->built_n = N;
             nl->built_cutoff = cutoff;
@@ -425,6 +483,15 @@ int tm_neighborlist_get_tiles(tm_neighborlist *nl, int capacity, int32_t *tile_r
         TM_HIP_CHECK(hipMemcpy(tile_rows, nl->st.tile_row.ptr, sizeof(int) * cnt, hipMemcpyDeviceToHost));
         TM_HIP_CHECK(hipMemcpy(tile_atoms, nl->st.tile_atoms.ptr, sizeof(uint32_t) * cnt * TILE,
                                hipMemcpyDeviceToHost));
+        // tiles hold sorted slots: map them to atom indices
+        std::vector<uint32_t> perm(nl->st.total_slots);
+        TM_HIP_CHECK(
+            hipMemcpy(perm.data(), nl->st.perm.ptr, sizeof(uint32_t) * perm.size(), hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < static_cast<size_t>(cnt) * TILE; q++) {
+            if (tile_atoms[q] != NB_INVALID) {
+                tile_atoms[q] = perm[tile_atoms[q]];
+            }
+        }
     });
 }
 
@@ -437,10 +504,13 @@ int tm_neighborlist_compute_block_bounds_host(tm_neighborlist *nl, int N, const 
         if (rc != TM_OK) {
             throw Error(g_err);
         }
-        const int nb = ceil_div(N, TILE);
+        // all blocks, or the column blocks (after the row blocks) of a row subset
+        const NbSys &y = nl->st.h_sys[0];
+        const int b0 = y.nrow > 0 ? y.row_blocks : 0;
+        const int nb = y.nblocks - b0;
         std::vector<float4> c(nb), e(nb);
-        TM_HIP_CHECK(hipMemcpy(c.data(), nl->st.bb_ctr.ptr, sizeof(float4) * nb, hipMemcpyDeviceToHost));
-        TM_HIP_CHECK(hipMemcpy(e.data(), nl->st.bb_ext.ptr, sizeof(float4) * nb, hipMemcpyDeviceToHost));
+        TM_HIP_CHECK(hipMemcpy(c.data(), nl->st.bb_ctr.ptr + b0, sizeof(float4) * nb, hipMemcpyDeviceToHost));
+        TM_HIP_CHECK(hipMemcpy(e.data(), nl->st.bb_ext.ptr + b0, sizeof(float4) * nb, hipMemcpyDeviceToHost));
         for (int b = 0; b < nb; b++) {
             ctr[b * 3 + 0] = c[b].x;
             ctr[b * 3 + 1] = c[b].y;

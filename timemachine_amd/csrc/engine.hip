@@ -358,6 +358,8 @@ void flatten(const PotentialPtr &p, const double *params, int P, Flat &out) {
         out.nb_params.assign(params, params + P);
         break;
     }
+    case PotKind::NonbondedInteractionGroup:
+        throw Error("Engine: NonbondedInteractionGroup is not supported in a Context (use it through Potential)");
     case PotKind::NonbondedPairList:
         out.pls.emplace_back(std::static_pointer_cast<NonbondedPairList>(p), std::vector<double>(params, params + P));
         break;

@@ -19,6 +19,7 @@
 
 #include <vector>
 
+#include "hd.h"
 #include "tm_common.h"
 
 namespace tmamd {
@@ -45,6 +46,16 @@ struct NbSys {
     int subset0;  // offset into the subset atom list, -1 when all atoms take part
     int sorted;   // 1: spatial (Hilbert) sort on rebuild, 0: identity order
     int interp;   // 1: parameters interpolated between two sets, p = (1 - lambda) p0 + lambda p1
+    // Row subset (interaction group, nonbonded_interaction_group.cu:19-468): the
+    // first nrow atoms of the subset list are rows, the rest columns, and only
+    // row-column pairs interact. Rows take slots [0, nrow), columns start at the
+    // first block after the rows, slots [row_blocks * 64, slot_end). nrow = 0 is
+    // the all-pairs (upper triangle) layout: row_blocks = nblocks,
+    // row_end = slot_end = k. Filled in by NbState::init.
+    int nrow;
+    int row_blocks; // blocks holding row atoms
+    int row_end;    // row slots are valid below this
+    int slot_end;   // column slots are valid below this
     float beta;
     float cutoff;
     float nb_cutoff; // neighborlist radius, FP32 predicate (cutoff + padding, plus a margin for potentials)
@@ -54,11 +65,18 @@ struct NbSys {
 };
 
 // Upper bound on tiles of one system: row block r holds at most
-// ceil((k - 64 r) / 64) full tiles plus one partial tile.
-inline int nb_tile_capacity(int k) {
+// ceil((k - 64 r) / 64) full tiles plus one partial tile. With a row subset a
+// row block holds at most ceil(ncol / 64) tiles.
+inline int nb_tile_capacity(int k, int nrow = 0) {
+    if (nrow > 0) {
+        return ceil_div(nrow, TILE) * (ceil_div(k - nrow, TILE) + 1);
+    }
     const int nb = ceil_div(k, TILE);
     return nb * (nb + 1) / 2 + nb;
 }
+
+// Slots of block b below this limit hold atoms.
+TM_HD inline int nb_block_end(const NbSys &S, int b) { return b < S.row_blocks ? S.row_end : S.slot_end; }
 
 struct NbState {
     int S = 0;

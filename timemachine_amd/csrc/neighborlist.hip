@@ -214,16 +214,23 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort(
     const double by = boxes[s * 9 + 4];
     const double bz = boxes[s * 9 + 8];
 
-    if (!S.sorted) {
-        for (int a = t; a < S.k; a += SORT_THREADS) {
-            perm[S.slot0 + a] = static_cast<uint32_t>(S.subset0 >= 0 ? subset[S.subset0 + a] : a);
+    // one segment, or rows then columns with a row subset; each is sorted on its own
+    const int nseg = S.nrow > 0 ? 2 : 1;
+    for (int sg = 0; sg < nseg; sg++) {
+        const int a_lo = sg == 1 ? S.nrow : 0;
+        const int a_hi = (nseg == 2 && sg == 0) ? S.nrow : S.k;
+        uint32_t *seg_perm = perm + S.slot0 + (sg == 1 ? S.row_blocks * TILE : 0);
+        if (!S.sorted) {
+            for (int a = a_lo + t; a < a_hi; a += SORT_THREADS) {
+                seg_perm[a - a_lo] = static_cast<uint32_t>(S.subset0 >= 0 ? subset[S.subset0 + a] : a);
+            }
+            continue;
         }
-    } else {
         for (int c = t; c < HILBERT_CELLS; c += SORT_THREADS) {
             count[c] = 0;
         }
         __syncthreads();
-        for (int a = t; a < S.k; a += SORT_THREADS) {
+        for (int a = a_lo + t; a < a_hi; a += SORT_THREADS) {
             const int atom = S.subset0 >= 0 ? subset[S.subset0 + a] : a;
             const int g = S.atom0 + atom;
             const uint32_t key =
@@ -242,30 +249,32 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort(
             count[c] = start[c];
         }
         __syncthreads();
-        for (int a = t; a < S.k; a += SORT_THREADS) {
+        for (int a = a_lo + t; a < a_hi; a += SORT_THREADS) {
             const int atom = S.subset0 >= 0 ? subset[S.subset0 + a] : a;
             const uint32_t key = keys[S.slot0 + a];
             const int p = atomicAdd(&count[key], 1);
-            perm[S.slot0 + p] = static_cast<uint32_t>(atom);
+            seg_perm[p] = static_cast<uint32_t>(atom);
         }
         __syncthreads();
         for (int c = t; c < HILBERT_CELLS; c += SORT_THREADS) {
             const int b = start[c];
             const int e = count[c];
-            uint32_t *seg = perm + S.slot0;
             for (int i = b + 1; i < e; i++) {
-                const uint32_t v = seg[i];
+                const uint32_t v = seg_perm[i];
                 int j = i - 1;
-                while (j >= b && seg[j] > v) {
-                    seg[j + 1] = seg[j];
+                while (j >= b && seg_perm[j] > v) {
+                    seg_perm[j + 1] = seg_perm[j];
                     j--;
                 }
-                seg[j + 1] = v;
+                seg_perm[j + 1] = v;
             }
         }
+        __syncthreads(); // count / start are reused by the next segment
     }
-    for (int a = S.k + t; a < S.nblocks * TILE; a += SORT_THREADS) {
-        perm[S.slot0 + a] = NB_INVALID;
+    for (int a = t; a < S.nblocks * TILE; a += SORT_THREADS) {
+        if (a >= nb_block_end(S, a / TILE)) {
+            perm[S.slot0 + a] = NB_INVALID;
+        }
     }
     for (int i = t; i < S.n_atoms * 3; i += SORT_THREADS) {
         nblist_x[S.atom0 * 3 + i] = x[S.atom0 * 3 + i];
@@ -385,7 +394,7 @@ __global__ __launch_bounds__(256) void k_block_bounds(
     const float4 ref = to_f32(pos[S.slot0 + b * TILE]);
     const int slot = b * TILE + lane;
     float px = ref.x, py = ref.y, pz = ref.z;
-    if (slot < S.k) {
+    if (slot < nb_block_end(S, b)) {
         const float4 p = to_f32(pos[S.slot0 + slot]);
         px = ref.x + min_image_f(p.x - ref.x, bx, ibx);
         py = ref.y + min_image_f(p.y - ref.y, by, iby);
@@ -428,7 +437,7 @@ __global__ __launch_bounds__(256) void k_find_tiles(
     const NbSys S = sys[s];
     const int wave = threadIdx.x >> 6;
     const int r = blockIdx.x * 4 + wave;
-    if (r >= S.nblocks) {
+    if (r >= S.row_blocks) {
         return;
     }
     const int lane = threadIdx.x & 63;
@@ -453,7 +462,7 @@ __global__ __launch_bounds__(256) void k_find_tiles(
     const float scr2 = scr * scr;
 
     const int i = r * TILE + lane;
-    const bool row_valid = i < S.k;
+    const bool row_valid = i < S.row_end;
     const float4 pi = row_valid ? to_f32(pos[S.slot0 + i]) : make_float4(0.f, 0.f, 0.f, 0.f);
     rows[lane] = pi;
     wave_lds_sync();
@@ -467,10 +476,10 @@ __global__ __launch_bounds__(256) void k_find_tiles(
     if (lane < SUBS) {
         const int a0 = r * TILE + lane * SUB;
         float4 c = make_float4(0.f, 0.f, 0.f, 0.f), e = make_float4(-1.f, -1.f, -1.f, 0.f); // empty
-        if (a0 < S.k) {
+        if (a0 < S.row_end) {
             const float4 ref = rows[lane * SUB];
             float lox = 0.f, loy = 0.f, loz = 0.f, hix = 0.f, hiy = 0.f, hiz = 0.f;
-            for (int a = 1; a < SUB && a0 + a < S.k; a++) {
+            for (int a = 1; a < SUB && a0 + a < S.row_end; a++) {
                 const float4 q = rows[lane * SUB + a];
                 const float dx = min_image_f(q.x - ref.x, bx, ibx);
                 const float dy = min_image_f(q.y - ref.y, by, iby);
@@ -491,7 +500,8 @@ __global__ __launch_bounds__(256) void k_find_tiles(
     wave_lds_sync();
 
     int nbuf = 0;
-    for (int cbase = r; cbase < S.nblocks; cbase += TILE) {
+    // the upper triangle c >= r, or every column block after the rows
+    for (int cbase = S.nrow > 0 ? S.row_blocks : r; cbase < S.nblocks; cbase += TILE) {
         const int c = cbase + lane;
         bool cand = false;
         if (c < S.nblocks) {
@@ -509,7 +519,7 @@ __global__ __launch_bounds__(256) void k_find_tiles(
             cc = bb_ctr[S.block0 + cbx];
             ce = bb_ext[S.block0 + cbx];
             const int jx = cbx * TILE + lane;
-            pj = jx < S.k ? to_f32(pos[S.slot0 + jx]) : make_float4(0.f, 0.f, 0.f, 0.f);
+            pj = jx < S.slot_end ? to_f32(pos[S.slot0 + jx]) : make_float4(0.f, 0.f, 0.f, 0.f);
         };
         int cb_next = -1;
         float4 cc_next, ce_next, pj_next;
@@ -540,7 +550,7 @@ __global__ __launch_bounds__(256) void k_find_tiles(
             }
             const uint64_t rmask = ballot(row_near);
             const int j = cb * TILE + lane;
-            const bool col_valid = j < S.k;
+            const bool col_valid = j < S.slot_end;
             bool pending = false;
             const float4 pj = pj_cur;
             if (col_valid && rmask) {
@@ -580,7 +590,7 @@ __global__ __launch_bounds__(256) void k_find_tiles(
                 int q = first;
                 while (near != 0 && !found) {
                     near &= ~(1u << q);
-                    const int a_end = min(SUB, S.k - (r * TILE + q * SUB));
+                    const int a_end = min(SUB, S.row_end - (r * TILE + q * SUB));
                     for (int a = 0; a < a_end && !found; a++) {
                         const float4 rq = rows[q * SUB + a];
                         const float dx = min_image_f(rq.x - pj.x, bx, ibx);
@@ -707,11 +717,25 @@ void NbState::init(const std::vector<NbSys> &sys_in, const std::vector<int> &sub
     int total_atoms_end = 0;
     for (int s = 0; s < S; s++) {
         NbSys &y = h_sys[s];
-        y.nblocks = ceil_div(y.k, TILE);
+        if (y.nrow > 0) {
+            if (y.nrow >= y.k || y.subset0 < 0) {
+                throw Error("NbState: a row subset needs column atoms and a subset list");
+            }
+            y.row_blocks = ceil_div(y.nrow, TILE);
+            y.nblocks = y.row_blocks + ceil_div(y.k - y.nrow, TILE);
+            y.row_end = y.nrow;
+            y.slot_end = y.row_blocks * TILE + (y.k - y.nrow);
+        } else {
+            y.nrow = 0;
+            y.nblocks = ceil_div(y.k, TILE);
+            y.row_blocks = y.nblocks;
+            y.row_end = y.k;
+            y.slot_end = y.k;
+        }
         y.slot0 = total_slots;
         y.block0 = total_blocks;
         y.tile0 = total_tiles;
-        y.tile_cap = nb_tile_capacity(y.k);
+        y.tile_cap = nb_tile_capacity(y.k, y.nrow);
         y.nb_cutoff = static_cast<float>(y.cutoff_d + y.padding_d + nb_margin);
         y.cutoff = static_cast<float>(y.cutoff_d);
         total_slots += y.nblocks * TILE;

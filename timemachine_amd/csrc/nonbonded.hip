@@ -214,8 +214,8 @@ __device__ __forceinline__ void tile_body(
     constexpr int PQ = DX ? 3 : 0;
     const int lane = threadIdx.x & 63;
     const uint32_t i = static_cast<uint32_t>(r * TILE + lane);
-    const uint32_t k = static_cast<uint32_t>(S.k);
-    const bool vi = i < k;
+    const uint32_t k = static_cast<uint32_t>(S.slot_end); // column slots (NB_INVALID pads) are below this
+    const bool vi = i < static_cast<uint32_t>(S.row_end);
 
     BoxDev B;
     B.bx = boxes[s * 9 + 0];
@@ -397,7 +397,7 @@ __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded_tiles(
         const int i = r * TILE + lane;
         const double4 pi = pos[Sd.slot0 + i];
         const float4 ai = par[Sd.slot0 + i];
-        const bool vj = jl < static_cast<uint32_t>(Sd.k);
+        const bool vj = jl < static_cast<uint32_t>(Sd.slot_end);
         const double4 pj = vj ? pos[Sd.slot0 + jl] : make_double4(0.0, 0.0, 0.0, 0.0);
         const float4 aj = vj ? par[Sd.slot0 + jl] : make_float4(0.f, 0.f, 0.f, 0.f);
         float4 dpi = make_float4(0.f, 0.f, 0.f, 0.f), dpj = dpi;

@@ -17,7 +17,16 @@
 
 namespace tmamd {
 
-enum class PotKind { NonbondedAllPairs, NonbondedPairList, HarmonicBond, HarmonicAngle, PeriodicTorsion, Summed, Fanout };
+enum class PotKind {
+    NonbondedAllPairs,
+    NonbondedInteractionGroup,
+    NonbondedPairList,
+    HarmonicBond,
+    HarmonicAngle,
+    PeriodicTorsion,
+    Summed,
+    Fanout
+};
 
 class Potential {
   public:
@@ -106,11 +115,15 @@ class NonbondedAllPairs : public Potential {
     int num_tiles(hipStream_t stream);
 
     int N_, K_;
+    int nrow_ = 0; // > 0: only pairs between the first nrow_ atoms of subset_ and the rest
     double beta_, cutoff_, padding_ = 0.1;
     bool sorted_ = true;
     bool interp_ = false;
     std::vector<int> plane_, offset_, subset_;
     NbState st_;
+
+  protected:
+    std::string name_ = "NonbondedAllPairs"; // in error messages
 
   private:
     void ensure_state();
@@ -118,6 +131,18 @@ class NonbondedAllPairs : public Potential {
     bool dirty_ = true;
     DeviceArray<int> d_plane_, d_offset_;
     DeviceArray<uint64_t> dp_tmp_; // du/dp at the interpolated parameters [N, 3]
+};
+
+// Pairs (i, j) with i a row atom and j any other atom
+// (nonbonded_interaction_group.cu:19-468): the all-pairs machinery with the
+// row atoms (ascending) first in the subset list and the rest after them; each
+// part is sorted on its own and tiles pair row blocks with column blocks only.
+class NonbondedInteractionGroup : public NonbondedAllPairs {
+  public:
+    NonbondedInteractionGroup(const std::vector<int> &row_atom_idxs, const std::vector<int> &lambda_plane_idxs,
+                              const std::vector<int> &lambda_offset_idxs, double beta, double cutoff,
+                              bool interpolated = false);
+    PotKind kind() const override { return PotKind::NonbondedInteractionGroup; }
 };
 
 class NonbondedPairList : public Potential {

@@ -193,6 +193,7 @@ void NonbondedAllPairs::ensure_state() {
     y.k = K_;
     y.red_idx = 0;
     y.subset0 = subset_.empty() ? -1 : 0;
+    y.nrow = nrow_;
     y.sorted = sorted_ ? 1 : 0;
     y.interp = interp_ ? 1 : 0;
     y.beta = static_cast<float>(beta_);
@@ -210,12 +211,12 @@ void NonbondedAllPairs::validate_host(int N, int P, const double *h_box) const {
 
 void NonbondedAllPairs::validate_sizes(int N, int P) const {
     if (N != N_) {
-        throw Error("NonbondedAllPairs::execute_device(): expected N == N_, got N=" + std::to_string(N) +
+        throw Error(name_ + "::execute_device(): expected N == N_, got N=" + std::to_string(N) +
                     ", N_=" + std::to_string(N_));
     }
     const int M = interp_ ? 2 : 1;
     if (P != M * N_ * 3) {
-        throw Error("NonbondedAllPairs::execute_device(): expected P == M*N_*3, got P=" + std::to_string(P) +
+        throw Error(name_ + "::execute_device(): expected P == M*N_*3, got P=" + std::to_string(P) +
                     ", M*N_*3=" + std::to_string(M * N_ * 3));
     }
 }
@@ -272,7 +273,7 @@ void NonbondedAllPairs::check_device_status(hipStream_t stream) {
             throw Error("Provided non-ortholinear box, unable to compute nonbonded energy");
         }
         if (status & NB_STATUS_TILE_OVERFLOW) {
-            throw Error("NonbondedAllPairs: neighborlist tile buffer overflow");
+            throw Error(name_ + ": neighborlist tile buffer overflow");
         }
     }
 }
@@ -295,6 +296,44 @@ static void nonbonded_du_dp_to_float(int P, const uint64_t *du_dp, double *out) 
 
 void NonbondedAllPairs::du_dp_fixed_to_float(int N, int P, const uint64_t *du_dp, double *out) const {
     nonbonded_du_dp_to_float(P, du_dp, out);
+}
+
+// ---------------------------------------------------------------- interaction group
+
+namespace {
+// rows ascending (the reference keeps them in a std::set), then every other atom ascending
+std::vector<int> rows_then_columns(const std::vector<int> &rows, int N) {
+    if (rows.empty()) {
+        throw Error("row_atom_idxs must be nonempty");
+    }
+    std::set<int> uniq(rows.begin(), rows.end());
+    if (uniq.size() != rows.size()) {
+        throw Error("atom indices must be unique");
+    }
+    if (*uniq.begin() < 0 || *uniq.rbegin() >= N) {
+        throw Error("indices values must be less than N");
+    }
+    if (static_cast<int>(rows.size()) >= N) {
+        throw Error("number of idxs must be less than N");
+    }
+    std::vector<int> out(uniq.begin(), uniq.end());
+    for (int a = 0; a < N; a++) {
+        if (!uniq.count(a)) {
+            out.push_back(a);
+        }
+    }
+    return out;
+}
+} // namespace
+
+NonbondedInteractionGroup::NonbondedInteractionGroup(const std::vector<int> &rows, const std::vector<int> &plane,
+                                                     const std::vector<int> &offset, double beta, double cutoff,
+                                                     bool interpolated)
+    : NonbondedAllPairs(plane, offset, beta, cutoff, nullptr, interpolated) {
+    name_ = "NonbondedInteractionGroup";
+    subset_ = rows_then_columns(rows, N_);
+    K_ = N_;
+    nrow_ = static_cast<int>(rows.size());
 }
 
 // ---------------------------------------------------------------- pair list

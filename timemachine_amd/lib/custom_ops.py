@@ -374,6 +374,51 @@ class NonbondedPairList_f32_negated_interpolated(NonbondedPairList_f32):
     _create = "tm_nonbonded_pair_list_interpolated_create"
 
 
+class NonbondedInteractionGroup_f32(NonbondedAllPairs_f32):
+    """NonbondedInteractionGroup (wrap_kernels.cpp:896-954): pairs (i, j) with i among
+    the row atoms and j among the others."""
+
+    _interpolated = 0
+
+    def __init__(
+        self,
+        kernel_dir,
+        row_atom_idxs_i,
+        lambda_plane_idxs_i,
+        lambda_offset_idxs_i,
+        beta,
+        cutoff,
+        transform_lambda_charge="lambda",
+        transform_lambda_sigma="lambda",
+        transform_lambda_epsilon="lambda",
+        transform_lambda_w="lambda",
+    ):
+        Potential.__init__(self)
+        _check_transforms(transform_lambda_charge, transform_lambda_sigma, transform_lambda_epsilon, transform_lambda_w)
+        rows = i32(row_atom_idxs_i)
+        plane = i32(lambda_plane_idxs_i)
+        offset = i32(lambda_offset_idxs_i)
+        if plane.size != offset.size:
+            raise RuntimeError("lambda offset idxs and plane idxs need to be equivalent")
+        call(
+            "tm_nonbonded_interaction_group_create",
+            offset.size,
+            rows.size,
+            pi(rows),
+            pi(plane),
+            pi(offset),
+            float(beta),
+            float(cutoff),
+            self._interpolated,
+            ctypes.byref(self._h),
+        )
+        self._keep = [rows, plane, offset]
+
+
+class NonbondedInteractionGroup_f32_interpolated(NonbondedInteractionGroup_f32):
+    _interpolated = 1
+
+
 class _CompositePotential(Potential):
     def get_potentials(self) -> List[Potential]:
         return list(self._pots)
@@ -422,6 +467,8 @@ NonbondedAllPairs_f64 = _f64_unavailable("NonbondedAllPairs_f64")
 NonbondedPairList_f64 = _f64_unavailable("NonbondedPairList_f64")
 NonbondedPairList_f64_negated = _f64_unavailable("NonbondedPairList_f64_negated")
 NonbondedAllPairs_f64_interpolated = _f64_unavailable("NonbondedAllPairs_f64_interpolated")
+NonbondedInteractionGroup_f64 = _f64_unavailable("NonbondedInteractionGroup_f64")
+NonbondedInteractionGroup_f64_interpolated = _f64_unavailable("NonbondedInteractionGroup_f64_interpolated")
 NonbondedPairList_f64_interpolated = _f64_unavailable("NonbondedPairList_f64_interpolated")
 NonbondedPairList_f64_negated_interpolated = _f64_unavailable("NonbondedPairList_f64_negated_interpolated")
 Neighborlist_f64 = _f64_unavailable("Neighborlist_f64")
@@ -501,26 +548,25 @@ class Neighborlist_f32(_Handle):
     def __init__(self, N: int):
         super().__init__()
         self._N = int(N)
+        self._rows = 0  # row atoms of a row subset, 0 = all pairs
         call("tm_neighborlist_create", self._N, ctypes.byref(self._h))
 
     def resize(self, size: int):
         call("tm_neighborlist_resize", self._h, int(size))
         self._N = int(size)
+        self._rows = 0
 
     def reset_row_idxs(self):
-        pass
+        call("tm_neighborlist_reset_row_idxs", self._h)
+        self._rows = 0
 
     def set_row_idxs(self, idxs):
-        idxs = np.asarray(idxs, dtype=np.uint32)
-        if idxs.size == 0:
-            raise RuntimeError("idxs can't be empty")
-        if len(set(idxs.tolist())) != idxs.size:
-            raise RuntimeError("atom indices must be unique")
-        if idxs.size >= self._N:
-            raise RuntimeError("number of idxs must be less than N")
-        if idxs.max() >= self._N:
-            raise RuntimeError("indices values must be less than N")
-        raise NotImplementedError("row subsets (interaction groups) are not implemented on the MI355X path yet")
+        """Later builds pair only these atoms (rows, blocks of 64 in the order given)
+        with every other atom (neighborlist.cu:215-243)."""
+        idxs = np.ascontiguousarray(np.asarray(idxs).reshape(-1), dtype=np.uint32)
+        call("tm_neighborlist_set_row_idxs", self._h, idxs.size,
+             idxs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+        self._rows = idxs.size
 
     def _build(self, coords, box, cutoff):
         coords = f64(coords)
@@ -545,7 +591,7 @@ class Neighborlist_f32(_Handle):
     def get_nblist(self, coords, box, cutoff) -> List[List[int]]:
         N = np.asarray(coords).shape[0]
         rows, atoms = self._build(coords, box, cutoff)
-        n_blocks = (N + self.BLOCK - 1) // self.BLOCK
+        n_blocks = ((self._rows or N) + self.BLOCK - 1) // self.BLOCK
         out: List[List[int]] = [[] for _ in range(n_blocks)]
         for r, a in zip(rows, atoms):
             out[int(r)].extend(int(j) for j in a if j < N)
@@ -559,7 +605,7 @@ class Neighborlist_f32(_Handle):
         if coords.shape[1] != 3:
             raise RuntimeError("Dimensions must be 3")
         N = coords.shape[0]
-        nb = (N + self.BLOCK - 1) // self.BLOCK
+        nb = (N - self._rows + self.BLOCK - 1) // self.BLOCK  # the column blocks of a row subset
         ctr = np.zeros((nb, 3))
         ext = np.zeros((nb, 3))
         call("tm_neighborlist_compute_block_bounds_host", self._h, N, pd(coords), pd(box), pd(ctr), pd(ext))

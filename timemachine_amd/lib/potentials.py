@@ -202,6 +202,17 @@ class NonbondedAllPairsInterpolated(NonbondedAllPairs):
         return ctor(_KERNEL_DIR, *self.args)
 
 
+class NonbondedInteractionGroup(NonbondedCustomOpWrapper):
+    """NonbondedInteractionGroup(row_atom_idxs, lambda_plane_idxs, lambda_offset_idxs, beta, cutoff)
+    (lib/potentials.py:389-390)."""
+
+
+class NonbondedInteractionGroupInterpolated(NonbondedInteractionGroup):
+    def unbound_impl(self, precision):
+        ctor = getattr(custom_ops, "NonbondedInteractionGroup" + _suffix(precision) + "_interpolated")
+        return ctor(_KERNEL_DIR, *self.args)
+
+
 class NonbondedPairList(NonbondedCustomOpWrapper):
     def get_idxs(self):
         return self.args[0]
