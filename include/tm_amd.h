@@ -160,6 +160,13 @@ int tm_neighborlist_reset_row_idxs(tm_neighborlist *nl);
 int tm_neighborlist_compute_block_bounds_host(tm_neighborlist *nl, int N, const double *x, const double *box,
                                               double *ctr, double *ext);
 
+/* ---- minimisation -------------------------------------------------------- */
+
+/* fire_minimize(x0, u_impls, box, lamb_sched), md/minimizer.py:35-73, on the device: x_out [N, 3]
+ * after n_sched - 1 FIRE steps of the summed potentials. */
+int tm_fire_minimize(int N, const double *x0, const double *box, int n_bps, tm_bound_potential *const *bps,
+                     int n_sched, const double *lambda_sched, double *x_out);
+
 /* ---- integrator, barostat, context ------------------------------------ */
 
 /* LangevinIntegrator(dt, ca, cbs, ccs, seed), wrap_kernels.cpp:314-332, integrator.cu:19-29 */
@@ -230,6 +237,14 @@ int tm_batch_multiple_steps_U(tm_batch *b, int n_steps, int n_windows, const dou
                               int x_interval, double *u_out, double *x_out, double *box_out, int use_graph);
 /* u and du/dlambda of every system at its current state. */
 int tm_batch_energies(tm_batch *b, double *u_out, double *du_dl_out);
+/* fire_minimize (md/minimizer.py:35-73, _vendored/fire.py:60-140) of every system flagged in
+ * active ([n] or NULL: all) on the device, in place: the FIRE state starts from the force at
+ * lambda_sched[s, 0], then one step per further entry; lambda_sched [n, n_sched] or NULL (the
+ * current lambdas, n_sched - 1 steps). Velocities, boxes and step counters are not changed. */
+int tm_batch_fire_minimize(tm_batch *b, int n_sched, const double *lambda_sched, const int32_t *active);
+/* Largest per-atom norm of the total force of every system at its current state and lambda
+ * (the sanity check after minimisation, fe/estimator.py:245-249, on the summed force). */
+int tm_batch_max_force(tm_batch *b, double *out);
 int tm_batch_get_x(tm_batch *b, int s, double *out);
 int tm_batch_get_v(tm_batch *b, int s, double *out);
 int tm_batch_get_box(tm_batch *b, int s, double *out);

@@ -3,7 +3,11 @@
 // before anything runs on a GPU. Built by tests/test_host_math.py with g++.
 #include <cstdint>
 
+#include <cmath>
+#include <vector>
+
 #include "bonded_terms.h"
+#include "fire.h"
 #include "fixed_point.h"
 #include "nb_pair.h"
 #include "philox.h"
@@ -107,5 +111,46 @@ extern "C" void hc_nonbonded_du_dx(int N, const double *x, const double *params,
             du_dx[3 * j + d] -= f[d];
         }
         u[0] += f64_to_fixed(t.u, FIXED_SCALE);
+    }
+}
+
+extern "C" float hc_half_round(float x) { return half_round(x); }
+
+// FIRE on F = -k (x - c) with the step functions of fire.h and sequential
+// sums; n_sched - 1 steps. Writes the final positions and, per step, dt,
+// alpha and n_pos.
+extern "C" void hc_fire_harmonic(int n3, const double *x0, const double *c, double k, int n_sched, double *x_out,
+                                 float *dt_out, float *alpha_out, int *npos_out) {
+    std::vector<double> x(x0, x0 + n3), v(n3, 0.0), f(n3);
+    for (int i = 0; i < n3; i++) {
+        f[i] = -k * (x[i] - c[i]);
+    }
+    float dt = FIRE_DT_START, alpha = FIRE_ALPHA_START;
+    int npos = 0;
+    for (int step = 1; step < n_sched; step++) {
+        for (int i = 0; i < n3; i++) {
+            x[i] = fire_shift(x[i], v[i], f[i], dt);
+        }
+        double sf = 0.0, sv = 0.0, sp = 0.0;
+        for (int i = 0; i < n3; i++) {
+            const double fn = -k * (x[i] - c[i]);
+            v[i] = fire_velocity(v[i], f[i], fn, dt);
+            f[i] = fn;
+            sf += fn * fn;
+            sv += v[i] * v[i];
+            sp += fn * v[i];
+        }
+        const double f_norm = std::sqrt(sf + static_cast<double>(FIRE_F_EPS));
+        const double v_norm = std::sqrt(sv);
+        for (int i = 0; i < n3; i++) {
+            v[i] = fire_mix(v[i], f[i], v_norm, f_norm, alpha, sp);
+        }
+        fire_adapt(sp, dt, alpha, npos);
+        dt_out[step - 1] = dt;
+        alpha_out[step - 1] = alpha;
+        npos_out[step - 1] = npos;
+    }
+    for (int i = 0; i < n3; i++) {
+        x_out[i] = x[i];
     }
 }

@@ -106,6 +106,9 @@ _SIGS = {
     "tm_batch_total_atoms": (c_int, [_P, POINTER(c_int)]),
     "tm_batch_multiple_steps": (c_int, [_P, c_int, _PD, c_int, c_int, _PD, _PD, _PD, c_int]),
     "tm_batch_energies": (c_int, [_P, _PD, _PD]),
+    "tm_batch_fire_minimize": (c_int, [_P, c_int, _PD, _PI]),
+    "tm_batch_max_force": (c_int, [_P, _PD]),
+    "tm_fire_minimize": (c_int, [c_int, _PD, _PD, c_int, _PPV, c_int, _PD, _PD]),
     "tm_batch_get_x": (c_int, [_P, c_int, _PD]),
     "tm_batch_get_v": (c_int, [_P, c_int, _PD]),
     "tm_batch_get_box": (c_int, [_P, c_int, _PD]),

@@ -836,6 +836,43 @@ int tm_batch_energies(tm_batch *b, double *u_out, double *du_dl_out) {
     return guarded([&] { b->eng->energies(u_out, du_dl_out); });
 }
 
+int tm_batch_fire_minimize(tm_batch *b, int n_sched, const double *lambda_sched, const int32_t *active) {
+    return guarded([&] {
+        std::vector<int> act;
+        if (active) {
+            act.assign(active, active + b->eng->num_systems());
+        }
+        b->eng->fire_minimize(n_sched, lambda_sched, active ? act.data() : nullptr);
+    });
+}
+
+int tm_batch_max_force(tm_batch *b, double *out) {
+    return guarded([&] { b->eng->max_force(out); });
+}
+
+int tm_fire_minimize(int N, const double *x0, const double *box, int n_bps, tm_bound_potential *const *bps,
+                     int n_sched, const double *lambda_sched, double *x_out) {
+    return guarded([&] {
+        require(x0, "x0");
+        require(box, "box");
+        require(lambda_sched, "lamb_sched");
+        require(x_out, "x_out");
+        SystemSpec sp;
+        sp.N = N;
+        sp.x.assign(x0, x0 + N * 3);
+        sp.v.assign(static_cast<size_t>(N) * 3, 0.0);
+        sp.box.assign(box, box + 9);
+        sp.integ.cbs.assign(N, 0.0); // no integrator steps are taken
+        sp.integ.ccs.assign(N, 0.0);
+        for (int i = 0; i < n_bps; i++) {
+            sp.bps.push_back(bps[i]->p);
+        }
+        Engine e({sp});
+        e.fire_minimize(n_sched, lambda_sched, nullptr);
+        e.get_x(0, x_out);
+    });
+}
+
 int tm_batch_get_x(tm_batch *b, int s, double *out) {
     return guarded([&] { b->eng->get_x(s, out); });
 }

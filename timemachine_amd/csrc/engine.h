@@ -89,6 +89,15 @@ class Engine {
     // Energies of every system at its current state and lambda (u, du_dl).
     void energies(double *u_out, double *du_dl_out);
 
+    // FIRE minimisation of the coordinates of every active system
+    // (md/minimizer.py:35-73: state from the force at schedule entry 0, then one
+    // step per further entry). lambda_sched: [S, n_sched] host array, or
+    // nullptr for the current lambdas; active: [S] flags or nullptr (all).
+    // Velocities, box and step counters are left alone.
+    void fire_minimize(int n_sched, const double *lambda_sched, const int *active);
+    // Largest per-atom force norm of every system at its current state and lambda.
+    void max_force(double *out);
+
     // Gradient observer for lambda-window averages of A = du/dlambda. Every
     // `interval` steps of run() (0: off), at the states where run() stores
     // du_dl with the same interval (start of run step j, (j + 1) % interval

@@ -5,6 +5,7 @@
 #include <map>
 
 #include "engine.h"
+#include "fire.h"
 #include "fixed_point.h"
 #include "philox.h"
 
@@ -114,6 +115,127 @@ __global__ void k_store_frames2(int A, const int *__restrict__ atom_sys, const i
 }
 
 // Energies at other lambda windows: every system evaluates window w.
+// ---- FIRE minimisation (md/minimizer.py:35-73, _vendored/fire.py:110-140; arithmetic in fire.h)
+
+__global__ void k_fire_init(int A3, const int *__restrict__ atom_sys, const int *__restrict__ active,
+                            const uint64_t *__restrict__ du_dx, double *__restrict__ fv, double *__restrict__ ff) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A3 || !active[atom_sys[i / 3]]) {
+        return;
+    }
+    fv[i] = 0.0;
+    ff[i] = -fixed_to_double(du_dx[i], FIXED_SCALE);
+}
+
+__global__ void k_fire_state_init(int S, float *__restrict__ dt, float *__restrict__ alpha, int *__restrict__ npos) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < S) {
+        dt[s] = FIRE_DT_START;
+        alpha[s] = FIRE_ALPHA_START;
+        npos[s] = 0;
+    }
+}
+
+__global__ void k_fire_move(int A3, const int *__restrict__ atom_sys, const int *__restrict__ active,
+                            const float *__restrict__ dt, const double *__restrict__ fv, const double *__restrict__ ff,
+                            double *__restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A3) {
+        return;
+    }
+    const int s = atom_sys[i / 3];
+    if (!active[s]) {
+        return;
+    }
+    x[i] = fire_shift(x[i], fv[i], ff[i], dt[s]);
+}
+
+constexpr int FIRE_THREADS = 1024;
+
+// One workgroup per system: new forces, half-step velocities, the three sums
+// (fixed tree order, so a system gives the same bits alone or in a batch),
+// velocity mixing, then dt / alpha / n_pos.
+__global__ __launch_bounds__(FIRE_THREADS) void k_fire_update(
+    const int *__restrict__ atom0, const int *__restrict__ natoms, const int *__restrict__ active,
+    const uint64_t *__restrict__ du_dx, double *__restrict__ fv, double *__restrict__ ff, float *__restrict__ dt,
+    float *__restrict__ alpha, int *__restrict__ npos) {
+    const int s = blockIdx.x;
+    if (!active[s]) {
+        return;
+    }
+    __shared__ double red[3][FIRE_THREADS];
+    __shared__ double stats[3];
+    const int t = threadIdx.x;
+    const int base = atom0[s] * 3;
+    const int n3 = natoms[s] * 3;
+    const float dts = dt[s];
+    double sf = 0.0, sv = 0.0, sp = 0.0;
+    for (int q = t; q < n3; q += FIRE_THREADS) {
+        const double f = -fixed_to_double(du_dx[base + q], FIXED_SCALE);
+        const double v = fire_velocity(fv[base + q], ff[base + q], f, dts);
+        fv[base + q] = v;
+        ff[base + q] = f;
+        sf += f * f;
+        sv += v * v;
+        sp += f * v;
+    }
+    red[0][t] = sf;
+    red[1][t] = sv;
+    red[2][t] = sp;
+    __syncthreads();
+    for (int stride = FIRE_THREADS / 2; stride > 0; stride >>= 1) {
+        if (t < stride) {
+            red[0][t] += red[0][t + stride];
+            red[1][t] += red[1][t + stride];
+            red[2][t] += red[2][t + stride];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        stats[0] = std::sqrt(red[0][0] + static_cast<double>(FIRE_F_EPS)); // |F|
+        stats[1] = std::sqrt(red[1][0]);                                    // |V|
+        stats[2] = red[2][0];                                               // P
+    }
+    __syncthreads();
+    const double f_norm = stats[0], v_norm = stats[1], P = stats[2];
+    const float a = alpha[s];
+    for (int q = t; q < n3; q += FIRE_THREADS) {
+        fv[base + q] = fire_mix(fv[base + q], ff[base + q], v_norm, f_norm, a, P);
+    }
+    if (t == 0) {
+        float d = dts, al = a;
+        int np = npos[s];
+        fire_adapt(P, d, al, np);
+        dt[s] = d;
+        alpha[s] = al;
+        npos[s] = np;
+    }
+}
+
+__global__ void k_max_force(const int *__restrict__ atom0, const int *__restrict__ natoms,
+                            const uint64_t *__restrict__ du_dx, double *__restrict__ out) {
+    const int s = blockIdx.x;
+    __shared__ double red[256];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < natoms[s]; i += 256) {
+        const uint64_t *g = du_dx + (atom0[s] + i) * 3;
+        const double fx = fixed_to_double(g[0], FIXED_SCALE), fy = fixed_to_double(g[1], FIXED_SCALE),
+                     fz = fixed_to_double(g[2], FIXED_SCALE);
+        m = fmax(m, std::sqrt(fx * fx + fy * fy + fz * fz));
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int stride = 128; stride > 0; stride >>= 1) {
+        if (static_cast<int>(threadIdx.x) < stride) {
+            red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + stride]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[s] = red[0];
+    }
+}
+
 __global__ void k_set_eval_lambda(int S, const double *__restrict__ windows, int w, double *__restrict__ lam_eval) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < S) {
@@ -1250,6 +1372,75 @@ void Engine::grad_stats(int s, int *n_frames, double *sum_a, double *sum_aa, dou
     TM_HIP_CHECK(hipMemcpyAsync(sum_a, I.obs_sum_a.ptr + s, sizeof(double), hipMemcpyDeviceToHost, stream_));
     TM_HIP_CHECK(hipMemcpyAsync(sum_aa, I.obs_sum_aa.ptr + s, sizeof(double), hipMemcpyDeviceToHost, stream_));
     TM_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void Engine::fire_minimize(int n_sched, const double *lambda_sched, const int *active) {
+    Impl &I = *impl_;
+    if (n_sched < 1) {
+        throw Error("fire_minimize: the lambda schedule needs at least one entry");
+    }
+    const int A3 = A_ * 3;
+    // schedule, step-major on the device: entry k of system s at k * S + s
+    std::vector<double> sched(static_cast<size_t>(n_sched) * S_);
+    std::vector<double> cur(S_);
+    if (!lambda_sched) {
+        I.lam.download(cur.data(), S_, stream_);
+        TM_HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+    for (int k = 0; k < n_sched; k++) {
+        for (int s = 0; s < S_; s++) {
+            sched[static_cast<size_t>(k) * S_ + s] = lambda_sched ? lambda_sched[static_cast<size_t>(s) * n_sched + k]
+                                                                  : cur[s];
+        }
+    }
+    std::vector<int> act(S_, 1);
+    if (active) {
+        for (int s = 0; s < S_; s++) {
+            act[s] = active[s] ? 1 : 0;
+        }
+    }
+    DeviceArray<double> d_sched, fv, ff;
+    DeviceArray<float> dt, alpha;
+    DeviceArray<int> npos, d_act;
+    upload_vec(d_sched, sched);
+    upload_vec(d_act, act);
+    fv.alloc(std::max(A3, 1));
+    ff.alloc(std::max(A3, 1));
+    dt.alloc(S_);
+    alpha.alloc(S_);
+    npos.alloc(S_);
+    const int TPB = 256;
+    const int gA = ceil_div(std::max(A3, 1), TPB);
+    k_fire_state_init<<<ceil_div(S_, 64), 64, 0, stream_>>>(S_, dt.ptr, alpha.ptr, npos.ptr);
+    TM_LAUNCH_CHECK();
+    launch_fill_u64(I.du_dx.ptr, A3, 0, stream_);
+    I.eval(I.x.ptr, I.box.ptr, I.du_dx.ptr, nullptr, nullptr, stream_, tile_grid, d_sched.ptr);
+    k_fire_init<<<gA, TPB, 0, stream_>>>(A3, I.atom_sys.ptr, d_act.ptr, I.du_dx.ptr, fv.ptr, ff.ptr);
+    TM_LAUNCH_CHECK();
+    for (int k = 1; k < n_sched; k++) {
+        k_fire_move<<<gA, TPB, 0, stream_>>>(A3, I.atom_sys.ptr, d_act.ptr, dt.ptr, fv.ptr, ff.ptr, I.x.ptr);
+        TM_LAUNCH_CHECK();
+        launch_fill_u64(I.du_dx.ptr, A3, 0, stream_);
+        I.eval(I.x.ptr, I.box.ptr, I.du_dx.ptr, nullptr, nullptr, stream_, tile_grid,
+               d_sched.ptr + static_cast<size_t>(k) * S_);
+        k_fire_update<<<S_, FIRE_THREADS, 0, stream_>>>(I.d_atom0.ptr, I.d_natoms.ptr, d_act.ptr, I.du_dx.ptr,
+                                                        fv.ptr, ff.ptr, dt.ptr, alpha.ptr, npos.ptr);
+        TM_LAUNCH_CHECK();
+    }
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+    check_status();
+}
+
+void Engine::max_force(double *out) {
+    Impl &I = *impl_;
+    launch_fill_u64(I.du_dx.ptr, A_ * 3, 0, stream_);
+    I.eval(I.x.ptr, I.box.ptr, I.du_dx.ptr, nullptr, nullptr, stream_, tile_grid);
+    DeviceArray<double> d(S_);
+    k_max_force<<<S_, 256, 0, stream_>>>(I.d_atom0.ptr, I.d_natoms.ptr, I.du_dx.ptr, d.ptr);
+    TM_LAUNCH_CHECK();
+    d.download(out, S_, stream_);
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+    check_status();
 }
 
 void Engine::energies(double *u_out, double *du_dl_out) {

@@ -937,6 +937,34 @@ class BatchedContexts(_Handle):
         call("tm_batch_energies", self._h, pd(u), pd(dl))
         return u, dl
 
+    def fire_minimize(self, lambda_schedules=None, n_steps=None, active=None):
+        """FIRE minimisation of every system (or those with ``active`` set) on the
+        device, in place (md/minimizer.py:35-73 for each system at once):
+        ``lambda_schedules`` [S, n_sched] (the state starts at entry 0, one step per
+        further entry), or ``n_steps`` steps at the current lambdas."""
+        S = len(self._ctxs)
+        if lambda_schedules is not None:
+            sched = f64(lambda_schedules).reshape(S, -1)
+            n_sched = sched.shape[1]
+        else:
+            if n_steps is None:
+                raise RuntimeError("give lambda_schedules or n_steps")
+            sched = None
+            n_sched = int(n_steps) + 1
+        act = None
+        if active is not None:
+            act = i32(np.asarray(active, dtype=bool).astype(np.int32)).reshape(-1)
+            if act.size != S:
+                raise RuntimeError("one active flag per system is required")
+        call("tm_batch_fire_minimize", self._h, n_sched, pd(sched), pi(act))
+
+    def max_force(self):
+        """Largest per-atom norm of the total force of every system at its current
+        state and lambda."""
+        out = np.zeros(len(self._ctxs))
+        call("tm_batch_max_force", self._h, pd(out))
+        return out
+
     def get_x_t(self, s):
         out = np.zeros((self.sizes[s], 3))
         call("tm_batch_get_x", self._h, int(s), pd(out))
