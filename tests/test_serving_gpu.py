@@ -84,3 +84,25 @@ def test_served_mix_equals_each_job_alone(jobs):
     assert max(rec["systems"] for rec in srv.log) < sum(j.windows for j in jobs)  # never all at once
     stats = serving.latency_stats(served)
     assert stats["p99_s"] >= stats["p50_s"] > 0
+
+
+def test_minimized_windows_served_equal_fifo(jobs):
+    """Jobs whose windows are FIRE-minimised at their lambdas first
+    (fe/estimator.py:242-249): the start coordinates differ from the unminimised
+    ones, and the served mix still equals FIFO bit for bit."""
+    import dataclasses
+
+    mjobs = [dataclasses.replace(j, minimize_steps=20) for j in jobs[:3]]
+    wins = serving.make_window_contexts(mjobs[0])
+    x_min = wins[0][0].get_x_t()
+    assert not np.array_equal(x_min, mjobs[0].system.x)
+    assert not np.array_equal(wins[0][0].get_x_t(), wins[-1][0].get_x_t())  # each window at its own lambda
+    budget = sum(serving.estimate_system_bytes(j.system.N) * j.windows for j in mjobs[:2]) + 1
+    srv = serving.Server(budget, max_chunk=30, du_dl_interval=10, prepare_ahead=2)  # prepared on a second thread
+    served = srv.run(mjobs)
+    inline = serving.Server(budget, max_chunk=30, du_dl_interval=10).run(mjobs)
+    alone = serving.run_fifo(mjobs, du_dl_interval=10)
+    for r, q, z in zip(served, alone, inline):
+        assert np.array_equal(r.du_dl, q.du_dl), r.job_id
+        assert np.array_equal(z.du_dl, q.du_dl), r.job_id
+    assert srv.setup_seconds > 0

@@ -422,6 +422,7 @@ int tm_neighborlist_reset_row_idxs(tm_neighborlist *nl) {
 int tm_neighborlist_build_host(tm_neighborlist *nl, int N, const double *x, const double *box, double cutoff,
                                int *n_tiles) {
     return guarded([&] {
+        DeviceLock lock; // synchronous copies on the null stream
         if (N != nl->n) {
             throw Error("N != N_");
         }
@@ -472,6 +473,7 @@ int tm_neighborlist_build_host(tm_neighborlist *nl, int N, const double *x, cons
 
 int tm_neighborlist_get_tiles(tm_neighborlist *nl, int capacity, int32_t *tile_rows, uint32_t *tile_atoms) {
     return guarded([&] {
+        DeviceLock lock; // synchronous copies on the null stream
         if (!nl->ready) {
             throw Error("Neighborlist: nothing built");
         }
@@ -498,6 +500,7 @@ int tm_neighborlist_get_tiles(tm_neighborlist *nl, int capacity, int32_t *tile_r
 int tm_neighborlist_compute_block_bounds_host(tm_neighborlist *nl, int N, const double *x, const double *box,
                                               double *ctr, double *ext) {
     return guarded([&] {
+        DeviceLock lock; // synchronous copies on the null stream
         int n_tiles = 0;
         const double cutoff = nl->ready ? nl->built_cutoff : 0.0;
         int rc = tm_neighborlist_build_host(nl, N, x, box, cutoff, &n_tiles);

@@ -212,6 +212,22 @@ __global__ __launch_bounds__(FIRE_THREADS) void k_fire_update(
     }
 }
 
+// Lambdas of FIRE step `*done + 1` (the schedule is step-major, [n_sched][S]).
+__global__ void k_fire_lambda(int S, int n_sched, const double *__restrict__ sched, const int *__restrict__ done,
+                              double *__restrict__ lam_cur) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < S) {
+        const int k = min(*done + 1, n_sched - 1);
+        lam_cur[s] = sched[static_cast<size_t>(k) * S + s];
+    }
+}
+
+__global__ void k_fire_count(int *__restrict__ done) {
+    if (threadIdx.x == 0) {
+        *done += 1;
+    }
+}
+
 __global__ void k_max_force(const int *__restrict__ atom0, const int *__restrict__ natoms,
                             const uint64_t *__restrict__ du_dx, double *__restrict__ out) {
     const int s = blockIdx.x;
@@ -661,6 +677,7 @@ struct Engine::Impl {
 };
 
 Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
+    DeviceLock lock; // allocations, null-stream uploads, stream creation
     Impl &I = *impl_;
     S_ = static_cast<int>(systems.size());
     if (S_ == 0) {
@@ -1016,14 +1033,16 @@ Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
         upload_vec(I.atom_mol, amol);
         I.centroids.alloc(std::max(I.nmol_total * 3, 1));
     }
-    TM_HIP_CHECK(hipDeviceSynchronize());
+    device_synchronize();
 }
 
 Engine::~Engine() {
+    DeviceLock lock;
     if (stream_) {
         (void)hipStreamSynchronize(stream_);
         (void)hipStreamDestroy(stream_);
     }
+    impl_.reset();
 }
 
 void Engine::set_lambdas(const std::vector<double> &lams) {
@@ -1169,6 +1188,7 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
             }
             auto it = cache.find(pattern);
             if (it == cache.end()) {
+                DeviceLock lock; // no other thread may allocate or synchronise the device meanwhile
                 hipGraph_t g;
                 TM_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
                 for (int q = 0; q < chunk; q++) {
@@ -1184,6 +1204,7 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
             j += chunk;
         }
         TM_HIP_CHECK(hipStreamSynchronize(stream_));
+        DeviceLock lock;
         for (auto &kv : cache) {
             TM_HIP_CHECK(hipGraphExecDestroy(kv.second));
         }
@@ -1321,7 +1342,7 @@ void Engine::set_grad_observer(int interval) {
     I.obs_sum_a.zero();
     I.obs_sum_aa.zero();
     I.obs_n.zero();
-    TM_HIP_CHECK(hipDeviceSynchronize());
+    device_synchronize();
 }
 
 void Engine::observe() {
@@ -1417,15 +1438,38 @@ void Engine::fire_minimize(int n_sched, const double *lambda_sched, const int *a
     I.eval(I.x.ptr, I.box.ptr, I.du_dx.ptr, nullptr, nullptr, stream_, tile_grid, d_sched.ptr);
     k_fire_init<<<gA, TPB, 0, stream_>>>(A3, I.atom_sys.ptr, d_act.ptr, I.du_dx.ptr, fv.ptr, ff.ptr);
     TM_LAUNCH_CHECK();
-    for (int k = 1; k < n_sched; k++) {
-        k_fire_move<<<gA, TPB, 0, stream_>>>(A3, I.atom_sys.ptr, d_act.ptr, dt.ptr, fv.ptr, ff.ptr, I.x.ptr);
-        TM_LAUNCH_CHECK();
-        launch_fill_u64(I.du_dx.ptr, A3, 0, stream_);
-        I.eval(I.x.ptr, I.box.ptr, I.du_dx.ptr, nullptr, nullptr, stream_, tile_grid,
-               d_sched.ptr + static_cast<size_t>(k) * S_);
-        k_fire_update<<<S_, FIRE_THREADS, 0, stream_>>>(I.d_atom0.ptr, I.d_natoms.ptr, d_act.ptr, I.du_dx.ptr,
-                                                        fv.ptr, ff.ptr, dt.ptr, alpha.ptr, npos.ptr);
-        TM_LAUNCH_CHECK();
+    if (n_sched > 1) {
+        // one step as a graph, replayed: the step's lambdas come from a device
+        // step counter, so every replay is the same launch sequence
+        DeviceArray<int> done(1);
+        DeviceArray<double> lam_cur(S_);
+        done.zero(stream_);
+        hipGraph_t g;
+        hipGraphExec_t ge;
+        {
+            DeviceLock lock;
+            TM_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            k_fire_move<<<gA, TPB, 0, stream_>>>(A3, I.atom_sys.ptr, d_act.ptr, dt.ptr, fv.ptr, ff.ptr, I.x.ptr);
+            TM_LAUNCH_CHECK();
+            launch_fill_u64(I.du_dx.ptr, A3, 0, stream_);
+            k_fire_lambda<<<ceil_div(S_, 64), 64, 0, stream_>>>(S_, n_sched, d_sched.ptr, done.ptr, lam_cur.ptr);
+            TM_LAUNCH_CHECK();
+            I.eval(I.x.ptr, I.box.ptr, I.du_dx.ptr, nullptr, nullptr, stream_, tile_grid, lam_cur.ptr);
+            k_fire_update<<<S_, FIRE_THREADS, 0, stream_>>>(I.d_atom0.ptr, I.d_natoms.ptr, d_act.ptr, I.du_dx.ptr,
+                                                            fv.ptr, ff.ptr, dt.ptr, alpha.ptr, npos.ptr);
+            TM_LAUNCH_CHECK();
+            k_fire_count<<<1, 64, 0, stream_>>>(done.ptr);
+            TM_LAUNCH_CHECK();
+            TM_HIP_CHECK(hipStreamEndCapture(stream_, &g));
+            TM_HIP_CHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+            TM_HIP_CHECK(hipGraphDestroy(g));
+        }
+        for (int k = 1; k < n_sched; k++) {
+            TM_HIP_CHECK(hipGraphLaunch(ge, stream_));
+        }
+        TM_HIP_CHECK(hipStreamSynchronize(stream_));
+        DeviceLock lock;
+        TM_HIP_CHECK(hipGraphExecDestroy(ge));
     }
     TM_HIP_CHECK(hipStreamSynchronize(stream_));
     check_status();

@@ -781,7 +781,7 @@ void NbState::init(const std::vector<NbSys> &sys_in, const std::vector<int> &sub
     nblist_x.zero();
     nblist_box.zero();
     launch_fill_i32(rebuild.ptr, S, 1, 0);
-    TM_HIP_CHECK(hipDeviceSynchronize());
+    device_synchronize();
 }
 
 void NbState::invalidate(hipStream_t stream) { launch_fill_i32(rebuild.ptr, S, 1, stream); }

@@ -591,8 +591,8 @@ inline int kernel_index(const void *du_dx, const void *du_dp, const void *du_dl,
 }
 
 int default_tile_grid() {
-    static int grid = 0;
-    if (grid == 0) {
+    // initialised once, thread-safe (engines may be built from several host threads)
+    static const int grid = [] {
         int dev = 0;
         TM_HIP_CHECK(hipGetDevice(&dev));
         int cus = 0;
@@ -600,8 +600,8 @@ int default_tile_grid() {
         // 64 two-wave workgroups per CU: LDS and VGPRs keep 8 resident per CU, the
         // rest queue behind them, which balances the uneven per-tile pair counts
         // better than a fully persistent grid (measured on MI355X: -15%)
-        grid = cus * 64;
-    }
+        return cus * 64;
+    }();
     return grid;
 }
 

@@ -170,7 +170,7 @@ NonbondedAllPairs::NonbondedAllPairs(const std::vector<int> &plane, const std::v
     d_offset_.alloc(std::max(N_, 1));
     d_plane_.upload(plane.data(), N_);
     d_offset_.upload(offset.data(), N_);
-    TM_HIP_CHECK(hipDeviceSynchronize());
+    device_synchronize();
 }
 
 void NonbondedAllPairs::set_nblist_padding(double padding) {
@@ -368,7 +368,7 @@ NonbondedPairList::NonbondedPairList(const std::vector<int> &pair_idxs, const st
     d_scales_.upload(scales.data(), M_ * 2);
     d_plane_.upload(plane.data(), N_);
     d_offset_.upload(offset.data(), N_);
-    TM_HIP_CHECK(hipDeviceSynchronize());
+    device_synchronize();
 }
 
 void NonbondedPairList::execute_dl_dp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box,
@@ -481,7 +481,7 @@ BondedPotential::BondedPotential(BondedKind bk, const std::vector<int> &idxs, co
     d_idxs_.upload(idxs.data(), idxs.size());
     d_mult_.upload(mult_.data(), n_);
     d_offset_.upload(offset_.data(), n_);
-    TM_HIP_CHECK(hipDeviceSynchronize());
+    device_synchronize();
 }
 
 PotKind BondedPotential::kind() const {
@@ -621,7 +621,7 @@ BoundPotential::BoundPotential(PotentialPtr p, const std::vector<int> &shp, cons
     h_params.assign(h_p, h_p + n);
     d_params.alloc(std::max(n, 1));
     d_params.upload(h_params.data(), n);
-    TM_HIP_CHECK(hipDeviceSynchronize());
+    device_synchronize();
 }
 
 int BoundPotential::size() const {

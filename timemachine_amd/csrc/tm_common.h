@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -33,6 +34,24 @@ struct HipError : public std::runtime_error {
 #define TM_LAUNCH_CHECK() TM_HIP_CHECK(hipGetLastError())
 
 inline int ceil_div(long long a, long long b) { return static_cast<int>((a + b - 1) / b); }
+
+// Several host threads may drive engines at once (a serving thread stepping
+// a batch while another prepares the next job on its own stream). HIP
+// invalidates a stream capture in one thread when another thread allocates,
+// frees, synchronises the device or uses the null stream meanwhile, so those
+// operations and every capture hold this lock (recursive: a holder may nest).
+inline std::recursive_mutex &device_mutex() {
+    static std::recursive_mutex m;
+    return m;
+}
+struct DeviceLock {
+    std::lock_guard<std::recursive_mutex> guard{device_mutex()};
+};
+
+inline void device_synchronize() {
+    DeviceLock lock;
+    TM_HIP_CHECK(hipDeviceSynchronize());
+}
 
 // Tile width of the neighborlist and of the pair kernel: one wave64 per tile.
 constexpr int TILE = 64;
@@ -64,6 +83,7 @@ template <typename T> struct DeviceArray {
     }
     ~DeviceArray() { release(); }
     void alloc(size_t count) {
+        DeviceLock lock;
         release();
         n = count;
         if (count > 0) {
@@ -72,6 +92,7 @@ template <typename T> struct DeviceArray {
     }
     void release() {
         if (ptr) {
+            DeviceLock lock;
             (void)hipFree(ptr);
             ptr = nullptr;
         }
@@ -83,7 +104,12 @@ template <typename T> struct DeviceArray {
             throw Error("DeviceArray::upload overflow");
         }
         if (count) {
-            TM_HIP_CHECK(hipMemcpyAsync(ptr, host, count * sizeof(T), hipMemcpyHostToDevice, s));
+            if (s == nullptr) {
+                DeviceLock lock; // null stream
+                TM_HIP_CHECK(hipMemcpyAsync(ptr, host, count * sizeof(T), hipMemcpyHostToDevice, s));
+            } else {
+                TM_HIP_CHECK(hipMemcpyAsync(ptr, host, count * sizeof(T), hipMemcpyHostToDevice, s));
+            }
         }
     }
     void download(T *host, size_t count, hipStream_t s = 0) const {
@@ -91,12 +117,22 @@ template <typename T> struct DeviceArray {
             throw Error("DeviceArray::download overflow");
         }
         if (count) {
-            TM_HIP_CHECK(hipMemcpyAsync(host, ptr, count * sizeof(T), hipMemcpyDeviceToHost, s));
+            if (s == nullptr) {
+                DeviceLock lock; // null stream
+                TM_HIP_CHECK(hipMemcpyAsync(host, ptr, count * sizeof(T), hipMemcpyDeviceToHost, s));
+            } else {
+                TM_HIP_CHECK(hipMemcpyAsync(host, ptr, count * sizeof(T), hipMemcpyDeviceToHost, s));
+            }
         }
     }
     void zero(hipStream_t s = 0) {
         if (n) {
-            TM_HIP_CHECK(hipMemsetAsync(ptr, 0, n * sizeof(T), s));
+            if (s == nullptr) {
+                DeviceLock lock; // null stream
+                TM_HIP_CHECK(hipMemsetAsync(ptr, 0, n * sizeof(T), s));
+            } else {
+                TM_HIP_CHECK(hipMemsetAsync(ptr, 0, n * sizeof(T), s));
+            }
         }
     }
 };

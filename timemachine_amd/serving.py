@@ -34,12 +34,14 @@ from __future__ import annotations
 import dataclasses
 import math
 import time
+from concurrent.futures import Future, ThreadPoolExecutor
 from typing import Callable, Dict, List, Optional, Sequence
 
 import numpy as np
 
 from timemachine_amd import testsystems as ts
 from timemachine_amd.lib import LangevinIntegrator, MonteCarloBarostat, custom_ops
+from timemachine_amd.md.minimizer import FORCE_LIMIT
 
 TILE = 64
 
@@ -56,6 +58,9 @@ class Job:
     dt: float = 0.0025
     friction: float = 1.0
     barostat_interval: int = 15
+    # FIRE steps at each window's lambda before its MD (fe/estimator.py:242-249
+    # minimises with a schedule of 100 entries, i.e. 99 steps); 0 skips it
+    minimize_steps: int = 0
 
     @property
     def windows(self) -> int:
@@ -108,12 +113,15 @@ class _Window:
 
 
 def make_window_contexts(job: Job):
-    """One Context per lambda window of the job (seeded per job and window)."""
+    """One Context per lambda window of the job (seeded per job and window),
+    FIRE-minimised at its lambda when the job asks for it."""
     sys_ = job.system
     out = []
+    # the windows of a job share its bound potentials (same parameters; each
+    # engine copies them into its own arena)
+    bps = [p.bound_impl(np.float32) for p in sys_.bound_potentials()]
     for w in range(job.windows):
         seed = job.seed * 1009 + w
-        bps = [p.bound_impl(np.float32) for p in sys_.bound_potentials()]
         intg = LangevinIntegrator(job.temperature, job.dt, job.friction, sys_.masses, seed).impl()
         baro = None
         if job.barostat_interval > 0:
@@ -122,6 +130,16 @@ def make_window_contexts(job: Job):
         v0 = ts.maxwell_boltzmann(sys_.masses, job.temperature, seed)
         ctx = custom_ops.Context(sys_.x, v0, sys_.box, intg, bps, barostat=baro)
         out.append((ctx, (bps, intg, baro)))
+    if job.minimize_steps > 0:
+        # all windows of the job minimised together on the device, then the
+        # coordinates go back into the contexts
+        batch = custom_ops.BatchedContexts([c for c, _ in out], job.lambdas)
+        batch.fire_minimize(n_steps=job.minimize_steps)
+        worst = batch.max_force()
+        if not np.all(worst < FORCE_LIMIT):
+            raise AssertionError(f"job {job.job_id}: forces much greater than expected after minimization")
+        batch.write_back()
+        del batch
     return out
 
 
@@ -130,7 +148,7 @@ class Server:
 
     def __init__(self, hbm_budget_bytes: int, max_chunk: int = 600, du_dl_interval: int = 10,
                  use_graph: bool = True, clock: Callable[[], float] = time.perf_counter,
-                 sleep: Callable[[float], None] = time.sleep):
+                 sleep: Callable[[float], None] = time.sleep, prepare_ahead: int = 0):
         if du_dl_interval <= 0 or max_chunk < du_dl_interval:
             raise ValueError("need 0 < du_dl_interval <= max_chunk")
         self.budget = int(hbm_budget_bytes)
@@ -141,6 +159,21 @@ class Server:
         self._sleep = sleep
         self.log: List[dict] = []  # one record per chunk (resident systems, steps, duration)
         self.repack_seconds = 0.0  # time spent building engines at admissions and retirements
+        # window contexts of arrived jobs, with their FIRE minimisation: built at
+        # admission on the serving thread (prepare_ahead = 0), or on a second host
+        # thread (its own engine and HIP stream) while the batch runs, at most
+        # prepare_ahead jobs ahead. The second thread overlaps host work only:
+        # allocations and frees synchronise the whole device, so its GPU work
+        # waits for the running chunk.
+        self.prepare_ahead = int(prepare_ahead)
+        self.setup_seconds = 0.0  # total preparation time (overlapped with chunks when prepare_ahead > 0)
+        self.setup_wait_seconds = 0.0  # time the serving thread waited for a preparation
+
+    def _prepare(self, job: Job):
+        t = time.perf_counter()
+        wins = make_window_contexts(job)
+        self.setup_seconds += time.perf_counter() - t
+        return wins
 
     def _job_bytes(self, job: Job) -> int:
         return job.windows * estimate_system_bytes(job.system.N)
@@ -159,61 +192,91 @@ class Server:
         used = 0
         admitted_at: Dict[int, float] = {}
         results: List[JobResult] = []
-        while pending or running:
-            # admit, first come first served, while the budget allows
-            new: List[_Window] = []
-            while pending and pending[0].arrival <= now() and used + self._job_bytes(pending[0]) <= self.budget:
-                job = pending.pop(0)
-                used += self._job_bytes(job)
-                admitted_at[job.job_id] = now()
-                for w, (ctx, keep) in enumerate(make_window_contexts(job)):
-                    new.append(_Window(job, w, ctx, keep))
-            if not running and not new:
-                # idle until the next arrival
-                self._sleep(max(0.0, pending[0].arrival - now()))
-                continue
-            if new:
-                tr = now()
-                lam = [r.job.lambdas[r.w] for r in running] + [n.job.lambdas[n.w] for n in new]
-                if batch is None:
-                    batch = custom_ops.BatchedContexts([n.ctx for n in new], lam)
-                else:
-                    batch = batch.repack(range(len(running)), [n.ctx for n in new], lam)
-                running += new
-                self.repack_seconds += now() - tr
-            chunk = min(self.max_chunk, min(r.remaining for r in running))
-            tc = now()
-            du_dl, _, _ = batch.multiple_steps(chunk, store_du_dl_interval=self.du_dl_interval,
-                                               use_graph=self.use_graph)
-            self.log.append({"t": tc, "systems": len(running), "atoms": int(sum(batch.sizes)), "steps": chunk,
-                             "seconds": now() - tc, "budget_used": used, "device_bytes": batch.device_bytes()})
-            for s, r in enumerate(running):
-                r.chunks.append(du_dl[s])
-                r.remaining -= chunk
-            done_jobs = {r.job.job_id for r in running if r.remaining == 0}
-            if done_jobs:
-                t_fin = now()
-                by_job: Dict[int, List[_Window]] = {}
-                for r in running:
-                    if r.job.job_id in done_jobs:
-                        by_job.setdefault(r.job.job_id, []).append(r)
-                for jid, ws in by_job.items():
-                    ws.sort(key=lambda r: r.w)
-                    job = ws[0].job
-                    series = np.stack([np.concatenate(r.chunks) for r in ws])
-                    results.append(JobResult(jid, series, ti_estimate(job.lambdas, series), job.arrival,
-                                             admitted_at[jid], t_fin))
-                    used -= self._job_bytes(job)
-                keep = [i for i, r in enumerate(running) if r.job.job_id not in done_jobs]
-                tr = now()
-                if keep:
-                    lam = [running[i].job.lambdas[running[i].w] for i in keep]
-                    batch = batch.repack(keep, [], lam)
-                else:
-                    batch = None
-                self.repack_seconds += now() - tr
-                running = [running[i] for i in keep]
+        pool = ThreadPoolExecutor(max_workers=1) if self.prepare_ahead > 0 else None
+        prepared: Dict[int, Future] = {}
+        try:
+            while pending or running:
+                batch, used = self._serve_once(pending, running, batch, used, admitted_at, results, pool, prepared,
+                                               now)
+        finally:
+            if pool is not None:
+                pool.shutdown(wait=True)
         return sorted(results, key=lambda r: r.job_id)
+
+    def _serve_once(self, pending, running, batch, used, admitted_at, results, pool, prepared, now):
+        """Admit what fits, run one chunk, retire finished jobs."""
+        if pool is not None:
+            # prepare arrived jobs in admission order, a few ahead
+            for job in pending[: self.prepare_ahead]:
+                if job.arrival <= now() and job.job_id not in prepared:
+                    prepared[job.job_id] = pool.submit(self._prepare, job)
+        # admit, first come first served, while the budget allows
+        new: List[_Window] = []
+        while pending and pending[0].arrival <= now() and used + self._job_bytes(pending[0]) <= self.budget:
+            job = pending[0]
+            if pool is not None:
+                fut = prepared.get(job.job_id)
+                if fut is None:
+                    fut = prepared[job.job_id] = pool.submit(self._prepare, job)
+                if not fut.done() and (running or new):
+                    break  # keep the batch busy while the job is prepared
+                tw = now()
+                wins = fut.result()
+                self.setup_wait_seconds += now() - tw
+                del prepared[job.job_id]
+            else:
+                wins = self._prepare(job)
+            pending.pop(0)
+            used += self._job_bytes(job)
+            admitted_at[job.job_id] = now()
+            for w, (ctx, keep) in enumerate(wins):
+                new.append(_Window(job, w, ctx, keep))
+        if not running and not new:
+            # idle until the next arrival
+            self._sleep(max(0.0, pending[0].arrival - now()))
+            return batch, used
+        if new:
+            tr = now()
+            lam = [r.job.lambdas[r.w] for r in running] + [n.job.lambdas[n.w] for n in new]
+            if batch is None:
+                batch = custom_ops.BatchedContexts([n.ctx for n in new], lam)
+            else:
+                batch = batch.repack(range(len(running)), [n.ctx for n in new], lam)
+            running += new
+            self.repack_seconds += now() - tr
+        chunk = min(self.max_chunk, min(r.remaining for r in running))
+        tc = now()
+        du_dl, _, _ = batch.multiple_steps(chunk, store_du_dl_interval=self.du_dl_interval,
+                                           use_graph=self.use_graph)
+        self.log.append({"t": tc, "systems": len(running), "atoms": int(sum(batch.sizes)), "steps": chunk,
+                         "seconds": now() - tc, "budget_used": used, "device_bytes": batch.device_bytes()})
+        for s, r in enumerate(running):
+            r.chunks.append(du_dl[s])
+            r.remaining -= chunk
+        done_jobs = {r.job.job_id for r in running if r.remaining == 0}
+        if done_jobs:
+            t_fin = now()
+            by_job: Dict[int, List[_Window]] = {}
+            for r in running:
+                if r.job.job_id in done_jobs:
+                    by_job.setdefault(r.job.job_id, []).append(r)
+            for jid, ws in by_job.items():
+                ws.sort(key=lambda r: r.w)
+                job = ws[0].job
+                series = np.stack([np.concatenate(r.chunks) for r in ws])
+                results.append(JobResult(jid, series, ti_estimate(job.lambdas, series), job.arrival,
+                                         admitted_at[jid], t_fin))
+                used -= self._job_bytes(job)
+            keep = [i for i, r in enumerate(running) if r.job.job_id not in done_jobs]
+            tr = now()
+            if keep:
+                lam = [running[i].job.lambdas[running[i].w] for i in keep]
+                batch = batch.repack(keep, [], lam)
+            else:
+                batch = None
+            self.repack_seconds += now() - tr
+            running[:] = [running[i] for i in keep]
+        return batch, used
 
 
 def run_fifo(jobs: Sequence[Job], du_dl_interval: int = 10, use_graph: bool = True,
@@ -236,9 +299,11 @@ def run_fifo(jobs: Sequence[Job], du_dl_interval: int = 10, use_graph: bool = Tr
 
 
 def synthetic_stream(n_jobs: int, rate_per_s: float, seed: int = 0, atoms=(1800, 36000), windows=(8, 48),
-                     steps=(2000, 20000), heavy=(20, 60), steps_multiple: int = 10) -> List[Job]:
+                     steps=(2000, 20000), heavy=(20, 60), steps_multiple: int = 10,
+                     minimize_steps: int = 0) -> List[Job]:
     """Poisson arrivals of ligand-in-water jobs: atoms log-uniform (the box is at
-    least 2.62 nm, twice cutoff plus padding), windows and steps uniform."""
+    least 2.62 nm, twice cutoff plus padding), windows and steps uniform; each
+    window FIRE-minimised for ``minimize_steps`` steps before its MD."""
     rng = np.random.default_rng(seed)
     t = 0.0
     jobs = []
@@ -263,7 +328,7 @@ def synthetic_stream(n_jobs: int, rate_per_s: float, seed: int = 0, atoms=(1800,
         n = int(rng.integers(steps[0], steps[1] + 1))
         n -= n % steps_multiple
         jobs.append(Job(i, sys_, np.linspace(0.0, 1.0, W), max(n, steps_multiple), int(rng.integers(1 << 30)),
-                        arrival=t))
+                        arrival=t, minimize_steps=minimize_steps))
     return jobs
 
 
