@@ -15,7 +15,14 @@ pytestmark = pytest.mark.gpu
 
 
 def test_windows_to_delta_g(gpu):
+    """Ligand charges scaled to 0.9 by parameter interpolation (no 4D offsets):
+    neighbouring windows overlap well, so BAR and MBAR must agree."""
+    import dataclasses
+
     s = ts.ligand_in_water(side=3.2, n_waters=1050, n_heavy=24, n_ligand=40, seed=5)
+    p1 = s.nb_params.copy()
+    p1[s.ligand_idxs, 0] *= 0.9
+    s = dataclasses.replace(s, nb_params_final=p1, lambda_offset_idxs=np.zeros_like(s.lambda_offset_idxs))
     lams = np.linspace(0.0, 1.0, 6)
     runs = []
     for _ in range(2):

@@ -17,6 +17,8 @@
 // Pair-list kernel (kernels/k_nonbonded_pair_list.cuh:17-216): one thread
 // per pair; with negated=1 it subtracts exactly what the tile kernel added,
 // which is how exclusions cancel.
+#include <type_traits>
+
 #include "nb_pair.h"
 #include "nb_state.h"
 #include "wave.h"
@@ -32,12 +34,10 @@ __device__ __forceinline__ void atomic_add_u64(uint64_t *p, uint64_t v) {
     atomicAdd(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(v));
 }
 
-constexpr int PASS1_UNROLL = 1; // rounds per pass-1 group (2 and 4 measured no faster)
-// Queued pairs per wave: fewer than 64 remain after a drain and a group adds
-// at most 64 per round, so 64 * (PASS1_UNROLL + 1) entries always suffice.
+// Queued pairs per wave: fewer than 64 remain after a drain and a round adds
+// at most 64, so 128 entries always suffice.
 constexpr int PAIR_RING = 128;
-static_assert(PAIR_RING >= TILE * (PASS1_UNROLL + 1), "pair ring too small");
-static_assert((PAIR_RING & (PAIR_RING - 1)) == 0 && TILE % PASS1_UNROLL == 0, "ring / unroll shape");
+static_assert((PAIR_RING & (PAIR_RING - 1)) == 0 && PAIR_RING >= 2 * TILE, "pair ring shape");
 
 // Per-wave LDS of the tile kernel. NACC accumulators per atom: x, y, z
 // forces, then q, sig, eps parameter derivatives when requested.
@@ -100,6 +100,7 @@ __device__ __forceinline__ double pair_delta(const double4 &pi, const double4 &p
 
 struct BoxDev {
     double bx, by, bz, ibx, iby, ibz;
+    double cut2; // squared cutoff: the exact test of pass 2
 };
 
 // Pass 2: evaluates n <= 64 queued pairs starting at ring entry `base`, one
@@ -124,6 +125,9 @@ __device__ __forceinline__ void eval_pairs(TileLds<NACC, INTERP> &L, const int b
         const float4 aj = L.get_par(1, sl);
         double dx, dy, dz, dw;
         const double d2 = pair_delta<ALCH>(pi, pj, B.bx, B.by, B.bz, B.ibx, B.iby, B.ibz, dx, dy, dz, dw);
+        if (!(d2 < B.cut2)) {
+            return; // queued by the FP32 screen of pass 1, outside the exact cutoff
+        }
         const PairTerms T = nb_pair_terms<U, NEED_DP>(d2, ai.x, aj.x, ai.y, aj.y, ai.z, aj.z, beta, 1.0, 1.0);
         if (DX) {
             const uint64_t fx = f64_to_fixed(T.prefactor * dx, FIXED_SCALE);
@@ -225,6 +229,7 @@ __device__ __forceinline__ void tile_body(
     B.iby = 1.0 / B.by;
     B.ibz = 1.0 / B.bz;
     const double cut2 = S.cutoff_d * S.cutoff_d;
+    B.cut2 = cut2;
     const double beta = static_cast<double>(S.beta);
 
     wave_lds_sync(); // the previous tile's LDS reads are complete
@@ -248,39 +253,115 @@ __device__ __forceinline__ void tile_body(
 
     uint64_t uacc = 0, dlacc = 0;
     int queued = 0, done = 0; // wave-uniform
-    // round r pairs lane l with column slot (l + r) mod 64. PASS1_UNROLL
-    // rounds are evaluated as independent chains; full batches of 64 queued
-    // pairs are drained once per group of rounds.
-    for (int round = 0; round < TILE; round += PASS1_UNROLL) {
-        double4 pj[PASS1_UNROLL];
-        uint32_t jc[PASS1_UNROLL];
-#pragma unroll
-        for (int q = 0; q < PASS1_UNROLL; q++) {
-            const int slot = (lane + round + q) & (TILE - 1);
-            pj[q] = L.template get_pos<ALCH>(1, slot);
-            jc[q] = L.colidx[slot];
+    // queue this lane's pair of the round when `in`; drain full batches of 64
+    auto queue = [&](const int slot, const bool in) {
+        const uint64_t m = ballot(in);
+        if (in) {
+            L.pij[(queued + lanes_below(m)) & (PAIR_RING - 1)] =
+                static_cast<uint32_t>(lane) | (static_cast<uint32_t>(slot) << 8);
         }
-        uint64_t m[PASS1_UNROLL];
-#pragma unroll
-        for (int q = 0; q < PASS1_UNROLL; q++) {
-            double dx, dy, dz, dw;
-            const double d2 = pair_delta<ALCH>(pi, pj[q], B.bx, B.by, B.bz, B.ibx, B.iby, B.ibz, dx, dy, dz, dw);
-            m[q] = ballot(vi && (jc[q] < k) && (i < jc[q]) && (d2 < cut2));
-        }
-#pragma unroll
-        for (int q = 0; q < PASS1_UNROLL; q++) {
-            if ((m[q] >> lane) & 1ull) {
-                const int slot = (lane + round + q) & (TILE - 1);
-                L.pij[(queued + lanes_below(m[q])) & (PAIR_RING - 1)] =
-                    static_cast<uint32_t>(lane) | (static_cast<uint32_t>(slot) << 8);
-            }
-            queued += __popcll(m[q]);
-        }
+        queued += __popcll(m);
         while (queued - done >= TILE) {
             wave_lds_sync();
             eval_pairs<ALCH, U, DX, DL, DP, DLDP, NACC, INTERP>(L, done, TILE, B, beta, uacc, dlacc);
             done += TILE;
             wave_lds_sync(); // later rounds reuse these ring entries
+        }
+    };
+
+    // Pass 1, round r pairs lane l with column slot (l + r) mod 64.
+    //
+    // Screen in FP32 on positions relative to the centre c of the row block,
+    // each atom imaged once (per axis a = min_image(x - c)). If every row atom
+    // is within e of c and e + cutoff < L / 2 on every axis, a pair within the
+    // cutoff has a_j - a_i equal to its minimum-image displacement: any other
+    // image differs by a multiple of L, so |a_j - a_i| >= L - cutoff > L / 2 + e,
+    // which a_j - a_i cannot reach. The 4D coordinate w is taken relative to
+    // c's as well. For a pair within the cutoff |a_j| <= e + cutoff, so the
+    // FP32 rounding of a_i, a_j and d2 stays near 1e-6 relative to the squared
+    // cutoff; the screen keeps a margin of 1e-5 and pass 2 repeats the exact
+    // FP64 test. The column values travel around the wave in registers (one
+    // lane rotation per round), so the screen reads no LDS.
+    // Otherwise (small boxes, wide blocks) pass 1 is the exact FP64 test.
+    const double c0x = L.pos[0][0][0], c0y = L.pos[0][1][0], c0z = L.pos[0][2][0]; // row lane 0 holds an atom
+    const double cw = L.pos[0][3][0];
+    double rx = min_image_d(pi.x - c0x, B.bx, B.ibx);
+    double ry = min_image_d(pi.y - c0y, B.by, B.iby);
+    double rz = min_image_d(pi.z - c0z, B.bz, B.ibz);
+    // centre of the row block's extent (lane 0 contributes 0, so padding lanes may too)
+    const float lox = wave_min(vi ? static_cast<float>(rx) : 0.0f), hix = wave_max(vi ? static_cast<float>(rx) : 0.0f);
+    const float loy = wave_min(vi ? static_cast<float>(ry) : 0.0f), hiy = wave_max(vi ? static_cast<float>(ry) : 0.0f);
+    const float loz = wave_min(vi ? static_cast<float>(rz) : 0.0f), hiz = wave_max(vi ? static_cast<float>(rz) : 0.0f);
+    const double mx = 0.5 * (static_cast<double>(lox) + static_cast<double>(hix));
+    const double my = 0.5 * (static_cast<double>(loy) + static_cast<double>(hiy));
+    const double mz = 0.5 * (static_cast<double>(loz) + static_cast<double>(hiz));
+    const double reach = S.cutoff_d + 1e-3; // 1e-3 nm covers the FP32 extents
+    const bool local = uniform((0.5 * (hix - lox) + reach < 0.5 * B.bx && 0.5 * (hiy - loy) + reach < 0.5 * B.by &&
+                                0.5 * (hiz - loz) + reach < 0.5 * B.bz)
+                                   ? 1
+                                   : 0);
+    if (local) {
+        const double cx = c0x + mx, cy = c0y + my, cz = c0z + mz;
+        rx -= mx;
+        ry -= my;
+        rz -= mz;
+        const bool vj0 = jl0 < k;
+        // this lane's column (padding far away: d2 overflows to +inf)
+        float qx = 1e30f, qy = 1e30f, qz = 1e30f, qw = 0.0f;
+        if (vj0) {
+            qx = static_cast<float>(min_image_d(pj0.x - cx, B.bx, B.ibx));
+            qy = static_cast<float>(min_image_d(pj0.y - cy, B.by, B.iby));
+            qz = static_cast<float>(min_image_d(pj0.z - cz, B.bz, B.ibz));
+            qw = ALCH ? static_cast<float>(pj0.w - cw) : 0.0f;
+        }
+        uint32_t qc = jl0;
+        const float fx = static_cast<float>(rx), fy = static_cast<float>(ry), fz = static_cast<float>(rz);
+        const float fw = ALCH ? static_cast<float>(pi.w - cw) : 0.0f;
+        // columns of the row's own block need i < j (upper triangle); later blocks always pass
+        const bool tri = uniform(ballot(vj0 && jl0 < static_cast<uint32_t>((r + 1) * TILE)) != 0 ? 1 : 0);
+        const float cut2f = static_cast<float>(cut2 * (1.0 + 1e-5));
+        const float cut2v = vi ? cut2f : -1.0f; // padding rows accept nothing
+        auto screen = [&](auto tri_tag) {
+            constexpr bool TRI = decltype(tri_tag)::value;
+            for (int round = 0; round < TILE; round++) {
+                // lane l holds column slot (l + round) mod 64
+                const float dx = qx - fx;
+                const float dy = qy - fy;
+                const float dz = qz - fz;
+                float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                if (ALCH) {
+                    const float dw = qw - fw;
+                    d2 = __builtin_fmaf(dw, dw, d2);
+                }
+                bool in = d2 < cut2v;
+                if constexpr (TRI) {
+                    in = in && i < qc;
+                }
+                queue((lane + round) & (TILE - 1), in);
+                qx = rot1(qx);
+                qy = rot1(qy);
+                qz = rot1(qz);
+                if (ALCH) {
+                    qw = rot1(qw);
+                }
+                if constexpr (TRI) {
+                    qc = rot1(qc);
+                }
+            }
+        };
+        if (tri) {
+            screen(std::true_type{});
+        } else {
+            screen(std::false_type{});
+        }
+    } else {
+        for (int round = 0; round < TILE; round++) {
+            const int slot = (lane + round) & (TILE - 1);
+            const double4 pj = L.template get_pos<ALCH>(1, slot);
+            const uint32_t jc = L.colidx[slot];
+            double dx, dy, dz, dw;
+            const double d2 = pair_delta<ALCH>(pi, pj, B.bx, B.by, B.bz, B.ibx, B.iby, B.ibz, dx, dy, dz, dw);
+            queue(slot, vi && (jc < k) && (i < jc) && (d2 < cut2));
         }
     }
     if (queued > done) {

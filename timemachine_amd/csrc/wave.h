@@ -30,7 +30,8 @@ __device__ __forceinline__ void wave_lds_sync() {
 // held by lane (l + 1) mod 64. GFX9-family DPP "wave_rol:1" (ctrl 0x134); all
 // lanes must be active.
 __device__ __forceinline__ uint32_t rot1_u32(uint32_t v) {
-    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x134, 0xF, 0xF, false));
+    // every lane has a source lane, so no "old" value is needed (bound_ctrl set)
+    return static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(v), 0x134, 0xF, 0xF, true));
 }
 __device__ __forceinline__ float rot1(float v) { return __uint_as_float(rot1_u32(__float_as_uint(v))); }
 __device__ __forceinline__ int rot1(int v) { return static_cast<int>(rot1_u32(static_cast<uint32_t>(v))); }
