@@ -1,5 +1,7 @@
 #pragma once
 
+#include <vector>
+
 #include "tm_common.h"
 
 namespace tmamd {
@@ -19,9 +21,30 @@ struct BondedTerms {
     const int *term_sys; // [n] system of each term (nullptr -> 0)
 };
 
+// Force slots: a term's force on the atom at position r of term q goes to
+// slot slot0[kind] + q * arity + r (3 fixed-point words), written by the term
+// kernels instead of a global atomic per atom and component; the incidence
+// lists (CSR over global atoms) name the slots of every atom.
+struct BondedIncidence {
+    int A;            // atoms
+    const int *off;   // [A + 1]
+    const uint32_t *items; // slot indices
+};
+
+// Host side: incidence lists of the terms of the three kinds (global atom
+// indices, idx[k] = [n_k, arity_k]); slot0[k] receives each kind's first slot.
+void build_bonded_incidence(int A, const std::vector<int> idx[3], std::vector<int> &off,
+                            std::vector<uint32_t> &items, int slot0[3]);
+
+// One thread per atom sums its slots and adds them to du_dx with a plain
+// read-modify-write (no other kernel writes du_dx meanwhile). The fixed-point
+// values are those the atomics would add, so the sums are bitwise the same.
+void launch_bonded_gather(const BondedIncidence &inc, const uint64_t *slots, uint64_t *du_dx, hipStream_t stream);
+
 // dl_dp: write d(du/dlambda)/dp into du_dp (every other output must be null).
+// slots: write the per-atom forces into force slots (see above) instead of du_dx.
 void launch_bonded(BondedKind kind, const BondedTerms &t, const double *x, const double *lambdas, double lambda0,
                    const int *red_idx, uint64_t *du_dx, uint64_t *du_dp, uint64_t *du_dl, uint64_t *u,
-                   hipStream_t stream, bool dl_dp = false);
+                   hipStream_t stream, bool dl_dp = false, uint64_t *slots = nullptr);
 
 } // namespace tmamd

@@ -563,6 +563,13 @@ struct Engine::Impl {
     // bonded
     int bt_n[3] = {0, 0, 0};
     DeviceArray<int> bt_idx[3], bt_mult[3], bt_offset[3], bt_sys[3];
+    // force slots and their incidence per atom: the force path gathers per atom
+    // instead of one global atomic per term, atom and component
+    DeviceArray<int> bt_inc_off;
+    DeviceArray<uint32_t> bt_inc_items;
+    DeviceArray<uint64_t> bt_slots;
+    int bt_slot0[3] = {0, 0, 0};
+    bool bt_gather = false;
     DeviceArray<double> bt_params[3];
 
     // barostat
@@ -609,12 +616,17 @@ struct Engine::Impl {
             a.lambdas = lamp;
             launch_pair_list(a, ddx, nullptr, ddl, uu, st);
         }
+        const bool gather = bt_gather && ddx;
         for (int k = 0; k < 3; k++) {
             if (bt_n[k] == 0) {
                 continue;
             }
             BondedTerms t{bt_n[k], bt_idx[k].ptr, bt_params[k].ptr, bt_mult[k].ptr, bt_offset[k].ptr, bt_sys[k].ptr};
-            launch_bonded(static_cast<BondedKind>(k), t, xp, lamp, 0.0, red_idx.ptr, ddx, nullptr, ddl, uu, st);
+            launch_bonded(static_cast<BondedKind>(k), t, xp, lamp, 0.0, red_idx.ptr, gather ? nullptr : ddx, nullptr,
+                          ddl, uu, st, false, gather ? bt_slots.ptr + static_cast<size_t>(bt_slot0[k]) * 3 : nullptr);
+        }
+        if (gather) {
+            launch_bonded_gather(BondedIncidence{A, bt_inc_off.ptr, bt_inc_items.ptr}, bt_slots.ptr, ddx, st);
         }
     }
 
@@ -954,6 +966,15 @@ Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
             upload_vec(I.bt_offset[k], off[k]);
             upload_vec(I.bt_sys[k], sys[k]);
             upload_vec(I.bt_params[k], prm[k]);
+        }
+        if (I.bt_n[0] + I.bt_n[1] + I.bt_n[2] > 0) {
+            std::vector<int> inc_off;
+            std::vector<uint32_t> inc_items;
+            build_bonded_incidence(A_, idx, inc_off, inc_items, I.bt_slot0);
+            upload_vec(I.bt_inc_off, inc_off);
+            upload_vec(I.bt_inc_items, inc_items);
+            I.bt_slots.alloc((idx[0].size() + idx[1].size() + idx[2].size()) * 3);
+            I.bt_gather = true;
         }
     }
     // barostat
