@@ -31,10 +31,11 @@ struct BondedIncidence {
     const uint32_t *items; // slot indices
 };
 
-// Host side: incidence lists of the terms of the three kinds (global atom
-// indices, idx[k] = [n_k, arity_k]); slot0[k] receives each kind's first slot.
-void build_bonded_incidence(int A, const std::vector<int> idx[3], std::vector<int> &off,
-                            std::vector<uint32_t> &items, int slot0[3]);
+// Host side: incidence lists over groups of terms (global atom indices,
+// idx[k] = [n_k, arity_k] flattened: position e is slot slot0[k] + e);
+// slot0 receives each group's first slot.
+void build_bonded_incidence(int A, const std::vector<const std::vector<int> *> &idx, std::vector<int> &off,
+                            std::vector<uint32_t> &items, std::vector<int> &slot0);
 
 // One thread per atom sums its slots and adds them to du_dx with a plain
 // read-modify-write (no other kernel writes du_dx meanwhile). The fixed-point

@@ -201,14 +201,15 @@ __global__ void k_bonded_gather(const BondedIncidence inc, const uint64_t *__res
 
 } // namespace
 
-void build_bonded_incidence(int A, const std::vector<int> idx[3], std::vector<int> &off,
-                            std::vector<uint32_t> &items, int slot0[3]) {
+void build_bonded_incidence(int A, const std::vector<const std::vector<int> *> &idx, std::vector<int> &off,
+                            std::vector<uint32_t> &items, std::vector<int> &slot0) {
     off.assign(static_cast<size_t>(A) + 1, 0);
+    slot0.assign(idx.size(), 0);
     size_t n_slots = 0;
-    for (int k = 0; k < 3; k++) {
+    for (size_t k = 0; k < idx.size(); k++) {
         slot0[k] = static_cast<int>(n_slots);
-        n_slots += idx[k].size();
-        for (int a : idx[k]) {
+        n_slots += idx[k]->size();
+        for (int a : *idx[k]) {
             off[a + 1] += 1;
         }
     }
@@ -220,10 +221,10 @@ void build_bonded_incidence(int A, const std::vector<int> idx[3], std::vector<in
     }
     items.assign(off[A], 0);
     std::vector<int> fill(off.begin(), off.end() - 1);
-    for (int k = 0; k < 3; k++) {
-        for (size_t e = 0; e < idx[k].size(); e++) {
-            // slot of (term e / arity, role e % arity) is slot0[k] + e
-            items[fill[idx[k][e]]++] = static_cast<uint32_t>(slot0[k] + e);
+    for (size_t k = 0; k < idx.size(); k++) {
+        const std::vector<int> &v = *idx[k];
+        for (size_t e = 0; e < v.size(); e++) {
+            items[fill[v[e]]++] = static_cast<uint32_t>(slot0[k] + e); // term e / arity, position e % arity
         }
     }
 }

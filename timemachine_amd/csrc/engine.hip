@@ -568,7 +568,8 @@ struct Engine::Impl {
     DeviceArray<int> bt_inc_off;
     DeviceArray<uint32_t> bt_inc_items;
     DeviceArray<uint64_t> bt_slots;
-    int bt_slot0[3] = {0, 0, 0};
+    std::vector<int> bt_slot0;      // first slot of bonds, angles, torsions, pair list, negated pair list
+    std::vector<int> h_pl_pairs[2]; // host copy of the pair lists (incidence only)
     bool bt_gather = false;
     DeviceArray<double> bt_params[3];
 
@@ -608,15 +609,18 @@ struct Engine::Impl {
                        lamp, 0.0, st);
             nb.compute(boxp, ddx, nullptr, ddl, uu, st, tile_grid);
         }
+        const bool gather = bt_gather && ddx;
         for (int neg = 0; neg < 2; neg++) {
             if (pl_n[neg] == 0) {
                 continue;
             }
             PairListArgs a = pl_args(neg, xp, boxp);
             a.lambdas = lamp;
+            if (gather) {
+                a.slots = bt_slots.ptr + static_cast<size_t>(bt_slot0[3 + neg]) * 3;
+            }
             launch_pair_list(a, ddx, nullptr, ddl, uu, st);
         }
-        const bool gather = bt_gather && ddx;
         for (int k = 0; k < 3; k++) {
             if (bt_n[k] == 0) {
                 continue;
@@ -895,6 +899,7 @@ Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
         }
         for (int neg = 0; neg < 2; neg++) {
             I.pl_n[neg] = static_cast<int>(psys[neg].size());
+            I.h_pl_pairs[neg] = pairs[neg];
             upload_vec(I.pl_pairs[neg], pairs[neg]);
             upload_vec(I.pl_scales[neg], scales[neg]);
             upload_vec(I.pl_sys[neg], psys[neg]);
@@ -967,15 +972,21 @@ Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
             upload_vec(I.bt_sys[k], sys[k]);
             upload_vec(I.bt_params[k], prm[k]);
         }
-        if (I.bt_n[0] + I.bt_n[1] + I.bt_n[2] > 0) {
+        if (I.bt_n[0] + I.bt_n[1] + I.bt_n[2] + I.pl_n[0] + I.pl_n[1] > 0) {
+            // force slots of bonded terms and pair lists, gathered per atom
             std::vector<int> inc_off;
             std::vector<uint32_t> inc_items;
-            build_bonded_incidence(A_, idx, inc_off, inc_items, I.bt_slot0);
+            build_bonded_incidence(A_, {&idx[0], &idx[1], &idx[2], &I.h_pl_pairs[0], &I.h_pl_pairs[1]}, inc_off,
+                                   inc_items, I.bt_slot0);
             upload_vec(I.bt_inc_off, inc_off);
             upload_vec(I.bt_inc_items, inc_items);
-            I.bt_slots.alloc((idx[0].size() + idx[1].size() + idx[2].size()) * 3);
+            I.bt_slots.alloc((idx[0].size() + idx[1].size() + idx[2].size() + I.h_pl_pairs[0].size() +
+                              I.h_pl_pairs[1].size()) *
+                             3);
             I.bt_gather = true;
         }
+        I.h_pl_pairs[0].clear();
+        I.h_pl_pairs[1].clear();
     }
     // barostat
     {

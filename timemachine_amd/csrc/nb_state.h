@@ -180,6 +180,10 @@ struct PairListArgs {
     const float *betas;    // [S] or nullptr
     float beta0;
     int negated;
+    // Force slots instead of du_dx atomics (engine force path): pair p writes its
+    // force on atom 0 to slots[2p] and on atom 1 to slots[2p + 1] (3 words each,
+    // zero outside the cutoff); a per-atom gather adds them to du_dx.
+    uint64_t *slots;
 };
 
 void launch_pair_list(

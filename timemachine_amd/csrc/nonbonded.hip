@@ -565,6 +565,7 @@ __global__ __launch_bounds__(256) void k_pair_list(
         const double dz = min_image_d(a.x[gi * 3 + 2] - a.x[gj * 3 + 2], bz, ibz);
         const double dw = wi - wj;
         const double d2 = __builtin_fma(dw, dw, pair_d2(dx, dy, dz));
+        uint64_t fs[3] = {0, 0, 0}; // force on atom 0 for the slots
         if (d2 < cutoff_d * cutoff_d) {
             // parameters as the tile kernel sees them: FP32 values of p, or of
             // p = (1 - lambda) p0 + lambda p1 when interpolated (k_gather)
@@ -601,12 +602,18 @@ __global__ __launch_bounds__(256) void k_pair_list(
                 const uint64_t fx = f64_to_fixed(T.prefactor * dx, FIXED_SCALE);
                 const uint64_t fy = f64_to_fixed(T.prefactor * dy, FIXED_SCALE);
                 const uint64_t fz = f64_to_fixed(T.prefactor * dz, FIXED_SCALE);
-                acc(du_dx + gi * 3 + 0, fx);
-                acc(du_dx + gi * 3 + 1, fy);
-                acc(du_dx + gi * 3 + 2, fz);
-                acc(du_dx + gj * 3 + 0, 0ull - fx);
-                acc(du_dx + gj * 3 + 1, 0ull - fy);
-                acc(du_dx + gj * 3 + 2, 0ull - fz);
+                if (a.slots) {
+                    fs[0] = neg ? 0ull - fx : fx;
+                    fs[1] = neg ? 0ull - fy : fy;
+                    fs[2] = neg ? 0ull - fz : fz;
+                } else {
+                    acc(du_dx + gi * 3 + 0, fx);
+                    acc(du_dx + gi * 3 + 1, fy);
+                    acc(du_dx + gi * 3 + 2, fz);
+                    acc(du_dx + gj * 3 + 0, 0ull - fx);
+                    acc(du_dx + gj * 3 + 1, 0ull - fy);
+                    acc(du_dx + gj * 3 + 2, 0ull - fz);
+                }
             }
             if (DP) {
                 acc(du_dp + gi * 3 + 0, f64_to_fixed((scale_q * qj) * T.erfc_inv_d, FIXED_SCALE));
@@ -648,6 +655,12 @@ __global__ __launch_bounds__(256) void k_pair_list(
             if (neg) {
                 fu = 0ull - fu;
                 fdl = 0ull - fdl;
+            }
+        }
+        if (DX && a.slots) {
+            for (int c = 0; c < 3; c++) {
+                a.slots[static_cast<size_t>(p) * 6 + c] = fs[c];
+                a.slots[static_cast<size_t>(p) * 6 + 3 + c] = 0ull - fs[c];
             }
         }
     }
