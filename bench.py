@@ -42,8 +42,8 @@ FLOP_PAIR_BASE = 71
 FLOP_PAIR_LJ = 13
 # HBM bytes per launch of the du_dx tile kernel on this workload: FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0 (factors from a 256 MiB device copy), measured with
-# rocprofv3 --pmc on MI355X; profiles/counters_lambda_windows_192x6372.txt.
-TILE_TRAFFIC_BYTES = 1.0612e9 + 3.3288e9
+# rocprofv3 --pmc on MI355X (summarised under profiles/ with the commands).
+TILE_TRAFFIC_BYTES = 0.6917e9 + 3.3288e9
 
 
 def log(msg: str) -> None:
@@ -235,8 +235,10 @@ def main() -> int:
         batch.set_tile_grid(args.tile_grid)
     elif not args.no_autotune and not args.profile_only:
         log("autotuning grid and neighborlist padding on a device copy of the batch")
-        tuned = batch.autotune(steps=30)
+        tuned = batch.autotune()
         log(f"autotuned: {tuned['grid']} workgroups, padding {tuned['padding']} nm")
+        for c in tuned["candidates"]:
+            log(f"  grid {c['grid']:6d} padding {c['padding']:.3f} nm: {c['ms_per_step']:.3f} ms/step")
     S = len(ctxs)
     use_graph = not args.no_graph
     log(f"warmup {args.warmup} batched steps")
