@@ -31,6 +31,8 @@ void hc_normal2(const uint32_t *bits, double *z) {
 
 uint64_t hc_round_to_fixed(double v) { return round_to_fixed(v); }
 
+void hc_round_to_fixed3(double a, double b, double c, uint64_t *out) { round_to_fixed3(a, b, c, out[0], out[1], out[2]); }
+
 uint64_t hc_f32_to_fixed(float v, float scale) { return f32_to_fixed(v, scale); }
 
 // out: prefactor, u, erfc_inv_d, sig_grad, eps_grad, lj

@@ -106,6 +106,19 @@ def test_round_to_fixed_matches_llrint(hc):
         assert hc.hc_round_to_fixed(v) == nblist.round_to_fixed(v), v
 
 
+def test_round_to_fixed3_equals_three_calls(hc):
+    """The three-value conversion (one range test) gives round_to_fixed's bits."""
+    hc.hc_round_to_fixed3.argtypes = [ctypes.c_double] * 3 + [P(ctypes.c_uint64)]
+    rng = np.random.default_rng(2)
+    pool = list(rng.normal(size=300) * 10.0 ** rng.uniform(-3, 17, 300))
+    pool += [2.0**51 - 0.5, 2.0**51, -(2.0**51), 2.0**63, float("nan"), float("inf"), -0.0, 2.5, -2.5]
+    out = np.zeros(3, dtype=np.uint64)
+    for _ in range(3000):
+        a, b, c = (float(pool[i]) for i in rng.integers(0, len(pool), 3))
+        hc.hc_round_to_fixed3(a, b, c, out.ctypes.data_as(P(ctypes.c_uint64)))
+        assert list(out) == [hc.hc_round_to_fixed(a), hc.hc_round_to_fixed(b), hc.hc_round_to_fixed(c)], (a, b, c)
+
+
 def test_f32_to_fixed_is_odd(hc):
     """Newton's third law in fixed point needs fixed(-v) == -fixed(v) exactly."""
     rng = np.random.default_rng(1)

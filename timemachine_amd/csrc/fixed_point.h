@@ -59,6 +59,27 @@ TM_HD inline uint64_t round_to_fixed(double d) {
     return (static_cast<uint64_t>(hi32) << 32) | lo32;
 }
 
+// Three values at once with the same result as three round_to_fixed calls:
+// one range test for the common case that all three take the add-magic path.
+TM_HD inline void round_to_fixed3(double a, double b, double c, uint64_t &fa, uint64_t &fb, uint64_t &fc) {
+    // comparisons are false for NaN, which the general path handles
+    const bool fast = (__builtin_fabs(a) < 0x1p51) & (__builtin_fabs(b) < 0x1p51) & (__builtin_fabs(c) < 0x1p51);
+    if (fast) {
+        const double ta = a + 0x1.8p52, tb = b + 0x1.8p52, tc = c + 0x1.8p52;
+        uint64_t ba, bb, bc;
+        __builtin_memcpy(&ba, &ta, sizeof(ba));
+        __builtin_memcpy(&bb, &tb, sizeof(bb));
+        __builtin_memcpy(&bc, &tc, sizeof(bc));
+        fa = ba - FIXED_MAGIC_BITS;
+        fb = bb - FIXED_MAGIC_BITS;
+        fc = bc - FIXED_MAGIC_BITS;
+        return;
+    }
+    fa = round_to_fixed(a);
+    fb = round_to_fixed(b);
+    fc = round_to_fixed(c);
+}
+
 // f32 value -> fixed point with scale 2^k (scale must be a power of two).
 TM_HD inline uint64_t f32_to_fixed(float v, float scale) {
     return round_to_fixed(static_cast<double>(v * scale));

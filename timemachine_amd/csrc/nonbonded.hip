@@ -130,9 +130,9 @@ __device__ __forceinline__ void eval_pairs(TileLds<NACC, INTERP> &L, const int b
         }
         const PairTerms T = nb_pair_terms<U, NEED_DP>(d2, ai.x, aj.x, ai.y, aj.y, ai.z, aj.z, beta, 1.0, 1.0);
         if (DX) {
-            const uint64_t fx = f64_to_fixed(T.prefactor * dx, FIXED_SCALE);
-            const uint64_t fy = f64_to_fixed(T.prefactor * dy, FIXED_SCALE);
-            const uint64_t fz = f64_to_fixed(T.prefactor * dz, FIXED_SCALE);
+            uint64_t fx, fy, fz; // = f64_to_fixed(T.prefactor * d, FIXED_SCALE) per component
+            round_to_fixed3((T.prefactor * dx) * FIXED_SCALE, (T.prefactor * dy) * FIXED_SCALE,
+                            (T.prefactor * dz) * FIXED_SCALE, fx, fy, fz);
             lds_add_u64(&L.acc[0][0][il], fx);
             lds_add_u64(&L.acc[0][1][il], fy);
             lds_add_u64(&L.acc[0][2][il], fz);
