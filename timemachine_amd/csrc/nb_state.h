@@ -12,6 +12,7 @@
 //   perm, pos (double4), par (float4) [slot0 .. slot0 + nblocks*64)  sorted order, padded to whole tiles
 //   bb_ctr, bb_ext (float4)       [block0 .. block0 + nblocks)
 //   tile_row, tile_atoms[64]      [tile0 .. tile0 + tile_cap)
+//   tile_order                    [tile0 .. tile0 + tile_cap)   tiles listed by row block
 // pos = (x, y, z, w) in FP64 and par = (q, sig, eps, dw/dlambda) in FP32 are the
 // values the pair arithmetic consumes (FP64 coordinate differences keep the
 // per-atom force error under 1e-5; parameters keep the reference's FP32).
@@ -99,6 +100,7 @@ struct NbState {
     DeviceArray<int> tile_count;  // [S]
     DeviceArray<int> tile_prefix; // [S + 1]
     DeviceArray<int> tile_row;    // [total_tiles]
+    DeviceArray<int> tile_order;  // [total_tiles] per system: local tile indices grouped by row block
     DeviceArray<uint32_t> tile_atoms; // [total_tiles * 64]
     DeviceArray<uint32_t> perm;   // [total_slots] sorted slot -> local atom index
     DeviceArray<uint32_t> keys;   // [total_slots] sort keys (scratch)

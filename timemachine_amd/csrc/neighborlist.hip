@@ -649,6 +649,74 @@ __global__ __launch_bounds__(256) void k_find_tiles(
     }
 }
 
+// Tiles of each rebuilt system listed by row block: tile_order[tile0 + p] is
+// the local index of the p-th tile in row order (tiles of one row in any
+// order; the fixed-point sums do not depend on it). The tile kernel walks the
+// list in this order so that one wave meets the tiles of a row block one after
+// another. Systems with more row blocks than the LDS histogram holds keep the
+// order of the list.
+constexpr int ORDER_ROWS = 4096;
+
+__global__ __launch_bounds__(256) void k_tile_order(
+    const NbSys *__restrict__ sys, const int *__restrict__ rebuild, const int *__restrict__ tile_count,
+    const int *__restrict__ tile_row, int *__restrict__ tile_order) {
+    const int s = blockIdx.x;
+    if (!rebuild[s]) {
+        return;
+    }
+    const NbSys S = sys[s];
+    const int n = min(tile_count[s], S.tile_cap);
+    const int rb = S.row_blocks;
+    const int t = threadIdx.x;
+    int *order = tile_order + S.tile0;
+    const int *rows = tile_row + S.tile0;
+    if (rb > ORDER_ROWS) {
+        for (int k = t; k < n; k += 256) {
+            order[k] = k;
+        }
+        return;
+    }
+    __shared__ int cnt[ORDER_ROWS];
+    __shared__ int part[256];
+    for (int r = t; r < rb; r += 256) {
+        cnt[r] = 0;
+    }
+    __syncthreads();
+    for (int k = t; k < n; k += 256) {
+        atomicAdd(&cnt[rows[k]], 1);
+    }
+    __syncthreads();
+    // exclusive scan of cnt[0, rb): thread t owns a contiguous range
+    const int per = (rb + 255) / 256;
+    const int lo = min(t * per, rb);
+    const int hi = min(lo + per, rb);
+    int sum = 0;
+    for (int r = lo; r < hi; r++) {
+        sum += cnt[r];
+    }
+    part[t] = sum;
+    __syncthreads();
+    if (t == 0) {
+        int acc = 0;
+        for (int i = 0; i < 256; i++) {
+            const int v = part[i];
+            part[i] = acc;
+            acc += v;
+        }
+    }
+    __syncthreads();
+    int run = part[t];
+    for (int r = lo; r < hi; r++) {
+        const int v = cnt[r];
+        cnt[r] = run;
+        run += v;
+    }
+    __syncthreads();
+    for (int k = t; k < n; k += 256) {
+        order[atomicAdd(&cnt[rows[k]], 1)] = k;
+    }
+}
+
 // One workgroup: clamp counts to capacity, exclusive prefix over systems
 // (tile_prefix[S] = total), clear the rebuild flags.
 __global__ __launch_bounds__(256) void k_tile_prefix(
@@ -757,6 +825,7 @@ void NbState::init(const std::vector<NbSys> &sys_in, const std::vector<int> &sub
     tile_count.alloc(S);
     tile_prefix.alloc(S + 1);
     tile_row.alloc(total_tiles);
+    tile_order.alloc(total_tiles);
     tile_atoms.alloc(static_cast<size_t>(total_tiles) * TILE);
     perm.alloc(total_slots);
     keys.alloc(total_slots);
@@ -818,6 +887,8 @@ void NbState::build_only(const double *x, const double *boxes, hipStream_t strea
         sys.ptr, boxes, rebuild.ptr, pos.ptr, bb_ctr.ptr, bb_ext.ptr, tile_count.ptr, tile_row.ptr, tile_atoms.ptr,
         status.ptr);
     TM_LAUNCH_CHECK();
+    k_tile_order<<<S, 256, 0, stream>>>(sys.ptr, rebuild.ptr, tile_count.ptr, tile_row.ptr, tile_order.ptr);
+    TM_LAUNCH_CHECK();
     k_tile_prefix<<<1, 256, 0, stream>>>(sys.ptr, S, tile_count.ptr, tile_prefix.ptr, rebuild.ptr);
     TM_LAUNCH_CHECK();
 }
@@ -857,6 +928,8 @@ void NbState::prepare(
     k_find_tiles<<<dim3(ceil_div(max_blocks, 4), S), 256, 0, stream>>>(
         sys.ptr, boxes, rebuild.ptr, pos.ptr, bb_ctr.ptr, bb_ext.ptr, tile_count.ptr, tile_row.ptr, tile_atoms.ptr,
         status.ptr);
+    TM_LAUNCH_CHECK();
+    k_tile_order<<<S, 256, 0, stream>>>(sys.ptr, rebuild.ptr, tile_count.ptr, tile_row.ptr, tile_order.ptr);
     TM_LAUNCH_CHECK();
     k_tile_prefix<<<1, 256, 0, stream>>>(sys.ptr, S, tile_count.ptr, tile_prefix.ptr, rebuild.ptr);
     TM_LAUNCH_CHECK();

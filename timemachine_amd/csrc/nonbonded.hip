@@ -211,6 +211,8 @@ __device__ __forceinline__ void tile_body(
     const float4 dpj,
     const uint32_t *__restrict__ perm,
     const double *__restrict__ boxes,
+    const bool row_fresh,
+    const bool row_flush,
     uint64_t *__restrict__ du_dx,
     uint64_t *__restrict__ du_dp,
     uint64_t *__restrict__ du_dl,
@@ -244,9 +246,12 @@ __device__ __forceinline__ void tile_body(
         L.dp.v[1][1][lane] = dpj.y;
         L.dp.v[1][2][lane] = dpj.z;
     }
+    // row accumulators carry over the tiles of one row run (see k_nonbonded_tiles)
 #pragma unroll
     for (int c = 0; c < NACC; c++) {
-        L.acc[0][c][lane] = 0;
+        if (row_fresh) {
+            L.acc[0][c][lane] = 0;
+        }
         L.acc[1][c][lane] = 0;
     }
     wave_lds_sync();
@@ -372,7 +377,7 @@ __device__ __forceinline__ void tile_body(
 
     const bool vj = jl0 < k;
     if (DX || DP || DLDP) {
-        if (vi) {
+        if (vi && row_flush) {
             const int gi = S.atom0 + static_cast<int>(perm[S.slot0 + i]);
             if (DX) {
 #pragma unroll
@@ -436,6 +441,7 @@ __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded_tiles(
     const NbSys *__restrict__ sys,
     const int S,
     const int *__restrict__ tile_prefix,
+    const int *__restrict__ tile_order,
     const int *__restrict__ tile_row,
     const uint32_t *__restrict__ tile_atoms,
     const uint32_t *__restrict__ perm,
@@ -454,10 +460,24 @@ __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded_tiles(
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     TileLds<NACC, INTERP> &L = lds[wave];
-    const int first = uniform(blockIdx.x * WAVES + wave);
-    const int stride = gridDim.x * WAVES;
+    // Each wave takes a contiguous range of the tile list. Tiles of a system
+    // are listed grouped by row block (tile_order), so consecutive tiles of a
+    // wave mostly share their row atoms: the row accumulators stay in LDS over
+    // such a run and go out with one set of global atomics at its end instead
+    // of one per tile (device-scope 64-bit atomics are the costly part of a
+    // tile's output). The d(du/dlambda)/dp kernel, which skips tiles, writes
+    // per tile.
+    const int w = uniform(blockIdx.x * WAVES + wave);
+    const int nwaves = gridDim.x * WAVES;
     const int total = tile_prefix[S];
-    for (int t = first; t < total; t += stride) {
+    const int chunk = (total + nwaves - 1) / nwaves;
+    int t = w * chunk;
+    const int t_end = min(total, t + chunk);
+    if (t >= t_end) {
+        return;
+    }
+    int s = 0;
+    {
         int lo = 0, hi = S - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -467,13 +487,26 @@ __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded_tiles(
                 hi = mid - 1;
             }
         }
-        const int s = lo;
+        s = lo;
+    }
+    bool open = false; // the row accumulators hold a run that is not written yet
+    for (; t < t_end; t++) {
+        while (tile_prefix[s + 1] <= t) {
+            s++; // tile_prefix[S] = total > t
+        }
         if (status[s] & NB_BAD_BOX) {
             continue;
         }
         const NbSys Sd = sys[s];
-        const int kt = Sd.tile0 + (t - tile_prefix[s]);
+        const int local = t - tile_prefix[s];
+        const int kt = Sd.tile0 + tile_order[Sd.tile0 + local];
         const int r = tile_row[kt];
+        bool last = true; // the next tile of this wave has another row (or system)
+        if (!DLDP && t + 1 < t_end && t + 1 < tile_prefix[s + 1]) {
+            last = tile_row[Sd.tile0 + tile_order[Sd.tile0 + local + 1]] != r;
+        }
+        const bool fresh = !open;
+        open = !last;
         const uint32_t jl = tile_atoms[static_cast<size_t>(kt) * TILE + lane];
         const int i = r * TILE + lane;
         const double4 pi = pos[Sd.slot0 + i];
@@ -495,15 +528,15 @@ __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded_tiles(
         const bool alch = ballot(moving) != 0;
         if (alch) {
             tile_body<true, U, DX, DL, DP, DLDP, NACC, INTERP>(L, Sd, s, r, jl, pi, ai, pj, aj, dpi, dpj, perm, boxes,
-                                                               du_dx, du_dp, du_dl, u);
+                                                               fresh, last, du_dx, du_dp, du_dl, u);
         } else if (!DLDP) {
             tile_body<false, U, DX, DL, DP, false, NACC, INTERP>(L, Sd, s, r, jl, pi, ai, pj, aj, dpi, dpj, perm,
-                                                                 boxes, du_dx, du_dp, du_dl, u);
+                                                                 boxes, fresh, last, du_dx, du_dp, du_dl, u);
         }
     }
 }
 
-using TileKernel = void (*)(const NbSys *, int, const int *, const int *, const uint32_t *, const uint32_t *,
+using TileKernel = void (*)(const NbSys *, int, const int *, const int *, const int *, const uint32_t *, const uint32_t *,
                             const double4 *, const float4 *, const float4 *, const double *, const int *, uint64_t *,
                             uint64_t *, uint64_t *, uint64_t *);
 
@@ -715,7 +748,7 @@ void NbState::compute(
     }
     const int grid = grid_override > 0 ? grid_override : default_tile_grid();
     const TileKernel kern = any_interp ? tile_kernels_interp[idx] : tile_kernels[idx];
-    kern<<<grid, NB_BLOCK, 0, stream>>>(sys.ptr, S, tile_prefix.ptr, tile_row.ptr, tile_atoms.ptr, perm.ptr, pos.ptr,
+    kern<<<grid, NB_BLOCK, 0, stream>>>(sys.ptr, S, tile_prefix.ptr, tile_order.ptr, tile_row.ptr, tile_atoms.ptr, perm.ptr, pos.ptr,
                                         par.ptr, any_interp ? dpar.ptr : nullptr, boxes, status.ptr, du_dx, du_dp,
                                         du_dl, u);
     TM_LAUNCH_CHECK();
@@ -737,7 +770,7 @@ void NbState::compute_dl_dp(const double *boxes, uint64_t *dl_dp, hipStream_t st
         throw Error("d(du/dlambda)/dp is not available for interpolated parameters");
     }
     k_nonbonded_tiles<false, false, false, false, true><<<grid, NB_BLOCK, 0, stream>>>(
-        sys.ptr, S, tile_prefix.ptr, tile_row.ptr, tile_atoms.ptr, perm.ptr, pos.ptr, par.ptr, nullptr, boxes,
+        sys.ptr, S, tile_prefix.ptr, tile_order.ptr, tile_row.ptr, tile_atoms.ptr, perm.ptr, pos.ptr, par.ptr, nullptr, boxes,
         status.ptr, nullptr, dl_dp, nullptr, nullptr);
     TM_LAUNCH_CHECK();
 }
