@@ -184,8 +184,10 @@ def serving_leg(n_jobs: int, seed: int = 5) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=150)
-    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=600,
+                    help="untimed batched steps first: the barostat expands the freshly built boxes by ~8%% over "
+                    "the first ~500 steps, and the step time settles with them")
     ap.add_argument("--windows", type=int, default=48)
     ap.add_argument("--replicas", type=int, default=4)
     ap.add_argument("--single-steps", type=int, default=300)

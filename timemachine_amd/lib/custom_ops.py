@@ -859,17 +859,22 @@ class BatchedContexts(_Handle):
         import time
 
         scratch = self.repack(range(len(self)), [], self._lam)
-        table = []
-        for pad in paddings:
-            scratch.set_nblist_padding(pad)
-            for g in grids:
+        # The step time drifts while a freshly built system relaxes, so the
+        # candidates are timed twice, in one order and then in the reverse
+        # order, and ranked by their mean.
+        kw = dict(store_du_dl_interval=store_du_dl_interval, use_graph=use_graph)
+        scratch.multiple_steps(steps, **kw)
+        order = [(float(pad), int(g)) for pad in paddings for g in grids]
+        times = {c: [] for c in order}
+        for sweep in (order, order[::-1]):
+            for pad, g in sweep:
+                scratch.set_nblist_padding(pad)
                 scratch.set_tile_grid(g)
-                scratch.multiple_steps(max(steps // 3, 1), store_du_dl_interval=store_du_dl_interval,
-                                       use_graph=use_graph)
+                scratch.multiple_steps(max(steps // 3, 1), **kw)
                 t0 = time.perf_counter()
-                scratch.multiple_steps(steps, store_du_dl_interval=store_du_dl_interval, use_graph=use_graph)
-                table.append({"grid": int(g), "padding": float(pad),
-                              "ms_per_step": 1e3 * (time.perf_counter() - t0) / steps})
+                scratch.multiple_steps(steps, **kw)
+                times[(pad, g)].append(1e3 * (time.perf_counter() - t0) / steps)
+        table = [{"grid": g, "padding": pad, "ms_per_step": sum(v) / len(v)} for (pad, g), v in times.items()]
         del scratch
         best = min(table, key=lambda r: r["ms_per_step"])
         self.set_tile_grid(best["grid"])
