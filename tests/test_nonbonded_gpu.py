@@ -209,6 +209,30 @@ def test_neighborlist_exact(gpu, size):
     assert nmis <= 2
 
 
+def _morton_order(x, box, bits=5):
+    """Atoms ordered along a Morton curve, so 64 consecutive atoms are compact."""
+    L = np.diag(box)
+    c = np.minimum((np.mod(x / L, 1.0) * (1 << bits)).astype(np.int64), (1 << bits) - 1)
+    key = np.zeros(len(x), dtype=np.int64)
+    for b in range(bits):
+        for a in range(3):
+            key |= ((c[:, a] >> b) & 1) << (3 * b + a)
+    return np.argsort(key, kind="stable")
+
+
+@pytest.mark.parametrize("cutoff", [1.0, 1.3, 1.9])
+@pytest.mark.parametrize("order", ["morton", "given"])
+def test_neighborlist_exact_full_box(ligand, cutoff, order):
+    """6372 atoms in a 4 nm box, rows as compact blocks (Morton order) or as the
+    system lists them (rows of the water lattice, wide blocks): the tile set is the
+    FP32 predicate's, bit for bit, at the cutoff and well beyond it."""
+    x = ligand.x[_morton_order(ligand.x, ligand.box)] if order == "morton" else ligand.x
+    nl = custom_ops.Neighborlist_f32(ligand.N)
+    test = nl.get_nblist(x, ligand.box, cutoff)
+    ref32 = onblist.fp32_ixn_list(x, ligand.box, cutoff, 64)
+    assert [sorted(t) for t in test] == ref32
+
+
 def test_neighborlist_resize_errors(gpu):
     nl = custom_ops.Neighborlist_f32(3)
     with pytest.raises(RuntimeError, match="size is must be at least 1"):
