@@ -42,30 +42,52 @@ FLOP_PAIR_BASE = 71
 FLOP_PAIR_LJ = 13
 # HBM bytes per launch of the du_dx tile kernel on this workload: FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0 (factors from a 256 MiB device copy), measured with
-# rocprofv3 --pmc on MI355X (summarised under profiles/ with the commands).
-TILE_TRAFFIC_BYTES = 0.6917e9 + 3.3288e9
+# rocprofv3 --pmc on MI355X (summarised under profiles/ with the commands; the
+# counter program profiles the boxes as built, ~4% more tiles than relaxed).
+TILE_TRAFFIC_BYTES = 0.8062e9 + 2.1219e9
+# The HIP-event kernel profile (roofline and kernel_ms) runs at one fixed
+# setting, the one of the committed rocprofv3 summaries, whatever the autotuner
+# chose for the timed steps.
+PROFILE_GRID = 32768
+PROFILE_PADDING_NM = 0.1
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def pair_counts(sys_, lam: float):
-    """Exact pairs within the 4D cutoff (and LJ pairs) at lambda, from 3D candidates."""
+def pair_counts(sys_, x, box, lam: float):
+    """Exact pairs within the 4D cutoff (and Lennard-Jones pairs) of one state.
+
+    Atoms are grouped by (w, LJ); for two groups whose w differ by dw the 4D
+    test d^2 + dw^2 < rc^2 is a 3D test at sqrt(rc^2 - dw^2), counted with
+    periodic k-d trees (strict inequality)."""
     from scipy.spatial import cKDTree
 
-    L = np.diag(sys_.box)
-    x = np.mod(sys_.x, L)
-    x = np.where(x >= L, 0.0, x)
-    pairs = cKDTree(x, boxsize=L).query_pairs(sys_.cutoff, output_type="ndarray")
-    d = sys_.x[pairs[:, 0]] - sys_.x[pairs[:, 1]]
-    d -= L * np.floor(d / L + 0.5)
-    w = sys_.cutoff * (sys_.lambda_plane_idxs + sys_.lambda_offset_idxs * lam)
-    dw = w[pairs[:, 0]] - w[pairs[:, 1]]
-    inside = (d * d).sum(1) + dw * dw < sys_.cutoff**2
-    eps = sys_.nb_params[:, 2]
-    lj = inside & (eps[pairs[:, 0]] != 0) & (eps[pairs[:, 1]] != 0)
-    return int(inside.sum()), int(lj.sum())
+    L = np.diag(box)
+    xw = np.mod(x, L)
+    xw = np.where(xw >= L, 0.0, xw)
+    rc = sys_.cutoff
+    w = rc * (sys_.lambda_plane_idxs + sys_.lambda_offset_idxs * lam)
+    lj = sys_.nb_params[:, 2] != 0
+    groups = []
+    for wk, lk in sorted(set(zip(w.tolist(), lj.tolist()))):
+        idx = np.nonzero((w == wk) & (lj == lk))[0]
+        groups.append((wk, lk, len(idx), cKDTree(xw[idx], boxsize=L)))
+    total = total_lj = 0
+    for a in range(len(groups)):
+        for b in range(a, len(groups)):
+            dw = groups[a][0] - groups[b][0]
+            if dw * dw >= rc * rc:
+                continue
+            r = np.nextafter(np.sqrt(rc * rc - dw * dw), 0.0)
+            c = int(groups[a][3].count_neighbors(groups[b][3], r))
+            if a == b:
+                c = (c - groups[a][2]) // 2  # self pairs and both orders
+            total += c
+            if groups[a][1] and groups[b][1]:
+                total_lj += c
+    return total, total_lj
 
 
 def make_contexts(sys_, lams, replicas, seed0=2022, barostat_interval=15):
@@ -257,7 +279,9 @@ def main() -> int:
     atom_steps = S * sys_.N * args.steps
     value = atom_steps / t_batch
 
-    log("profiling kernels (HIP events)")
+    log(f"profiling kernels (HIP events) at grid {PROFILE_GRID}, padding {PROFILE_PADDING_NM} nm")
+    batch.set_tile_grid(PROFILE_GRID)
+    batch.set_nblist_padding(PROFILE_PADDING_NM)
     prof = batch.profile(reps=10)
     tiles = batch.total_tiles()
 
@@ -271,13 +295,14 @@ def main() -> int:
     single_rate = sys_.N * args.single_steps / t_single
 
     # roofline of the pair kernel: algorithmic FLOP per launch / HIP-event duration
-    log("counting pairs for the roofline")
-    flop = 0.0
-    pairs_total = 0
-    for lam in lams:
-        p, plj = pair_counts(sys_, float(lam))
-        flop += args.replicas * (FLOP_PAIR_BASE * p + FLOP_PAIR_LJ * plj)
-        pairs_total += args.replicas * p
+    log("counting pairs for the roofline (the state the kernel profile ran on)")
+    from concurrent.futures import ThreadPoolExecutor
+
+    states = [(batch.get_x_t(s), batch.get_box(s), lam_list[s]) for s in range(S)]
+    with ThreadPoolExecutor(min(16, os.cpu_count() or 1)) as ex:
+        counts = list(ex.map(lambda st: pair_counts(sys_, st[0], st[1], float(st[2])), states))
+    pairs_total = sum(p for p, _ in counts)
+    flop = float(sum(FLOP_PAIR_BASE * p + FLOP_PAIR_LJ * plj for p, plj in counts))
     achieved = flop / (prof["nonbonded"] * 1e-3) / 1e12
     roofline = {
         "bound": "valu",
