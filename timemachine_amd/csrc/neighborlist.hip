@@ -414,7 +414,10 @@ __device__ __forceinline__ float box_gap(float d, float ext, float b, float ib) 
     return fmaxf(0.0f, fabsf(d) - ext);
 }
 
-// grid (ceil(max_blocks / 4), S), 256 threads: one wave per row block.
+// grid (S, ceil(max_blocks / 4)), 256 threads: one wave per row block. The
+// system index runs fastest so that the first row blocks of every system,
+// which scan the most column blocks (upper triangle), are dispatched first
+// and the launch does not end on a tail of long waves.
 // Column blocks c >= r are screened by bounding boxes (with a small margin
 // so the screen never rejects what the exact test would accept); candidate
 // column atoms are then tested exactly against the row atoms with the FP32
@@ -430,13 +433,13 @@ __global__ __launch_bounds__(256) void k_find_tiles(
     int *__restrict__ tile_row,
     uint32_t *__restrict__ tile_atoms,
     int *__restrict__ status) {
-    const int s = blockIdx.y;
+    const int s = blockIdx.x;
     if (!rebuild[s]) {
         return;
     }
     const NbSys S = sys[s];
     const int wave = threadIdx.x >> 6;
-    const int r = blockIdx.x * 4 + wave;
+    const int r = blockIdx.y * 4 + wave;
     if (r >= S.row_blocks) {
         return;
     }
@@ -883,7 +886,7 @@ void NbState::build_only(const double *x, const double *boxes, hipStream_t strea
     k_block_bounds<<<dim3(ceil_div(max_blocks, 4), S), 256, 0, stream>>>(sys.ptr, boxes, rebuild.ptr, pos.ptr,
                                                                          bb_ctr.ptr, bb_ext.ptr);
     TM_LAUNCH_CHECK();
-    k_find_tiles<<<dim3(ceil_div(max_blocks, 4), S), 256, 0, stream>>>(
+    k_find_tiles<<<dim3(S, ceil_div(max_blocks, 4)), 256, 0, stream>>>(
         sys.ptr, boxes, rebuild.ptr, pos.ptr, bb_ctr.ptr, bb_ext.ptr, tile_count.ptr, tile_row.ptr, tile_atoms.ptr,
         status.ptr);
     TM_LAUNCH_CHECK();
@@ -925,7 +928,7 @@ void NbState::prepare(
     k_block_bounds<<<dim3(ceil_div(max_blocks, 4), S), 256, 0, stream>>>(sys.ptr, boxes, rebuild.ptr, pos.ptr,
                                                                          bb_ctr.ptr, bb_ext.ptr);
     TM_LAUNCH_CHECK();
-    k_find_tiles<<<dim3(ceil_div(max_blocks, 4), S), 256, 0, stream>>>(
+    k_find_tiles<<<dim3(S, ceil_div(max_blocks, 4)), 256, 0, stream>>>(
         sys.ptr, boxes, rebuild.ptr, pos.ptr, bb_ctr.ptr, bb_ext.ptr, tile_count.ptr, tile_row.ptr, tile_atoms.ptr,
         status.ptr);
     TM_LAUNCH_CHECK();
