@@ -119,6 +119,32 @@ def test_run_to_run_and_batch_bitwise(ligand):
         np.testing.assert_array_equal(batch.get_box(s), alone[s][5])
 
 
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_deferred_barostat_moves_bitwise(ligand, use_graph):
+    """Inside one multiple_steps call a barostat move is evaluated together with
+    the next step's forces; one step per call never defers. Both give the same
+    trajectory, du/dlambda series and boxes, bit for bit."""
+    lams = [0.0, 0.6]
+
+    def batch():
+        made = [_context(ligand, seed=300 + s, barostat_interval=3) for s in range(len(lams))]
+        return custom_ops.BatchedContexts([m[0] for m in made], lams), made
+
+    b1, keep1 = batch()
+    dl1, _, _ = b1.multiple_steps(31, store_du_dl_interval=1, use_graph=use_graph)
+    b2, keep2 = batch()
+    dl2 = np.concatenate([b2.multiple_steps(1, store_du_dl_interval=1, use_graph=use_graph)[0] for _ in range(31)],
+                         axis=1)
+    np.testing.assert_array_equal(dl1, dl2)
+    vols = set()
+    for s in range(len(lams)):
+        np.testing.assert_array_equal(b1.get_x_t(s), b2.get_x_t(s))
+        np.testing.assert_array_equal(b1.get_v_t(s), b2.get_v_t(s))
+        np.testing.assert_array_equal(b1.get_box(s), b2.get_box(s))
+        vols.add(float(np.prod(np.diag(b1.get_box(s)))))
+    assert all(abs(v - 4.0**3) > 1e-9 for v in vols)  # moves were accepted
+
+
 def test_graph_and_eager_paths_agree(ligand):
     ctxs = [_context(ligand, seed=7, barostat_interval=5)[0] for _ in range(2)]
     b1 = custom_ops.BatchedContexts([ctxs[0]], [0.3])

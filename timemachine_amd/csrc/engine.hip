@@ -478,6 +478,23 @@ __global__ void k_baro_apply(int A, const int *__restrict__ atom_sys, const int 
     x[a * 3 + 2] = x_after[a * 3 + 2];
 }
 
+// A barostat move evaluated with forces (see Engine::run): systems that
+// accepted the move take the forces and du/dlambda of the moved state.
+__global__ void k_baro_take_trial(int A, int S, const int *__restrict__ atom_sys, const int *__restrict__ accept,
+                                  const uint64_t *__restrict__ dx_trial, uint64_t *__restrict__ dx,
+                                  const uint64_t *__restrict__ dl_trial, uint64_t *__restrict__ dl) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < S && dl && accept[a]) {
+        dl[a] = dl_trial[a];
+    }
+    if (a >= A || !accept[atom_sys[a]]) {
+        return;
+    }
+    dx[a * 3 + 0] = dx_trial[a * 3 + 0];
+    dx[a * 3 + 1] = dx_trial[a * 3 + 1];
+    dx[a * 3 + 2] = dx_trial[a * 3 + 2];
+}
+
 // Flattened potential terms of one system.
 struct Flat {
     std::shared_ptr<NonbondedAllPairs> nb;
@@ -1113,16 +1130,63 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
     launch_fill_i32(I.run_step.ptr, S_, 0, stream_);
     launch_fill_u64(I.du_dl_buf.ptr, S_, 0, stream_);
 
+    // Deferred barostat moves. A move due after step j needs U(x) and U(x')
+    // at the state after the Langevin update, and step j + 1 then computes the
+    // forces at x or x'. When nothing reads the state in between (no frame or
+    // window energies stored at step j, not the run's last step) and lambda
+    // does not change between the steps, the move is evaluated at the start of
+    // step j + 1 instead, with forces: U(x) comes with the forces at x, U(x')
+    // with the forces at x', and each system keeps the forces of the state it
+    // ends in. The same values in the same order as evaluating the move first,
+    // two force evaluations instead of one plus two energy-only ones.
+    const bool can_defer = I.has_baro && !lambda_sched;
+    DeviceArray<uint64_t> dx_trial, dl_trial;
+    if (can_defer) {
+        dx_trial.alloc(static_cast<size_t>(A_) * 3);
+        dl_trial.alloc(S_);
+    }
+    uint64_t *const dl_buf = du_dl_interval > 0 ? I.du_dl_buf.ptr : nullptr;
+
     const int TPB = 256;
     const int obs_k = I.obs_interval;
-    auto one_step = [&](bool baro_now, bool obs_now, bool u_now) {
+    auto one_step = [&](bool baro_now, bool obs_now, bool u_now, bool defer_now, bool resolve_now) {
         if (lambda_sched) {
             k_set_lambda<<<ceil_div(S_, 64), 64, 0, stream_>>>(S_, sched.ptr, n_steps, I.run_step.ptr, I.lam.ptr);
             TM_LAUNCH_CHECK();
         }
         launch_fill_u64(I.du_dx.ptr, static_cast<size_t>(A_) * 3, 0, stream_);
-        I.eval(I.x.ptr, I.box.ptr, I.du_dx.ptr, du_dl_interval > 0 ? I.du_dl_buf.ptr : nullptr, nullptr, stream_,
-               tile_grid);
+        if (resolve_now) {
+            // the move set up at the end of the previous step
+            const BaroDev b = I.baro_dev();
+            launch_fill_u64(dx_trial.ptr, static_cast<size_t>(A_) * 3, 0, stream_);
+            launch_fill_u64(dl_trial.ptr, S_, 0, stream_);
+            I.eval(I.x.ptr, I.box.ptr, I.du_dx.ptr, dl_buf, I.b_u_before.ptr, stream_, tile_grid);
+            if (I.nmol_total > 0) {
+                k_baro_zero_centroids<<<ceil_div(I.nmol_total, TPB), TPB, 0, stream_>>>(
+                    I.nmol_total, I.mol_sys.ptr, I.b_trig.ptr, I.centroids.ptr);
+                TM_LAUNCH_CHECK();
+                k_baro_centroids<<<ceil_div(I.G, TPB), TPB, 0, stream_>>>(I.G, I.g_atom.ptr, I.g_mol.ptr,
+                                                                       I.mol_sys.ptr, I.b_trig.ptr, I.x.ptr,
+                                                                       I.centroids.ptr);
+                TM_LAUNCH_CHECK();
+            }
+            k_baro_rescale<<<ceil_div(A_, TPB), TPB, 0, stream_>>>(
+                A_, I.atom_sys.ptr, I.atom_mol.ptr, I.mol_count.ptr, I.b_trig.ptr, I.b_lscale.ptr, I.box.ptr,
+                I.centroids.ptr, I.x.ptr, I.x_after.ptr);
+            TM_LAUNCH_CHECK();
+            I.eval(I.x_after.ptr, I.box_after.ptr, dx_trial.ptr, dl_buf ? dl_trial.ptr : nullptr, I.b_u_after.ptr,
+                   stream_, tile_grid);
+            k_baro_decide<<<ceil_div(S_, 64), 64, 0, stream_>>>(S_, b, I.box.ptr, I.box_after.ptr);
+            TM_LAUNCH_CHECK();
+            k_baro_apply<<<ceil_div(A_, TPB), TPB, 0, stream_>>>(A_, I.atom_sys.ptr, I.b_accept.ptr, I.x_after.ptr,
+                                                               I.x.ptr);
+            TM_LAUNCH_CHECK();
+            k_baro_take_trial<<<ceil_div(std::max(A_, S_), TPB), TPB, 0, stream_>>>(
+                A_, S_, I.atom_sys.ptr, I.b_accept.ptr, dx_trial.ptr, I.du_dx.ptr, dl_trial.ptr, dl_buf);
+            TM_LAUNCH_CHECK();
+        } else {
+            I.eval(I.x.ptr, I.box.ptr, I.du_dx.ptr, dl_buf, nullptr, stream_, tile_grid);
+        }
         if (obs_now) {
             I.observe(stream_, tile_grid);
         }
@@ -1134,7 +1198,7 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
             const BaroDev b = I.baro_dev();
             k_baro_setup<<<ceil_div(S_, 64), 64, 0, stream_>>>(S_, b, I.box.ptr, I.box_after.ptr);
             TM_LAUNCH_CHECK();
-            if (baro_now) {
+            if (baro_now && !defer_now) {
                 I.eval(I.x.ptr, I.box.ptr, nullptr, nullptr, I.b_u_before.ptr, stream_, tile_grid);
                 if (I.nmol_total > 0) {
                     k_baro_zero_centroids<<<ceil_div(I.nmol_total, TPB), TPB, 0, stream_>>>(
@@ -1199,6 +1263,11 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
     auto obs_at = [&](int j) { return obs_k > 0 && (j + 1) % obs_k == 0; };
     // energies at the windows: the state after run step j, (j + 1) % u_interval == 0
     auto u_at = [&](int j) { return Fu > 0 && (j + 1) % u_interval == 0; };
+    // a barostat move after step j evaluated at the start of step j + 1 (above)
+    auto defer_at = [&](int j) {
+        return can_defer && baro_at(j) && j + 1 < n_steps && !u_at(j) && !(x_interval > 0 && (j + 1) % x_interval == 0);
+    };
+    auto resolve_at = [&](int j) { return j > 0 && defer_at(j - 1); };
 
     int j = 0;
     if (use_graph) {
@@ -1216,7 +1285,8 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
             std::vector<uint8_t> pattern(chunk);
             for (int q = 0; q < chunk; q++) {
                 pattern[q] = static_cast<uint8_t>((baro_at(j + q) ? 1 : 0) | (obs_at(j + q) ? 2 : 0) |
-                                                  (u_at(j + q) ? 4 : 0));
+                                                  (u_at(j + q) ? 4 : 0) | (defer_at(j + q) ? 8 : 0) |
+                                                  (resolve_at(j + q) ? 16 : 0));
             }
             auto it = cache.find(pattern);
             if (it == cache.end()) {
@@ -1224,7 +1294,8 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
                 hipGraph_t g;
                 TM_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
                 for (int q = 0; q < chunk; q++) {
-                    one_step((pattern[q] & 1) != 0, (pattern[q] & 2) != 0, (pattern[q] & 4) != 0);
+                    one_step((pattern[q] & 1) != 0, (pattern[q] & 2) != 0, (pattern[q] & 4) != 0,
+                             (pattern[q] & 8) != 0, (pattern[q] & 16) != 0);
                 }
                 TM_HIP_CHECK(hipStreamEndCapture(stream_, &g));
                 hipGraphExec_t ge;
@@ -1242,7 +1313,7 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
         }
     }
     for (; j < n_steps; j++) {
-        one_step(baro_at(j), obs_at(j), u_at(j));
+        one_step(baro_at(j), obs_at(j), u_at(j), defer_at(j), resolve_at(j));
     }
     for (int s = 0; s < S_; s++) {
         if (I.h_has_baro[s]) {

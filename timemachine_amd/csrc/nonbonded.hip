@@ -213,10 +213,10 @@ __device__ __forceinline__ void tile_body(
     const double *__restrict__ boxes,
     const bool row_fresh,
     const bool row_flush,
+    uint64_t &uacc,
+    uint64_t &dlacc,
     uint64_t *__restrict__ du_dx,
-    uint64_t *__restrict__ du_dp,
-    uint64_t *__restrict__ du_dl,
-    uint64_t *__restrict__ u) {
+    uint64_t *__restrict__ du_dp) {
     constexpr int PQ = DX ? 3 : 0;
     const int lane = threadIdx.x & 63;
     const uint32_t i = static_cast<uint32_t>(r * TILE + lane);
@@ -256,7 +256,6 @@ __device__ __forceinline__ void tile_body(
     }
     wave_lds_sync();
 
-    uint64_t uacc = 0, dlacc = 0;
     int queued = 0, done = 0; // wave-uniform
     // queue this lane's pair of the round when `in`; drain full batches of 64
     auto queue = [&](const int slot, const bool in) {
@@ -420,18 +419,6 @@ __device__ __forceinline__ void tile_body(
             }
         }
     }
-    if (U) {
-        const uint64_t tot = wave_sum_u64(uacc);
-        if (lane == 0 && tot != 0) {
-            atomic_add_u64(u + S.red_idx, tot);
-        }
-    }
-    if (DL && ALCH) {
-        const uint64_t tot = wave_sum_u64(dlacc);
-        if (lane == 0 && tot != 0) {
-            atomic_add_u64(du_dl + S.red_idx, tot);
-        }
-    }
 }
 
 // Every wave takes a contiguous range of the tile list of all systems (the
@@ -494,6 +481,10 @@ __global__ __launch_bounds__(NB_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
         s = lo;
     }
     bool open = false; // the row accumulators hold a run that is not written yet
+    // u and du/dl of the system's tiles so far (per lane); written with one
+    // wave-reduced atomic per system and wave, not per tile (every tile of a
+    // system adds to the same word)
+    uint64_t uacc = 0, dlacc = 0;
     for (; t < t_end; t++) {
         while (tile_prefix[s + 1] <= t) {
             s++; // tile_prefix[S] = total > t
@@ -511,6 +502,7 @@ __global__ __launch_bounds__(NB_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
         }
         const bool fresh = !open;
         open = !last;
+        const bool sys_last = t + 1 >= t_end || t + 1 >= tile_prefix[s + 1];
         const uint32_t jl = tile_atoms[static_cast<size_t>(kt) * TILE + lane];
         const int i = r * TILE + lane;
         const double4 pi = pos[Sd.slot0 + i];
@@ -532,10 +524,26 @@ __global__ __launch_bounds__(NB_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
         const bool alch = ballot(moving) != 0;
         if (alch) {
             tile_body<true, U, DX, DL, DP, DLDP, NACC, INTERP>(L, Sd, s, r, jl, pi, ai, pj, aj, dpi, dpj, perm, boxes,
-                                                               fresh, last, du_dx, du_dp, du_dl, u);
+                                                               fresh, last, uacc, dlacc, du_dx, du_dp);
         } else if (!DLDP) {
             tile_body<false, U, DX, DL, DP, false, NACC, INTERP>(L, Sd, s, r, jl, pi, ai, pj, aj, dpi, dpj, perm,
-                                                                 boxes, fresh, last, du_dx, du_dp, du_dl, u);
+                                                                 boxes, fresh, last, uacc, dlacc, du_dx, du_dp);
+        }
+        if (sys_last && (U || DL)) {
+            if (U) {
+                const uint64_t tot = wave_sum_u64(uacc);
+                if (lane == 0 && tot != 0) {
+                    atomic_add_u64(u + Sd.red_idx, tot);
+                }
+            }
+            if (DL) {
+                const uint64_t tot = wave_sum_u64(dlacc);
+                if (lane == 0 && tot != 0) {
+                    atomic_add_u64(du_dl + Sd.red_idx, tot);
+                }
+            }
+            uacc = 0;
+            dlacc = 0;
         }
     }
 }
