@@ -145,6 +145,21 @@ def test_deferred_barostat_moves_bitwise(ligand, use_graph):
     assert all(abs(v - 4.0**3) > 1e-9 for v in vols)  # moves were accepted
 
 
+def test_deferred_barostat_moves_with_lambda_schedule_bitwise(ligand):
+    """Context.multiple_steps with a lambda schedule that holds for a few steps and
+    then jumps: moves are deferred only where lambda does not change. One call and
+    one step per call agree bit for bit."""
+    sched = np.repeat([0.2, 0.5, 0.8, 0.35, 0.9], 7)[:31]
+    c1, keep1, _ = _context(ligand, seed=41, barostat_interval=3)
+    dl1, _, _ = c1.multiple_steps(sched, store_du_dl_interval=1)
+    c2, keep2, _ = _context(ligand, seed=41, barostat_interval=3)
+    dl2 = np.concatenate([c2.multiple_steps(sched[j : j + 1], store_du_dl_interval=1)[0] for j in range(len(sched))])
+    np.testing.assert_array_equal(dl1, dl2)
+    np.testing.assert_array_equal(c1.get_x_t(), c2.get_x_t())
+    np.testing.assert_array_equal(c1.get_v_t(), c2.get_v_t())
+    np.testing.assert_array_equal(c1.get_box(), c2.get_box())
+
+
 def test_graph_and_eager_paths_agree(ligand):
     ctxs = [_context(ligand, seed=7, barostat_interval=5)[0] for _ in range(2)]
     b1 = custom_ops.BatchedContexts([ctxs[0]], [0.3])

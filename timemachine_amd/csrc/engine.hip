@@ -1133,13 +1133,14 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
     // Deferred barostat moves. A move due after step j needs U(x) and U(x')
     // at the state after the Langevin update, and step j + 1 then computes the
     // forces at x or x'. When nothing reads the state in between (no frame or
-    // window energies stored at step j, not the run's last step) and lambda
-    // does not change between the steps, the move is evaluated at the start of
-    // step j + 1 instead, with forces: U(x) comes with the forces at x, U(x')
-    // with the forces at x', and each system keeps the forces of the state it
-    // ends in. The same values in the same order as evaluating the move first,
-    // two force evaluations instead of one plus two energy-only ones.
-    const bool can_defer = I.has_baro && !lambda_sched;
+    // window energies stored at step j, not the run's last step) and no
+    // system's lambda changes between the steps, the move is evaluated at the
+    // start of step j + 1 instead, with forces: U(x) comes with the forces at
+    // x, U(x') with the forces at x', and each system keeps the forces of the
+    // state it ends in. The same values in the same order as evaluating the
+    // move first, two force evaluations instead of one plus two energy-only
+    // ones.
+    const bool can_defer = I.has_baro;
     DeviceArray<uint64_t> dx_trial, dl_trial;
     if (can_defer) {
         dx_trial.alloc(static_cast<size_t>(A_) * 3);
@@ -1264,8 +1265,21 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
     // energies at the windows: the state after run step j, (j + 1) % u_interval == 0
     auto u_at = [&](int j) { return Fu > 0 && (j + 1) % u_interval == 0; };
     // a barostat move after step j evaluated at the start of step j + 1 (above)
+    auto same_lambda = [&](int j) {
+        if (!lambda_sched) {
+            return true;
+        }
+        for (int s = 0; s < S_; s++) {
+            if (lambda_sched[static_cast<size_t>(s) * n_steps + j] !=
+                lambda_sched[static_cast<size_t>(s) * n_steps + j + 1]) {
+                return false;
+            }
+        }
+        return true;
+    };
     auto defer_at = [&](int j) {
-        return can_defer && baro_at(j) && j + 1 < n_steps && !u_at(j) && !(x_interval > 0 && (j + 1) % x_interval == 0);
+        return can_defer && baro_at(j) && j + 1 < n_steps && !u_at(j) &&
+               !(x_interval > 0 && (j + 1) % x_interval == 0) && same_lambda(j);
     };
     auto resolve_at = [&](int j) { return j > 0 && defer_at(j - 1); };
 
