@@ -140,6 +140,28 @@ def test_tile_kernel_equals_pair_list_bitwise(ligand, lam):
     assert np.array_equal(ga, gb)
 
 
+@pytest.mark.parametrize("n", [40, 100, 150, 250, 300, 701])
+def test_tile_kernel_equals_pair_list_few_blocks(water, n):
+    """1 to 11 row blocks, some atoms alchemical: every pair within the cutoff lands
+    in exactly one tile (the diagonal blocks keep i < j only), so the tile kernel
+    matches the pair list over all candidate pairs bit for bit."""
+    x = water.x[:n]
+    params = water.nb_params[:n]
+    plane = np.zeros(n, np.int32)
+    offset = (np.arange(n) % 7 == 0).astype(np.int32)
+    pairs = onb.candidate_pairs(x, water.box, water.cutoff).astype(np.int32)
+    ones = np.ones((len(pairs), 2))
+    allp = custom_ops.NonbondedAllPairs_f32(None, plane, offset, water.beta, water.cutoff)
+    plist = custom_ops.NonbondedPairList_f32(None, pairs, ones, plane, offset, water.beta, water.cutoff)
+    for lam in (0.0, 0.3):
+        a = allp.execute_fixed_raw(x, params, water.box, lam)
+        b = plist.execute_fixed_raw(x, params, water.box, lam)
+        assert np.array_equal(a[0], b[0])
+        assert np.array_equal(a[1], b[1])
+        assert np.sum(a[2], dtype=np.uint64) == np.sum(b[2], dtype=np.uint64)
+        assert np.sum(a[3], dtype=np.uint64) == np.sum(b[3], dtype=np.uint64)
+
+
 def test_selective_outputs_are_bitwise_consistent(ligand):
     """All 16 output selections give the same bits for what they compute."""
     pot = ligand.nonbonded().unbound_impl(F32)
