@@ -237,14 +237,43 @@ def test_consistency_random_rows(gpu):
     _consistency(s, rows, False)
 
 
-def test_context_rejects_interaction_group(gpu):
-    s = ts.ligand_in_water(side=3.2, n_waters=1050, n_heavy=24, n_ligand=40, seed=5)
+def test_context_with_interaction_group(gpu):
+    """A Context whose potentials include an interaction group (ligand rows against
+    the rest, beside the full nonbonded and bonded terms): the engine's fixed-point
+    du_dx equals the sum of the separate Potential executions bit for bit, and a
+    batch of two such systems steps each exactly as it steps alone."""
+    from timemachine_amd import _lib
     from timemachine_amd.lib import LangevinIntegrator
 
+    s = ts.ligand_in_water(side=3.2, n_waters=1050, n_heavy=24, n_ligand=40, seed=5)
     ig = potentials.NonbondedInteractionGroup(s.ligand_idxs, s.lambda_plane_idxs, s.lambda_offset_idxs, s.beta,
-                                              s.cutoff).bind(s.nb_params)
-    bps = [ig.bound_impl(F32)]
-    intg = LangevinIntegrator(300.0, 0.0025, 1.0, s.masses, 1).impl()
-    with pytest.raises(RuntimeError, match="NonbondedInteractionGroup"):
-        ctx = custom_ops.Context(s.x, np.zeros_like(s.x), s.box, intg, bps)
-        ctx.step(0.5)
+                                              s.cutoff)
+    bound = list(s.bound_potentials()) + [ig.bind(s.nb_params)]
+
+    def context(seed):
+        bps = [bp.bound_impl(F32) for bp in bound]
+        intg = LangevinIntegrator(300.0, 0.0025, 1.0, s.masses, seed).impl()
+        v0 = ts.maxwell_boltzmann(s.masses, 300.0, seed)
+        return custom_ops.Context(s.x, v0, s.box, intg, bps), (bps, intg)
+
+    ctx, keep = context(1)
+    ctx.step(0.5)
+    raw = np.zeros(s.N * 3, dtype=np.uint64)
+    _lib.call("tm_context_get_du_dx_t_minus_1", ctx.handle, _lib.pu64(raw))
+    total = np.zeros(s.N * 3, dtype=np.uint64)
+    for bp in bound:
+        total += bp.unbound_impl(F32).execute_fixed_raw(s.x, bp.params, s.box, 0.5)[0]  # modulo 2^64
+    np.testing.assert_array_equal(raw, total)
+
+    lams = [0.2, 0.7]
+    alone = []
+    for k, lam in enumerate(lams):
+        c, kp = context(10 + k)
+        du_dl, _, _ = c.multiple_steps(np.full(12, lam), store_du_dl_interval=1)
+        alone.append((du_dl, c.get_x_t()))
+    made = [context(10 + k) for k in range(len(lams))]
+    batch = custom_ops.BatchedContexts([m[0] for m in made], lams)
+    du_dl, _, _ = batch.multiple_steps(12, store_du_dl_interval=1)
+    for k in range(len(lams)):
+        np.testing.assert_array_equal(du_dl[k], alone[k][0])
+        np.testing.assert_array_equal(batch.get_x_t(k), alone[k][1])

@@ -499,6 +499,8 @@ __global__ void k_baro_take_trial(int A, int S, const int *__restrict__ atom_sys
 struct Flat {
     std::shared_ptr<NonbondedAllPairs> nb;
     std::vector<double> nb_params;
+    std::shared_ptr<NonbondedAllPairs> ig; // a NonbondedInteractionGroup (rows then columns)
+    std::vector<double> ig_params;
     std::vector<std::pair<std::shared_ptr<NonbondedPairList>, std::vector<double>>> pls;
     std::vector<std::pair<std::shared_ptr<BondedPotential>, std::vector<double>>> bonded;
 };
@@ -513,8 +515,14 @@ void flatten(const PotentialPtr &p, const double *params, int P, Flat &out) {
         out.nb_params.assign(params, params + P);
         break;
     }
-    case PotKind::NonbondedInteractionGroup:
-        throw Error("Engine: NonbondedInteractionGroup is not supported in a Context (use it through Potential)");
+    case PotKind::NonbondedInteractionGroup: {
+        if (out.ig) {
+            throw Error("Engine: at most one NonbondedInteractionGroup per system is supported");
+        }
+        out.ig = std::static_pointer_cast<NonbondedAllPairs>(p);
+        out.ig_params.assign(params, params + P);
+        break;
+    }
     case PotKind::NonbondedPairList:
         out.pls.emplace_back(std::static_pointer_cast<NonbondedPairList>(p), std::vector<double>(params, params + P));
         break;
@@ -565,6 +573,11 @@ struct Engine::Impl {
     NbState nb;
     DeviceArray<double> nb_params, nb_params1; // nb_params1: final parameters of interpolated systems
     DeviceArray<int> nb_plane, nb_offset;
+    // an interaction group per system (row-subset layout), same arrays
+    bool has_ig = false;
+    NbState ig;
+    DeviceArray<double> ig_params, ig_params1;
+    DeviceArray<int> ig_plane, ig_offset;
 
     // pair lists (index 0: plain, 1: negated)
     int pl_n[2] = {0, 0};
@@ -625,6 +638,11 @@ struct Engine::Impl {
             nb.prepare(xp, nb_params.ptr, nb_params1.n ? nb_params1.ptr : nullptr, boxp, nb_plane.ptr, nb_offset.ptr,
                        lamp, 0.0, st);
             nb.compute(boxp, ddx, nullptr, ddl, uu, st, tile_grid);
+        }
+        if (has_ig) {
+            ig.prepare(xp, ig_params.ptr, ig_params1.n ? ig_params1.ptr : nullptr, boxp, ig_plane.ptr, ig_offset.ptr,
+                       lamp, 0.0, st);
+            ig.compute(boxp, ddx, nullptr, ddl, uu, st, tile_grid);
         }
         const bool gather = bt_gather && ddx;
         for (int neg = 0; neg < 2; neg++) {
@@ -802,32 +820,31 @@ Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
             flatten(bp->potential, bp->h_params.data(), bp->size(), flats[s]);
         }
     }
-    int n_nb = 0;
-    for (auto &f : flats) {
-        n_nb += f.nb ? 1 : 0;
-    }
-    if (n_nb != 0 && n_nb != S_) {
-        throw Error("Engine: either every system or none has a NonbondedAllPairs");
-    }
-    I.has_nb = n_nb == S_;
-    if (I.has_nb) {
+    // One device nonbonded state over every system's term of one kind (the
+    // all-pairs potential, or the interaction group): parameters, plane and
+    // offset indices packed in atom order, one descriptor per system.
+    // Returns whether the parameters are interpolated.
+    auto build_nb = [&](auto term, const char *what, NbState &st, DeviceArray<double> &d_params,
+                        DeviceArray<double> &d_params1, DeviceArray<int> &d_plane, DeviceArray<int> &d_offset,
+                        std::vector<double> &h_params) {
         std::vector<NbSys> ys(S_);
         std::vector<int> subset;
         std::vector<double> params(A_ * 3, 0.0), params1;
         std::vector<int> plane(A_, 0), offset(A_, 0);
-        const bool interp0 = flats[0].nb->interp_;
+        const bool interp0 = term(0).first->interp_;
         if (interp0) {
             params1.assign(A_ * 3, 0.0);
         }
         for (int s = 0; s < S_; s++) {
-            const auto &nb = flats[s].nb;
+            const auto &nb = term(s).first;
+            const std::vector<double> &fp = *term(s).second;
             if (nb->N_ != I.natoms[s]) {
-                throw Error("Engine: NonbondedAllPairs N does not match the system");
+                throw Error(std::string("Engine: ") + what + " N does not match the system");
             }
             if (nb->interp_ != interp0) {
                 throw Error("Engine: either every system or none has interpolated nonbonded parameters");
             }
-            if (flats[s].nb_params.size() != static_cast<size_t>((interp0 ? 6 : 3) * I.natoms[s])) {
+            if (fp.size() != static_cast<size_t>((interp0 ? 6 : 3) * I.natoms[s])) {
                 throw Error("Engine: nonbonded parameter size mismatch");
             }
             NbSys y{};
@@ -837,13 +854,13 @@ Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
             y.red_idx = s;
             y.subset0 = nb->subset_.empty() ? -1 : static_cast<int>(subset.size());
             subset.insert(subset.end(), nb->subset_.begin(), nb->subset_.end());
+            y.nrow = nb->nrow_;
             y.sorted = nb->sorted_ ? 1 : 0;
             y.interp = interp0 ? 1 : 0;
             y.beta = static_cast<float>(nb->beta_);
             y.cutoff_d = nb->cutoff_;
             y.padding_d = nb->padding_;
             ys[s] = y;
-            const auto &fp = flats[s].nb_params;
             const size_t n3 = static_cast<size_t>(I.natoms[s]) * 3;
             std::copy(fp.begin(), fp.begin() + n3, params.begin() + I.atom0[s] * 3);
             if (interp0) {
@@ -852,16 +869,42 @@ Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
             std::copy(nb->plane_.begin(), nb->plane_.end(), plane.begin() + I.atom0[s]);
             std::copy(nb->offset_.begin(), nb->offset_.end(), offset.begin() + I.atom0[s]);
         }
-        I.nb.nb_margin = NB_LIST_MARGIN;
-        I.nb.init(ys, subset);
-        h_nb_params = params;
-        upload_vec(I.nb_params, params);
+        st.nb_margin = NB_LIST_MARGIN;
+        st.init(ys, subset);
+        h_params = params;
+        upload_vec(d_params, params);
         if (interp0) {
-            upload_vec(I.nb_params1, params1);
-            I.interp = true;
+            upload_vec(d_params1, params1);
         }
-        upload_vec(I.nb_plane, plane);
-        upload_vec(I.nb_offset, offset);
+        upload_vec(d_plane, plane);
+        upload_vec(d_offset, offset);
+        return interp0;
+    };
+    int n_nb = 0, n_ig = 0;
+    for (auto &f : flats) {
+        n_nb += f.nb ? 1 : 0;
+        n_ig += f.ig ? 1 : 0;
+    }
+    if (n_nb != 0 && n_nb != S_) {
+        throw Error("Engine: either every system or none has a NonbondedAllPairs");
+    }
+    if (n_ig != 0 && n_ig != S_) {
+        throw Error("Engine: either every system or none has a NonbondedInteractionGroup");
+    }
+    I.has_nb = n_nb == S_;
+    if (I.has_nb) {
+        I.interp = build_nb([&](int s) { return std::make_pair(flats[s].nb, &flats[s].nb_params); },
+                            "NonbondedAllPairs", I.nb, I.nb_params, I.nb_params1, I.nb_plane, I.nb_offset,
+                            h_nb_params);
+    }
+    I.has_ig = n_ig == S_;
+    if (I.has_ig) {
+        std::vector<double> h_ig_params;
+        const bool ig_interp =
+            build_nb([&](int s) { return std::make_pair(flats[s].ig, &flats[s].ig_params); },
+                     "NonbondedInteractionGroup", I.ig, I.ig_params, I.ig_params1, I.ig_plane, I.ig_offset,
+                     h_ig_params);
+        I.interp = I.interp || ig_interp;
     }
     // pair lists
     {
@@ -1365,6 +1408,9 @@ void Engine::set_nblist_padding(double padding) {
     if (I.has_nb) {
         I.nb.set_padding(padding, stream_);
     }
+    if (I.has_ig) {
+        I.ig.set_padding(padding, stream_);
+    }
 }
 
 void Engine::copy_state_from(Engine &src, int src_s, int dst_s) {
@@ -1400,6 +1446,9 @@ void Engine::copy_state_from(Engine &src, int src_s, int dst_s) {
     if (I.has_nb) {
         I.nb.invalidate(stream_); // the new arena layout needs new neighborlists
     }
+    if (I.has_ig) {
+        I.ig.invalidate(stream_);
+    }
     TM_HIP_CHECK(hipStreamSynchronize(stream_));
 }
 
@@ -1413,6 +1462,9 @@ void Engine::set_grad_observer(int interval) {
     }
     if (interval > 0 && I.interp) {
         throw Error("Engine::set_grad_observer: not available for interpolated nonbonded parameters");
+    }
+    if (interval > 0 && I.has_ig) {
+        throw Error("Engine::set_grad_observer: not available with a NonbondedInteractionGroup");
     }
     if (I.Pg == 0) {
         // flat parameter space: nonbonded [A, 3], then bond [B, 2], angle [A, 2], torsion [T, 3]
@@ -1676,13 +1728,16 @@ void Engine::get_barostat_state(int s, double *vscale, int *accepted, int *attem
 
 void Engine::check_status() {
     Impl &I = *impl_;
-    if (!I.has_nb) {
-        return;
+    std::vector<int> st(S_, 0), st_ig(S_, 0);
+    if (I.has_nb) {
+        I.nb.status.download(st.data(), S_, stream_);
     }
-    std::vector<int> st(S_);
-    I.nb.status.download(st.data(), S_, stream_);
+    if (I.has_ig) {
+        I.ig.status.download(st_ig.data(), S_, stream_);
+    }
     TM_HIP_CHECK(hipStreamSynchronize(stream_));
     for (int s = 0; s < S_; s++) {
+        st[s] |= st_ig[s];
         if (st[s] & NB_STATUS_BOX_TOO_SMALL) {
             throw Error("Cutoff with padding is more than half of the box width, neighborlist is no longer reliable");
         }
@@ -1759,6 +1814,10 @@ size_t Engine::device_bytes() const {
     if (I.has_nb) {
         b += I.nb.tile_atoms.bytes() + I.nb.tile_row.bytes() + I.nb.pos.bytes() * 2 + I.nb.perm.bytes() * 2 +
              I.nb.nblist_x.bytes() + I.nb_params.bytes();
+    }
+    if (I.has_ig) {
+        b += I.ig.tile_atoms.bytes() + I.ig.tile_row.bytes() + I.ig.pos.bytes() * 2 + I.ig.perm.bytes() * 2 +
+             I.ig.nblist_x.bytes() + I.ig_params.bytes();
     }
     return b;
 }
