@@ -574,6 +574,7 @@ struct Engine::Impl {
     DeviceArray<double> nb_params, nb_params1; // nb_params1: final parameters of interpolated systems
     DeviceArray<int> nb_plane, nb_offset;
     // an interaction group per system (row-subset layout), same arrays
+    DeviceArray<uint64_t> dx_trial, dl_trial; // forces and du/dl of a deferred barostat move's state
     bool has_ig = false;
     NbState ig;
     DeviceArray<double> ig_params, ig_params1;
@@ -1184,11 +1185,12 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
     // move first, two force evaluations instead of one plus two energy-only
     // ones.
     const bool can_defer = I.has_baro;
-    DeviceArray<uint64_t> dx_trial, dl_trial;
-    if (can_defer) {
-        dx_trial.alloc(static_cast<size_t>(A_) * 3);
-        dl_trial.alloc(S_);
+    if (can_defer && I.dx_trial.n == 0) {
+        I.dx_trial.alloc(static_cast<size_t>(A_) * 3);
+        I.dl_trial.alloc(S_);
     }
+    DeviceArray<uint64_t> &dx_trial = I.dx_trial;
+    DeviceArray<uint64_t> &dl_trial = I.dl_trial;
     uint64_t *const dl_buf = du_dl_interval > 0 ? I.du_dl_buf.ptr : nullptr;
 
     const int TPB = 256;
