@@ -432,14 +432,17 @@ __global__ __launch_bounds__(256) void k_find_tiles(
     int *__restrict__ tile_count,
     int *__restrict__ tile_row,
     uint32_t *__restrict__ tile_atoms,
-    int *__restrict__ status) {
+    int *__restrict__ status,
+    const int pieces) {
     const int s = blockIdx.x;
     if (!rebuild[s]) {
         return;
     }
     const NbSys S = sys[s];
     const int wave = threadIdx.x >> 6;
-    const int r = blockIdx.y * 4 + wave;
+    const int rq = blockIdx.y * 4 + wave;
+    const int r = rq / pieces;
+    const int piece = rq - r * pieces;
     if (r >= S.row_blocks) {
         return;
     }
@@ -503,11 +506,16 @@ __global__ __launch_bounds__(256) void k_find_tiles(
     wave_lds_sync();
 
     int nbuf = 0;
-    // the upper triangle c >= r, or every column block after the rows
-    for (int cbase = S.nrow > 0 ? S.row_blocks : r; cbase < S.nblocks; cbase += TILE) {
+    // the upper triangle c >= r, or every column block after the rows; with
+    // several pieces per row block, this wave's contiguous share of them
+    const int c_first = S.nrow > 0 ? S.row_blocks : r;
+    const int n_cols = S.nblocks - c_first;
+    const int c_begin = c_first + static_cast<int>(static_cast<long long>(n_cols) * piece / pieces);
+    const int c_end = c_first + static_cast<int>(static_cast<long long>(n_cols) * (piece + 1) / pieces);
+    for (int cbase = c_begin; cbase < c_end; cbase += TILE) {
         const int c = cbase + lane;
         bool cand = false;
-        if (c < S.nblocks) {
+        if (c < c_end) {
             const float4 cc = bb_ctr[S.block0 + c];
             const float4 ce = bb_ext[S.block0 + c];
             const float gx = box_gap(rc.x - cc.x, re.x + ce.x, bx, ibx);
@@ -858,6 +866,22 @@ void NbState::init(const std::vector<NbSys> &sys_in, const std::vector<int> &sub
 
 void NbState::invalidate(hipStream_t stream) { launch_fill_i32(rebuild.ptr, S, 1, stream); }
 
+// Waves per row block in the neighborlist build. One wave scans a row
+// block's column blocks one candidate after another, so a build of few
+// systems takes as long as its longest row block (the first ones, upper
+// triangle) while most of the GPU idles. Below ~2k waves in all, each row
+// block's column range is split into up to 8 contiguous pieces, one wave
+// each (every piece ends in its own partial tile: a few more tiles for the
+// pair kernel). Large batches keep one wave per row block.
+int NbState::find_pieces() const {
+    const long long waves = static_cast<long long>(S) * max_blocks;
+    int q = 1;
+    while (q < 8 && waves * q * 2 <= 4096) {
+        q *= 2;
+    }
+    return q;
+}
+
 void NbState::set_padding(double padding, hipStream_t stream) {
     if (!(padding >= 0.0)) {
         throw Error("neighborlist padding must be >= 0");
@@ -886,9 +910,9 @@ void NbState::build_only(const double *x, const double *boxes, hipStream_t strea
     k_block_bounds<<<dim3(ceil_div(max_blocks, 4), S), 256, 0, stream>>>(sys.ptr, boxes, rebuild.ptr, pos.ptr,
                                                                          bb_ctr.ptr, bb_ext.ptr);
     TM_LAUNCH_CHECK();
-    k_find_tiles<<<dim3(S, ceil_div(max_blocks, 4)), 256, 0, stream>>>(
+    k_find_tiles<<<dim3(S, ceil_div(max_blocks * find_pieces(), 4)), 256, 0, stream>>>(
         sys.ptr, boxes, rebuild.ptr, pos.ptr, bb_ctr.ptr, bb_ext.ptr, tile_count.ptr, tile_row.ptr, tile_atoms.ptr,
-        status.ptr);
+        status.ptr, find_pieces());
     TM_LAUNCH_CHECK();
     k_tile_order<<<S, 256, 0, stream>>>(sys.ptr, rebuild.ptr, tile_count.ptr, tile_row.ptr, tile_order.ptr);
     TM_LAUNCH_CHECK();
@@ -928,9 +952,9 @@ void NbState::prepare(
     k_block_bounds<<<dim3(ceil_div(max_blocks, 4), S), 256, 0, stream>>>(sys.ptr, boxes, rebuild.ptr, pos.ptr,
                                                                          bb_ctr.ptr, bb_ext.ptr);
     TM_LAUNCH_CHECK();
-    k_find_tiles<<<dim3(S, ceil_div(max_blocks, 4)), 256, 0, stream>>>(
+    k_find_tiles<<<dim3(S, ceil_div(max_blocks * find_pieces(), 4)), 256, 0, stream>>>(
         sys.ptr, boxes, rebuild.ptr, pos.ptr, bb_ctr.ptr, bb_ext.ptr, tile_count.ptr, tile_row.ptr, tile_atoms.ptr,
-        status.ptr);
+        status.ptr, find_pieces());
     TM_LAUNCH_CHECK();
     k_tile_order<<<S, 256, 0, stream>>>(sys.ptr, rebuild.ptr, tile_count.ptr, tile_row.ptr, tile_order.ptr);
     TM_LAUNCH_CHECK();
