@@ -161,7 +161,7 @@ def serving_leg(n_jobs: int, seed: int = 5) -> dict:
     work = sum(j.atom_steps for j in jobs)
     budget = 32 * 2**30
     log(f"serving: {n_jobs} jobs, {work:.3g} atom-steps; saturated run for capacity")
-    srv = serving.Server(budget, max_chunk=200, du_dl_interval=10)
+    srv = serving.Server(budget, max_chunk=60, du_dl_interval=10)
     t0 = time.perf_counter()
     srv.run(jobs)
     capacity = work / (time.perf_counter() - t0)
@@ -172,7 +172,7 @@ def serving_leg(n_jobs: int, seed: int = 5) -> dict:
         t += rng.exponential(1.0 / rate)
         j.arrival = t
     log(f"serving: Poisson arrivals at {rate:.3g} jobs/s (80% of capacity {capacity:.3g} atom-steps/s)")
-    srv = serving.Server(budget, max_chunk=200, du_dl_interval=10)
+    srv = serving.Server(budget, max_chunk=60, du_dl_interval=10)
     t0 = time.perf_counter()
     served = srv.run(jobs)
     span = time.perf_counter() - t0
