@@ -13,8 +13,8 @@ Modules (each function cites the reference file:line it restates):
               (C restatement in nblist_ref.c, built by oracle/build.py)
   integrator  Langevin coefficients and the BAOAB step
   barostat    MC barostat move and acceptance
+  fire        FIRE minimisation with the reference's numpy types
 
-How each function is pinned (golden data, closed forms, brute force,
-finite differences, invariants) is listed in DESIGN.md and tested in
-tests/test_oracle.py.
+Each function is pinned by golden data, closed forms, brute force, finite
+differences or invariants in tests/test_oracle.py (and tests/test_fire.py).
 """
