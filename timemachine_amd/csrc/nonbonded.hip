@@ -4,14 +4,16 @@
 // Tile kernel (restates the semantics of kernels/k_nonbonded.cuh:311-671 for
 // wave64): one wave per 64 x 64 tile. The reference evaluates every
 // (row, column) slot of a tile; here a first pass over the 64 rounds
-// (lane l meets column slot (l + r) mod 64 in round r) computes only the
-// exact FP64 4D distance and compacts the pairs inside the cutoff into an
-// LDS queue, and a second pass evaluates the queued pairs 64 at a time with
-// every lane busy. Force, du/dp, du/dl and energy terms are rounded to
+// (lane l meets column slot (l + r) mod 64 in round r) screens the pairs
+// (FP32 in a frame local to the row block, or the exact FP64 test) and
+// compacts those inside the cutoff into an LDS queue, and a second pass
+// evaluates the queued pairs 64 at a time with every lane busy, repeating the
+// exact FP64 cutoff test. Force, du/dp, du/dl and energy terms are rounded to
 // 64-bit fixed point per pair and summed exactly (LDS u64 atomics per atom,
-// registers for u and du/dl); results leave through 64-bit global atomics
-// (row and column atoms) and one wave-reduced atomic per tile for u and
-// du/dl. Integer sums do not depend on the order, so the outputs are
+// registers for u and du/dl). Column atoms leave through 64-bit global
+// atomics at the end of each tile, row atoms at the end of a run of tiles of
+// the same row block, u and du/dl with one wave-reduced atomic per system
+// and wave. Integer sums do not depend on the order, so the outputs are
 // bitwise those of evaluating the pairs one by one.
 //
 // Pair-list kernel (kernels/k_nonbonded_pair_list.cuh:17-216): one thread
