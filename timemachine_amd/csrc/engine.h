@@ -12,7 +12,10 @@
 // Per-step sequence (context.cu:240-280 restated for S systems):
 //   zero du_dx -> nonbonded (rebuild check, sort, gather, neighborlist, tiles)
 //   -> pair lists -> bonded -> du_dl store -> BAOAB (integrator.cu:37-86)
-//   -> every `interval` steps the MC barostat (barostat.cu:286-373).
+//   -> every `interval` steps the MC barostat (barostat.cu:286-373); inside a
+//   run, a move whose state nothing reads before the next step is evaluated
+//   at the start of that step together with its forces (same values, same
+//   order, see Engine::run).
 #pragma once
 
 #include <memory>

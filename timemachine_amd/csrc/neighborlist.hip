@@ -9,9 +9,11 @@
 //    the same results, only the tile occupancy changes);
 //  * row block r of the upper-triangular list holds every column atom j >= 64 r
 //    (sorted order) within nb_cutoff of at least one atom of the block.
-// The design is wave64-native: one wave per 64-atom row block, column atoms
-// tested lane-parallel, interacting atoms compacted with ballot/mbcnt into LDS
-// and emitted as full 64-atom tiles.
+// The design is wave64-native: one wave per 64-atom row block (or per piece
+// of its column range when few systems rebuild), column atoms tested
+// lane-parallel, interacting atoms compacted with ballot/mbcnt into LDS and
+// emitted as 64-atom tiles; each rebuilt system's tiles are then listed by
+// row block for the tile kernel.
 #include <algorithm>
 
 #include "nb_pair.h"
