@@ -48,7 +48,7 @@ TILE_TRAFFIC_BYTES = 0.8105e9 + 2.1209e9
 # The HIP-event kernel profile (roofline and kernel_ms) runs at one fixed
 # setting, the one of the committed rocprofv3 summaries, whatever the autotuner
 # chose for the timed steps.
-PROFILE_GRID = 32768
+PROFILE_GRID = 65536
 PROFILE_PADDING_NM = 0.1
 
 
