@@ -44,7 +44,7 @@ FLOP_PAIR_LJ = 13
 # + WRITE_SIZE x 1.0 (factors from a 256 MiB device copy), measured with
 # rocprofv3 --pmc on MI355X (summarised under profiles/ with the commands; the
 # counter program profiles the boxes as built, ~4% more tiles than relaxed).
-TILE_TRAFFIC_BYTES = 0.8105e9 + 2.1209e9
+TILE_TRAFFIC_BYTES = 0.9846e9 + 2.1206e9
 # The HIP-event kernel profile (roofline and kernel_ms) runs at one fixed
 # setting, the one of the committed rocprofv3 summaries, whatever the autotuner
 # chose for the timed steps.
