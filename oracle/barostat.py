@@ -1,7 +1,10 @@
 """Monte Carlo barostat reference (timemachine/cpp/src/barostat.cu:139-373).
 
 Pressure in internal units is bar * AVOGADRO * 1e-25 (barostat.cu:350) and
-kT uses BOLTZ = 0.008314462618 (barostat.cu:12).
+kT uses BOLTZ = 0.008314462618 (barostat.cu:12). The volume proposals and
+acceptance draws are inputs here; their stream is parity unpinned (curand in
+the reference, barostat.cu:298; Philox4x32-10 on the device), only the
+volume distribution is checked.
 """
 
 from __future__ import annotations

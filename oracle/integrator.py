@@ -5,6 +5,11 @@ Boltzmann constant of timemachine/constants.py:3-6); baoab_step restates
 integrator.py:81-88 / cpp/src/integrator.cu:37-69 with cb = -dt/m as
 lib/__init__.py:13-17 passes it:
   v_mid = v + cb * du_dx;  v' = ca * v_mid + cc * noise;  x' = x + dt/2 (v_mid + v').
+
+The noise is an input here. Its stream is parity unpinned: the reference draws
+it from curand XORWOW (integrator.cu:25-27), a closed library, while the device
+draws Philox4x32-10 keyed by (seed, atom, step); only its distribution is
+checked (Philox known-answer vectors, Box-Muller moments, friction-0 runs).
 """
 
 from __future__ import annotations
