@@ -846,7 +846,7 @@ class BatchedContexts(_Handle):
     def set_nblist_padding(self, padding: float):
         call("tm_batch_set_nblist_padding", self._h, float(padding))
 
-    def autotune(self, steps: int = 45, grids=(32768, 65536), paddings=(0.05, 0.075, 0.1, 0.15),
+    def autotune(self, steps: int = 45, grids=(65536, 131072), paddings=(0.05, 0.075, 0.1, 0.15),
                  store_du_dl_interval: int = 1, use_graph: bool = True) -> dict:
         """Chooses the pair-kernel grid and the neighborlist padding (extension).
 
