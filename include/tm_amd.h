@@ -256,7 +256,7 @@ int tm_batch_write_back(tm_batch *b);
 int tm_batch_profile(tm_batch *b, int reps, double *ms_out);
 /* Tiles in the current neighborlists of all systems. */
 int tm_batch_total_tiles(tm_batch *b, int *out);
-/* One-wave workgroups of the pair (tile) kernel (0 = default, CUs x 128). No reference counterpart
+/* One-wave workgroups of the pair (tile) kernel (0 = default, CUs x 128; at most 2^24). No reference counterpart
  * (the reference launches one block per tile); used by the autotuner. */
 int tm_batch_set_tile_grid(tm_batch *b, int grid);
 /* Neighborlist padding of every system of the batch (nm); the counterpart of

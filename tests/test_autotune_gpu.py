@@ -49,3 +49,7 @@ def test_padding_and_grid_do_not_change_bits(job):
     assert np.array_equal(np.concatenate([dl1, dl2], axis=1), dl_ref)
     with pytest.raises(RuntimeError, match="padding must be"):
         b.set_nblist_padding(-1.0)
+    with pytest.raises(RuntimeError, match="tile grid must be"):
+        b.set_tile_grid(-1)
+    with pytest.raises(RuntimeError, match="tile grid must be"):
+        b.set_tile_grid((1 << 24) + 1)

@@ -914,8 +914,10 @@ int tm_batch_total_tiles(tm_batch *b, int *out) {
 
 int tm_batch_set_tile_grid(tm_batch *b, int grid) {
     return guarded([&] {
-        if (grid < 0) {
-            throw Error("tile grid must be >= 0");
+        // the kernel forms wave * chunk in 32-bit integers; 2^24 workgroups keeps
+        // that below 2^31 for any tile count that fits in device memory
+        if (grid < 0 || grid > (1 << 24)) {
+            throw Error("tile grid must be in [0, 16777216]");
         }
         b->eng->tile_grid = grid;
     });
