@@ -431,9 +431,9 @@ __device__ __forceinline__ void tile_body(
 // Every wave takes a contiguous range of the tile list of all systems (the
 // grid is sized to the machine, the tile count is read on the device).
 // amdgpu_waves_per_eu(4): at most 128 VGPRs, so four waves share a SIMD (the
-// LDS of two-wave workgroups allows four as well). The du_dx + du_dl
-// instance needs 130 registers unconstrained; held to 128 (one 8-byte value
-// spills, reloaded once per tile) it ran 0.3 ms per batched step faster.
+// 10 KB of LDS per wave allows four as well). The du_dx + du_dl instance
+// fits in 128 without spills; holding it there (it once took 130) ran 0.3 ms
+// per batched step faster. The u + du_dx + du_dl instance spills 16 bytes.
 template <bool U, bool DX, bool DL, bool DP, bool DLDP = false, bool INTERP = false>
 __global__ __launch_bounds__(NB_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_nonbonded_tiles(
     const NbSys *__restrict__ sys,
