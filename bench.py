@@ -31,15 +31,23 @@ if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
 
 METRIC = "atom-steps/s (batched λ-windows vs single system); p50/p99 job latency"
-PEAK_FP64_TFLOPS = 78.6  # MI355X published FP64 peak (vector and matrix are equal)
-# Algorithmic FP64 work per pair inside the cutoff, du_dx only (FMA = 2,
-# FP32 rsq / rcp / exp seeds = 1): 15 (delta, min-image) + 5 (d^2) + 10
-# (1/d Newton, d, 1/d^2, beta d, x^2) + 4 (exp with remainder) + 6 (1/(1+px)
-# Newton) + 14 (degree-7 erfc polynomial) + 8 (charge product, Coulomb
-# prefactor) + 9 (projection, fixed-point scaling) = 71; + 13 when both
-# epsilons are nonzero (Lennard-Jones).
-FLOP_PAIR_BASE = 71
-FLOP_PAIR_LJ = 13
+PEAK_FP64_TFLOPS = 78.6  # MI355X published FP64 vector peak
+# Algorithmic work per unique pair inside the cutoff, du_dx only, counted from
+# the reference's FP32 pair code (FMA = 2; rsqrt / exp / rcp = 1): 20 for the
+# displacement, minimum image and d^2 (k_nonbonded.cuh:426-434), 27 for the
+# electrostatic force with the A&S erfc (nonbonded_common.cuh:38-48, 91-97),
+# 9 for the force projection and fixed-point scaling (k_nonbonded.cuh:488-496);
+# + 25 when both epsilons are nonzero (Lennard-Jones, nonbonded_common.cuh:
+# 121-139); + 3 for the 4D displacement of pairs with a nonzero w
+# (k_nonbonded.cuh:437-442).
+FLOP_PAIR_BASE = 56
+FLOP_PAIR_LJ = 25
+FLOP_PAIR_ALCH = 3
+# The same count for this build's FP64 pair code (nb_pair.h, with its Newton
+# steps and the exp remainder): 71 per pair, + 13 for Lennard-Jones. Reported
+# beside the algorithmic figure, not used for the roofline.
+IMPL_FLOP_PAIR_BASE = 71
+IMPL_FLOP_PAIR_LJ = 13
 # HBM bytes per launch of the du_dx tile kernel on this workload: FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0 (factors from a 256 MiB device copy), measured with
 # rocprofv3 --pmc on MI355X (summarised under profiles/ with the commands; the
@@ -57,11 +65,12 @@ def log(msg: str) -> None:
 
 
 def pair_counts(sys_, x, box, lam: float):
-    """Exact pairs within the 4D cutoff (and Lennard-Jones pairs) of one state.
+    """Exact pairs within the 4D cutoff of one state: (all, Lennard-Jones, with w != 0).
 
     Atoms are grouped by (w, LJ); for two groups whose w differ by dw the 4D
     test d^2 + dw^2 < rc^2 is a 3D test at sqrt(rc^2 - dw^2), counted with
-    periodic k-d trees (strict inequality)."""
+    periodic k-d trees (strict inequality). A pair is alchemical when either
+    atom has w != 0."""
     from scipy.spatial import cKDTree
 
     L = np.diag(box)
@@ -74,7 +83,7 @@ def pair_counts(sys_, x, box, lam: float):
     for wk, lk in sorted(set(zip(w.tolist(), lj.tolist()))):
         idx = np.nonzero((w == wk) & (lj == lk))[0]
         groups.append((wk, lk, len(idx), cKDTree(xw[idx], boxsize=L)))
-    total = total_lj = 0
+    total = total_lj = total_alch = 0
     for a in range(len(groups)):
         for b in range(a, len(groups)):
             dw = groups[a][0] - groups[b][0]
@@ -87,29 +96,24 @@ def pair_counts(sys_, x, box, lam: float):
             total += c
             if groups[a][1] and groups[b][1]:
                 total_lj += c
-    return total, total_lj
+            if groups[a][0] != 0.0 or groups[b][0] != 0.0:
+                total_alch += c
+    return total, total_lj, total_alch
 
 
-def make_contexts(sys_, lams, replicas, seed0=2022, barostat_interval=15):
-    from timemachine_amd import testsystems as ts
-    from timemachine_amd.lib import LangevinIntegrator, MonteCarloBarostat, custom_ops
-
-    ctxs, keep, lam_list = [], [], []
-    for r in range(replicas):
-        for lam in lams:
-            seed = seed0 + 1000 * r + len(ctxs)
-            bps = [p.bound_impl(np.float32) for p in sys_.bound_potentials()]
-            intg = LangevinIntegrator(300.0, 0.0025, 1.0, sys_.masses, seed).impl()
-            baro = MonteCarloBarostat(sys_.N, 1.0, 300.0, sys_.groups, barostat_interval, seed + 7).impl(bps)
-            v0 = ts.maxwell_boltzmann(sys_.masses, 300.0, seed)
-            ctxs.append(custom_ops.Context(sys_.x, v0, sys_.box, intg, bps, barostat=baro))
-            keep.append((bps, intg, baro))
-            lam_list.append(float(lam))
-    return ctxs, keep, lam_list
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
-def cpu_baseline(sys_, budget_s: float = 12.0):
-    """FP64 oracle (reference formulas) MD steps on the host cores, bounded sample."""
+def cpu_baseline(sys_, runs: int = 5, steps_per_run: int = 3):
+    """FP64 oracle (reference formulas) MD steps on the host cores: one warm-up
+    run, then the median of `runs` timed runs of `steps_per_run` steps each."""
     import torch
 
     from oracle import bonded as ob
@@ -124,39 +128,51 @@ def cpu_baseline(sys_, budget_s: float = 12.0):
     x = sys_.x.copy()
     v = np.zeros_like(x)
     lam = 0.5
-    steps = 0
-    t0 = time.perf_counter()
-    while True:
-        f = onb.nonbonded_pairs(x, sys_.nb_params, sys_.box, lam, sys_.exclusion_idxs, sys_.scales, sys_.beta,
-                                sys_.cutoff, sys_.lambda_plane_idxs, sys_.lambda_offset_idxs)[0]
-        f = f + ob.bonded("bond", x, sys_.bond_params, lam, sys_.bond_idxs)[0]
-        f = f + ob.bonded("angle", x, sys_.angle_params, lam, sys_.angle_idxs)[0]
-        f = f + ob.bonded("torsion", x, sys_.torsion_params, lam, sys_.torsion_idxs)[0]
-        x, v = oi.baoab_step(x, v, f, 0.0025, ca, cbs, cc, rng.normal(size=x.shape))
-        steps += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
+
+    def one_run():
+        nonlocal x, v
+        t0 = time.perf_counter()
+        for _ in range(steps_per_run):
+            f = onb.nonbonded_pairs(x, sys_.nb_params, sys_.box, lam, sys_.exclusion_idxs, sys_.scales, sys_.beta,
+                                    sys_.cutoff, sys_.lambda_plane_idxs, sys_.lambda_offset_idxs)[0]
+            f = f + ob.bonded("bond", x, sys_.bond_params, lam, sys_.bond_idxs)[0]
+            f = f + ob.bonded("angle", x, sys_.angle_params, lam, sys_.angle_idxs)[0]
+            f = f + ob.bonded("torsion", x, sys_.torsion_params, lam, sys_.torsion_idxs)[0]
+            x, v = oi.baoab_step(x, v, f, 0.0025, ca, cbs, cc, rng.normal(size=x.shape))
+        return time.perf_counter() - t0
+
+    one_run()  # warm-up
+    times = sorted(one_run() for _ in range(runs))
+    med = times[len(times) // 2]
     return {
-        "value": sys_.N * steps / dt,
+        "value": sys_.N * steps_per_run / med,
         "unit": "atom-steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{steps} FP64 BAOAB steps (pair-list nonbonded + bonded, Torch CPU) of the {sys_.N}-atom "
-        f"ligand-in-water system at lambda 0.5, {dt:.1f} s",
+        "cpu_model": cpu_model(),
+        "host_cpus": os.cpu_count(),
+        "runs_s": times,
+        "sample": f"median of {runs} runs (after 1 warm-up run) of {steps_per_run} FP64 BAOAB steps (pair-list "
+        f"nonbonded + bonded, Torch CPU, {threads} threads) of the {sys_.N}-atom ligand-in-water system at "
+        "lambda 0.5",
     }
 
 
-def serving_leg(n_jobs: int, seed: int = 5) -> dict:
+def serving_leg(n_jobs: int, step_scale: int, seed: int = 5) -> dict:
     """Continuous batching of an uneven job stream vs one job at a time (FIFO).
 
-    The stream is a reduced form of the serving workload (ligand-in-water jobs,
-    atoms log-uniform, windows and steps uniform). Capacity is measured with all
-    jobs queued at t = 0; the timed stream then arrives as a Poisson process at
-    80% of it, and the same arrivals are replayed through FIFO."""
+    The serving workload's distributions: ligand-in-water jobs with atoms
+    log-uniform in [1800, 36000] (the lower end is 1800, not 1500, because a
+    box must be at least twice cutoff plus padding, 2.62 nm, and that box holds
+    ~1.8k atoms), ligands of 20-60 heavy atoms, U{8..48} windows and
+    U{2000..20000} steps divided by ``step_scale`` (every window FIRE-minimised
+    for 99 steps first, as the reference's simulate does). Capacity is measured
+    with all jobs queued at t = 0; the timed stream then arrives as a Poisson
+    process at 80% of it, and the same arrivals are replayed through FIFO."""
     from timemachine_amd import serving
 
-    cfg = dict(atoms=(1800, 9000), windows=(4, 16), steps=(200, 1000), heavy=(20, 40), minimize_steps=99)
+    steps = (2000 // step_scale, 20000 // step_scale)
+    cfg = dict(atoms=(1800, 36000), windows=(8, 48), steps=steps, heavy=(20, 60), minimize_steps=99)
     jobs = serving.synthetic_stream(n_jobs, rate_per_s=0.0, seed=seed, **cfg)
     work = sum(j.atom_steps for j in jobs)
     budget = 32 * 2**30
@@ -184,8 +200,11 @@ def serving_leg(n_jobs: int, seed: int = 5) -> dict:
     st, sf = serving.latency_stats(served), serving.latency_stats(fifo)
     return {
         "stream": f"{n_jobs} ligand-in-water jobs, atoms log-uniform {cfg['atoms']}, windows U{cfg['windows']}, "
-        f"steps U{cfg['steps']}, {cfg['minimize_steps']} FIRE steps per window first, Poisson arrivals at 80% "
-        "of measured capacity",
+        f"steps U{cfg['steps']} (the serving workload's U(2000, 20000) divided by {step_scale}), "
+        f"{cfg['minimize_steps']} FIRE steps per window first, Poisson arrivals at 80% of measured capacity",
+        "step_scale": step_scale,
+        "job_atoms": [int(j.system.N) for j in jobs],
+        "job_windows": [int(j.windows) for j in jobs],
         "jobs": n_jobs,
         "atom_steps": work,
         "capacity_atom_steps_per_s": capacity,
@@ -214,7 +233,9 @@ def main() -> int:
     ap.add_argument("--replicas", type=int, default=4)
     ap.add_argument("--single-steps", type=int, default=300)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--serving-jobs", type=int, default=24, help="jobs in the serving leg (0 skips it)")
+    ap.add_argument("--serving-jobs", type=int, default=16, help="jobs in the serving leg (0 skips it)")
+    ap.add_argument("--serving-step-scale", type=int, default=20,
+                    help="the serving jobs' U(2000, 20000) steps are divided by this")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--tile-grid", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true", help="skip the grid / padding autotuner")
@@ -245,15 +266,15 @@ def main() -> int:
 
     from timemachine_amd import _lib
     from timemachine_amd import testsystems as ts
-    from timemachine_amd.lib import custom_ops
+    from timemachine_amd.fe import lambda_windows as lw
 
     _lib.load()
     assert _lib.device_count() >= 1, "no HIP device visible"
     sys_ = ts.ligand_in_water()
     lams = np.linspace(0.0, 1.0, args.windows)
-    log(f"building {args.windows * args.replicas} contexts of {sys_.N} atoms")
-    ctxs, keep, lam_list = make_contexts(sys_, lams, args.replicas)
-    batch = custom_ops.BatchedContexts(ctxs, lam_list)
+    log(f"building {args.windows * args.replicas} lambda windows of {sys_.N} atoms in one engine")
+    lam_list, seeds = lw.window_layout(lams, args.replicas)
+    batch = lw.build_batch(sys_, lams, args.replicas, barostat_interval=15)
     tuned = None
     if args.tile_grid:
         batch.set_tile_grid(args.tile_grid)
@@ -263,7 +284,7 @@ def main() -> int:
         log(f"autotuned: {tuned['grid']} workgroups, padding {tuned['padding']} nm")
         for c in tuned["candidates"]:
             log(f"  grid {c['grid']:6d} padding {c['padding']:.3f} nm: {c['ms_per_step']:.3f} ms/step")
-    S = len(ctxs)
+    S = len(batch)
     use_graph = not args.no_graph
     log(f"warmup {args.warmup} batched steps")
     batch.multiple_steps(args.warmup, store_du_dl_interval=1, use_graph=use_graph)
@@ -278,6 +299,7 @@ def main() -> int:
     assert np.all(np.isfinite(du_dl))
     atom_steps = S * sys_.N * args.steps
     value = atom_steps / t_batch
+    dG, dl_means = lw.ti_free_energy(lams, du_dl, args.replicas)
 
     log(f"profiling kernels (HIP events) at grid {PROFILE_GRID}, padding {PROFILE_PADDING_NM} nm")
     batch.set_tile_grid(PROFILE_GRID)
@@ -286,8 +308,7 @@ def main() -> int:
     tiles = batch.total_tiles()
 
     # one system at a time, the same kernels through the Context API
-    single_ctx, single_keep, _ = make_contexts(sys_, [0.5], 1, seed0=99)
-    single = single_ctx[0]
+    single, single_keep = lw.window_context(sys_, 99, barostat_interval=15)
     single.multiple_steps(np.full(args.warmup, 0.5), store_du_dl_interval=1)
     t0 = time.perf_counter()
     single.multiple_steps(np.full(args.single_steps, 0.5), store_du_dl_interval=1)
@@ -301,8 +322,11 @@ def main() -> int:
     states = [(batch.get_x_t(s), batch.get_box(s), lam_list[s]) for s in range(S)]
     with ThreadPoolExecutor(min(16, os.cpu_count() or 1)) as ex:
         counts = list(ex.map(lambda st: pair_counts(sys_, st[0], st[1], float(st[2])), states))
-    pairs_total = sum(p for p, _ in counts)
-    flop = float(sum(FLOP_PAIR_BASE * p + FLOP_PAIR_LJ * plj for p, plj in counts))
+    pairs_total = sum(c[0] for c in counts)
+    pairs_lj = sum(c[1] for c in counts)
+    pairs_alch = sum(c[2] for c in counts)
+    flop = float(FLOP_PAIR_BASE * pairs_total + FLOP_PAIR_LJ * pairs_lj + FLOP_PAIR_ALCH * pairs_alch)
+    impl_flop = float(IMPL_FLOP_PAIR_BASE * pairs_total + IMPL_FLOP_PAIR_LJ * pairs_lj)
     achieved = flop / (prof["nonbonded"] * 1e-3) / 1e12
     roofline = {
         "bound": "valu",
@@ -312,15 +336,21 @@ def main() -> int:
         "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP64_TFLOPS,
         "traffic": TILE_TRAFFIC_BYTES,
+        "flop_per_pair": {"base": FLOP_PAIR_BASE, "lj": FLOP_PAIR_LJ, "alchemical": FLOP_PAIR_ALCH},
         "pairs_per_launch": pairs_total,
+        "lj_pairs_per_launch": pairs_lj,
+        "alchemical_pairs_per_launch": pairs_alch,
         "flop_per_launch": flop,
+        "impl_flop_per_launch": impl_flop,
         "avg_launch_ms": prof["nonbonded"],
         "tiles_per_launch": tiles,
         "pair_evaluations_per_launch": tiles * 64 * 64,
     }
+    log(f"roofline state: {pairs_total} pairs ({pairs_lj} LJ, {pairs_alch} alchemical), {tiles} tiles, "
+        f"{flop:.4g} FLOP per launch in {prof['nonbonded']:.4f} ms")
     serve = None
     if args.serving_jobs > 0:
-        serve = serving_leg(args.serving_jobs)
+        serve = serving_leg(args.serving_jobs, args.serving_step_scale)
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (FP64 oracle)")
@@ -348,6 +378,14 @@ def main() -> int:
             "du_dl_interval": 1,
             "hipgraph": use_graph,
             "autotuned": None if tuned is None else {"grid": tuned["grid"], "padding_nm": tuned["padding"]},
+        },
+        "ti_free_energy_kj_mol": dG,
+        "du_dl_window_means": dl_means.tolist(),
+        "autotune": None if tuned is None else {
+            "chosen": {k: v for k, v in tuned.items() if k != "candidates"},
+            "candidates": tuned["candidates"],
+            "spread": max(c["ms_per_step"] for c in tuned["candidates"])
+            / min(c["ms_per_step"] for c in tuned["candidates"]) - 1.0,
         },
         "single_system": {
             "value": single_rate,

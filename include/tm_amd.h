@@ -153,6 +153,11 @@ int tm_neighborlist_get_tiles(tm_neighborlist *nl, int capacity, int32_t *tile_r
 /* Neighborlist::set_row_idxs, neighborlist.cu:215-243: later builds pair only the NR row atoms
  * with the other atoms. Same validation and messages. */
 int tm_neighborlist_set_row_idxs(tm_neighborlist *nl, int NR, const uint32_t *row_idxs);
+/* Waves per row block of later builds: 0 = chosen from the launch size (8 for
+ * one system, 1 for large batches), or 1, 2, 4, 8. The tile set does not
+ * depend on it. No reference counterpart (the reference always uses one
+ * thread block per row block, k_neighborlist.cuh:168-186); for tests. */
+int tm_neighborlist_set_pieces(tm_neighborlist *nl, int pieces);
 /* Neighborlist::reset_row_idxs, neighborlist.cu:245-260 */
 int tm_neighborlist_reset_row_idxs(tm_neighborlist *nl);
 /* Neighborlist::compute_block_bounds_host, neighborlist.cu:56-81 (64-atom blocks; with row
@@ -222,6 +227,25 @@ void tm_batch_release(tm_batch *b);
  * lambdas: [n_keep + n_new]. `old` is left unchanged and may then be released. */
 int tm_batch_repack(tm_batch *old, int n_keep, const int32_t *keep, int n_new, tm_context *const *new_contexts,
                     const double *lambdas, tm_batch **out);
+/* W lambda windows of one system in one engine, without a Context per window (no reference
+ * counterpart; what fe/estimator.py:172-281 builds one Context at a time for). Every window
+ * starts from x0 [N, 3] and box [3, 3] with its own velocities v0 [W, N, 3], the potentials
+ * bps, the integrator's coefficients with seed seeds[w], and, if barostat is not NULL, the
+ * barostat's settings with seed barostat_seeds[w]; lambdas [W]. A window is bitwise the
+ * Context built from the same arguments. tm_batch_write_back skips these systems. */
+int tm_batch_create_windows(int W, int N, const double *x0, const double *v0, const double *box,
+                            tm_integrator *intg, const uint64_t *seeds, int n_bps, tm_bound_potential *const *bps,
+                            tm_barostat *barostat, const uint64_t *barostat_seeds, const double *lambdas,
+                            tm_batch **out);
+/* Continuous batching (no reference counterpart): a new batch whose system i is system
+ * systems[i] of srcs[i] (state copied device to device; each source system at most once);
+ * lambdas [n]. The sources are left unchanged and may then be released. */
+int tm_batch_gather(int n, tm_batch *const *srcs, const int32_t *systems, const double *lambdas, tm_batch **out);
+/* The barostat move system s made in the last step (no reference counterpart; the values
+ * k_decide_move uses, barostat.cu:220-279): out[9] = triggered, accepted, u_before, u_after,
+ * u_after - u_before, volume change, Metropolis draw, volume scale after adaptation, length
+ * scale. All zero when that step made no move. */
+int tm_batch_get_barostat_move(tm_batch *b, int s, double *out);
 /* Device bytes held by the batch's engine (admission control). */
 int tm_batch_device_bytes(tm_batch *b, uint64_t *out);
 int tm_batch_num_systems(tm_batch *b, int *out);

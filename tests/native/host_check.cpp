@@ -116,6 +116,48 @@ extern "C" void hc_nonbonded_du_dx(int N, const double *x, const double *params,
     }
 }
 
+// The same model with the FP32 pair terms (nb_pair_terms_f32): FP64
+// displacement, minimum image and d^2 (4D with w), FP32 terms, FP64
+// projection of the FP32 prefactor onto the FP64 displacement.
+// Also writes du/dlambda.
+// Pairs m < M_all are added in full (the tile kernel); pairs m >= M_all are
+// exclusions, subtracted with their scales (the negated pair-list kernel).
+extern "C" void hc_nonbonded_du_dx_f32(int N, const double *x, const double *w, const double *dwdl,
+                                       const double *params, const double *box, double beta, double cutoff, int M,
+                                       const int64_t *pairs, const double *scales, int M_all, int f32_terms,
+                                       uint64_t *du_dx, uint64_t *u, uint64_t *du_dl) {
+    const double bx = box[0], by = box[4], bz = box[8];
+    const double ibx = 1.0 / bx, iby = 1.0 / by, ibz = 1.0 / bz;
+    for (int m = 0; m < M; m++) {
+        const bool neg = m >= M_all;
+        const double sq = neg ? scales[2 * (m - M_all)] : 1.0, slj = neg ? scales[2 * (m - M_all) + 1] : 1.0;
+        const int i = (int)pairs[2 * m], j = (int)pairs[2 * m + 1];
+        const double dx = min_image_d(x[3 * i] - x[3 * j], bx, ibx);
+        const double dy = min_image_d(x[3 * i + 1] - x[3 * j + 1], by, iby);
+        const double dz = min_image_d(x[3 * i + 2] - x[3 * j + 2], bz, ibz);
+        const double dw = w[i] - w[j];
+        const double d2 = __builtin_fma(dw, dw, pair_d2(dx, dy, dz));
+        if (!(d2 < cutoff * cutoff)) continue;
+        const float qi = (float)params[3 * i], qj = (float)params[3 * j];
+        const float si = (float)params[3 * i + 1], sj = (float)params[3 * j + 1];
+        const float ei = (float)params[3 * i + 2], ej = (float)params[3 * j + 2];
+        PairTerms t = f32_terms ? nb_pair_terms_f32<true, false>(d2, qi, qj, si, sj, ei, ej, (float)beta, (float)sq,
+                                                                 (float)slj)
+                                : nb_pair_terms<true, false>(d2, qi, qj, si, sj, ei, ej, (double)(float)beta, sq, slj);
+        const uint64_t f[3] = {f64_to_fixed(t.prefactor * dx, FIXED_SCALE), f64_to_fixed(t.prefactor * dy, FIXED_SCALE),
+                               f64_to_fixed(t.prefactor * dz, FIXED_SCALE)};
+        const uint64_t fu = f64_to_fixed(t.u, FIXED_SCALE);
+        const uint64_t fdl =
+            f64_to_fixed(dw * t.prefactor * ((double)(float)dwdl[i] - (double)(float)dwdl[j]), FIXED_SCALE);
+        for (int d = 0; d < 3; d++) {
+            du_dx[3 * i + d] += neg ? 0 - f[d] : f[d];
+            du_dx[3 * j + d] -= neg ? 0 - f[d] : f[d];
+        }
+        u[0] += neg ? 0 - fu : fu;
+        du_dl[0] += neg ? 0 - fdl : fdl;
+    }
+}
+
 extern "C" float hc_half_round(float x) { return half_round(x); }
 
 // FIRE on F = -k (x - c) with the step functions of fire.h and sequential

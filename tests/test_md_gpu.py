@@ -207,3 +207,30 @@ def test_engine_forces_equal_potential_sum_bitwise(ligand):
         pot = bp.unbound_impl(np.float32)
         total += pot.execute_fixed_raw(ligand.x, bp.params, ligand.box, 0.5)[0]  # modulo 2^64
     np.testing.assert_array_equal(raw, total)
+
+
+def test_empty_schedule_raises(ligand):
+    """An empty lambda schedule resolves the store intervals to 0, which the
+    reference rejects (context.cu:60-65)."""
+    ctx, keep, _ = _context(ligand)
+    with pytest.raises(RuntimeError, match="store_du_dl_interval <= 0"):
+        ctx.multiple_steps(np.zeros(0))
+    with pytest.raises(RuntimeError, match="store_u_interval <= 0"):
+        ctx.multiple_steps_U(0.0, 0, [0.0, 1.0], 0, 0)
+
+
+def test_engine_rejects_pair_lists_with_different_planes(ligand):
+    """Pair lists of one system share one per-atom plane / offset array in the
+    engine: a second list with other lambda offsets is refused rather than
+    evaluated with the first list's w coordinates."""
+    from timemachine_amd.lib import potentials
+
+    pl1 = potentials.NonbondedPairListNegated(ligand.exclusion_idxs, ligand.scales, ligand.lambda_plane_idxs,
+                                              ligand.lambda_offset_idxs, 2.0, 1.2).bind(ligand.nb_params)
+    other_offsets = np.zeros_like(ligand.lambda_offset_idxs)
+    pl2 = potentials.NonbondedPairList(ligand.exclusion_idxs[:5], ligand.scales[:5], ligand.lambda_plane_idxs,
+                                       other_offsets, 2.0, 1.2).bind(ligand.nb_params)
+    bps = [pl1.bound_impl(F32), pl2.bound_impl(F32)]
+    intg = LangevinIntegrator(300.0, 0.0025, 1.0, ligand.masses, 1).impl()
+    with pytest.raises(RuntimeError, match="must share parameters, lambda plane and offset"):
+        custom_ops.Context(ligand.x, np.zeros_like(ligand.x), ligand.box, intg, bps)

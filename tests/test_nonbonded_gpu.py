@@ -255,6 +255,21 @@ def test_neighborlist_exact_full_box(ligand, cutoff, order):
     assert [sorted(t) for t in test] == ref32
 
 
+@pytest.mark.parametrize("pieces", [1, 2, 8])
+def test_neighborlist_pieces_exact_full_box(ligand, pieces):
+    """The build's waves per row block (1: large batches; 8: one system alone)
+    do not change the tile set: the 6372-atom box at 1.3 nm equals the FP32
+    predicate bit for bit with each split."""
+    x = ligand.x[_morton_order(ligand.x, ligand.box)]
+    nl = custom_ops.Neighborlist_f32(ligand.N)
+    nl.set_pieces(pieces)
+    test = nl.get_nblist(x, ligand.box, 1.3)
+    ref32 = onblist.fp32_ixn_list(x, ligand.box, 1.3, 64)
+    assert [sorted(t) for t in test] == ref32
+    with pytest.raises(RuntimeError, match="pieces must be"):
+        nl.set_pieces(3)
+
+
 def test_neighborlist_resize_errors(gpu):
     nl = custom_ops.Neighborlist_f32(3)
     with pytest.raises(RuntimeError, match="size is must be at least 1"):

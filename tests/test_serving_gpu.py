@@ -93,10 +93,10 @@ def test_minimized_windows_served_equal_fifo(jobs):
     import dataclasses
 
     mjobs = [dataclasses.replace(j, minimize_steps=20) for j in jobs[:3]]
-    wins = serving.make_window_contexts(mjobs[0])
-    x_min = wins[0][0].get_x_t()
+    wins = serving.make_window_batch(mjobs[0])
+    x_min = wins.get_x_t(0)
     assert not np.array_equal(x_min, mjobs[0].system.x)
-    assert not np.array_equal(wins[0][0].get_x_t(), wins[-1][0].get_x_t())  # each window at its own lambda
+    assert not np.array_equal(wins.get_x_t(0), wins.get_x_t(len(wins) - 1))  # each window at its own lambda
     budget = sum(serving.estimate_system_bytes(j.system.N) * j.windows for j in mjobs[:2]) + 1
     srv = serving.Server(budget, max_chunk=30, du_dl_interval=10, prepare_ahead=2)  # prepared on a second thread
     served = srv.run(mjobs)
@@ -106,3 +106,69 @@ def test_minimized_windows_served_equal_fifo(jobs):
         assert np.array_equal(r.du_dl, q.du_dl), r.job_id
         assert np.array_equal(z.du_dl, q.du_dl), r.job_id
     assert srv.setup_seconds > 0
+
+
+def test_window_batch_equals_contexts_bitwise(jobs):
+    """BatchedContexts.from_windows (one engine, no Context per window) steps every
+    window bitwise as the batch of the W Contexts built from the same arguments."""
+    job = jobs[2]
+    ctxs = serving.make_window_contexts(job)
+    ref = custom_ops.BatchedContexts([c for c, _ in ctxs], job.lambdas)
+    dl_ref, fr_ref, bx_ref = ref.multiple_steps(20, store_du_dl_interval=1, store_x_interval=10)
+    win = serving.make_window_batch(job)
+    dl, fr, bx = win.multiple_steps(20, store_du_dl_interval=1, store_x_interval=10)
+    np.testing.assert_array_equal(dl, dl_ref)
+    np.testing.assert_array_equal(bx, bx_ref)
+    for s in range(job.windows):
+        np.testing.assert_array_equal(fr[s], fr_ref[s])
+        np.testing.assert_array_equal(win.get_v_t(s), ref.get_v_t(s))
+    with pytest.raises(RuntimeError, match="one integrator seed per window"):
+        sys_ = job.system
+        custom_ops.BatchedContexts.from_windows(sys_.x, np.zeros((2, sys_.N, 3)), sys_.box, None, [1], [], [0.0, 1.0])
+
+
+def test_gather_hands_state_over_bitwise(jobs):
+    """gather takes systems from two batches (device to device) and continues them
+    bitwise; a source system may be taken only once."""
+    a = serving.make_window_batch(jobs[0])
+    b = serving.make_window_batch(jobs[1])
+    ra = serving.make_window_batch(jobs[0])
+    rb = serving.make_window_batch(jobs[1])
+    a.multiple_steps(10, store_du_dl_interval=1)
+    b.multiple_steps(10, store_du_dl_interval=1)
+    ra.multiple_steps(10, store_du_dl_interval=1)
+    rb.multiple_steps(10, store_du_dl_interval=1)
+    dl_a, _, _ = ra.multiple_steps(15, store_du_dl_interval=1)
+    dl_b, _, _ = rb.multiple_steps(15, store_du_dl_interval=1)
+    g = custom_ops.BatchedContexts.gather([(b, 1), (a, 2), (a, 0)],
+                                          [jobs[1].lambdas[1], jobs[0].lambdas[2], jobs[0].lambdas[0]])
+    del a, b
+    dl, _, _ = g.multiple_steps(15, store_du_dl_interval=1)
+    np.testing.assert_array_equal(dl[0], dl_b[1])
+    np.testing.assert_array_equal(dl[1], dl_a[2])
+    np.testing.assert_array_equal(dl[2], dl_a[0])
+    np.testing.assert_array_equal(g.get_x_t(1), ra.get_x_t(2))
+    np.testing.assert_array_equal(g.get_box(0), rb.get_box(1))
+    with pytest.raises(RuntimeError, match="each source system may be taken once"):
+        custom_ops.BatchedContexts.gather([(ra, 0), (ra, 0)], [0.0, 0.0])
+    with pytest.raises(RuntimeError, match="out of range"):
+        custom_ops.BatchedContexts.gather([(ra, 7)], [0.0])
+
+
+@pytest.mark.slow
+def test_served_largest_job_beside_smallest_equals_alone(gpu):
+    """The serving workload's extremes in one engine: a 48-window job of ~36k atoms
+    (7.1 nm box) beside an 8-window job of ~1.8k atoms (2.62 nm box, the smallest
+    box that holds twice cutoff plus padding). 30 steps served (chunks of 10,
+    barostat every 15) equal each job run alone, bit for bit."""
+    big = _small_job(10, 12000, 48, 30, 0.0, heavy=60, barostat=15)
+    small = _small_job(11, 580, 8, 30, 0.0, heavy=20, barostat=15)
+    print(f"big job N={big.system.N} x {big.windows}, small job N={small.system.N} x {small.windows}", flush=True)
+    assert big.system.N >= 35000 and small.system.N <= 1900
+    srv = serving.Server(64 * 2**30, max_chunk=10, du_dl_interval=10)
+    served = srv.run([big, small])
+    alone = serving.run_fifo([big, small], du_dl_interval=10)
+    assert max(rec["systems"] for rec in srv.log) == 56  # both jobs resident together
+    for r, q in zip(served, alone):
+        np.testing.assert_array_equal(r.du_dl, q.du_dl)
+        assert np.all(np.isfinite(r.du_dl))

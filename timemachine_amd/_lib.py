@@ -78,6 +78,7 @@ _SIGS = {
     "tm_neighborlist_compute_block_bounds_host": (c_int, [_P, c_int, _PD, _PD, _PD, _PD]),
     "tm_neighborlist_set_row_idxs": (c_int, [_P, c_int, POINTER(c_uint32)]),
     "tm_neighborlist_reset_row_idxs": (c_int, [_P]),
+    "tm_neighborlist_set_pieces": (c_int, [_P, c_int]),
     "tm_langevin_integrator_create": (c_int, [c_int, c_double, c_double, _PD, _PD, c_uint64, _PPV]),
     "tm_integrator_release": (None, [_P]),
     "tm_barostat_create": (
@@ -114,6 +115,11 @@ _SIGS = {
     "tm_batch_get_box": (c_int, [_P, c_int, _PD]),
     "tm_batch_write_back": (c_int, [_P]),
     "tm_batch_repack": (c_int, [_P, c_int, _PI, c_int, _PPV, _PD, _PPV]),
+    "tm_batch_create_windows": (
+        c_int,
+        [c_int, c_int, _PD, _PD, _PD, _P, _PU64, c_int, _PPV, _P, _PU64, _PD, _PPV],
+    ),
+    "tm_batch_gather": (c_int, [c_int, _PPV, _PI, _PD, _PPV]),
     "tm_batch_device_bytes": (c_int, [_P, _PU64]),
     "tm_batch_set_tile_grid": (c_int, [_P, c_int]),
     "tm_batch_set_nblist_padding": (c_int, [_P, c_double]),
@@ -122,6 +128,7 @@ _SIGS = {
     "tm_batch_grad_size": (c_int, [_P, c_int, POINTER(c_int)]),
     "tm_batch_grad_stats": (c_int, [_P, c_int, POINTER(c_int), _PD, _PD, _PD, _PD, _PD]),
     "tm_batch_profile": (c_int, [_P, c_int, _PD]),
+    "tm_batch_get_barostat_move": (c_int, [_P, c_int, _PD]),
     "tm_batch_total_tiles": (c_int, [_P, POINTER(c_int)]),
 }
 

@@ -47,7 +47,9 @@ struct tm_context {
 };
 
 struct tm_batch {
-    std::vector<tm_context *> ctxs;
+    std::vector<tm_context *> ctxs; // the context of each system, nullptr for systems made by
+                                    // tm_batch_create_windows (nothing to write back to)
+    std::vector<SystemSpec> specs;  // potentials, integrator and barostat of each system
     std::unique_ptr<Engine> eng;
 };
 
@@ -412,6 +414,15 @@ int tm_neighborlist_set_row_idxs(tm_neighborlist *nl, int NR, const uint32_t *ro
     });
 }
 
+int tm_neighborlist_set_pieces(tm_neighborlist *nl, int pieces) {
+    return guarded([&] {
+        if (pieces != 0 && pieces != 1 && pieces != 2 && pieces != 4 && pieces != 8) {
+            throw Error("pieces must be 0 (automatic), 1, 2, 4 or 8");
+        }
+        nl->st.pieces_override = pieces;
+    });
+}
+
 int tm_neighborlist_reset_row_idxs(tm_neighborlist *nl) {
     return guarded([&] {
         nl->rows.clear();
@@ -736,6 +747,7 @@ int tm_batch_create(int n, tm_context *const *contexts, const double *lambdas, t
                 specs.push_back(c->spec);
                 b->ctxs.push_back(c);
             }
+            b->specs = specs;
             b->eng.reset(new Engine(specs));
             std::vector<double> lam(lambdas ? lambdas : nullptr, lambdas ? lambdas + n : nullptr);
             if (!lambdas) {
@@ -771,14 +783,13 @@ int tm_batch_repack(tm_batch *old, int n_keep, const int32_t *keep, int n_new, t
         auto *b = new tm_batch();
         try {
             for (int i = 0; i < n_keep; i++) {
-                tm_context *c = old->ctxs[keep[i]];
-                SystemSpec sp = c->spec;
+                SystemSpec sp = old->specs[keep[i]];
                 sp.state_on_device = true; // handed over below, device to device
                 sp.x.clear();
                 sp.v.clear();
                 sp.box.clear();
                 specs.push_back(std::move(sp));
-                b->ctxs.push_back(c);
+                b->ctxs.push_back(old->ctxs[keep[i]]);
             }
             for (int i = 0; i < n_new; i++) {
                 tm_context *c = new_contexts[i];
@@ -786,12 +797,111 @@ int tm_batch_repack(tm_batch *old, int n_keep, const int32_t *keep, int n_new, t
                 specs.push_back(c->spec);
                 b->ctxs.push_back(c);
             }
+            b->specs = specs;
             b->eng.reset(new Engine(specs));
             for (int i = 0; i < n_keep; i++) {
                 b->eng->copy_state_from(*old->eng, keep[i], i);
             }
             std::vector<double> lam(lambdas, lambdas + n_keep + n_new);
             b->eng->set_lambdas(lam);
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+    });
+}
+
+int tm_batch_create_windows(int W, int N, const double *x0, const double *v0, const double *box,
+                            tm_integrator *intg, const uint64_t *seeds, int n_bps, tm_bound_potential *const *bps,
+                            tm_barostat *barostat, const uint64_t *barostat_seeds, const double *lambdas,
+                            tm_batch **out) {
+    return guarded([&] {
+        require(x0, "x0");
+        require(v0, "v0");
+        require(box, "box");
+        require(intg, "integrator");
+        require(seeds, "seeds");
+        require(lambdas, "lambdas");
+        if (W <= 0) {
+            throw Error("tm_batch_create_windows: at least one window is required");
+        }
+        if (static_cast<int>(intg->spec.cbs.size()) != N) {
+            throw Error("Context: integrator size does not match N");
+        }
+        if (barostat) {
+            require(barostat_seeds, "barostat_seeds");
+        }
+        SystemSpec base;
+        base.N = N;
+        base.x.assign(x0, x0 + static_cast<size_t>(N) * 3);
+        base.box.assign(box, box + 9);
+        base.integ = intg->spec;
+        for (int i = 0; i < n_bps; i++) {
+            require(bps[i], "bound potential");
+            base.bps.push_back(bps[i]->p);
+        }
+        std::vector<SystemSpec> specs(W, base);
+        for (int w = 0; w < W; w++) {
+            SystemSpec &sp = specs[w];
+            sp.v.assign(v0 + static_cast<size_t>(w) * N * 3, v0 + static_cast<size_t>(w + 1) * N * 3);
+            sp.integ.seed = seeds[w];
+            if (barostat) {
+                sp.baro = std::make_shared<BarostatSpec>(*barostat->s->spec);
+                sp.baro->seed = barostat_seeds[w];
+                sp.baro_step = 0;
+            }
+        }
+        auto *b = new tm_batch();
+        try {
+            b->ctxs.assign(W, nullptr);
+            b->specs = specs;
+            b->eng.reset(new Engine(specs));
+            b->eng->set_lambdas(std::vector<double>(lambdas, lambdas + W));
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+    });
+}
+
+int tm_batch_gather(int n, tm_batch *const *srcs, const int32_t *systems, const double *lambdas, tm_batch **out) {
+    return guarded([&] {
+        require(srcs, "srcs");
+        require(systems, "systems");
+        require(lambdas, "lambdas");
+        if (n <= 0) {
+            throw Error("tm_batch_gather: the new batch must hold at least one system");
+        }
+        std::set<std::pair<const tm_batch *, int>> seen;
+        for (int i = 0; i < n; i++) {
+            require(srcs[i], "source batch");
+            if (systems[i] < 0 || systems[i] >= srcs[i]->eng->num_systems()) {
+                throw Error("tm_batch_gather: system index out of range");
+            }
+            if (!seen.insert({srcs[i], systems[i]}).second) {
+                throw Error("tm_batch_gather: each source system may be taken once");
+            }
+        }
+        std::vector<SystemSpec> specs;
+        auto *b = new tm_batch();
+        try {
+            for (int i = 0; i < n; i++) {
+                SystemSpec sp = srcs[i]->specs[systems[i]];
+                sp.state_on_device = true; // copied below, device to device
+                sp.x.clear();
+                sp.v.clear();
+                sp.box.clear();
+                specs.push_back(std::move(sp));
+                b->ctxs.push_back(srcs[i]->ctxs[systems[i]]);
+            }
+            b->specs = specs;
+            b->eng.reset(new Engine(specs));
+            for (int i = 0; i < n; i++) {
+                b->eng->copy_state_from(*srcs[i]->eng, systems[i], i);
+            }
+            b->eng->set_lambdas(std::vector<double>(lambdas, lambdas + n));
         } catch (...) {
             delete b;
             throw;
@@ -891,6 +1001,9 @@ int tm_batch_write_back(tm_batch *b) {
         Engine &e = *b->eng;
         for (size_t s = 0; s < b->ctxs.size(); s++) {
             tm_context *c = b->ctxs[s];
+            if (!c) {
+                continue; // a system made by tm_batch_create_windows
+            }
             e.get_x(static_cast<int>(s), c->spec.x.data());
             e.get_v(static_cast<int>(s), c->spec.v.data());
             e.get_box(static_cast<int>(s), c->spec.box.data());
@@ -901,6 +1014,13 @@ int tm_batch_write_back(tm_batch *b) {
             }
             c->eng.reset();
         }
+    });
+}
+
+int tm_batch_get_barostat_move(tm_batch *b, int s, double *out) {
+    return guarded([&] {
+        require(out, "out");
+        b->eng->get_barostat_move(s, out);
     });
 }
 

@@ -125,6 +125,11 @@ class Engine {
     void set_box(int s, const double *in);
     int64_t get_step(int s);
     void get_barostat_state(int s, double *vscale, int *accepted, int *attempted, int *baro_step);
+    // The barostat move of system s decided in the last step (all zero when that
+    // step made none, k_baro_setup clears it each step): [triggered, accepted,
+    // u_before, u_after, u_after - u_before (from the fixed-point sums), volume
+    // change, Metropolis draw r2, volume scale after adaptation, length scale].
+    void get_barostat_move(int s, double *out);
     // Raises the reference's box messages if a system's box went bad.
     void check_status();
     // Average durations (ms) over `reps` launches, timed with HIP events on the

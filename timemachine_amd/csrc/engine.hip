@@ -316,7 +316,7 @@ __global__ void k_obs_accum_sys(int S, const uint64_t *__restrict__ dl, double *
 struct BaroDev {
     int *has;          // [S]
     int *interval;     // [S]
-    int *bstep;        // [S] barostat call counter (reset by set_interval)
+    int *bstep;        // [S] barostat call counter modulo the interval (reset by set_interval)
     uint64_t *seed;    // [S]
     double *pressure;  // [S] internal units (bar * AVOGADRO * 1e-25)
     double *kT;        // [S]
@@ -333,7 +333,13 @@ struct BaroDev {
     uint64_t *u_after;  // [S]
 };
 
-__global__ void k_baro_setup(int S, BaroDev b, const double *__restrict__ box, double *__restrict__ box_after) {
+// The move's random numbers are keyed on the system's absolute MD step (the
+// counter that also keys the Langevin noise, carried across engine rebuilds),
+// not on the call counter, which only holds the phase within the interval: a
+// rebuilt engine (batch hand-over, set_pressure, set_interval) continues the
+// stream instead of repeating its first draws.
+__global__ void k_baro_setup(int S, BaroDev b, const int64_t *__restrict__ step, const double *__restrict__ box,
+                             double *__restrict__ box_after) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) {
         return;
@@ -349,13 +355,14 @@ __global__ void k_baro_setup(int S, BaroDev b, const double *__restrict__ box, d
         return;
     }
     const int call = b.bstep[s] + 1;
-    b.bstep[s] = call;
+    b.bstep[s] = call % b.interval[s];
     if (call % b.interval[s] != 0) {
         return;
     }
     b.trig[s] = 1;
     const uint64_t sd = b.seed[s];
-    Philox4 c{{static_cast<uint32_t>(call), 0u, 0u, RNG_STREAM_BAROSTAT}};
+    const uint64_t st = static_cast<uint64_t>(step[s]);
+    Philox4 c{{0u, static_cast<uint32_t>(st), static_cast<uint32_t>(st >> 32), RNG_STREAM_BAROSTAT}};
     const Philox4 r = philox4x32_10(c, static_cast<uint32_t>(sd), static_cast<uint32_t>(sd >> 32));
     const double r1 = u01_open0(r.v[0], r.v[1]);
     b.r2[s] = u01_open0(r.v[2], r.v[3]);
@@ -917,6 +924,7 @@ Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
         int pl_interp = -1; // unknown, 0 or 1: every pair list of the batch must agree
         for (int s = 0; s < S_; s++) {
             const std::vector<double> *first = nullptr;
+            const NonbondedPairList *first_pl = nullptr;
             for (auto &pr : flats[s].pls) {
                 const auto &pl = pr.first;
                 if (pl->N_ != I.natoms[s]) {
@@ -936,6 +944,7 @@ Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
                 }
                 if (!first) {
                     first = &pr.second;
+                    first_pl = pl.get();
                     std::copy(pr.second.begin(), pr.second.begin() + n3, params.begin() + I.atom0[s] * 3);
                     if (pl_interp) {
                         std::copy(pr.second.begin() + n3, pr.second.begin() + 2 * n3, params1.begin() + I.atom0[s] * 3);
@@ -945,8 +954,11 @@ Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
                     cutoffs[s] = pl->cutoff_;
                     betas[s] = static_cast<float>(pl->beta_);
                 } else if (*first != pr.second || pl->cutoff_ != cutoffs[s] ||
-                           static_cast<float>(pl->beta_) != betas[s]) {
-                    throw Error("Engine: pair lists of one system must share parameters, beta and cutoff");
+                           static_cast<float>(pl->beta_) != betas[s] || pl->plane_ != first_pl->plane_ ||
+                           pl->offset_ != first_pl->offset_) {
+                    // the merged lists share one per-atom parameter, plane and offset array
+                    throw Error("Engine: pair lists of one system must share parameters, lambda plane and offset "
+                                "indices, beta and cutoff");
                 }
                 const int neg = pl->negated_ ? 1 : 0;
                 for (int m = 0; m < pl->M_; m++) {
@@ -1242,7 +1254,7 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
         TM_LAUNCH_CHECK();
         if (I.has_baro) {
             const BaroDev b = I.baro_dev();
-            k_baro_setup<<<ceil_div(S_, 64), 64, 0, stream_>>>(S_, b, I.box.ptr, I.box_after.ptr);
+            k_baro_setup<<<ceil_div(S_, 64), 64, 0, stream_>>>(S_, b, I.step.ptr, I.box.ptr, I.box_after.ptr);
             TM_LAUNCH_CHECK();
             if (baro_now && !defer_now) {
                 I.eval(I.x.ptr, I.box.ptr, nullptr, nullptr, I.b_u_before.ptr, stream_, tile_grid);
@@ -1726,6 +1738,40 @@ void Engine::get_barostat_state(int s, double *vscale, int *accepted, int *attem
     TM_HIP_CHECK(hipMemcpyAsync(accepted, I.b_accepted.ptr + s, sizeof(int), hipMemcpyDeviceToHost, stream_));
     TM_HIP_CHECK(hipMemcpyAsync(attempted, I.b_attempted.ptr + s, sizeof(int), hipMemcpyDeviceToHost, stream_));
     TM_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void Engine::get_barostat_move(int s, double *out) {
+    Impl &I = *impl_;
+    for (int k = 0; k < 9; k++) {
+        out[k] = 0.0;
+    }
+    if (s < 0 || s >= S_) {
+        throw Error("system index out of range");
+    }
+    if (!I.has_baro || !I.h_has_baro[s]) {
+        return;
+    }
+    int trig = 0, acc = 0;
+    uint64_t ub = 0, ua = 0;
+    double dv = 0.0, r2 = 0.0, vs = 0.0, ls = 0.0;
+    TM_HIP_CHECK(hipMemcpyAsync(&trig, I.b_trig.ptr + s, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    TM_HIP_CHECK(hipMemcpyAsync(&acc, I.b_accept.ptr + s, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    TM_HIP_CHECK(hipMemcpyAsync(&ub, I.b_u_before.ptr + s, sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
+    TM_HIP_CHECK(hipMemcpyAsync(&ua, I.b_u_after.ptr + s, sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
+    TM_HIP_CHECK(hipMemcpyAsync(&dv, I.b_dvol.ptr + s, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    TM_HIP_CHECK(hipMemcpyAsync(&r2, I.b_r2.ptr + s, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    TM_HIP_CHECK(hipMemcpyAsync(&vs, I.b_vscale.ptr + s, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    TM_HIP_CHECK(hipMemcpyAsync(&ls, I.b_lscale.ptr + s, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+    out[0] = trig;
+    out[1] = acc;
+    out[2] = fixed_to_double(ub, FIXED_SCALE);
+    out[3] = fixed_to_double(ua, FIXED_SCALE);
+    out[4] = fixed_to_double(ua - ub, FIXED_SCALE); // the energy change exactly as the decision saw it
+    out[5] = dv;
+    out[6] = r2;
+    out[7] = vs;
+    out[8] = ls;
 }
 
 void Engine::check_status() {

@@ -150,6 +150,76 @@ TM_HD inline PairTerms nb_pair_terms(
     return o;
 }
 
+// The same pair terms with FP32 arithmetic after the FP64 squared distance
+// (the reference's FP32 kernel split, nonbonded_common.cuh:31-140, with this
+// file's erfc form evaluated in FP32): one FP32 rsq / rcp / 2^x each, no
+// Newton steps, FP32 products. Coordinates stay FP64 up to d^2, which keeps
+// the periodic minimum image exact. Results are returned widened to FP64.
+template <bool COMPUTE_U, bool COMPUTE_DP>
+TM_HD inline PairTerms nb_pair_terms_f32(
+    double d2_64,
+    float qi,
+    float qj,
+    float sig_i,
+    float sig_j,
+    float eps_i,
+    float eps_j,
+    float beta,
+    float scale_q,
+    float scale_lj) {
+    PairTerms o;
+    const float d2 = static_cast<float>(d2_64);
+    const float inv_d = TM_RSQF(d2);
+    const float d = d2 * inv_d;
+    const float inv_d2 = inv_d * inv_d;
+    const float x = beta * d;
+    const float e = TM_EXP2F(-(x * x) * static_cast<float>(LOG2_E));
+    const float t = TM_RCPF(__builtin_fmaf(static_cast<float>(ERFC_P), x, 1.0f));
+    float poly = static_cast<float>(ERFC_C7);
+    poly = __builtin_fmaf(poly, t, static_cast<float>(ERFC_C6));
+    poly = __builtin_fmaf(poly, t, static_cast<float>(ERFC_C5));
+    poly = __builtin_fmaf(poly, t, static_cast<float>(ERFC_C4));
+    poly = __builtin_fmaf(poly, t, static_cast<float>(ERFC_C3));
+    poly = __builtin_fmaf(poly, t, static_cast<float>(ERFC_C2));
+    poly = __builtin_fmaf(poly, t, static_cast<float>(ERFC_C1));
+    poly = __builtin_fmaf(poly, t, static_cast<float>(ERFC_C0));
+    const float erfc_x = e * poly;
+    const float sq = scale_q * (qi * qj);
+    const float erfc_inv_d = inv_d * erfc_x;
+    float pre = -(sq * inv_d * inv_d2) * (__builtin_fmaf(static_cast<float>(TWO_OVER_SQRT_PI), x, poly) * e);
+    float u = 0.0f;
+    if (COMPUTE_U) {
+        u = sq * erfc_inv_d;
+    }
+    o.sig_grad = 0.0;
+    o.eps_grad = 0.0;
+    o.lj = (eps_i != 0.0f) && (eps_j != 0.0f);
+    if (o.lj) {
+        const float eps_ij = eps_i * eps_j;
+        const float sig_ij = sig_i + sig_j;
+        const float s1 = sig_ij * inv_d;
+        const float s2 = s1 * s1;
+        const float s4 = s2 * s2;
+        const float s6 = s4 * s2;
+        const float se = scale_lj * eps_ij;
+        pre = pre - se * (s6 * inv_d2) * __builtin_fmaf(s6, 48.0f, -24.0f);
+        if (COMPUTE_U || COMPUTE_DP) {
+            const float lj_shape = (s6 - 1.0f) * s6;
+            if (COMPUTE_U) {
+                u = u + 4.0f * se * lj_shape;
+            }
+            if (COMPUTE_DP) {
+                o.sig_grad = 24.0f * se * (sig_ij * s4 * inv_d2) * (2.0f * s6 - 1.0f);
+                o.eps_grad = 4.0f * scale_lj * lj_shape;
+            }
+        }
+    }
+    o.prefactor = pre;
+    o.u = u;
+    o.erfc_inv_d = erfc_inv_d;
+    return o;
+}
+
 // du/dlambda of one pair: the 4D term dw * prefactor * (dw/dl_i - dw/dl_j)
 // and, with interpolated parameters (dp/dl = p1 - p0), the parameter terms
 // (k_nonbonded.cuh:464-507): du/dq_i dq_i/dl + ... for q, sig and eps.

@@ -120,6 +120,7 @@ struct NbState {
     // Force a rebuild of every system on the next prepare().
     void invalidate(hipStream_t stream);
     int find_pieces() const; // waves per row block in the neighborlist build
+    int pieces_override = 0; // > 0: waves per row block forced to this (1, 2, 4 or 8)
     // Neighborlist padding of every system (results do not depend on it; it
     // trades rebuild frequency against tiles). Forces a rebuild.
     void set_padding(double padding, hipStream_t stream);

@@ -876,6 +876,9 @@ void NbState::invalidate(hipStream_t stream) { launch_fill_i32(rebuild.ptr, S, 1
 // each (every piece ends in its own partial tile: a few more tiles for the
 // pair kernel). Large batches keep one wave per row block.
 int NbState::find_pieces() const {
+    if (pieces_override > 0) {
+        return pieces_override;
+    }
     const long long waves = static_cast<long long>(S) * max_blocks;
     int q = 1;
     while (q < 8 && waves * q * 2 <= 4096) {

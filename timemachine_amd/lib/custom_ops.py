@@ -560,6 +560,10 @@ class Neighborlist_f32(_Handle):
         call("tm_neighborlist_reset_row_idxs", self._h)
         self._rows = 0
 
+    def set_pieces(self, pieces: int):
+        """Waves per row block of later builds (0 = automatic); the tile set does not change."""
+        call("tm_neighborlist_set_pieces", self._h, int(pieces))
+
     def set_row_idxs(self, idxs):
         """Later builds pair only these atoms (rows, blocks of 64 in the order given)
         with every other atom (neighborlist.cu:215-243)."""
@@ -718,6 +722,11 @@ class Context(_Handle):
         n = sched.size
         dl_int = n if store_du_dl_interval <= 0 else int(store_du_dl_interval)
         x_int = n if store_x_interval <= 0 else int(store_x_interval)
+        # an empty schedule resolves both intervals to 0 (context.cu:60-65)
+        if dl_int <= 0:
+            raise RuntimeError("store_du_dl_interval <= 0")
+        if x_int <= 0:
+            raise RuntimeError("store_x_interval <= 0")
         if n % x_int != 0:
             print("warning:: length of lambda_schedule modulo store_x_interval does not equal zero")
         if n % dl_int != 0:
@@ -747,6 +756,10 @@ class Context(_Handle):
         n_steps = int(n_steps)
         u_int = n_steps if store_u_interval <= 0 else int(store_u_interval)
         x_int = n_steps if store_x_interval <= 0 else int(store_x_interval)
+        if u_int <= 0:
+            raise RuntimeError("store_u_interval <= 0")
+        if x_int <= 0:
+            raise RuntimeError("store_x_interval <= 0")
         if n_steps % x_int != 0:
             print("warning:: n_steps modulo store_x_interval does not equal zero")
         if n_steps % u_int != 0:
@@ -811,14 +824,58 @@ class BatchedContexts(_Handle):
         return len(self._ctxs)
 
     @classmethod
-    def _wrap(cls, handle, contexts, lambdas):
+    def _wrap(cls, handle, contexts, lambdas, sizes=None):
         obj = cls.__new__(cls)
         _Handle.__init__(obj)
         obj._h = handle
         obj._ctxs = list(contexts)
         obj._lam = f64(lambdas).reshape(-1)
-        obj.sizes = [c.N for c in obj._ctxs]
+        obj.sizes = list(sizes) if sizes is not None else [c.N for c in obj._ctxs]
         return obj
+
+    @classmethod
+    def from_windows(cls, x0, v0s, box, integrator, seeds, bps, lambdas, barostat=None, barostat_seeds=None):
+        """W lambda windows of one system in one engine, no Context per window.
+
+        Window w starts from x0 and box with velocities v0s[w], the bound
+        potentials ``bps``, ``integrator``'s coefficients with seed seeds[w] and,
+        with a barostat, its settings with seed barostat_seeds[w]. Bitwise the
+        batch of the W Contexts built from the same arguments."""
+        x0 = f64(x0)
+        N = x0.shape[0]
+        lam = f64(lambdas).reshape(-1)
+        W = lam.size
+        v0s = f64(v0s).reshape(W, N, 3)
+        seeds = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint64).reshape(-1))
+        if seeds.size != W:
+            raise RuntimeError("one integrator seed per window is required")
+        bseeds = None
+        if barostat is not None:
+            bseeds = np.ascontiguousarray(np.asarray(barostat_seeds, dtype=np.uint64).reshape(-1))
+            if bseeds.size != W:
+                raise RuntimeError("one barostat seed per window is required")
+        bps = list(bps)
+        arr = _lib.handle_array([bp.handle for bp in bps])
+        h = ctypes.c_void_p()
+        call("tm_batch_create_windows", W, N, pd(x0), pd(v0s), pd(f64(box)), integrator.handle, pu64(seeds),
+             len(bps), arr, barostat.handle if barostat is not None else None, pu64(bseeds), pd(lam),
+             ctypes.byref(h))
+        return cls._wrap(h, [None] * W, lam, sizes=[N] * W)
+
+    @staticmethod
+    def gather(parts, lambdas):
+        """A new batch whose system i is system parts[i][1] of batch parts[i][0]
+        (state copied on the device); the sources are left unchanged."""
+        parts = list(parts)
+        lam = f64(lambdas).reshape(-1)
+        if lam.size != len(parts):
+            raise RuntimeError("one lambda per system of the new batch is required")
+        srcs = _lib.handle_array([b.handle for b, _ in parts])
+        idx = i32([int(s) for _, s in parts])
+        h = ctypes.c_void_p()
+        call("tm_batch_gather", len(parts), srcs, pi(idx), pd(lam), ctypes.byref(h))
+        return BatchedContexts._wrap(h, [b._ctxs[int(s)] for b, s in parts], lam,
+                                     sizes=[b.sizes[int(s)] for b, s in parts])
 
     def repack(self, keep: Sequence[int], new_contexts: Sequence[Context], lambdas: Sequence[float]):
         """A new batch with systems ``keep`` of this one (state handed over on the
@@ -833,7 +890,8 @@ class BatchedContexts(_Handle):
         h = ctypes.c_void_p()
         call("tm_batch_repack", self._h, int(keep.size), pi(keep) if keep.size else None, len(new_contexts), arr,
              pd(lam), ctypes.byref(h))
-        return BatchedContexts._wrap(h, [self._ctxs[int(k)] for k in keep] + new_contexts, lam)
+        return BatchedContexts._wrap(h, [self._ctxs[int(k)] for k in keep] + new_contexts, lam,
+                                     sizes=[self.sizes[int(k)] for k in keep] + [c.N for c in new_contexts])
 
     def device_bytes(self) -> int:
         n = ctypes.c_uint64(0)
@@ -987,6 +1045,14 @@ class BatchedContexts(_Handle):
 
     def write_back(self):
         call("tm_batch_write_back", self._h)
+
+    def barostat_move(self, s: int) -> dict:
+        """The barostat move system s made in the last step (all zero if none)."""
+        out = np.zeros(9)
+        call("tm_batch_get_barostat_move", self._h, int(s), pd(out))
+        keys = ["triggered", "accepted", "u_before", "u_after", "delta_u", "delta_volume", "r2", "volume_scale",
+                "length_scale"]
+        return dict(zip(keys, out.tolist()))
 
     def profile(self, reps: int = 20) -> dict:
         """Per-kernel-group durations (ms) timed with HIP events on the batch's stream."""

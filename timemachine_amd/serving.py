@@ -101,27 +101,29 @@ def ti_estimate(lambdas: Sequence[float], du_dl: np.ndarray) -> float:
 
 
 class _Window:
-    __slots__ = ("job", "w", "remaining", "chunks", "ctx", "keep")
+    __slots__ = ("job", "w", "remaining", "chunks")
 
-    def __init__(self, job: Job, w: int, ctx, keep):
+    def __init__(self, job: Job, w: int):
         self.job = job
         self.w = w
         self.remaining = job.n_steps
         self.chunks: List[np.ndarray] = []
-        self.ctx = ctx
-        self.keep = keep
+
+
+def window_seed(job: Job, w: int) -> int:
+    """Integrator seed of window w of a job (the barostat's is this + 17)."""
+    return job.seed * 1009 + w
 
 
 def make_window_contexts(job: Job):
-    """One Context per lambda window of the job (seeded per job and window),
-    FIRE-minimised at its lambda when the job asks for it."""
+    """One Context per lambda window of the job, seeded as make_window_batch
+    seeds its windows (not FIRE-minimised): the one-Context-per-window form of
+    the same job, for checks against the batch."""
     sys_ = job.system
     out = []
-    # the windows of a job share its bound potentials (same parameters; each
-    # engine copies them into its own arena)
     bps = [p.bound_impl(np.float32) for p in sys_.bound_potentials()]
     for w in range(job.windows):
-        seed = job.seed * 1009 + w
+        seed = window_seed(job, w)
         intg = LangevinIntegrator(job.temperature, job.dt, job.friction, sys_.masses, seed).impl()
         baro = None
         if job.barostat_interval > 0:
@@ -130,17 +132,31 @@ def make_window_contexts(job: Job):
         v0 = ts.maxwell_boltzmann(sys_.masses, job.temperature, seed)
         ctx = custom_ops.Context(sys_.x, v0, sys_.box, intg, bps, barostat=baro)
         out.append((ctx, (bps, intg, baro)))
+    return out
+
+
+def make_window_batch(job: Job) -> custom_ops.BatchedContexts:
+    """All lambda windows of a job in one engine (one constructor call, no
+    Context per window), seeded per job and window, and FIRE-minimised at each
+    window's lambda on the device when the job asks for it."""
+    sys_ = job.system
+    bps = [p.bound_impl(np.float32) for p in sys_.bound_potentials()]
+    seeds = [window_seed(job, w) for w in range(job.windows)]
+    # the integrator's seed is replaced per window; its coefficients are shared
+    intg = LangevinIntegrator(job.temperature, job.dt, job.friction, sys_.masses, seeds[0]).impl()
+    baro = None
+    if job.barostat_interval > 0:
+        baro = MonteCarloBarostat(sys_.N, 1.0, job.temperature, sys_.groups, job.barostat_interval,
+                                  seeds[0] + 17).impl(bps)
+    v0s = np.stack([ts.maxwell_boltzmann(sys_.masses, job.temperature, sd) for sd in seeds])
+    batch = custom_ops.BatchedContexts.from_windows(sys_.x, v0s, sys_.box, intg, seeds, bps, job.lambdas,
+                                                     barostat=baro, barostat_seeds=[sd + 17 for sd in seeds])
     if job.minimize_steps > 0:
-        # all windows of the job minimised together on the device, then the
-        # coordinates go back into the contexts
-        batch = custom_ops.BatchedContexts([c for c, _ in out], job.lambdas)
         batch.fire_minimize(n_steps=job.minimize_steps)
         worst = batch.max_force()
         if not np.all(worst < FORCE_LIMIT):
             raise AssertionError(f"job {job.job_id}: forces much greater than expected after minimization")
-        batch.write_back()
-        del batch
-    return out
+    return batch
 
 
 class Server:
@@ -171,7 +187,7 @@ class Server:
 
     def _prepare(self, job: Job):
         t = time.perf_counter()
-        wins = make_window_contexts(job)
+        wins = make_window_batch(job)
         self.setup_seconds += time.perf_counter() - t
         return wins
 
@@ -212,6 +228,7 @@ class Server:
                     prepared[job.job_id] = pool.submit(self._prepare, job)
         # admit, first come first served, while the budget allows
         new: List[_Window] = []
+        new_batches = []
         while pending and pending[0].arrival <= now() and used + self._job_bytes(pending[0]) <= self.budget:
             job = pending[0]
             if pool is not None:
@@ -229,8 +246,9 @@ class Server:
             pending.pop(0)
             used += self._job_bytes(job)
             admitted_at[job.job_id] = now()
-            for w, (ctx, keep) in enumerate(wins):
-                new.append(_Window(job, w, ctx, keep))
+            new_batches.append(wins)
+            for w in range(job.windows):
+                new.append(_Window(job, w))
         if not running and not new:
             # idle until the next arrival
             self._sleep(max(0.0, pending[0].arrival - now()))
@@ -238,10 +256,13 @@ class Server:
         if new:
             tr = now()
             lam = [r.job.lambdas[r.w] for r in running] + [n.job.lambdas[n.w] for n in new]
-            if batch is None:
-                batch = custom_ops.BatchedContexts([n.ctx for n in new], lam)
+            parts = [(batch, s) for s in range(len(running))]
+            parts += [(jb, w) for jb in new_batches for w in range(len(jb))]
+            if batch is None and len(new_batches) == 1:
+                batch = new_batches[0]  # the job's own engine becomes the batch
             else:
-                batch = batch.repack(range(len(running)), [n.ctx for n in new], lam)
+                batch = custom_ops.BatchedContexts.gather(parts, lam)
+            del new_batches
             running += new
             self.repack_seconds += now() - tr
         chunk = min(self.max_chunk, min(r.remaining for r in running))
@@ -290,8 +311,7 @@ def run_fifo(jobs: Sequence[Job], du_dl_interval: int = 10, use_graph: bool = Tr
         if job.arrival > now():
             sleep(job.arrival - now())
         ta = now()
-        wins = make_window_contexts(job)
-        batch = custom_ops.BatchedContexts([c for c, _ in wins], job.lambdas)
+        batch = make_window_batch(job)
         du_dl, _, _ = batch.multiple_steps(job.n_steps, store_du_dl_interval=du_dl_interval, use_graph=use_graph)
         del batch
         results.append(JobResult(job.job_id, du_dl, ti_estimate(job.lambdas, du_dl), job.arrival, ta, now()))
