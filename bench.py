@@ -158,7 +158,19 @@ def cpu_baseline(sys_, runs: int = 5, steps_per_run: int = 3):
     }
 
 
-def serving_leg(n_jobs: int, step_scale: int, seed: int = 5) -> dict:
+def serving_jobs(n_jobs: int, step_scale: int, seed: int = 5):
+    """The serving stream's jobs (built on the host in worker processes, before
+    this process touches the GPU)."""
+    from timemachine_amd import serving
+
+    steps = (2000 // step_scale, 20000 // step_scale)
+    cfg = dict(atoms=(1800, 36000), windows=(8, 48), steps=steps, heavy=(20, 60), minimize_steps=99)
+    workers = min(16, os.cpu_count() or 1)
+    log(f"building {n_jobs} serving jobs on the host ({workers} processes)")
+    return serving.synthetic_stream(n_jobs, rate_per_s=0.0, seed=seed, workers=workers, **cfg), cfg
+
+
+def serving_leg(jobs, cfg, step_scale: int, seed: int = 5) -> dict:
     """Continuous batching of an uneven job stream vs one job at a time (FIFO).
 
     The serving workload's distributions: ligand-in-water jobs with atoms
@@ -171,9 +183,7 @@ def serving_leg(n_jobs: int, step_scale: int, seed: int = 5) -> dict:
     process at 80% of it, and the same arrivals are replayed through FIFO."""
     from timemachine_amd import serving
 
-    steps = (2000 // step_scale, 20000 // step_scale)
-    cfg = dict(atoms=(1800, 36000), windows=(8, 48), steps=steps, heavy=(20, 60), minimize_steps=99)
-    jobs = serving.synthetic_stream(n_jobs, rate_per_s=0.0, seed=seed, **cfg)
+    n_jobs = len(jobs)
     work = sum(j.atom_steps for j in jobs)
     budget = 32 * 2**30
     log(f"serving: {n_jobs} jobs, {work:.3g} atom-steps; saturated run for capacity")
@@ -222,6 +232,110 @@ def serving_leg(n_jobs: int, step_scale: int, seed: int = 5) -> dict:
     }
 
 
+def differentiation_leg(sys_, n_windows: int = 16, n_frames: int = 500, frame_interval: int = 10,
+                        fd_frames: int = 10, seed: int = 2024) -> dict:
+    """Gradient of the TI loss L = (sum_w c_w <du/dlambda>_w - t)^2, t ~ N(0, 5),
+    w.r.t. the ligand's charges, sigmas, epsilons and torsion force constants,
+    through the torch.autograd.Function over the du_dp and d(du/dlambda)/dp
+    kernels, frames recomputed in the backward pass (not stored). Then central
+    differences (h = 1e-4) of the reweighted window averages on the first
+    ``fd_frames`` observed frames of every window, against the same estimator
+    on those frames, for six parameters spread over the blocks."""
+    import torch
+
+    from timemachine_amd.fe import window_gradients as wg
+    from timemachine_amd.lib import potentials
+
+    lams = np.linspace(0.0, 1.0, n_windows)
+    target = float(np.random.default_rng(seed).normal(0.0, 5.0))
+    windows = wg.LambdaWindows(sys_, lams, n_frames=n_frames, frame_interval=frame_interval, seed=seed,
+                               minimize_steps=99)
+    nb = torch.tensor(sys_.nb_params, dtype=torch.float64, requires_grad=True)
+    tk = torch.tensor(sys_.torsion_params[:, 0], dtype=torch.float64, requires_grad=True)
+    log(f"differentiation: {n_windows} windows x {n_frames} frames (every {frame_interval} steps), recompute")
+    t0 = time.perf_counter()
+    loss = wg.ti_loss(nb, tk, windows, target, recompute=True)
+    t_fwd = time.perf_counter() - t0
+    loss.backward()
+    t_total = time.perf_counter() - t0
+    lig = sys_.ligand_idxs
+    g = nb.grad.numpy()
+    grads = {"q": g[lig, 0], "sig": g[lig, 1], "eps": g[lig, 2], "torsion_k": tk.grad.numpy()}
+
+    # finite differences on the first fd_frames observed frames of every window
+    fdw = wg.LambdaWindows(sys_, lams, n_frames=fd_frames, frame_interval=frame_interval, seed=seed,
+                           minimize_steps=99)
+    batch, _ = fdw.batch()
+    batch.set_grad_observer(0)
+    frames = [[] for _ in lams]
+    for _ in range(fd_frames):
+        batch.multiple_steps(frame_interval - 1)
+        for w in range(n_windows):
+            frames[w].append((batch.get_x_t(w), batch.get_box(w)))
+        batch.observe()
+        batch.multiple_steps(1)
+    stats = [batch.grad_stats(w) for w in range(n_windows)]
+    del batch
+    c = wg.trapezoid_weights(lams)
+    analytic = fdw.layout.split(sum(c[w] * wg.window_gradient(stats[w], fdw.kT) for w in range(n_windows)))
+    nb_impl = sys_.nonbonded().unbound_impl(np.float32)
+    tor_impl = potentials.PeriodicTorsion(sys_.torsion_idxs).unbound_impl(np.float32)
+
+    def ti(nb_params, torsion_params, u_ref):
+        total, us = 0.0, []
+        for w, lam in enumerate(lams):
+            a_f, u_f = [], []
+            for x, box in frames[w]:
+                _, _, dl, u = nb_impl.execute_selective(x, nb_params, box, lam, False, False, True, True)
+                _, _, _, ut = tor_impl.execute_selective(x, torsion_params, box, lam, False, False, False, True)
+                a_f.append(dl)
+                u_f.append(u + ut)
+            u_f = np.asarray(u_f)
+            us.append(u_f)
+            if u_ref is not None:
+                logw = -(u_f - u_ref[w]) / fdw.kT
+                wts = np.exp(logw - logw.max())
+                total += c[w] * float(np.sum(np.asarray(a_f) * wts) / np.sum(wts))
+        return total, us
+
+    _, u0 = ti(sys_.nb_params, sys_.torsion_params, None)
+    h = 1e-4
+    picks = [("q", int(lig[0]), 0), ("q", int(lig[5]), 0), ("sig", int(lig[3]), 1), ("sig", int(lig[10]), 1),
+             ("eps", int(lig[7]), 2), ("torsion_k", 2, None)]
+    checks = []
+    for name, i, col in picks:
+        pp, pm = sys_.nb_params.copy(), sys_.nb_params.copy()
+        tp, tm = sys_.torsion_params.copy(), sys_.torsion_params.copy()
+        if col is None:
+            tp[i, 0] += h
+            tm[i, 0] -= h
+            ana = float(analytic["torsion"][i, 0])
+        else:
+            pp[i, col] += h
+            pm[i, col] -= h
+            ana = float(analytic["nonbonded"][i, col])
+        fp, _ = ti(pp, tp, u0)
+        fm, _ = ti(pm, tm, u0)
+        fd = (fp - fm) / (2 * h)
+        checks.append({"param": f"{name}[{i}]", "analytic": ana, "central_fd": fd,
+                       "rel_err": abs(ana - fd) / max(abs(fd), 1e-12)})
+        log(f"  d(TI)/d{name}[{i}]: estimator {ana:.6e}, central difference {fd:.6e}")
+    return {
+        "workload": f"{n_windows} lambda windows x {n_frames} frames (every {frame_interval} steps, NVT) of the "
+        f"{sys_.N}-atom ligand system; L = (TI - t)^2, t = {target:.4f}",
+        "loss": float(loss.detach()),
+        "forward_s": t_fwd,
+        "forward_backward_s": t_total,
+        "device_bytes": windows.last_device_bytes,
+        "gradient_norms": {k: float(np.linalg.norm(v)) for k, v in grads.items()},
+        "n_params": {k: int(v.size) for k, v in grads.items()},
+        "fd_frames_per_window": fd_frames,
+        "fd_h": h,
+        "fd_checks": checks,
+        "fd_max_rel_err": max(ch["rel_err"] for ch in checks),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,6 +350,8 @@ def main() -> int:
     ap.add_argument("--serving-jobs", type=int, default=16, help="jobs in the serving leg (0 skips it)")
     ap.add_argument("--serving-step-scale", type=int, default=20,
                     help="the serving jobs' U(2000, 20000) steps are divided by this")
+    ap.add_argument("--diff-frames", type=int, default=500,
+                    help="frames per window of the differentiation leg (16 windows; 0 skips it)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--tile-grid", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true", help="skip the grid / padding autotuner")
@@ -264,6 +380,10 @@ def main() -> int:
         log(f"copied {n} bytes ({n // 1024} KiB read, {n // 1024} KiB written)")
         return 0
 
+    stream = None
+    if args.serving_jobs > 0 and not args.profile_only:
+        stream = serving_jobs(args.serving_jobs, args.serving_step_scale)
+
     from timemachine_amd import _lib
     from timemachine_amd import testsystems as ts
     from timemachine_amd.fe import lambda_windows as lw
@@ -272,7 +392,8 @@ def main() -> int:
     assert _lib.device_count() >= 1, "no HIP device visible"
     sys_ = ts.ligand_in_water()
     lams = np.linspace(0.0, 1.0, args.windows)
-    log(f"building {args.windows * args.replicas} lambda windows of {sys_.N} atoms in one engine")
+    log(f"building {args.windows * args.replicas} lambda windows of {sys_.N} atoms in one engine "
+        "(FIRE-minimised at their lambdas)")
     lam_list, seeds = lw.window_layout(lams, args.replicas)
     batch = lw.build_batch(sys_, lams, args.replicas, barostat_interval=15)
     tuned = None
@@ -308,7 +429,8 @@ def main() -> int:
     tiles = batch.total_tiles()
 
     # one system at a time, the same kernels through the Context API
-    single, single_keep = lw.window_context(sys_, 99, barostat_interval=15)
+    x_min = lw.build_batch(sys_, [0.5], 1, seed0=99, barostat_interval=15).get_x_t(0)  # FIRE-minimised start
+    single, single_keep = lw.window_context(sys_, 99, barostat_interval=15, x0=x_min)
     single.multiple_steps(np.full(args.warmup, 0.5), store_du_dl_interval=1)
     t0 = time.perf_counter()
     single.multiple_steps(np.full(args.single_steps, 0.5), store_du_dl_interval=1)
@@ -348,9 +470,12 @@ def main() -> int:
     }
     log(f"roofline state: {pairs_total} pairs ({pairs_lj} LJ, {pairs_alch} alchemical), {tiles} tiles, "
         f"{flop:.4g} FLOP per launch in {prof['nonbonded']:.4f} ms")
+    diff = None
+    if args.diff_frames > 0:
+        diff = differentiation_leg(sys_, n_frames=args.diff_frames)
     serve = None
-    if args.serving_jobs > 0:
-        serve = serving_leg(args.serving_jobs, args.serving_step_scale)
+    if stream is not None:
+        serve = serving_leg(stream[0], stream[1], args.serving_step_scale)
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (FP64 oracle)")
@@ -397,6 +522,7 @@ def main() -> int:
         "kernel_ms": prof,
         "roofline": roofline,
         "serving": serve,
+        "differentiation": diff,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -246,6 +246,16 @@ int tm_batch_gather(int n, tm_batch *const *srcs, const int32_t *systems, const 
  * u_after - u_before, volume change, Metropolis draw, volume scale after adaptation, length
  * scale. All zero when that step made no move. */
 int tm_batch_get_barostat_move(tm_batch *b, int s, double *out);
+/* Per-system fault isolation (no reference counterpart; the reference raises for its one system,
+ * nonbonded_all_pairs.cu:315-329). Status bits of each system so far, out [n]: 1 box too small for
+ * cutoff + padding, 2 box not orthorhombic, 4 neighborlist tile buffer overflow, 8 coordinates or
+ * velocities not finite, 16 a fixed-point force sum holds an out-of-range pair term. A faulted
+ * system does not change any other system's results. */
+int tm_batch_get_status(tm_batch *b, int32_t *out);
+/* raise = 0: tm_batch_multiple_steps returns TM_OK with faulted systems (read them with
+ * tm_batch_get_status) instead of failing the whole batch; 1 (default): it raises the first
+ * system's error. */
+int tm_batch_set_raise_faults(tm_batch *b, int raise);
 /* Device bytes held by the batch's engine (admission control). */
 int tm_batch_device_bytes(tm_batch *b, uint64_t *out);
 int tm_batch_num_systems(tm_batch *b, int *out);

@@ -142,19 +142,32 @@ def test_partial_group_idxs(ligand):
 
 
 def test_varying_pressure(ligand):
-    """1000 atm shrinks the box; back at 1 atm (set_pressure, and set_interval,
-    which restarts the call counter) it grows again (tests/test_barostat.py:196-259)."""
+    """Pressure changes take effect (tests/test_barostat.py:196-259): from the
+    same start, 1000 atm ends with a smaller box than 1 atm. Two identical
+    contexts then continue from the same 1000 atm state, one switched to 1 atm
+    with set_pressure (and set_interval, which restarts the call counter): it
+    ends with the larger box. (The freshly built synthetic box first relaxes at
+    any pressure, so volumes are compared with each other, not with the start.)"""
     atm = 1.01325
-    ctx, keep = _context(ligand, 2021, 3, pressure=1000.0 * atm, dt=0.0015)[:2]
-    V0 = float(np.prod(np.diag(ligand.box)))
-    ctx.multiple_steps(np.ones(1000))
-    V1 = float(np.prod(np.diag(ctx.get_box())))
-    assert V0 - V1 > 0.4
-    baro = keep[2]
-    baro.set_pressure(1.0 * atm)
-    baro.set_interval(2)
-    ctx.multiple_steps(np.ones(2000))
-    assert float(np.prod(np.diag(ctx.get_box()))) > V1
+    hi, keep_hi = _context(ligand, 2021, 3, pressure=1000.0 * atm, dt=0.0015)[:2]
+    hi2, keep_hi2 = _context(ligand, 2021, 3, pressure=1000.0 * atm, dt=0.0015)[:2]
+    lo, keep_lo = _context(ligand, 2021, 3, pressure=1.0 * atm, dt=0.0015)[:2]
+    for c in (hi, hi2, lo):
+        c.multiple_steps(np.ones(1000))
+    V_hi = float(np.prod(np.diag(hi.get_box())))
+    V_lo = float(np.prod(np.diag(lo.get_box())))
+    assert float(np.prod(np.diag(hi2.get_box()))) == V_hi  # identical copies
+    print(f"after 1000 steps: V(1000 atm) {V_hi:.4f}, V(1 atm) {V_lo:.4f}", flush=True)
+    assert V_lo - V_hi > 0.4
+    keep_hi[2].set_pressure(1.0 * atm)
+    keep_hi[2].set_interval(2)
+    keep_hi2[2].set_interval(2)
+    hi.multiple_steps(np.ones(2000))
+    hi2.multiple_steps(np.ones(2000))
+    V_sw = float(np.prod(np.diag(hi.get_box())))
+    V_kept = float(np.prod(np.diag(hi2.get_box())))
+    print(f"after 2000 more: switched to 1 atm {V_sw:.4f}, kept at 1000 atm {V_kept:.4f}", flush=True)
+    assert V_sw - V_kept > 0.4
 
 
 def test_move_stream_continues_across_engine_rebuilds(ligand):

@@ -172,3 +172,50 @@ def test_served_largest_job_beside_smallest_equals_alone(gpu):
     for r, q in zip(served, alone):
         np.testing.assert_array_equal(r.du_dl, q.du_dl)
         assert np.all(np.isfinite(r.du_dl))
+
+
+def test_faulted_job_is_evicted_alone(jobs):
+    """A job whose box is smaller than twice cutoff plus padding (2.4 nm) is served
+    beside healthy jobs: its windows are flagged on the device (the pair kernel
+    skips them), the job leaves with the reference's box error, and every other
+    job's du/dlambda series is bitwise the one it gets alone. A job that fails
+    while it is prepared (FIRE on the bad box raises) fails alone as well."""
+    import dataclasses
+
+    side = 2.4
+    sys_bad = ts.ligand_in_water(side=side, n_waters=440, n_heavy=8, n_ligand=12, seed=77)
+    assert sys_bad.box[0, 0] < 2 * (1.2 + 0.1)
+    bad = serving.Job(9, sys_bad, np.linspace(0, 1, 3), 40, seed=99, arrival=0.0, barostat_interval=5)
+    mix = [jobs[0], bad, jobs[1], jobs[3]]
+    srv = serving.Server(64 * 2**30, max_chunk=20, du_dl_interval=10)
+    served = {r.job_id: r for r in srv.run(mix)}
+    alone = {r.job_id: r for r in serving.run_fifo(mix, du_dl_interval=10)}
+    assert served[9].du_dl is None and "Cutoff with padding" in served[9].error
+    assert alone[9].du_dl is None and "Cutoff with padding" in alone[9].error
+    for j in (0, 1, 3):
+        assert served[j].error is None
+        np.testing.assert_array_equal(served[j].du_dl, alone[j].du_dl)
+    assert serving.latency_stats(list(served.values()))["failed"] == 1
+    # failing during preparation (FIRE raises on the bad box)
+    bad_min = dataclasses.replace(bad, minimize_steps=5)
+    res = {r.job_id: r for r in serving.Server(64 * 2**30, max_chunk=20, du_dl_interval=10).run([bad_min, jobs[3]])}
+    assert res[9].du_dl is None and "Cutoff with padding" in res[9].error
+    np.testing.assert_array_equal(res[3].du_dl, alone[3].du_dl)
+
+
+def test_batch_status_reports_bad_box_without_raising(gpu):
+    """set_raise_faults(False): multiple_steps returns, status() flags only the
+    system with the bad box; with the default policy the batch raises."""
+    good = ts.ligand_in_water(side=3.0, n_waters=800, n_heavy=12, n_ligand=22, seed=3)
+    bad = ts.ligand_in_water(side=2.4, n_waters=440, n_heavy=8, n_ligand=12, seed=77)
+    jg = serving.Job(0, good, np.array([0.2, 0.7]), 20, seed=1, barostat_interval=0)
+    jb = serving.Job(1, bad, np.array([0.5]), 20, seed=2, barostat_interval=0)
+    a, b = serving.make_window_batch(jg), serving.make_window_batch(jb)
+    g = custom_ops.BatchedContexts.gather([(a, 0), (b, 0), (a, 1)], [0.2, 0.5, 0.7])
+    g.set_raise_faults(False)
+    g.multiple_steps(10, store_du_dl_interval=5)
+    st = g.status()
+    assert st[1] & custom_ops.STATUS_BOX_TOO_SMALL and st[0] == 0 and st[2] == 0
+    g.set_raise_faults(True)
+    with pytest.raises(RuntimeError, match="Cutoff with padding is more than half of the box width"):
+        g.multiple_steps(5, store_du_dl_interval=5)

@@ -129,6 +129,8 @@ _SIGS = {
     "tm_batch_grad_stats": (c_int, [_P, c_int, POINTER(c_int), _PD, _PD, _PD, _PD, _PD]),
     "tm_batch_profile": (c_int, [_P, c_int, _PD]),
     "tm_batch_get_barostat_move": (c_int, [_P, c_int, _PD]),
+    "tm_batch_get_status": (c_int, [_P, _PI]),
+    "tm_batch_set_raise_faults": (c_int, [_P, c_int]),
     "tm_batch_total_tiles": (c_int, [_P, POINTER(c_int)]),
 }
 

@@ -51,6 +51,7 @@ struct tm_batch {
                                     // tm_batch_create_windows (nothing to write back to)
     std::vector<SystemSpec> specs;  // potentials, integrator and barostat of each system
     std::unique_ptr<Engine> eng;
+    bool raise_faults = true; // false: faulted systems are reported by tm_batch_get_status only
 };
 
 struct tm_neighborlist {
@@ -926,7 +927,22 @@ int tm_batch_multiple_steps(tm_batch *b, int n_steps, const double *lambda_sched
                             double *du_dl_out, double *x_out, double *box_out, int use_graph) {
     return guarded([&] {
         b->eng->run(n_steps, lambda_sched, du_dl_interval, x_interval, du_dl_out, x_out, box_out, use_graph != 0);
-        b->eng->check_status();
+        if (b->raise_faults) {
+            b->eng->check_status();
+        }
+    });
+}
+
+int tm_batch_set_raise_faults(tm_batch *b, int raise) {
+    return guarded([&] { b->raise_faults = raise != 0; });
+}
+
+int tm_batch_get_status(tm_batch *b, int32_t *out) {
+    return guarded([&] {
+        require(out, "out");
+        std::vector<int> st(b->eng->num_systems());
+        b->eng->system_status(st.data());
+        std::copy(st.begin(), st.end(), out);
     });
 }
 

@@ -27,6 +27,11 @@
 
 namespace tmamd {
 
+// Per-system status bits raised by the engine, beside the neighborlist's
+// NB_STATUS_* bits (nb_state.h). Sticky for the life of the engine.
+constexpr int ENGINE_STATUS_NONFINITE = 8;      // coordinates or velocities not finite after a step
+constexpr int ENGINE_STATUS_FIXED_INVALID = 16; // a fixed-point force sum holds an out-of-range pair term
+
 struct IntegratorSpec {
     double dt = 0.0;
     double ca = 0.0;
@@ -130,6 +135,11 @@ class Engine {
     // u_before, u_after, u_after - u_before (from the fixed-point sums), volume
     // change, Metropolis draw r2, volume scale after adaptation, length scale].
     void get_barostat_move(int s, double *out);
+    // OR of the status bits (NB_STATUS_*, ENGINE_STATUS_*) raised for each
+    // system so far: out [S]. A faulted system does not disturb the others
+    // (the pair kernel skips systems with a bad box; every other term is per
+    // system), so callers may retire it and keep the rest.
+    void system_status(int *out);
     // Raises the reference's box messages if a system's box went bad.
     void check_status();
     // Average durations (ms) over `reps` launches, timed with HIP events on the

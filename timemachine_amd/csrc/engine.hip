@@ -30,11 +30,16 @@ __global__ void k_set_lambda(int S, const double *__restrict__ sched, int n_step
 // BAOAB rotated by half a step (integrator.cu:37-69, integrator.py:81-88):
 //   v_mid = v + cb * F, v' = ca v_mid + cc xi, x += dt/2 (v_mid + v'),  cb = -dt/m.
 // Noise: Philox keyed by the system seed, counter (2 * atom + block, step).
+// A system whose new coordinates or velocities are not finite, or whose
+// fixed-point force holds an out-of-range pair term (a non-finite or huge pair
+// contribution converts to 2^63, which leaves |sum| >= 2^62 for any force below
+// 2^26 kJ/mol/nm), raises a status bit for that system only.
 __global__ void k_langevin(int A, const int *__restrict__ atom_sys, const int *__restrict__ atom0,
                            const int64_t *__restrict__ step, const uint64_t *__restrict__ seed,
                            const double *__restrict__ ca, const double *__restrict__ dt,
                            const double *__restrict__ cbs, const double *__restrict__ ccs,
-                           const uint64_t *__restrict__ du_dx, double *__restrict__ x, double *__restrict__ v) {
+                           const uint64_t *__restrict__ du_dx, double *__restrict__ x, double *__restrict__ v,
+                           int *__restrict__ sys_status) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= A) {
         return;
@@ -50,12 +55,25 @@ __global__ void k_langevin(int A, const int *__restrict__ atom_sys, const int *_
     normal2(philox4x32_10(c0, k0, k1), z[0], z[1]);
     normal2(philox4x32_10(c1, k0, k1), z[2], z[3]);
     const double cb = cbs[a], cc = ccs[a], cas = ca[s], h = 0.5 * dt[s];
+    int bad = 0;
     for (int d = 0; d < 3; d++) {
-        const double f = fixed_to_double(du_dx[a * 3 + d], FIXED_SCALE);
+        const uint64_t fx = du_dx[a * 3 + d];
+        const int64_t fi = static_cast<int64_t>(fx);
+        if (fi >= (int64_t(1) << 62) || fi <= -(int64_t(1) << 62)) {
+            bad |= ENGINE_STATUS_FIXED_INVALID;
+        }
+        const double f = fixed_to_double(fx, FIXED_SCALE);
         const double v_mid = v[a * 3 + d] + cb * f;
         const double v_new = cas * v_mid + cc * z[d];
         v[a * 3 + d] = v_new;
-        x[a * 3 + d] += h * (v_mid + v_new);
+        const double x_new = x[a * 3 + d] + h * (v_mid + v_new);
+        x[a * 3 + d] = x_new;
+        if (!isfinite(x_new) || !isfinite(v_new)) {
+            bad |= ENGINE_STATUS_NONFINITE;
+        }
+    }
+    if (bad) {
+        atomicOr(sys_status + s, bad);
     }
 }
 
@@ -574,6 +592,7 @@ struct Engine::Impl {
     DeviceArray<uint64_t> du_dx, seed, u_buf, du_dl_buf;
     DeviceArray<int64_t> step;
     DeviceArray<int> atom_sys, d_atom0, d_natoms, red_idx, run_step;
+    DeviceArray<int> sys_status; // [S] ENGINE_STATUS_* bits (sticky)
 
     // nonbonded
     bool has_nb = false;
@@ -811,6 +830,8 @@ Engine::Engine(const std::vector<SystemSpec> &systems) : impl_(new Impl) {
     upload_vec(I.d_atom0, I.atom0);
     upload_vec(I.d_natoms, I.natoms);
     upload_vec(I.red_idx, hred);
+    I.sys_status.alloc(S_);
+    I.sys_status.zero();
     I.run_step.alloc(S_);
     I.run_step.zero();
     I.du_dx.alloc(std::max(A_ * 3, 1));
@@ -1250,7 +1271,7 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
         }
         k_langevin<<<ceil_div(A_, TPB), TPB, 0, stream_>>>(A_, I.atom_sys.ptr, I.d_atom0.ptr, I.step.ptr, I.seed.ptr,
                                                          I.ca.ptr, I.dt.ptr, I.cbs.ptr, I.ccs.ptr, I.du_dx.ptr,
-                                                         I.x.ptr, I.v.ptr);
+                                                         I.x.ptr, I.v.ptr, I.sys_status.ptr);
         TM_LAUNCH_CHECK();
         if (I.has_baro) {
             const BaroDev b = I.baro_dev();
@@ -1774,18 +1795,26 @@ void Engine::get_barostat_move(int s, double *out) {
     out[8] = ls;
 }
 
-void Engine::check_status() {
+void Engine::system_status(int *out) {
     Impl &I = *impl_;
-    std::vector<int> st(S_, 0), st_ig(S_, 0);
+    std::vector<int> st(S_, 0), st_ig(S_, 0), st_sys(S_, 0);
     if (I.has_nb) {
         I.nb.status.download(st.data(), S_, stream_);
     }
     if (I.has_ig) {
         I.ig.status.download(st_ig.data(), S_, stream_);
     }
+    I.sys_status.download(st_sys.data(), S_, stream_);
     TM_HIP_CHECK(hipStreamSynchronize(stream_));
     for (int s = 0; s < S_; s++) {
-        st[s] |= st_ig[s];
+        out[s] = st[s] | st_ig[s] | st_sys[s];
+    }
+}
+
+void Engine::check_status() {
+    std::vector<int> st(S_, 0);
+    system_status(st.data());
+    for (int s = 0; s < S_; s++) {
         if (st[s] & NB_STATUS_BOX_TOO_SMALL) {
             throw Error("Cutoff with padding is more than half of the box width, neighborlist is no longer reliable");
         }
@@ -1794,6 +1823,13 @@ void Engine::check_status() {
         }
         if (st[s] & NB_STATUS_TILE_OVERFLOW) {
             throw Error("Engine: neighborlist tile buffer overflow");
+        }
+        if (st[s] & ENGINE_STATUS_FIXED_INVALID) {
+            throw Error("Engine: system " + std::to_string(s) +
+                        ": a force left the fixed-point range (non-finite or huge pair terms)");
+        }
+        if (st[s] & ENGINE_STATUS_NONFINITE) {
+            throw Error("Engine: system " + std::to_string(s) + ": coordinates or velocities are not finite");
         }
     }
 }
@@ -1831,12 +1867,13 @@ void Engine::profile(int reps, double *ms_out) {
         I.has_nb = had;
     });
     DeviceArray<double> xs(static_cast<size_t>(A_) * 3), vs(static_cast<size_t>(A_) * 3);
+    DeviceArray<int> scratch_status(S_); // the timed copies must not flag the systems
     TM_HIP_CHECK(hipMemcpyAsync(xs.ptr, I.x.ptr, xs.bytes(), hipMemcpyDeviceToDevice, stream_));
     TM_HIP_CHECK(hipMemcpyAsync(vs.ptr, I.v.ptr, vs.bytes(), hipMemcpyDeviceToDevice, stream_));
     ms_out[3] = timed([&] {
         k_langevin<<<ceil_div(A_, 256), 256, 0, stream_>>>(A_, I.atom_sys.ptr, I.d_atom0.ptr, I.step.ptr, I.seed.ptr,
                                                          I.ca.ptr, I.dt.ptr, I.cbs.ptr, I.ccs.ptr, I.du_dx.ptr,
-                                                         xs.ptr, vs.ptr);
+                                                         xs.ptr, vs.ptr, scratch_status.ptr);
     });
     TM_HIP_CHECK(hipStreamSynchronize(stream_));
     launch_fill_u64(I.du_dx.ptr, static_cast<size_t>(A_) * 3, 0, stream_);

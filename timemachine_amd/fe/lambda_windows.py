@@ -37,9 +37,14 @@ def window_layout(lambdas: Sequence[float], replicas: int, seed0: int = 2022) ->
 
 
 def build_batch(sys_: ts.System, lambdas: Sequence[float], replicas: int, seed0: int = 2022,
-                barostat_interval: int = 15) -> custom_ops.BatchedContexts:
+                barostat_interval: int = 15, minimize_steps: int = 99) -> custom_ops.BatchedContexts:
     """All windows x replicas of ``sys_`` in one engine (BAOAB at 2.5 fs, 300 K,
-    1/ps; MC barostat at 1 bar every ``barostat_interval`` steps, 0 = none)."""
+    1/ps; MC barostat at 1 bar every ``barostat_interval`` steps, 0 = none).
+    Every window is first FIRE-minimised at its own lambda for
+    ``minimize_steps`` steps on the device, as the reference's simulate does
+    before its MD (fe/estimator.py:242-249): the synthetic boxes start with
+    close contacts that 2.5 fs steps do not survive. Minimisation leaves
+    velocities, box, step counters and barostat state alone."""
     lams, seeds = window_layout(lambdas, replicas, seed0)
     bps = [p.bound_impl(np.float32) for p in sys_.bound_potentials()]
     intg = LangevinIntegrator(TEMPERATURE, DT, FRICTION, sys_.masses, seeds[0]).impl()
@@ -48,13 +53,18 @@ def build_batch(sys_: ts.System, lambdas: Sequence[float], replicas: int, seed0:
         baro = MonteCarloBarostat(sys_.N, PRESSURE_BAR, TEMPERATURE, sys_.groups, barostat_interval,
                                   seeds[0] + 7).impl(bps)
     v0s = np.stack([ts.maxwell_boltzmann(sys_.masses, TEMPERATURE, sd) for sd in seeds])
-    return custom_ops.BatchedContexts.from_windows(sys_.x, v0s, sys_.box, intg, seeds, bps, lams, barostat=baro,
-                                                   barostat_seeds=[sd + 7 for sd in seeds])
+    batch = custom_ops.BatchedContexts.from_windows(sys_.x, v0s, sys_.box, intg, seeds, bps, lams, barostat=baro,
+                                                    barostat_seeds=[sd + 7 for sd in seeds])
+    if minimize_steps > 0:
+        batch.fire_minimize(n_steps=minimize_steps)
+    return batch
 
 
-def window_context(sys_: ts.System, seed: int, barostat_interval: int = 15):
+def window_context(sys_: ts.System, seed: int, barostat_interval: int = 15, x0=None):
     """One window alone through the Context API, seeded as ``build_batch`` seeds
-    the window with integrator seed ``seed``. Returns (context, handles to keep)."""
+    the window with integrator seed ``seed``, starting from ``x0`` (default: the
+    system's coordinates; pass a minimised batch window's coordinates to
+    continue it alone). Returns (context, handles to keep)."""
     bps = [p.bound_impl(np.float32) for p in sys_.bound_potentials()]
     intg = LangevinIntegrator(TEMPERATURE, DT, FRICTION, sys_.masses, seed).impl()
     baro = None
@@ -62,7 +72,7 @@ def window_context(sys_: ts.System, seed: int, barostat_interval: int = 15):
         baro = MonteCarloBarostat(sys_.N, PRESSURE_BAR, TEMPERATURE, sys_.groups, barostat_interval,
                                   seed + 7).impl(bps)
     v0 = ts.maxwell_boltzmann(sys_.masses, TEMPERATURE, seed)
-    ctx = custom_ops.Context(sys_.x, v0, sys_.box, intg, bps, barostat=baro)
+    ctx = custom_ops.Context(sys_.x if x0 is None else x0, v0, sys_.box, intg, bps, barostat=baro)
     return ctx, (bps, intg, baro)
 
 

@@ -107,6 +107,7 @@ class LambdaWindows:
         friction: float = 1.0,
         barostat_interval: int = 0,
         seed: int = 2024,
+        minimize_steps: int = 0,
     ):
         if n_frames <= 0 or frame_interval <= 0:
             raise ValueError("n_frames and frame_interval must be positive")
@@ -126,6 +127,17 @@ class LambdaWindows:
         self.x0 = np.array(system.x, dtype=np.float64)
         self.box0 = np.array(system.box, dtype=np.float64)
         self.v0 = [ts.maxwell_boltzmann(system.masses, self.temperature, self.seed + w) for w in range(len(lambdas))]
+        self.last_device_bytes = 0  # device memory of the last batch built (its engine's arena)
+        if minimize_steps > 0:
+            # the checkpoint is FIRE-minimised once, at the middle lambda
+            mid = float(self.lambdas[len(self.lambdas) // 2])
+            bps = [p.bound_impl(np.float32) for p in system.bound_potentials()]
+            intg = LangevinIntegrator(self.temperature, self.dt, self.friction, system.masses, self.seed).impl()
+            one = custom_ops.BatchedContexts.from_windows(self.x0, self.v0[:1], self.box0, intg, [self.seed], bps,
+                                                          [mid])
+            one.fire_minimize(n_steps=minimize_steps)
+            self.x0 = one.get_x_t(0)
+            del one
 
     def _system_with(self, nb_params: Optional[np.ndarray], torsion_params: Optional[np.ndarray]) -> ts.System:
         s = dataclasses.replace(self.system)
@@ -136,21 +148,23 @@ class LambdaWindows:
         return s
 
     def batch(self, nb_params=None, torsion_params=None) -> Tuple[custom_ops.BatchedContexts, list]:
-        """A fresh batch at the checkpoint with the given parameters (and the
-        objects that must stay alive while it is used)."""
+        """A fresh batch of all windows at the checkpoint with the given
+        parameters, built in one engine (window w: integrator seed
+        seed + 7919 w, barostat seed + 1), and a list of objects to keep alive
+        while it is used (empty: the engine holds its potentials)."""
         sys_ = self._system_with(nb_params, torsion_params)
-        ctxs, keep = [], []
-        for w, lam in enumerate(self.lambdas):
-            seed = self.seed + 7919 * w
-            bps = [p.bound_impl(np.float32) for p in sys_.bound_potentials()]
-            intg = LangevinIntegrator(self.temperature, self.dt, self.friction, sys_.masses, seed).impl()
-            baro = None
-            if self.barostat_interval > 0:
-                baro = MonteCarloBarostat(sys_.N, 1.0, self.temperature, sys_.groups, self.barostat_interval,
-                                          seed + 1).impl(bps)
-            ctxs.append(custom_ops.Context(self.x0, self.v0[w], self.box0, intg, bps, barostat=baro))
-            keep.append((bps, intg, baro))
-        return custom_ops.BatchedContexts(ctxs, self.lambdas), [ctxs, keep]
+        bps = [p.bound_impl(np.float32) for p in sys_.bound_potentials()]
+        seeds = [self.seed + 7919 * w for w in range(len(self.lambdas))]
+        intg = LangevinIntegrator(self.temperature, self.dt, self.friction, sys_.masses, seeds[0]).impl()
+        baro = None
+        if self.barostat_interval > 0:
+            baro = MonteCarloBarostat(sys_.N, 1.0, self.temperature, sys_.groups, self.barostat_interval,
+                                      seeds[0] + 1).impl(bps)
+        batch = custom_ops.BatchedContexts.from_windows(self.x0, np.stack(self.v0), self.box0, intg, seeds, bps,
+                                                        self.lambdas, barostat=baro,
+                                                        barostat_seeds=[sd + 1 for sd in seeds])
+        self.last_device_bytes = batch.device_bytes()
+        return batch, []
 
     def sample(self, nb_params=None, torsion_params=None, observe: bool = True):
         """(mean du/dlambda per window [W], du/dlambda series [W, n_frames],

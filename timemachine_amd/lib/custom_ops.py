@@ -30,6 +30,24 @@ from timemachine_amd._lib import call, f64, fixed_to_float, i32, pd, pi, pu64, w
 FIXED_EXPONENT_DU_DSIG = 2.0**37
 FIXED_EXPONENT_DU_DEPS = 2.0**38
 
+# per-system fault bits of BatchedContexts.status()
+STATUS_BOX_TOO_SMALL = 1
+STATUS_BOX_NOT_ORTHO = 2
+STATUS_TILE_OVERFLOW = 4
+STATUS_NONFINITE = 8
+STATUS_FIXED_INVALID = 16
+STATUS_MESSAGES = {
+    STATUS_BOX_TOO_SMALL: "Cutoff with padding is more than half of the box width, neighborlist is no longer reliable",
+    STATUS_BOX_NOT_ORTHO: "Provided non-ortholinear box, unable to compute nonbonded energy",
+    STATUS_TILE_OVERFLOW: "neighborlist tile buffer overflow",
+    STATUS_NONFINITE: "coordinates or velocities are not finite",
+    STATUS_FIXED_INVALID: "a force left the fixed-point range (non-finite or huge pair terms)",
+}
+
+
+def status_message(bits: int) -> str:
+    return "; ".join(msg for bit, msg in STATUS_MESSAGES.items() if bits & bit)
+
 
 class _Handle:
     _release = None
@@ -1045,6 +1063,17 @@ class BatchedContexts(_Handle):
 
     def write_back(self):
         call("tm_batch_write_back", self._h)
+
+    def status(self) -> np.ndarray:
+        """Fault bits of every system so far (STATUS_* flags; 0 = healthy)."""
+        out = np.zeros(len(self._ctxs), dtype=np.int32)
+        call("tm_batch_get_status", self._h, pi(out))
+        return out
+
+    def set_raise_faults(self, raise_faults: bool):
+        """False: multiple_steps returns with faulted systems flagged in status()
+        instead of raising for the whole batch."""
+        call("tm_batch_set_raise_faults", self._h, 1 if raise_faults else 0)
 
     def barostat_move(self, s: int) -> dict:
         """The barostat move system s made in the last step (all zero if none)."""

@@ -74,11 +74,12 @@ class Job:
 @dataclasses.dataclass
 class JobResult:
     job_id: int
-    du_dl: np.ndarray  # [W, n_steps // du_dl_interval]
-    dG: float  # TI estimate, trapezoid over the windows of the per-window mean du/dlambda
+    du_dl: Optional[np.ndarray]  # [W, n_steps // du_dl_interval]; None for a failed job
+    dG: float  # TI estimate, trapezoid over the windows of the per-window mean du/dlambda (nan if failed)
     arrival: float
     admitted: float
     finished: float
+    error: Optional[str] = None  # why the job failed (its own fault; other jobs are unaffected)
 
     @property
     def latency(self) -> float:
@@ -186,8 +187,14 @@ class Server:
         self.setup_wait_seconds = 0.0  # time the serving thread waited for a preparation
 
     def _prepare(self, job: Job):
+        """The job's window batch, or the exception that preparing it raised
+        (a bad box, forces too large after minimisation): only that job fails."""
         t = time.perf_counter()
-        wins = make_window_batch(job)
+        try:
+            wins = make_window_batch(job)
+            wins.set_raise_faults(False)
+        except (RuntimeError, AssertionError) as e:
+            wins = e
         self.setup_seconds += time.perf_counter() - t
         return wins
 
@@ -244,14 +251,19 @@ class Server:
             else:
                 wins = self._prepare(job)
             pending.pop(0)
+            if isinstance(wins, Exception):
+                t_fail = now()
+                results.append(JobResult(job.job_id, None, float("nan"), job.arrival, t_fail, t_fail, error=str(wins)))
+                continue
             used += self._job_bytes(job)
             admitted_at[job.job_id] = now()
             new_batches.append(wins)
             for w in range(job.windows):
                 new.append(_Window(job, w))
         if not running and not new:
-            # idle until the next arrival
-            self._sleep(max(0.0, pending[0].arrival - now()))
+            if pending:
+                # idle until the next arrival
+                self._sleep(max(0.0, pending[0].arrival - now()))
             return batch, used
         if new:
             tr = now()
@@ -262,6 +274,7 @@ class Server:
                 batch = new_batches[0]  # the job's own engine becomes the batch
             else:
                 batch = custom_ops.BatchedContexts.gather(parts, lam)
+                batch.set_raise_faults(False)
             del new_batches
             running += new
             self.repack_seconds += now() - tr
@@ -274,7 +287,14 @@ class Server:
         for s, r in enumerate(running):
             r.chunks.append(du_dl[s])
             r.remaining -= chunk
-        done_jobs = {r.job.job_id for r in running if r.remaining == 0}
+        # a job with a faulted window (bad box, non-finite state, fixed-point
+        # overflow) leaves with an error; the other jobs are not affected
+        status = batch.status()
+        failed: Dict[int, str] = {}
+        for s, r in enumerate(running):
+            if status[s] and r.job.job_id not in failed:
+                failed[r.job.job_id] = f"window {r.w}: " + custom_ops.status_message(int(status[s]))
+        done_jobs = {r.job.job_id for r in running if r.remaining == 0} | set(failed)
         if done_jobs:
             t_fin = now()
             by_job: Dict[int, List[_Window]] = {}
@@ -284,15 +304,20 @@ class Server:
             for jid, ws in by_job.items():
                 ws.sort(key=lambda r: r.w)
                 job = ws[0].job
-                series = np.stack([np.concatenate(r.chunks) for r in ws])
-                results.append(JobResult(jid, series, ti_estimate(job.lambdas, series), job.arrival,
-                                         admitted_at[jid], t_fin))
+                if jid in failed:
+                    results.append(JobResult(jid, None, float("nan"), job.arrival, admitted_at[jid], t_fin,
+                                             error=failed[jid]))
+                else:
+                    series = np.stack([np.concatenate(r.chunks) for r in ws])
+                    results.append(JobResult(jid, series, ti_estimate(job.lambdas, series), job.arrival,
+                                             admitted_at[jid], t_fin))
                 used -= self._job_bytes(job)
             keep = [i for i, r in enumerate(running) if r.job.job_id not in done_jobs]
             tr = now()
             if keep:
                 lam = [running[i].job.lambdas[running[i].w] for i in keep]
                 batch = batch.repack(keep, [], lam)
+                batch.set_raise_faults(False)
             else:
                 batch = None
             self.repack_seconds += now() - tr
@@ -311,48 +336,75 @@ def run_fifo(jobs: Sequence[Job], du_dl_interval: int = 10, use_graph: bool = Tr
         if job.arrival > now():
             sleep(job.arrival - now())
         ta = now()
-        batch = make_window_batch(job)
-        du_dl, _, _ = batch.multiple_steps(job.n_steps, store_du_dl_interval=du_dl_interval, use_graph=use_graph)
-        del batch
+        try:
+            batch = make_window_batch(job)
+            batch.set_raise_faults(False)
+            du_dl, _, _ = batch.multiple_steps(job.n_steps, store_du_dl_interval=du_dl_interval,
+                                               use_graph=use_graph)
+            status = batch.status()
+            del batch
+        except (RuntimeError, AssertionError) as e:
+            results.append(JobResult(job.job_id, None, float("nan"), job.arrival, ta, now(), error=str(e)))
+            continue
+        bad = np.nonzero(status)[0]
+        if bad.size:
+            msg = f"window {int(bad[0])}: " + custom_ops.status_message(int(status[bad[0]]))
+            results.append(JobResult(job.job_id, None, float("nan"), job.arrival, ta, now(), error=msg))
+            continue
         results.append(JobResult(job.job_id, du_dl, ti_estimate(job.lambdas, du_dl), job.arrival, ta, now()))
     return results
 
 
+def _build_job_system(n_target: float, n_heavy: int, seed: int) -> ts.System:
+    """A ligand-in-water system of about n_target atoms (the box is at least
+    2.62 nm, twice cutoff plus padding), deterministic in its arguments."""
+    rng = np.random.default_rng(seed)
+    n_waters = max(int((n_target - 1.6 * n_heavy) / 3), 1)
+    side = max((n_waters / 33.4) ** (1.0 / 3.0), 2.62)
+    n_lig = int(round(n_heavy * 1.6))
+    for _ in range(12):
+        try:
+            return ts.ligand_in_water(side=side, n_waters=n_waters, n_heavy=n_heavy, n_ligand=n_lig,
+                                      seed=int(rng.integers(1 << 30)))
+        except ValueError:
+            n_lig = max(n_heavy, n_lig - max(1, n_heavy // 10))  # no room for every hydrogen
+    raise RuntimeError("could not build a synthetic ligand")
+
+
 def synthetic_stream(n_jobs: int, rate_per_s: float, seed: int = 0, atoms=(1800, 36000), windows=(8, 48),
                      steps=(2000, 20000), heavy=(20, 60), steps_multiple: int = 10,
-                     minimize_steps: int = 0) -> List[Job]:
-    """Poisson arrivals of ligand-in-water jobs: atoms log-uniform (the box is at
-    least 2.62 nm, twice cutoff plus padding), windows and steps uniform; each
-    window FIRE-minimised for ``minimize_steps`` steps before its MD."""
+                     minimize_steps: int = 0, workers: int = 1) -> List[Job]:
+    """Poisson arrivals of ligand-in-water jobs: atoms log-uniform, windows and
+    steps uniform; each window FIRE-minimised for ``minimize_steps`` steps
+    before its MD. The systems are built on the host, in ``workers`` processes
+    (started fresh, so call this before the process uses the GPU)."""
     rng = np.random.default_rng(seed)
     t = 0.0
-    jobs = []
+    specs = []
     for i in range(n_jobs):
         t += rng.exponential(1.0 / rate_per_s) if rate_per_s > 0 else 0.0
         n_target = math.exp(rng.uniform(math.log(atoms[0]), math.log(atoms[1])))
         n_heavy = int(rng.integers(heavy[0], heavy[1] + 1))
-        n_waters = max(int((n_target - 1.6 * n_heavy) / 3), 1)
-        side = max((n_waters / 33.4) ** (1.0 / 3.0), 2.62)
-        sys_ = None
-        for attempt in range(8):  # the random tree sometimes has no room for every hydrogen
-            n_lig = int(round(n_heavy * (1.6 - 0.1 * attempt)))
-            try:
-                sys_ = ts.ligand_in_water(side=side, n_waters=n_waters, n_heavy=n_heavy, n_ligand=n_lig,
-                                          seed=int(rng.integers(1 << 30)))
-                break
-            except ValueError:
-                continue
-        if sys_ is None:
-            raise RuntimeError("could not build a synthetic ligand")
+        sys_seed = int(rng.integers(1 << 30))
         W = int(rng.integers(windows[0], windows[1] + 1))
         n = int(rng.integers(steps[0], steps[1] + 1))
         n -= n % steps_multiple
-        jobs.append(Job(i, sys_, np.linspace(0.0, 1.0, W), max(n, steps_multiple), int(rng.integers(1 << 30)),
-                        arrival=t, minimize_steps=minimize_steps))
-    return jobs
+        specs.append((i, t, n_target, n_heavy, sys_seed, W, max(n, steps_multiple), int(rng.integers(1 << 30))))
+    args = [(sp[2], sp[3], sp[4]) for sp in specs]
+    if workers > 1:
+        import multiprocessing as mp
+        from concurrent.futures import ProcessPoolExecutor
+
+        with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
+            systems = list(ex.map(_build_job_system, *zip(*args)))
+    else:
+        systems = [_build_job_system(*a) for a in args]
+    return [Job(i, sys_, np.linspace(0.0, 1.0, W), n, job_seed, arrival=t, minimize_steps=minimize_steps)
+            for (i, t, _, _, _, W, n, job_seed), sys_ in zip(specs, systems)]
 
 
 def latency_stats(results: Sequence[JobResult]) -> dict:
-    lat = np.array([r.latency for r in results])
+    """Latency percentiles of the jobs that completed (failed jobs are counted apart)."""
+    lat = np.array([r.latency for r in results if r.error is None])
     return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
-            "mean_s": float(lat.mean()), "jobs": len(lat)}
+            "mean_s": float(lat.mean()), "jobs": len(lat), "failed": sum(r.error is not None for r in results)}

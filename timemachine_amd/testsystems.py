@@ -161,12 +161,18 @@ def _place_waters(
     n_side = 1
     while n_side**3 < n_waters * (1.05 if avoid is not None else 1.0):
         n_side += 1
-    spacing = side / n_side
-    g = (np.arange(n_side) + 0.5) * spacing
-    sites = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
-    if avoid is not None and len(avoid):
-        dmin = _min_image_dist(sites, avoid, box_diag).min(1)
-        sites = sites[dmin > avoid_radius + 0.5 * spacing]
+    while True:
+        spacing = side / n_side
+        g = (np.arange(n_side) + 0.5) * spacing
+        sites = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+        if avoid is not None and len(avoid):
+            dmin = _min_image_dist(sites, avoid, box_diag).min(1)
+            sites = sites[dmin > avoid_radius + 0.5 * spacing]
+        # a large ligand in a small box displaces more than 5% of the sites:
+        # a finer lattice then
+        if len(sites) >= n_waters or spacing < 0.25:
+            break
+        n_side += 1
     if len(sites) < n_waters:
         raise ValueError(f"only {len(sites)} lattice sites for {n_waters} waters")
     sites = sites[np.sort(rng.choice(len(sites), n_waters, replace=False))]
