@@ -73,6 +73,18 @@ TM_HD inline double exp_neg(double a) {
     return e - e * ((t - static_cast<double>(tf)) * LN_2);
 }
 
+// P(t) = sum_k ERFC_Ck t^k in Estrin's form: four independent first-degree
+// pieces, then t^2 and t^4 (9 operations, depth 4 instead of Horner's 7).
+TM_HD inline double erfc_poly(double t) {
+    const double t2 = t * t;
+    const double t4 = t2 * t2;
+    const double p01 = __builtin_fma(ERFC_C1, t, ERFC_C0);
+    const double p23 = __builtin_fma(ERFC_C3, t, ERFC_C2);
+    const double p45 = __builtin_fma(ERFC_C5, t, ERFC_C4);
+    const double p67 = __builtin_fma(ERFC_C7, t, ERFC_C6);
+    return __builtin_fma(__builtin_fma(p67, t2, p45), t4, __builtin_fma(p23, t2, p01));
+}
+
 struct PairTerms {
     double prefactor;  // (dU/dd) / d : du/dx_i = prefactor * (x_i - x_j)
     double u;          // pair energy (valid when requested)
@@ -106,15 +118,7 @@ TM_HD inline PairTerms nb_pair_terms(
 
     const double x = beta * d;
     const double e = exp_neg(x * x);
-    const double t = rcp_nr(1.0 + ERFC_P * x);
-    double poly = ERFC_C7;
-    poly = __builtin_fma(poly, t, ERFC_C6);
-    poly = __builtin_fma(poly, t, ERFC_C5);
-    poly = __builtin_fma(poly, t, ERFC_C4);
-    poly = __builtin_fma(poly, t, ERFC_C3);
-    poly = __builtin_fma(poly, t, ERFC_C2);
-    poly = __builtin_fma(poly, t, ERFC_C1);
-    poly = __builtin_fma(poly, t, ERFC_C0);
+    const double poly = erfc_poly(rcp_nr(1.0 + ERFC_P * x));
     const double erfc_x = e * poly;
 
     const double sq = scale_q * (qi * qj);
@@ -260,15 +264,7 @@ TM_HD inline PairDlDp nb_pair_dl_dp(double d2, double sig_i, double sig_j, doubl
 
     const double x = beta * d;
     const double e = exp_neg(x * x);
-    const double t = rcp_nr(1.0 + ERFC_P * x);
-    double poly = ERFC_C7;
-    poly = __builtin_fma(poly, t, ERFC_C6);
-    poly = __builtin_fma(poly, t, ERFC_C5);
-    poly = __builtin_fma(poly, t, ERFC_C4);
-    poly = __builtin_fma(poly, t, ERFC_C3);
-    poly = __builtin_fma(poly, t, ERFC_C2);
-    poly = __builtin_fma(poly, t, ERFC_C1);
-    poly = __builtin_fma(poly, t, ERFC_C0);
+    const double poly = erfc_poly(rcp_nr(1.0 + ERFC_P * x));
     o.es_unit = -(inv_d * inv_d2) * ((TWO_OVER_SQRT_PI * x + poly) * e);
     o.lj_unit = 0.0;
     o.lj_dsig = 0.0;

@@ -85,6 +85,10 @@ template <int NACC, bool INTERP> struct TileLds {
 __device__ __forceinline__ void lds_add_u64(uint64_t *p, uint64_t v) {
     atomicAdd(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(v));
 }
+// p -= v modulo 2^64 (one LDS atomic, no separate negation)
+__device__ __forceinline__ void lds_sub_u64(uint64_t *p, uint64_t v) {
+    __hip_atomic_fetch_sub(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
 // Minimum-image displacement and squared 4D distance of a pair. Both passes
 // and the pair-list kernel evaluate exactly these expressions, so the pass-1
@@ -137,15 +141,17 @@ __device__ __forceinline__ void eval_pairs(TileLds<NACC, INTERP> &L, const int b
         }
         const PairTerms T = nb_pair_terms<U, NEED_DP>(d2, ai.x, aj.x, ai.y, aj.y, ai.z, aj.z, beta, 1.0, 1.0);
         if (DX) {
-            uint64_t fx, fy, fz; // = f64_to_fixed(T.prefactor * d, FIXED_SCALE) per component
-            round_to_fixed3((T.prefactor * dx) * FIXED_SCALE, (T.prefactor * dy) * FIXED_SCALE,
-                            (T.prefactor * dz) * FIXED_SCALE, fx, fy, fz);
+            // fixed(prefactor * d) per component; scaling by 2^36 is exact, so
+            // (prefactor 2^36) d rounds exactly as (prefactor d) 2^36
+            const double ps = T.prefactor * FIXED_SCALE;
+            uint64_t fx, fy, fz;
+            round_to_fixed3(ps * dx, ps * dy, ps * dz, fx, fy, fz);
             lds_add_u64(&L.acc[0][0][il], fx);
             lds_add_u64(&L.acc[0][1][il], fy);
             lds_add_u64(&L.acc[0][2][il], fz);
-            lds_add_u64(&L.acc[1][0][sl], 0ull - fx);
-            lds_add_u64(&L.acc[1][1][sl], 0ull - fy);
-            lds_add_u64(&L.acc[1][2][sl], 0ull - fz);
+            lds_sub_u64(&L.acc[1][0][sl], fx);
+            lds_sub_u64(&L.acc[1][1][sl], fy);
+            lds_sub_u64(&L.acc[1][2][sl], fz);
         }
         if (DP) {
             lds_add_u64(&L.acc[0][PQ + 0][il], f64_to_fixed((1.0 * aj.x) * T.erfc_inv_d, FIXED_SCALE));
@@ -651,9 +657,10 @@ __global__ __launch_bounds__(256) void k_pair_list(
                 }
             };
             if (DX) {
-                const uint64_t fx = f64_to_fixed(T.prefactor * dx, FIXED_SCALE);
-                const uint64_t fy = f64_to_fixed(T.prefactor * dy, FIXED_SCALE);
-                const uint64_t fz = f64_to_fixed(T.prefactor * dz, FIXED_SCALE);
+                const double ps = T.prefactor * FIXED_SCALE; // as the tile kernel rounds
+                const uint64_t fx = round_to_fixed(ps * dx);
+                const uint64_t fy = round_to_fixed(ps * dy);
+                const uint64_t fz = round_to_fixed(ps * dz);
                 if (a.slots) {
                     fs[0] = neg ? 0ull - fx : fx;
                     fs[1] = neg ? 0ull - fy : fy;
