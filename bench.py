@@ -354,7 +354,7 @@ def main() -> int:
                     help="frames per window of the differentiation leg (16 windows; 0 skips it)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--tile-grid", type=int, default=0)
-    ap.add_argument("--no-autotune", action="store_true", help="skip the grid / padding autotuner")
+    ap.add_argument("--no-autotune", action="store_true", help="skip the occupancy / grid / padding autotuner")
     ap.add_argument("--profile-only", type=int, default=0, metavar="REPS",
                     help="build the batch, warm up, launch each kernel group REPS times eagerly and exit "
                     "(for rocprofv3 counter runs)")
@@ -400,11 +400,13 @@ def main() -> int:
     if args.tile_grid:
         batch.set_tile_grid(args.tile_grid)
     elif not args.no_autotune and not args.profile_only:
-        log("autotuning grid and neighborlist padding on a device copy of the batch")
+        log("autotuning the pair kernel's occupancy build and grid, then the neighborlist padding, "
+            "on a device copy of the batch")
         tuned = batch.autotune()
-        log(f"autotuned: {tuned['grid']} workgroups, padding {tuned['padding']} nm")
+        log(f"autotuned: {tuned['waves']} waves per SIMD, {tuned['grid']} workgroups, padding {tuned['padding']} nm")
         for c in tuned["candidates"]:
-            log(f"  grid {c['grid']:6d} padding {c['padding']:.3f} nm: {c['ms_per_step']:.3f} ms/step")
+            log(f"  [{c['stage']}] waves {c['waves']} grid {c['grid']:6d} padding {c['padding']:.3f} nm: "
+                f"{c['ms_per_step']:.3f} ms/step")
     S = len(batch)
     use_graph = not args.no_graph
     log(f"warmup {args.warmup} batched steps")
@@ -502,7 +504,8 @@ def main() -> int:
             "barostat_interval": 15,
             "du_dl_interval": 1,
             "hipgraph": use_graph,
-            "autotuned": None if tuned is None else {"grid": tuned["grid"], "padding_nm": tuned["padding"]},
+            "autotuned": None if tuned is None else {"waves_per_simd": tuned["waves"], "grid": tuned["grid"],
+                                                     "padding_nm": tuned["padding"]},
         },
         "ti_free_energy_kj_mol": dG,
         "du_dl_window_means": dl_means.tolist(),

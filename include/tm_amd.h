@@ -293,6 +293,9 @@ int tm_batch_total_tiles(tm_batch *b, int *out);
 /* One-wave workgroups of the pair (tile) kernel (0 = default, CUs x 128; at most 2^24). No reference counterpart
  * (the reference launches one block per tile); used by the autotuner. */
 int tm_batch_set_tile_grid(tm_batch *b, int grid);
+/* Waves per SIMD the pair (tile) kernel is compiled for: 4 (default, at most 128 VGPRs), 3 or 2 (up to 168 / 256
+ * VGPRs, fewer waves in flight). No reference counterpart; used by the autotuner. Results do not depend on it. */
+int tm_batch_set_tile_waves(tm_batch *b, int waves);
 /* Neighborlist padding of every system of the batch (nm); the counterpart of
  * NonbondedAllPairs::set_nblist_padding (nonbonded_all_pairs.cu:180-187) for a batch.
  * Results do not depend on it. */

@@ -1,6 +1,7 @@
-"""Grid / padding autotuner (GPU): it changes speed only. A batch that was
-autotuned (on its device copy) and then stepped gives bitwise the trajectory of
-an untuned batch; explicit padding and grid settings do not change bits."""
+"""Occupancy / grid / padding autotuner (GPU): it changes speed only. A batch
+that was autotuned (on its device copy) and then stepped gives bitwise the
+trajectory of an untuned batch; explicit occupancy builds, padding and grid
+settings do not change bits."""
 
 import numpy as np
 import pytest
@@ -27,9 +28,11 @@ def test_autotune_keeps_trajectories(job):
     ref, keep_r = _batch(job)
     dl_ref, _, _ = ref.multiple_steps(40, store_du_dl_interval=1)
     tuned, keep_t = _batch(job)
-    res = tuned.autotune(steps=6, grids=(2048, 8192), paddings=(0.1, 0.2))
+    res = tuned.autotune(steps=6, waves=(4, 2), grids=(2048, 8192), paddings=(0.1, 0.2))
     assert {r["grid"] for r in res["candidates"]} == {2048, 8192}
-    assert res["ms_per_step"] == min(r["ms_per_step"] for r in res["candidates"])
+    assert {r["waves"] for r in res["candidates"]} == {4, 2}
+    # the choice is the fastest of the last (padding) stage
+    assert res["ms_per_step"] == min(r["ms_per_step"] for r in res["candidates"] if r["stage"] == "padding")
     dl, _, _ = tuned.multiple_steps(40, store_du_dl_interval=1)
     assert np.array_equal(dl, dl_ref)
     for s in range(len(job.lambdas)):
@@ -42,11 +45,17 @@ def test_padding_and_grid_do_not_change_bits(job):
     b, keep_b = _batch(job)
     b.set_nblist_padding(0.05)
     b.set_tile_grid(512)
-    dl1, _, _ = b.multiple_steps(12, store_du_dl_interval=1)
+    b.set_tile_waves(2)
+    dl1, _, _ = b.multiple_steps(8, store_du_dl_interval=1)
     b.set_nblist_padding(0.2)
     b.set_tile_grid(0)
-    dl2, _, _ = b.multiple_steps(13, store_du_dl_interval=1)
-    assert np.array_equal(np.concatenate([dl1, dl2], axis=1), dl_ref)
+    b.set_tile_waves(3)
+    dl2, _, _ = b.multiple_steps(8, store_du_dl_interval=1)
+    b.set_tile_waves(4)
+    dl3, _, _ = b.multiple_steps(9, store_du_dl_interval=1)
+    assert np.array_equal(np.concatenate([dl1, dl2, dl3], axis=1), dl_ref)
+    with pytest.raises(RuntimeError, match="waves per SIMD must be 2, 3 or 4"):
+        b.set_tile_waves(5)
     with pytest.raises(RuntimeError, match="padding must be"):
         b.set_nblist_padding(-1.0)
     with pytest.raises(RuntimeError, match="tile grid must be"):

@@ -1059,6 +1059,10 @@ int tm_batch_set_tile_grid(tm_batch *b, int grid) {
     });
 }
 
+int tm_batch_set_tile_waves(tm_batch *b, int waves) {
+    return guarded([&] { b->eng->set_tile_waves(waves); });
+}
+
 int tm_batch_set_nblist_padding(tm_batch *b, double padding) {
     return guarded([&] { b->eng->set_nblist_padding(padding); });
 }

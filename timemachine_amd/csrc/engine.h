@@ -158,6 +158,8 @@ class Engine {
     void set_nblist_padding(double padding);
 
     int tile_grid = 0; // nonbonded grid override (autotuner), 0 = default
+    // Waves per SIMD of the pair kernel build (4, 3 or 2; autotuner).
+    void set_tile_waves(int waves);
 
   private:
     struct Impl;

@@ -1438,6 +1438,15 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
     TM_HIP_CHECK(hipStreamSynchronize(stream_));
 }
 
+void Engine::set_tile_waves(int waves) {
+    if (waves < 2 || waves > 4) {
+        throw Error("tile kernel waves per SIMD must be 2, 3 or 4");
+    }
+    Impl &I = *impl_;
+    I.nb.tile_waves = waves;
+    I.ig.tile_waves = waves;
+}
+
 void Engine::set_nblist_padding(double padding) {
     Impl &I = *impl_;
     if (I.has_nb) {

@@ -121,6 +121,9 @@ struct NbState {
     void invalidate(hipStream_t stream);
     int find_pieces() const; // waves per row block in the neighborlist build
     int pieces_override = 0; // > 0: waves per row block forced to this (1, 2, 4 or 8)
+    // Waves per SIMD the pair kernel is compiled for (4, 3 or 2; 4 = at most 128
+    // VGPRs). Results do not depend on it.
+    int tile_waves = 4;
     // Neighborlist padding of every system (results do not depend on it; it
     // trades rebuild frequency against tiles). Forces a rebuild.
     void set_padding(double padding, hipStream_t stream);

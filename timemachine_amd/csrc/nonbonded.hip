@@ -41,10 +41,9 @@ __device__ __forceinline__ void atomic_add_u64(uint64_t *p, uint64_t v) {
     atomicAdd(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(v));
 }
 
-// Queued pairs per wave: fewer than 64 remain after a drain and a round adds
+// Queued pairs per wave: fewer than 64 wait between rounds and a round adds
 // at most 64, so 128 entries always suffice.
-constexpr int PAIR_RING = 128;
-static_assert((PAIR_RING & (PAIR_RING - 1)) == 0 && PAIR_RING >= 2 * TILE, "pair ring shape");
+constexpr int PAIR_QUEUE = 2 * TILE;
 
 // Per-wave LDS of the tile kernel. NACC accumulators per atom: x, y, z
 // forces, then q, sig, eps parameter derivatives when requested.
@@ -60,7 +59,7 @@ template <int NACC, bool INTERP> struct TileLds {
     double pos[2][4][TILE];      // [row, column][x, y, z, w][lane / slot]
     float par[2][4][TILE];       // [row, column][q, sig, eps, dw/dlambda][lane / slot]
     uint32_t colidx[TILE];       // column sorted index (NB_INVALID for padding)
-    uint32_t pij[PAIR_RING];     // queued pair: row lane | column slot << 8
+    uint32_t pij[PAIR_QUEUE];    // queued pair: row lane | screen round << 8 (column slot = lane + round mod 64)
     uint64_t acc[2][NACC][TILE]; // [row, column][component][lane / slot]
     TileDp<INTERP> dp;
 
@@ -114,22 +113,22 @@ struct BoxDev {
     double cut2; // squared cutoff: the exact test of pass 2
 };
 
-// Pass 2: evaluates n <= 64 queued pairs starting at ring entry `base`, one
+// Pass 2: evaluates the first n <= 64 queued pairs, one
 // pair per lane, and adds the fixed-point terms to the LDS accumulators of
 // the pair's row and column atom (u and du/dl to per-lane registers).
 // DLDP: instead of du/dp, accumulate d(du/dlambda)/dp into the du_dp slots
 // (same fixed-point exponents); only alchemical tiles contribute.
 template <bool ALCH, bool U, bool DX, bool DL, bool DP, bool DLDP, int NACC, bool INTERP>
-__device__ __forceinline__ void eval_pairs(TileLds<NACC, INTERP> &L, const int base, const int n, const BoxDev &B,
+__device__ __forceinline__ void eval_pairs(TileLds<NACC, INTERP> &L, const int n, const BoxDev &B,
                                            const double beta, uint64_t &uacc, uint64_t &dlacc) {
     constexpr int PQ = DX ? 3 : 0; // first parameter-derivative component
     // interpolated parameters add du/dp . dp/dl to du/dl, so the LJ parameter gradients are needed
     constexpr bool NEED_DP = DP || (INTERP && ALCH && DL);
     const int lane = threadIdx.x & 63;
     if (lane < n) {
-        const uint32_t ij = L.pij[(base + lane) & (PAIR_RING - 1)];
+        const uint32_t ij = L.pij[lane];
         const int il = static_cast<int>(ij & 0xFFu);
-        const int sl = static_cast<int>(ij >> 8);
+        const int sl = static_cast<int>((ij + (ij >> 8)) & (TILE - 1));
         const double4 pi = L.template get_pos<ALCH>(0, il);
         const double4 pj = L.template get_pos<ALCH>(1, sl);
         const float4 ai = L.get_par(0, il);
@@ -205,7 +204,7 @@ __device__ __forceinline__ void eval_pairs(TileLds<NACC, INTERP> &L, const int b
 // Pass 1: 64 rounds; in round r lane l pairs its row atom with column slot
 // (l + r) mod 64 (read from LDS one round ahead), computes the exact FP64 4D
 // distance and queues the indices of the pairs inside the cutoff (ballot +
-// mbcnt compaction into an LDS ring). Whenever 64 pairs are queued, pass 2
+// mbcnt compaction into an LDS queue). Whenever 64 pairs are queued, pass 2
 // recomputes their displacements and evaluates them with every lane busy.
 // Fixed-point sums are exact, so the order of evaluation does not change
 // any output bit.
@@ -224,6 +223,8 @@ __device__ __forceinline__ void tile_body(
     const float4 dpj,
     const uint32_t *__restrict__ perm,
     const double *__restrict__ boxes,
+    const float4 *__restrict__ bb_ctr,
+    const float4 *__restrict__ bb_ext,
     const bool row_fresh,
     const bool row_flush,
     uint64_t &uacc,
@@ -269,20 +270,29 @@ __device__ __forceinline__ void tile_body(
     }
     wave_lds_sync();
 
-    int queued = 0, done = 0; // wave-uniform
-    // queue this lane's pair of the round when `in`; drain full batches of 64
-    auto queue = [&](const int slot, const bool in) {
+    // Entries [0, queued) of L.pij wait for pass 2 (wave-uniform count, below
+    // 64 between rounds).
+    int queued = 0;
+    // queue this lane's pair of the round when `in`; at 64 or more, evaluate
+    // the first 64 and move the rest (fewer than 64) to the front
+    auto queue = [&](const int round, const bool in) {
         const uint64_t m = ballot(in);
         if (in) {
-            L.pij[(queued + lanes_below(m)) & (PAIR_RING - 1)] =
-                static_cast<uint32_t>(lane) | (static_cast<uint32_t>(slot) << 8);
+            const int at = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), queued));
+            L.pij[at] = static_cast<uint32_t>(lane) | (static_cast<uint32_t>(round) << 8);
         }
         queued += __popcll(m);
-        while (queued - done >= TILE) {
+        if (queued >= TILE) {
             wave_lds_sync();
-            eval_pairs<ALCH, U, DX, DL, DP, DLDP, NACC, INTERP>(L, done, TILE, B, beta, uacc, dlacc);
-            done += TILE;
-            wave_lds_sync(); // later rounds reuse these ring entries
+            eval_pairs<ALCH, U, DX, DL, DP, DLDP, NACC, INTERP>(L, TILE, B, beta, uacc, dlacc);
+            wave_lds_sync(); // pass 2 has read the first 64 entries
+            queued -= TILE;
+            const uint32_t rest = L.pij[TILE + (lane & (TILE - 1))];
+            if (lane < queued) {
+                L.pij[lane] = rest;
+            }
+            wave_lds_sync();
         }
     };
 
@@ -294,34 +304,30 @@ __device__ __forceinline__ void tile_body(
     // cutoff has a_j - a_i equal to its minimum-image displacement: any other
     // image differs by a multiple of L, so |a_j - a_i| >= L - cutoff > L / 2 + e,
     // which a_j - a_i cannot reach. The 4D coordinate w is taken relative to
-    // c's as well. For a pair within the cutoff |a_j| <= e + cutoff, so the
+    // row lane 0's. For a pair within the cutoff |a_j| <= e + cutoff, so the
     // FP32 rounding of a_i, a_j and d2 stays near 1e-6 relative to the squared
     // cutoff; the screen keeps a margin of 1e-5 and pass 2 repeats the exact
     // FP64 test. The column values travel around the wave in registers (one
     // lane rotation per round), so the screen reads no LDS.
+    // c and e come from the block's bounding box of the last neighborlist
+    // build (k_block_bounds, FP32): since then the box is unchanged and no atom
+    // has moved more than padding / 2 (else the list would have been rebuilt),
+    // so every row atom is within the box's half-extent + padding / 2 of its
+    // centre, plus 1e-4 nm for the FP32 rounding of the build.
     // Otherwise (small boxes, wide blocks) pass 1 is the exact FP64 test.
-    const double c0x = L.pos[0][0][0], c0y = L.pos[0][1][0], c0z = L.pos[0][2][0]; // row lane 0 holds an atom
-    const double cw = L.pos[0][3][0];
-    double rx = min_image_d(pi.x - c0x, B.bx, B.ibx);
-    double ry = min_image_d(pi.y - c0y, B.by, B.iby);
-    double rz = min_image_d(pi.z - c0z, B.bz, B.ibz);
-    // centre of the row block's extent (lane 0 contributes 0, so padding lanes may too)
-    const float lox = wave_min(vi ? static_cast<float>(rx) : 0.0f), hix = wave_max(vi ? static_cast<float>(rx) : 0.0f);
-    const float loy = wave_min(vi ? static_cast<float>(ry) : 0.0f), hiy = wave_max(vi ? static_cast<float>(ry) : 0.0f);
-    const float loz = wave_min(vi ? static_cast<float>(rz) : 0.0f), hiz = wave_max(vi ? static_cast<float>(rz) : 0.0f);
-    const double mx = 0.5 * (static_cast<double>(lox) + static_cast<double>(hix));
-    const double my = 0.5 * (static_cast<double>(loy) + static_cast<double>(hiy));
-    const double mz = 0.5 * (static_cast<double>(loz) + static_cast<double>(hiz));
+    const double cw = L.pos[0][3][0]; // row lane 0 holds an atom
+    const float4 bc = bb_ctr[S.block0 + r];
+    const float4 be = bb_ext[S.block0 + r];
+    const double slack = 0.5 * S.padding_d + 1e-4;
     const double reach = S.cutoff_d + 1e-3; // 1e-3 nm covers the FP32 extents
-    const bool local = uniform((0.5 * (hix - lox) + reach < 0.5 * B.bx && 0.5 * (hiy - loy) + reach < 0.5 * B.by &&
-                                0.5 * (hiz - loz) + reach < 0.5 * B.bz)
-                                   ? 1
-                                   : 0);
+    const bool local = static_cast<double>(be.x) + slack + reach < 0.5 * B.bx &&
+                       static_cast<double>(be.y) + slack + reach < 0.5 * B.by &&
+                       static_cast<double>(be.z) + slack + reach < 0.5 * B.bz;
     if (local) {
-        const double cx = c0x + mx, cy = c0y + my, cz = c0z + mz;
-        rx -= mx;
-        ry -= my;
-        rz -= mz;
+        const double cx = bc.x, cy = bc.y, cz = bc.z;
+        const double rx = min_image_d(pi.x - cx, B.bx, B.ibx);
+        const double ry = min_image_d(pi.y - cy, B.by, B.iby);
+        const double rz = min_image_d(pi.z - cz, B.bz, B.ibz);
         const bool vj0 = jl0 < k;
         // this lane's column (padding far away: d2 overflows to +inf)
         float qx = 1e30f, qy = 1e30f, qz = 1e30f, qw = 0.0f;
@@ -354,7 +360,7 @@ __device__ __forceinline__ void tile_body(
                 if constexpr (TRI) {
                     in = in && i < qc;
                 }
-                queue((lane + round) & (TILE - 1), in);
+                queue(round, in);
                 qx = rot1(qx);
                 qy = rot1(qy);
                 qz = rot1(qz);
@@ -378,12 +384,12 @@ __device__ __forceinline__ void tile_body(
             const uint32_t jc = L.colidx[slot];
             double dx, dy, dz, dw;
             const double d2 = pair_delta<ALCH>(pi, pj, B.bx, B.by, B.bz, B.ibx, B.iby, B.ibz, dx, dy, dz, dw);
-            queue(slot, vi && (jc < k) && (i < jc) && (d2 < cut2));
+            queue(round, vi && (jc < k) && (i < jc) && (d2 < cut2));
         }
     }
-    if (queued > done) {
+    if (queued > 0) {
         wave_lds_sync();
-        eval_pairs<ALCH, U, DX, DL, DP, DLDP, NACC, INTERP>(L, done, queued - done, B, beta, uacc, dlacc);
+        eval_pairs<ALCH, U, DX, DL, DP, DLDP, NACC, INTERP>(L, queued, B, beta, uacc, dlacc);
     }
     wave_lds_sync();
 
@@ -436,12 +442,13 @@ __device__ __forceinline__ void tile_body(
 
 // Every wave takes a contiguous range of the tile list of all systems (the
 // grid is sized to the machine, the tile count is read on the device).
-// amdgpu_waves_per_eu(4): at most 128 VGPRs, so four waves share a SIMD (the
-// 10 KB of LDS per wave allows four as well). The du_dx + du_dl instance
-// fits in 128 without spills; holding it there (it once took 130) ran 0.3 ms
-// per batched step faster. The u + du_dx + du_dl instance spills 16 bytes.
-template <bool U, bool DX, bool DL, bool DP, bool DLDP = false, bool INTERP = false>
-__global__ __launch_bounds__(NB_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_nonbonded_tiles(
+// WAVES_PER_SIMD = 4 (the default build): at most 128 VGPRs, so four waves
+// share a SIMD (the 10 KB of LDS per wave allows four as well); 3 and 2 let
+// the compiler use up to 168 and 256 VGPRs with fewer waves to hide latency.
+// The engine's instances exist in all three forms and the autotuner picks
+// one (NbState::tile_waves).
+template <bool U, bool DX, bool DL, bool DP, bool DLDP = false, bool INTERP = false, int WAVES_PER_SIMD = 4>
+__global__ __launch_bounds__(NB_BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES_PER_SIMD))) void k_nonbonded_tiles(
     const NbSys *__restrict__ sys,
     const int S,
     const int *__restrict__ tile_prefix,
@@ -453,6 +460,8 @@ __global__ __launch_bounds__(NB_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
     const float4 *__restrict__ par,
     const float4 *__restrict__ dpar,
     const double *__restrict__ boxes,
+    const float4 *__restrict__ bb_ctr,
+    const float4 *__restrict__ bb_ext,
     const int *__restrict__ status,
     uint64_t *__restrict__ du_dx,
     uint64_t *__restrict__ du_dp,
@@ -537,10 +546,10 @@ __global__ __launch_bounds__(NB_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
         const bool alch = ballot(moving) != 0;
         if (alch) {
             tile_body<true, U, DX, DL, DP, DLDP, NACC, INTERP>(L, Sd, s, r, jl, pi, ai, pj, aj, dpi, dpj, perm, boxes,
-                                                               fresh, last, uacc, dlacc, du_dx, du_dp);
+                                                               bb_ctr, bb_ext, fresh, last, uacc, dlacc, du_dx, du_dp);
         } else if (!DLDP) {
             tile_body<false, U, DX, DL, DP, false, NACC, INTERP>(L, Sd, s, r, jl, pi, ai, pj, aj, dpi, dpj, perm,
-                                                                 boxes, fresh, last, uacc, dlacc, du_dx, du_dp);
+                                                                 boxes, bb_ctr, bb_ext, fresh, last, uacc, dlacc, du_dx, du_dp);
         }
         if (sys_last && (U || DL)) {
             if (U) {
@@ -562,8 +571,8 @@ __global__ __launch_bounds__(NB_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
 }
 
 using TileKernel = void (*)(const NbSys *, int, const int *, const int *, const int *, const uint32_t *, const uint32_t *,
-                            const double4 *, const float4 *, const float4 *, const double *, const int *, uint64_t *,
-                            uint64_t *, uint64_t *, uint64_t *);
+                            const double4 *, const float4 *, const float4 *, const double *, const float4 *,
+                            const float4 *, const int *, uint64_t *, uint64_t *, uint64_t *, uint64_t *);
 
 // index bits: 8 = u, 4 = du_dx, 2 = du_dl, 1 = du_dp
 const TileKernel tile_kernels[16] = {
@@ -585,6 +594,32 @@ const TileKernel tile_kernels_interp[16] = {
     k_nonbonded_tiles<1, 1, 0, 0, 0, 1>, k_nonbonded_tiles<1, 1, 0, 1, 0, 1>, k_nonbonded_tiles<1, 1, 1, 0, 0, 1>,
     k_nonbonded_tiles<1, 1, 1, 1, 0, 1>,
 };
+
+// The pair kernel for output set idx (bits as above) at the given waves per
+// SIMD: the instances the MD engine launches (du_dx, du_dx + du_dl, u + du_dx,
+// u + du_dx + du_dl, plain parameters) exist at 4, 3 and 2; everything else
+// only at 4.
+TileKernel tile_kernel(const int idx, const bool interp, const int waves) {
+    if (!interp && waves == 3) {
+        switch (idx) {
+        case 4: return k_nonbonded_tiles<0, 1, 0, 0, false, false, 3>;
+        case 6: return k_nonbonded_tiles<0, 1, 1, 0, false, false, 3>;
+        case 12: return k_nonbonded_tiles<1, 1, 0, 0, false, false, 3>;
+        case 14: return k_nonbonded_tiles<1, 1, 1, 0, false, false, 3>;
+        default: break;
+        }
+    }
+    if (!interp && waves == 2) {
+        switch (idx) {
+        case 4: return k_nonbonded_tiles<0, 1, 0, 0, false, false, 2>;
+        case 6: return k_nonbonded_tiles<0, 1, 1, 0, false, false, 2>;
+        case 12: return k_nonbonded_tiles<1, 1, 0, 0, false, false, 2>;
+        case 14: return k_nonbonded_tiles<1, 1, 1, 0, false, false, 2>;
+        default: break;
+        }
+    }
+    return interp ? tile_kernels_interp[idx] : tile_kernels[idx];
+}
 
 template <bool U, bool DX, bool DL, bool DP, bool DLDP = false>
 __global__ __launch_bounds__(256) void k_pair_list(
@@ -774,10 +809,10 @@ void NbState::compute(
         return;
     }
     const int grid = grid_override > 0 ? grid_override : default_tile_grid();
-    const TileKernel kern = any_interp ? tile_kernels_interp[idx] : tile_kernels[idx];
+    const TileKernel kern = tile_kernel(idx, any_interp, tile_waves);
     kern<<<grid, NB_BLOCK, 0, stream>>>(sys.ptr, S, tile_prefix.ptr, tile_order.ptr, tile_row.ptr, tile_atoms.ptr, perm.ptr, pos.ptr,
-                                        par.ptr, any_interp ? dpar.ptr : nullptr, boxes, status.ptr, du_dx, du_dp,
-                                        du_dl, u);
+                                        par.ptr, any_interp ? dpar.ptr : nullptr, boxes, bb_ctr.ptr, bb_ext.ptr,
+                                        status.ptr, du_dx, du_dp, du_dl, u);
     TM_LAUNCH_CHECK();
 }
 
@@ -798,7 +833,7 @@ void NbState::compute_dl_dp(const double *boxes, uint64_t *dl_dp, hipStream_t st
     }
     k_nonbonded_tiles<false, false, false, false, true><<<grid, NB_BLOCK, 0, stream>>>(
         sys.ptr, S, tile_prefix.ptr, tile_order.ptr, tile_row.ptr, tile_atoms.ptr, perm.ptr, pos.ptr, par.ptr, nullptr, boxes,
-        status.ptr, nullptr, dl_dp, nullptr, nullptr);
+        bb_ctr.ptr, bb_ext.ptr, status.ptr, nullptr, dl_dp, nullptr, nullptr);
     TM_LAUNCH_CHECK();
 }
 

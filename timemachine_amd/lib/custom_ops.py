@@ -922,41 +922,61 @@ class BatchedContexts(_Handle):
     def set_nblist_padding(self, padding: float):
         call("tm_batch_set_nblist_padding", self._h, float(padding))
 
-    def autotune(self, steps: int = 45, grids=(65536, 131072), paddings=(0.05, 0.075, 0.1, 0.15),
+    def set_tile_waves(self, waves: int):
+        """Waves per SIMD the pair kernel is compiled for: 4 (default), 3 or 2 (extension)."""
+        call("tm_batch_set_tile_waves", self._h, int(waves))
+
+    def autotune(self, steps: int = 45, waves=(4, 3, 2), grids=(32768, 65536), paddings=(0.075, 0.1, 0.15),
                  store_du_dl_interval: int = 1, use_graph: bool = True) -> dict:
-        """Chooses the pair-kernel grid and the neighborlist padding (extension).
+        """Chooses the pair kernel's occupancy build and grid, then the
+        neighborlist padding (extension).
 
         Every candidate is timed on a scratch copy of this batch made on the
         device (repack), so this batch's trajectories are untouched; results do
-        not depend on either setting (fixed-point sums are order-independent and
-        the tile kernel tests the exact cutoff), only the speed does. Padding
-        trades neighborlist rebuilds against tiles, the grid trades occupancy
-        against load balance. Applies and returns the fastest setting."""
+        not depend on any of the settings (fixed-point sums are
+        order-independent and the tile kernel tests the exact cutoff), only
+        the speed does. The occupancy build (4, 3 or 2 waves per SIMD) trades
+        latency hiding against registers, the grid (one-wave workgroups)
+        trades load balance against scheduling, padding trades neighborlist
+        rebuilds against tiles. Stage 1 times waves x grids at the first
+        padding in ``paddings`` that is 0.1 nm (else the first); stage 2 times
+        the paddings at the stage-1 winner. Applies and returns the fastest
+        setting."""
         import time
 
         scratch = self.repack(range(len(self)), [], self._lam)
-        # The step time drifts while a freshly built system relaxes, so the
-        # candidates are timed twice, in one order and then in the reverse
-        # order, and ranked by their mean.
         kw = dict(store_du_dl_interval=store_du_dl_interval, use_graph=use_graph)
         scratch.multiple_steps(steps, **kw)
-        order = [(float(pad), int(g)) for pad in paddings for g in grids]
-        times = {c: [] for c in order}
-        for sweep in (order, order[::-1]):
-            for pad, g in sweep:
-                scratch.set_nblist_padding(pad)
-                scratch.set_tile_grid(g)
-                scratch.multiple_steps(max(steps // 3, 1), **kw)
-                t0 = time.perf_counter()
-                scratch.multiple_steps(steps, **kw)
-                times[(pad, g)].append(1e3 * (time.perf_counter() - t0) / steps)
-        table = [{"grid": g, "padding": pad, "ms_per_step": sum(v) / len(v)} for (pad, g), v in times.items()]
+        table = []
+
+        def stage(name, cands):
+            # The step time drifts while a freshly built system relaxes, so the
+            # candidates are timed twice, in one order and then in the reverse
+            # order, and ranked by their mean.
+            times = {c: [] for c in cands}
+            for sweep in (cands, cands[::-1]):
+                for w, g, pad in sweep:
+                    scratch.set_tile_waves(w)
+                    scratch.set_tile_grid(g)
+                    scratch.set_nblist_padding(pad)
+                    scratch.multiple_steps(max(steps // 3, 1), **kw)
+                    t0 = time.perf_counter()
+                    scratch.multiple_steps(steps, **kw)
+                    times[(w, g, pad)].append(1e3 * (time.perf_counter() - t0) / steps)
+            rows = [{"stage": name, "waves": w, "grid": g, "padding": pad, "ms_per_step": sum(v) / len(v)}
+                    for (w, g, pad), v in times.items()]
+            table.extend(rows)
+            return min(rows, key=lambda r: r["ms_per_step"])
+
+        pad0 = 0.1 if any(abs(p - 0.1) < 1e-12 for p in paddings) else float(paddings[0])
+        best = stage("occupancy x grid", [(int(w), int(g), pad0) for w in waves for g in grids])
+        best = stage("padding", [(best["waves"], best["grid"], float(p)) for p in paddings])
         del scratch
-        best = min(table, key=lambda r: r["ms_per_step"])
+        self.set_tile_waves(best["waves"])
         self.set_tile_grid(best["grid"])
         self.set_nblist_padding(best["padding"])
-        return {"grid": best["grid"], "padding": best["padding"], "ms_per_step": best["ms_per_step"],
-                "candidates": table}
+        return {"waves": best["waves"], "grid": best["grid"], "padding": best["padding"],
+                "ms_per_step": best["ms_per_step"], "candidates": table}
 
     def multiple_steps(self, n_steps, lambda_schedules=None, store_du_dl_interval=0, store_x_interval=0, use_graph=True):
         S = len(self._ctxs)
