@@ -104,6 +104,17 @@ void tm_potential_release(tm_potential *p);
 int tm_potential_execute_device(tm_potential *p, int N, int P, const double *d_x, const double *d_p,
                                 const double *d_box, double lambda, uint64_t *d_du_dx, uint64_t *d_du_dp,
                                 uint64_t *d_du_dl, uint64_t *d_u, void *stream);
+/* tm_potential_execute_device for a NonbondedAllPairs potential that allocates no device memory.
+ * No counterpart in the reference: it is for callers that cannot allocate (code under stream
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The neighborlist and tile
+ * arrays live in a fixed 16 MiB static device array per GPU (up to about 21,000 atoms; larger
+ * systems fail with TM_ERR_INVALID), so the potential must cover all atoms with plain
+ * parameters (no atom subset, row subset or interpolation). Same inputs, outputs and fixed-point
+ * results as tm_potential_execute_device. Every call on one GPU shares that array: calls that
+ * overlap on one GPU (two streams or threads) race; sequential calls do not affect each other. */
+int tm_potential_execute_device_noalloc(tm_potential *p, int N, int P, const double *d_x, const double *d_p,
+                                        const double *d_box, double lambda, uint64_t *d_du_dx, uint64_t *d_du_dp,
+                                        uint64_t *d_du_dl, uint64_t *d_u, void *stream);
 /* Potential::execute_host, potential.cu:110-185 (du_dl / u are [N] partial sums). */
 int tm_potential_execute_host(tm_potential *p, int N, int P, const double *x, const double *params,
                               const double *box, double lambda, uint64_t *du_dx, uint64_t *du_dp, uint64_t *du_dl,

@@ -52,6 +52,10 @@ _SIGS = {
         c_int,
         [_P, c_int, c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     ),
+    "tm_potential_execute_device_noalloc": (
+        c_int,
+        [_P, c_int, c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    ),
     "tm_potential_execute_host": (
         c_int,
         [_P, c_int, c_int, _PD, _PD, _PD, c_double, _PU64, _PU64, _PU64, _PU64],

@@ -308,6 +308,15 @@ int tm_potential_execute_device(tm_potential *p, int N, int P, const double *d_x
     });
 }
 
+int tm_potential_execute_device_noalloc(tm_potential *p, int N, int P, const double *d_x, const double *d_p,
+                                        const double *d_box, double lambda, uint64_t *d_du_dx, uint64_t *d_du_dp,
+                                        uint64_t *d_du_dl, uint64_t *d_u, void *stream) {
+    return guarded([&] {
+        execute_device_noalloc(*p->p, N, P, d_x, d_p, d_box, lambda, d_du_dx, d_du_dp, d_du_dl, d_u,
+                               static_cast<hipStream_t>(stream));
+    });
+}
+
 int tm_potential_execute_host(tm_potential *p, int N, int P, const double *x, const double *params,
                               const double *box, double lambda, uint64_t *du_dx, uint64_t *du_dp, uint64_t *du_dl,
                               uint64_t *u) {

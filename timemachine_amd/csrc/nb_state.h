@@ -117,6 +117,14 @@ struct NbState {
     // the rest is laid out here. subset_atoms concatenates subset lists.
     void init(const std::vector<NbSys> &sys_in, const std::vector<int> &subset_atoms);
 
+    // init() for one system without allocating: every array is a view into
+    // arena (at least view_bytes(y) bytes, 256-byte aligned), the descriptor is
+    // written and the counters cleared by one kernel on stream, and the first
+    // prepare() rebuilds. All atoms of one system: no subset, row subset or
+    // interpolated parameters.
+    static size_t view_bytes(const NbSys &y);
+    void init_view(const NbSys &y, void *arena, size_t arena_bytes, hipStream_t stream);
+
     // Force a rebuild of every system on the next prepare().
     void invalidate(hipStream_t stream);
     int find_pieces() const; // waves per row block in the neighborlist build

@@ -790,6 +790,126 @@ void launch_fill_i32(int *p, size_t n, int v, hipStream_t stream) {
     TM_LAUNCH_CHECK();
 }
 
+// Block, slot and tile layout of one system placed at the given offsets.
+static void lay_out(NbSys &y, int slot0, int block0, int tile0, double nb_margin) {
+    if (y.nrow > 0) {
+        if (y.nrow >= y.k || y.subset0 < 0) {
+            throw Error("NbState: a row subset needs column atoms and a subset list");
+        }
+        y.row_blocks = ceil_div(y.nrow, TILE);
+        y.nblocks = y.row_blocks + ceil_div(y.k - y.nrow, TILE);
+        y.row_end = y.nrow;
+        y.slot_end = y.row_blocks * TILE + (y.k - y.nrow);
+    } else {
+        y.nrow = 0;
+        y.nblocks = ceil_div(y.k, TILE);
+        y.row_blocks = y.nblocks;
+        y.row_end = y.k;
+        y.slot_end = y.k;
+    }
+    y.slot0 = slot0;
+    y.block0 = block0;
+    y.tile0 = tile0;
+    y.tile_cap = nb_tile_capacity(y.k, y.nrow);
+    y.nb_cutoff = static_cast<float>(y.cutoff_d + y.padding_d + nb_margin);
+    y.cutoff = static_cast<float>(y.cutoff_d);
+}
+
+// Writes the descriptor of the one system of a view state and clears its
+// counters (the memory holds whatever the last user left); the first
+// prepare() rebuilds.
+__global__ void k_init_view(const NbSys y, NbSys *sys, int *rebuild, int *status, int *tile_count,
+                            int *tile_prefix) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        sys[0] = y;
+        rebuild[0] = 1;
+        status[0] = 0;
+        tile_count[0] = 0;
+        tile_prefix[0] = 0;
+        tile_prefix[1] = 0;
+    }
+}
+
+namespace {
+// Carves consecutive 256-byte aligned arrays out of an arena; with a null
+// arena it only counts the bytes.
+struct Carver {
+    unsigned char *base;
+    size_t used = 0;
+    template <typename T> T *take(size_t count) {
+        used = (used + 255) & ~static_cast<size_t>(255);
+        T *p = base ? reinterpret_cast<T *>(base + used) : nullptr;
+        used += count * sizeof(T);
+        return p;
+    }
+};
+} // namespace
+
+size_t NbState::view_bytes(const NbSys &sys_in) {
+    NbSys y = sys_in;
+    lay_out(y, 0, 0, 0, 0.0);
+    Carver c{nullptr};
+    const size_t slots = static_cast<size_t>(y.nblocks) * TILE;
+    c.take<NbSys>(1);
+    c.take<int>(1);  // rebuild
+    c.take<int>(1);  // status
+    c.take<int>(1);  // tile_count
+    c.take<int>(2);  // tile_prefix
+    c.take<int>(y.tile_cap);
+    c.take<int>(y.tile_cap);
+    c.take<uint32_t>(static_cast<size_t>(y.tile_cap) * TILE);
+    c.take<uint32_t>(slots); // perm
+    c.take<uint32_t>(slots); // keys
+    c.take<double4>(slots);
+    c.take<float4>(slots);
+    c.take<float4>(y.nblocks);
+    c.take<float4>(y.nblocks);
+    c.take<double>(static_cast<size_t>(y.n_atoms) * 3);
+    c.take<double>(9);
+    return c.used;
+}
+
+void NbState::init_view(const NbSys &sys_in, void *arena, size_t arena_bytes, hipStream_t stream) {
+    if (sys_in.subset0 >= 0 || sys_in.nrow > 0 || sys_in.interp != 0 || sys_in.atom0 != 0) {
+        throw Error("NbState::init_view: one system of all atoms, no subset, no interpolation");
+    }
+    const size_t need = view_bytes(sys_in);
+    if (need > arena_bytes) {
+        throw Error("NbState::init_view: " + std::to_string(need) + " bytes needed, " + std::to_string(arena_bytes) +
+                    " available");
+    }
+    NbSys y = sys_in;
+    lay_out(y, 0, 0, 0, nb_margin);
+    S = 1;
+    total_slots = max_slots = y.nblocks * TILE;
+    total_blocks = max_blocks = y.nblocks;
+    total_tiles = y.tile_cap;
+    max_atoms = y.n_atoms;
+    any_interp = false;
+    Carver c{static_cast<unsigned char *>(arena)};
+    const size_t slots = static_cast<size_t>(total_slots);
+    sys.view(c.take<NbSys>(1), 1);
+    rebuild.view(c.take<int>(1), 1);
+    status.view(c.take<int>(1), 1);
+    tile_count.view(c.take<int>(1), 1);
+    tile_prefix.view(c.take<int>(2), 2);
+    tile_row.view(c.take<int>(total_tiles), total_tiles);
+    tile_order.view(c.take<int>(total_tiles), total_tiles);
+    tile_atoms.view(c.take<uint32_t>(static_cast<size_t>(total_tiles) * TILE), static_cast<size_t>(total_tiles) * TILE);
+    perm.view(c.take<uint32_t>(slots), slots);
+    keys.view(c.take<uint32_t>(slots), slots);
+    pos.view(c.take<double4>(slots), slots);
+    par.view(c.take<float4>(slots), slots);
+    bb_ctr.view(c.take<float4>(total_blocks), total_blocks);
+    bb_ext.view(c.take<float4>(total_blocks), total_blocks);
+    nblist_x.view(c.take<double>(static_cast<size_t>(y.n_atoms) * 3), static_cast<size_t>(y.n_atoms) * 3);
+    nblist_box.view(c.take<double>(9), 9);
+    subset.view(nullptr, 0);
+    dpar.view(nullptr, 0);
+    k_init_view<<<1, 64, 0, stream>>>(y, sys.ptr, rebuild.ptr, status.ptr, tile_count.ptr, tile_prefix.ptr);
+    TM_LAUNCH_CHECK();
+}
+
 void NbState::init(const std::vector<NbSys> &sys_in, const std::vector<int> &subset_atoms) {
     S = static_cast<int>(sys_in.size());
     h_sys = sys_in;
@@ -798,27 +918,7 @@ void NbState::init(const std::vector<NbSys> &sys_in, const std::vector<int> &sub
     int total_atoms_end = 0;
     for (int s = 0; s < S; s++) {
         NbSys &y = h_sys[s];
-        if (y.nrow > 0) {
-            if (y.nrow >= y.k || y.subset0 < 0) {
-                throw Error("NbState: a row subset needs column atoms and a subset list");
-            }
-            y.row_blocks = ceil_div(y.nrow, TILE);
-            y.nblocks = y.row_blocks + ceil_div(y.k - y.nrow, TILE);
-            y.row_end = y.nrow;
-            y.slot_end = y.row_blocks * TILE + (y.k - y.nrow);
-        } else {
-            y.nrow = 0;
-            y.nblocks = ceil_div(y.k, TILE);
-            y.row_blocks = y.nblocks;
-            y.row_end = y.k;
-            y.slot_end = y.k;
-        }
-        y.slot0 = total_slots;
-        y.block0 = total_blocks;
-        y.tile0 = total_tiles;
-        y.tile_cap = nb_tile_capacity(y.k, y.nrow);
-        y.nb_cutoff = static_cast<float>(y.cutoff_d + y.padding_d + nb_margin);
-        y.cutoff = static_cast<float>(y.cutoff_d);
+        lay_out(y, total_slots, total_blocks, total_tiles, nb_margin);
         total_slots += y.nblocks * TILE;
         total_blocks += y.nblocks;
         total_tiles += y.tile_cap;

@@ -113,6 +113,12 @@ class NonbondedAllPairs : public Potential {
     void disable_hilbert_sort();
     // Number of tiles of the current neighborlist (synchronises).
     int num_tiles(hipStream_t stream);
+    // execute_device with every intermediate array in caller-provided device
+    // memory (arena, arena_bytes): allocates nothing and keeps no state between
+    // calls. All atoms, plain parameters (no subset, row subset or interpolation).
+    void execute_device_in(void *arena, size_t arena_bytes, int N, int P, const double *d_x, const double *d_p,
+                           const double *d_box, double lambda, uint64_t *d_du_dx, uint64_t *d_du_dp,
+                           uint64_t *d_du_dl, uint64_t *d_u, hipStream_t stream);
 
     int N_, K_;
     int nrow_ = 0; // > 0: only pairs between the first nrow_ atoms of subset_ and the rest
@@ -126,12 +132,21 @@ class NonbondedAllPairs : public Potential {
     std::string name_ = "NonbondedAllPairs"; // in error messages
 
   private:
+    NbSys descriptor() const; // the one-system neighborlist descriptor of this potential
     void ensure_state();
     void validate_sizes(int N, int P) const;
     bool dirty_ = true;
     DeviceArray<int> d_plane_, d_offset_;
     DeviceArray<uint64_t> dp_tmp_; // du/dp at the interpolated parameters [N, 3]
 };
+
+// Allocation-free execute_device of a NonbondedAllPairs (noalloc.hip): the
+// intermediate arrays live in a static device array of NOALLOC_SCRATCH_BYTES
+// per device (about 21,000 atoms at most), shared by every call on that device.
+constexpr size_t NOALLOC_SCRATCH_BYTES = static_cast<size_t>(16) << 20;
+void execute_device_noalloc(Potential &pot, int N, int P, const double *d_x, const double *d_p, const double *d_box,
+                            double lambda, uint64_t *d_du_dx, uint64_t *d_du_dp, uint64_t *d_du_dl, uint64_t *d_u,
+                            hipStream_t stream);
 
 // Pairs (i, j) with i a row atom and j any other atom
 // (nonbonded_interaction_group.cu:19-468): the all-pairs machinery with the

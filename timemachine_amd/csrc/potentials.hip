@@ -183,10 +183,7 @@ void NonbondedAllPairs::disable_hilbert_sort() {
     dirty_ = true;
 }
 
-void NonbondedAllPairs::ensure_state() {
-    if (!dirty_) {
-        return;
-    }
+NbSys NonbondedAllPairs::descriptor() const {
     NbSys y{};
     y.atom0 = 0;
     y.n_atoms = N_;
@@ -199,9 +196,42 @@ void NonbondedAllPairs::ensure_state() {
     y.beta = static_cast<float>(beta_);
     y.cutoff_d = cutoff_;
     y.padding_d = padding_;
+    return y;
+}
+
+void NonbondedAllPairs::ensure_state() {
+    if (!dirty_) {
+        return;
+    }
     st_.nb_margin = NB_LIST_MARGIN;
-    st_.init({y}, subset_);
+    st_.init({descriptor()}, subset_);
     dirty_ = false;
+}
+
+void NonbondedAllPairs::execute_device_in(void *arena, size_t arena_bytes, int N, int P, const double *d_x,
+                                          const double *d_p, const double *d_box, double lambda, uint64_t *d_du_dx,
+                                          uint64_t *d_du_dp, uint64_t *d_du_dl, uint64_t *d_u, hipStream_t stream) {
+    validate_sizes(N, P);
+    if (interp_ || !subset_.empty() || nrow_ > 0) {
+        throw Error(name_ + "::execute_device_noalloc(): takes all atoms with plain parameters "
+                            "(no atom subset, row subset or interpolation)");
+    }
+    if (K_ == 0) {
+        return;
+    }
+    const NbSys y = descriptor();
+    const size_t need = NbState::view_bytes(y);
+    if (need > arena_bytes) {
+        throw Error(name_ + "::execute_device_noalloc(): N=" + std::to_string(N) + " needs " + std::to_string(need) +
+                    " bytes of scratch, more than the " + std::to_string(arena_bytes) + "-byte maximum");
+    }
+    // a state whose arrays all lie in the arena: nothing is allocated, and the
+    // first prepare() rebuilds (the arena holds the previous caller's data)
+    NbState st;
+    st.nb_margin = NB_LIST_MARGIN;
+    st.init_view(y, arena, arena_bytes, stream);
+    st.prepare(d_x, d_p, nullptr, d_box, d_plane_.ptr, d_offset_.ptr, nullptr, lambda, stream);
+    st.compute(d_box, d_du_dx, d_du_dp, d_du_dl, d_u, stream);
 }
 
 void NonbondedAllPairs::validate_host(int N, int P, const double *h_box) const {

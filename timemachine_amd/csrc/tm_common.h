@@ -63,21 +63,25 @@ constexpr double ONE_4PI_EPS0 = 138.935456;
 template <typename T> struct DeviceArray {
     T *ptr = nullptr;
     size_t n = 0;
+    bool owned = true; // false: a view of memory this array does not free
     DeviceArray() = default;
     explicit DeviceArray(size_t count) { alloc(count); }
     DeviceArray(const DeviceArray &) = delete;
     DeviceArray &operator=(const DeviceArray &) = delete;
-    DeviceArray(DeviceArray &&o) noexcept : ptr(o.ptr), n(o.n) {
+    DeviceArray(DeviceArray &&o) noexcept : ptr(o.ptr), n(o.n), owned(o.owned) {
         o.ptr = nullptr;
         o.n = 0;
+        o.owned = true;
     }
     DeviceArray &operator=(DeviceArray &&o) noexcept {
         if (this != &o) {
             release();
             ptr = o.ptr;
             n = o.n;
+            owned = o.owned;
             o.ptr = nullptr;
             o.n = 0;
+            o.owned = true;
         }
         return *this;
     }
@@ -90,13 +94,22 @@ template <typename T> struct DeviceArray {
             TM_HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&ptr), count * sizeof(T)));
         }
     }
+    // Points at count elements of existing device memory (no allocation;
+    // release() forgets the pointer without freeing it).
+    void view(T *p, size_t count) {
+        release();
+        ptr = count > 0 ? p : nullptr;
+        n = count;
+        owned = false;
+    }
     void release() {
-        if (ptr) {
+        if (ptr && owned) {
             DeviceLock lock;
             (void)hipFree(ptr);
-            ptr = nullptr;
         }
+        ptr = nullptr;
         n = 0;
+        owned = true;
     }
     size_t bytes() const { return n * sizeof(T); }
     void upload(const T *host, size_t count, hipStream_t s = 0) {
