@@ -51,8 +51,14 @@ IMPL_FLOP_PAIR_LJ = 13
 # HBM bytes per launch of the du_dx tile kernel on this workload: FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0 (factors from a 256 MiB device copy), measured with
 # rocprofv3 --pmc on MI355X (summarised under profiles/ with the commands; the
-# counter program profiles the boxes as built, ~4% more tiles than relaxed).
-TILE_TRAFFIC_BYTES = 0.9846e9 + 2.1206e9
+# counter program profiles the boxes as built after FIRE, 0.1% fewer tiles than
+# the relaxed state of the kernel trace).
+TILE_TRAFFIC_BYTES = 1.0164e9 + 2.0985e9
+# Serving admits jobs while the resident atoms stay under this: the batched
+# engine's step time is ~0.12 ms + atoms / 3.6e8 atom-steps/s on one MI355X,
+# so 800k resident atoms run at ~95% of the peak rate, and admitting more only
+# makes every resident job finish later.
+SERVING_MAX_RESIDENT_ATOMS = 800_000
 # The HIP-event kernel profile (roofline and kernel_ms) runs at one fixed
 # setting, the one of the committed rocprofv3 summaries, whatever the autotuner
 # chose for the timed steps.
@@ -170,6 +176,18 @@ def serving_jobs(n_jobs: int, step_scale: int, seed: int = 5):
     return serving.synthetic_stream(n_jobs, rate_per_s=0.0, seed=seed, workers=workers, **cfg), cfg
 
 
+def small_serving_jobs(n_jobs: int, seed: int = 9):
+    """A second stream of small jobs (1.8k-4k atoms, 4-12 windows, 400-1000
+    steps): each job alone leaves most of the GPU idle, which is where
+    continuous batching pays."""
+    from timemachine_amd import serving
+
+    cfg = dict(atoms=(1800, 4000), windows=(4, 12), steps=(400, 1000), heavy=(20, 30), minimize_steps=99)
+    workers = min(16, os.cpu_count() or 1)
+    log(f"building {n_jobs} small serving jobs on the host ({workers} processes)")
+    return serving.synthetic_stream(n_jobs, rate_per_s=0.0, seed=seed, workers=workers, **cfg), cfg
+
+
 def serving_leg(jobs, cfg, step_scale: int, seed: int = 5) -> dict:
     """Continuous batching of an uneven job stream vs one job at a time (FIFO).
 
@@ -185,9 +203,10 @@ def serving_leg(jobs, cfg, step_scale: int, seed: int = 5) -> dict:
 
     n_jobs = len(jobs)
     work = sum(j.atom_steps for j in jobs)
+    fire_work = sum(j.system.N * j.windows * j.minimize_steps for j in jobs)
     budget = 32 * 2**30
     log(f"serving: {n_jobs} jobs, {work:.3g} atom-steps; saturated run for capacity")
-    srv = serving.Server(budget, max_chunk=60, du_dl_interval=10)
+    srv = serving.Server(budget, max_chunk=600, du_dl_interval=10, max_resident_atoms=SERVING_MAX_RESIDENT_ATOMS)
     t0 = time.perf_counter()
     srv.run(jobs)
     capacity = work / (time.perf_counter() - t0)
@@ -198,7 +217,7 @@ def serving_leg(jobs, cfg, step_scale: int, seed: int = 5) -> dict:
         t += rng.exponential(1.0 / rate)
         j.arrival = t
     log(f"serving: Poisson arrivals at {rate:.3g} jobs/s (80% of capacity {capacity:.3g} atom-steps/s)")
-    srv = serving.Server(budget, max_chunk=60, du_dl_interval=10)
+    srv = serving.Server(budget, max_chunk=600, du_dl_interval=10, max_resident_atoms=SERVING_MAX_RESIDENT_ATOMS)
     t0 = time.perf_counter()
     served = srv.run(jobs)
     span = time.perf_counter() - t0
@@ -217,6 +236,8 @@ def serving_leg(jobs, cfg, step_scale: int, seed: int = 5) -> dict:
         "job_windows": [int(j.windows) for j in jobs],
         "jobs": n_jobs,
         "atom_steps": work,
+        "fire_atom_steps": fire_work,
+        "max_resident_atoms": SERVING_MAX_RESIDENT_ATOMS,
         "capacity_atom_steps_per_s": capacity,
         "arrival_rate_per_s": rate,
         "p50_latency_s": st["p50_s"],
@@ -225,6 +246,7 @@ def serving_leg(jobs, cfg, step_scale: int, seed: int = 5) -> dict:
         "max_resident_systems": max(r["systems"] for r in srv.log),
         "chunks": len(srv.log),
         "chunk_seconds": sum(r["seconds"] for r in srv.log),
+        "chunk_atom_steps_per_s": work / sum(r["seconds"] for r in srv.log),
         "repack_seconds": srv.repack_seconds,
         "setup_seconds": srv.setup_seconds,
         "fifo": {"p50_latency_s": sf["p50_s"], "p99_latency_s": sf["p99_s"], "atom_steps_per_s": work / span_fifo},
@@ -348,7 +370,7 @@ def main() -> int:
     ap.add_argument("--single-steps", type=int, default=300)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serving-jobs", type=int, default=16, help="jobs in the serving leg (0 skips it)")
-    ap.add_argument("--serving-step-scale", type=int, default=20,
+    ap.add_argument("--serving-step-scale", type=int, default=10,
                     help="the serving jobs' U(2000, 20000) steps are divided by this")
     ap.add_argument("--diff-frames", type=int, default=500,
                     help="frames per window of the differentiation leg (16 windows; 0 skips it)")
@@ -380,9 +402,10 @@ def main() -> int:
         log(f"copied {n} bytes ({n // 1024} KiB read, {n // 1024} KiB written)")
         return 0
 
-    stream = None
+    stream = small_stream = None
     if args.serving_jobs > 0 and not args.profile_only:
         stream = serving_jobs(args.serving_jobs, args.serving_step_scale)
+        small_stream = small_serving_jobs(args.serving_jobs)
 
     from timemachine_amd import _lib
     from timemachine_amd import testsystems as ts
@@ -475,9 +498,12 @@ def main() -> int:
     diff = None
     if args.diff_frames > 0:
         diff = differentiation_leg(sys_, n_frames=args.diff_frames)
-    serve = None
+    serve = serve_small = None
     if stream is not None:
         serve = serving_leg(stream[0], stream[1], args.serving_step_scale)
+        serve_small = serving_leg(small_stream[0], small_stream[1], 1, seed=11)
+        serve_small["stream"] = serve_small["stream"].replace(
+            "steps U(400, 1000) (the serving workload's U(2000, 20000) divided by 1)", "steps U(400, 1000)")
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (FP64 oracle)")
@@ -525,6 +551,7 @@ def main() -> int:
         "kernel_ms": prof,
         "roofline": roofline,
         "serving": serve,
+        "serving_small_jobs": serve_small,
         "differentiation": diff,
         "cpu_baseline": cpu,
     }

@@ -170,6 +170,38 @@ def test_graph_and_eager_paths_agree(ligand):
     np.testing.assert_array_equal(b1.get_x_t(0), b2.get_x_t(0))
 
 
+def test_graphs_kept_across_runs_equal_eager(ligand):
+    """An engine keeps the graph executables of runs without frames or a lambda
+    schedule and replays them in later calls. Runs of several lengths (the
+    du/dl buffer grows past its first size, which drops the kept graphs), a
+    run without du/dl, a change of tile grid and occupancy build in between
+    (new graphs), all against the eager path: du/dl series and final states
+    bit for bit."""
+    lams = [0.25, 0.75]
+
+    def batch():
+        made = [_context(ligand, seed=500 + s, barostat_interval=5) for s in range(len(lams))]
+        return custom_ops.BatchedContexts([m[0] for m in made], lams), made
+
+    bg, keep_g = batch()
+    be, keep_e = batch()
+    plan = [(17, 1), (30, 1), (45, 5), (20, 0), (130, 1), (30, 1), (30, 1)]
+    for k, (n, interval) in enumerate(plan):
+        if k == 5:
+            for b in (bg, be):
+                b.set_tile_grid(4096)
+                b.set_tile_waves(3)
+        print(f"run {k}: {n} steps, du/dl every {interval}", flush=True)
+        dg, _, _ = bg.multiple_steps(n, store_du_dl_interval=interval, use_graph=True)
+        de, _, _ = be.multiple_steps(n, store_du_dl_interval=interval, use_graph=False)
+        np.testing.assert_array_equal(dg, de)
+        assert dg.shape == (len(lams), n // interval if interval else 0)
+    for s in range(len(lams)):
+        np.testing.assert_array_equal(bg.get_x_t(s), be.get_x_t(s))
+        np.testing.assert_array_equal(bg.get_v_t(s), be.get_v_t(s))
+        np.testing.assert_array_equal(bg.get_box(s), be.get_box(s))
+
+
 def test_barostat_moves_volume_and_validates(ligand):
     ctx, keep, _ = _context(ligand, barostat_interval=3)
     v0 = np.prod(np.diag(ctx.get_box()))

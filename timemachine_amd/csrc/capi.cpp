@@ -980,12 +980,12 @@ int tm_batch_fire_minimize(tm_batch *b, int n_sched, const double *lambda_sched,
         if (active) {
             act.assign(active, active + b->eng->num_systems());
         }
-        b->eng->fire_minimize(n_sched, lambda_sched, active ? act.data() : nullptr);
+        b->eng->fire_minimize(n_sched, lambda_sched, active ? act.data() : nullptr, b->raise_faults);
     });
 }
 
 int tm_batch_max_force(tm_batch *b, double *out) {
-    return guarded([&] { b->eng->max_force(out); });
+    return guarded([&] { b->eng->max_force(out, b->raise_faults); });
 }
 
 int tm_fire_minimize(int N, const double *x0, const double *box, int n_bps, tm_bound_potential *const *bps,

@@ -102,9 +102,10 @@ class Engine {
     // step per further entry). lambda_sched: [S, n_sched] host array, or
     // nullptr for the current lambdas; active: [S] flags or nullptr (all).
     // Velocities, box and step counters are left alone.
-    void fire_minimize(int n_sched, const double *lambda_sched, const int *active);
+    // raise = false: faulted systems are left flagged in the status instead of throwing
+    void fire_minimize(int n_sched, const double *lambda_sched, const int *active, bool raise = true);
     // Largest per-atom force norm of every system at its current state and lambda.
-    void max_force(double *out);
+    void max_force(double *out, bool raise = true);
 
     // Gradient observer for lambda-window averages of A = du/dlambda. Every
     // `interval` steps of run() (0: off), at the states where run() stores

@@ -601,6 +601,24 @@ struct Engine::Impl {
     DeviceArray<int> nb_plane, nb_offset;
     // an interaction group per system (row-subset layout), same arrays
     DeviceArray<uint64_t> dx_trial, dl_trial; // forces and du/dl of a deferred barostat move's state
+    // Graph executables of run() kept across calls (runs without a lambda
+    // schedule, frames or window energies): keyed by the step pattern of a
+    // graph chunk and every setting its launches bake in. They write du/dl
+    // into run_dl, one row of run_dl_cap entries per system.
+    std::map<std::vector<int>, hipGraphExec_t> graphs;
+    DeviceArray<uint64_t> run_dl;
+    int run_dl_cap = 0;
+    DeviceArray<double> run_box_frames; // never written by those runs (no frames), its address is baked in
+    void clear_graphs() {
+        if (graphs.empty()) {
+            return;
+        }
+        DeviceLock lock;
+        for (auto &kv : graphs) {
+            (void)hipGraphExecDestroy(kv.second);
+        }
+        graphs.clear();
+    }
     bool has_ig = false;
     NbState ig;
     DeviceArray<double> ig_params, ig_params1;
@@ -1168,6 +1186,9 @@ Engine::~Engine() {
         (void)hipStreamSynchronize(stream_);
         (void)hipStreamDestroy(stream_);
     }
+    if (impl_) {
+        impl_->clear_graphs();
+    }
     impl_.reset();
 }
 
@@ -1201,9 +1222,36 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
         sched.alloc(static_cast<size_t>(S_) * n_steps);
         sched.upload(lambda_sched, static_cast<size_t>(S_) * n_steps, stream_);
     }
-    DeviceArray<uint64_t> dl_traj(std::max<size_t>(static_cast<size_t>(S_) * K, 1));
-    DeviceArray<double> frames(std::max<size_t>(static_cast<size_t>(F) * A_ * 3, 1));
-    DeviceArray<double> box_frames(std::max<size_t>(static_cast<size_t>(S_) * F * 9, 1));
+    // Runs without a lambda schedule, frames or window energies keep their graph
+    // executables on the engine (the serving loop and the benchmark call run()
+    // once per chunk of steps); du/dl then goes to the engine's run_dl, with a
+    // row stride (Kst) that does not change from run to run.
+    const bool keep_graphs = use_graph && !lambda_sched && Fu == 0 && x_interval <= 0;
+    DeviceArray<uint64_t> dl_local;
+    DeviceArray<double> frames(keep_graphs ? 1 : std::max<size_t>(static_cast<size_t>(F) * A_ * 3, 1));
+    DeviceArray<double> box_local;
+    uint64_t *dl_ptr = nullptr;
+    double *box_frames_ptr = nullptr;
+    int Kst = K;
+    if (keep_graphs) {
+        if (I.run_dl_cap < std::max(K, 1)) {
+            const int cap = std::max({K, 2 * I.run_dl_cap, 64});
+            I.clear_graphs(); // their launches point at the old buffer
+            I.run_dl.alloc(static_cast<size_t>(S_) * cap);
+            I.run_dl_cap = cap;
+        }
+        if (I.run_box_frames.n == 0) {
+            I.run_box_frames.alloc(9);
+        }
+        dl_ptr = I.run_dl.ptr;
+        Kst = I.run_dl_cap;
+        box_frames_ptr = I.run_box_frames.ptr;
+    } else {
+        dl_local.alloc(std::max<size_t>(static_cast<size_t>(S_) * K, 1));
+        box_local.alloc(std::max<size_t>(static_cast<size_t>(S_) * F * 9, 1));
+        dl_ptr = dl_local.ptr;
+        box_frames_ptr = box_local.ptr;
+    }
     launch_fill_i32(I.run_step.ptr, S_, 0, stream_);
     launch_fill_u64(I.du_dl_buf.ptr, S_, 0, stream_);
 
@@ -1318,8 +1366,8 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
             TM_LAUNCH_CHECK();
         }
         k_step_end<<<ceil_div(S_, 64), 64, 0, stream_>>>(S_, I.step.ptr, I.run_step.ptr, I.du_dl_buf.ptr,
-                                                        du_dl_interval, K, dl_traj.ptr, x_interval, F, I.box.ptr,
-                                                        box_frames.ptr);
+                                                        du_dl_interval, Kst, dl_ptr, x_interval, F, I.box.ptr,
+                                                        box_frames_ptr);
         TM_LAUNCH_CHECK();
     };
 
@@ -1372,14 +1420,17 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
                 }
             }
         }
-        std::map<std::vector<uint8_t>, hipGraphExec_t> cache;
+        // per-run executables unless this run keeps them on the engine
+        std::map<std::vector<int>, hipGraphExec_t> run_cache;
+        std::map<std::vector<int>, hipGraphExec_t> &cache = keep_graphs ? I.graphs : run_cache;
         while (j + chunk <= n_steps) {
-            std::vector<uint8_t> pattern(chunk);
+            // the step pattern and every value the captured launches bake in
+            std::vector<int> pattern(chunk);
             for (int q = 0; q < chunk; q++) {
-                pattern[q] = static_cast<uint8_t>((baro_at(j + q) ? 1 : 0) | (obs_at(j + q) ? 2 : 0) |
-                                                  (u_at(j + q) ? 4 : 0) | (defer_at(j + q) ? 8 : 0) |
-                                                  (resolve_at(j + q) ? 16 : 0));
+                pattern[q] = (baro_at(j + q) ? 1 : 0) | (obs_at(j + q) ? 2 : 0) | (u_at(j + q) ? 4 : 0) |
+                             (defer_at(j + q) ? 8 : 0) | (resolve_at(j + q) ? 16 : 0);
             }
+            pattern.insert(pattern.end(), {du_dl_interval, Kst, tile_grid, I.nb.tile_waves, obs_k});
             auto it = cache.find(pattern);
             if (it == cache.end()) {
                 DeviceLock lock; // no other thread may allocate or synchronise the device meanwhile
@@ -1400,7 +1451,7 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
         }
         TM_HIP_CHECK(hipStreamSynchronize(stream_));
         DeviceLock lock;
-        for (auto &kv : cache) {
+        for (auto &kv : run_cache) {
             TM_HIP_CHECK(hipGraphExecDestroy(kv.second));
         }
     }
@@ -1415,7 +1466,8 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
     TM_HIP_CHECK(hipStreamSynchronize(stream_));
     if (du_dl_out && K > 0) {
         std::vector<uint64_t> h(static_cast<size_t>(S_) * K);
-        dl_traj.download(h.data(), h.size(), stream_);
+        TM_HIP_CHECK(hipMemcpy2DAsync(h.data(), sizeof(uint64_t) * K, dl_ptr, sizeof(uint64_t) * Kst,
+                                      sizeof(uint64_t) * K, S_, hipMemcpyDeviceToHost, stream_));
         TM_HIP_CHECK(hipStreamSynchronize(stream_));
         for (size_t i = 0; i < h.size(); i++) {
             du_dl_out[i] = fixed_to_double(h[i], FIXED_SCALE);
@@ -1433,7 +1485,7 @@ void Engine::run(int n_steps, const double *lambda_sched, int du_dl_interval, in
         }
     }
     if (box_out && F > 0) {
-        box_frames.download(box_out, static_cast<size_t>(S_) * F * 9, stream_);
+        box_local.download(box_out, static_cast<size_t>(S_) * F * 9, stream_);
     }
     TM_HIP_CHECK(hipStreamSynchronize(stream_));
 }
@@ -1538,6 +1590,7 @@ void Engine::set_grad_observer(int interval) {
         }
         upload_vec(I.obs_sys, esys);
         upload_vec(I.obs_inv_scale, inv);
+        I.clear_graphs(); // kept graphs point at the observer's old buffers
         I.obs_dudp.alloc(I.Pg);
         I.obs_dldp.alloc(I.Pg);
         I.obs_dl.alloc(S_);
@@ -1608,7 +1661,7 @@ void Engine::grad_stats(int s, int *n_frames, double *sum_a, double *sum_aa, dou
     TM_HIP_CHECK(hipStreamSynchronize(stream_));
 }
 
-void Engine::fire_minimize(int n_sched, const double *lambda_sched, const int *active) {
+void Engine::fire_minimize(int n_sched, const double *lambda_sched, const int *active, bool raise) {
     Impl &I = *impl_;
     if (n_sched < 1) {
         throw Error("fire_minimize: the lambda schedule needs at least one entry");
@@ -1685,10 +1738,12 @@ void Engine::fire_minimize(int n_sched, const double *lambda_sched, const int *a
         TM_HIP_CHECK(hipGraphExecDestroy(ge));
     }
     TM_HIP_CHECK(hipStreamSynchronize(stream_));
-    check_status();
+    if (raise) {
+        check_status();
+    }
 }
 
-void Engine::max_force(double *out) {
+void Engine::max_force(double *out, bool raise) {
     Impl &I = *impl_;
     launch_fill_u64(I.du_dx.ptr, A_ * 3, 0, stream_);
     I.eval(I.x.ptr, I.box.ptr, I.du_dx.ptr, nullptr, nullptr, stream_, tile_grid);
@@ -1697,7 +1752,9 @@ void Engine::max_force(double *out) {
     TM_LAUNCH_CHECK();
     d.download(out, S_, stream_);
     TM_HIP_CHECK(hipStreamSynchronize(stream_));
-    check_status();
+    if (raise) {
+        check_status();
+    }
 }
 
 void Engine::energies(double *u_out, double *du_dl_out) {

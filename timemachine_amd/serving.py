@@ -136,10 +136,10 @@ def make_window_contexts(job: Job):
     return out
 
 
-def make_window_batch(job: Job) -> custom_ops.BatchedContexts:
+def make_window_batch(job: Job, minimize: bool = True) -> custom_ops.BatchedContexts:
     """All lambda windows of a job in one engine (one constructor call, no
     Context per window), seeded per job and window, and FIRE-minimised at each
-    window's lambda on the device when the job asks for it."""
+    window's lambda on the device when the job asks for it (and ``minimize``)."""
     sys_ = job.system
     bps = [p.bound_impl(np.float32) for p in sys_.bound_potentials()]
     seeds = [window_seed(job, w) for w in range(job.windows)]
@@ -152,7 +152,7 @@ def make_window_batch(job: Job) -> custom_ops.BatchedContexts:
     v0s = np.stack([ts.maxwell_boltzmann(sys_.masses, job.temperature, sd) for sd in seeds])
     batch = custom_ops.BatchedContexts.from_windows(sys_.x, v0s, sys_.box, intg, seeds, bps, job.lambdas,
                                                      barostat=baro, barostat_seeds=[sd + 17 for sd in seeds])
-    if job.minimize_steps > 0:
+    if minimize and job.minimize_steps > 0:
         batch.fire_minimize(n_steps=job.minimize_steps)
         worst = batch.max_force()
         if not np.all(worst < FORCE_LIMIT):
@@ -160,14 +160,69 @@ def make_window_batch(job: Job) -> custom_ops.BatchedContexts:
     return batch
 
 
+def minimize_together(jobs: Sequence[Job], batches: Sequence[custom_ops.BatchedContexts]):
+    """FIRE-minimises the windows of several jobs in one engine (one launch
+    sequence for all of them instead of one per job); jobs with the same
+    number of FIRE steps share a run. FIRE is per system (each window's own
+    step size and mixing), so every window ends bitwise where its own job's
+    batch would. Returns, per job, the batch and the window indices holding
+    its minimised windows, or the AssertionError of a job whose forces stayed
+    too large, or the RuntimeError of a job with a faulted window (bad box)."""
+    out: List = [None] * len(jobs)
+    groups: Dict[int, List[int]] = {}
+    for k, job in enumerate(jobs):
+        if job.minimize_steps > 0:
+            groups.setdefault(job.minimize_steps, []).append(k)
+        else:
+            out[k] = (batches[k], list(range(job.windows)))
+    for n_steps, ks in groups.items():
+        if len(ks) == 1:
+            merged, offs = batches[ks[0]], [0]
+        else:
+            parts, lam, offs = [], [], []
+            for k in ks:
+                offs.append(len(parts))
+                parts += [(batches[k], w) for w in range(jobs[k].windows)]
+                lam += [float(v) for v in jobs[k].lambdas]
+            merged = custom_ops.BatchedContexts.gather(parts, lam)
+        # a window with a bad box (or another fault) fails its job only: the
+        # engine flags it, skips its pairs and keeps minimising the others
+        merged.set_raise_faults(False)
+        merged.fire_minimize(n_steps=n_steps)
+        worst = merged.max_force()
+        status = merged.status()
+        for k, o in zip(ks, offs):
+            W = jobs[k].windows
+            bad = np.nonzero(status[o : o + W])[0]
+            if bad.size:
+                out[k] = RuntimeError(f"window {int(bad[0])}: " + custom_ops.status_message(int(status[o + bad[0]])))
+            elif np.all(worst[o : o + W] < FORCE_LIMIT):
+                out[k] = (merged, list(range(o, o + W)))
+            else:
+                out[k] = AssertionError(f"job {jobs[k].job_id}: forces much greater than expected after minimization")
+    return out
+
+
 class Server:
-    """Continuous batching of jobs into one engine under an HBM budget."""
+    """Continuous batching of jobs into one engine under an HBM budget.
+
+    Admission is first come first served while the job fits the HBM budget
+    and, when ``max_resident_atoms`` is set, while the resident atoms stay
+    under it (the engine's throughput saturates at about a million atoms on
+    one MI355X; admitting beyond that only slows every resident job). A job
+    is always admitted onto an idle GPU. A chunk runs until the next event
+    the loop has to act on: a window finishing, or the next arrival while
+    that job would fit, estimated from the last chunk's step time; at most
+    ``max_chunk`` steps."""
 
     def __init__(self, hbm_budget_bytes: int, max_chunk: int = 600, du_dl_interval: int = 10,
                  use_graph: bool = True, clock: Callable[[], float] = time.perf_counter,
-                 sleep: Callable[[float], None] = time.sleep, prepare_ahead: int = 0):
+                 sleep: Callable[[float], None] = time.sleep, prepare_ahead: int = 0,
+                 max_resident_atoms: Optional[int] = None):
         if du_dl_interval <= 0 or max_chunk < du_dl_interval:
             raise ValueError("need 0 < du_dl_interval <= max_chunk")
+        self.max_resident_atoms = None if max_resident_atoms is None else int(max_resident_atoms)
+        self._step_s = None  # seconds per step of the last chunk
         self.budget = int(hbm_budget_bytes)
         self.max_chunk = int(max_chunk) - int(max_chunk) % int(du_dl_interval)
         self.du_dl_interval = int(du_dl_interval)
@@ -201,6 +256,15 @@ class Server:
     def _job_bytes(self, job: Job) -> int:
         return job.windows * estimate_system_bytes(job.system.N)
 
+    def _fits(self, job: Job, used: int, resident_atoms: int, busy: bool) -> bool:
+        """Whether job can be admitted now: within the HBM budget and, when
+        the GPU is busy, within the resident-atom cap."""
+        if used + self._job_bytes(job) > self.budget:
+            return False
+        if busy and self.max_resident_atoms is not None:
+            return resident_atoms + job.system.N * job.windows <= self.max_resident_atoms
+        return True
+
     def run(self, jobs: Sequence[Job]) -> List[JobResult]:
         for j in jobs:
             if j.n_steps % self.du_dl_interval != 0:
@@ -233,33 +297,59 @@ class Server:
             for job in pending[: self.prepare_ahead]:
                 if job.arrival <= now() and job.job_id not in prepared:
                     prepared[job.job_id] = pool.submit(self._prepare, job)
-        # admit, first come first served, while the budget allows
-        new: List[_Window] = []
-        new_batches = []
-        while pending and pending[0].arrival <= now() and used + self._job_bytes(pending[0]) <= self.budget:
+        # admit, first come first served, while the HBM budget and the atom cap allow
+        resident = sum(r.job.system.N for r in running)
+        new_parts = []  # (job, batch, that batch's system index of each window)
+        to_minimize: List[Job] = []
+        unminimized = []
+        while pending and pending[0].arrival <= now() and self._fits(
+                pending[0], used, resident, bool(running or new_parts or to_minimize)):
             job = pending[0]
             if pool is not None:
                 fut = prepared.get(job.job_id)
                 if fut is None:
                     fut = prepared[job.job_id] = pool.submit(self._prepare, job)
-                if not fut.done() and (running or new):
+                if not fut.done() and (running or new_parts):
                     break  # keep the batch busy while the job is prepared
                 tw = now()
                 wins = fut.result()
                 self.setup_wait_seconds += now() - tw
                 del prepared[job.job_id]
             else:
-                wins = self._prepare(job)
+                t = time.perf_counter()
+                try:
+                    wins = make_window_batch(job, minimize=False)
+                except (RuntimeError, AssertionError) as e:
+                    wins = e
+                self.setup_seconds += time.perf_counter() - t
             pending.pop(0)
             if isinstance(wins, Exception):
                 t_fail = now()
                 results.append(JobResult(job.job_id, None, float("nan"), job.arrival, t_fail, t_fail, error=str(wins)))
                 continue
             used += self._job_bytes(job)
+            resident += job.system.N * job.windows
             admitted_at[job.job_id] = now()
-            new_batches.append(wins)
-            for w in range(job.windows):
-                new.append(_Window(job, w))
+            if pool is not None:
+                new_parts.append((job, wins, list(range(job.windows))))
+            else:
+                to_minimize.append(job)
+                unminimized.append(wins)
+        if to_minimize:
+            # the windows of every job admitted now are minimised in one engine
+            t = time.perf_counter()
+            done = minimize_together(to_minimize, unminimized)
+            self.setup_seconds += time.perf_counter() - t
+            del unminimized
+            for job, m in zip(to_minimize, done):
+                if isinstance(m, Exception):
+                    t_fail = now()
+                    results.append(JobResult(job.job_id, None, float("nan"), job.arrival, admitted_at[job.job_id],
+                                             t_fail, error=str(m)))
+                    used -= self._job_bytes(job)
+                    continue
+                new_parts.append((job, m[0], m[1]))
+        new: List[_Window] = [_Window(job, w) for job, _, _ in new_parts for w in range(job.windows)]
         if not running and not new:
             if pending:
                 # idle until the next arrival
@@ -269,21 +359,33 @@ class Server:
             tr = now()
             lam = [r.job.lambdas[r.w] for r in running] + [n.job.lambdas[n.w] for n in new]
             parts = [(batch, s) for s in range(len(running))]
-            parts += [(jb, w) for jb in new_batches for w in range(len(jb))]
-            if batch is None and len(new_batches) == 1:
-                batch = new_batches[0]  # the job's own engine becomes the batch
+            parts += [(b, i) for _, b, idxs in new_parts for i in idxs]
+            only = new_parts[0] if len(new_parts) == 1 else None
+            if batch is None and only is not None and only[2] == list(range(len(only[1]))):
+                batch = only[1]  # the job's own engine becomes the batch
             else:
                 batch = custom_ops.BatchedContexts.gather(parts, lam)
-                batch.set_raise_faults(False)
-            del new_batches
+            batch.set_raise_faults(False)
+            del new_parts
             running += new
             self.repack_seconds += now() - tr
         chunk = min(self.max_chunk, min(r.remaining for r in running))
+        if pending and self._fits(pending[0], used, sum(r.job.system.N for r in running), True):
+            # stop at the first boundary after the next job arrives (it would be admitted there)
+            wait = pending[0].arrival - now()
+            if self._step_s is None:
+                steps = 60
+            else:
+                steps = int(math.ceil(max(wait, 0.0) / self._step_s))
+            steps = max(self.du_dl_interval, -(-steps // self.du_dl_interval) * self.du_dl_interval)
+            chunk = min(chunk, steps)
         tc = now()
         du_dl, _, _ = batch.multiple_steps(chunk, store_du_dl_interval=self.du_dl_interval,
                                            use_graph=self.use_graph)
+        dt = now() - tc
+        self._step_s = dt / chunk
         self.log.append({"t": tc, "systems": len(running), "atoms": int(sum(batch.sizes)), "steps": chunk,
-                         "seconds": now() - tc, "budget_used": used, "device_bytes": batch.device_bytes()})
+                         "seconds": dt, "budget_used": used, "device_bytes": batch.device_bytes()})
         for s, r in enumerate(running):
             r.chunks.append(du_dl[s])
             r.remaining -= chunk
