@@ -135,6 +135,25 @@ int tm_potential_execute_dl_dp_device(tm_potential *p, int N, int P, const doubl
                                       const double *d_box, double lambda, uint64_t *d_dl_dp, void *stream);
 int tm_potential_execute_dl_dp_host(tm_potential *p, int N, int P, const double *x, const double *params,
                                     const double *box, double lambda, uint64_t *dl_dp);
+/* Hessian-vector product: the directional derivative of (du/dx, du/dp, du/dlambda) along a
+ * tangent (vx [N, 3], vp [P], vl) of coordinates, parameters and lambda, accumulated into
+ * caller-zeroed hx [N, 3] (fixed point 2^36, like du_dx), hp [P] (du_dp exponents; decode with
+ * tm_potential_du_dp_fixed_to_float) and hl [1] (2^36). Every accumulated term must stay below
+ * 2^63 / 2^36 ~ 1.3e8 in magnitude: scale the tangent (the product is linear in it) to at most ~1.
+ * The host variant does that itself and returns decoded FP64 outputs (hx [N, 3], hp [P],
+ * hl [1]). By the symmetry of the Hessian it is also
+ * the vector-Jacobian product of the gradient with (vx, vp, vl): the adjoint of du/dx that
+ * reverse-mode differentiation of a loss on forces or on a trajectory needs. Null vx / vp: zero
+ * tangent; a null output is not requested. No counterpart in the reference's custom ops: the
+ * reference differentiates its JAX potentials (timemachine/potentials/nonbonded.py:133-264,
+ * bonded.py:150-332) for second derivatives, and describes the trajectory adjoint in
+ * attic/docs/paper.tex:27. Potentials with interpolated parameters return an error. */
+int tm_potential_execute_hvp_device(tm_potential *p, int N, int P, const double *d_x, const double *d_p,
+                                    const double *d_box, double lambda, const double *d_vx, const double *d_vp,
+                                    double vl, uint64_t *d_hx, uint64_t *d_hp, uint64_t *d_hl, void *stream);
+int tm_potential_execute_hvp_host(tm_potential *p, int N, int P, const double *x, const double *params,
+                                  const double *box, double lambda, const double *vx, const double *vp, double vl,
+                                  double *hx, double *hp, double *hl);
 
 /* BoundPotential(potential, params), bound_potential.cu:6-12 */
 int tm_bound_potential_create(tm_potential *p, int ndim, const int32_t *shape, const double *params,

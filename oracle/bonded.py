@@ -89,3 +89,13 @@ def bonded_dl_dp(kind, x, params, lamb, idxs, mult=None, offset=None):
         return np.zeros_like(params)
     (gp,) = torch.autograd.grad(gl, (pt,), allow_unused=True)
     return np.zeros_like(params) if gp is None else gp.numpy()
+
+
+def bonded_energy_fn(kind, idxs, mult=None, offset=None):
+    """U(xt, pt, lt) of one bonded kind as a Torch function (params shaped
+    [n_terms, n_params]), for oracle.nonbonded.hvp()."""
+
+    def fn(xt, pt, lt):
+        return _ENERGY[kind](xt, pt, lt, idxs, mult, offset)
+
+    return fn

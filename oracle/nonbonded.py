@@ -239,6 +239,67 @@ def nonbonded_pairs_dl_dp(
     return _dl_dp(fn, x, params, lamb)
 
 
+def hvp(energy_fn, x, params, lamb, vx=None, vp=None, vl=0.0):
+    """H (vx, vp, vl) with H the Hessian of U in (x, params, lambda), by Torch
+    FP64 double backward: the gradient of <grad U, v> (the reference takes such
+    second derivatives from jax.grad of its potentials; the trajectory adjoint
+    of attic/docs/paper.tex:27 needs them). Returns (hx, hp, hl)."""
+    xt, pt, lt = _t(x, True), _t(params, True), _t(lamb, True)
+    u = energy_fn(xt, pt, lt)
+    gx, gp, gl = torch.autograd.grad(u, (xt, pt, lt), create_graph=True, allow_unused=True)
+    s = torch.zeros((), dtype=torch.float64)
+    if vx is not None and gx is not None:
+        s = s + (gx * _t(vx).reshape(gx.shape)).sum()
+    if vp is not None and gp is not None:
+        s = s + (gp * _t(vp).reshape(gp.shape)).sum()
+    if vl and gl is not None:
+        s = s + gl * float(vl)
+    if not s.requires_grad:
+        return np.zeros_like(xt.detach().numpy()), np.zeros_like(pt.detach().numpy()), 0.0
+    hx, hp, hl = torch.autograd.grad(s, (xt, pt, lt), allow_unused=True)
+    hx = np.zeros_like(xt.detach().numpy()) if hx is None else hx.numpy()
+    hp = np.zeros_like(pt.detach().numpy()) if hp is None else hp.numpy()
+    return hx, hp, 0.0 if hl is None else float(hl)
+
+
+def pair_energy_fn(x, box, exclusion_idxs, scales, beta, cutoff, plane, offset, pairs=None):
+    """U(xt, pt, lt) of nonbonded_pairs (all pairs within the cutoff of x, minus
+    the scaled excluded pairs) as a Torch function, for hvp()."""
+    x = np.asarray(x, dtype=np.float64)
+    if pairs is None:
+        pairs = candidate_pairs(x, box, cutoff)
+    box_diag = torch.tensor(np.diag(np.asarray(box)).copy(), dtype=torch.float64)
+    plane_t = torch.as_tensor(np.asarray(plane), dtype=torch.float64)
+    offset_t = torch.as_tensor(np.asarray(offset), dtype=torch.float64)
+    excl = np.asarray(exclusion_idxs, dtype=np.int64).reshape(-1, 2)
+    sc = torch.as_tensor(np.asarray(scales, dtype=np.float64).reshape(-1, 2))
+
+    def fn(xt, pt, lt):
+        u = _pair_energy(xt, pt, lt, pairs, box_diag, beta, cutoff, plane_t, offset_t)
+        if len(excl):
+            u = u - _pair_energy(xt, pt, lt, excl, box_diag, beta, cutoff, plane_t, offset_t, sc[:, 0], sc[:, 1])
+        return u
+
+    return fn
+
+
+def pair_list_energy_fn(pairs, scales, box, beta, cutoff, plane, offset, negated=False):
+    """U(xt, pt, lt) of a pair list with per-pair (charge, LJ) scales: the
+    scaled pair terms of the listed pairs within the cutoff, negated on request
+    (nonbonded.py:266-315 on the listed pairs)."""
+    pairs = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
+    box_diag = torch.tensor(np.diag(np.asarray(box)).copy(), dtype=torch.float64)
+    plane_t = torch.as_tensor(np.asarray(plane), dtype=torch.float64)
+    offset_t = torch.as_tensor(np.asarray(offset), dtype=torch.float64)
+    sc = torch.as_tensor(np.asarray(scales, dtype=np.float64).reshape(-1, 2))
+    sign = -1.0 if negated else 1.0
+
+    def fn(xt, pt, lt):
+        return sign * _pair_energy(xt, pt, lt, pairs, box_diag, beta, cutoff, plane_t, offset_t, sc[:, 0], sc[:, 1])
+
+    return fn
+
+
 def force_rel_err(test: np.ndarray, ref: np.ndarray) -> np.ndarray:
     """Per-atom |dF| / max(|F_ref|, 1) (tests/common.py:386-409, the OpenMM convention)."""
     num = np.linalg.norm(np.asarray(test) - np.asarray(ref), axis=-1)

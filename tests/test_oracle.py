@@ -457,3 +457,92 @@ def test_nblist_subset_reference_equals_brute_force_and_is_order_independent(rng
     assert set(np.concatenate(ref).tolist()) == set(np.concatenate(ref2).tolist())
     f32 = nblist.fp32_ixn_list_rows(x, box, 0.9, rows, 64)
     assert sum(len(set(a) ^ set(b)) for a, b in zip(ref, f32)) <= 2
+
+
+# ------------------------------------------------------------- second order
+
+
+def _grad_all(fn, x, p, lam):
+    xt, pt, lt = nonbonded._t(x, True), nonbonded._t(p, True), nonbonded._t(lam, True)
+    gx, gp, gl = torch.autograd.grad(fn(xt, pt, lt), (xt, pt, lt), allow_unused=True)
+    return gx.numpy(), gp.numpy(), 0.0 if gl is None else float(gl)
+
+
+def test_hvp_matches_central_differences_of_the_gradient(rng):
+    """hvp() (double backward) equals the central difference of the FP64
+    gradient along the tangent (vx, vp, vl), for every output: an independent
+    route that a wrong sign, dropped term or transposed operand fails."""
+    x, params, box, lam, excl, scales, plane, offset = _tiny_system(rng)
+    fn = nonbonded.pair_energy_fn(x, box, excl, scales, 2.0, 1.2, plane, offset, pairs=np.array(
+        [(i, j) for i in range(len(x)) for j in range(i + 1, len(x))]))
+    vx = rng.normal(size=x.shape)
+    vp = rng.normal(size=params.shape) * np.array([1.0, 0.01, 0.1])
+    # eps == 0 switches LJ off (gradients included, nonbonded.py:214-225): the
+    # energy is not smooth in eps there, so the tangent leaves those eps at 0
+    vp[params[:, 2] == 0.0, 2] = 0.0
+    vl = 0.7
+    hx, hp, hl = nonbonded.hvp(fn, x, params, lam, vx, vp, vl)
+    h = 1e-6
+    gp_ = _grad_all(fn, x + h * vx, params + h * vp, lam + h * vl)
+    gm_ = _grad_all(fn, x - h * vx, params - h * vp, lam - h * vl)
+    for a, plus, minus in zip((hx, hp, hl), gp_, gm_):
+        fd = (np.asarray(plus) - np.asarray(minus)) / (2 * h)
+        np.testing.assert_allclose(a, fd, rtol=1e-5, atol=1e-5 * max(np.abs(fd).max(), 1.0))
+
+
+def test_hvp_is_symmetric(rng):
+    """u . H v == v . H u over (x, params, lambda): H is a Hessian."""
+    x, params, box, lam, excl, scales, plane, offset = _tiny_system(rng)
+    fn = nonbonded.pair_energy_fn(x, box, excl, scales, 2.0, 1.2, plane, offset)
+    u = (rng.normal(size=x.shape), rng.normal(size=params.shape), 0.3)
+    v = (rng.normal(size=x.shape), rng.normal(size=params.shape), -1.1)
+    hu = nonbonded.hvp(fn, x, params, lam, *u)
+    hv = nonbonded.hvp(fn, x, params, lam, *v)
+    a = sum(float(np.sum(p * q)) for p, q in zip(v, hu))
+    b = sum(float(np.sum(p * q)) for p, q in zip(u, hv))
+    assert abs(a - b) <= 1e-10 * max(abs(a), 1.0)
+
+
+def test_bond_hessian_closed_form(rng):
+    """U = kb/2 (r - b0)^2: H_ii = kb (n n^T + (1 - b0 / r) (I - n n^T)),
+    H_ij = -H_ii; d(du/dx_i)/dkb = (r - b0) n, d(du/dx_i)/db0 = -kb n."""
+    x = rng.normal(size=(2, 3))
+    kb, b0 = 350.0, 0.11
+    d = x[0] - x[1]
+    r = np.linalg.norm(d)
+    n = d / r
+    Hii = kb * (np.outer(n, n) + (1 - b0 / r) * (np.eye(3) - np.outer(n, n)))
+    vx = rng.normal(size=(2, 3))
+    vp = np.array([[0.3, -0.2]])
+    fn = bonded.bonded_energy_fn("bond", np.array([[0, 1]]))
+    hx, hp, hl = nonbonded.hvp(fn, x, np.array([[kb, b0]]), 0.0, vx, vp, 0.0)
+    want0 = Hii @ (vx[0] - vx[1]) + vp[0, 0] * (r - b0) * n + vp[0, 1] * (-kb * n)
+    np.testing.assert_allclose(hx[0], want0, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(hx[1], -want0, rtol=1e-12, atol=1e-12)
+    # du/dkb = (r - b0)^2 / 2, du/db0 = -kb (r - b0): their tangents along (vx, vp)
+    dr = n @ (vx[0] - vx[1])
+    want_p = [(r - b0) * dr - (r - b0) * vp[0, 1], -vp[0, 0] * (r - b0) - kb * (dr - vp[0, 1])]
+    np.testing.assert_allclose(hp[0], want_p, rtol=1e-12, atol=1e-12)
+    assert hl == 0.0
+
+
+@pytest.mark.parametrize("kind,arity,params", [
+    ("angle", 3, [[75.0, 1.91]]),
+    ("torsion", 4, [[2.5, 0.4, 3.0]]),
+])
+def test_bonded_hvp_central_differences(rng, kind, arity, params):
+    """Angle and torsion hvp() (with a lambda-dependent prefactor) against
+    central differences of the FP64 gradient."""
+    x = rng.normal(size=(arity, 3))
+    idxs = np.arange(arity)[None, :]
+    fn = bonded.bonded_energy_fn(kind, idxs, mult=[2], offset=[1])
+    params = np.asarray(params, dtype=np.float64)
+    vx = rng.normal(size=x.shape)
+    vp = rng.normal(size=params.shape) * 0.1
+    hx, hp, hl = nonbonded.hvp(fn, x, params, 0.4, vx, vp, 0.5)
+    h = 1e-6
+    gp_ = _grad_all(fn, x + h * vx, params + h * vp, 0.4 + h * 0.5)
+    gm_ = _grad_all(fn, x - h * vx, params - h * vp, 0.4 - h * 0.5)
+    for a, plus, minus in zip((hx, hp, hl), gp_, gm_):
+        fd = (np.asarray(plus) - np.asarray(minus)) / (2 * h)
+        np.testing.assert_allclose(a, fd, rtol=1e-5, atol=1e-6 * max(np.abs(fd).max(), 1.0))

@@ -70,6 +70,15 @@ _SIGS = {
         [_P, c_int, c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p],
     ),
     "tm_potential_execute_dl_dp_host": (c_int, [_P, c_int, c_int, _PD, _PD, _PD, c_double, _PU64]),
+    "tm_potential_execute_hvp_device": (
+        c_int,
+        [_P, c_int, c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_double, c_void_p, c_void_p,
+         c_void_p, c_void_p],
+    ),
+    "tm_potential_execute_hvp_host": (
+        c_int,
+        [_P, c_int, c_int, _PD, _PD, _PD, c_double, _PD, _PD, c_double, _PD, _PD, _PD],
+    ),
     "tm_bound_potential_create": (c_int, [_P, c_int, _PI, _PD, _PPV]),
     "tm_bound_potential_release": (None, [_P]),
     "tm_bound_potential_size": (c_int, [_P, POINTER(c_int)]),

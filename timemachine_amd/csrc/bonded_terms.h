@@ -19,9 +19,16 @@
 
 #include <cmath>
 
+#include "dual.h"
 #include "hd.h"
 
 namespace tmamd {
+
+// x_a - x_b as T. FP64 positions subtract in FP64 and convert (the kernels'
+// expression); positions that carry a tangent (Dual) subtract as duals, which
+// gives the terms' directional derivatives (the Hessian-vector products).
+template <typename T> TM_HD inline T pos_diff(double a, double b) { return static_cast<T>(a - b); }
+template <typename T> TM_HD inline T pos_diff(const Dual &a, const Dual &b) { return a - b; }
 
 template <typename T> struct BondOut {
     T g[2][3]; // du/dx of atoms i, j
@@ -48,16 +55,17 @@ template <typename T> TM_HD inline T lam_prefactor(int mult, int offset, double 
     return static_cast<T>(offset + mult * lambda);
 }
 
-template <typename T>
-TM_HD inline BondOut<T> bond_term(const double *xi, const double *xj, T kb, T b0, int mult, T pf) {
+template <typename T, typename P = double>
+TM_HD inline BondOut<T> bond_term(const P *xi, const P *xj, T kb, T b0, int mult, T pf) {
+    using std::sqrt;
     BondOut<T> o;
     T dx[3];
     T d2 = 0;
     for (int d = 0; d < 3; d++) {
-        dx[d] = static_cast<T>(xi[d] - xj[d]);
+        dx[d] = pos_diff<T>(xi[d], xj[d]);
         d2 = d2 + dx[d] * dx[d];
     }
-    const T dij = std::sqrt(d2);
+    const T dij = sqrt(d2);
     const T db = dij - b0;
     const T half_db2 = T(0.5) * db * db;
     for (int d = 0; d < 3; d++) {
@@ -72,23 +80,26 @@ TM_HD inline BondOut<T> bond_term(const double *xi, const double *xj, T kb, T b0
     return o;
 }
 
-template <typename T>
-TM_HD inline AngleOut<T> angle_term(const double *xi, const double *xj, const double *xk, T ka, T a0, int mult, T pf) {
+template <typename T, typename P = double>
+TM_HD inline AngleOut<T> angle_term(const P *xi, const P *xj, const P *xk, T ka, T a0, int mult, T pf) {
+    using std::cos;
+    using std::sin;
+    using std::sqrt;
     AngleOut<T> o;
     T a[3], b[3];
     T aa = 0, bb = 0, ab = 0;
     for (int d = 0; d < 3; d++) {
-        a[d] = static_cast<T>(xi[d] - xj[d]);
-        b[d] = static_cast<T>(xk[d] - xj[d]);
+        a[d] = pos_diff<T>(xi[d], xj[d]);
+        b[d] = pos_diff<T>(xk[d], xj[d]);
         aa = aa + a[d] * a[d];
         bb = bb + b[d] * b[d];
         ab = ab + a[d] * b[d];
     }
-    const T na = std::sqrt(aa);
-    const T nb = std::sqrt(bb);
+    const T na = sqrt(aa);
+    const T nb = sqrt(bb);
     const T nab = na * nb;
     const T cos_t = ab / nab;
-    const T delta = cos_t - std::cos(a0);
+    const T delta = cos_t - cos(a0);
     const T dudc = pf * ka * delta;
     for (int d = 0; d < 3; d++) {
         const T dca = b[d] / nab - cos_t * (a[d] / aa);
@@ -99,7 +110,7 @@ TM_HD inline AngleOut<T> angle_term(const double *xi, const double *xj, const do
     }
     const T half_d2 = T(0.5) * delta * delta;
     o.dka = half_d2 * pf;
-    o.da0 = pf * ka * delta * std::sin(a0);
+    o.da0 = pf * ka * delta * sin(a0);
     o.u = ka * half_d2 * pf;
     o.dl = static_cast<T>(mult) * (ka * half_d2);
     return o;
@@ -113,15 +124,19 @@ template <typename T> TM_HD inline void cross3(const T *a, const T *b, T *c) {
 
 template <typename T> TM_HD inline T dot3(const T *a, const T *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
-template <typename T>
-TM_HD inline TorsionOut<T> torsion_term(const double *xi, const double *xj, const double *xk, const double *xl, T kt,
-                                        T phase, T period, int mult, T pf) {
+template <typename T, typename P = double>
+TM_HD inline TorsionOut<T> torsion_term(const P *xi, const P *xj, const P *xk, const P *xl, T kt, T phase, T period,
+                                        int mult, T pf) {
+    using std::atan2;
+    using std::cos;
+    using std::sin;
+    using std::sqrt;
     TorsionOut<T> o;
     T rij[3], rkj[3], rkl[3];
     for (int d = 0; d < 3; d++) {
-        rij[d] = static_cast<T>(xj[d] - xi[d]);
-        rkj[d] = static_cast<T>(xj[d] - xk[d]);
-        rkl[d] = static_cast<T>(xl[d] - xk[d]);
+        rij[d] = pos_diff<T>(xj[d], xi[d]);
+        rkj[d] = pos_diff<T>(xj[d], xk[d]);
+        rkl[d] = pos_diff<T>(xl[d], xk[d]);
     }
     T n1[3], n2[3], n3[3];
     cross3(rij, rkj, n1);
@@ -130,10 +145,10 @@ TM_HD inline TorsionOut<T> torsion_term(const double *xi, const double *xj, cons
     const T n1n1 = dot3(n1, n1);
     const T n2n2 = dot3(n2, n2);
     const T rkj2 = dot3(rkj, rkj);
-    const T rkj_n = std::sqrt(rkj2);
+    const T rkj_n = sqrt(rkj2);
     const T y = dot3(n3, rkj) / rkj_n;
     const T x = dot3(n1, n2);
-    const T phi = std::atan2(y, x);
+    const T phi = atan2(y, x);
     // dphi/dx for the four atoms (Blondel & Karplus 1996, J. Comput. Chem. 17, 1132)
     const T fij = dot3(rij, rkj) / rkj2;
     const T fkl = dot3(rkl, rkj) / rkj2;
@@ -145,8 +160,8 @@ TM_HD inline TorsionOut<T> torsion_term(const double *xi, const double *xj, cons
         dphi[2][d] = (fkl - T(1)) * dphi[3][d] - fij * dphi[0][d];
     }
     const T arg = period * phi - phase;
-    const T s = std::sin(arg);
-    const T c = std::cos(arg);
+    const T s = sin(arg);
+    const T c = cos(arg);
     const T dudphi = -pf * kt * period * s;
     for (int a = 0; a < 4; a++) {
         for (int d = 0; d < 3; d++) {

@@ -353,6 +353,21 @@ int tm_potential_execute_dl_dp_host(tm_potential *p, int N, int P, const double 
     });
 }
 
+int tm_potential_execute_hvp_device(tm_potential *p, int N, int P, const double *d_x, const double *d_p,
+                                    const double *d_box, double lambda, const double *d_vx, const double *d_vp,
+                                    double vl, uint64_t *d_hx, uint64_t *d_hp, uint64_t *d_hl, void *stream) {
+    return guarded([&] {
+        p->p->execute_hvp_device(N, P, d_x, d_p, d_box, lambda, d_vx, d_vp, vl, d_hx, d_hp, d_hl,
+                                 static_cast<hipStream_t>(stream));
+    });
+}
+
+int tm_potential_execute_hvp_host(tm_potential *p, int N, int P, const double *x, const double *params,
+                                  const double *box, double lambda, const double *vx, const double *vp, double vl,
+                                  double *hx, double *hp, double *hl) {
+    return guarded([&] { p->p->execute_hvp_host(N, P, x, params, box, lambda, vx, vp, vl, hx, hp, hl); });
+}
+
 int tm_potential_du_dp_fixed_to_float(tm_potential *p, int N, int P, const uint64_t *du_dp, double *out) {
     return guarded([&] { p->p->du_dp_fixed_to_float(N, P, du_dp, out); });
 }

@@ -1,0 +1,443 @@
+// Directional derivatives of the gradients (Hessian-vector products) of the
+// bonded and nonbonded potentials. With tangents vx on the coordinates, vp on
+// the parameters and vl on lambda, the tangents of du/dx, du/dp and du/dlambda
+// are the rows of H (vx, vp, vl), H the Hessian of U in (x, p, lambda)
+// (dual.h). By the symmetry of H they are also the vector-Jacobian product of
+// the gradient with (vx, vp, vl): the adjoint of du/dx that reverse-mode
+// differentiation through forces and MD trajectories needs, and the
+// double-backward of the torch.autograd.Function over the potentials.
+//
+// The reference has no second-order custom ops; its design notes describe
+// differentiating through Langevin dynamics with Hessians and mixed second
+// derivatives of the energy (attic/docs/paper.tex:27). The energies here are
+// the ones of the force kernels: the bonded terms of bonded_terms.h evaluated
+// in dual arithmetic, and the nonbonded pair energy of nonbonded.py:133-264
+// (erfc-damped Coulomb plus LJ 12-6 on the 4D distance, parameters at their
+// FP32 values as the force kernels see them) with exact derivatives in FP64.
+// Tangents accumulate in 64-bit fixed point with the exponents of du_dx
+// (2^36), du_dp (bonded 2^36; nonbonded 2^36 / 2^37 / 2^38 for q / sigma /
+// epsilon) and du_dl (2^36), so they decode like the gradients and do not
+// depend on the order of the terms.
+#include <algorithm>
+
+#include "bonded.h"
+#include "bonded_terms.h"
+#include "dual.h"
+#include "fixed_point.h"
+#include "hvp.h"
+#include "nb_pair.h"
+#include "nb_state.h"
+#include "wave.h"
+
+namespace tmamd {
+
+namespace {
+
+__device__ __forceinline__ void atomic_add_fixed(uint64_t *p, uint64_t f) {
+    if (f != 0) {
+        atomicAdd(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(f));
+    }
+}
+
+__device__ __forceinline__ uint64_t fixed_signed(double v, double scale, bool negate) {
+    const uint64_t f = f64_to_fixed(v, scale);
+    return negate ? 0ull - f : f;
+}
+
+// hl += the wave's sum of v (every lane must reach it)
+__device__ __forceinline__ void wave_add_fixed(uint64_t *hl, uint64_t v) {
+    const uint64_t tot = wave_sum_u64(v);
+    if (lane_id() == 0) {
+        atomic_add_fixed(hl, tot);
+    }
+}
+
+// ------------------------------------------------------------------ bonded
+
+template <int KIND>
+__global__ __launch_bounds__(128) void k_bonded_hvp(const BondedTerms t, const double *__restrict__ x,
+                                                    const double *__restrict__ vx, const double *__restrict__ vp,
+                                                    double lambda, double vl, uint64_t *__restrict__ hx,
+                                                    uint64_t *__restrict__ hp, uint64_t *__restrict__ hl) {
+    constexpr int AR = KIND == 0 ? 2 : (KIND == 1 ? 3 : 4);
+    constexpr int NP = KIND == 2 ? 3 : 2;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t fl = 0;
+    if (q < t.n) {
+        Dual pos[AR][3];
+        int at[AR];
+        for (int a = 0; a < AR; a++) {
+            at[a] = t.idx[q * AR + a];
+            for (int d = 0; d < 3; d++) {
+                pos[a][d] = Dual(x[at[a] * 3 + d], vx ? vx[at[a] * 3 + d] : 0.0);
+            }
+        }
+        Dual prm[NP];
+        for (int c = 0; c < NP; c++) {
+            prm[c] = Dual(t.params[q * NP + c], vp ? vp[q * NP + c] : 0.0);
+        }
+        const int mult = t.mult ? t.mult[q] : 0;
+        const int off = t.offset ? t.offset[q] : 1;
+        // prefactor offset + mult lambda (the force kernels' value), tangent mult vl
+        const Dual pf(lam_prefactor<double>(mult, off, lambda), mult * vl);
+        Dual g[AR][3];
+        Dual dp[NP];
+        Dual dl;
+        if constexpr (KIND == 0) {
+            const BondOut<Dual> o = bond_term<Dual, Dual>(pos[0], pos[1], prm[0], prm[1], mult, pf);
+            for (int a = 0; a < AR; a++) {
+                for (int d = 0; d < 3; d++) {
+                    g[a][d] = o.g[a][d];
+                }
+            }
+            dp[0] = o.dkb;
+            dp[1] = o.db0;
+            dl = o.dl;
+        } else if constexpr (KIND == 1) {
+            const AngleOut<Dual> o = angle_term<Dual, Dual>(pos[0], pos[1], pos[2], prm[0], prm[1], mult, pf);
+            for (int a = 0; a < AR; a++) {
+                for (int d = 0; d < 3; d++) {
+                    g[a][d] = o.g[a][d];
+                }
+            }
+            dp[0] = o.dka;
+            dp[1] = o.da0;
+            dl = o.dl;
+        } else {
+            const TorsionOut<Dual> o =
+                torsion_term<Dual, Dual>(pos[0], pos[1], pos[2], pos[3], prm[0], prm[1], prm[2], mult, pf);
+            for (int a = 0; a < AR; a++) {
+                for (int d = 0; d < 3; d++) {
+                    g[a][d] = o.g[a][d];
+                }
+            }
+            dp[0] = o.dk;
+            dp[1] = o.dphase;
+            dp[2] = o.dperiod;
+            dl = o.dl;
+        }
+        if (hx) {
+            for (int a = 0; a < AR; a++) {
+                for (int d = 0; d < 3; d++) {
+                    atomic_add_fixed(hx + at[a] * 3 + d, f64_to_fixed(g[a][d].d, FIXED_SCALE));
+                }
+            }
+        }
+        if (hp) {
+            for (int c = 0; c < NP; c++) {
+                atomic_add_fixed(hp + q * NP + c, f64_to_fixed(dp[c].d, FIXED_SCALE));
+            }
+        }
+        fl = f64_to_fixed(dl.d, FIXED_SCALE);
+    }
+    if (hl) {
+        wave_add_fixed(hl, fl);
+    }
+}
+
+// --------------------------------------------------------------- nonbonded
+
+// Gradient of one pair's energy in dual arithmetic. dv = 3D minimum-image
+// displacement x_i - x_j, dw = w_i - w_j (w = (plane + offset lambda) cutoff),
+// dwdl = d(dw)/dlambda. du/dx_i = pre dv, du/dx_j = -pre dv; the charge,
+// sigma (one value for both atoms: sig_ij = sig_i + sig_j) and epsilon
+// derivatives of each atom; du/dlambda = pre dw dwdl.
+struct PairGradDual {
+    Dual pre;
+    Dual dq_i, dq_j, dsig, deps_i, deps_j;
+    Dual dl;
+    bool lj;
+};
+
+__device__ __forceinline__ PairGradDual pair_grad_dual(const Dual *dv, const Dual &dw, double dwdl, const Dual &qi,
+                                                       const Dual &qj, const Dual &si, const Dual &sj, const Dual &ei,
+                                                       const Dual &ej, double beta, double sq, double slj) {
+    PairGradDual o;
+    const Dual d2 = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2] + dw * dw;
+    const Dual r = sqrt(d2);
+    const Dual inv_r = Dual(1.0) / r;
+    const Dual bx = Dual(beta) * r;
+    const Dual ec = erfc(bx);
+    const Dual ex = exp(-(bx * bx));
+    // u_es = sq q_i q_j erfc(beta r) / r;  (du/dr) / r = -sq q_i q_j (erfc / r + 2 beta / sqrt(pi) e^{-b^2 r^2}) / r^2
+    const Dual qij = Dual(sq) * qi * qj;
+    o.pre = -(qij * (ec * inv_r + Dual(TWO_OVER_SQRT_PI * beta) * ex) * inv_r * inv_r);
+    const Dual es_unit = Dual(sq) * ec * inv_r;
+    o.dq_i = qj * es_unit;
+    o.dq_j = qi * es_unit;
+    o.lj = ei.v != 0.0 && ej.v != 0.0;
+    if (o.lj) {
+        const Dual sig_ij = si + sj;
+        const Dual s1 = sig_ij * inv_r;
+        const Dual s2 = s1 * s1;
+        const Dual s6 = s2 * s2 * s2;
+        const Dual s12 = s6 * s6;
+        const Dual se = Dual(slj) * ei * ej;
+        // u_lj = 4 se (s12 - s6); (du/dr) / r = 4 se (6 s6 - 12 s12) / r^2
+        o.pre = o.pre + Dual(4.0) * se * (Dual(6.0) * s6 - Dual(12.0) * s12) * inv_r * inv_r;
+        o.dsig = Dual(4.0) * se * (Dual(12.0) * s12 - Dual(6.0) * s6) / sig_ij;
+        const Dual lj_shape = Dual(4.0 * slj) * (s12 - s6);
+        o.deps_i = lj_shape * ej;
+        o.deps_j = lj_shape * ei;
+    }
+    o.dl = o.pre * dw * Dual(dwdl);
+    return o;
+}
+
+// parameters as the force kernels see them (their FP32 values), tangents in FP64
+__device__ __forceinline__ Dual dual_param(const double *p, const double *vp, int atom, int c) {
+    return Dual(static_cast<double>(static_cast<float>(p[atom * 3 + c])), vp ? vp[atom * 3 + c] : 0.0);
+}
+
+// Fixed-point tangents of one pair's gradient: du_dx (3) and du_dp (3) of
+// atoms i and j, and du_dl. Zero for pairs outside the cutoff.
+struct PairTangents {
+    uint64_t xi[3], xj[3], pi[3], pj[3], l;
+};
+
+__device__ __forceinline__ PairTangents zero_tangents() {
+    PairTangents t;
+    for (int d = 0; d < 3; d++) {
+        t.xi[d] = t.xj[d] = t.pi[d] = t.pj[d] = 0;
+    }
+    t.l = 0;
+    return t;
+}
+
+__device__ __forceinline__ PairTangents pair_tangents(const PairGradDual &o, const Dual *dv, bool neg) {
+    PairTangents t = zero_tangents();
+    for (int d = 0; d < 3; d++) {
+        const Dual f = o.pre * dv[d];
+        t.xi[d] = fixed_signed(f.d, FIXED_SCALE, neg);
+        t.xj[d] = 0ull - t.xi[d];
+    }
+    t.pi[0] = fixed_signed(o.dq_i.d, FIXED_SCALE, neg);
+    t.pj[0] = fixed_signed(o.dq_j.d, FIXED_SCALE, neg);
+    if (o.lj) {
+        t.pi[1] = t.pj[1] = fixed_signed(o.dsig.d, FIXED_SCALE_DSIG, neg);
+        t.pi[2] = fixed_signed(o.deps_i.d, FIXED_SCALE_DEPS, neg);
+        t.pj[2] = fixed_signed(o.deps_j.d, FIXED_SCALE_DEPS, neg);
+    }
+    t.l = fixed_signed(o.dl.d, FIXED_SCALE, neg);
+    return t;
+}
+
+// One thread per pair of the list (one system), as k_pair_list.
+__global__ __launch_bounds__(256) void k_pair_list_hvp(const PairListArgs a, const double *__restrict__ vx,
+                                                       const double *__restrict__ vp, double vl,
+                                                       uint64_t *__restrict__ hx, uint64_t *__restrict__ hp,
+                                                       uint64_t *__restrict__ hl) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    PairTangents t = zero_tangents();
+    int gi = 0, gj = 0;
+    bool in = false;
+    if (p < a.M) {
+        gi = a.pairs[p * 2 + 0];
+        gj = a.pairs[p * 2 + 1];
+        const double lam = a.lambda0;
+        const double cutoff = a.cutoff0;
+        const int pli = a.plane ? a.plane[gi] : 0, plj = a.plane ? a.plane[gj] : 0;
+        const int ofi = a.offset ? a.offset[gi] : 0, ofj = a.offset ? a.offset[gj] : 0;
+        const double dwdl = (ofi - ofj) * cutoff;
+        const Dual dw((pli + ofi * lam) * cutoff - (plj + ofj * lam) * cutoff, dwdl * vl);
+        const double L[3] = {a.boxes[0], a.boxes[4], a.boxes[8]};
+        Dual dv[3];
+        for (int d = 0; d < 3; d++) {
+            // the image shift is a constant: the tangent is vx_i - vx_j
+            const double tan = vx ? vx[gi * 3 + d] - vx[gj * 3 + d] : 0.0;
+            dv[d] = Dual(min_image_d(a.x[gi * 3 + d] - a.x[gj * 3 + d], L[d], 1.0 / L[d]), tan);
+        }
+        const double d2 = __builtin_fma(dw.v, dw.v, pair_d2(dv[0].v, dv[1].v, dv[2].v));
+        in = d2 < cutoff * cutoff;
+        if (in) {
+            const PairGradDual o = pair_grad_dual(
+                dv, dw, dwdl, dual_param(a.params, vp, gi, 0), dual_param(a.params, vp, gj, 0),
+                dual_param(a.params, vp, gi, 1), dual_param(a.params, vp, gj, 1), dual_param(a.params, vp, gi, 2),
+                dual_param(a.params, vp, gj, 2), static_cast<double>(a.beta0), a.scales[p * 2 + 0],
+                a.scales[p * 2 + 1]);
+            t = pair_tangents(o, dv, a.negated != 0);
+        }
+    }
+    if (in) {
+        for (int c = 0; c < 3; c++) {
+            if (hx) {
+                atomic_add_fixed(hx + gi * 3 + c, t.xi[c]);
+                atomic_add_fixed(hx + gj * 3 + c, t.xj[c]);
+            }
+            if (hp) {
+                atomic_add_fixed(hp + gi * 3 + c, t.pi[c]);
+                atomic_add_fixed(hp + gj * 3 + c, t.pj[c]);
+            }
+        }
+    }
+    if (hl) {
+        wave_add_fixed(hl, t.l);
+    }
+}
+
+// All pairs of one system from the tile list of the last neighborlist build:
+// one wave per tile, lane l holding row slot 64 r + l, the tile's 64 column
+// atoms visited in turn (the tiles, pair test and cutoff of the force kernel).
+// Row terms stay in registers until the tile ends; column terms are summed
+// over the wave (exact fixed-point sums) before one atomic per value.
+__global__ __launch_bounds__(64) void k_tiles_hvp(
+    const NbSys *__restrict__ sys, const int *__restrict__ tile_prefix, const int *__restrict__ tile_row,
+    const uint32_t *__restrict__ tile_atoms, const uint32_t *__restrict__ perm, const double4 *__restrict__ pos,
+    const double *__restrict__ params, const int *__restrict__ offset, const double *__restrict__ boxes,
+    const int *__restrict__ status, const double *__restrict__ vx, const double *__restrict__ vp, double vl,
+    uint64_t *__restrict__ hx, uint64_t *__restrict__ hp, uint64_t *__restrict__ hl) {
+    if (status[0] & (NB_STATUS_BOX_TOO_SMALL | NB_STATUS_BOX_NOT_ORTHO)) {
+        return;
+    }
+    const NbSys S = sys[0];
+    const int total = tile_prefix[1];
+    const double L[3] = {boxes[0], boxes[4], boxes[8]};
+    const double beta = static_cast<double>(S.beta);
+    const double cut2 = S.cutoff_d * S.cutoff_d;
+    const int lane = threadIdx.x;
+    uint64_t acc_l = 0;
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const int kt = S.tile0 + t;
+        const int r = tile_row[kt];
+        const uint32_t i = static_cast<uint32_t>(r * TILE + lane);
+        const bool vi = i < static_cast<uint32_t>(S.row_end);
+        double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
+        int gi = 0, ofi = 0;
+        Dual qi, si, ei;
+        double vxi[3] = {0.0, 0.0, 0.0};
+        if (vi) {
+            pi = pos[S.slot0 + i];
+            gi = S.atom0 + static_cast<int>(perm[S.slot0 + i]);
+            ofi = offset ? offset[gi] : 0;
+            qi = dual_param(params, vp, gi, 0);
+            si = dual_param(params, vp, gi, 1);
+            ei = dual_param(params, vp, gi, 2);
+            if (vx) {
+                for (int d = 0; d < 3; d++) {
+                    vxi[d] = vx[gi * 3 + d];
+                }
+            }
+        }
+        uint64_t rx[3] = {0, 0, 0}, rp[3] = {0, 0, 0};
+        const uint32_t jl = tile_atoms[static_cast<size_t>(kt) * TILE + lane];
+        for (int c = 0; c < TILE; c++) {
+            const uint32_t j = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(jl), c));
+            if (j >= static_cast<uint32_t>(S.slot_end)) {
+                continue; // padding column (uniform over the wave)
+            }
+            const double4 pj = pos[S.slot0 + j];
+            const int gj = S.atom0 + static_cast<int>(perm[S.slot0 + j]);
+            Dual dv[3];
+            const double pd[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
+            for (int d = 0; d < 3; d++) {
+                const double tan = vx ? vxi[d] - vx[gj * 3 + d] : 0.0;
+                dv[d] = Dual(min_image_d(pd[d], L[d], 1.0 / L[d]), tan);
+            }
+            const int ofj = offset ? offset[gj] : 0;
+            const double dwdl = (ofi - ofj) * S.cutoff_d;
+            const Dual dw(pi.w - pj.w, dwdl * vl);
+            const double d2 = __builtin_fma(dw.v, dw.v, pair_d2(dv[0].v, dv[1].v, dv[2].v));
+            const bool in = vi && i < j && d2 < cut2;
+            if (ballot(in) == 0) {
+                continue; // uniform
+            }
+            PairTangents tg = zero_tangents();
+            if (in) {
+                const PairGradDual o =
+                    pair_grad_dual(dv, dw, dwdl, qi, dual_param(params, vp, gj, 0), si, dual_param(params, vp, gj, 1),
+                                   ei, dual_param(params, vp, gj, 2), beta, 1.0, 1.0);
+                tg = pair_tangents(o, dv, false);
+            }
+            for (int d = 0; d < 3; d++) {
+                rx[d] += tg.xi[d];
+                rp[d] += tg.pi[d];
+            }
+            acc_l += tg.l;
+            // column atom j: one atomic per value for the whole wave
+            uint64_t cs[6];
+            for (int d = 0; d < 3; d++) {
+                cs[d] = wave_sum_u64(tg.xj[d]);
+                cs[3 + d] = wave_sum_u64(tg.pj[d]);
+            }
+            if (lane == 0) {
+                for (int d = 0; d < 3; d++) {
+                    if (hx) {
+                        atomic_add_fixed(hx + gj * 3 + d, cs[d]);
+                    }
+                    if (hp) {
+                        atomic_add_fixed(hp + gj * 3 + d, cs[3 + d]);
+                    }
+                }
+            }
+        }
+        if (vi) {
+            for (int d = 0; d < 3; d++) {
+                if (hx) {
+                    atomic_add_fixed(hx + gi * 3 + d, rx[d]);
+                }
+                if (hp) {
+                    atomic_add_fixed(hp + gi * 3 + d, rp[d]);
+                }
+            }
+        }
+    }
+    if (hl) {
+        wave_add_fixed(hl, acc_l);
+    }
+}
+
+} // namespace
+
+void launch_bonded_hvp(BondedKind kind, const BondedTerms &t, const double *x, const double *vx, const double *vp,
+                       double lambda, double vl, uint64_t *hx, uint64_t *hp, uint64_t *hl, hipStream_t stream) {
+    if (t.n == 0 || (!hx && !hp && !hl)) {
+        return;
+    }
+    if (t.term_sys) {
+        throw Error("Hessian-vector products: one system only");
+    }
+    const int grid = ceil_div(t.n, 128);
+    switch (kind) {
+    case BondedKind::Bond:
+        k_bonded_hvp<0><<<grid, 128, 0, stream>>>(t, x, vx, vp, lambda, vl, hx, hp, hl);
+        break;
+    case BondedKind::Angle:
+        k_bonded_hvp<1><<<grid, 128, 0, stream>>>(t, x, vx, vp, lambda, vl, hx, hp, hl);
+        break;
+    case BondedKind::Torsion:
+        k_bonded_hvp<2><<<grid, 128, 0, stream>>>(t, x, vx, vp, lambda, vl, hx, hp, hl);
+        break;
+    }
+    TM_LAUNCH_CHECK();
+}
+
+void launch_pair_list_hvp(const PairListArgs &a, const double *vx, const double *vp, double vl, uint64_t *hx,
+                          uint64_t *hp, uint64_t *hl, hipStream_t stream) {
+    if (a.M == 0 || (!hx && !hp && !hl)) {
+        return;
+    }
+    if (a.params1 || a.pair_sys || a.lambdas || a.cutoffs || a.betas || a.slots) {
+        throw Error("Hessian-vector products: one system with plain parameters only");
+    }
+    k_pair_list_hvp<<<ceil_div(a.M, 256), 256, 0, stream>>>(a, vx, vp, vl, hx, hp, hl);
+    TM_LAUNCH_CHECK();
+}
+
+void launch_tiles_hvp(const NbState &st, const double *params, const int *offset, const double *boxes,
+                      const double *vx, const double *vp, double vl, uint64_t *hx, uint64_t *hp, uint64_t *hl,
+                      hipStream_t stream) {
+    if (!hx && !hp && !hl) {
+        return;
+    }
+    if (st.S != 1 || st.any_interp) {
+        throw Error("Hessian-vector products: one system with plain parameters only");
+    }
+    // one wave per tile, the grid striding over the tile count read on the device
+    const int grid = std::max(1, std::min(st.total_tiles, 8192));
+    k_tiles_hvp<<<grid, TILE, 0, stream>>>(st.sys.ptr, st.tile_prefix.ptr, st.tile_row.ptr, st.tile_atoms.ptr,
+                                            st.perm.ptr, st.pos.ptr, params, offset, boxes, st.status.ptr, vx, vp, vl,
+                                            hx, hp, hl);
+    TM_LAUNCH_CHECK();
+}
+
+} // namespace tmamd

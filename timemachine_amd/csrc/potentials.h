@@ -55,6 +55,29 @@ class Potential {
     void execute_dl_dp_host(int N, int P, const double *h_x, const double *h_p, const double *h_box, double lambda,
                             uint64_t *h_dl_dp);
 
+    // Hessian-vector product: the directional derivative of (du/dx, du/dp,
+    // du/dlambda) along (d_vx [N, 3], d_vp [P], vl), accumulated into
+    // caller-zeroed d_hx [N, 3] (2^36), d_hp [P] (du_dp exponents, decoded by
+    // du_dp_fixed_to_float) and d_hl [1] (2^36). Each accumulated term must
+    // stay below 2^63 / 2^36 ~ 1.3e8 in magnitude: scale the tangent (the
+    // product is linear in it) so that it is at most ~1. By the symmetry of the
+    // Hessian this is also the vector-Jacobian product of the gradient with
+    // (vx, vp, vl), the adjoint of du/dx that reverse-mode differentiation
+    // through forces and trajectories needs. d_vx / d_vp null: zero tangent;
+    // a null output is not requested. No reference counterpart (the reference
+    // takes second derivatives from its JAX potentials). Throws for potentials
+    // without it (interpolated parameters).
+    virtual void execute_hvp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box,
+                                    double lambda, const double *d_vx, const double *d_vp, double vl, uint64_t *d_hx,
+                                    uint64_t *d_hp, uint64_t *d_hl, hipStream_t stream);
+    // Host variant: decoded FP64 outputs. The tangent is scaled by a power of
+    // two to at most 1 in magnitude before the device call (the product is
+    // linear in it) and the outputs scaled back, which keeps the fixed-point
+    // accumulators in range for tangents of any size.
+    void execute_hvp_host(int N, int P, const double *h_x, const double *h_p, const double *h_box, double lambda,
+                          const double *h_vx, const double *h_vp, double vl, double *h_hx, double *h_hp,
+                          double *h_hl);
+
     // Host-side validation possible before launching with host inputs.
     virtual void validate_host(int N, int P, const double *h_box) const {}
     // Reads device status flags (synchronises the stream) and raises the
@@ -105,6 +128,9 @@ class NonbondedAllPairs : public Potential {
                         hipStream_t stream) override;
     void execute_dl_dp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
                               uint64_t *d_dl_dp, hipStream_t stream) override;
+    void execute_hvp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
+                            const double *d_vx, const double *d_vp, double vl, uint64_t *d_hx, uint64_t *d_hp,
+                            uint64_t *d_hl, hipStream_t stream) override;
     void validate_host(int N, int P, const double *h_box) const override;
     void check_device_status(hipStream_t stream) override;
     void du_dp_fixed_to_float(int N, int P, const uint64_t *du_dp, double *out) const override;
@@ -171,6 +197,9 @@ class NonbondedPairList : public Potential {
                         hipStream_t stream) override;
     void execute_dl_dp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
                               uint64_t *d_dl_dp, hipStream_t stream) override;
+    void execute_hvp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
+                            const double *d_vx, const double *d_vp, double vl, uint64_t *d_hx, uint64_t *d_hp,
+                            uint64_t *d_hl, hipStream_t stream) override;
     void du_dp_fixed_to_float(int N, int P, const uint64_t *du_dp, double *out) const override;
 
     int N_, M_;
@@ -197,6 +226,9 @@ class BondedPotential : public Potential {
                         hipStream_t stream) override;
     void execute_dl_dp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
                               uint64_t *d_dl_dp, hipStream_t stream) override;
+    void execute_hvp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
+                            const double *d_vx, const double *d_vp, double vl, uint64_t *d_hx, uint64_t *d_hp,
+                            uint64_t *d_hl, hipStream_t stream) override;
 
     BondedKind bk_;
     int n_;
@@ -215,6 +247,9 @@ class SummedPotential : public Potential {
                         hipStream_t stream) override;
     void execute_dl_dp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
                               uint64_t *d_dl_dp, hipStream_t stream) override;
+    void execute_hvp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
+                            const double *d_vx, const double *d_vp, double vl, uint64_t *d_hx, uint64_t *d_hp,
+                            uint64_t *d_hl, hipStream_t stream) override;
     void validate_host(int N, int P, const double *h_box) const override;
     void check_device_status(hipStream_t stream) override;
     void du_dp_fixed_to_float(int N, int P, const uint64_t *du_dp, double *out) const override;
@@ -232,6 +267,9 @@ class FanoutSummedPotential : public Potential {
                         hipStream_t stream) override;
     void execute_dl_dp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
                               uint64_t *d_dl_dp, hipStream_t stream) override;
+    void execute_hvp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
+                            const double *d_vx, const double *d_vp, double vl, uint64_t *d_hx, uint64_t *d_hp,
+                            uint64_t *d_hl, hipStream_t stream) override;
     void validate_host(int N, int P, const double *h_box) const override;
     void check_device_status(hipStream_t stream) override;
     void du_dp_fixed_to_float(int N, int P, const uint64_t *du_dp, double *out) const override;

@@ -1,9 +1,11 @@
 // Host-side potential classes (see potentials.h).
+#include <cmath>
 #include <cstring>
 #include <numeric>
 #include <set>
 
 #include "fixed_point.h"
+#include "hvp.h"
 #include "potentials.h"
 
 namespace tmamd {
@@ -71,6 +73,101 @@ void Potential::execute_dl_dp_host(int N, int P, const double *h_x, const double
     execute_dl_dp_device(N, P, x.ptr, P > 0 ? p.ptr : nullptr, box.ptr, lambda, dl_dp.ptr, 0);
     check_device_status(0);
     dl_dp.download(h_dl_dp, P);
+    TM_HIP_CHECK(hipStreamSynchronize(0));
+}
+
+void Potential::execute_hvp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box,
+                                   double lambda, const double *d_vx, const double *d_vp, double vl, uint64_t *d_hx,
+                                   uint64_t *d_hp, uint64_t *d_hl, hipStream_t stream) {
+    throw Error("execute_hvp: not available for this potential");
+}
+
+void Potential::execute_hvp_host(int N, int P, const double *h_x, const double *h_p, const double *h_box,
+                                 double lambda, const double *h_vx, const double *h_vp, double vl, double *h_hx,
+                                 double *h_hp, double *h_hl) {
+    validate_host(N, P, h_box);
+    const size_t n3 = static_cast<size_t>(N) * 3;
+    // tangent scale 2^-e with the scaled tangent at most 1 in magnitude (exact)
+    double vmax = std::fabs(vl);
+    for (size_t i = 0; h_vx && i < n3; i++) {
+        vmax = std::max(vmax, std::fabs(h_vx[i]));
+    }
+    for (int i = 0; h_vp && i < P; i++) {
+        vmax = std::max(vmax, std::fabs(h_vp[i]));
+    }
+    if (!std::isfinite(vmax)) {
+        throw Error("execute_hvp: tangent is not finite");
+    }
+    int e = 0;
+    if (vmax > 0.0) {
+        std::frexp(vmax, &e); // vmax = m 2^e, 0.5 <= m < 1
+    }
+    const double scale = std::ldexp(1.0, -e);
+    DeviceArray<double> x(std::max<size_t>(n3, 1)), box(9), p(std::max(P, 1)), vx, vp;
+    x.upload(h_x, n3);
+    box.upload(h_box, 9);
+    if (P > 0) {
+        p.upload(h_p, P);
+    }
+    std::vector<double> tmp;
+    if (h_vx) {
+        tmp.resize(n3);
+        for (size_t i = 0; i < n3; i++) {
+            tmp[i] = h_vx[i] * scale;
+        }
+        vx.alloc(std::max<size_t>(n3, 1));
+        vx.upload(tmp.data(), n3);
+        TM_HIP_CHECK(hipStreamSynchronize(0)); // tmp is reused below
+    }
+    if (h_vp && P > 0) {
+        tmp.resize(P);
+        for (int i = 0; i < P; i++) {
+            tmp[i] = h_vp[i] * scale;
+        }
+        vp.alloc(P);
+        vp.upload(tmp.data(), P);
+        TM_HIP_CHECK(hipStreamSynchronize(0));
+    }
+    DeviceArray<uint64_t> hx, hp, hl;
+    if (h_hx) {
+        hx.alloc(std::max<size_t>(n3, 1));
+        hx.zero();
+    }
+    if (h_hp) {
+        hp.alloc(std::max(P, 1));
+        hp.zero();
+    }
+    if (h_hl) {
+        hl.alloc(1);
+        hl.zero();
+    }
+    execute_hvp_device(N, P, x.ptr, P > 0 ? p.ptr : nullptr, box.ptr, lambda, vx.ptr, vp.ptr, vl * scale, hx.ptr,
+                       hp.ptr, hl.ptr, 0);
+    check_device_status(0);
+    std::vector<uint64_t> fixed;
+    if (h_hx) {
+        fixed.resize(std::max<size_t>(n3, 1));
+        hx.download(fixed.data(), n3);
+        TM_HIP_CHECK(hipStreamSynchronize(0));
+        for (size_t i = 0; i < n3; i++) {
+            h_hx[i] = fixed_to_double(fixed[i], FIXED_SCALE) / scale;
+        }
+    }
+    if (h_hp) {
+        fixed.resize(std::max(P, 1));
+        hp.download(fixed.data(), P);
+        TM_HIP_CHECK(hipStreamSynchronize(0));
+        du_dp_fixed_to_float(N, P, fixed.data(), h_hp);
+        for (int i = 0; i < P; i++) {
+            h_hp[i] /= scale;
+        }
+    }
+    if (h_hl) {
+        uint64_t f = 0;
+        hl.download(&f, 1);
+        TM_HIP_CHECK(hipStreamSynchronize(0));
+        h_hl[0] = fixed_to_double(f, FIXED_SCALE) / scale;
+    }
     TM_HIP_CHECK(hipStreamSynchronize(0));
 }
 
@@ -286,6 +383,21 @@ void NonbondedAllPairs::execute_dl_dp_device(int N, int P, const double *d_x, co
     st_.compute_dl_dp(d_box, d_dl_dp, stream);
 }
 
+void NonbondedAllPairs::execute_hvp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box,
+                                           double lambda, const double *d_vx, const double *d_vp, double vl, uint64_t *d_hx,
+                                           uint64_t *d_hp, uint64_t *d_hl, hipStream_t stream) {
+    validate_sizes(N, P);
+    if (interp_) {
+        throw Error(name_ + "::execute_hvp(): interpolated parameters are not supported");
+    }
+    ensure_state();
+    if (K_ == 0) {
+        return;
+    }
+    st_.prepare(d_x, d_p, nullptr, d_box, d_plane_.ptr, d_offset_.ptr, nullptr, lambda, stream);
+    launch_tiles_hvp(st_, d_p, d_offset_.ptr, d_box, d_vx, d_vp, vl, d_hx, d_hp, d_hl, stream);
+}
+
 void NonbondedAllPairs::check_device_status(hipStream_t stream) {
     if (dirty_ || K_ == 0) {
         return;
@@ -422,6 +534,16 @@ void NonbondedPairList::execute_device(int N, int P, const double *d_x, const do
     }
 }
 
+void NonbondedPairList::execute_hvp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box,
+                                           double lambda, const double *d_vx, const double *d_vp, double vl, uint64_t *d_hx,
+                                           uint64_t *d_hp, uint64_t *d_hl, hipStream_t stream) {
+    const PairListArgs a = args(N, P, d_x, d_p, d_box, lambda);
+    if (interp_) {
+        throw Error("NonbondedPairList::execute_hvp(): interpolated parameters are not supported");
+    }
+    launch_pair_list_hvp(a, d_vx, d_vp, vl, d_hx, d_hp, d_hl, stream);
+}
+
 PairListArgs NonbondedPairList::args(int N, int P, const double *d_x, const double *d_p, const double *d_box,
                                      double lambda) const {
     if (N != N_) {
@@ -548,6 +670,17 @@ void BondedPotential::execute_dl_dp_device(int N, int P, const double *d_x, cons
     launch_bonded(bk_, t, d_x, nullptr, lambda, nullptr, nullptr, d_dl_dp, nullptr, nullptr, stream, true);
 }
 
+void BondedPotential::execute_hvp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box,
+                                         double lambda, const double *d_vx, const double *d_vp, double vl, uint64_t *d_hx,
+                                         uint64_t *d_hp, uint64_t *d_hl, hipStream_t stream) {
+    if (n_ == 0) {
+        return;
+    }
+    execute_device(N, P, d_x, d_p, d_box, lambda, nullptr, nullptr, nullptr, nullptr, stream); // size validation
+    BondedTerms t{n_, d_idxs_.ptr, d_p, d_mult_.ptr, d_offset_.ptr, nullptr};
+    launch_bonded_hvp(bk_, t, d_x, d_vx, d_vp, lambda, vl, d_hx, d_hp, d_hl, stream);
+}
+
 // ---------------------------------------------------------------- sums
 
 SummedPotential::SummedPotential(const std::vector<PotentialPtr> &pots, const std::vector<int> &sizes)
@@ -581,6 +714,21 @@ void SummedPotential::execute_dl_dp_device(int N, int P, const double *d_x, cons
     int off = 0;
     for (size_t i = 0; i < pots_.size(); i++) {
         pots_[i]->execute_dl_dp_device(N, sizes_[i], d_x, d_p + off, d_box, lambda, d_dl_dp + off, stream);
+        off += sizes_[i];
+    }
+}
+
+void SummedPotential::execute_hvp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box,
+                                         double lambda, const double *d_vx, const double *d_vp, double vl, uint64_t *d_hx,
+                                         uint64_t *d_hp, uint64_t *d_hl, hipStream_t stream) {
+    if (P != P_) {
+        throw Error("SummedPotential::execute_device(): expected " + std::to_string(P_) + " parameters, got " +
+                    std::to_string(P));
+    }
+    int off = 0;
+    for (size_t i = 0; i < pots_.size(); i++) {
+        pots_[i]->execute_hvp_device(N, sizes_[i], d_x, d_p + off, d_box, lambda, d_vx, d_vp ? d_vp + off : nullptr,
+                                     vl, d_hx, d_hp ? d_hp + off : nullptr, d_hl, stream);
         off += sizes_[i];
     }
 }
@@ -620,6 +768,15 @@ void FanoutSummedPotential::execute_dl_dp_device(int N, int P, const double *d_x
                                                  hipStream_t stream) {
     for (auto &p : pots_) {
         p->execute_dl_dp_device(N, P, d_x, d_p, d_box, lambda, d_dl_dp, stream);
+    }
+}
+
+void FanoutSummedPotential::execute_hvp_device(int N, int P, const double *d_x, const double *d_p,
+                                               const double *d_box, double lambda, const double *d_vx,
+                                               const double *d_vp, double vl, uint64_t *d_hx, uint64_t *d_hp,
+                                               uint64_t *d_hl, hipStream_t stream) {
+    for (auto &p : pots_) {
+        p->execute_hvp_device(N, P, d_x, d_p, d_box, lambda, d_vx, d_vp, vl, d_hx, d_hp, d_hl, stream);
     }
 }
 

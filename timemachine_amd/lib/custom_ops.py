@@ -150,6 +150,35 @@ class Potential(_Handle):
              pd(box), float(lam), pu64(out))
         return self._du_dp_to_float(N, P, out, params.shape)
 
+    def execute_hvp(self, coords, params, box, lam, v_coords=None, v_params=None, v_lam=0.0,
+                    compute_hx=True, compute_hp=True, compute_hl=True):
+        """Hessian-vector product: the directional derivatives of (du_dx, du_dp,
+        du_dl) along the tangent (v_coords [N, 3], v_params shaped like params,
+        v_lam), i.e. H @ (v_coords, v_params, v_lam) with H the Hessian of U in
+        (coords, params, lambda). By the symmetry of H this is also the
+        vector-Jacobian product of the gradient with the tangent, which is what
+        reverse-mode differentiation through forces needs. None tangents are
+        zero. Returns (hx [N, 3], hp shaped like params, hl float), None for
+        outputs not computed. Extension of this library (the reference takes
+        second derivatives from its JAX potentials)."""
+        coords = f64(coords)
+        params = f64(params)
+        box = f64(box)
+        N = coords.shape[0]
+        P = params.size
+        vx = None if v_coords is None else f64(v_coords).reshape(N, 3)
+        vp = None if v_params is None or P == 0 else f64(v_params).reshape(P)
+        hx = np.zeros(N * 3, np.float64) if compute_hx else None
+        hp = np.zeros(max(P, 1), np.float64) if compute_hp else None
+        hl = np.zeros(1, np.float64) if compute_hl else None
+        call("tm_potential_execute_hvp_host", self._h, N, P, pd(coords), pd(params.reshape(-1)) if P else None,
+             pd(box), float(lam), None if vx is None else pd(vx), None if vp is None else pd(vp), float(v_lam),
+             pd(hx), pd(hp), pd(hl))
+        r_x = hx.reshape(N, 3) if compute_hx else None
+        r_p = hp[:P].reshape(params.shape) if compute_hp else None
+        r_l = float(hl[0]) if compute_hl else None
+        return r_x, r_p, r_l
+
     def execute_selective_batch(
         self, coords, params, boxes, lambs, compute_du_dx, compute_du_dp, compute_du_dl, compute_u
     ):
