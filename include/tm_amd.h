@@ -347,6 +347,23 @@ int tm_batch_grad_size(tm_batch *b, int s, int *out);
 int tm_batch_grad_stats(tm_batch *b, int s, int *n_frames, double *sum_a, double *sum_aa, double *sum_g,
                         double *sum_ag, double *sum_h);
 
+/* Trajectory (pathwise) adjoint, no reference counterpart: the reference's design notes
+ * describe differentiating through the Langevin trajectory with Hessian-vector products
+ * (attic/docs/paper.tex:27); its code computes none. adjoint_forward runs n_steps of every
+ * system like tm_batch_multiple_steps (no lambda schedule), stores du_dl every du_dl_interval
+ * steps into du_dl_out [S, n_steps / du_dl_interval] and keeps checkpoints of the state every
+ * checkpoint_interval steps (a multiple of du_dl_interval). adjoint_backward takes the loss's
+ * cotangent of each stored du_dl value, dl_bar [S, n_steps / du_dl_interval], recomputes each
+ * segment of frames from its checkpoint (checked bitwise against the forward run) and walks the
+ * steps backwards with one Hessian-vector product per step. grad_out: the gradient in every
+ * system's flat parameters, systems in order (tm_batch_grad_size each); x0_bar, v0_bar
+ * [total atoms, 3] (may be null): cotangents of the initial coordinates and velocities. Not
+ * available with a barostat, interpolated parameters or an interaction group. */
+int tm_batch_adjoint_forward(tm_batch *b, int n_steps, int du_dl_interval, int checkpoint_interval,
+                             double *du_dl_out);
+int tm_batch_adjoint_backward(tm_batch *b, const double *dl_bar, double *grad_out, double *x0_bar,
+                              double *v0_bar);
+
 #ifdef __cplusplus
 }
 #endif

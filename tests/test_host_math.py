@@ -325,3 +325,29 @@ def test_fp32_pair_terms_error_budget(hc, which, lam):
     assert ferr < 3e-6 and uerr < 5e-7 and dlerr < 1e-6, out["f64"]
     assert out["f32"][1] < 1e-6  # FP32 terms: energy within target
     assert out["f32"][0] > ferr  # and the force error the FP64 terms avoid
+
+
+def test_numpy_langevin_noise_matches_the_device_header(hc):
+    """oracle.philox.langevin_noise draws what philox.h draws for the integrator:
+    the same counters, key and Box-Muller normals (bitwise Philox words;
+    normals to libm rounding)."""
+    from oracle import philox
+
+    seed, step, n = 0x1234_5678_9ABC_DEF0, (1 << 33) + 17, 50
+    want = philox.langevin_noise(seed, step, n)
+    got = np.zeros((n, 3))
+    out = np.zeros(4, dtype=np.uint32)
+    z = np.zeros(2)
+    for a in range(n):
+        for blk in range(2):
+            c = np.array([2 * a + blk, step & 0xFFFFFFFF, step >> 32, philox.STREAM_LANGEVIN], dtype=np.uint32)
+            hc.hc_philox(c.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint32(seed & 0xFFFFFFFF),
+                         ctypes.c_uint32(seed >> 32), out.ctypes.data_as(ctypes.c_void_p))
+            words = philox.philox4x32_10(c[None, :], seed & 0xFFFFFFFF, seed >> 32)[0]
+            assert words.tolist() == out.tolist()
+            hc.hc_normal2(out.ctypes.data_as(ctypes.c_void_p), z.ctypes.data_as(P(ctypes.c_double)))
+            if blk == 0:
+                got[a, 0], got[a, 1] = z
+            else:
+                got[a, 2] = z[0]
+    np.testing.assert_allclose(got, want, rtol=1e-14, atol=1e-14)

@@ -546,3 +546,59 @@ def test_bonded_hvp_central_differences(rng, kind, arity, params):
     for a, plus, minus in zip((hx, hp, hl), gp_, gm_):
         fd = (np.asarray(plus) - np.asarray(minus)) / (2 * h)
         np.testing.assert_allclose(a, fd, rtol=1e-5, atol=1e-6 * max(np.abs(fd).max(), 1.0))
+
+
+def test_trajectory_backprop_matches_finite_differences(rng):
+    """oracle.trajectory: the gradient of a loss on the du/dlambda values of a
+    short BAOAB trajectory (with noise) equals central differences of the
+    forward trajectory in the parameters and in the initial coordinates."""
+    from oracle import trajectory
+
+    x, params, box, lam, excl, scales, plane, offset = _tiny_system(rng)
+    N = len(x)
+    fn = nonbonded.pair_energy_fn(x, box, excl, scales, 2.0, 1.2, plane, offset,
+                                  pairs=np.array([(i, j) for i in range(N) for j in range(i + 1, N)]))
+
+    def energy(xt, ps, lt):
+        return fn(xt, ps[0], lt)
+
+    masses = rng.uniform(4.0, 16.0, N)
+    dt, n_steps, interval = 0.001, 12, 3
+    ca, cb, cc = integrator.langevin_coefficients(300.0, dt, 1.0, masses)
+    cbs = -cb
+    noise = rng.normal(size=(n_steps, N, 3))
+    v0 = rng.normal(size=(N, 3)) * 0.5
+    dl_bar = rng.normal(size=n_steps // interval)
+    dls, (gp,), xb, vb = trajectory.trajectory_loss_grad(energy, x, v0, [params], lam, n_steps, interval, dt, ca,
+                                                         cbs, cc, dl_bar, noise)
+    assert len(dls) == n_steps // interval
+
+    def loss(p, x0):
+        d = trajectory.trajectory_du_dl(energy, x0, v0, [p], lam, n_steps, interval, dt, ca, cbs, cc, noise)
+        return float(np.dot(dl_bar, d))
+
+    dp = rng.normal(size=params.shape) * np.array([1.0, 0.01, 0.1])
+    dp[params[:, 2] == 0.0, 2] = 0.0
+    h = 1e-5
+    fd = (loss(params + h * dp, x) - loss(params - h * dp, x)) / (2 * h)
+    assert abs(fd - np.sum(gp * dp)) <= 1e-5 * max(abs(fd), 1e-3)
+    dx = rng.normal(size=x.shape)
+    fdx = (loss(params, x + h * dx) - loss(params, x - h * dx)) / (2 * h)
+    assert abs(fdx - np.sum(xb * dx)) <= 1e-5 * max(abs(fdx), 1e-3)
+
+
+def test_numpy_philox_known_answers_and_normals():
+    """oracle.philox: the Random123 known-answer vectors of philox4x32-10, and
+    Box-Muller normals with mean 0 and variance 1."""
+    from oracle import philox
+
+    kat = [
+        ([0, 0, 0, 0], (0, 0), [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+        ([0xFFFFFFFF] * 4, (0xFFFFFFFF, 0xFFFFFFFF), [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+        ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], (0xA4093822, 0x299F31D0),
+         [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+    ]
+    for ctr, key, want in kat:
+        assert philox.philox4x32_10(np.array([ctr], np.uint32), *key)[0].tolist() == want
+    z = philox.langevin_noise(1234, 7, 20000)
+    assert abs(z.mean()) < 0.02 and abs(z.var() - 1.0) < 0.03

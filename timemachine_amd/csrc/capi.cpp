@@ -1118,4 +1118,19 @@ int tm_batch_grad_stats(tm_batch *b, int s, int *n_frames, double *sum_a, double
     });
 }
 
+int tm_batch_adjoint_forward(tm_batch *b, int n_steps, int du_dl_interval, int checkpoint_interval,
+                             double *du_dl_out) {
+    return guarded([&] {
+        b->eng->adjoint_forward(n_steps, du_dl_interval, checkpoint_interval, du_dl_out);
+        if (b->raise_faults) {
+            b->eng->check_status();
+        }
+    });
+}
+
+int tm_batch_adjoint_backward(tm_batch *b, const double *dl_bar, double *grad_out, double *x0_bar,
+                              double *v0_bar) {
+    return guarded([&] { b->eng->adjoint_backward(dl_bar, grad_out, x0_bar, v0_bar); });
+}
+
 } // extern "C"

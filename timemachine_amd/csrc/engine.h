@@ -122,6 +122,25 @@ class Engine {
     void grad_stats(int s, int *n_frames, double *sum_a, double *sum_aa, double *sum_g, double *sum_ag,
                     double *sum_h);
 
+    // Trajectory (pathwise) adjoint of a loss on the du/dlambda values of a run,
+    // by reverse mode through the Langevin steps (attic/docs/paper.tex:27).
+    // adjoint_forward runs n_steps like run() (no lambda schedule; du_dl stored
+    // every du_dl_interval steps into du_dl_out [S, n_steps / du_dl_interval])
+    // and keeps the state at the start of every checkpoint_interval steps (a
+    // multiple of du_dl_interval). adjoint_backward takes the loss's cotangent
+    // of each stored value, dl_bar [S, F], and walks the steps backwards, one
+    // segment at a time: it recomputes the segment's frames from its checkpoint
+    // (bitwise the forward run's, checked at the segment end) and takes one
+    // Hessian-vector product per step. Outputs: grad_out, per system in system
+    // order, the gradient in that system's flat parameters (grad_size(s) each,
+    // grad_stats order); x0_bar, v0_bar [A, 3] (may be null) the cotangents of
+    // the initial coordinates and velocities. The engine ends in the state after
+    // the forward run. Memory: (n_steps / checkpoint_interval + 1) checkpoints
+    // of x and v, plus checkpoint_interval frames of x. Not available with a
+    // barostat, interpolated parameters or an interaction group.
+    void adjoint_forward(int n_steps, int du_dl_interval, int checkpoint_interval, double *du_dl_out);
+    void adjoint_backward(const double *dl_bar, double *grad_out, double *x0_bar, double *v0_bar);
+
     void get_x(int s, double *out);
     void get_v(int s, double *out);
     void get_box(int s, double *out);
@@ -163,6 +182,8 @@ class Engine {
     void set_tile_waves(int waves);
 
   private:
+    void flat_layout();
+    void download_flat(const double *src, int s, double *out);
     struct Impl;
     std::unique_ptr<Impl> impl_;
     int S_ = 0;

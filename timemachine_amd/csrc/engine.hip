@@ -5,6 +5,7 @@
 #include <map>
 
 #include "engine.h"
+#include "hvp.h"
 #include "fire.h"
 #include "fixed_point.h"
 #include "philox.h"
@@ -324,6 +325,110 @@ __global__ void k_obs_accum_sys(int S, const uint64_t *__restrict__ dl, double *
     sum_a[s] += a;
     sum_aa[s] += a * a;
     n_frames[s] += 1;
+}
+
+// ------------------------------------------------------ trajectory adjoint
+// Reverse mode through the BAOAB step of k_langevin. With the step
+//   v_mid = v + cb g(x),  v' = ca v_mid + cc xi,  x' = x + h (v_mid + v'),
+// (g = du/dx, h = dt / 2), output cotangents (xb', vb') give
+//   vb_mid = ca vb' + h (1 + ca) xb',  vb = vb_mid,  gb = cb vb_mid,
+//   xb = xb' + H_xx gb + H_xl lb,  pb += H_px gb + H_pl lb,
+// where lb is the cotangent of the du/dlambda stored at x (zero when the
+// step stores none). The noise is additive and drops out. The Hessian
+// products come from one HVP call with tangent (gb, 0, lb), scaled per system
+// by a power of two to at most 1 so the fixed-point accumulators stay in
+// range (the product is linear in the tangent; the scale is undone below).
+
+// vb, gb; per-system max |gb| (bit pattern of a non-negative double orders as
+// the value) into smax
+__global__ void k_adj_pre(int A, const int *__restrict__ atom_sys, const double *__restrict__ ca,
+                          const double *__restrict__ dt, const double *__restrict__ cbs,
+                          const double *__restrict__ xb, double *__restrict__ vb, double *__restrict__ gb,
+                          unsigned long long *__restrict__ smax) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= A) {
+        return;
+    }
+    const int s = atom_sys[a];
+    const double c = ca[s], h = 0.5 * dt[s], cb = cbs[a];
+    double m = 0.0;
+    for (int d = 0; d < 3; d++) {
+        const double vm = c * vb[a * 3 + d] + h * (1.0 + c) * xb[a * 3 + d];
+        vb[a * 3 + d] = vm;
+        const double g = cb * vm;
+        gb[a * 3 + d] = g;
+        m = fmax(m, fabs(g));
+    }
+    if (m > 0.0) {
+        atomicMax(smax + s, static_cast<unsigned long long>(__double_as_longlong(m)));
+    }
+}
+
+// lambda cotangent of step j (the du/dlambda stored at the start of step j
+// when (j + 1) % interval == 0), the tangent scale 2^-e (scaled tangent at
+// most 1 in magnitude), the scaled lambda tangent
+__global__ void k_adj_scale(int S, int j, int interval, int F, const double *__restrict__ dl_bar,
+                            const unsigned long long *__restrict__ smax, double *__restrict__ scale,
+                            double *__restrict__ vls) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) {
+        return;
+    }
+    double lb = 0.0;
+    if (interval > 0 && (j + 1) % interval == 0) {
+        const int f = (j + 1) / interval - 1;
+        if (f < F) {
+            lb = dl_bar[static_cast<size_t>(s) * F + f];
+        }
+    }
+    const double m = fmax(__longlong_as_double(static_cast<long long>(smax[s])), fabs(lb));
+    int e = 0;
+    if (m > 0.0) {
+        frexp(m, &e);
+    }
+    const double sc = ldexp(1.0, -e);
+    scale[s] = sc;
+    vls[s] = lb * sc;
+}
+
+__global__ void k_adj_scale_vec(int A, const int *__restrict__ atom_sys, const double *__restrict__ scale,
+                                const double *__restrict__ gb, double *__restrict__ vx) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A * 3) {
+        return;
+    }
+    vx[i] = gb[i] * scale[atom_sys[i / 3]];
+}
+
+// xb += H gb (decoded and unscaled)
+__global__ void k_adj_post_x(int A, const int *__restrict__ atom_sys, const double *__restrict__ scale,
+                             const uint64_t *__restrict__ hx, double *__restrict__ xb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A * 3) {
+        return;
+    }
+    xb[i] += fixed_to_double(hx[i], FIXED_SCALE) / scale[atom_sys[i / 3]];
+}
+
+// pb += H_p. gb per flat parameter element (one thread per element, steps in
+// order: deterministic)
+__global__ void k_adj_post_p(int Pg, const int *__restrict__ elem_sys, const double *__restrict__ elem_inv_scale,
+                             const double *__restrict__ scale, const uint64_t *__restrict__ hp,
+                             double *__restrict__ pb) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= Pg) {
+        return;
+    }
+    pb[e] += static_cast<double>(static_cast<int64_t>(hp[e])) * elem_inv_scale[e] / scale[elem_sys[e]];
+}
+
+// flag = 1 if a differs from b anywhere (bitwise)
+__global__ void k_adj_compare(size_t n, const double *__restrict__ a, const double *__restrict__ b,
+                              int *__restrict__ flag) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n && __double_as_longlong(a[i]) != __double_as_longlong(b[i])) {
+        flag[0] = 1;
+    }
 }
 
 // ---------------------------------------------------------------- barostat
@@ -667,6 +772,12 @@ struct Engine::Impl {
     DeviceArray<uint64_t> obs_dudp, obs_dldp, obs_dl;
     DeviceArray<int> obs_sys, obs_n;
     DeviceArray<double> obs_inv_scale, obs_sum_g, obs_sum_ag, obs_sum_h, obs_sum_a, obs_sum_aa;
+
+    // trajectory adjoint: checkpoints of the last adjoint_forward (state at the
+    // start of every segment of ck_interval steps)
+    int adj_steps = 0, adj_dl_interval = 0, adj_ck_interval = 0;
+    std::vector<DeviceArray<double>> ck_x, ck_v;
+    std::vector<DeviceArray<int64_t>> ck_step;
 
     BaroDev baro_dev() {
         return BaroDev{b_has.ptr,      b_interval.ptr, b_bstep.ptr,    b_seed.ptr,     b_pressure.ptr,  b_kT.ptr,
@@ -1548,6 +1659,63 @@ void Engine::copy_state_from(Engine &src, int src_s, int dst_s) {
     TM_HIP_CHECK(hipStreamSynchronize(stream_));
 }
 
+// Flat parameter space of the gradient observer and the trajectory adjoint:
+// nonbonded [A, 3] rows, then bond [B, 2], angle [A, 2], torsion [T, 3] terms;
+// the system and the fixed-point exponent of every element.
+void Engine::flat_layout() {
+    Impl &I = *impl_;
+    if (I.Pg > 0) {
+        return;
+    }
+    static const int np[3] = {2, 2, 3};
+    I.bt_off[0] = A_ * 3;
+    I.bt_off[1] = I.bt_off[0] + I.bt_n[0] * np[0];
+    I.bt_off[2] = I.bt_off[1] + I.bt_n[1] * np[1];
+    const int Pg = I.bt_off[2] + I.bt_n[2] * np[2];
+    std::vector<int> esys(Pg);
+    std::vector<double> inv(Pg, 1.0 / FIXED_SCALE);
+    for (int a = 0; a < A_; a++) {
+        const int s = static_cast<int>(std::upper_bound(I.atom0.begin(), I.atom0.end(), a) - I.atom0.begin()) - 1;
+        for (int c = 0; c < 3; c++) {
+            esys[a * 3 + c] = s;
+        }
+        inv[a * 3 + 1] = 1.0 / FIXED_SCALE_DSIG;
+        inv[a * 3 + 2] = 1.0 / FIXED_SCALE_DEPS;
+    }
+    for (int k = 0; k < 3; k++) {
+        for (int s = 0; s < S_; s++) {
+            for (int t = I.bt_first[k][s]; t < I.bt_first[k][s] + I.bt_count[k][s]; t++) {
+                for (int c = 0; c < np[k]; c++) {
+                    esys[I.bt_off[k] + t * np[k] + c] = s;
+                }
+            }
+        }
+    }
+    upload_vec(I.obs_sys, esys);
+    upload_vec(I.obs_inv_scale, inv);
+    I.Pg = Pg;
+}
+
+// System s's elements of a flat [Pg] device array, in grad_stats order.
+void Engine::download_flat(const double *src, int s, double *out) {
+    Impl &I = *impl_;
+    static const int np[3] = {2, 2, 3};
+    std::vector<std::pair<size_t, size_t>> ranges; // (device offset, count) in output order
+    ranges.emplace_back(static_cast<size_t>(h_atom0_[s]) * 3, static_cast<size_t>(h_n_[s]) * 3);
+    for (int k = 0; k < 3; k++) {
+        ranges.emplace_back(static_cast<size_t>(I.bt_off[k]) + static_cast<size_t>(I.bt_first[k][s]) * np[k],
+                            static_cast<size_t>(I.bt_count[k][s]) * np[k]);
+    }
+    size_t o = 0;
+    for (auto &r : ranges) {
+        if (r.second > 0) {
+            TM_HIP_CHECK(hipMemcpyAsync(out + o, src + r.first, r.second * sizeof(double), hipMemcpyDeviceToHost,
+                                        stream_));
+        }
+        o += r.second;
+    }
+}
+
 void Engine::set_grad_observer(int interval) {
     Impl &I = *impl_;
     if (interval < 0) {
@@ -1562,34 +1730,8 @@ void Engine::set_grad_observer(int interval) {
     if (interval > 0 && I.has_ig) {
         throw Error("Engine::set_grad_observer: not available with a NonbondedInteractionGroup");
     }
-    if (I.Pg == 0) {
-        // flat parameter space: nonbonded [A, 3], then bond [B, 2], angle [A, 2], torsion [T, 3]
-        static const int np[3] = {2, 2, 3};
-        I.bt_off[0] = A_ * 3;
-        I.bt_off[1] = I.bt_off[0] + I.bt_n[0] * np[0];
-        I.bt_off[2] = I.bt_off[1] + I.bt_n[1] * np[1];
-        I.Pg = I.bt_off[2] + I.bt_n[2] * np[2];
-        std::vector<int> esys(I.Pg);
-        std::vector<double> inv(I.Pg, 1.0 / FIXED_SCALE);
-        for (int a = 0; a < A_; a++) {
-            const int s = static_cast<int>(std::upper_bound(I.atom0.begin(), I.atom0.end(), a) - I.atom0.begin()) - 1;
-            for (int c = 0; c < 3; c++) {
-                esys[a * 3 + c] = s;
-            }
-            inv[a * 3 + 1] = 1.0 / FIXED_SCALE_DSIG;
-            inv[a * 3 + 2] = 1.0 / FIXED_SCALE_DEPS;
-        }
-        for (int k = 0; k < 3; k++) {
-            for (int s = 0; s < S_; s++) {
-                for (int t = I.bt_first[k][s]; t < I.bt_first[k][s] + I.bt_count[k][s]; t++) {
-                    for (int c = 0; c < np[k]; c++) {
-                        esys[I.bt_off[k] + t * np[k] + c] = s;
-                    }
-                }
-            }
-        }
-        upload_vec(I.obs_sys, esys);
-        upload_vec(I.obs_inv_scale, inv);
+    flat_layout();
+    if (I.obs_sum_g.n == 0) {
         I.clear_graphs(); // kept graphs point at the observer's old buffers
         I.obs_dudp.alloc(I.Pg);
         I.obs_dldp.alloc(I.Pg);
@@ -1613,7 +1755,7 @@ void Engine::set_grad_observer(int interval) {
 
 void Engine::observe() {
     Impl &I = *impl_;
-    if (I.Pg == 0) {
+    if (I.obs_sum_g.n == 0) {
         throw Error("Engine::observe: call set_grad_observer first");
     }
     I.observe(stream_, tile_grid);
@@ -1633,31 +1775,193 @@ int Engine::grad_size(int s) const {
 void Engine::grad_stats(int s, int *n_frames, double *sum_a, double *sum_aa, double *sum_g, double *sum_ag,
                         double *sum_h) {
     Impl &I = *impl_;
-    if (I.Pg == 0) {
+    if (I.obs_sum_g.n == 0) {
         throw Error("Engine::grad_stats: call set_grad_observer first");
     }
-    static const int np[3] = {2, 2, 3};
-    std::vector<std::pair<size_t, size_t>> ranges; // (device offset, count) in output order
-    ranges.emplace_back(static_cast<size_t>(h_atom0_[s]) * 3, static_cast<size_t>(h_n_[s]) * 3);
-    for (int k = 0; k < 3; k++) {
-        ranges.emplace_back(static_cast<size_t>(I.bt_off[k]) + static_cast<size_t>(I.bt_first[k][s]) * np[k],
-                            static_cast<size_t>(I.bt_count[k][s]) * np[k]);
-    }
-    double *outs[3] = {sum_g, sum_ag, sum_h};
-    const DeviceArray<double> *srcs[3] = {&I.obs_sum_g, &I.obs_sum_ag, &I.obs_sum_h};
-    for (int q = 0; q < 3; q++) {
-        size_t o = 0;
-        for (auto &r : ranges) {
-            if (r.second > 0) {
-                TM_HIP_CHECK(hipMemcpyAsync(outs[q] + o, srcs[q]->ptr + r.first, r.second * sizeof(double),
-                                            hipMemcpyDeviceToHost, stream_));
-            }
-            o += r.second;
-        }
-    }
+    download_flat(I.obs_sum_g.ptr, s, sum_g);
+    download_flat(I.obs_sum_ag.ptr, s, sum_ag);
+    download_flat(I.obs_sum_h.ptr, s, sum_h);
     TM_HIP_CHECK(hipMemcpyAsync(n_frames, I.obs_n.ptr + s, sizeof(int), hipMemcpyDeviceToHost, stream_));
     TM_HIP_CHECK(hipMemcpyAsync(sum_a, I.obs_sum_a.ptr + s, sizeof(double), hipMemcpyDeviceToHost, stream_));
     TM_HIP_CHECK(hipMemcpyAsync(sum_aa, I.obs_sum_aa.ptr + s, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void Engine::adjoint_forward(int n_steps, int du_dl_interval, int checkpoint_interval, double *du_dl_out) {
+    Impl &I = *impl_;
+    if (I.has_baro) {
+        throw Error("Engine::adjoint: not available with a barostat (its moves are not differentiable)");
+    }
+    if (I.interp || I.has_ig || !I.pl_shares_nb) {
+        throw Error("Engine::adjoint: needs plain nonbonded parameters shared by the pair lists and no "
+                    "interaction group");
+    }
+    if (n_steps <= 0 || du_dl_interval <= 0 || checkpoint_interval <= 0 ||
+        checkpoint_interval % du_dl_interval != 0) {
+        throw Error("Engine::adjoint_forward: need n_steps > 0 and a checkpoint interval that is a positive "
+                    "multiple of the du_dl interval");
+    }
+    flat_layout();
+    const int K = checkpoint_interval;
+    const int nseg = ceil_div(n_steps, K);
+    const int F = n_steps / du_dl_interval;
+    const size_t n3 = static_cast<size_t>(A_) * 3;
+    I.ck_x.resize(nseg + 1);
+    I.ck_v.resize(nseg + 1);
+    I.ck_step.resize(nseg + 1);
+    auto save = [&](int c) {
+        if (I.ck_x[c].n != std::max<size_t>(n3, 1)) {
+            I.ck_x[c].alloc(std::max<size_t>(n3, 1));
+            I.ck_v[c].alloc(std::max<size_t>(n3, 1));
+            I.ck_step[c].alloc(S_);
+        }
+        TM_HIP_CHECK(hipMemcpyAsync(I.ck_x[c].ptr, I.x.ptr, n3 * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        TM_HIP_CHECK(hipMemcpyAsync(I.ck_v[c].ptr, I.v.ptr, n3 * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        TM_HIP_CHECK(hipMemcpyAsync(I.ck_step[c].ptr, I.step.ptr, S_ * sizeof(int64_t), hipMemcpyDeviceToDevice,
+                                    stream_));
+    };
+    I.adj_steps = 0; // invalid until the run completes
+    std::vector<double> dl;
+    for (int c = 0; c < nseg; c++) {
+        save(c);
+        const int Kc = std::min(K, n_steps - c * K);
+        const int Fc = Kc / du_dl_interval;
+        dl.assign(static_cast<size_t>(S_) * std::max(Fc, 1), 0.0);
+        run(Kc, nullptr, du_dl_interval, 0, Fc > 0 ? dl.data() : nullptr, nullptr, nullptr, true);
+        const int f0 = c * (K / du_dl_interval);
+        for (int s = 0; du_dl_out && s < S_; s++) {
+            for (int f = 0; f < Fc; f++) {
+                du_dl_out[static_cast<size_t>(s) * F + f0 + f] = dl[static_cast<size_t>(s) * Fc + f];
+            }
+        }
+    }
+    save(nseg);
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+    I.adj_steps = n_steps;
+    I.adj_dl_interval = du_dl_interval;
+    I.adj_ck_interval = K;
+}
+
+void Engine::adjoint_backward(const double *dl_bar, double *grad_out, double *x0_bar, double *v0_bar) {
+    Impl &I = *impl_;
+    if (I.adj_steps == 0) {
+        throw Error("Engine::adjoint_backward: call adjoint_forward first");
+    }
+    const int n_steps = I.adj_steps, K = I.adj_ck_interval, interval = I.adj_dl_interval;
+    const int nseg = ceil_div(n_steps, K);
+    const int F = n_steps / interval;
+    const size_t n3 = static_cast<size_t>(A_) * 3;
+    const int TPB = 256;
+    const int gA = ceil_div(std::max(A_, 1), TPB), gA3 = ceil_div(std::max<size_t>(n3, 1), TPB);
+    const int gP = ceil_div(std::max(I.Pg, 1), TPB), gS = ceil_div(S_, 64);
+    DeviceArray<double> d_dl_bar(std::max<size_t>(static_cast<size_t>(S_) * F, 1));
+    if (F > 0) {
+        d_dl_bar.upload(dl_bar, static_cast<size_t>(S_) * F, stream_);
+    }
+    DeviceArray<double> xb(std::max<size_t>(n3, 1)), vb(std::max<size_t>(n3, 1)), gb(std::max<size_t>(n3, 1)),
+        vx(std::max<size_t>(n3, 1)), pb(std::max(I.Pg, 1)), scale(S_), vls(S_);
+    DeviceArray<uint64_t> hx(std::max<size_t>(n3, 1)), hp(std::max(I.Pg, 1)), smax(S_);
+    DeviceArray<double> seg(std::max<size_t>(static_cast<size_t>(std::min(K, n_steps)) * n3, 1));
+    DeviceArray<int> flag(1);
+    xb.zero(stream_);
+    vb.zero(stream_);
+    pb.zero(stream_);
+    flag.zero(stream_);
+    auto restore = [&](int c) {
+        TM_HIP_CHECK(hipMemcpyAsync(I.x.ptr, I.ck_x[c].ptr, n3 * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        TM_HIP_CHECK(hipMemcpyAsync(I.v.ptr, I.ck_v[c].ptr, n3 * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        TM_HIP_CHECK(hipMemcpyAsync(I.step.ptr, I.ck_step[c].ptr, S_ * sizeof(int64_t), hipMemcpyDeviceToDevice,
+                                    stream_));
+    };
+    // one step of run() without barostat, frames or observer (the same launches)
+    auto plain_step = [&]() {
+        launch_fill_u64(I.du_dx.ptr, n3, 0, stream_);
+        I.eval(I.x.ptr, I.box.ptr, I.du_dx.ptr, I.du_dl_buf.ptr, nullptr, stream_, tile_grid);
+        k_langevin<<<gA, TPB, 0, stream_>>>(A_, I.atom_sys.ptr, I.d_atom0.ptr, I.step.ptr, I.seed.ptr, I.ca.ptr,
+                                            I.dt.ptr, I.cbs.ptr, I.ccs.ptr, I.du_dx.ptr, I.x.ptr, I.v.ptr,
+                                            I.sys_status.ptr);
+        TM_LAUNCH_CHECK();
+        k_step_end<<<gS, 64, 0, stream_>>>(S_, I.step.ptr, I.run_step.ptr, I.du_dl_buf.ptr, 1, 0, nullptr, 0, 0,
+                                          I.box.ptr, nullptr);
+        TM_LAUNCH_CHECK();
+    };
+    HvpTangent tan;
+    tan.vx = vx.ptr;
+    tan.vls = vls.ptr;
+    for (int c = nseg - 1; c >= 0; c--) {
+        const int Kc = std::min(K, n_steps - c * K);
+        restore(c);
+        launch_fill_i32(I.run_step.ptr, S_, 0, stream_);
+        launch_fill_u64(I.du_dl_buf.ptr, S_, 0, stream_);
+        for (int t = 0; t < Kc; t++) {
+            TM_HIP_CHECK(hipMemcpyAsync(seg.ptr + static_cast<size_t>(t) * n3, I.x.ptr, n3 * sizeof(double),
+                                        hipMemcpyDeviceToDevice, stream_));
+            plain_step();
+        }
+        // the recomputed segment ends where the forward run's did
+        k_adj_compare<<<gA3, TPB, 0, stream_>>>(n3, I.x.ptr, I.ck_x[c + 1].ptr, flag.ptr);
+        TM_LAUNCH_CHECK();
+        for (int t = Kc - 1; t >= 0; t--) {
+            const int j = c * K + t;
+            const double *xt = seg.ptr + static_cast<size_t>(t) * n3;
+            launch_fill_u64(smax.ptr, S_, 0, stream_);
+            k_adj_pre<<<gA, TPB, 0, stream_>>>(A_, I.atom_sys.ptr, I.ca.ptr, I.dt.ptr, I.cbs.ptr, xb.ptr, vb.ptr,
+                                               gb.ptr, reinterpret_cast<unsigned long long *>(smax.ptr));
+            TM_LAUNCH_CHECK();
+            k_adj_scale<<<gS, 64, 0, stream_>>>(S_, j, interval, F, d_dl_bar.ptr,
+                                               reinterpret_cast<const unsigned long long *>(smax.ptr), scale.ptr,
+                                               vls.ptr);
+            TM_LAUNCH_CHECK();
+            k_adj_scale_vec<<<gA3, TPB, 0, stream_>>>(A_, I.atom_sys.ptr, scale.ptr, gb.ptr, vx.ptr);
+            TM_LAUNCH_CHECK();
+            launch_fill_u64(hx.ptr, n3, 0, stream_);
+            launch_fill_u64(hp.ptr, I.Pg, 0, stream_);
+            if (I.has_nb) {
+                I.nb.prepare(xt, I.nb_params.ptr, nullptr, I.box.ptr, I.nb_plane.ptr, I.nb_offset.ptr, I.lam.ptr,
+                             0.0, stream_);
+                launch_tiles_hvp(I.nb, I.nb_params.ptr, I.nb_offset.ptr, I.box.ptr, tan, HvpOut{hx.ptr, hp.ptr, nullptr},
+                                 stream_);
+            }
+            for (int neg = 0; neg < 2; neg++) {
+                if (I.pl_n[neg] > 0) {
+                    launch_pair_list_hvp(I.pl_args(neg, xt, I.box.ptr), tan, HvpOut{hx.ptr, hp.ptr, nullptr}, stream_);
+                }
+            }
+            for (int k = 0; k < 3; k++) {
+                if (I.bt_n[k] == 0) {
+                    continue;
+                }
+                BondedTerms bt{I.bt_n[k], I.bt_idx[k].ptr, I.bt_params[k].ptr, I.bt_mult[k].ptr, I.bt_offset[k].ptr,
+                               I.bt_sys[k].ptr};
+                launch_bonded_hvp(static_cast<BondedKind>(k), bt, xt, I.lam.ptr, 0.0, I.red_idx.ptr, tan,
+                                  HvpOut{hx.ptr, hp.ptr + I.bt_off[k], nullptr}, stream_);
+            }
+            k_adj_post_x<<<gA3, TPB, 0, stream_>>>(A_, I.atom_sys.ptr, scale.ptr, hx.ptr, xb.ptr);
+            TM_LAUNCH_CHECK();
+            k_adj_post_p<<<gP, TPB, 0, stream_>>>(I.Pg, I.obs_sys.ptr, I.obs_inv_scale.ptr, scale.ptr, hp.ptr, pb.ptr);
+            TM_LAUNCH_CHECK();
+        }
+    }
+    restore(nseg); // the state after the forward run
+    int mismatch = 0;
+    flag.download(&mismatch, 1, stream_);
+    TM_HIP_CHECK(hipStreamSynchronize(stream_));
+    if (mismatch) {
+        throw Error("Engine::adjoint_backward: a recomputed segment differs from the forward run");
+    }
+    if (grad_out) {
+        size_t o = 0;
+        for (int s = 0; s < S_; s++) {
+            download_flat(pb.ptr, s, grad_out + o);
+            o += grad_size(s);
+        }
+    }
+    if (x0_bar) {
+        xb.download(x0_bar, n3, stream_);
+    }
+    if (v0_bar) {
+        vb.download(v0_bar, n3, stream_);
+    }
     TM_HIP_CHECK(hipStreamSynchronize(stream_));
 }
 

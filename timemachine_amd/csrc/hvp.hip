@@ -44,37 +44,34 @@ __device__ __forceinline__ uint64_t fixed_signed(double v, double scale, bool ne
     return negate ? 0ull - f : f;
 }
 
-// hl += the wave's sum of v (every lane must reach it)
-__device__ __forceinline__ void wave_add_fixed(uint64_t *hl, uint64_t v) {
-    const uint64_t tot = wave_sum_u64(v);
-    if (lane_id() == 0) {
-        atomic_add_fixed(hl, tot);
-    }
-}
-
 // ------------------------------------------------------------------ bonded
 
 template <int KIND>
 __global__ __launch_bounds__(128) void k_bonded_hvp(const BondedTerms t, const double *__restrict__ x,
-                                                    const double *__restrict__ vx, const double *__restrict__ vp,
-                                                    double lambda, double vl, uint64_t *__restrict__ hx,
-                                                    uint64_t *__restrict__ hp, uint64_t *__restrict__ hl) {
+                                                    const double *__restrict__ lambdas, double lambda0,
+                                                    const int *__restrict__ red_idx, const HvpTangent v,
+                                                    const HvpOut out) {
     constexpr int AR = KIND == 0 ? 2 : (KIND == 1 ? 3 : 4);
     constexpr int NP = KIND == 2 ? 3 : 2;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t fl = 0;
+    int red = -1;
     if (q < t.n) {
+        const int s = t.term_sys ? t.term_sys[q] : 0;
+        const double lambda = lambdas ? lambdas[s] : lambda0;
+        const double vl = v.vls ? v.vls[s] : v.vl;
+        red = red_idx ? red_idx[s] : 0;
         Dual pos[AR][3];
         int at[AR];
         for (int a = 0; a < AR; a++) {
             at[a] = t.idx[q * AR + a];
             for (int d = 0; d < 3; d++) {
-                pos[a][d] = Dual(x[at[a] * 3 + d], vx ? vx[at[a] * 3 + d] : 0.0);
+                pos[a][d] = Dual(x[at[a] * 3 + d], v.vx ? v.vx[at[a] * 3 + d] : 0.0);
             }
         }
         Dual prm[NP];
         for (int c = 0; c < NP; c++) {
-            prm[c] = Dual(t.params[q * NP + c], vp ? vp[q * NP + c] : 0.0);
+            prm[c] = Dual(t.params[q * NP + c], v.vp ? v.vp[q * NP + c] : 0.0);
         }
         const int mult = t.mult ? t.mult[q] : 0;
         const int off = t.offset ? t.offset[q] : 1;
@@ -116,22 +113,22 @@ __global__ __launch_bounds__(128) void k_bonded_hvp(const BondedTerms t, const d
             dp[2] = o.dperiod;
             dl = o.dl;
         }
-        if (hx) {
+        if (out.hx) {
             for (int a = 0; a < AR; a++) {
                 for (int d = 0; d < 3; d++) {
-                    atomic_add_fixed(hx + at[a] * 3 + d, f64_to_fixed(g[a][d].d, FIXED_SCALE));
+                    atomic_add_fixed(out.hx + at[a] * 3 + d, f64_to_fixed(g[a][d].d, FIXED_SCALE));
                 }
             }
         }
-        if (hp) {
+        if (out.hp) {
             for (int c = 0; c < NP; c++) {
-                atomic_add_fixed(hp + q * NP + c, f64_to_fixed(dp[c].d, FIXED_SCALE));
+                atomic_add_fixed(out.hp + q * NP + c, f64_to_fixed(dp[c].d, FIXED_SCALE));
             }
         }
         fl = f64_to_fixed(dl.d, FIXED_SCALE);
     }
-    if (hl) {
-        wave_add_fixed(hl, fl);
+    if (out.hl) {
+        wave_add_by_key(out.hl, red, fl);
     }
 }
 
@@ -222,119 +219,130 @@ __device__ __forceinline__ PairTangents pair_tangents(const PairGradDual &o, con
     return t;
 }
 
-// One thread per pair of the list (one system), as k_pair_list.
-__global__ __launch_bounds__(256) void k_pair_list_hvp(const PairListArgs a, const double *__restrict__ vx,
-                                                       const double *__restrict__ vp, double vl,
-                                                       uint64_t *__restrict__ hx, uint64_t *__restrict__ hp,
-                                                       uint64_t *__restrict__ hl) {
+// One thread per pair of the list, as k_pair_list (one or many systems).
+__global__ __launch_bounds__(256) void k_pair_list_hvp(const PairListArgs a, const HvpTangent v, const HvpOut out) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     PairTangents t = zero_tangents();
-    int gi = 0, gj = 0;
+    int gi = 0, gj = 0, red = -1;
     bool in = false;
     if (p < a.M) {
+        const int s = a.pair_sys ? a.pair_sys[p] : 0;
         gi = a.pairs[p * 2 + 0];
         gj = a.pairs[p * 2 + 1];
-        const double lam = a.lambda0;
-        const double cutoff = a.cutoff0;
+        red = a.red_idx ? a.red_idx[s] : 0;
+        const double lam = a.lambdas ? a.lambdas[s] : a.lambda0;
+        const double cutoff = a.cutoffs ? a.cutoffs[s] : a.cutoff0;
+        const double beta = static_cast<double>(a.betas ? a.betas[s] : a.beta0);
+        const double vl = v.vls ? v.vls[s] : v.vl;
         const int pli = a.plane ? a.plane[gi] : 0, plj = a.plane ? a.plane[gj] : 0;
         const int ofi = a.offset ? a.offset[gi] : 0, ofj = a.offset ? a.offset[gj] : 0;
         const double dwdl = (ofi - ofj) * cutoff;
         const Dual dw((pli + ofi * lam) * cutoff - (plj + ofj * lam) * cutoff, dwdl * vl);
-        const double L[3] = {a.boxes[0], a.boxes[4], a.boxes[8]};
+        const double *box = a.boxes + s * 9;
+        const double L[3] = {box[0], box[4], box[8]};
         Dual dv[3];
         for (int d = 0; d < 3; d++) {
             // the image shift is a constant: the tangent is vx_i - vx_j
-            const double tan = vx ? vx[gi * 3 + d] - vx[gj * 3 + d] : 0.0;
+            const double tan = v.vx ? v.vx[gi * 3 + d] - v.vx[gj * 3 + d] : 0.0;
             dv[d] = Dual(min_image_d(a.x[gi * 3 + d] - a.x[gj * 3 + d], L[d], 1.0 / L[d]), tan);
         }
         const double d2 = __builtin_fma(dw.v, dw.v, pair_d2(dv[0].v, dv[1].v, dv[2].v));
         in = d2 < cutoff * cutoff;
         if (in) {
             const PairGradDual o = pair_grad_dual(
-                dv, dw, dwdl, dual_param(a.params, vp, gi, 0), dual_param(a.params, vp, gj, 0),
-                dual_param(a.params, vp, gi, 1), dual_param(a.params, vp, gj, 1), dual_param(a.params, vp, gi, 2),
-                dual_param(a.params, vp, gj, 2), static_cast<double>(a.beta0), a.scales[p * 2 + 0],
+                dv, dw, dwdl, dual_param(a.params, v.vp, gi, 0), dual_param(a.params, v.vp, gj, 0),
+                dual_param(a.params, v.vp, gi, 1), dual_param(a.params, v.vp, gj, 1),
+                dual_param(a.params, v.vp, gi, 2), dual_param(a.params, v.vp, gj, 2), beta, a.scales[p * 2 + 0],
                 a.scales[p * 2 + 1]);
             t = pair_tangents(o, dv, a.negated != 0);
         }
     }
     if (in) {
         for (int c = 0; c < 3; c++) {
-            if (hx) {
-                atomic_add_fixed(hx + gi * 3 + c, t.xi[c]);
-                atomic_add_fixed(hx + gj * 3 + c, t.xj[c]);
+            if (out.hx) {
+                atomic_add_fixed(out.hx + gi * 3 + c, t.xi[c]);
+                atomic_add_fixed(out.hx + gj * 3 + c, t.xj[c]);
             }
-            if (hp) {
-                atomic_add_fixed(hp + gi * 3 + c, t.pi[c]);
-                atomic_add_fixed(hp + gj * 3 + c, t.pj[c]);
+            if (out.hp) {
+                atomic_add_fixed(out.hp + gi * 3 + c, t.pi[c]);
+                atomic_add_fixed(out.hp + gj * 3 + c, t.pj[c]);
             }
         }
     }
-    if (hl) {
-        wave_add_fixed(hl, t.l);
+    if (out.hl) {
+        wave_add_by_key(out.hl, red, t.l);
     }
 }
 
-// All pairs of one system from the tile list of the last neighborlist build:
-// one wave per tile, lane l holding row slot 64 r + l, the tile's 64 column
-// atoms visited in turn (the tiles, pair test and cutoff of the force kernel).
-// Row terms stay in registers until the tile ends; column terms are summed
-// over the wave (exact fixed-point sums) before one atomic per value.
+// All pairs of the tile lists of the last neighborlist build (one or many
+// systems): one wave per tile, lane l holding row slot 64 r + l, the tile's 64
+// column atoms visited in turn (the tiles, pair test and cutoff of the force
+// kernel). Row terms stay in registers until the tile ends; column terms are
+// summed over the wave (exact fixed-point sums) before one atomic per value.
 __global__ __launch_bounds__(64) void k_tiles_hvp(
-    const NbSys *__restrict__ sys, const int *__restrict__ tile_prefix, const int *__restrict__ tile_row,
+    const NbSys *__restrict__ sys, int S, const int *__restrict__ tile_prefix, const int *__restrict__ tile_row,
     const uint32_t *__restrict__ tile_atoms, const uint32_t *__restrict__ perm, const double4 *__restrict__ pos,
     const double *__restrict__ params, const int *__restrict__ offset, const double *__restrict__ boxes,
-    const int *__restrict__ status, const double *__restrict__ vx, const double *__restrict__ vp, double vl,
-    uint64_t *__restrict__ hx, uint64_t *__restrict__ hp, uint64_t *__restrict__ hl) {
-    if (status[0] & (NB_STATUS_BOX_TOO_SMALL | NB_STATUS_BOX_NOT_ORTHO)) {
-        return;
-    }
-    const NbSys S = sys[0];
-    const int total = tile_prefix[1];
-    const double L[3] = {boxes[0], boxes[4], boxes[8]};
-    const double beta = static_cast<double>(S.beta);
-    const double cut2 = S.cutoff_d * S.cutoff_d;
+    const int *__restrict__ status, const HvpTangent v, const HvpOut out) {
+    const int total = tile_prefix[S];
     const int lane = threadIdx.x;
-    uint64_t acc_l = 0;
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        const int kt = S.tile0 + t;
+        int lo = 0, hi = S - 1; // the system of tile t
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (tile_prefix[mid] <= t) {
+                lo = mid;
+            } else {
+                hi = mid - 1;
+            }
+        }
+        const int s = lo;
+        if (status[s] & (NB_STATUS_BOX_TOO_SMALL | NB_STATUS_BOX_NOT_ORTHO)) {
+            continue;
+        }
+        const NbSys Sd = sys[s];
+        const double L[3] = {boxes[s * 9 + 0], boxes[s * 9 + 4], boxes[s * 9 + 8]};
+        const double beta = static_cast<double>(Sd.beta);
+        const double cut2 = Sd.cutoff_d * Sd.cutoff_d;
+        const double vl = v.vls ? v.vls[s] : v.vl;
+        const int kt = Sd.tile0 + (t - tile_prefix[s]);
         const int r = tile_row[kt];
         const uint32_t i = static_cast<uint32_t>(r * TILE + lane);
-        const bool vi = i < static_cast<uint32_t>(S.row_end);
+        const bool vi = i < static_cast<uint32_t>(Sd.row_end);
         double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
         int gi = 0, ofi = 0;
         Dual qi, si, ei;
         double vxi[3] = {0.0, 0.0, 0.0};
         if (vi) {
-            pi = pos[S.slot0 + i];
-            gi = S.atom0 + static_cast<int>(perm[S.slot0 + i]);
+            pi = pos[Sd.slot0 + i];
+            gi = Sd.atom0 + static_cast<int>(perm[Sd.slot0 + i]);
             ofi = offset ? offset[gi] : 0;
-            qi = dual_param(params, vp, gi, 0);
-            si = dual_param(params, vp, gi, 1);
-            ei = dual_param(params, vp, gi, 2);
-            if (vx) {
+            qi = dual_param(params, v.vp, gi, 0);
+            si = dual_param(params, v.vp, gi, 1);
+            ei = dual_param(params, v.vp, gi, 2);
+            if (v.vx) {
                 for (int d = 0; d < 3; d++) {
-                    vxi[d] = vx[gi * 3 + d];
+                    vxi[d] = v.vx[gi * 3 + d];
                 }
             }
         }
-        uint64_t rx[3] = {0, 0, 0}, rp[3] = {0, 0, 0};
+        uint64_t rx[3] = {0, 0, 0}, rp[3] = {0, 0, 0}, acc_l = 0;
         const uint32_t jl = tile_atoms[static_cast<size_t>(kt) * TILE + lane];
         for (int c = 0; c < TILE; c++) {
             const uint32_t j = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(jl), c));
-            if (j >= static_cast<uint32_t>(S.slot_end)) {
+            if (j >= static_cast<uint32_t>(Sd.slot_end)) {
                 continue; // padding column (uniform over the wave)
             }
-            const double4 pj = pos[S.slot0 + j];
-            const int gj = S.atom0 + static_cast<int>(perm[S.slot0 + j]);
+            const double4 pj = pos[Sd.slot0 + j];
+            const int gj = Sd.atom0 + static_cast<int>(perm[Sd.slot0 + j]);
             Dual dv[3];
             const double pd[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
             for (int d = 0; d < 3; d++) {
-                const double tan = vx ? vxi[d] - vx[gj * 3 + d] : 0.0;
+                const double tan = v.vx ? vxi[d] - v.vx[gj * 3 + d] : 0.0;
                 dv[d] = Dual(min_image_d(pd[d], L[d], 1.0 / L[d]), tan);
             }
             const int ofj = offset ? offset[gj] : 0;
-            const double dwdl = (ofi - ofj) * S.cutoff_d;
+            const double dwdl = (ofi - ofj) * Sd.cutoff_d;
             const Dual dw(pi.w - pj.w, dwdl * vl);
             const double d2 = __builtin_fma(dw.v, dw.v, pair_d2(dv[0].v, dv[1].v, dv[2].v));
             const bool in = vi && i < j && d2 < cut2;
@@ -343,9 +351,9 @@ __global__ __launch_bounds__(64) void k_tiles_hvp(
             }
             PairTangents tg = zero_tangents();
             if (in) {
-                const PairGradDual o =
-                    pair_grad_dual(dv, dw, dwdl, qi, dual_param(params, vp, gj, 0), si, dual_param(params, vp, gj, 1),
-                                   ei, dual_param(params, vp, gj, 2), beta, 1.0, 1.0);
+                const PairGradDual o = pair_grad_dual(dv, dw, dwdl, qi, dual_param(params, v.vp, gj, 0), si,
+                                                      dual_param(params, v.vp, gj, 1), ei,
+                                                      dual_param(params, v.vp, gj, 2), beta, 1.0, 1.0);
                 tg = pair_tangents(o, dv, false);
             }
             for (int d = 0; d < 3; d++) {
@@ -361,82 +369,79 @@ __global__ __launch_bounds__(64) void k_tiles_hvp(
             }
             if (lane == 0) {
                 for (int d = 0; d < 3; d++) {
-                    if (hx) {
-                        atomic_add_fixed(hx + gj * 3 + d, cs[d]);
+                    if (out.hx) {
+                        atomic_add_fixed(out.hx + gj * 3 + d, cs[d]);
                     }
-                    if (hp) {
-                        atomic_add_fixed(hp + gj * 3 + d, cs[3 + d]);
+                    if (out.hp) {
+                        atomic_add_fixed(out.hp + gj * 3 + d, cs[3 + d]);
                     }
                 }
             }
         }
         if (vi) {
             for (int d = 0; d < 3; d++) {
-                if (hx) {
-                    atomic_add_fixed(hx + gi * 3 + d, rx[d]);
+                if (out.hx) {
+                    atomic_add_fixed(out.hx + gi * 3 + d, rx[d]);
                 }
-                if (hp) {
-                    atomic_add_fixed(hp + gi * 3 + d, rp[d]);
+                if (out.hp) {
+                    atomic_add_fixed(out.hp + gi * 3 + d, rp[d]);
                 }
             }
         }
-    }
-    if (hl) {
-        wave_add_fixed(hl, acc_l);
+        if (out.hl) {
+            const uint64_t tot = wave_sum_u64(acc_l);
+            if (lane == 0) {
+                atomic_add_fixed(out.hl + Sd.red_idx, tot);
+            }
+        }
     }
 }
 
 } // namespace
 
-void launch_bonded_hvp(BondedKind kind, const BondedTerms &t, const double *x, const double *vx, const double *vp,
-                       double lambda, double vl, uint64_t *hx, uint64_t *hp, uint64_t *hl, hipStream_t stream) {
-    if (t.n == 0 || (!hx && !hp && !hl)) {
+void launch_bonded_hvp(BondedKind kind, const BondedTerms &t, const double *x, const double *lambdas, double lambda0,
+                       const int *red_idx, const HvpTangent &v, const HvpOut &o, hipStream_t stream) {
+    if (t.n == 0 || (!o.hx && !o.hp && !o.hl)) {
         return;
-    }
-    if (t.term_sys) {
-        throw Error("Hessian-vector products: one system only");
     }
     const int grid = ceil_div(t.n, 128);
     switch (kind) {
     case BondedKind::Bond:
-        k_bonded_hvp<0><<<grid, 128, 0, stream>>>(t, x, vx, vp, lambda, vl, hx, hp, hl);
+        k_bonded_hvp<0><<<grid, 128, 0, stream>>>(t, x, lambdas, lambda0, red_idx, v, o);
         break;
     case BondedKind::Angle:
-        k_bonded_hvp<1><<<grid, 128, 0, stream>>>(t, x, vx, vp, lambda, vl, hx, hp, hl);
+        k_bonded_hvp<1><<<grid, 128, 0, stream>>>(t, x, lambdas, lambda0, red_idx, v, o);
         break;
     case BondedKind::Torsion:
-        k_bonded_hvp<2><<<grid, 128, 0, stream>>>(t, x, vx, vp, lambda, vl, hx, hp, hl);
+        k_bonded_hvp<2><<<grid, 128, 0, stream>>>(t, x, lambdas, lambda0, red_idx, v, o);
         break;
     }
     TM_LAUNCH_CHECK();
 }
 
-void launch_pair_list_hvp(const PairListArgs &a, const double *vx, const double *vp, double vl, uint64_t *hx,
-                          uint64_t *hp, uint64_t *hl, hipStream_t stream) {
-    if (a.M == 0 || (!hx && !hp && !hl)) {
+void launch_pair_list_hvp(const PairListArgs &a, const HvpTangent &v, const HvpOut &o, hipStream_t stream) {
+    if (a.M == 0 || (!o.hx && !o.hp && !o.hl)) {
         return;
     }
-    if (a.params1 || a.pair_sys || a.lambdas || a.cutoffs || a.betas || a.slots) {
-        throw Error("Hessian-vector products: one system with plain parameters only");
+    if (a.params1 || a.slots) {
+        throw Error("Hessian-vector products: plain parameters only");
     }
-    k_pair_list_hvp<<<ceil_div(a.M, 256), 256, 0, stream>>>(a, vx, vp, vl, hx, hp, hl);
+    k_pair_list_hvp<<<ceil_div(a.M, 256), 256, 0, stream>>>(a, v, o);
     TM_LAUNCH_CHECK();
 }
 
 void launch_tiles_hvp(const NbState &st, const double *params, const int *offset, const double *boxes,
-                      const double *vx, const double *vp, double vl, uint64_t *hx, uint64_t *hp, uint64_t *hl,
-                      hipStream_t stream) {
-    if (!hx && !hp && !hl) {
+                      const HvpTangent &v, const HvpOut &o, hipStream_t stream) {
+    if (st.S == 0 || (!o.hx && !o.hp && !o.hl)) {
         return;
     }
-    if (st.S != 1 || st.any_interp) {
-        throw Error("Hessian-vector products: one system with plain parameters only");
+    if (st.any_interp) {
+        throw Error("Hessian-vector products: plain parameters only");
     }
     // one wave per tile, the grid striding over the tile count read on the device
-    const int grid = std::max(1, std::min(st.total_tiles, 8192));
-    k_tiles_hvp<<<grid, TILE, 0, stream>>>(st.sys.ptr, st.tile_prefix.ptr, st.tile_row.ptr, st.tile_atoms.ptr,
-                                            st.perm.ptr, st.pos.ptr, params, offset, boxes, st.status.ptr, vx, vp, vl,
-                                            hx, hp, hl);
+    const int grid = std::max(1, std::min(st.total_tiles, 16384));
+    k_tiles_hvp<<<grid, TILE, 0, stream>>>(st.sys.ptr, st.S, st.tile_prefix.ptr, st.tile_row.ptr, st.tile_atoms.ptr,
+                                            st.perm.ptr, st.pos.ptr, params, offset, boxes, st.status.ptr, v, o);
     TM_LAUNCH_CHECK();
 }
 

@@ -10,6 +10,14 @@
 
 namespace tmamd {
 
+static HvpTangent hvp_tangent(const double *vx, const double *vp, double vl) {
+    HvpTangent t;
+    t.vx = vx;
+    t.vp = vp;
+    t.vl = vl;
+    return t;
+}
+
 void Potential::du_dp_fixed_to_float(int N, int P, const uint64_t *du_dp, double *out) const {
     for (int i = 0; i < P; i++) {
         out[i] = fixed_to_double(du_dp[i], FIXED_SCALE);
@@ -395,7 +403,7 @@ void NonbondedAllPairs::execute_hvp_device(int N, int P, const double *d_x, cons
         return;
     }
     st_.prepare(d_x, d_p, nullptr, d_box, d_plane_.ptr, d_offset_.ptr, nullptr, lambda, stream);
-    launch_tiles_hvp(st_, d_p, d_offset_.ptr, d_box, d_vx, d_vp, vl, d_hx, d_hp, d_hl, stream);
+    launch_tiles_hvp(st_, d_p, d_offset_.ptr, d_box, hvp_tangent(d_vx, d_vp, vl), HvpOut{d_hx, d_hp, d_hl}, stream);
 }
 
 void NonbondedAllPairs::check_device_status(hipStream_t stream) {
@@ -541,7 +549,7 @@ void NonbondedPairList::execute_hvp_device(int N, int P, const double *d_x, cons
     if (interp_) {
         throw Error("NonbondedPairList::execute_hvp(): interpolated parameters are not supported");
     }
-    launch_pair_list_hvp(a, d_vx, d_vp, vl, d_hx, d_hp, d_hl, stream);
+    launch_pair_list_hvp(a, hvp_tangent(d_vx, d_vp, vl), HvpOut{d_hx, d_hp, d_hl}, stream);
 }
 
 PairListArgs NonbondedPairList::args(int N, int P, const double *d_x, const double *d_p, const double *d_box,
@@ -678,7 +686,8 @@ void BondedPotential::execute_hvp_device(int N, int P, const double *d_x, const 
     }
     execute_device(N, P, d_x, d_p, d_box, lambda, nullptr, nullptr, nullptr, nullptr, stream); // size validation
     BondedTerms t{n_, d_idxs_.ptr, d_p, d_mult_.ptr, d_offset_.ptr, nullptr};
-    launch_bonded_hvp(bk_, t, d_x, d_vx, d_vp, lambda, vl, d_hx, d_hp, d_hl, stream);
+    launch_bonded_hvp(bk_, t, d_x, nullptr, lambda, nullptr, hvp_tangent(d_vx, d_vp, vl), HvpOut{d_hx, d_hp, d_hl},
+                      stream);
 }
 
 // ---------------------------------------------------------------- sums

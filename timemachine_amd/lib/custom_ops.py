@@ -1171,3 +1171,40 @@ class BatchedContexts(_Handle):
         call("tm_batch_grad_stats", self._h, int(s), ctypes.byref(n), ctypes.byref(a), ctypes.byref(aa), pd(g),
              pd(ag), pd(h))
         return {"n": n.value, "sum_a": a.value, "sum_aa": aa.value, "sum_g": g, "sum_ag": ag, "sum_h": h}
+
+    # -- trajectory adjoint (extension): reverse mode through the Langevin steps
+
+    def adjoint_forward(self, n_steps: int, du_dl_interval: int, checkpoint_interval: int) -> np.ndarray:
+        """Run n_steps of every system (as multiple_steps without a lambda
+        schedule) keeping checkpoints every checkpoint_interval steps, for
+        adjoint_backward. Returns the stored du_dl [S, n_steps // du_dl_interval]."""
+        F = n_steps // du_dl_interval if du_dl_interval > 0 else 0
+        out = np.zeros((len(self), max(F, 1)), np.float64)
+        call("tm_batch_adjoint_forward", self._h, int(n_steps), int(du_dl_interval), int(checkpoint_interval),
+             pd(out))
+        self._adj_frames = F
+        return out[:, :F].copy()
+
+    def adjoint_backward(self, dl_bar, want_initial_state: bool = False):
+        """Gradient of a loss on the du_dl values of the last adjoint_forward,
+        given its cotangent dl_bar [S, F]: a list with each system's gradient
+        in its flat parameters (grad_size(s) each, grad_stats order), and with
+        want_initial_state also the cotangents of the initial coordinates and
+        velocities ([total atoms, 3] each)."""
+        S = len(self)
+        F = getattr(self, "_adj_frames", None)
+        if F is None:
+            raise RuntimeError("Engine::adjoint_backward: call adjoint_forward first")
+        dl_bar = f64(dl_bar)
+        if dl_bar.size != S * F:
+            raise RuntimeError(f"adjoint_backward: dl_bar needs {S} x {F} values (systems x stored du_dl frames)")
+        dl_bar = dl_bar.reshape(S, F)
+        sizes = [self.grad_size(s) for s in range(S)]
+        grad = np.zeros(max(sum(sizes), 1))
+        A = int(sum(self.sizes))
+        xb = np.zeros((A, 3)) if want_initial_state else None
+        vb = np.zeros((A, 3)) if want_initial_state else None
+        call("tm_batch_adjoint_backward", self._h, pd(np.ascontiguousarray(dl_bar)), pd(grad), pd(xb), pd(vb))
+        offs = np.cumsum([0] + sizes)
+        grads = [grad[offs[s]:offs[s + 1]].copy() for s in range(S)]
+        return (grads, xb, vb) if want_initial_state else grads
