@@ -192,3 +192,54 @@ def test_adjoint_error_paths(system):
         b.adjoint_backward(np.ones((1, 1)))
     with pytest.raises(RuntimeError, match="multiple"):
         b.adjoint_forward(10, 3, 4)
+
+
+@pytest.mark.parametrize("recompute", [True, False])
+def test_ti_loss_adjoint_method_matches_fp64_backprop(system, recompute):
+    """The torch.autograd.Function over lambda windows with method="adjoint":
+    the gradient of L = (sum_w c_w <du/dlambda>_w - target)^2 (equilibration
+    steps included in the differentiated run) in the nonbonded parameters and
+    torsion k, against FP64 backpropagation through each window's trajectory
+    with its own noise, summed over the windows."""
+    import torch
+
+    from timemachine_amd.fe import window_gradients as wg
+
+    s = system
+    lams = [0.25, 0.75]
+    n_frames, interval, equil = 4, 3, 6
+    windows = wg.LambdaWindows(s, lams, n_frames, interval, equil_steps=equil, dt=0.0015, seed=11)
+    nb = torch.tensor(s.nb_params, dtype=torch.float64, requires_grad=True)
+    tk = torch.tensor(s.torsion_params[:, 0], dtype=torch.float64, requires_grad=True)
+    target = -3.0
+    loss = wg.ti_loss(nb, tk, windows, target, recompute=recompute, method="adjoint")
+    g_nb, g_tk = torch.autograd.grad(loss, (nb, tk))
+    # oracle
+    ca, cb, cc = ointeg.langevin_coefficients(windows.temperature, windows.dt, windows.friction, s.masses)
+    energy = _oracle_energy(s)
+    c = wg.trapezoid_weights(lams)
+    n_steps = equil + n_frames * interval
+    means, ref_grads = [], []
+    for w, lam in enumerate(lams):
+        seed = windows.seed + 7919 * w
+        noise = np.stack([ophilox.langevin_noise(seed, j, s.N) for j in range(n_steps)])
+        dls = otraj.trajectory_du_dl(energy, windows.x0, windows.v0[w], _blocks(s), lam, n_steps, interval,
+                                     windows.dt, ca, -cb, cc, noise)
+        means.append(dls[equil // interval:].mean())
+    lbar = 2.0 * (float(np.dot(c, means)) - target)
+    for w, lam in enumerate(lams):
+        seed = windows.seed + 7919 * w
+        noise = np.stack([ophilox.langevin_noise(seed, j, s.N) for j in range(n_steps)])
+        dl_bar = np.zeros(n_steps // interval)
+        dl_bar[equil // interval:] = lbar * c[w] / n_frames
+        _, grads, _, _ = otraj.trajectory_loss_grad(energy, windows.x0, windows.v0[w], _blocks(s), lam, n_steps,
+                                                    interval, windows.dt, ca, -cb, cc, dl_bar, noise)
+        ref_grads.append(grads)
+    r_nb = sum(g[0] for g in ref_grads)
+    r_tk = sum(g[3][:, 0] for g in ref_grads)
+    assert abs(float(loss) - (float(np.dot(c, means)) - target) ** 2) <= 1e-5 * max(float(loss), 1.0)
+    for name, got, ref in [("nonbonded", g_nb.numpy(), r_nb), ("torsion k", g_tk.numpy(), r_tk)]:
+        err = np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-12)
+        print(f"[ti_loss adjoint recompute={recompute}] d loss / d {name}: max err / max|ref| = {err:.2e}",
+              flush=True)
+        assert err <= 1e-4

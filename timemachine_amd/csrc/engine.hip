@@ -9,6 +9,7 @@
 #include "fire.h"
 #include "fixed_point.h"
 #include "philox.h"
+#include "wave.h"
 
 namespace tmamd {
 
@@ -346,21 +347,33 @@ __global__ void k_adj_pre(int A, const int *__restrict__ atom_sys, const double 
                           const double *__restrict__ xb, double *__restrict__ vb, double *__restrict__ gb,
                           unsigned long long *__restrict__ smax) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= A) {
-        return;
-    }
-    const int s = atom_sys[a];
-    const double c = ca[s], h = 0.5 * dt[s], cb = cbs[a];
+    const bool act = a < A;
+    const int s = act ? atom_sys[a] : -1;
     double m = 0.0;
-    for (int d = 0; d < 3; d++) {
-        const double vm = c * vb[a * 3 + d] + h * (1.0 + c) * xb[a * 3 + d];
-        vb[a * 3 + d] = vm;
-        const double g = cb * vm;
-        gb[a * 3 + d] = g;
-        m = fmax(m, fabs(g));
+    if (act) {
+        const double c = ca[s], h = 0.5 * dt[s], cb = cbs[a];
+        for (int d = 0; d < 3; d++) {
+            const double vm = c * vb[a * 3 + d] + h * (1.0 + c) * xb[a * 3 + d];
+            vb[a * 3 + d] = vm;
+            const double g = cb * vm;
+            gb[a * 3 + d] = g;
+            m = fmax(m, fabs(g));
+        }
     }
-    if (m > 0.0) {
-        atomicMax(smax + s, static_cast<unsigned long long>(__double_as_longlong(m)));
+    // one atomic per wave when its atoms belong to one system (atoms of a
+    // system are contiguous), else one per atom
+    uint64_t bits = static_cast<uint64_t>(__double_as_longlong(m));
+    const int s0 = __builtin_amdgcn_readfirstlane(s);
+    if (__all(!act || s == s0)) {
+        for (int o = 32; o >= 1; o >>= 1) {
+            const uint64_t other = shfl_xor_u64(bits, o);
+            bits = other > bits ? other : bits;
+        }
+        if (lane_id() == 0 && s0 >= 0 && bits != 0) {
+            atomicMax(smax + s0, static_cast<unsigned long long>(bits));
+        }
+    } else if (act && bits != 0) {
+        atomicMax(smax + s, static_cast<unsigned long long>(bits));
     }
 }
 

@@ -275,15 +275,38 @@ __global__ __launch_bounds__(256) void k_pair_list_hvp(const PairListArgs a, con
 }
 
 // All pairs of the tile lists of the last neighborlist build (one or many
-// systems): one wave per tile, lane l holding row slot 64 r + l, the tile's 64
-// column atoms visited in turn (the tiles, pair test and cutoff of the force
-// kernel). Row terms stay in registers until the tile ends; column terms are
-// summed over the wave (exact fixed-point sums) before one atomic per value.
+// systems), one wave per tile, in the force kernel's two passes: the screen
+// pairs row lane l with column slot (l + r) mod 64 in round r and compacts the
+// pairs inside the cutoff (FP64 test, as the force kernel) into an LDS queue
+// (ballot / mbcnt); every 64 queued pairs are evaluated one per lane, their
+// fixed-point tangents added to per-atom LDS accumulators of the tile's rows
+// and columns, which go to global memory once per tile and atom.
+struct HvpTileLds {
+    // [0] rows, [1] columns: position (x, y, z, w), parameters at their FP32
+    // values, tangents, lambda offset, sorted slot and global atom index
+    double px[2][TILE], py[2][TILE], pz[2][TILE], pw[2][TILE];
+    double q[2][TILE], sg[2][TILE], ep[2][TILE];
+    double tq[2][TILE], ts[2][TILE], te[2][TILE];
+    double tx[2][TILE], ty[2][TILE], tz[2][TILE];
+    int off[2][TILE];
+    uint32_t slot[2][TILE];
+    int gidx[2][TILE];
+    uint64_t acc[2][6][TILE]; // du_dx (3) and du_dp (3) tangents of each row / column atom
+    uint16_t pq[2 * TILE];    // queued pair: row lane | column lane << 6
+};
+
+__device__ __forceinline__ void lds_add(uint64_t *p, uint64_t v) {
+    if (v != 0) {
+        atomicAdd(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(v));
+    }
+}
+
 __global__ __launch_bounds__(64) void k_tiles_hvp(
     const NbSys *__restrict__ sys, int S, const int *__restrict__ tile_prefix, const int *__restrict__ tile_row,
     const uint32_t *__restrict__ tile_atoms, const uint32_t *__restrict__ perm, const double4 *__restrict__ pos,
     const double *__restrict__ params, const int *__restrict__ offset, const double *__restrict__ boxes,
     const int *__restrict__ status, const HvpTangent v, const HvpOut out) {
+    __shared__ HvpTileLds L;
     const int total = tile_prefix[S];
     const int lane = threadIdx.x;
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
@@ -301,90 +324,125 @@ __global__ __launch_bounds__(64) void k_tiles_hvp(
             continue;
         }
         const NbSys Sd = sys[s];
-        const double L[3] = {boxes[s * 9 + 0], boxes[s * 9 + 4], boxes[s * 9 + 8]};
+        const double bx = boxes[s * 9 + 0], by = boxes[s * 9 + 4], bz = boxes[s * 9 + 8];
+        const double ibx = 1.0 / bx, iby = 1.0 / by, ibz = 1.0 / bz;
         const double beta = static_cast<double>(Sd.beta);
         const double cut2 = Sd.cutoff_d * Sd.cutoff_d;
         const double vl = v.vls ? v.vls[s] : v.vl;
         const int kt = Sd.tile0 + (t - tile_prefix[s]);
         const int r = tile_row[kt];
-        const uint32_t i = static_cast<uint32_t>(r * TILE + lane);
-        const bool vi = i < static_cast<uint32_t>(Sd.row_end);
-        double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
-        int gi = 0, ofi = 0;
-        Dual qi, si, ei;
-        double vxi[3] = {0.0, 0.0, 0.0};
-        if (vi) {
-            pi = pos[Sd.slot0 + i];
-            gi = Sd.atom0 + static_cast<int>(perm[Sd.slot0 + i]);
-            ofi = offset ? offset[gi] : 0;
-            qi = dual_param(params, v.vp, gi, 0);
-            si = dual_param(params, v.vp, gi, 1);
-            ei = dual_param(params, v.vp, gi, 2);
-            if (v.vx) {
-                for (int d = 0; d < 3; d++) {
-                    vxi[d] = v.vx[gi * 3 + d];
-                }
+
+        // stage the tile's rows and columns
+        wave_lds_sync(); // the previous tile's LDS reads are done
+        for (int side = 0; side < 2; side++) {
+            const uint32_t sl = side == 0 ? static_cast<uint32_t>(r * TILE + lane)
+                                          : tile_atoms[static_cast<size_t>(kt) * TILE + lane];
+            const bool valid = side == 0 ? sl < static_cast<uint32_t>(Sd.row_end)
+                                         : sl < static_cast<uint32_t>(Sd.slot_end);
+            double4 p4 = make_double4(0.0, 0.0, 0.0, 0.0);
+            int g = 0;
+            if (valid) {
+                p4 = pos[Sd.slot0 + sl];
+                g = Sd.atom0 + static_cast<int>(perm[Sd.slot0 + sl]);
+            }
+            L.px[side][lane] = p4.x;
+            L.py[side][lane] = p4.y;
+            L.pz[side][lane] = p4.z;
+            L.pw[side][lane] = p4.w;
+            L.q[side][lane] = valid ? static_cast<double>(static_cast<float>(params[g * 3 + 0])) : 0.0;
+            L.sg[side][lane] = valid ? static_cast<double>(static_cast<float>(params[g * 3 + 1])) : 0.0;
+            L.ep[side][lane] = valid ? static_cast<double>(static_cast<float>(params[g * 3 + 2])) : 0.0;
+            L.tq[side][lane] = (valid && v.vp) ? v.vp[g * 3 + 0] : 0.0;
+            L.ts[side][lane] = (valid && v.vp) ? v.vp[g * 3 + 1] : 0.0;
+            L.te[side][lane] = (valid && v.vp) ? v.vp[g * 3 + 2] : 0.0;
+            L.tx[side][lane] = (valid && v.vx) ? v.vx[g * 3 + 0] : 0.0;
+            L.ty[side][lane] = (valid && v.vx) ? v.vx[g * 3 + 1] : 0.0;
+            L.tz[side][lane] = (valid && v.vx) ? v.vx[g * 3 + 2] : 0.0;
+            L.off[side][lane] = (valid && offset) ? offset[g] : 0;
+            L.slot[side][lane] = valid ? sl : NB_INVALID;
+            L.gidx[side][lane] = g;
+            for (int k = 0; k < 6; k++) {
+                L.acc[side][k][lane] = 0;
             }
         }
-        uint64_t rx[3] = {0, 0, 0}, rp[3] = {0, 0, 0}, acc_l = 0;
-        const uint32_t jl = tile_atoms[static_cast<size_t>(kt) * TILE + lane];
-        for (int c = 0; c < TILE; c++) {
-            const uint32_t j = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(jl), c));
-            if (j >= static_cast<uint32_t>(Sd.slot_end)) {
-                continue; // padding column (uniform over the wave)
+        wave_lds_sync();
+
+        uint64_t acc_l = 0;
+        // pass 2: the first n queued pairs, one per lane
+        auto eval = [&](int n) {
+            if (lane < n) {
+                const int e = L.pq[lane];
+                const int il = e & 63, jl = e >> 6;
+                Dual dv[3];
+                dv[0] = Dual(min_image_d(L.px[0][il] - L.px[1][jl], bx, ibx), L.tx[0][il] - L.tx[1][jl]);
+                dv[1] = Dual(min_image_d(L.py[0][il] - L.py[1][jl], by, iby), L.ty[0][il] - L.ty[1][jl]);
+                dv[2] = Dual(min_image_d(L.pz[0][il] - L.pz[1][jl], bz, ibz), L.tz[0][il] - L.tz[1][jl]);
+                const double dwdl = (L.off[0][il] - L.off[1][jl]) * Sd.cutoff_d;
+                const Dual dw(L.pw[0][il] - L.pw[1][jl], dwdl * vl);
+                const PairGradDual o = pair_grad_dual(
+                    dv, dw, dwdl, Dual(L.q[0][il], L.tq[0][il]), Dual(L.q[1][jl], L.tq[1][jl]),
+                    Dual(L.sg[0][il], L.ts[0][il]), Dual(L.sg[1][jl], L.ts[1][jl]), Dual(L.ep[0][il], L.te[0][il]),
+                    Dual(L.ep[1][jl], L.te[1][jl]), beta, 1.0, 1.0);
+                const PairTangents tg = pair_tangents(o, dv, false);
+                for (int k = 0; k < 3; k++) {
+                    lds_add(&L.acc[0][k][il], tg.xi[k]);
+                    lds_add(&L.acc[1][k][jl], tg.xj[k]);
+                    lds_add(&L.acc[0][3 + k][il], tg.pi[k]);
+                    lds_add(&L.acc[1][3 + k][jl], tg.pj[k]);
+                }
+                acc_l += tg.l;
             }
-            const double4 pj = pos[Sd.slot0 + j];
-            const int gj = Sd.atom0 + static_cast<int>(perm[Sd.slot0 + j]);
-            Dual dv[3];
-            const double pd[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
-            for (int d = 0; d < 3; d++) {
-                const double tan = v.vx ? vxi[d] - v.vx[gj * 3 + d] : 0.0;
-                dv[d] = Dual(min_image_d(pd[d], L[d], 1.0 / L[d]), tan);
+        };
+        int queued = 0;
+        const bool vi = L.slot[0][lane] != NB_INVALID;
+        const uint32_t islot = L.slot[0][lane];
+        const double xi = L.px[0][lane], yi = L.py[0][lane], zi = L.pz[0][lane], wi = L.pw[0][lane];
+        for (int round = 0; round < TILE; round++) {
+            const int c = (lane + round) & (TILE - 1);
+            const uint32_t jslot = L.slot[1][c];
+            bool in = false;
+            if (vi && jslot != NB_INVALID && islot < jslot) {
+                const double dx = min_image_d(xi - L.px[1][c], bx, ibx);
+                const double dy = min_image_d(yi - L.py[1][c], by, iby);
+                const double dz = min_image_d(zi - L.pz[1][c], bz, ibz);
+                const double dw = wi - L.pw[1][c];
+                in = __builtin_fma(dw, dw, pair_d2(dx, dy, dz)) < cut2;
             }
-            const int ofj = offset ? offset[gj] : 0;
-            const double dwdl = (ofi - ofj) * Sd.cutoff_d;
-            const Dual dw(pi.w - pj.w, dwdl * vl);
-            const double d2 = __builtin_fma(dw.v, dw.v, pair_d2(dv[0].v, dv[1].v, dv[2].v));
-            const bool in = vi && i < j && d2 < cut2;
-            if (ballot(in) == 0) {
-                continue; // uniform
-            }
-            PairTangents tg = zero_tangents();
+            const uint64_t m = ballot(in);
             if (in) {
-                const PairGradDual o = pair_grad_dual(dv, dw, dwdl, qi, dual_param(params, v.vp, gj, 0), si,
-                                                      dual_param(params, v.vp, gj, 1), ei,
-                                                      dual_param(params, v.vp, gj, 2), beta, 1.0, 1.0);
-                tg = pair_tangents(o, dv, false);
+                const int at = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), queued));
+                L.pq[at] = static_cast<uint16_t>(lane | (c << 6));
             }
-            for (int d = 0; d < 3; d++) {
-                rx[d] += tg.xi[d];
-                rp[d] += tg.pi[d];
+            queued += __popcll(m);
+            if (queued >= TILE) {
+                wave_lds_sync();
+                eval(TILE);
+                wave_lds_sync(); // pass 2 has read the first 64 entries
+                queued -= TILE;
+                const uint16_t rest = L.pq[TILE + lane];
+                if (lane < queued) {
+                    L.pq[lane] = rest;
+                }
+                wave_lds_sync();
             }
-            acc_l += tg.l;
-            // column atom j: one atomic per value for the whole wave
-            uint64_t cs[6];
-            for (int d = 0; d < 3; d++) {
-                cs[d] = wave_sum_u64(tg.xj[d]);
-                cs[3 + d] = wave_sum_u64(tg.pj[d]);
-            }
-            if (lane == 0) {
-                for (int d = 0; d < 3; d++) {
+        }
+        if (queued > 0) {
+            wave_lds_sync();
+            eval(queued);
+        }
+        wave_lds_sync();
+        // the tile's per-atom tangents to global memory
+        for (int side = 0; side < 2; side++) {
+            if (L.slot[side][lane] != NB_INVALID) {
+                const int g = L.gidx[side][lane];
+                for (int k = 0; k < 3; k++) {
                     if (out.hx) {
-                        atomic_add_fixed(out.hx + gj * 3 + d, cs[d]);
+                        atomic_add_fixed(out.hx + g * 3 + k, L.acc[side][k][lane]);
                     }
                     if (out.hp) {
-                        atomic_add_fixed(out.hp + gj * 3 + d, cs[3 + d]);
+                        atomic_add_fixed(out.hp + g * 3 + k, L.acc[side][3 + k][lane]);
                     }
-                }
-            }
-        }
-        if (vi) {
-            for (int d = 0; d < 3; d++) {
-                if (out.hx) {
-                    atomic_add_fixed(out.hx + gi * 3 + d, rx[d]);
-                }
-                if (out.hp) {
-                    atomic_add_fixed(out.hp + gi * 3 + d, rp[d]);
                 }
             }
         }

@@ -19,6 +19,14 @@ The estimator is the derivative of the reweighted average
 sum_f A_f(p) w_f(p) / sum_f w_f(p), w_f = exp(-(U_f(p) - U_f(p0)) / kT), at
 p = p0, which is what the finite-difference test checks.
 
+The pathwise alternative (method="adjoint") differentiates the sampled
+trajectories themselves: reverse mode through every Langevin step, one
+Hessian-vector product of the energy per step, frames recomputed from
+checkpoints (the route of attic/docs/paper.tex:27). It is exact for the
+finite run at hand; its variance grows with the run length (the trajectory's
+sensitivity to its own past grows with its Lyapunov exponent), so it suits
+short horizons, and the covariance estimator the long-run ensemble average.
+
 The reference differentiates through JAX (timemachine/fe/functional.py:18-119
 wraps the custom ops in a custom_jvp); its free-energy loss pipeline
 (timemachine/fe/estimator.py) has no counterpart here beyond TI.
@@ -166,6 +174,40 @@ class LambdaWindows:
         self.last_device_bytes = batch.device_bytes()
         return batch, []
 
+    def checkpoint_interval(self) -> int:
+        """Steps between the adjoint's checkpoints: about the square root of the
+        run (memory and recomputation both O(sqrt(steps))), a multiple of the
+        frame interval."""
+        total = self.equil_steps + self.n_frames * self.frame_interval
+        k = max(1, int(round(np.sqrt(total) / self.frame_interval)))
+        return k * self.frame_interval
+
+    def adjoint_run(self, nb_params=None, torsion_params=None):
+        """The trajectory adjoint's forward pass: the equilibration and sampled
+        steps in one run with checkpoints (du/dlambda stored every
+        frame_interval steps from the start). Returns (batch, du_dl series of the
+        sampled frames [W, n_frames]); the batch holds the checkpoints for
+        adjoint_gradients."""
+        if self.barostat_interval > 0:
+            raise RuntimeError("the trajectory adjoint needs constant volume (no barostat)")
+        if self.equil_steps % self.frame_interval != 0:
+            raise RuntimeError("the trajectory adjoint needs equil_steps to be a multiple of frame_interval")
+        batch, _ = self.batch(nb_params, torsion_params)
+        total = self.equil_steps + self.n_frames * self.frame_interval
+        du_dl = batch.adjoint_forward(total, self.frame_interval, self.checkpoint_interval())
+        return batch, du_dl[:, self.equil_steps // self.frame_interval:]
+
+    def adjoint_gradients(self, batch, grad_means: np.ndarray) -> np.ndarray:
+        """Pathwise gradient (flat parameters of one system) of
+        sum_w grad_means[w] <du/dlambda>_w through the trajectories of the last
+        adjoint_run: the windows share the parameters, so their gradients add."""
+        e = self.equil_steps // self.frame_interval
+        W = len(self.lambdas)
+        dl_bar = np.zeros((W, e + self.n_frames))
+        dl_bar[:, e:] = np.asarray(grad_means, dtype=np.float64)[:, None] / self.n_frames
+        grads = batch.adjoint_backward(dl_bar)
+        return np.sum(grads, axis=0)
+
     def sample(self, nb_params=None, torsion_params=None, observe: bool = True):
         """(mean du/dlambda per window [W], du/dlambda series [W, n_frames],
         observer sums per window or None)."""
@@ -184,39 +226,66 @@ class LambdaWindows:
 class WindowMeanDuDl(torch.autograd.Function):
     """<du/dlambda>_w for every window as a differentiable function of the
     nonbonded parameters [N, 3] (q, sig/2, sqrt(eps) convention) and the
-    torsion force constants [T]."""
+    torsion force constants [T].
+
+    method "covariance": the thermodynamic gradient above (frames observed on
+    the device; with recompute the backward pass re-runs the windows).
+    method "adjoint": the pathwise derivative of the trajectory averages,
+    reverse mode through every Langevin step (Engine trajectory adjoint: one
+    Hessian-vector product per step, frames recomputed from checkpoints). With
+    recompute the forward pass is a plain run and the backward pass repeats it
+    with checkpoints; without, the forward pass keeps the checkpoints on the
+    device until the backward pass. Constant volume only."""
 
     @staticmethod
-    def forward(ctx, nb_params, torsion_k, windows: LambdaWindows, recompute: bool = True):
+    def forward(ctx, nb_params, torsion_k, windows: LambdaWindows, recompute: bool = True,
+                method: str = "covariance"):
+        if method not in ("covariance", "adjoint"):
+            raise ValueError(f"unknown gradient method {method!r}")
         nb = nb_params.detach().cpu().numpy().astype(np.float64)
         tp = np.array(windows.system.torsion_params, dtype=np.float64)
         tp[:, 0] = torsion_k.detach().cpu().numpy()
-        means, _, stats = windows.sample(nb, tp, observe=not recompute)
         ctx.windows = windows
         ctx.params = (nb, tp)
-        ctx.stats = stats
+        ctx.method = method
+        ctx.stats = None
+        ctx.batch = None
+        if method == "adjoint" and not recompute:
+            ctx.batch, du_dl = windows.adjoint_run(nb, tp)
+            means = du_dl.mean(axis=1)
+        else:
+            means, _, stats = windows.sample(nb, tp, observe=(method == "covariance" and not recompute))
+            ctx.stats = stats
         return torch.as_tensor(means, dtype=torch.float64)
 
     @staticmethod
     def backward(ctx, grad_out):
         windows: LambdaWindows = ctx.windows
-        stats = ctx.stats
-        if stats is None:
-            # recompute the frames from the checkpoint, observing this time
-            _, _, stats = windows.sample(*ctx.params, observe=True)
         go = grad_out.detach().cpu().numpy()
-        total = np.zeros(windows.layout.size)
-        for w, st in enumerate(stats):
-            total += go[w] * window_gradient(st, windows.kT)
+        if ctx.method == "adjoint":
+            batch = ctx.batch
+            if batch is None:
+                batch, _ = windows.adjoint_run(*ctx.params)  # recompute with checkpoints
+            total = windows.adjoint_gradients(batch, go)
+            ctx.batch = None
+            del batch
+        else:
+            stats = ctx.stats
+            if stats is None:
+                # recompute the frames from the checkpoint, observing this time
+                _, _, stats = windows.sample(*ctx.params, observe=True)
+            total = np.zeros(windows.layout.size)
+            for w, st in enumerate(stats):
+                total += go[w] * window_gradient(st, windows.kT)
         parts = windows.layout.split(total)
         g_nb = torch.as_tensor(parts["nonbonded"].copy(), dtype=torch.float64)
         g_tk = torch.as_tensor(parts["torsion"][:, 0].copy(), dtype=torch.float64)
-        return g_nb, g_tk, None, None
+        return g_nb, g_tk, None, None, None
 
 
 def ti_loss(nb_params: torch.Tensor, torsion_k: torch.Tensor, windows: LambdaWindows, target: float,
-            recompute: bool = True) -> torch.Tensor:
+            recompute: bool = True, method: str = "covariance") -> torch.Tensor:
     """L = (sum_w c_w <du/dlambda>_w - target)^2 with trapezoid weights c_w."""
-    means = WindowMeanDuDl.apply(nb_params, torsion_k, windows, recompute)
+    means = WindowMeanDuDl.apply(nb_params, torsion_k, windows, recompute, method)
     c = torch.as_tensor(trapezoid_weights(windows.lambdas), dtype=torch.float64)
     return (torch.sum(c * means) - target) ** 2
