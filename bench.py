@@ -284,6 +284,55 @@ def differentiation_leg(sys_, n_windows: int = 16, n_frames: int = 500, frame_in
     g = nb.grad.numpy()
     grads = {"q": g[lig, 0], "sig": g[lig, 1], "eps": g[lig, 2], "torsion_k": tk.grad.numpy()}
 
+    # the same loss through the trajectory adjoint (pathwise derivative of the
+    # same runs: reverse mode through every step, one Hessian-vector product per
+    # step, frames recomputed from checkpoints kept by the forward pass)
+    nb_a = torch.tensor(sys_.nb_params, dtype=torch.float64, requires_grad=True)
+    tk_a = torch.tensor(sys_.torsion_params[:, 0], dtype=torch.float64, requires_grad=True)
+    log(f"differentiation: trajectory adjoint, checkpoints every {windows.checkpoint_interval()} steps")
+    t0 = time.perf_counter()
+    loss_a = wg.ti_loss(nb_a, tk_a, windows, target, recompute=False, method="adjoint")
+    t_fwd_a = time.perf_counter() - t0
+    loss_a.backward()
+    t_total_a = time.perf_counter() - t0
+    ga = nb_a.grad.numpy()
+    grads_a = {"q": ga[lig, 0], "sig": ga[lig, 1], "eps": ga[lig, 2], "torsion_k": tk_a.grad.numpy()}
+    flat_c = np.concatenate([v.ravel() for v in grads.values()])
+    flat_a = np.concatenate([v.ravel() for v in grads_a.values()])
+    adjoint = {
+        "loss": float(loss_a.detach()),
+        "loss_equal_to_covariance_run": bool(float(loss_a.detach()) == float(loss.detach())),
+        "forward_s": t_fwd_a,
+        "forward_backward_s": t_total_a,
+        "steps": windows.equil_steps + windows.n_frames * windows.frame_interval,
+        "checkpoint_interval": windows.checkpoint_interval(),
+        "gradient_norms": {k: float(np.linalg.norm(v)) for k, v in grads_a.items()},
+        "cosine_to_covariance_gradient": float(np.dot(flat_a, flat_c) /
+                                               max(np.linalg.norm(flat_a) * np.linalg.norm(flat_c), 1e-300)),
+    }
+    log(f"  adjoint: forward {t_fwd_a:.2f} s, forward + backward {t_total_a:.2f} s, "
+        f"cosine to the covariance gradient {adjoint['cosine_to_covariance_gradient']:.3f}")
+    # truncated sweep: each stored value's sensitivity followed back 100 steps (0.25 ps)
+    windows.adjoint_horizon = 100
+    nb_t = torch.tensor(sys_.nb_params, dtype=torch.float64, requires_grad=True)
+    tk_t = torch.tensor(sys_.torsion_params[:, 0], dtype=torch.float64, requires_grad=True)
+    t0 = time.perf_counter()
+    wg.ti_loss(nb_t, tk_t, windows, target, recompute=False, method="adjoint").backward()
+    t_trunc = time.perf_counter() - t0
+    windows.adjoint_horizon = 0
+    gt = nb_t.grad.numpy()
+    grads_t = {"q": gt[lig, 0], "sig": gt[lig, 1], "eps": gt[lig, 2], "torsion_k": tk_t.grad.numpy()}
+    flat_t = np.concatenate([v.ravel() for v in grads_t.values()])
+    adjoint["truncated"] = {
+        "horizon_steps": 100,
+        "forward_backward_s": t_trunc,
+        "gradient_norms": {k: float(np.linalg.norm(v)) for k, v in grads_t.items()},
+        "cosine_to_covariance_gradient": float(np.dot(flat_t, flat_c) /
+                                               max(np.linalg.norm(flat_t) * np.linalg.norm(flat_c), 1e-300)),
+    }
+    log(f"  adjoint truncated to 100 steps: {t_trunc:.2f} s, cosine to the covariance gradient "
+        f"{adjoint['truncated']['cosine_to_covariance_gradient']:.3f}")
+
     # finite differences on the first fd_frames observed frames of every window
     fdw = wg.LambdaWindows(sys_, lams, n_frames=fd_frames, frame_interval=frame_interval, seed=seed,
                            minimize_steps=99)
@@ -355,6 +404,7 @@ def differentiation_leg(sys_, n_windows: int = 16, n_frames: int = 500, frame_in
         "fd_h": h,
         "fd_checks": checks,
         "fd_max_rel_err": max(ch["rel_err"] for ch in checks),
+        "adjoint": adjoint,
     }
 
 
@@ -382,6 +432,9 @@ def main() -> int:
                     "(for rocprofv3 counter runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="one 256 MiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--adjoint-only", type=int, default=0, metavar="STEPS",
+                    help="trajectory adjoint of 16 lambda windows of the ligand system over STEPS steps "
+                    "(forward with checkpoints, backward), print its timing and exit (for rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -400,6 +453,24 @@ def main() -> int:
         dst.copy_(src)
         torch.cuda.synchronize()
         log(f"copied {n} bytes ({n // 1024} KiB read, {n // 1024} KiB written)")
+        return 0
+
+    if args.adjoint_only:
+        from timemachine_amd import testsystems as ts
+        from timemachine_amd.fe import window_gradients as wg
+
+        sys_a = ts.ligand_in_water()
+        n = args.adjoint_only - args.adjoint_only % 10
+        windows = wg.LambdaWindows(sys_a, np.linspace(0.0, 1.0, 16), n_frames=max(n // 10, 1), frame_interval=10,
+                                   seed=2024, minimize_steps=99)
+        t0 = time.perf_counter()
+        batch, du_dl = windows.adjoint_run()
+        t1 = time.perf_counter()
+        windows.adjoint_gradients(batch, np.ones(16))
+        t2 = time.perf_counter()
+        steps = windows.n_frames * windows.frame_interval
+        log(f"adjoint of 16 x {sys_a.N} atoms over {steps} steps: forward {(t1 - t0) / steps * 1e3:.3f} ms/step, "
+            f"backward {(t2 - t1) / steps * 1e3:.3f} ms/step")
         return 0
 
     stream = small_stream = None

@@ -361,8 +361,10 @@ int tm_batch_grad_stats(tm_batch *b, int s, int *n_frames, double *sum_a, double
  * available with a barostat, interpolated parameters or an interaction group. */
 int tm_batch_adjoint_forward(tm_batch *b, int n_steps, int du_dl_interval, int checkpoint_interval,
                              double *du_dl_out);
+/* horizon > 0: truncated reverse sweep, the state cotangents cleared every `horizon` steps
+ * (a stored value's sensitivity followed back at most to the start of its block); 0: full. */
 int tm_batch_adjoint_backward(tm_batch *b, const double *dl_bar, double *grad_out, double *x0_bar,
-                              double *v0_bar);
+                              double *v0_bar, int horizon);
 
 #ifdef __cplusplus
 }

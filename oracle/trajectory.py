@@ -35,10 +35,13 @@ def trajectory_loss_grad(
     ccs: np.ndarray,
     dl_bar: np.ndarray,
     noise: np.ndarray | None = None,
+    detach_every: int = 0,
 ):
     """Returns (du_dl [F], grads [one array per params entry], x0_bar, v0_bar)
     for the loss L = sum_f dl_bar[f] du_dl_f, where energy(x, [p...], lam) is a
-    Torch function and noise [n_steps, N, 3] (or None: zero)."""
+    Torch function and noise [n_steps, N, 3] (or None: zero). detach_every > 0
+    cuts the state's history at the start of every block of that many steps
+    (truncated backpropagation; x0_bar and v0_bar are then the first block's)."""
     x = torch.tensor(np.asarray(x0, np.float64), requires_grad=True)
     v = torch.tensor(np.asarray(v0, np.float64), requires_grad=True)
     x_leaf, v_leaf = x, v
@@ -48,6 +51,9 @@ def trajectory_loss_grad(
     loss = torch.zeros((), dtype=torch.float64)
     dls = []
     for j in range(n_steps):
+        if detach_every > 0 and j > 0 and j % detach_every == 0:
+            x = x.detach().requires_grad_(True)
+            v = v.detach()
         # a lambda leaf per step: du/dlambda is the partial derivative at this
         # step's coordinates (which depend on lambda through earlier forces)
         lt = torch.tensor(float(lam), dtype=torch.float64, requires_grad=True)

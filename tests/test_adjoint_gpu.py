@@ -243,3 +243,33 @@ def test_ti_loss_adjoint_method_matches_fp64_backprop(system, recompute):
         print(f"[ti_loss adjoint recompute={recompute}] d loss / d {name}: max err / max|ref| = {err:.2e}",
               flush=True)
         assert err <= 1e-4
+
+
+def test_truncated_adjoint_matches_truncated_backprop(system):
+    """horizon = 6: the reverse sweep's state cotangents cleared every 6 steps
+    equals FP64 backpropagation with the state's history cut at the same
+    steps; horizon >= n_steps is the full sweep, bitwise."""
+    s = system
+    dt = 0.0015
+    ca, cb, _ = ointeg.langevin_coefficients(300.0, dt, 1.0, s.masses)
+    ccs = np.zeros(s.N)
+    n_steps, interval, ck, H = 24, 3, 9, 6
+    rng = np.random.default_rng(13)
+    v0 = rng.normal(size=(s.N, 3)) * np.sqrt(2.49 / s.masses)[:, None]
+    b = _batch(s, [0.4], dt, ca, -cb, ccs, v0=v0)
+    b.adjoint_forward(n_steps, interval, ck)
+    dl_bar = rng.normal(size=(1, n_steps // interval))
+    (g_t,) = b.adjoint_backward(dl_bar, horizon=H)
+    (g_full,) = b.adjoint_backward(dl_bar)
+    (g_long,) = b.adjoint_backward(dl_bar, horizon=n_steps)
+    np.testing.assert_array_equal(g_long, g_full)
+    _, rgrads, _, _ = otraj.trajectory_loss_grad(_oracle_energy(s), s.x, v0, _blocks(s), 0.4, n_steps, interval, dt,
+                                                 ca, -cb, ccs, dl_bar[0], detach_every=H)
+    o = 0
+    for name, ref in zip(["nonbonded", "bond", "angle", "torsion"], rgrads):
+        got = g_t[o:o + ref.size].reshape(ref.shape)
+        o += ref.size
+        err = np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-12)
+        print(f"[horizon {H}] d loss / d {name}: max err / max|ref| = {err:.2e}", flush=True)
+        assert err <= 1e-4
+    assert np.abs(g_t - g_full).max() > 0.0

@@ -142,7 +142,7 @@ _SIGS = {
     "tm_batch_grad_size": (c_int, [_P, c_int, POINTER(c_int)]),
     "tm_batch_grad_stats": (c_int, [_P, c_int, POINTER(c_int), _PD, _PD, _PD, _PD, _PD]),
     "tm_batch_adjoint_forward": (c_int, [_P, c_int, c_int, c_int, _PD]),
-    "tm_batch_adjoint_backward": (c_int, [_P, _PD, _PD, _PD, _PD]),
+    "tm_batch_adjoint_backward": (c_int, [_P, _PD, _PD, _PD, _PD, c_int]),
     "tm_batch_profile": (c_int, [_P, c_int, _PD]),
     "tm_batch_get_barostat_move": (c_int, [_P, c_int, _PD]),
     "tm_batch_get_status": (c_int, [_P, _PI]),

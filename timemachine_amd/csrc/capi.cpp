@@ -1129,8 +1129,8 @@ int tm_batch_adjoint_forward(tm_batch *b, int n_steps, int du_dl_interval, int c
 }
 
 int tm_batch_adjoint_backward(tm_batch *b, const double *dl_bar, double *grad_out, double *x0_bar,
-                              double *v0_bar) {
-    return guarded([&] { b->eng->adjoint_backward(dl_bar, grad_out, x0_bar, v0_bar); });
+                              double *v0_bar, int horizon) {
+    return guarded([&] { b->eng->adjoint_backward(dl_bar, grad_out, x0_bar, v0_bar, horizon); });
 }
 
 } // extern "C"

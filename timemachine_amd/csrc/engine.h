@@ -136,10 +136,15 @@ class Engine {
     // grad_stats order); x0_bar, v0_bar [A, 3] (may be null) the cotangents of
     // the initial coordinates and velocities. The engine ends in the state after
     // the forward run. Memory: (n_steps / checkpoint_interval + 1) checkpoints
-    // of x and v, plus checkpoint_interval frames of x. Not available with a
-    // barostat, interpolated parameters or an interaction group.
+    // of x and v, plus checkpoint_interval frames of x. horizon > 0 truncates
+    // the reverse sweep: the state cotangents are cleared at every multiple of
+    // `horizon` steps, so a stored value's sensitivity is followed back at most
+    // to the start of its block (biased, but bounded where the full adjoint
+    // grows with the trajectory's Lyapunov exponent); x0_bar and v0_bar are
+    // then those of the first block. Not available with a barostat,
+    // interpolated parameters or an interaction group.
     void adjoint_forward(int n_steps, int du_dl_interval, int checkpoint_interval, double *du_dl_out);
-    void adjoint_backward(const double *dl_bar, double *grad_out, double *x0_bar, double *v0_bar);
+    void adjoint_backward(const double *dl_bar, double *grad_out, double *x0_bar, double *v0_bar, int horizon = 0);
 
     void get_x(int s, double *out);
     void get_v(int s, double *out);

@@ -1855,10 +1855,13 @@ void Engine::adjoint_forward(int n_steps, int du_dl_interval, int checkpoint_int
     I.adj_ck_interval = K;
 }
 
-void Engine::adjoint_backward(const double *dl_bar, double *grad_out, double *x0_bar, double *v0_bar) {
+void Engine::adjoint_backward(const double *dl_bar, double *grad_out, double *x0_bar, double *v0_bar, int horizon) {
     Impl &I = *impl_;
     if (I.adj_steps == 0) {
         throw Error("Engine::adjoint_backward: call adjoint_forward first");
+    }
+    if (horizon < 0) {
+        throw Error("Engine::adjoint_backward: horizon must be >= 0");
     }
     const int n_steps = I.adj_steps, K = I.adj_ck_interval, interval = I.adj_dl_interval;
     const int nseg = ceil_div(n_steps, K);
@@ -1953,6 +1956,11 @@ void Engine::adjoint_backward(const double *dl_bar, double *grad_out, double *x0
             TM_LAUNCH_CHECK();
             k_adj_post_p<<<gP, TPB, 0, stream_>>>(I.Pg, I.obs_sys.ptr, I.obs_inv_scale.ptr, scale.ptr, hp.ptr, pb.ptr);
             TM_LAUNCH_CHECK();
+            if (horizon > 0 && j > 0 && j % horizon == 0) {
+                // truncated sweep: sensitivities stop at the start of the block
+                launch_fill_u64(reinterpret_cast<uint64_t *>(xb.ptr), n3, 0, stream_);
+                launch_fill_u64(reinterpret_cast<uint64_t *>(vb.ptr), n3, 0, stream_);
+            }
         }
     }
     restore(nseg); // the state after the forward run

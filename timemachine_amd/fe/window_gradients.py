@@ -116,6 +116,7 @@ class LambdaWindows:
         barostat_interval: int = 0,
         seed: int = 2024,
         minimize_steps: int = 0,
+        adjoint_horizon: int = 0,
     ):
         if n_frames <= 0 or frame_interval <= 0:
             raise ValueError("n_frames and frame_interval must be positive")
@@ -130,6 +131,8 @@ class LambdaWindows:
         self.friction = float(friction)
         self.barostat_interval = int(barostat_interval)
         self.seed = int(seed)
+        # method="adjoint": reverse sweep truncated to blocks of this many steps (0: the whole run)
+        self.adjoint_horizon = int(adjoint_horizon)
         self.layout = FlatLayout(system.N, len(system.bond_idxs), len(system.angle_idxs), len(system.torsion_idxs))
         # checkpoint: the starting state of every window
         self.x0 = np.array(system.x, dtype=np.float64)
@@ -205,7 +208,7 @@ class LambdaWindows:
         W = len(self.lambdas)
         dl_bar = np.zeros((W, e + self.n_frames))
         dl_bar[:, e:] = np.asarray(grad_means, dtype=np.float64)[:, None] / self.n_frames
-        grads = batch.adjoint_backward(dl_bar)
+        grads = batch.adjoint_backward(dl_bar, horizon=self.adjoint_horizon)
         return np.sum(grads, axis=0)
 
     def sample(self, nb_params=None, torsion_params=None, observe: bool = True):

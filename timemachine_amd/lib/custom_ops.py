@@ -1185,12 +1185,13 @@ class BatchedContexts(_Handle):
         self._adj_frames = F
         return out[:, :F].copy()
 
-    def adjoint_backward(self, dl_bar, want_initial_state: bool = False):
+    def adjoint_backward(self, dl_bar, want_initial_state: bool = False, horizon: int = 0):
         """Gradient of a loss on the du_dl values of the last adjoint_forward,
         given its cotangent dl_bar [S, F]: a list with each system's gradient
         in its flat parameters (grad_size(s) each, grad_stats order), and with
         want_initial_state also the cotangents of the initial coordinates and
-        velocities ([total atoms, 3] each)."""
+        velocities ([total atoms, 3] each). horizon > 0 truncates the reverse
+        sweep to blocks of that many steps."""
         S = len(self)
         F = getattr(self, "_adj_frames", None)
         if F is None:
@@ -1204,7 +1205,8 @@ class BatchedContexts(_Handle):
         A = int(sum(self.sizes))
         xb = np.zeros((A, 3)) if want_initial_state else None
         vb = np.zeros((A, 3)) if want_initial_state else None
-        call("tm_batch_adjoint_backward", self._h, pd(np.ascontiguousarray(dl_bar)), pd(grad), pd(xb), pd(vb))
+        call("tm_batch_adjoint_backward", self._h, pd(np.ascontiguousarray(dl_bar)), pd(grad), pd(xb), pd(vb),
+             int(horizon))
         offs = np.cumsum([0] + sizes)
         grads = [grad[offs[s]:offs[s + 1]].copy() for s in range(S)]
         return (grads, xb, vb) if want_initial_state else grads
