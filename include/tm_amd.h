@@ -326,6 +326,9 @@ int tm_batch_set_tile_grid(tm_batch *b, int grid);
 /* Waves per SIMD the pair (tile) kernel is compiled for: 4 (default, at most 128 VGPRs), 3 or 2 (up to 168 / 256
  * VGPRs, fewer waves in flight). No reference counterpart; used by the autotuner. Results do not depend on it. */
 int tm_batch_set_tile_waves(tm_batch *b, int waves);
+/* Waves per row block in the neighborlist build (1, 2, 4 or 8; 0 = automatic). No reference
+ * counterpart; used by the autotuner. Results do not depend on it. */
+int tm_batch_set_build_pieces(tm_batch *b, int pieces);
 /* Neighborlist padding of every system of the batch (nm); the counterpart of
  * NonbondedAllPairs::set_nblist_padding (nonbonded_all_pairs.cu:180-187) for a batch.
  * Results do not depend on it. */

@@ -136,6 +136,7 @@ _SIGS = {
     "tm_batch_device_bytes": (c_int, [_P, _PU64]),
     "tm_batch_set_tile_grid": (c_int, [_P, c_int]),
     "tm_batch_set_tile_waves": (c_int, [_P, c_int]),
+    "tm_batch_set_build_pieces": (c_int, [_P, c_int]),
     "tm_batch_set_nblist_padding": (c_int, [_P, c_double]),
     "tm_batch_set_grad_observer": (c_int, [_P, c_int]),
     "tm_batch_observe": (c_int, [_P]),

@@ -1083,6 +1083,10 @@ int tm_batch_set_tile_grid(tm_batch *b, int grid) {
     });
 }
 
+int tm_batch_set_build_pieces(tm_batch *b, int pieces) {
+    return guarded([&] { b->eng->set_build_pieces(pieces); });
+}
+
 int tm_batch_set_tile_waves(tm_batch *b, int waves) {
     return guarded([&] { b->eng->set_tile_waves(waves); });
 }

@@ -185,6 +185,9 @@ class Engine {
     int tile_grid = 0; // nonbonded grid override (autotuner), 0 = default
     // Waves per SIMD of the pair kernel build (4, 3 or 2; autotuner).
     void set_tile_waves(int waves);
+    // Waves per row block in the neighborlist build (1, 2, 4 or 8; 0 =
+    // automatic). Results do not depend on it.
+    void set_build_pieces(int pieces);
 
   private:
     void flat_layout();

@@ -1623,6 +1623,15 @@ void Engine::set_tile_waves(int waves) {
     I.ig.tile_waves = waves;
 }
 
+void Engine::set_build_pieces(int pieces) {
+    if (pieces != 0 && pieces != 1 && pieces != 2 && pieces != 4 && pieces != 8) {
+        throw Error("neighborlist build pieces must be 0 (automatic), 1, 2, 4 or 8");
+    }
+    Impl &I = *impl_;
+    I.nb.pieces_override = pieces;
+    I.ig.pieces_override = pieces;
+}
+
 void Engine::set_nblist_padding(double padding) {
     Impl &I = *impl_;
     if (I.has_nb) {

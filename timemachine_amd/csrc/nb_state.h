@@ -27,6 +27,7 @@ namespace tmamd {
 
 constexpr uint32_t NB_INVALID = 0xFFFFFFFFu;
 constexpr double NB_LIST_MARGIN = 1e-5; // nm, see NbState::nb_margin
+constexpr int NB_MAX_PIECES = 8;        // waves per row block in the neighborlist build
 
 // Status bits raised on the device (box checks of
 // timemachine/cpp/src/nonbonded_all_pairs.cu:315-329, tile buffer overflow).
@@ -110,6 +111,11 @@ struct NbState {
     bool any_interp = false;
     DeviceArray<float4> bb_ctr;   // [total_blocks]
     DeviceArray<float4> bb_ext;   // [total_blocks]
+    // leftover column atoms of each row block's build pieces, merged into tiles
+    // after the scan: [total_blocks * NB_MAX_PIECES * 64] and counts
+    // [total_blocks * NB_MAX_PIECES] (empty for views: one piece)
+    DeviceArray<uint32_t> piece_buf;
+    DeviceArray<int> piece_cnt;
     DeviceArray<double> nblist_x; // [sum n_atoms * 3] coordinates at the last build
     DeviceArray<double> nblist_box; // [S * 9]
 
@@ -128,6 +134,7 @@ struct NbState {
     // Force a rebuild of every system on the next prepare().
     void invalidate(hipStream_t stream);
     int find_pieces() const; // waves per row block in the neighborlist build
+    void find_tiles(const double *boxes, hipStream_t stream); // the build's tile scan (and piece merge)
     int pieces_override = 0; // > 0: waves per row block forced to this (1, 2, 4 or 8)
     // Waves per SIMD the pair kernel is compiled for (4, 3 or 2; 4 = at most 128
     // VGPRs). Results do not depend on it.

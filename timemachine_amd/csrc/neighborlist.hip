@@ -435,7 +435,9 @@ __global__ __launch_bounds__(256) void k_find_tiles(
     int *__restrict__ tile_row,
     uint32_t *__restrict__ tile_atoms,
     int *__restrict__ status,
-    const int pieces) {
+    const int pieces,
+    uint32_t *__restrict__ piece_buf,
+    int *__restrict__ piece_cnt) {
     const int s = blockIdx.x;
     if (!rebuild[s]) {
         return;
@@ -644,8 +646,73 @@ __global__ __launch_bounds__(256) void k_find_tiles(
             }
         }
     }
-    if (nbuf > 0) {
+    if (pieces > 1) {
+        // this piece's last (< 64) column atoms: k_merge_pieces packs the
+        // leftovers of a row block's pieces into full tiles and one partial
+        const size_t slot = static_cast<size_t>(S.block0 + r) * NB_MAX_PIECES + piece;
+        if (lane < nbuf) {
+            piece_buf[slot * TILE + lane] = buf[lane];
+        }
+        if (lane == 0) {
+            piece_cnt[slot] = nbuf;
+        }
+    } else if (nbuf > 0) {
         const uint32_t v = lane < nbuf ? buf[lane] : NB_INVALID;
+        int tile = 0;
+        if (lane == 0) {
+            tile = atomicAdd(tile_count + s, 1);
+        }
+        tile = __shfl(tile, 0, 64);
+        if (tile < S.tile_cap) {
+            tile_atoms[static_cast<size_t>(S.tile0 + tile) * TILE + lane] = v;
+            if (lane == 0) {
+                tile_row[S.tile0 + tile] = r;
+            }
+        } else if (lane == 0) {
+            atomicOr(status + s, NB_STATUS_TILE_OVERFLOW);
+        }
+    }
+}
+
+// The leftovers of a row block's pieces (each < 64 column atoms) as full
+// tiles and at most one partial tile, so splitting a row block's scan over
+// several waves adds no tiles. grid (S, ceil(max_blocks / 4)), one wave per
+// row block of a rebuilt system.
+__global__ __launch_bounds__(256) void k_merge_pieces(
+    const NbSys *__restrict__ sys,
+    const int *__restrict__ rebuild,
+    const int pieces,
+    const uint32_t *__restrict__ piece_buf,
+    const int *__restrict__ piece_cnt,
+    int *__restrict__ tile_count,
+    int *__restrict__ tile_row,
+    uint32_t *__restrict__ tile_atoms,
+    int *__restrict__ status) {
+    const int s = blockIdx.x;
+    if (!rebuild[s]) {
+        return;
+    }
+    const NbSys S = sys[s];
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.y * 4 + wave;
+    if (r >= S.row_blocks) {
+        return;
+    }
+    __shared__ uint32_t merged[4][NB_MAX_PIECES * TILE];
+    uint32_t *b = merged[wave];
+    int n = 0;
+    for (int p = 0; p < pieces; p++) {
+        const size_t slot = static_cast<size_t>(S.block0 + r) * NB_MAX_PIECES + p;
+        const int c = piece_cnt[slot];
+        if (lane < c) {
+            b[n + lane] = piece_buf[slot * TILE + lane];
+        }
+        n += c;
+    }
+    wave_lds_sync();
+    for (int base = 0; base < n; base += TILE) {
+        const uint32_t v = base + lane < n ? b[base + lane] : NB_INVALID;
         int tile = 0;
         if (lane == 0) {
             tile = atomicAdd(tile_count + s, 1);
@@ -955,6 +1022,8 @@ void NbState::init(const std::vector<NbSys> &sys_in, const std::vector<int> &sub
     }
     bb_ctr.alloc(total_blocks);
     bb_ext.alloc(total_blocks);
+    piece_buf.alloc(static_cast<size_t>(total_blocks) * NB_MAX_PIECES * TILE);
+    piece_cnt.alloc(static_cast<size_t>(total_blocks) * NB_MAX_PIECES);
     nblist_x.alloc(static_cast<size_t>(total_atoms_end) * 3);
     nblist_box.alloc(static_cast<size_t>(S) * 9);
     status.zero();
@@ -969,20 +1038,24 @@ void NbState::init(const std::vector<NbSys> &sys_in, const std::vector<int> &sub
 void NbState::invalidate(hipStream_t stream) { launch_fill_i32(rebuild.ptr, S, 1, stream); }
 
 // Waves per row block in the neighborlist build. One wave scans a row
-// block's column blocks one candidate after another, so a build of few
-// systems takes as long as its longest row block (the first ones, upper
-// triangle) while most of the GPU idles. Below ~2k waves in all, each row
-// block's column range is split into up to 8 contiguous pieces, one wave
-// each (every piece ends in its own partial tile: a few more tiles for the
-// pair kernel). Large batches keep one wave per row block.
+// block's column blocks one candidate after another, so a build takes as
+// long as its longest row block (the first ones, upper triangle) however few
+// systems rebuild, while most of the GPU idles. Each row block's column range
+// is split into up to 8 contiguous pieces, one wave each; the pieces'
+// leftovers are merged (k_merge_pieces), so the tile list is as short as with
+// one wave. Only batches far beyond the GPU's wave slots (where every wave
+// would queue anyway) use fewer pieces. Views (no staging arrays) use one.
 int NbState::find_pieces() const {
+    if (piece_cnt.n == 0) {
+        return 1;
+    }
     if (pieces_override > 0) {
         return pieces_override;
     }
     const long long waves = static_cast<long long>(S) * max_blocks;
-    int q = 1;
-    while (q < 8 && waves * q * 2 <= 4096) {
-        q *= 2;
+    int q = NB_MAX_PIECES;
+    while (q > 1 && waves * q > 262144) {
+        q /= 2;
     }
     return q;
 }
@@ -1001,6 +1074,20 @@ void NbState::set_padding(double padding, hipStream_t stream) {
     TM_HIP_CHECK(hipStreamSynchronize(stream));
 }
 
+void NbState::find_tiles(const double *boxes, hipStream_t stream) {
+    const int pieces = find_pieces();
+    k_find_tiles<<<dim3(S, ceil_div(max_blocks * pieces, 4)), 256, 0, stream>>>(
+        sys.ptr, boxes, rebuild.ptr, pos.ptr, bb_ctr.ptr, bb_ext.ptr, tile_count.ptr, tile_row.ptr, tile_atoms.ptr,
+        status.ptr, pieces, piece_buf.ptr, piece_cnt.ptr);
+    TM_LAUNCH_CHECK();
+    if (pieces > 1) {
+        k_merge_pieces<<<dim3(S, ceil_div(max_blocks, 4)), 256, 0, stream>>>(
+            sys.ptr, rebuild.ptr, pieces, piece_buf.ptr, piece_cnt.ptr, tile_count.ptr, tile_row.ptr, tile_atoms.ptr,
+            status.ptr);
+        TM_LAUNCH_CHECK();
+    }
+}
+
 void NbState::build_only(const double *x, const double *boxes, hipStream_t stream) {
     const int gx_atoms = ceil_div(std::max(max_atoms, 9), 256);
     k_check_rebuild<<<dim3(gx_atoms, S), 256, 0, stream>>>(sys.ptr, subset.ptr, x, nblist_x.ptr, boxes,
@@ -1015,10 +1102,7 @@ void NbState::build_only(const double *x, const double *boxes, hipStream_t strea
     k_block_bounds<<<dim3(ceil_div(max_blocks, 4), S), 256, 0, stream>>>(sys.ptr, boxes, rebuild.ptr, pos.ptr,
                                                                          bb_ctr.ptr, bb_ext.ptr);
     TM_LAUNCH_CHECK();
-    k_find_tiles<<<dim3(S, ceil_div(max_blocks * find_pieces(), 4)), 256, 0, stream>>>(
-        sys.ptr, boxes, rebuild.ptr, pos.ptr, bb_ctr.ptr, bb_ext.ptr, tile_count.ptr, tile_row.ptr, tile_atoms.ptr,
-        status.ptr, find_pieces());
-    TM_LAUNCH_CHECK();
+    find_tiles(boxes, stream);
     k_tile_order<<<S, 256, 0, stream>>>(sys.ptr, rebuild.ptr, tile_count.ptr, tile_row.ptr, tile_order.ptr);
     TM_LAUNCH_CHECK();
     k_tile_prefix<<<1, 256, 0, stream>>>(sys.ptr, S, tile_count.ptr, tile_prefix.ptr, rebuild.ptr);
@@ -1057,10 +1141,7 @@ void NbState::prepare(
     k_block_bounds<<<dim3(ceil_div(max_blocks, 4), S), 256, 0, stream>>>(sys.ptr, boxes, rebuild.ptr, pos.ptr,
                                                                          bb_ctr.ptr, bb_ext.ptr);
     TM_LAUNCH_CHECK();
-    k_find_tiles<<<dim3(S, ceil_div(max_blocks * find_pieces(), 4)), 256, 0, stream>>>(
-        sys.ptr, boxes, rebuild.ptr, pos.ptr, bb_ctr.ptr, bb_ext.ptr, tile_count.ptr, tile_row.ptr, tile_atoms.ptr,
-        status.ptr, find_pieces());
-    TM_LAUNCH_CHECK();
+    find_tiles(boxes, stream);
     k_tile_order<<<S, 256, 0, stream>>>(sys.ptr, rebuild.ptr, tile_count.ptr, tile_row.ptr, tile_order.ptr);
     TM_LAUNCH_CHECK();
     k_tile_prefix<<<1, 256, 0, stream>>>(sys.ptr, S, tile_count.ptr, tile_prefix.ptr, rebuild.ptr);

@@ -951,6 +951,11 @@ class BatchedContexts(_Handle):
     def set_nblist_padding(self, padding: float):
         call("tm_batch_set_nblist_padding", self._h, float(padding))
 
+    def set_build_pieces(self, pieces: int):
+        """Waves per row block in the neighborlist build (1, 2, 4 or 8; 0 =
+        automatic). Results do not depend on it."""
+        call("tm_batch_set_build_pieces", self._h, int(pieces))
+
     def set_tile_waves(self, waves: int):
         """Waves per SIMD the pair kernel is compiled for: 4 (default), 3 or 2 (extension)."""
         call("tm_batch_set_tile_waves", self._h, int(waves))
