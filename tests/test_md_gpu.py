@@ -267,3 +267,27 @@ def test_engine_rejects_pair_lists_with_different_planes(ligand):
     intg = LangevinIntegrator(300.0, 0.0025, 1.0, ligand.masses, 1).impl()
     with pytest.raises(RuntimeError, match="must share parameters, lambda plane and offset"):
         custom_ops.Context(ligand.x, np.zeros_like(ligand.x), ligand.box, intg, bps)
+
+
+def test_build_pieces_do_not_change_bits(ligand):
+    """The neighborlist build split over 1, 2, 4 or 8 waves per row block (the
+    pieces' leftovers merged into tiles) gives the same tile count and bitwise
+    the same trajectories, barostat on."""
+    lams = [0.2, 0.6, 0.9]
+    ref = None
+    for pieces in (1, 2, 4, 8):
+        made = [_context(ligand, seed=700 + s, barostat_interval=5) for s in range(len(lams))]
+        b = custom_ops.BatchedContexts([m[0] for m in made], lams)
+        b.set_build_pieces(pieces)
+        du_dl, _, _ = b.multiple_steps(40, store_du_dl_interval=1)
+        out = (du_dl, [b.get_x_t(s) for s in range(len(lams))], b.total_tiles())
+        print(f"pieces {pieces}: tiles {out[2]}", flush=True)
+        if ref is None:
+            ref = out
+        else:
+            np.testing.assert_array_equal(out[0], ref[0])
+            for a, c in zip(out[1], ref[1]):
+                np.testing.assert_array_equal(a, c)
+            assert out[2] == ref[2]
+    with pytest.raises(RuntimeError, match="pieces"):
+        b.set_build_pieces(3)
