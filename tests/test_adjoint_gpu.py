@@ -237,7 +237,8 @@ def test_ti_loss_adjoint_method_matches_fp64_backprop(system, recompute):
         ref_grads.append(grads)
     r_nb = sum(g[0] for g in ref_grads)
     r_tk = sum(g[3][:, 0] for g in ref_grads)
-    assert abs(float(loss) - (float(np.dot(c, means)) - target) ** 2) <= 1e-5 * max(float(loss), 1.0)
+    lv = float(loss.detach())
+    assert abs(lv - (float(np.dot(c, means)) - target) ** 2) <= 1e-5 * max(lv, 1.0)
     for name, got, ref in [("nonbonded", g_nb.numpy(), r_nb), ("torsion k", g_tk.numpy(), r_tk)]:
         err = np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-12)
         print(f"[ti_loss adjoint recompute={recompute}] d loss / d {name}: max err / max|ref| = {err:.2e}",
