@@ -154,8 +154,10 @@ __device__ __forceinline__ PairGradDual pair_grad_dual(const Dual *dv, const Dua
     const Dual r = sqrt(d2);
     const Dual inv_r = Dual(1.0) / r;
     const Dual bx = Dual(beta) * r;
-    const Dual ec = erfc(bx);
-    const Dual ex = exp(-(bx * bx));
+    // exp(-x^2) once for both: erfc'(x) = -2 / sqrt(pi) exp(-x^2), (exp(-x^2))' = -2 x exp(-x^2)
+    const double e = std::exp(-(bx.v * bx.v));
+    const Dual ec(std::erfc(bx.v), -TWO_OVER_SQRT_PI * e * bx.d);
+    const Dual ex(e, -2.0 * bx.v * e * bx.d);
     // u_es = sq q_i q_j erfc(beta r) / r;  (du/dr) / r = -sq q_i q_j (erfc / r + 2 beta / sqrt(pi) e^{-b^2 r^2}) / r^2
     const Dual qij = Dual(sq) * qi * qj;
     o.pre = -(qij * (ec * inv_r + Dual(TWO_OVER_SQRT_PI * beta) * ex) * inv_r * inv_r);
