@@ -147,7 +147,9 @@ int tm_potential_execute_dl_dp_host(tm_potential *p, int N, int P, const double 
  * tangent; a null output is not requested. No counterpart in the reference's custom ops: the
  * reference differentiates its JAX potentials (timemachine/potentials/nonbonded.py:133-264,
  * bonded.py:150-332) for second derivatives, and describes the trajectory adjoint in
- * attic/docs/paper.tex:27. Potentials with interpolated parameters return an error. */
+ * attic/docs/paper.tex:27. Interpolated parameters ([2N, 3], used at (1 - lambda) p0 + lambda p1)
+ * are differentiated through that interpolation: hp is [2N, 3] and hl includes the parameters'
+ * lambda dependence. */
 int tm_potential_execute_hvp_device(tm_potential *p, int N, int P, const double *d_x, const double *d_p,
                                     const double *d_box, double lambda, const double *d_vx, const double *d_vp,
                                     double vl, uint64_t *d_hx, uint64_t *d_hp, uint64_t *d_hl, void *stream);

@@ -218,11 +218,93 @@ def test_hvp_of_zero_tangent_is_zero(ligand):
     assert not np.any(hx) and not np.any(hp) and hl == 0.0
 
 
-def test_interpolated_hvp_raises(gpu):
+@pytest.fixture(scope="module")
+def mutated(gpu):
     s = ts.mutate_ligand(ts.ligand_in_water(side=3.0, n_waters=800, n_heavy=12, n_ligand=22, seed=3))
+    r = lambda p: p.astype(np.float32).astype(np.float64)
+    return dataclasses.replace(s, nb_params=r(s.nb_params), nb_params_final=r(s.nb_params_final))
+
+
+@pytest.fixture(scope="module")
+def mutated_soft(mutated):
+    """The parts alone (all pairs, or exclusions) need du/dp itself in range
+    (the chain rule through p(lambda) uses it): at bonded ligand distances
+    du/dsigma reaches ~9e7, beyond the 2^63 / 2^37 ~ 6.7e7 range of that
+    column, where the full potential's two parts cancel. Ligand epsilons
+    (both ends) x 0.1 keep it at ~1e6."""
+    lig = np.asarray(mutated.ligand_idxs)
+    p0, p1 = mutated.nb_params.copy(), mutated.nb_params_final.copy()
+    for p in (p0, p1):
+        p[lig, 2] = (p[lig, 2] * 0.1).astype(np.float32)
+    return dataclasses.replace(mutated, nb_params=p0, nb_params_final=p1)
+
+
+def _interpolated_f32(energy_fn):
+    """onb.interpolated with p(lambda) valued at its FP32 rounding, the value
+    the kernels evaluate; the derivatives are the exact interpolation's (the
+    rounding enters as a constant offset)."""
+
+    def fn(xt, pt, lt):
+        n = pt.shape[0] // 2
+        p = (1 - lt) * pt[:n] + lt * pt[n:]
+        return energy_fn(xt, p + (p.detach().float().double() - p.detach()), lt)
+
+    return fn
+
+
+def _interp_tangents(s, rng, ligand_scale=1.0):
+    """Tangents on (coords, initial params, final params); epsilon tangents
+    stay 0 where epsilon is 0 at both ends (LJ off throughout)."""
+    vx, vp0 = _tangents(s, rng, ligand_scale)
+    _, vp1 = _tangents(s, rng, ligand_scale)
+    off = (s.nb_params[:, 2] == 0.0) & (s.nb_params_final[:, 2] == 0.0)
+    vp0[off, 2] = 0.0
+    vp1[off, 2] = 0.0
+    return vx, np.concatenate([vp0, vp1])
+
+
+@pytest.mark.parametrize("lam", [0.35, 0.8])
+def test_interpolated_all_pairs_hvp(mutated_soft, lam):
+    """NonbondedAllPairs over [2N, 3] parameters p(lambda) = (1 - lambda) p0 +
+    lambda p1 (the lambda tangent moves the ligand parameters too, so it is
+    kept small like the ligand's: see _tangents)."""
+    rng = np.random.default_rng(21)
+    s = mutated_soft
+    pot = potentials.NonbondedAllPairsInterpolated(s.lambda_plane_idxs, s.lambda_offset_idxs, s.beta,
+                                                   s.cutoff).unbound_impl(F32)
+    vx, vp = _interp_tangents(s, rng, ligand_scale=1e-4)
+    out = pot.execute_hvp(s.x, s.nb_params_all, s.box, lam, vx, vp, 1e-4)
+    fn = onb.pair_energy_fn(s.x, s.box, np.zeros((0, 2), int), np.zeros((0, 2)), s.beta, s.cutoff,
+                            s.lambda_plane_idxs, s.lambda_offset_idxs)
+    ref = onb.hvp(_interpolated_f32(fn), s.x, s.nb_params_all, lam, vx, vp, 1e-4)
+    _check(out, ref, f"interpolated all pairs lam={lam}")
+
+
+def test_interpolated_exclusions_hvp(mutated_soft):
+    rng = np.random.default_rng(22)
+    s = mutated_soft
+    pot = potentials.NonbondedPairListNegatedInterpolated(s.exclusion_idxs, s.scales, s.lambda_plane_idxs,
+                                                          s.lambda_offset_idxs, s.beta, s.cutoff).unbound_impl(F32)
+    vx, vp = _interp_tangents(s, rng, ligand_scale=1e-4)
+    out = pot.execute_hvp(s.x, s.nb_params_all, s.box, 0.6, vx, vp, -1e-4)
+    fn = onb.pair_list_energy_fn(s.exclusion_idxs, s.scales, s.box, s.beta, s.cutoff, s.lambda_plane_idxs,
+                                 s.lambda_offset_idxs, negated=True)
+    ref = onb.hvp(_interpolated_f32(fn), s.x, s.nb_params_all, 0.6, vx, vp, -1e-4)
+    _check(out, ref, "interpolated exclusions")
+
+
+def test_interpolated_nonbonded_hvp(mutated):
+    """The full interpolated Nonbonded potential with unscaled tangents: the
+    all-pairs and exclusion terms of bonded ligand pairs leave the fixed-point
+    range alone and cancel in the shared sums before the one chain rule
+    through p(lambda)."""
+    rng = np.random.default_rng(23)
+    s = mutated
     pot = s.nonbonded().unbound_impl(F32)
-    with pytest.raises(RuntimeError, match="interpolated"):
-        pot.execute_hvp(s.x, s.nb_params_all, s.box, 0.5, np.ones((s.N, 3)))
+    vx, vp = _interp_tangents(s, rng)
+    out = pot.execute_hvp(s.x, s.nb_params_all, s.box, 0.45, vx, vp, 0.7)
+    ref = onb.hvp(_interpolated_f32(_net_nonbonded_fn(s)), s.x, s.nb_params_all, 0.45, vx, vp, 0.7)
+    _check(out, ref, "interpolated nonbonded")
 
 
 def _net_nonbonded_fn(s):

@@ -190,21 +190,36 @@ __device__ __forceinline__ Dual dual_param(const double *p, const double *vp, in
 
 // Fixed-point tangents of one pair's gradient: du_dx (3) and du_dp (3) of
 // atoms i and j, and du_dl. Zero for pairs outside the cutoff.
-struct PairTangents {
+// With G, also the values of du_dp (gi, gj).
+template <bool G> struct PairTangents {
     uint64_t xi[3], xj[3], pi[3], pj[3], l;
+    uint64_t gi[G ? 3 : 1], gj[G ? 3 : 1];
 };
 
-__device__ __forceinline__ PairTangents zero_tangents() {
-    PairTangents t;
+template <bool G> __device__ __forceinline__ PairTangents<G> zero_tangents() {
+    PairTangents<G> t;
     for (int d = 0; d < 3; d++) {
         t.xi[d] = t.xj[d] = t.pi[d] = t.pj[d] = 0;
+        if constexpr (G) {
+            t.gi[d] = t.gj[d] = 0;
+        }
     }
     t.l = 0;
     return t;
 }
 
-__device__ __forceinline__ PairTangents pair_tangents(const PairGradDual &o, const Dual *dv, bool neg) {
-    PairTangents t = zero_tangents();
+template <bool G>
+__device__ __forceinline__ PairTangents<G> pair_tangents(const PairGradDual &o, const Dual *dv, bool neg) {
+    PairTangents<G> t = zero_tangents<G>();
+    if constexpr (G) {
+        t.gi[0] = fixed_signed(o.dq_i.v, FIXED_SCALE, neg);
+        t.gj[0] = fixed_signed(o.dq_j.v, FIXED_SCALE, neg);
+        if (o.lj) {
+            t.gi[1] = t.gj[1] = fixed_signed(o.dsig.v, FIXED_SCALE_DSIG, neg);
+            t.gi[2] = fixed_signed(o.deps_i.v, FIXED_SCALE_DEPS, neg);
+            t.gj[2] = fixed_signed(o.deps_j.v, FIXED_SCALE_DEPS, neg);
+        }
+    }
     for (int d = 0; d < 3; d++) {
         const Dual f = o.pre * dv[d];
         t.xi[d] = fixed_signed(f.d, FIXED_SCALE, neg);
@@ -222,9 +237,10 @@ __device__ __forceinline__ PairTangents pair_tangents(const PairGradDual &o, con
 }
 
 // One thread per pair of the list, as k_pair_list (one or many systems).
+template <bool G>
 __global__ __launch_bounds__(256) void k_pair_list_hvp(const PairListArgs a, const HvpTangent v, const HvpOut out) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    PairTangents t = zero_tangents();
+    PairTangents<G> t = zero_tangents<G>();
     int gi = 0, gj = 0, red = -1;
     bool in = false;
     if (p < a.M) {
@@ -256,7 +272,7 @@ __global__ __launch_bounds__(256) void k_pair_list_hvp(const PairListArgs a, con
                 dual_param(a.params, v.vp, gi, 1), dual_param(a.params, v.vp, gj, 1),
                 dual_param(a.params, v.vp, gi, 2), dual_param(a.params, v.vp, gj, 2), beta, a.scales[p * 2 + 0],
                 a.scales[p * 2 + 1]);
-            t = pair_tangents(o, dv, a.negated != 0);
+            t = pair_tangents<G>(o, dv, a.negated != 0);
         }
     }
     if (in) {
@@ -268,6 +284,10 @@ __global__ __launch_bounds__(256) void k_pair_list_hvp(const PairListArgs a, con
             if (out.hp) {
                 atomic_add_fixed(out.hp + gi * 3 + c, t.pi[c]);
                 atomic_add_fixed(out.hp + gj * 3 + c, t.pj[c]);
+            }
+            if constexpr (G) {
+                atomic_add_fixed(out.gp + gi * 3 + c, t.gi[c]);
+                atomic_add_fixed(out.gp + gj * 3 + c, t.gj[c]);
             }
         }
     }
@@ -283,7 +303,7 @@ __global__ __launch_bounds__(256) void k_pair_list_hvp(const PairListArgs a, con
 // (ballot / mbcnt); every 64 queued pairs are evaluated one per lane, their
 // fixed-point tangents added to per-atom LDS accumulators of the tile's rows
 // and columns, which go to global memory once per tile and atom.
-struct HvpTileLds {
+template <int NACC> struct HvpTileLds {
     // [0] rows, [1] columns: position (x, y, z, w), parameters at their FP32
     // values, tangents, lambda offset, sorted slot and global atom index
     double px[2][TILE], py[2][TILE], pz[2][TILE], pw[2][TILE];
@@ -293,7 +313,7 @@ struct HvpTileLds {
     int off[2][TILE];
     uint32_t slot[2][TILE];
     int gidx[2][TILE];
-    uint64_t acc[2][6][TILE]; // du_dx (3) and du_dp (3) tangents of each row / column atom
+    uint64_t acc[2][NACC][TILE]; // du_dx (3) and du_dp (3) tangents, [du_dp (3)], of each row / column atom
     uint16_t pq[2 * TILE];    // queued pair: row lane | column lane << 6
 };
 
@@ -303,12 +323,14 @@ __device__ __forceinline__ void lds_add(uint64_t *p, uint64_t v) {
     }
 }
 
+template <bool G>
 __global__ __launch_bounds__(64) void k_tiles_hvp(
     const NbSys *__restrict__ sys, int S, const int *__restrict__ tile_prefix, const int *__restrict__ tile_row,
     const uint32_t *__restrict__ tile_atoms, const uint32_t *__restrict__ perm, const double4 *__restrict__ pos,
     const double *__restrict__ params, const int *__restrict__ offset, const double *__restrict__ boxes,
     const int *__restrict__ status, const HvpTangent v, const HvpOut out) {
-    __shared__ HvpTileLds L;
+    constexpr int NACC = G ? 9 : 6;
+    __shared__ HvpTileLds<NACC> L;
     const int total = tile_prefix[S];
     const int lane = threadIdx.x;
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
@@ -363,7 +385,7 @@ __global__ __launch_bounds__(64) void k_tiles_hvp(
             L.off[side][lane] = (valid && offset) ? offset[g] : 0;
             L.slot[side][lane] = valid ? sl : NB_INVALID;
             L.gidx[side][lane] = g;
-            for (int k = 0; k < 6; k++) {
+            for (int k = 0; k < NACC; k++) {
                 L.acc[side][k][lane] = 0;
             }
         }
@@ -385,12 +407,16 @@ __global__ __launch_bounds__(64) void k_tiles_hvp(
                     dv, dw, dwdl, Dual(L.q[0][il], L.tq[0][il]), Dual(L.q[1][jl], L.tq[1][jl]),
                     Dual(L.sg[0][il], L.ts[0][il]), Dual(L.sg[1][jl], L.ts[1][jl]), Dual(L.ep[0][il], L.te[0][il]),
                     Dual(L.ep[1][jl], L.te[1][jl]), beta, 1.0, 1.0);
-                const PairTangents tg = pair_tangents(o, dv, false);
+                const PairTangents<G> tg = pair_tangents<G>(o, dv, false);
                 for (int k = 0; k < 3; k++) {
                     lds_add(&L.acc[0][k][il], tg.xi[k]);
                     lds_add(&L.acc[1][k][jl], tg.xj[k]);
                     lds_add(&L.acc[0][3 + k][il], tg.pi[k]);
                     lds_add(&L.acc[1][3 + k][jl], tg.pj[k]);
+                    if constexpr (G) {
+                        lds_add(&L.acc[0][6 + k][il], tg.gi[k]);
+                        lds_add(&L.acc[1][6 + k][jl], tg.gj[k]);
+                    }
                 }
                 acc_l += tg.l;
             }
@@ -445,6 +471,9 @@ __global__ __launch_bounds__(64) void k_tiles_hvp(
                     if (out.hp) {
                         atomic_add_fixed(out.hp + g * 3 + k, L.acc[side][3 + k][lane]);
                     }
+                    if constexpr (G) {
+                        atomic_add_fixed(out.gp + g * 3 + k, L.acc[side][6 + k][lane]);
+                    }
                 }
             }
         }
@@ -457,7 +486,72 @@ __global__ __launch_bounds__(64) void k_tiles_hvp(
     }
 }
 
+// p(lambda) and its tangent, one thread per parameter word.
+__global__ __launch_bounds__(256) void k_interp_hvp_params(int n3, double lam, const double *__restrict__ p,
+                                                           const double *__restrict__ vp, double vl,
+                                                           double *__restrict__ p_lam, double *__restrict__ vp_lam) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) {
+        return;
+    }
+    const double a = p[i], b = p[n3 + i];
+    p_lam[i] = (1.0 - lam) * a + lam * b;
+    double t = (b - a) * vl;
+    if (vp) {
+        t += (1.0 - lam) * vp[i] + lam * vp[n3 + i];
+    }
+    vp_lam[i] = t;
+}
+
+// Chain rule through p(lambda), one thread per parameter word; the du/dlambda
+// part is summed per wave in fixed point.
+__global__ __launch_bounds__(256) void k_interp_hvp_chain(int n3, double lam, const double *__restrict__ p,
+                                                          const double *__restrict__ vp, double vl,
+                                                          const uint64_t *__restrict__ g, const uint64_t *__restrict__ dg,
+                                                          uint64_t *__restrict__ hp, uint64_t *__restrict__ hl) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t l = 0;
+    if (i < n3) {
+        const int c = i % 3;
+        const double scale = c == 0 ? FIXED_SCALE : (c == 1 ? FIXED_SCALE_DSIG : FIXED_SCALE_DEPS);
+        const double gv = fixed_to_double(g[i], scale);
+        const double dv = fixed_to_double(dg[i], scale);
+        if (hp) {
+            hp[i] += f64_to_fixed((1.0 - lam) * dv - vl * gv, scale);
+            hp[n3 + i] += f64_to_fixed(lam * dv + vl * gv, scale);
+        }
+        const double dvp = vp ? vp[n3 + i] - vp[i] : 0.0;
+        l = f64_to_fixed(dv * (p[n3 + i] - p[i]) + gv * dvp, FIXED_SCALE);
+    }
+    if (hl) {
+        const uint64_t tot = wave_sum_u64(l);
+        if ((threadIdx.x & 63) == 0) {
+            atomic_add_fixed(hl, tot);
+        }
+    }
+}
+
 } // namespace
+
+void launch_interp_hvp_params(int N, double lambda, const double *params, const double *vp, double vl, double *p_lam,
+                              double *vp_lam, hipStream_t stream) {
+    const int n3 = N * 3;
+    if (n3 == 0) {
+        return;
+    }
+    k_interp_hvp_params<<<ceil_div(n3, 256), 256, 0, stream>>>(n3, lambda, params, vp, vl, p_lam, vp_lam);
+    TM_LAUNCH_CHECK();
+}
+
+void launch_interp_hvp_chain(int N, double lambda, const double *params, const double *vp, double vl, const uint64_t *g,
+                             const uint64_t *dg, uint64_t *hp, uint64_t *hl, hipStream_t stream) {
+    const int n3 = N * 3;
+    if (n3 == 0 || (!hp && !hl)) {
+        return;
+    }
+    k_interp_hvp_chain<<<ceil_div(n3, 256), 256, 0, stream>>>(n3, lambda, params, vp, vl, g, dg, hp, hl);
+    TM_LAUNCH_CHECK();
+}
 
 void launch_bonded_hvp(BondedKind kind, const BondedTerms &t, const double *x, const double *lambdas, double lambda0,
                        const int *red_idx, const HvpTangent &v, const HvpOut &o, hipStream_t stream) {
@@ -480,28 +574,36 @@ void launch_bonded_hvp(BondedKind kind, const BondedTerms &t, const double *x, c
 }
 
 void launch_pair_list_hvp(const PairListArgs &a, const HvpTangent &v, const HvpOut &o, hipStream_t stream) {
-    if (a.M == 0 || (!o.hx && !o.hp && !o.hl)) {
+    if (a.M == 0 || (!o.hx && !o.hp && !o.hl && !o.gp)) {
         return;
     }
     if (a.params1 || a.slots) {
         throw Error("Hessian-vector products: plain parameters only");
     }
-    k_pair_list_hvp<<<ceil_div(a.M, 256), 256, 0, stream>>>(a, v, o);
+    if (o.gp) {
+        k_pair_list_hvp<true><<<ceil_div(a.M, 256), 256, 0, stream>>>(a, v, o);
+    } else {
+        k_pair_list_hvp<false><<<ceil_div(a.M, 256), 256, 0, stream>>>(a, v, o);
+    }
     TM_LAUNCH_CHECK();
 }
 
 void launch_tiles_hvp(const NbState &st, const double *params, const int *offset, const double *boxes,
                       const HvpTangent &v, const HvpOut &o, hipStream_t stream) {
-    if (st.S == 0 || (!o.hx && !o.hp && !o.hl)) {
+    if (st.S == 0 || (!o.hx && !o.hp && !o.hl && !o.gp)) {
         return;
-    }
-    if (st.any_interp) {
-        throw Error("Hessian-vector products: plain parameters only");
     }
     // one wave per tile, the grid striding over the tile count read on the device
     const int grid = std::max(1, std::min(st.total_tiles, 16384));
-    k_tiles_hvp<<<grid, TILE, 0, stream>>>(st.sys.ptr, st.S, st.tile_prefix.ptr, st.tile_row.ptr, st.tile_atoms.ptr,
-                                            st.perm.ptr, st.pos.ptr, params, offset, boxes, st.status.ptr, v, o);
+    if (o.gp) {
+        k_tiles_hvp<true><<<grid, TILE, 0, stream>>>(st.sys.ptr, st.S, st.tile_prefix.ptr, st.tile_row.ptr,
+                                                      st.tile_atoms.ptr, st.perm.ptr, st.pos.ptr, params, offset,
+                                                      boxes, st.status.ptr, v, o);
+    } else {
+        k_tiles_hvp<false><<<grid, TILE, 0, stream>>>(st.sys.ptr, st.S, st.tile_prefix.ptr, st.tile_row.ptr,
+                                                       st.tile_atoms.ptr, st.perm.ptr, st.pos.ptr, params, offset,
+                                                       boxes, st.status.ptr, v, o);
+    }
     TM_LAUNCH_CHECK();
 }
 

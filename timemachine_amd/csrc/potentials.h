@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "bonded.h"
+#include "hvp.h"
 #include "nb_state.h"
 #include "tm_common.h"
 
@@ -65,11 +66,20 @@ class Potential {
     // (vx, vp, vl), the adjoint of du/dx that reverse-mode differentiation
     // through forces and trajectories needs. d_vx / d_vp null: zero tangent;
     // a null output is not requested. No reference counterpart (the reference
-    // takes second derivatives from its JAX potentials). Throws for potentials
-    // without it (interpolated parameters).
+    // takes second derivatives from its JAX potentials).
     virtual void execute_hvp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box,
                                     double lambda, const double *d_vx, const double *d_vp, double vl, uint64_t *d_hx,
                                     uint64_t *d_hp, uint64_t *d_hl, hipStream_t stream);
+    // Interpolated nonbonded potentials (params [2N, 3], used as
+    // p(lambda) = (1 - lambda) p0 + lambda p1) return true and give the
+    // plain-parameter product at p(lambda): tangents along (d_vx, vp_lam, vl)
+    // into d_hx, dg [N, 3] and d_hl, du/dp at p(lambda) into g [N, 3] (exact
+    // FP64 derivatives, as the tangents; g and dg null: not wanted).
+    // interpolated_hvp() completes the product.
+    virtual bool interpolated_nonbonded() const { return false; }
+    virtual void hvp_at_interpolated(int N, const double *d_x, const double *p_lam, const double *d_box, double lambda,
+                                     const double *d_vx, const double *vp_lam, double vl, uint64_t *d_hx, uint64_t *g,
+                                     uint64_t *dg, uint64_t *d_hl, hipStream_t stream);
     // Host variant: decoded FP64 outputs. The tangent is scaled by a power of
     // two to at most 1 in magnitude before the device call (the product is
     // linear in it) and the outputs scaled back, which keeps the fixed-point
@@ -116,6 +126,17 @@ class Potential {
 
 using PotentialPtr = std::shared_ptr<Potential>;
 
+// Hessian-vector product (execute_hvp_device) of potentials that share the
+// interpolated parameters d_p [2N, 3]: p(lambda) and its tangent once, the
+// plain products of every potential at p(lambda) into shared du/dp and
+// tangent buffers, then one chain rule through p(lambda)
+// (launch_interp_hvp_chain). Terms that leave the fixed-point range in one
+// potential and cancel bitwise against another's (the all-pairs terms of
+// excluded pairs against the exclusion list) cancel before the decode.
+void interpolated_hvp(const std::vector<Potential *> &pots, InterpHvpScratch &tmp, int N, const double *d_x,
+                      const double *d_p, const double *d_box, double lambda, const double *d_vx, const double *d_vp,
+                      double vl, uint64_t *d_hx, uint64_t *d_hp, uint64_t *d_hl, hipStream_t stream);
+
 class NonbondedAllPairs : public Potential {
   public:
     // interpolated: params are [2N, 3], initial then final, used as
@@ -131,6 +152,10 @@ class NonbondedAllPairs : public Potential {
     void execute_hvp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
                             const double *d_vx, const double *d_vp, double vl, uint64_t *d_hx, uint64_t *d_hp,
                             uint64_t *d_hl, hipStream_t stream) override;
+    bool interpolated_nonbonded() const override { return interp_; }
+    void hvp_at_interpolated(int N, const double *d_x, const double *p_lam, const double *d_box, double lambda,
+                             const double *d_vx, const double *vp_lam, double vl, uint64_t *d_hx, uint64_t *g,
+                             uint64_t *dg, uint64_t *d_hl, hipStream_t stream) override;
     void validate_host(int N, int P, const double *h_box) const override;
     void check_device_status(hipStream_t stream) override;
     void du_dp_fixed_to_float(int N, int P, const uint64_t *du_dp, double *out) const override;
@@ -164,6 +189,7 @@ class NonbondedAllPairs : public Potential {
     bool dirty_ = true;
     DeviceArray<int> d_plane_, d_offset_;
     DeviceArray<uint64_t> dp_tmp_; // du/dp at the interpolated parameters [N, 3]
+    InterpHvpScratch hvp_tmp_;
 };
 
 // Allocation-free execute_device of a NonbondedAllPairs (noalloc.hip): the
@@ -200,6 +226,10 @@ class NonbondedPairList : public Potential {
     void execute_hvp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda,
                             const double *d_vx, const double *d_vp, double vl, uint64_t *d_hx, uint64_t *d_hp,
                             uint64_t *d_hl, hipStream_t stream) override;
+    bool interpolated_nonbonded() const override { return interp_; }
+    void hvp_at_interpolated(int N, const double *d_x, const double *p_lam, const double *d_box, double lambda,
+                             const double *d_vx, const double *vp_lam, double vl, uint64_t *d_hx, uint64_t *g,
+                             uint64_t *dg, uint64_t *d_hl, hipStream_t stream) override;
     void du_dp_fixed_to_float(int N, int P, const uint64_t *du_dp, double *out) const override;
 
     int N_, M_;
@@ -213,6 +243,7 @@ class NonbondedPairList : public Potential {
     PairListArgs args(int N, int P, const double *d_x, const double *d_p, const double *d_box, double lambda) const;
     DeviceArray<int> d_pairs_, d_plane_, d_offset_;
     DeviceArray<uint64_t> dp_tmp_;
+    InterpHvpScratch hvp_tmp_;
     DeviceArray<double> d_scales_;
 };
 
@@ -274,6 +305,9 @@ class FanoutSummedPotential : public Potential {
     void check_device_status(hipStream_t stream) override;
     void du_dp_fixed_to_float(int N, int P, const uint64_t *du_dp, double *out) const override;
     std::vector<PotentialPtr> pots_;
+
+  private:
+    InterpHvpScratch hvp_tmp_;
 };
 
 class BoundPotential {

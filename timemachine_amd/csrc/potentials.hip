@@ -90,6 +90,35 @@ void Potential::execute_hvp_device(int N, int P, const double *d_x, const double
     throw Error("execute_hvp: not available for this potential");
 }
 
+void Potential::hvp_at_interpolated(int N, const double *d_x, const double *p_lam, const double *d_box, double lambda,
+                                    const double *d_vx, const double *vp_lam, double vl, uint64_t *d_hx, uint64_t *g,
+                                    uint64_t *dg, uint64_t *d_hl, hipStream_t stream) {
+    throw Error("execute_hvp: not an interpolated nonbonded potential");
+}
+
+void interpolated_hvp(const std::vector<Potential *> &pots, InterpHvpScratch &tmp, int N, const double *d_x,
+                      const double *d_p, const double *d_box, double lambda, const double *d_vx, const double *d_vp,
+                      double vl, uint64_t *d_hx, uint64_t *d_hp, uint64_t *d_hl, hipStream_t stream) {
+    if (N == 0) {
+        return;
+    }
+    const size_t n3 = static_cast<size_t>(N) * 3;
+    tmp.reserve(N);
+    double *p_lam = tmp.par.ptr, *vp_lam = tmp.par.ptr + n3;
+    const bool chain = d_hp || d_hl; // du/dlambda needs the du/dp part too
+    uint64_t *g = chain ? tmp.grad.ptr : nullptr, *dg = chain ? tmp.grad.ptr + n3 : nullptr;
+    launch_interp_hvp_params(N, lambda, d_p, d_vp, vl, p_lam, vp_lam, stream);
+    if (chain) {
+        launch_fill_u64(tmp.grad.ptr, 2 * n3, 0, stream);
+    }
+    for (Potential *p : pots) {
+        p->hvp_at_interpolated(N, d_x, p_lam, d_box, lambda, d_vx, vp_lam, vl, d_hx, g, dg, d_hl, stream);
+    }
+    if (chain) {
+        launch_interp_hvp_chain(N, lambda, d_p, d_vp, vl, g, dg, d_hp, d_hl, stream);
+    }
+}
+
 void Potential::execute_hvp_host(int N, int P, const double *h_x, const double *h_p, const double *h_box,
                                  double lambda, const double *h_vx, const double *h_vp, double vl, double *h_hx,
                                  double *h_hp, double *h_hl) {
@@ -396,7 +425,8 @@ void NonbondedAllPairs::execute_hvp_device(int N, int P, const double *d_x, cons
                                            uint64_t *d_hp, uint64_t *d_hl, hipStream_t stream) {
     validate_sizes(N, P);
     if (interp_) {
-        throw Error(name_ + "::execute_hvp(): interpolated parameters are not supported");
+        interpolated_hvp({this}, hvp_tmp_, N, d_x, d_p, d_box, lambda, d_vx, d_vp, vl, d_hx, d_hp, d_hl, stream);
+        return;
     }
     ensure_state();
     if (K_ == 0) {
@@ -404,6 +434,22 @@ void NonbondedAllPairs::execute_hvp_device(int N, int P, const double *d_x, cons
     }
     st_.prepare(d_x, d_p, nullptr, d_box, d_plane_.ptr, d_offset_.ptr, nullptr, lambda, stream);
     launch_tiles_hvp(st_, d_p, d_offset_.ptr, d_box, hvp_tangent(d_vx, d_vp, vl), HvpOut{d_hx, d_hp, d_hl}, stream);
+}
+
+void NonbondedAllPairs::hvp_at_interpolated(int N, const double *d_x, const double *p_lam, const double *d_box,
+                                            double lambda, const double *d_vx, const double *vp_lam, double vl,
+                                            uint64_t *d_hx, uint64_t *g, uint64_t *dg, uint64_t *d_hl,
+                                            hipStream_t stream) {
+    validate_sizes(N, N * 6);
+    ensure_state();
+    if (K_ == 0) {
+        return;
+    }
+    // p(lambda) gathered as plain parameters (the interpolation's du/dlambda
+    // part is the caller's chain rule)
+    st_.prepare(d_x, p_lam, nullptr, d_box, d_plane_.ptr, d_offset_.ptr, nullptr, lambda, stream);
+    launch_tiles_hvp(st_, p_lam, d_offset_.ptr, d_box, hvp_tangent(d_vx, vp_lam, vl), HvpOut{d_hx, dg, d_hl, g},
+                     stream);
 }
 
 void NonbondedAllPairs::check_device_status(hipStream_t stream) {
@@ -545,11 +591,22 @@ void NonbondedPairList::execute_device(int N, int P, const double *d_x, const do
 void NonbondedPairList::execute_hvp_device(int N, int P, const double *d_x, const double *d_p, const double *d_box,
                                            double lambda, const double *d_vx, const double *d_vp, double vl, uint64_t *d_hx,
                                            uint64_t *d_hp, uint64_t *d_hl, hipStream_t stream) {
-    const PairListArgs a = args(N, P, d_x, d_p, d_box, lambda);
     if (interp_) {
-        throw Error("NonbondedPairList::execute_hvp(): interpolated parameters are not supported");
+        args(N, P, d_x, d_p, d_box, lambda); // size checks
+        interpolated_hvp({this}, hvp_tmp_, N, d_x, d_p, d_box, lambda, d_vx, d_vp, vl, d_hx, d_hp, d_hl, stream);
+        return;
     }
-    launch_pair_list_hvp(a, hvp_tangent(d_vx, d_vp, vl), HvpOut{d_hx, d_hp, d_hl}, stream);
+    launch_pair_list_hvp(args(N, P, d_x, d_p, d_box, lambda), hvp_tangent(d_vx, d_vp, vl), HvpOut{d_hx, d_hp, d_hl},
+                         stream);
+}
+
+void NonbondedPairList::hvp_at_interpolated(int N, const double *d_x, const double *p_lam, const double *d_box,
+                                            double lambda, const double *d_vx, const double *vp_lam, double vl,
+                                            uint64_t *d_hx, uint64_t *g, uint64_t *dg, uint64_t *d_hl,
+                                            hipStream_t stream) {
+    PairListArgs a = args(N, N * 6, d_x, p_lam, d_box, lambda);
+    a.params1 = nullptr; // p(lambda) as plain parameters
+    launch_pair_list_hvp(a, hvp_tangent(d_vx, vp_lam, vl), HvpOut{d_hx, dg, d_hl, g}, stream);
 }
 
 PairListArgs NonbondedPairList::args(int N, int P, const double *d_x, const double *d_p, const double *d_box,
@@ -784,6 +841,21 @@ void FanoutSummedPotential::execute_hvp_device(int N, int P, const double *d_x, 
                                                const double *d_box, double lambda, const double *d_vx,
                                                const double *d_vp, double vl, uint64_t *d_hx, uint64_t *d_hp,
                                                uint64_t *d_hl, hipStream_t stream) {
+    std::vector<Potential *> interp;
+    for (auto &p : pots_) {
+        if (p->interpolated_nonbonded()) {
+            interp.push_back(p.get());
+        }
+    }
+    if (!interp.empty() && interp.size() == pots_.size()) {
+        // one chain rule over the summed products (the all-pairs and exclusion
+        // terms of a Nonbonded potential cancel before it)
+        if (P != N * 6) {
+            throw Error("FanoutSummedPotential::execute_hvp(): expected P == 2*N*3 for interpolated parameters");
+        }
+        interpolated_hvp(interp, hvp_tmp_, N, d_x, d_p, d_box, lambda, d_vx, d_vp, vl, d_hx, d_hp, d_hl, stream);
+        return;
+    }
     for (auto &p : pots_) {
         p->execute_hvp_device(N, P, d_x, d_p, d_box, lambda, d_vx, d_vp, vl, d_hx, d_hp, d_hl, stream);
     }
