@@ -351,3 +351,46 @@ def test_numpy_langevin_noise_matches_the_device_header(hc):
             else:
                 got[a, 2] = z[0]
     np.testing.assert_allclose(got, want, rtol=1e-14, atol=1e-14)
+
+
+def _imaged_fp32_screen(a, i, box, cut2f):
+    """The tile kernel's pass-1 screen for boxes too small for the one-image
+    frame (nonbonded.hip, tile_body): FP32 positions a in [-L/2, L/2] relative
+    to the row block's centre, per axis the nearer image min(|t|, L - |t|) of
+    the difference t, then d2 against the widened FP32 cutoff (fused
+    multiply-adds modelled in FP64, which holds the product of two FP32
+    values exactly)."""
+    f32 = np.float32
+    t = np.abs(a - a[i])
+    t = np.minimum(t, f32(box) - t)
+    d2 = (np.float64(t[:, 1]) * t[:, 1] + np.float64(t[:, 2] * t[:, 2])).astype(f32)
+    d2 = (np.float64(t[:, 0]) * t[:, 0] + d2).astype(f32)
+    return d2 < cut2f
+
+
+@pytest.mark.parametrize("margin, expect_misses", [(1e-4, False), (-1e-6, True)])
+def test_imaged_fp32_screen_keeps_every_pair_within_cutoff(margin, expect_misses):
+    """Every pair within the cutoff (exact FP64 minimum image) passes the
+    screen on boxes from the smallest allowed (2 (cutoff + padding)) to 12 nm,
+    with half the atoms placed at the cutoff distance +-1e-6 nm from others
+    across the periodic faces; without the margin the same test finds misses."""
+    rng = np.random.default_rng(0)
+    cutoff = 1.2
+    cut2 = cutoff * cutoff
+    cut2f = np.float32(cut2 * (1.0 + margin))
+    misses = 0
+    for trial in range(120):
+        box = rng.uniform(2 * (cutoff + 0.1), 12.0)
+        x = rng.uniform(0.0, box, size=(256, 3))
+        v = rng.normal(size=(128, 3))
+        v /= np.linalg.norm(v, axis=1)[:, None]
+        x[128:] = np.mod(x[:128] + v * rng.uniform(cutoff - 1e-6, cutoff + 1e-6, size=(128, 1)), box)
+        centre = x[:64].mean(0)
+        a = x - centre
+        a = (a - box * np.rint(a / box)).astype(np.float32)
+        for i in range(64):
+            d = x - x[i]
+            d -= box * np.rint(d / box)
+            exact = (d * d).sum(1) < cut2
+            misses += int(np.sum(exact & ~_imaged_fp32_screen(a, i, box, cut2f)))
+    assert (misses > 0) == expect_misses

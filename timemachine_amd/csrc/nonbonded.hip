@@ -300,21 +300,25 @@ __device__ __forceinline__ void tile_body(
     //
     // Screen in FP32 on positions relative to the centre c of the row block,
     // each atom imaged once (per axis a = min_image(x - c)). If every row atom
-    // is within e of c and e + cutoff < L / 2 on every axis, a pair within the
-    // cutoff has a_j - a_i equal to its minimum-image displacement: any other
-    // image differs by a multiple of L, so |a_j - a_i| >= L - cutoff > L / 2 + e,
-    // which a_j - a_i cannot reach. The 4D coordinate w is taken relative to
-    // row lane 0's. For a pair within the cutoff |a_j| <= e + cutoff, so the
-    // FP32 rounding of a_i, a_j and d2 stays near 1e-6 relative to the squared
-    // cutoff; the screen keeps a margin of 1e-5 and pass 2 repeats the exact
-    // FP64 test. The column values travel around the wave in registers (one
-    // lane rotation per round), so the screen reads no LDS.
+    // is within e of c and e + cutoff < L / 2 on every axis ("local"), a pair
+    // within the cutoff has a_j - a_i equal to its minimum-image displacement:
+    // any other image differs by a multiple of L, so |a_j - a_i| >= L - cutoff
+    // > L / 2 + e, which a_j - a_i cannot reach. Otherwise (small boxes, wide
+    // blocks) the screen images the difference once more in FP32: with both
+    // a in [-L/2, L/2], t = a_j - a_i has |t| <= L, so its nearest image is
+    // the nearer of t and t -+ L, at min(|t|, L - |t|) per axis. The 4D
+    // coordinate w is taken relative to row lane 0's. For a pair
+    // within the cutoff the FP32 rounding of a_i, a_j, the image shift and d2
+    // stays near 1e-6 relative to the squared cutoff; the screen keeps a
+    // margin of 1e-5 (local) or 1e-4 (imaged, whose coordinates reach L / 2)
+    // and pass 2 repeats the exact FP64 test. The column values travel around
+    // the wave in registers (one lane rotation per round), so the screen reads
+    // no LDS.
     // c and e come from the block's bounding box of the last neighborlist
     // build (k_block_bounds, FP32): since then the box is unchanged and no atom
     // has moved more than padding / 2 (else the list would have been rebuilt),
     // so every row atom is within the box's half-extent + padding / 2 of its
     // centre, plus 1e-4 nm for the FP32 rounding of the build.
-    // Otherwise (small boxes, wide blocks) pass 1 is the exact FP64 test.
     const double cw = L.pos[0][3][0]; // row lane 0 holds an atom
     const float4 bc = bb_ctr[S.block0 + r];
     const float4 be = bb_ext[S.block0 + r];
@@ -323,14 +327,16 @@ __device__ __forceinline__ void tile_body(
     const bool local = static_cast<double>(be.x) + slack + reach < 0.5 * B.bx &&
                        static_cast<double>(be.y) + slack + reach < 0.5 * B.by &&
                        static_cast<double>(be.z) + slack + reach < 0.5 * B.bz;
-    if (local) {
+    {
         const double cx = bc.x, cy = bc.y, cz = bc.z;
         const double rx = min_image_d(pi.x - cx, B.bx, B.ibx);
         const double ry = min_image_d(pi.y - cy, B.by, B.iby);
         const double rz = min_image_d(pi.z - cz, B.bz, B.ibz);
         const bool vj0 = jl0 < k;
-        // this lane's column (padding far away: d2 overflows to +inf)
-        float qx = 1e30f, qy = 1e30f, qz = 1e30f, qw = 0.0f;
+        // this lane's column; padding columns are NaN, whose d2 (NaN) fails
+        // every comparison (IEEE semantics: no fast-math flags in this build)
+        const float nan = __builtin_nanf("");
+        float qx = nan, qy = nan, qz = nan, qw = 0.0f;
         if (vj0) {
             qx = static_cast<float>(min_image_d(pj0.x - cx, B.bx, B.ibx));
             qy = static_cast<float>(min_image_d(pj0.y - cy, B.by, B.iby));
@@ -340,17 +346,27 @@ __device__ __forceinline__ void tile_body(
         uint32_t qc = jl0;
         const float fx = static_cast<float>(rx), fy = static_cast<float>(ry), fz = static_cast<float>(rz);
         const float fw = ALCH ? static_cast<float>(pi.w - cw) : 0.0f;
+        // box edges of the imaged screen in scalar registers (wave-uniform)
+        const float lx = uniform_f(static_cast<float>(B.bx)), ly = uniform_f(static_cast<float>(B.by)),
+                    lz = uniform_f(static_cast<float>(B.bz));
         // columns of the row's own block need i < j (upper triangle); later blocks always pass
         const bool tri = uniform(ballot(vj0 && jl0 < static_cast<uint32_t>((r + 1) * TILE)) != 0 ? 1 : 0);
-        const float cut2f = static_cast<float>(cut2 * (1.0 + 1e-5));
+        const float cut2f = static_cast<float>(cut2 * (local ? 1.0 + 1e-5 : 1.0 + 1e-4));
         const float cut2v = vi ? cut2f : -1.0f; // padding rows accept nothing
+        const bool wrap = !local;
         auto screen = [&](auto tri_tag) {
             constexpr bool TRI = decltype(tri_tag)::value;
             for (int round = 0; round < TILE; round++) {
                 // lane l holds column slot (l + round) mod 64
-                const float dx = qx - fx;
-                const float dy = qy - fy;
-                const float dz = qz - fz;
+                float dx = qx - fx;
+                float dy = qy - fy;
+                float dz = qz - fz;
+                if (wrap) {
+                    // |dx| <= L: the nearer of the two images
+                    dx = __builtin_fminf(__builtin_fabsf(dx), lx - __builtin_fabsf(dx));
+                    dy = __builtin_fminf(__builtin_fabsf(dy), ly - __builtin_fabsf(dy));
+                    dz = __builtin_fminf(__builtin_fabsf(dz), lz - __builtin_fabsf(dz));
+                }
                 float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
                 if (ALCH) {
                     const float dw = qw - fw;
@@ -376,15 +392,6 @@ __device__ __forceinline__ void tile_body(
             screen(std::true_type{});
         } else {
             screen(std::false_type{});
-        }
-    } else {
-        for (int round = 0; round < TILE; round++) {
-            const int slot = (lane + round) & (TILE - 1);
-            const double4 pj = L.template get_pos<ALCH>(1, slot);
-            const uint32_t jc = L.colidx[slot];
-            double dx, dy, dz, dw;
-            const double d2 = pair_delta<ALCH>(pi, pj, B.bx, B.by, B.bz, B.ibx, B.iby, B.ibz, dx, dy, dz, dw);
-            queue(round, vi && (jc < k) && (i < jc) && (d2 < cut2));
         }
     }
     if (queued > 0) {

@@ -18,6 +18,9 @@ __device__ __forceinline__ int lanes_below(uint64_t mask) {
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uniform_f(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
 
 // Orders LDS traffic between lanes of one wave (no workgroup barrier).
 __device__ __forceinline__ void wave_lds_sync() {
