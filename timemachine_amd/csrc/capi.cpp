@@ -1,4 +1,5 @@
 // extern "C" boundary (include/tm_amd.h) over the C++ classes.
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <set>
@@ -518,9 +519,25 @@ int tm_neighborlist_get_tiles(tm_neighborlist *nl, int capacity, int32_t *tile_r
         if (cnt > capacity) {
             throw Error("Neighborlist: capacity too small");
         }
-        TM_HIP_CHECK(hipMemcpy(tile_rows, nl->st.tile_row.ptr, sizeof(int) * cnt, hipMemcpyDeviceToHost));
-        TM_HIP_CHECK(hipMemcpy(tile_atoms, nl->st.tile_atoms.ptr, sizeof(uint32_t) * cnt * TILE,
+        // the valid tiles, in row order (tile slots are interleaved over
+        // several counters, so the stored tiles have holes)
+        std::vector<int> order(cnt);
+        TM_HIP_CHECK(hipMemcpy(order.data(), nl->st.tile_order.ptr, sizeof(int) * cnt, hipMemcpyDeviceToHost));
+        int used = 0;
+        for (int k : order) {
+            used = std::max(used, k + 1);
+        }
+        std::vector<int> rows(used);
+        std::vector<uint32_t> atoms(static_cast<size_t>(used) * TILE);
+        TM_HIP_CHECK(hipMemcpy(rows.data(), nl->st.tile_row.ptr, sizeof(int) * used, hipMemcpyDeviceToHost));
+        TM_HIP_CHECK(hipMemcpy(atoms.data(), nl->st.tile_atoms.ptr, sizeof(uint32_t) * atoms.size(),
                                hipMemcpyDeviceToHost));
+        for (int q = 0; q < cnt; q++) {
+            tile_rows[q] = rows[order[q]];
+            std::copy(atoms.begin() + static_cast<size_t>(order[q]) * TILE,
+                      atoms.begin() + static_cast<size_t>(order[q] + 1) * TILE,
+                      tile_atoms + static_cast<size_t>(q) * TILE);
+        }
         // tiles hold sorted slots: map them to atom indices
         std::vector<uint32_t> perm(nl->st.total_slots);
         TM_HIP_CHECK(

@@ -325,8 +325,8 @@ __device__ __forceinline__ void lds_add(uint64_t *p, uint64_t v) {
 
 template <bool G>
 __global__ __launch_bounds__(64) void k_tiles_hvp(
-    const NbSys *__restrict__ sys, int S, const int *__restrict__ tile_prefix, const int *__restrict__ tile_row,
-    const uint32_t *__restrict__ tile_atoms, const uint32_t *__restrict__ perm, const double4 *__restrict__ pos,
+    const NbSys *__restrict__ sys, int S, const int *__restrict__ tile_prefix, const int *__restrict__ tile_order,
+    const int *__restrict__ tile_row, const uint32_t *__restrict__ tile_atoms, const uint32_t *__restrict__ perm, const double4 *__restrict__ pos,
     const double *__restrict__ params, const int *__restrict__ offset, const double *__restrict__ boxes,
     const int *__restrict__ status, const HvpTangent v, const HvpOut out) {
     constexpr int NACC = G ? 9 : 6;
@@ -353,7 +353,7 @@ __global__ __launch_bounds__(64) void k_tiles_hvp(
         const double beta = static_cast<double>(Sd.beta);
         const double cut2 = Sd.cutoff_d * Sd.cutoff_d;
         const double vl = v.vls ? v.vls[s] : v.vl;
-        const int kt = Sd.tile0 + (t - tile_prefix[s]);
+        const int kt = Sd.tile0 + tile_order[Sd.tile0 + (t - tile_prefix[s])];
         const int r = tile_row[kt];
 
         // stage the tile's rows and columns
@@ -596,11 +596,13 @@ void launch_tiles_hvp(const NbState &st, const double *params, const int *offset
     // one wave per tile, the grid striding over the tile count read on the device
     const int grid = std::max(1, std::min(st.total_tiles, 16384));
     if (o.gp) {
-        k_tiles_hvp<true><<<grid, TILE, 0, stream>>>(st.sys.ptr, st.S, st.tile_prefix.ptr, st.tile_row.ptr,
+        k_tiles_hvp<true><<<grid, TILE, 0, stream>>>(st.sys.ptr, st.S, st.tile_prefix.ptr, st.tile_order.ptr,
+                                                      st.tile_row.ptr,
                                                       st.tile_atoms.ptr, st.perm.ptr, st.pos.ptr, params, offset,
                                                       boxes, st.status.ptr, v, o);
     } else {
-        k_tiles_hvp<false><<<grid, TILE, 0, stream>>>(st.sys.ptr, st.S, st.tile_prefix.ptr, st.tile_row.ptr,
+        k_tiles_hvp<false><<<grid, TILE, 0, stream>>>(st.sys.ptr, st.S, st.tile_prefix.ptr, st.tile_order.ptr,
+                                                       st.tile_row.ptr,
                                                        st.tile_atoms.ptr, st.perm.ptr, st.pos.ptr, params, offset,
                                                        boxes, st.status.ptr, v, o);
     }

@@ -11,8 +11,9 @@
 //   x, params, plane, offset      [atom0 .. atom0 + n_atoms)         caller / engine arrays, original order
 //   perm, pos (double4), par (float4) [slot0 .. slot0 + nblocks*64)  sorted order, padded to whole tiles
 //   bb_ctr, bb_ext (float4)       [block0 .. block0 + nblocks)
-//   tile_row, tile_atoms[64]      [tile0 .. tile0 + tile_cap)
-//   tile_order                    [tile0 .. tile0 + tile_cap)   tiles listed by row block
+//   tile_row, tile_atoms[64]      [tile0 .. tile0 + tile_cap)   local tile k = j + NB_TILE_LANES c
+//   tile_order                    [tile0 .. tile0 + n_tiles)    the valid local tiles, listed by row block
+//   tile_count                    [s * NB_TILE_LANES + j]       tiles handed out by counter j
 // pos = (x, y, z, w) in FP64 and par = (q, sig, eps, dw/dlambda) in FP32 are the
 // values the pair arithmetic consumes (FP64 coordinate differences keep the
 // per-atom force error under 1e-5; parameters keep the reference's FP32).
@@ -28,6 +29,20 @@ namespace tmamd {
 constexpr uint32_t NB_INVALID = 0xFFFFFFFFu;
 constexpr double NB_LIST_MARGIN = 1e-5; // nm, see NbState::nb_margin
 constexpr int NB_MAX_PIECES = 8;        // waves per row block in the neighborlist build
+
+// Tile slots of a system are handed out by NB_TILE_LANES counters: row block r
+// takes local tiles j + NB_TILE_LANES c, j = r mod NB_TILE_LANES, c from its
+// counter. One counter per system serialised the build on its returning
+// atomics (~185 ns each on MI355X: ~1 ms for the ~5k tiles of a 34k-atom
+// system); interleaving leaves holes past each counter's end, which the tile
+// order skips (tile_cap, the all-pairs bound, leaves room for them).
+constexpr int NB_TILE_LANES = 16;
+
+// Valid tiles of counter j: its count, limited to the indices below the capacity.
+TM_HD inline int nb_lane_tiles(int count, int j, int cap) {
+    const int lim = cap > j ? (cap - j + NB_TILE_LANES - 1) / NB_TILE_LANES : 0;
+    return count < lim ? count : lim;
+}
 
 // Status bits raised on the device (box checks of
 // timemachine/cpp/src/nonbonded_all_pairs.cu:315-329, tile buffer overflow).
@@ -66,15 +81,26 @@ struct NbSys {
     double padding_d;
 };
 
-// Upper bound on tiles of one system: row block r holds at most
-// ceil((k - 64 r) / 64) full tiles plus one partial tile. With a row subset a
-// row block holds at most ceil(ncol / 64) tiles.
+// Tile slots of one system: row block r holds at most ceil((k - 64 r) / 64)
+// full tiles plus one partial tile (with a row subset, ceil(ncol / 64) plus
+// one), and counter j hands out slots j + NB_TILE_LANES c for the row blocks
+// r = j mod NB_TILE_LANES, so the slots stay below the largest
+// j + NB_TILE_LANES (sum of those rows' bounds - 1) + 1.
 inline int nb_tile_capacity(int k, int nrow = 0) {
-    if (nrow > 0) {
-        return ceil_div(nrow, TILE) * (ceil_div(k - nrow, TILE) + 1);
+    const int rows = nrow > 0 ? ceil_div(nrow, TILE) : ceil_div(k, TILE);
+    long long cap = 0;
+    for (int j = 0; j < NB_TILE_LANES && j < rows; j++) {
+        long long sum = 0;
+        for (int r = j; r < rows; r += NB_TILE_LANES) {
+            sum += (nrow > 0 ? ceil_div(k - nrow, TILE) : ceil_div(k - TILE * r, TILE)) + 1;
+        }
+        const long long top = j + NB_TILE_LANES * (sum - 1) + 1;
+        cap = top > cap ? top : cap;
     }
-    const int nb = ceil_div(k, TILE);
-    return nb * (nb + 1) / 2 + nb;
+    if (cap > 0x7fffffffLL) {
+        throw Error("neighborlist: too many tiles for one system");
+    }
+    return static_cast<int>(cap);
 }
 
 // Slots of block b below this limit hold atoms.
@@ -98,7 +124,7 @@ struct NbState {
     DeviceArray<int> subset;      // subset atom lists (local indices)
     DeviceArray<int> rebuild;     // [S]
     DeviceArray<int> status;      // [S]
-    DeviceArray<int> tile_count;  // [S]
+    DeviceArray<int> tile_count;  // [S * NB_TILE_LANES]
     DeviceArray<int> tile_prefix; // [S + 1]
     DeviceArray<int> tile_row;    // [total_tiles]
     DeviceArray<int> tile_order;  // [total_tiles] per system: local tile indices grouped by row block

@@ -284,8 +284,8 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort(
     if (t < 9) {
         nblist_box[s * 9 + t] = boxes[s * 9 + t];
     }
-    if (t == 0) {
-        tile_count[s] = 0;
+    if (t < NB_TILE_LANES) {
+        tile_count[s * NB_TILE_LANES + t] = 0;
     }
 }
 
@@ -416,15 +416,19 @@ __device__ __forceinline__ float box_gap(float d, float ext, float b, float ib) 
     return fmaxf(0.0f, fabsf(d) - ext);
 }
 
-// grid (S, ceil(max_blocks / 4)), 256 threads: one wave per row block. The
-// system index runs fastest so that the first row blocks of every system,
-// which scan the most column blocks (upper triangle), are dispatched first
-// and the launch does not end on a tail of long waves.
+// grid S x max_blocks x pieces (1D), one-wave workgroups: one wave per row
+// block (or piece of one). The system index runs fastest so that the first row
+// blocks of every system, which scan the most column blocks (upper
+// triangle), are dispatched first and the launch does not end on a tail of
+// long waves. A wave's work varies several-fold (in large boxes most of a
+// row block's candidates are near it in the Hilbert order, so in its first
+// piece), and a workgroup frees its slot only when all its waves are done:
+// one wave per workgroup lets the next wave start as soon as one ends.
 // Column blocks c >= r are screened by bounding boxes (with a small margin
 // so the screen never rejects what the exact test would accept); candidate
 // column atoms are then tested exactly against the row atoms with the FP32
 // min-image predicate d^2 < nb_cutoff^2 (oracle/nblist_ref.c restates it).
-__global__ __launch_bounds__(256) void k_find_tiles(
+__global__ __launch_bounds__(64) void k_find_tiles(
     const NbSys *__restrict__ sys,
     const double *__restrict__ boxes,
     const int *__restrict__ rebuild,
@@ -437,28 +441,27 @@ __global__ __launch_bounds__(256) void k_find_tiles(
     int *__restrict__ status,
     const int pieces,
     uint32_t *__restrict__ piece_buf,
-    int *__restrict__ piece_cnt) {
-    const int s = blockIdx.x;
+    int *__restrict__ piece_cnt,
+    const int n_sys) {
+    const int s = static_cast<int>(blockIdx.x % static_cast<uint32_t>(n_sys));
+    const int rq = static_cast<int>(blockIdx.x / static_cast<uint32_t>(n_sys));
     if (!rebuild[s]) {
         return;
     }
     const NbSys S = sys[s];
-    const int wave = threadIdx.x >> 6;
-    const int rq = blockIdx.y * 4 + wave;
     const int r = rq / pieces;
     const int piece = rq - r * pieces;
     if (r >= S.row_blocks) {
         return;
     }
+    const int tile_lane = r & (NB_TILE_LANES - 1); // this row block's tile counter
     const int lane = threadIdx.x & 63;
     constexpr int SUB = 8;
     constexpr int SUBS = TILE / SUB;
-    __shared__ float4 row_lds[4][TILE];
-    __shared__ uint32_t buf_lds[4][2 * TILE];
-    __shared__ float4 sub_ctr[4][SUBS];
-    __shared__ float4 sub_ext[4][SUBS];
-    float4 *rows = row_lds[wave];
-    uint32_t *buf = buf_lds[wave];
+    __shared__ float4 rows[TILE];
+    __shared__ uint32_t buf[2 * TILE];
+    __shared__ float4 sctr[SUBS];
+    __shared__ float4 sext[SUBS];
 
     const float bx = static_cast<float>(boxes[s * 9 + 0]);
     const float by = static_cast<float>(boxes[s * 9 + 4]);
@@ -481,8 +484,6 @@ __global__ __launch_bounds__(256) void k_find_tiles(
 
     // bounding boxes of the 8 row sub-blocks of 8 atoms (lane q < 8 builds
     // sub-block q; extents are relative to its first atom, min-image)
-    float4 *sctr = sub_ctr[wave];
-    float4 *sext = sub_ext[wave];
     if (lane < SUBS) {
         const int a0 = r * TILE + lane * SUB;
         float4 c = make_float4(0.f, 0.f, 0.f, 0.f), e = make_float4(-1.f, -1.f, -1.f, 0.f); // empty
@@ -627,7 +628,7 @@ __global__ __launch_bounds__(256) void k_find_tiles(
             if (nbuf >= TILE) {
                 int tile = 0;
                 if (lane == 0) {
-                    tile = atomicAdd(tile_count + s, 1);
+                    tile = tile_lane + NB_TILE_LANES * atomicAdd(tile_count + s * NB_TILE_LANES + tile_lane, 1);
                 }
                 tile = __shfl(tile, 0, 64);
                 if (tile < S.tile_cap) {
@@ -660,7 +661,7 @@ __global__ __launch_bounds__(256) void k_find_tiles(
         const uint32_t v = lane < nbuf ? buf[lane] : NB_INVALID;
         int tile = 0;
         if (lane == 0) {
-            tile = atomicAdd(tile_count + s, 1);
+            tile = tile_lane + NB_TILE_LANES * atomicAdd(tile_count + s * NB_TILE_LANES + tile_lane, 1);
         }
         tile = __shfl(tile, 0, 64);
         if (tile < S.tile_cap) {
@@ -715,7 +716,8 @@ __global__ __launch_bounds__(256) void k_merge_pieces(
         const uint32_t v = base + lane < n ? b[base + lane] : NB_INVALID;
         int tile = 0;
         if (lane == 0) {
-            tile = atomicAdd(tile_count + s, 1);
+            const int j = r & (NB_TILE_LANES - 1);
+            tile = j + NB_TILE_LANES * atomicAdd(tile_count + s * NB_TILE_LANES + j, 1);
         }
         tile = __shfl(tile, 0, 64);
         if (tile < S.tile_cap) {
@@ -733,8 +735,9 @@ __global__ __launch_bounds__(256) void k_merge_pieces(
 // the local index of the p-th tile in row order (tiles of one row in any
 // order; the fixed-point sums do not depend on it). The tile kernel walks the
 // list in this order so that one wave meets the tiles of a row block one after
-// another. Systems with more row blocks than the LDS histogram holds keep the
-// order of the list.
+// another. The valid local tiles are those each counter handed out
+// (j + NB_TILE_LANES c, c below the counter's count); systems with more row
+// blocks than the LDS histogram holds list them counter by counter.
 constexpr int ORDER_ROWS = 4096;
 
 __global__ __launch_bounds__(256) void k_tile_order(
@@ -745,14 +748,33 @@ __global__ __launch_bounds__(256) void k_tile_order(
         return;
     }
     const NbSys S = sys[s];
-    const int n = min(tile_count[s], S.tile_cap);
     const int rb = S.row_blocks;
     const int t = threadIdx.x;
     int *order = tile_order + S.tile0;
     const int *rows = tile_row + S.tile0;
+    __shared__ int lane_n[NB_TILE_LANES], lane_start[NB_TILE_LANES + 1];
+    if (t == 0) {
+        int acc = 0;
+        for (int j = 0; j < NB_TILE_LANES; j++) {
+            lane_n[j] = nb_lane_tiles(tile_count[s * NB_TILE_LANES + j], j, S.tile_cap);
+            lane_start[j] = acc;
+            acc += lane_n[j];
+        }
+        lane_start[NB_TILE_LANES] = acc;
+    }
+    __syncthreads();
+    const int n = lane_start[NB_TILE_LANES];
+    // the q-th valid tile (counter by counter) as a local tile index, q < n
+    auto local_tile = [&](int q) {
+        int j = 0;
+        while (q >= lane_start[j + 1]) {
+            j++;
+        }
+        return j + NB_TILE_LANES * (q - lane_start[j]);
+    };
     if (rb > ORDER_ROWS) {
-        for (int k = t; k < n; k += 256) {
-            order[k] = k;
+        for (int q = t; q < n; q += 256) {
+            order[q] = local_tile(q);
         }
         return;
     }
@@ -762,8 +784,8 @@ __global__ __launch_bounds__(256) void k_tile_order(
         cnt[r] = 0;
     }
     __syncthreads();
-    for (int k = t; k < n; k += 256) {
-        atomicAdd(&cnt[rows[k]], 1);
+    for (int q = t; q < n; q += 256) {
+        atomicAdd(&cnt[rows[local_tile(q)]], 1);
     }
     __syncthreads();
     // exclusive scan of cnt[0, rb): thread t owns a contiguous range
@@ -792,12 +814,14 @@ __global__ __launch_bounds__(256) void k_tile_order(
         run += v;
     }
     __syncthreads();
-    for (int k = t; k < n; k += 256) {
+    for (int q = t; q < n; q += 256) {
+        const int k = local_tile(q);
         order[atomicAdd(&cnt[rows[k]], 1)] = k;
     }
 }
 
-// One workgroup: clamp counts to capacity, exclusive prefix over systems
+// One workgroup: valid tiles of each system (counts limited to the capacity),
+// exclusive prefix over systems
 // (tile_prefix[S] = total), clear the rebuild flags.
 __global__ __launch_bounds__(256) void k_tile_prefix(
     const NbSys *__restrict__ sys, int S, int *__restrict__ tile_count, int *__restrict__ tile_prefix,
@@ -806,12 +830,18 @@ __global__ __launch_bounds__(256) void k_tile_prefix(
     const int t = threadIdx.x;
     const int per = (S + 255) / 256;
     int sum = 0;
+    auto tiles_of = [&](int s) {
+        const int cap = sys[s].tile_cap;
+        int c = 0;
+        for (int j = 0; j < NB_TILE_LANES; j++) {
+            c += nb_lane_tiles(tile_count[s * NB_TILE_LANES + j], j, cap);
+        }
+        return c;
+    };
     for (int k = 0; k < per; k++) {
         const int s = t * per + k;
         if (s < S) {
-            int c = tile_count[s];
-            const int cap = sys[s].tile_cap;
-            sum += c < cap ? c : cap;
+            sum += tiles_of(s);
         }
     }
     part[t] = sum;
@@ -831,9 +861,7 @@ __global__ __launch_bounds__(256) void k_tile_prefix(
         const int s = t * per + k;
         if (s < S) {
             tile_prefix[s] = run;
-            int c = tile_count[s];
-            const int cap = sys[s].tile_cap;
-            run += c < cap ? c : cap;
+            run += tiles_of(s);
             rebuild[s] = 0;
         }
     }
@@ -891,7 +919,9 @@ __global__ void k_init_view(const NbSys y, NbSys *sys, int *rebuild, int *status
         sys[0] = y;
         rebuild[0] = 1;
         status[0] = 0;
-        tile_count[0] = 0;
+        for (int j = 0; j < NB_TILE_LANES; j++) {
+            tile_count[j] = 0;
+        }
         tile_prefix[0] = 0;
         tile_prefix[1] = 0;
     }
@@ -920,7 +950,7 @@ size_t NbState::view_bytes(const NbSys &sys_in) {
     c.take<NbSys>(1);
     c.take<int>(1);  // rebuild
     c.take<int>(1);  // status
-    c.take<int>(1);  // tile_count
+    c.take<int>(NB_TILE_LANES); // tile_count
     c.take<int>(2);  // tile_prefix
     c.take<int>(y.tile_cap);
     c.take<int>(y.tile_cap);
@@ -958,7 +988,7 @@ void NbState::init_view(const NbSys &sys_in, void *arena, size_t arena_bytes, hi
     sys.view(c.take<NbSys>(1), 1);
     rebuild.view(c.take<int>(1), 1);
     status.view(c.take<int>(1), 1);
-    tile_count.view(c.take<int>(1), 1);
+    tile_count.view(c.take<int>(NB_TILE_LANES), NB_TILE_LANES);
     tile_prefix.view(c.take<int>(2), 2);
     tile_row.view(c.take<int>(total_tiles), total_tiles);
     tile_order.view(c.take<int>(total_tiles), total_tiles);
@@ -1002,7 +1032,7 @@ void NbState::init(const std::vector<NbSys> &sys_in, const std::vector<int> &sub
     }
     rebuild.alloc(S);
     status.alloc(S);
-    tile_count.alloc(S);
+    tile_count.alloc(static_cast<size_t>(S) * NB_TILE_LANES);
     tile_prefix.alloc(S + 1);
     tile_row.alloc(total_tiles);
     tile_order.alloc(total_tiles);
@@ -1076,9 +1106,17 @@ void NbState::set_padding(double padding, hipStream_t stream) {
 
 void NbState::find_tiles(const double *boxes, hipStream_t stream) {
     const int pieces = find_pieces();
-    k_find_tiles<<<dim3(S, ceil_div(max_blocks * pieces, 4)), 256, 0, stream>>>(
-        sys.ptr, boxes, rebuild.ptr, pos.ptr, bb_ctr.ptr, bb_ext.ptr, tile_count.ptr, tile_row.ptr, tile_atoms.ptr,
-        status.ptr, pieces, piece_buf.ptr, piece_cnt.ptr);
+    const size_t waves = static_cast<size_t>(S) * max_blocks * pieces;
+    if (waves == 0) {
+        return;
+    }
+    if (waves > 0x7fffffffu) {
+        throw Error("neighborlist build: too many row blocks for one launch");
+    }
+    k_find_tiles<<<static_cast<unsigned>(waves), 64, 0, stream>>>(sys.ptr, boxes, rebuild.ptr, pos.ptr, bb_ctr.ptr,
+                                                                 bb_ext.ptr, tile_count.ptr, tile_row.ptr,
+                                                                 tile_atoms.ptr, status.ptr, pieces, piece_buf.ptr,
+                                                                 piece_cnt.ptr, S);
     TM_LAUNCH_CHECK();
     if (pieces > 1) {
         k_merge_pieces<<<dim3(S, ceil_div(max_blocks, 4)), 256, 0, stream>>>(
