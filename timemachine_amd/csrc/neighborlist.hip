@@ -416,19 +416,16 @@ __device__ __forceinline__ float box_gap(float d, float ext, float b, float ib) 
     return fmaxf(0.0f, fabsf(d) - ext);
 }
 
-// grid S x max_blocks x pieces (1D), one-wave workgroups: one wave per row
-// block (or piece of one). The system index runs fastest so that the first row
-// blocks of every system, which scan the most column blocks (upper
+// grid (S, ceil(max_blocks / 4) x pieces), 256 threads: one wave per row
+// block (or piece of one). The system index runs fastest so that the first
+// row blocks of every system, which scan the most column blocks (upper
 // triangle), are dispatched first and the launch does not end on a tail of
-// long waves. A wave's work varies several-fold (in large boxes most of a
-// row block's candidates are near it in the Hilbert order, so in its first
-// piece), and a workgroup frees its slot only when all its waves are done:
-// one wave per workgroup lets the next wave start as soon as one ends.
+// long waves.
 // Column blocks c >= r are screened by bounding boxes (with a small margin
 // so the screen never rejects what the exact test would accept); candidate
 // column atoms are then tested exactly against the row atoms with the FP32
 // min-image predicate d^2 < nb_cutoff^2 (oracle/nblist_ref.c restates it).
-__global__ __launch_bounds__(64) void k_find_tiles(
+__global__ __launch_bounds__(256) void k_find_tiles(
     const NbSys *__restrict__ sys,
     const double *__restrict__ boxes,
     const int *__restrict__ rebuild,
@@ -441,16 +438,20 @@ __global__ __launch_bounds__(64) void k_find_tiles(
     int *__restrict__ status,
     const int pieces,
     uint32_t *__restrict__ piece_buf,
-    int *__restrict__ piece_cnt,
-    const int n_sys) {
-    const int s = static_cast<int>(blockIdx.x % static_cast<uint32_t>(n_sys));
-    const int rq = static_cast<int>(blockIdx.x / static_cast<uint32_t>(n_sys));
+    int *__restrict__ piece_cnt) {
+    const int s = blockIdx.x;
     if (!rebuild[s]) {
         return;
     }
     const NbSys S = sys[s];
-    const int r = rq / pieces;
-    const int piece = rq - r * pieces;
+    // the 4 waves of a workgroup take the same piece of 4 consecutive row
+    // blocks: a row block's first piece holds most of its candidates (those
+    // near it in the Hilbert order), and a workgroup's slot is free only when
+    // all its waves are done, so its waves should carry similar work
+    const int wave = threadIdx.x >> 6;
+    const int group = blockIdx.y / pieces;
+    const int piece = blockIdx.y - group * pieces;
+    const int r = group * 4 + wave;
     if (r >= S.row_blocks) {
         return;
     }
@@ -458,10 +459,14 @@ __global__ __launch_bounds__(64) void k_find_tiles(
     const int lane = threadIdx.x & 63;
     constexpr int SUB = 8;
     constexpr int SUBS = TILE / SUB;
-    __shared__ float4 rows[TILE];
-    __shared__ uint32_t buf[2 * TILE];
-    __shared__ float4 sctr[SUBS];
-    __shared__ float4 sext[SUBS];
+    __shared__ float4 row_lds[4][TILE];
+    __shared__ uint32_t buf_lds[4][2 * TILE];
+    __shared__ float4 sub_ctr[4][SUBS];
+    __shared__ float4 sub_ext[4][SUBS];
+    float4 *rows = row_lds[wave];
+    uint32_t *buf = buf_lds[wave];
+    float4 *sctr = sub_ctr[wave];
+    float4 *sext = sub_ext[wave];
 
     const float bx = static_cast<float>(boxes[s * 9 + 0]);
     const float by = static_cast<float>(boxes[s * 9 + 4]);
@@ -1106,17 +1111,9 @@ void NbState::set_padding(double padding, hipStream_t stream) {
 
 void NbState::find_tiles(const double *boxes, hipStream_t stream) {
     const int pieces = find_pieces();
-    const size_t waves = static_cast<size_t>(S) * max_blocks * pieces;
-    if (waves == 0) {
-        return;
-    }
-    if (waves > 0x7fffffffu) {
-        throw Error("neighborlist build: too many row blocks for one launch");
-    }
-    k_find_tiles<<<static_cast<unsigned>(waves), 64, 0, stream>>>(sys.ptr, boxes, rebuild.ptr, pos.ptr, bb_ctr.ptr,
-                                                                 bb_ext.ptr, tile_count.ptr, tile_row.ptr,
-                                                                 tile_atoms.ptr, status.ptr, pieces, piece_buf.ptr,
-                                                                 piece_cnt.ptr, S);
+    k_find_tiles<<<dim3(S, ceil_div(max_blocks, 4) * pieces), 256, 0, stream>>>(
+        sys.ptr, boxes, rebuild.ptr, pos.ptr, bb_ctr.ptr, bb_ext.ptr, tile_count.ptr, tile_row.ptr, tile_atoms.ptr,
+        status.ptr, pieces, piece_buf.ptr, piece_cnt.ptr);
     TM_LAUNCH_CHECK();
     if (pieces > 1) {
         k_merge_pieces<<<dim3(S, ceil_div(max_blocks, 4)), 256, 0, stream>>>(
