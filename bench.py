@@ -420,7 +420,7 @@ def main() -> int:
     ap.add_argument("--single-steps", type=int, default=300)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serving-jobs", type=int, default=16, help="jobs in the serving leg (0 skips it)")
-    ap.add_argument("--serving-step-scale", type=int, default=10,
+    ap.add_argument("--serving-step-scale", type=int, default=5,
                     help="the serving jobs' U(2000, 20000) steps are divided by this")
     ap.add_argument("--diff-frames", type=int, default=500,
                     help="frames per window of the differentiation leg (16 windows; 0 skips it)")
