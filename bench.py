@@ -53,7 +53,7 @@ IMPL_FLOP_PAIR_LJ = 13
 # rocprofv3 --pmc on MI355X (summarised under profiles/ with the commands; the
 # counter program profiles the boxes as built after FIRE, 0.1% fewer tiles than
 # the relaxed state of the kernel trace).
-TILE_TRAFFIC_BYTES = 1.0164e9 + 2.0985e9
+TILE_TRAFFIC_BYTES = 1.0172e9 + 2.1028e9
 # Serving admits jobs while the resident atoms stay under this: the batched
 # engine's step time is ~0.12 ms + atoms / 3.6e8 atom-steps/s on one MI355X,
 # so 800k resident atoms run at ~95% of the peak rate, and admitting more only
