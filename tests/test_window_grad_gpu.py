@@ -209,4 +209,4 @@ def test_differentiable_potential_energy(small):
     np.testing.assert_array_equal(x.grad.numpy(), ref[0])
     np.testing.assert_array_equal(p.grad.numpy(), ref[1])
     assert float(lam.grad) == ref[2]
-    assert float(u) == ref[3]
+    assert float(u.detach()) == ref[3]
