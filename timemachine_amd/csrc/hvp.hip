@@ -328,7 +328,8 @@ __global__ __launch_bounds__(64) void k_tiles_hvp(
     const NbSys *__restrict__ sys, int S, const int *__restrict__ tile_prefix, const int *__restrict__ tile_order,
     const int *__restrict__ tile_row, const uint32_t *__restrict__ tile_atoms, const uint32_t *__restrict__ perm, const double4 *__restrict__ pos,
     const double *__restrict__ params, const int *__restrict__ offset, const double *__restrict__ boxes,
-    const int *__restrict__ status, const HvpTangent v, const HvpOut out) {
+    const float4 *__restrict__ bb_ctr, const float4 *__restrict__ bb_ext, const int *__restrict__ status,
+    const HvpTangent v, const HvpOut out) {
     constexpr int NACC = G ? 9 : 6;
     __shared__ HvpTileLds<NACC> L;
     const int total = tile_prefix[S];
@@ -403,6 +404,10 @@ __global__ __launch_bounds__(64) void k_tiles_hvp(
                 dv[2] = Dual(min_image_d(L.pz[0][il] - L.pz[1][jl], bz, ibz), L.tz[0][il] - L.tz[1][jl]);
                 const double dwdl = (L.off[0][il] - L.off[1][jl]) * Sd.cutoff_d;
                 const Dual dw(L.pw[0][il] - L.pw[1][jl], dwdl * vl);
+                // the exact FP64 test of the force kernels (the screen is a superset)
+                if (!(__builtin_fma(dw.v, dw.v, pair_d2(dv[0].v, dv[1].v, dv[2].v)) < cut2)) {
+                    return;
+                }
                 const PairGradDual o = pair_grad_dual(
                     dv, dw, dwdl, Dual(L.q[0][il], L.tq[0][il]), Dual(L.q[1][jl], L.tq[1][jl]),
                     Dual(L.sg[0][il], L.ts[0][il]), Dual(L.sg[1][jl], L.ts[1][jl]), Dual(L.ep[0][il], L.te[0][il]),
@@ -424,18 +429,50 @@ __global__ __launch_bounds__(64) void k_tiles_hvp(
         int queued = 0;
         const bool vi = L.slot[0][lane] != NB_INVALID;
         const uint32_t islot = L.slot[0][lane];
-        const double xi = L.px[0][lane], yi = L.py[0][lane], zi = L.pz[0][lane], wi = L.pw[0][lane];
+        // Screen in FP32 with the column values rotating through the wave, on
+        // positions relative to the row block's centre from the last build,
+        // imaged once more per pair when the block is too wide for one image
+        // (as the force kernel's pass 1, same margins); eval repeats the exact
+        // FP64 test.
+        const float4 bc = bb_ctr[Sd.block0 + r];
+        const float4 be = bb_ext[Sd.block0 + r];
+        const double slack = 0.5 * Sd.padding_d + 1e-4;
+        const double reach = Sd.cutoff_d + 1e-3;
+        const bool local = static_cast<double>(be.x) + slack + reach < 0.5 * bx &&
+                           static_cast<double>(be.y) + slack + reach < 0.5 * by &&
+                           static_cast<double>(be.z) + slack + reach < 0.5 * bz;
+        const double cx = bc.x, cy = bc.y, cz = bc.z, cw = L.pw[0][0];
+        const float nan = __builtin_nanf("");
+        const bool vj0 = L.slot[1][lane] != NB_INVALID;
+        float qx = nan, qy = nan, qz = nan, qw = 0.0f;
+        if (vj0) {
+            qx = static_cast<float>(min_image_d(L.px[1][lane] - cx, bx, ibx));
+            qy = static_cast<float>(min_image_d(L.py[1][lane] - cy, by, iby));
+            qz = static_cast<float>(min_image_d(L.pz[1][lane] - cz, bz, ibz));
+            qw = static_cast<float>(L.pw[1][lane] - cw);
+        }
+        uint32_t qc = L.slot[1][lane];
+        const float fx = static_cast<float>(min_image_d(L.px[0][lane] - cx, bx, ibx));
+        const float fy = static_cast<float>(min_image_d(L.py[0][lane] - cy, by, iby));
+        const float fz = static_cast<float>(min_image_d(L.pz[0][lane] - cz, bz, ibz));
+        const float fw = static_cast<float>(L.pw[0][lane] - cw);
+        const float lx = uniform_f(static_cast<float>(bx)), ly = uniform_f(static_cast<float>(by)),
+                    lz = uniform_f(static_cast<float>(bz));
+        const float cut2v = vi ? static_cast<float>(cut2 * (local ? 1.0 + 1e-5 : 1.0 + 1e-4)) : -1.0f;
+        const bool wrap = !local;
         for (int round = 0; round < TILE; round++) {
-            const int c = (lane + round) & (TILE - 1);
-            const uint32_t jslot = L.slot[1][c];
-            bool in = false;
-            if (vi && jslot != NB_INVALID && islot < jslot) {
-                const double dx = min_image_d(xi - L.px[1][c], bx, ibx);
-                const double dy = min_image_d(yi - L.py[1][c], by, iby);
-                const double dz = min_image_d(zi - L.pz[1][c], bz, ibz);
-                const double dw = wi - L.pw[1][c];
-                in = __builtin_fma(dw, dw, pair_d2(dx, dy, dz)) < cut2;
+            const int c = (lane + round) & (TILE - 1); // the column slot this lane holds
+            float dx = qx - fx;
+            float dy = qy - fy;
+            float dz = qz - fz;
+            if (wrap) {
+                dx = __builtin_fminf(__builtin_fabsf(dx), lx - __builtin_fabsf(dx));
+                dy = __builtin_fminf(__builtin_fabsf(dy), ly - __builtin_fabsf(dy));
+                dz = __builtin_fminf(__builtin_fabsf(dz), lz - __builtin_fabsf(dz));
             }
+            const float dw = qw - fw;
+            const float d2 = __builtin_fmaf(dw, dw, __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)));
+            const bool in = d2 < cut2v && islot < qc; // padding columns: NaN d2 and NB_INVALID slot
             const uint64_t m = ballot(in);
             if (in) {
                 const int at = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
@@ -454,6 +491,11 @@ __global__ __launch_bounds__(64) void k_tiles_hvp(
                 }
                 wave_lds_sync();
             }
+            qx = rot1(qx);
+            qy = rot1(qy);
+            qz = rot1(qz);
+            qw = rot1(qw);
+            qc = rot1(qc);
         }
         if (queued > 0) {
             wave_lds_sync();
@@ -599,12 +641,12 @@ void launch_tiles_hvp(const NbState &st, const double *params, const int *offset
         k_tiles_hvp<true><<<grid, TILE, 0, stream>>>(st.sys.ptr, st.S, st.tile_prefix.ptr, st.tile_order.ptr,
                                                       st.tile_row.ptr,
                                                       st.tile_atoms.ptr, st.perm.ptr, st.pos.ptr, params, offset,
-                                                      boxes, st.status.ptr, v, o);
+                                                      boxes, st.bb_ctr.ptr, st.bb_ext.ptr, st.status.ptr, v, o);
     } else {
         k_tiles_hvp<false><<<grid, TILE, 0, stream>>>(st.sys.ptr, st.S, st.tile_prefix.ptr, st.tile_order.ptr,
                                                        st.tile_row.ptr,
                                                        st.tile_atoms.ptr, st.perm.ptr, st.pos.ptr, params, offset,
-                                                       boxes, st.status.ptr, v, o);
+                                                       boxes, st.bb_ctr.ptr, st.bb_ext.ptr, st.status.ptr, v, o);
     }
     TM_LAUNCH_CHECK();
 }
