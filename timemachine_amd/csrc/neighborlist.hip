@@ -186,9 +186,35 @@ __device__ void block_exclusive_scan_cells(int *cells, int *wave_tot) {
     __syncthreads();
 }
 
+// Ascending bitonic sorting network over SORT_NET values in registers (every
+// index is a compile-time constant once the loops unroll).
+constexpr int SORT_NET = 16;
+__device__ __forceinline__ void bitonic_sort(uint32_t (&v)[SORT_NET]) {
+#pragma unroll
+    for (int k = 2; k <= SORT_NET; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+            for (int i = 0; i < SORT_NET; i++) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const uint32_t lo = min(v[i], v[l]);
+                    const uint32_t hi = max(v[i], v[l]);
+                    const bool up = (i & k) == 0;
+                    v[i] = up ? lo : hi;
+                    v[l] = up ? hi : lo;
+                }
+            }
+        }
+    }
+}
+
 // One workgroup per system; only systems flagged for rebuild do work.
-// Counting sort of the nonbonded atoms by Hilbert cell, then insertion sort
-// by atom index inside each cell so the permutation is deterministic.
+// Counting sort of the nonbonded atoms by Hilbert cell, then atom index order
+// inside each cell so the permutation is deterministic: a sorting network in
+// registers for cells of up to 16 atoms (a 34k-atom system has ~8 per cell;
+// the insertion sort in global memory it replaces took ~170 us there), an
+// insertion sort for larger cells.
 // Also records the coordinates/box the neighborlist is built from and
 // resets the tile counter.
 __global__ __launch_bounds__(SORT_THREADS) void k_sort(
@@ -261,6 +287,26 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort(
         for (int c = t; c < HILBERT_CELLS; c += SORT_THREADS) {
             const int b = start[c];
             const int e = count[c];
+            if (e - b <= 1) {
+                continue;
+            }
+            if (e - b <= SORT_NET) {
+                // in registers: a bitonic network over SORT_NET entries, the
+                // unused ones past the cell's end (NB_INVALID sorts last)
+                uint32_t v[SORT_NET];
+#pragma unroll
+                for (int i = 0; i < SORT_NET; i++) {
+                    v[i] = b + i < e ? seg_perm[b + i] : NB_INVALID;
+                }
+                bitonic_sort(v);
+#pragma unroll
+                for (int i = 0; i < SORT_NET; i++) {
+                    if (b + i < e) {
+                        seg_perm[b + i] = v[i];
+                    }
+                }
+                continue;
+            }
             for (int i = b + 1; i < e; i++) {
                 const uint32_t v = seg_perm[i];
                 int j = i - 1;
