@@ -52,16 +52,17 @@ def test_neighborlist_row_subset_exact(gpu, size):
     for r in (rows, shuffled):
         nl.set_row_idxs(r)
         for _ in range(2):
-            test = nl.get_nblist(x, box, cutoff)
+            test = nl.get_nblist(x, box, cutoff, block_size=64)
             ref = onblist.fp32_ixn_list_rows(x, box, cutoff, r, 64)
             assert [sorted(t) for t in test] == ref
+        assert nl.get_nblist(x, box, cutoff) == onblist.fp32_ixn_list_rows(x, box, cutoff, r, 32)
         ref64 = onblist.build_reference_ixn_list_with_subset(x, box, cutoff, r, 64)
         assert sum(len(set(a) ^ set(b)) for a, b in zip(ref64, ref)) <= 2
     a = set(np.concatenate(onblist.fp32_ixn_list_rows(x, box, cutoff, rows, 64)).tolist())
     b = set(np.concatenate(onblist.fp32_ixn_list_rows(x, box, cutoff, shuffled, 64)).tolist())
     assert a == b
     nl.reset_row_idxs()
-    assert [sorted(t) for t in nl.get_nblist(x, box, cutoff)] == onblist.fp32_ixn_list(x, box, cutoff, 64)
+    assert [sorted(t) for t in nl.get_nblist(x, box, cutoff, 64)] == onblist.fp32_ixn_list(x, box, cutoff, 64)
 
 
 def test_neighborlist_ligand_rows(gpu):
@@ -69,11 +70,14 @@ def test_neighborlist_ligand_rows(gpu):
     s = ts.ligand_in_water()
     nl = custom_ops.Neighborlist_f32(s.N)
     nl.set_row_idxs(s.ligand_idxs.astype(np.uint32))
-    test = nl.get_nblist(s.x, s.box, 1.0)
+    test = nl.get_nblist(s.x, s.box, 1.0, block_size=64)
     assert len(test) == 2  # 72 ligand atoms: two row blocks of 64
     assert [sorted(t) for t in test] == onblist.fp32_ixn_list_rows(s.x, s.box, 1.0, s.ligand_idxs, 64)
+    test = nl.get_nblist(s.x, s.box, 1.0)
+    assert len(test) == 3  # and three of 32
+    assert test == onblist.fp32_ixn_list_rows(s.x, s.box, 1.0, s.ligand_idxs, 32)
     nl.reset_row_idxs()
-    assert len(nl.get_nblist(s.x, s.box, 1.0)) == (s.N + 63) // 64
+    assert len(nl.get_nblist(s.x, s.box, 1.0, block_size=64)) == (s.N + 63) // 64
 
 
 def test_neighborlist_invalid_row_idxs(gpu):

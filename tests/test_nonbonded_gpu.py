@@ -223,12 +223,18 @@ def test_neighborlist_exact(gpu, size):
     cutoff = 1.0
     nl = custom_ops.Neighborlist_f32(size)
     for _ in range(2):
-        test = nl.get_nblist(x, box, cutoff)
+        test = nl.get_nblist(x, box, cutoff, block_size=64)
         ref32 = onblist.fp32_ixn_list(x, box, cutoff, 64)
         assert [sorted(t) for t in test] == ref32
     ref64 = onblist.build_reference_ixn_list(x, box, cutoff, 64)
     nmis = sum(len(set(a) ^ set(b)) for a, b in zip(ref64, ref32))
     assert nmis <= 2
+    # the reference's 32-atom row blocks (its default)
+    test = nl.get_nblist(x, box, cutoff)
+    fp32_blocks32 = onblist.fp32_ixn_list(x, box, cutoff, 32)
+    assert test == fp32_blocks32
+    ref_blocks32 = onblist.build_reference_ixn_list(x, box, cutoff, 32)
+    assert sum(len(set(a) ^ set(b)) for a, b in zip(ref_blocks32, fp32_blocks32)) <= 2
 
 
 def _morton_order(x, box, bits=5):
@@ -250,9 +256,10 @@ def test_neighborlist_exact_full_box(ligand, cutoff, order):
     FP32 predicate's, bit for bit, at the cutoff and well beyond it."""
     x = ligand.x[_morton_order(ligand.x, ligand.box)] if order == "morton" else ligand.x
     nl = custom_ops.Neighborlist_f32(ligand.N)
-    test = nl.get_nblist(x, ligand.box, cutoff)
+    test = nl.get_nblist(x, ligand.box, cutoff, block_size=64)
     ref32 = onblist.fp32_ixn_list(x, ligand.box, cutoff, 64)
     assert [sorted(t) for t in test] == ref32
+    assert nl.get_nblist(x, ligand.box, cutoff) == onblist.fp32_ixn_list(x, ligand.box, cutoff, 32)
 
 
 @pytest.mark.parametrize("pieces", [1, 2, 8])
@@ -263,7 +270,7 @@ def test_neighborlist_pieces_exact_full_box(ligand, pieces):
     x = ligand.x[_morton_order(ligand.x, ligand.box)]
     nl = custom_ops.Neighborlist_f32(ligand.N)
     nl.set_pieces(pieces)
-    test = nl.get_nblist(x, ligand.box, 1.3)
+    test = nl.get_nblist(x, ligand.box, 1.3, block_size=64)
     ref32 = onblist.fp32_ixn_list(x, ligand.box, 1.3, 64)
     assert [sorted(t) for t in test] == ref32
     with pytest.raises(RuntimeError, match="pieces must be"):
@@ -281,7 +288,7 @@ def test_neighborlist_resize_errors(gpu):
 def test_neighborlist_covers_all_pairs_within_cutoff(ligand):
     """Every pair i < j within the cutoff has j in the list of i's row block."""
     nl = custom_ops.Neighborlist_f32(ligand.N)
-    lists = nl.get_nblist(ligand.x, ligand.box, 1.3)
+    lists = nl.get_nblist(ligand.x, ligand.box, 1.3, block_size=64)
     M = np.zeros((len(lists), ligand.N), dtype=bool)
     for r, lst in enumerate(lists):
         M[r, lst] = True

@@ -10,8 +10,9 @@ is no CPU path.
 Differences a caller can observe:
   * arithmetic is FP32 with 64-bit fixed-point accumulation; ``*_f64`` names
     exist but raise (the MI355X path is the FP32 one);
-  * the neighborlist works on 64-atom blocks (one wave64 per tile), so
-    ``Neighborlist_f32.get_nblist`` returns one list per 64-atom row block;
+  * the neighborlist works on 64-atom blocks (one wave64 per tile);
+    ``Neighborlist_f32.get_nblist`` returns the reference's 32-atom row
+    blocks by default (``block_size=64`` gives the tiles' own blocks);
   * Langevin and barostat noise come from a counter-based generator
     (Philox4x32-10), so trajectories differ from curand's but are bitwise
     reproducible, checkpointable, and identical alone or in a batch.
@@ -596,6 +597,7 @@ class Neighborlist_f32(_Handle):
         super().__init__()
         self._N = int(N)
         self._rows = 0  # row atoms of a row subset, 0 = all pairs
+        self._row_idxs = np.zeros(0, dtype=np.int64)
         call("tm_neighborlist_create", self._N, ctypes.byref(self._h))
 
     def resize(self, size: int):
@@ -618,6 +620,7 @@ class Neighborlist_f32(_Handle):
         call("tm_neighborlist_set_row_idxs", self._h, idxs.size,
              idxs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
         self._rows = idxs.size
+        self._row_idxs = idxs.astype(np.int64)
 
     def _build(self, coords, box, cutoff):
         coords = f64(coords)
@@ -639,14 +642,48 @@ class Neighborlist_f32(_Handle):
         """Raw tiles: (row block per tile [T], column atoms [T, 64], 0xFFFFFFFF = padding)."""
         return self._build(coords, box, cutoff)
 
-    def get_nblist(self, coords, box, cutoff) -> List[List[int]]:
-        N = np.asarray(coords).shape[0]
-        rows, atoms = self._build(coords, box, cutoff)
-        n_blocks = ((self._rows or N) + self.BLOCK - 1) // self.BLOCK
+    def get_nblist(self, coords, box, cutoff, block_size: int = 32) -> List[List[int]]:
+        """Interacting column atoms of every row block (wrap_kernels.cpp:55-73):
+        row blocks of ``block_size`` atoms (32, the reference's, or 64, the
+        tiles' own) in atom order, or in the order of set_row_idxs. The 32-atom
+        lists are the 64-atom block's columns kept by the build's own FP32
+        predicate against each half's atoms (every such column is in the
+        64-atom list: it is within the cutoff of one of the block's atoms)."""
+        if block_size not in (32, self.BLOCK):
+            raise RuntimeError(f"Block size must be 32 or {self.BLOCK}.")
+        coords64 = f64(coords)
+        N = coords64.shape[0]
+        rows, atoms = self._build(coords64, box, cutoff)
+        n_rows = self._rows or N
+        n_blocks = (n_rows + self.BLOCK - 1) // self.BLOCK
         out: List[List[int]] = [[] for _ in range(n_blocks)]
         for r, a in zip(rows, atoms):
             out[int(r)].extend(int(j) for j in a if j < N)
-        return out
+        if block_size == self.BLOCK:
+            return out
+        # the build's predicate: FP32 coordinates, d - b rint(d / b) per axis
+        # with b = float(box), then dx^2 + dy^2 + dz^2 < float(cutoff)^2
+        x32 = coords64.astype(np.float32)
+        b = np.diagonal(f64(box)).astype(np.float32)
+        ib = (np.float32(1.0) / b).astype(np.float32)
+        cut = np.float32(cutoff)
+        cut2 = np.float32(cut * cut)
+        row_atoms = self._row_idxs if self._rows else np.arange(N)
+        halves: List[List[int]] = []
+        for r in range((n_rows + 31) // 32):
+            cols = np.asarray(out[r // 2], dtype=np.int64)
+            ri = row_atoms[32 * r: min(32 * r + 32, n_rows)]
+            if not self._rows:
+                cols = cols[cols >= 32 * r]  # upper triangle from the half's first atom
+            if cols.size == 0:
+                halves.append([])
+                continue
+            d = x32[ri][:, None, :] - x32[cols][None, :, :]
+            d = (d - b * np.rint(d * ib)).astype(np.float32)
+            d2 = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+            keep = np.any(d2 < cut2, axis=0)
+            halves.append(sorted(int(j) for j in cols[keep]))
+        return halves
 
     def compute_block_bounds(self, coords, box, block_size):
         if block_size != self.BLOCK:
